@@ -1,0 +1,145 @@
+"""ctypes driver for the C restatement (oracle/_build/liboracle.so).
+
+Test infrastructure only -- the checker for the HIP path.  Same calling
+convention as oracle/ref.py: fields are float64 (nz, ncol) C-order arrays,
+updated in place; absent optional fields are simply missing from the dict.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .ref import DEFAULT_FLAGS, DIAG2_FIELDS, DIAG3_FIELDS, FLAG_ORDER, STATE_FIELDS
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "_build", "liboracle.so")
+
+_lib = None
+_tables_keepalive = None
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        _lib = ctypes.CDLL(ORACLE_SO)
+        vp = ctypes.c_void_p
+        _lib.oracle_init.restype = ctypes.c_int
+        _lib.oracle_init.argtypes = [ctypes.POINTER(ctypes.c_int), vp, vp, vp, vp]
+        _lib.oracle_run.restype = ctypes.c_int
+        _lib.oracle_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                    ctypes.POINTER(vp), vp, vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]
+        for fn in ("oracle_gamma_p", "oracle_rslf", "oracle_rsif"):
+            getattr(_lib, fn).restype = ctypes.c_double
+            getattr(_lib, fn).argtypes = [ctypes.c_double, ctypes.c_double]
+        _lib.oracle_snow_moments.restype = None
+        _lib.oracle_snow_moments.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+        _lib.oracle_get_array.restype = ctypes.c_int
+        _lib.oracle_get_array.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        _lib.oracle_qg_entry.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(ctypes.c_double)]
+        _lib.oracle_qs_entry.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_double)]
+        _lib.oracle_fz_rain_entry.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_double)]
+        _lib.oracle_fz_cloud_entry.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_double)]
+    return _lib
+
+
+def init(tables, **flags):
+    """Initialise constants and small tables; big tables are referenced, not copied."""
+    global _tables_keepalive
+    f = dict(DEFAULT_FLAGS)
+    f.update(flags)
+    arr = (ctypes.c_int * 18)(*([int(f[k]) for k in FLAG_ORDER] + [0]))
+    keep = dict(ccn=np.ascontiguousarray(tables["ccn"], dtype=np.float32),
+                fz=np.ascontiguousarray(tables["fz"], dtype=np.float64),
+                qs=np.ascontiguousarray(tables["qs"], dtype=np.float64),
+                qg=np.ascontiguousarray(tables["qg"], dtype=np.float64))
+    rc = lib().oracle_init(arr, *[keep[k].ctypes.data for k in ("ccn", "fz", "qs", "qg")])
+    if rc != 0:
+        raise RuntimeError(f"oracle_init: unsupported option ({rc})")
+    _tables_keepalive = keep
+    return f
+
+
+def run(state, dt, itimestep, land=None, nlev=None):
+    nz, ncol = state["p"].shape
+    ptrs = (ctypes.c_void_p * 19)()
+    for i, name in enumerate(STATE_FIELDS):
+        a = state.get(name)
+        if a is None:
+            ptrs[i] = None
+            continue
+        assert a.shape == (nz, ncol) and a.dtype == np.float64 and a.flags.c_contiguous, name
+        ptrs[i] = a.ctypes.data
+    d2 = {k: np.zeros(ncol) for k in DIAG2_FIELDS}
+    d3 = {k: np.zeros((nz, ncol)) for k in DIAG3_FIELDS}
+    p2 = (ctypes.c_void_p * 8)(*[d2[k].ctypes.data for k in DIAG2_FIELDS])
+    p3 = (ctypes.c_void_p * 7)(*[d3[k].ctypes.data for k in DIAG3_FIELDS])
+    landp = nlevp = None
+    if land is not None:
+        land = np.ascontiguousarray(land, dtype=np.int32)
+        landp = land.ctypes.data
+    if nlev is not None:
+        nlev = np.ascontiguousarray(nlev, dtype=np.int32)
+        nlevp = nlev.ctypes.data
+    rc = lib().oracle_run(ncol, nz, float(dt), int(itimestep), ptrs, landp, nlevp, p2, p3)
+    if rc != 0:
+        raise RuntimeError(f"oracle_run failed ({rc})")
+    return d2, d3
+
+
+def gamma_p(a, x):
+    return lib().oracle_gamma_p(a, x)
+
+
+def rslf(p, t):
+    return lib().oracle_rslf(p, t)
+
+
+def rsif(p, t):
+    return lib().oracle_rsif(p, t)
+
+
+def snow_moments(rs, tc):
+    out = (ctypes.c_double * 10)()
+    lib().oracle_snow_moments(rs, tc, out)
+    return np.array(out[:])
+
+
+_ARRAY_SIZES = {1: 10000, 2: 10000, 3: 370000, 4: 3520, 5: 3520, 6: 3520, 7: 1100, 8: 220}
+
+
+def get_array(which):
+    n = _ARRAY_SIZES[which]
+    out = np.zeros(n)
+    lib().oracle_get_array(which, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
+    return out
+
+
+def qg_entry(i, j, n3, k, m):
+    out = (ctypes.c_double * 5)()
+    lib().oracle_qg_entry(i, j, n3, k, m, out)
+    return np.array(out[:])
+
+
+def qs_entry(i, j, k, m):
+    out = (ctypes.c_double * 12)()
+    lib().oracle_qs_entry(i, j, k, m, out)
+    return np.array(out[:])
+
+
+def fz_rain_entry(i, j, k, m):
+    out = (ctypes.c_double * 4)()
+    lib().oracle_fz_rain_entry(i, j, k, m, out)
+    return np.array(out[:])
+
+
+def fz_cloud_entry(i, j, k, m):
+    out = (ctypes.c_double * 2)()
+    lib().oracle_fz_cloud_entry(i, j, k, m, out)
+    return np.array(out[:])

@@ -1,0 +1,2563 @@
+/* Test oracle: plain-C FP64 restatement of the TEMPO column step.
+ * TEST INFRASTRUCTURE ONLY -- see tempo_oracle.h.  Citations are to the
+ * reference sources (src/...).  Compiled with -ffp-contract=off and no
+ * fast-math so that every operation rounds exactly as in the reference
+ * FP64 build at -O0. */
+#include "tempo_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NZMAX 512
+
+/* ---- fixed parameters: src/module_mp_tempo_params.F90:39-167 ---- */
+static const double av_r = 4854.0, bv_r = 1.0, av_s = 40.0, bv_s = 0.55, fv_s = 100.0;
+static const double bv_c = 2.0, bv_i = 1.0, av_g_old = 442.0, bv_g_old = 0.89;
+static const double av_g_new = 161.794724, bv_g_new = 0.640961647, fv_r = 195.0;
+static const double av_c = 0.316946e8, a_coeff = 0.47244157, b_coeff = 0.54698726, av_i = 1493.9;
+static const double am_s = 0.069, bm_s = 2.0, bm_g = 3.0, bm_i = 3.0, bm_r = 3.0;
+static const double rho_i = 890.0, xm0i = 1.e-12, d0c = 1.e-6, d0r = 50.e-6, d0s = 300.e-6;
+static const double d0g = 350.e-6, d0r_max = 2.5e-3, c_cube = 0.5, c_sqrd = 0.15;
+static const double mu_r = 0.0, mu_s = 0.6357, mu_g = 0.0, mu_i = 0.0;
+static const double nu_c_scale = 1000.e6;
+static const double naccn0 = 300.0e6, naccn1 = 50.0e6, nain0 = 1.5e6, nain1 = 0.5e6;
+static const double nwfa_default = 11.1e6, aero_max = 9999.e6, hgfrz = 235.16;
+static const double nt_c_o = 50.e6, nt_c_l = 100.e6, nt_c_max = 1999.e6, nt_c_min = 2.0;
+static const double tno = 5.0, ato = 0.304, rho_s = 100.0;
+static const double demott_nuc_ssati = 0.25, demott_nuc_tempc = -20.0, max_ni = 4999.e3;
+static const double icenuc_max = 1000.e3, rime_threshold = 2.0, rime_conversion = 0.95;
+static const double ef_si = 0.05, ef_rs = 0.95, ef_rg = 0.75, ef_ri = 0.95;
+static const double autocon_nr_factor = 10.0, timestep_conversion_rime_to_rain = 120.0;
+static const double min_qv = 1.e-10, r1 = 1.e-12, r2 = 1.e-6, low_limit_mass_for_precip = 1.e-9;
+static const double eps = 1.e-15, meters3_to_liters = 1000.0, gonv_min = 1.e2, gonv_max = 1.e6;
+static const double rho_g[9] = {50.0, 100.0, 200.0, 300.0, 400.0, 500.0, 600.0, 700.0, 800.0};
+static const double sc = 0.632, earth_gravity = 9.8;
+static const double pi = 3.1415926536, t0 = 273.15, rho_w = 1000.0, lsub = 2.834e6, lvap0 = 2.5e6;
+static const double rv = 461.5, rdry = 287.04, cp = 1004.0, r_uni = 8.314;
+static const double kap0 = 490.6, kap1 = 17.46, lam0 = 20.78, lam1 = 3.29;
+#define IDX_BG1 6
+#define NRHG 9
+static const double nifa_default_ = 0.5e6 * 0.01; /* nain1*0.01_wp, :102 */
+
+static const double sa[10] = {5.065339, -0.062659, -3.032362, 0.029469, -0.000285,
+                              0.31255, 0.000204, 0.003199, 0.0, -0.015952};
+static const double sb[10] = {0.476221, -0.015896, 0.165977, 0.007468, -0.000141,
+                              0.060366, 0.000079, 0.000594, 0.0, -0.003577};
+static const double tc_tab[9] = {-0.01, -5.0, -10.0, -15.0, -20.0, -25.0, -30.0, -35.0, -40.0};
+static const double ta_na[7] = {10.0, 31.6, 100.0, 316.0, 1000.0, 3160.0, 10000.0};
+static const double ta_ww[9] = {0.01, 0.0316, 0.1, 0.316, 1.0, 3.16, 10.0, 31.6, 100.0};
+static const double ta_tk[7] = {243.15, 253.15, 263.15, 273.15, 283.15, 293.15, 303.15};
+
+/* The bin vectors are literal Fortran constants; build them from the same
+ * decimal literals so they round identically ("1.e-6_wp", "2.e-6_wp", ...). */
+static double lit(int mant, int expo)
+{
+    /* strtod of the literal "<mant>.e<expo>" is the correctly rounded value,
+     * exactly what the Fortran compiler stores for the literal */
+    char buf[32];
+    int i = 0, v = mant;
+    char tmp[8];
+    int n = 0;
+    do { tmp[n++] = (char)('0' + v % 10); v /= 10; } while (v);
+    while (n) buf[i++] = tmp[--n];
+    buf[i++] = '.';
+    buf[i++] = 'e';
+    if (expo < 0) { buf[i++] = '-'; expo = -expo; }
+    n = 0;
+    do { tmp[n++] = (char)('0' + expo % 10); expo /= 10; } while (expo);
+    while (n) buf[i++] = tmp[--n];
+    buf[i] = 0;
+    return strtod(buf, NULL);
+}
+
+static void decade_table(double *out, int n, int e0)
+{
+    for (int idx = 1; idx <= n; ++idx) {
+        int e = e0 + (idx - 1) / 9, d = (idx - 1) % 9 + 1;
+        out[idx - 1] = lit(d, e);
+    }
+}
+
+static double r_c[37], r_i[64], r_r[37], r_s[37], r_g[37], n0r_exp[37], n0g_exp[37];
+static double nt_i[55], nt_in[55];
+
+/* compiler-rt __powidf2: real**integer as lowered by flang (llvm.powi) */
+static double powi(double a, int b)
+{
+    int recip = b < 0;
+    double r = 1.0;
+    for (;;) {
+        if (b & 1) r *= a;
+        b /= 2;
+        if (b == 0) break;
+        a *= a;
+    }
+    return recip ? 1.0 / r : r;
+}
+
+static int nint_(double x) { return (int)lround(x); }
+static double dmax(double a, double b) { return a > b ? a : b; }
+static double dmin(double a, double b) { return a < b ? a : b; }
+static int imax(int a, int b) { return a > b ? a : b; }
+static int imin(int a, int b) { return a < b ? a : b; }
+
+/* ---- derived constants: src/module_mp_tempo_params.F90:411-658 ---- */
+typedef struct {
+    int hail;
+    double av_g[9], bv_g[9];
+    double am_i, am_r, am_g[9], ar_volume, lfus, olfus, roverrv, orv, rho_not, rho_not0, sc3;
+    double d0i, xm0s, xm0g, obmi, obmr, oams, obms, ocms, obmg, oamg[9], ocmg[9];
+    double cce[5][15], ccg[5][15], ocg1[15], ocg2[15];
+    double cie[7], cig[7], oig1, oig2;
+    double cre[13], crg[13], ore1, org1, org2, org3;
+    double cse[17], csg[17];
+    double cge[12][9], cgg[12][9], oge1, ogg1, ogg2, ogg3;
+    double t1_qr_qc, t1_qr_qi, t2_qr_qi, t1_qs_qc, t1_qs_qi, t1_qr_ev, t2_qr_ev;
+    double t1_qs_sd, t2_qs_sd, t1_qs_me, t2_qs_me, t1_qg_sd, t1_qg_me;
+    int nic2, nii2, nir2, nis2, nig2, nii3, nir3, nig3, niin2;
+    double nic1;
+    double dc[100], dtc[100], di[100], dti[100], dr[100], dtr[100], ds[100], dts[100];
+    double dg[100], dtg[100], t_nc[100], hbins[50], dhbins[50];
+} params_t;
+
+static params_t P;   /* run-time constants (hail flag of the run) */
+static params_t PT;  /* table-builder constants (hail-aware forced on) */
+
+#define CCE(a, n) (p->cce[(a) - 1][(n) - 1])
+#define CCG(a, n) (p->ccg[(a) - 1][(n) - 1])
+#define CGE(a, m) (p->cge[(a) - 1][(m) - 1])
+#define CGG(a, m) (p->cgg[(a) - 1][(m) - 1])
+
+/* create_bins, src/module_mp_tempo_params.F90:695-722 */
+static void create_bins(int numbins, double lowbin, double highbin, double *bins, double *deltabins)
+{
+    double xdx[NZMAX];
+    xdx[0] = lowbin;
+    xdx[numbins] = highbin;
+    for (int n = 2; n <= numbins; ++n)
+        xdx[n - 1] = exp((double)(n - 1) / (double)numbins * log(xdx[numbins] / xdx[0]) + log(xdx[0]));
+    for (int n = 1; n <= numbins; ++n) bins[n - 1] = sqrt(xdx[n - 1] * xdx[n]);
+    if (deltabins)
+        for (int n = 1; n <= numbins; ++n) deltabins[n - 1] = xdx[n] - xdx[n - 1];
+}
+
+static void init_params(params_t *p, int hail)
+{
+    memset(p, 0, sizeof(*p));
+    p->hail = hail;
+    /* av_g, bv_g defaults and initialize_graupel_vars (:298-303, :411-426) */
+    static const double avg0[9] = {45.9173813, 67.0867386, 98.0158463, 122.353378, 143.204224,
+                                   161.794724, 178.762115, 194.488785, 209.225876};
+    for (int m = 0; m < 9; ++m) { p->av_g[m] = avg0[m]; p->bv_g[m] = 0.640961647; }
+    p->av_g[IDX_BG1 - 1] = hail ? av_g_new : av_g_old;
+    p->bv_g[IDX_BG1 - 1] = hail ? bv_g_new : bv_g_old;
+
+    /* initialize_parameters (:429-610) */
+    p->am_i = pi * rho_i / 6.0;
+    p->am_r = pi * rho_w / 6.0;
+    for (int m = 0; m < 9; ++m) p->am_g[m] = pi * rho_g[m] / 6.0;
+    p->ar_volume = 4.0 / 3.0 * pi * powi(2.5e-6, 3);
+    p->lfus = lsub - lvap0;
+    p->olfus = 1.0 / p->lfus;
+    p->roverrv = rdry / rv;
+    p->orv = 1.0 / rv;
+    p->rho_not = 101325.0 / (rdry * 298.0);
+    p->rho_not0 = 101325.0 / (rdry * t0);
+    p->sc3 = pow(sc, 1.0 / 3.0);
+    p->d0i = pow(xm0i / p->am_i, 1.0 / bm_i);
+    p->xm0s = am_s * pow(d0s, bm_s);
+    p->xm0g = p->am_g[NRHG - 1] * pow(d0g, bm_g);
+    p->obmi = 1.0 / bm_i;
+    p->obmr = 1.0 / bm_r;
+    p->oams = 1.0 / am_s;
+    p->obms = 1.0 / bm_s;
+    p->ocms = pow(p->oams, p->obms);
+    p->obmg = 1.0 / bm_g;
+    for (int m = 0; m < 9; ++m) {
+        p->oamg[m] = 1.0 / p->am_g[m];
+        p->ocmg[m] = pow(p->oamg[m], p->obmg);
+    }
+    for (int n = 1; n <= 15; ++n) {
+        CCE(1, n) = n + 1.0;
+        CCE(2, n) = bm_r + n + 1.0;
+        CCE(3, n) = bm_r + n + 4.0;
+        CCE(4, n) = n + bv_c + 1.0;
+        CCE(5, n) = bm_r + n + bv_c + 1.0;
+        for (int a = 1; a <= 5; ++a) CCG(a, n) = tgamma(CCE(a, n));
+        p->ocg1[n - 1] = 1.0 / CCG(1, n);
+        p->ocg2[n - 1] = 1.0 / CCG(2, n);
+    }
+    p->cie[0] = mu_i + 1.0;
+    p->cie[1] = bm_i + mu_i + 1.0;
+    p->cie[2] = bm_i + mu_i + bv_i + 1.0;
+    p->cie[3] = mu_i + bv_i + 1.0;
+    p->cie[4] = mu_i + 2.0;
+    p->cie[5] = bm_i * 0.5 + mu_i + bv_i + 1.0;
+    p->cie[6] = bm_i * 0.5 + mu_i + 1.0;
+    for (int n = 0; n < 7; ++n) p->cig[n] = tgamma(p->cie[n]);
+    p->oig1 = 1.0 / p->cig[0];
+    p->oig2 = 1.0 / p->cig[1];
+    p->cre[0] = bm_r + 1.0;
+    p->cre[1] = mu_r + 1.0;
+    p->cre[2] = bm_r + mu_r + 1.0;
+    p->cre[3] = bm_r * 2.0 + mu_r + 1.0;
+    p->cre[4] = mu_r + bv_r + 1.0;
+    p->cre[5] = bm_r + mu_r + bv_r + 1.0;
+    p->cre[6] = bm_r * 0.5 + mu_r + bv_r + 1.0;
+    p->cre[7] = bm_r + mu_r + bv_r + 3.0;
+    p->cre[8] = mu_r + bv_r + 3.0;
+    p->cre[9] = mu_r + 2.0;
+    p->cre[10] = 0.5 * (bv_r + 5.0 + 2.0 * mu_r);
+    p->cre[11] = bm_r * 0.5 + mu_r + 1.0;
+    p->cre[12] = bm_r * 2.0 + mu_r + bv_r + 1.0;
+    for (int n = 0; n < 13; ++n) p->crg[n] = tgamma(p->cre[n]);
+    p->ore1 = 1.0 / p->cre[0];
+    p->org1 = 1.0 / p->crg[0];
+    p->org2 = 1.0 / p->crg[1];
+    p->org3 = 1.0 / p->crg[2];
+    p->cse[0] = bm_s + 1.0;
+    p->cse[1] = bm_s + 2.0;
+    p->cse[2] = bm_s * 2.0;
+    p->cse[3] = bm_s + bv_s + 1.0;
+    p->cse[4] = bm_s * 2.0 + bv_s + 1.0;
+    p->cse[5] = bm_s * 2.0 + 1.0;
+    p->cse[6] = bm_s + mu_s + 1.0;
+    p->cse[7] = bm_s + mu_s + 2.0;
+    p->cse[8] = bm_s + mu_s + 3.0;
+    p->cse[9] = bm_s + mu_s + bv_s + 1.0;
+    p->cse[10] = bm_s * 2.0 + mu_s + bv_s + 1.0;
+    p->cse[11] = bm_s * 2.0 + mu_s + 1.0;
+    p->cse[12] = bv_s + 2.0;
+    p->cse[13] = bm_s + bv_s;
+    p->cse[14] = mu_s + 1.0;
+    p->cse[15] = 1.0 + (1.0 + bv_s) / 2.0;
+    p->cse[16] = bm_s + bv_s + 2.0;
+    for (int n = 0; n < 17; ++n) p->csg[n] = tgamma(p->cse[n]);
+    for (int m = 1; m <= 9; ++m) {
+        CGE(1, m) = bm_g + 1.0;
+        CGE(2, m) = mu_g + 1.0;
+        CGE(3, m) = bm_g + mu_g + 1.0;
+        CGE(4, m) = bm_g * 2. + mu_g + 1.0;
+        CGE(10, m) = mu_g + 2.0;
+        CGE(12, m) = bm_g * 0.5 + mu_g + 1.0;
+    }
+    for (int m = 1; m <= 9; ++m) {
+        CGE(5, m) = bm_g * 2.0 + mu_g + p->bv_g[m - 1] + 1.0;
+        CGE(6, m) = bm_g + mu_g + p->bv_g[m - 1] + 1.0;
+        CGE(7, m) = bm_g * 0.5 + mu_g + p->bv_g[m - 1] + 1.0;
+        CGE(8, m) = mu_g + p->bv_g[m - 1] + 1.0;
+        CGE(9, m) = mu_g + p->bv_g[m - 1] + 3.0;
+        CGE(11, m) = 0.5 * (p->bv_g[m - 1] + 5.0 + 2.0 * mu_g);
+    }
+    for (int m = 1; m <= 9; ++m)
+        for (int n = 1; n <= 12; ++n) CGG(n, m) = tgamma(CGE(n, m));
+    p->oge1 = 1.0 / CGE(1, 1);
+    p->ogg1 = 1.0 / CGG(1, 1);
+    p->ogg2 = 1.0 / CGG(2, 1);
+    p->ogg3 = 1.0 / CGG(3, 1);
+    p->t1_qr_qc = pi * 0.25 * av_r * p->crg[8];
+    p->t1_qr_qi = pi * 0.25 * av_r * p->crg[8];
+    p->t2_qr_qi = pi * 0.25 * p->am_r * av_r * p->crg[7];
+    p->t1_qs_qc = pi * 0.25 * av_s;
+    p->t1_qs_qi = pi * 0.25 * av_s;
+    p->t1_qr_ev = 0.78 * p->crg[9];
+    p->t2_qr_ev = 0.308 * p->sc3 * sqrt(av_r) * p->crg[10];
+    p->t1_qs_sd = 0.86;
+    p->t2_qs_sd = 0.28 * p->sc3 * sqrt(av_s);
+    p->t1_qs_me = pi * 4.0 * c_sqrd * p->olfus * 0.86;
+    p->t2_qs_me = pi * 4.0 * c_sqrd * p->olfus * 0.28 * p->sc3 * sqrt(av_s);
+    p->t1_qg_sd = 0.86 * CGG(10, 1);
+    p->t1_qg_me = pi * 4.0 * c_cube * p->olfus * 0.86 * CGG(10, 1);
+
+    /* initialize_bins_for_tables (:613-658) */
+    p->nic2 = nint_(log10(r_c[0]));
+    p->nii2 = nint_(log10(r_i[0]));
+    p->nir2 = nint_(log10(r_r[0]));
+    p->nis2 = nint_(log10(r_s[0]));
+    p->nig2 = nint_(log10(r_g[0]));
+    p->nii3 = nint_(log10(nt_i[0]));
+    p->nir3 = nint_(log10(n0r_exp[0]));
+    p->nig3 = nint_(log10(n0g_exp[0]));
+    p->niin2 = nint_(log10(nt_in[0]));
+    p->dc[0] = d0c;
+    p->dtc[0] = d0c;
+    for (int n = 2; n <= 100; ++n) {
+        p->dc[n - 1] = p->dc[n - 2] + 1.0e-6;
+        p->dtc[n - 1] = (p->dc[n - 1] - p->dc[n - 2]);
+    }
+    create_bins(100, p->d0i, 2.0 * d0s, p->di, p->dti);
+    create_bins(100, d0r, 0.005, p->dr, p->dtr);
+    create_bins(100, d0s, 0.02, p->ds, p->dts);
+    create_bins(100, d0g, 0.05, p->dg, p->dtg);
+    create_bins(100, 1.0, 3000.0, p->t_nc, NULL);
+    for (int n = 0; n < 100; ++n) p->t_nc[n] = p->t_nc[n] * 1.0e6;
+    p->nic1 = log(p->t_nc[99] / p->t_nc[0]);
+    /* initialize_bins_for_hail_size (:661-671) */
+    create_bins(50, 500.e-6, 0.075, p->hbins, p->dhbins);
+}
+
+/* ---- init-time tables: src/module_mp_tempo_utils.F90:349-527 ---- */
+static double t_efrw[100][100], t_efsw[100][100];  /* [j(cloud)][i] -> Fortran (i,j) */
+static double tnc_wev[100][37][100];               /* [k][j][i] -> Fortran (i,j,k) */
+static double tps_iaus[55][64], tni_iaus[55][64], tpi_ide[55][64];
+#define EFRW(i, j) t_efrw[(j) - 1][(i) - 1]
+#define EFSW(i, j) t_efsw[(j) - 1][(i) - 1]
+
+static int get_nuc(double nc) /* src/module_mp_tempo_utils.F90:334-346 */
+{
+    if ((nu_c_scale / nc) >= 12.5) return 15;
+    return imin(15, nint_(nu_c_scale / nc) + 2);
+}
+
+static void compute_efrw(const params_t *p) /* :349-400 */
+{
+    for (int j = 1; j <= 100; ++j)
+        for (int i = 1; i <= 100; ++i) {
+            double ef_rw = 0.0, dcj = p->dc[j - 1], dri = p->dr[i - 1];
+            double pp = dcj / dri;
+            if (dri < 50.e-6 || dcj < 3.e-6) {
+                EFRW(i, j) = 0.0;
+            } else if (pp > 0.25) {
+                double x = dcj * 1.e6;
+                if (dri < 75.e-6)
+                    ef_rw = 0.026794 * x - 0.20604;
+                else if (dri < 125.e-6)
+                    ef_rw = -0.00066842 * x * x + 0.061542 * x - 0.37089;
+                else if (dri < 175.e-6)
+                    ef_rw = 4.091e-06 * x * x * x * x - 0.00030908 * x * x * x + 0.0066237 * x * x -
+                            0.0013687 * x - 0.073022;
+                else if (dri < 250.e-6)
+                    ef_rw = 9.6719e-5 * x * x * x - 0.0068901 * x * x + 0.17305 * x - 0.65988;
+                else if (dri < 350.e-6)
+                    ef_rw = 9.0488e-5 * x * x * x - 0.006585 * x * x + 0.16606 * x - 0.56125;
+                else
+                    ef_rw = 0.00010721 * x * x * x - 0.0072962 * x * x + 0.1704 * x - 0.46929;
+            } else {
+                double vtr = -0.1021 + 4.932e3 * dri - 0.9551e6 * dri * dri + 0.07934e9 * dri * dri * dri -
+                             0.002362e12 * dri * dri * dri * dri;
+                double stokes = dcj * dcj * vtr * rho_w / (9.0 * 1.718e-5 * dri);
+                double reynolds = 9.0 * stokes / (pp * pp * rho_w);
+                double f = log(reynolds);
+                double g = -0.1007 - 0.358 * f + 0.0261 * f * f;
+                double k0 = exp(g);
+                double z = log(stokes / (k0 + 1.e-15));
+                double h = 0.1465 + 1.302 * z - 0.607 * z * z + 0.293 * z * z * z;
+                double yc0 = 2.0 / pi * atan(h);
+                ef_rw = (yc0 + pp) * (yc0 + pp) / ((1. + pp) * (1. + pp));
+            }
+            EFRW(i, j) = dmax(0.0, dmin(ef_rw, 0.95));
+        }
+}
+
+static void compute_efsw(const params_t *p) /* :403-440 */
+{
+    for (int j = 1; j <= 100; ++j) {
+        double dcj = p->dc[j - 1];
+        double vtc = 1.19e4 * (1.0e4 * dcj * dcj * 0.25);
+        for (int i = 1; i <= 100; ++i) {
+            double dsi = p->ds[i - 1];
+            double vts = av_s * pow(dsi, bv_s) * exp(-fv_s * dsi) - vtc;
+            double ds_m = pow(am_s * pow(dsi, bm_s) / p->am_r, p->obmr);
+            double pp = dcj / ds_m;
+            if (pp > 0.25 || dsi < d0s || dcj < 6.e-6 || vts < 1.e-3) {
+                EFSW(i, j) = 0.0;
+            } else {
+                double stokes = dcj * dcj * vts * rho_w / (9. * 1.718e-5 * ds_m);
+                double reynolds = 9.0 * stokes / (pp * pp * rho_w);
+                double f = log(reynolds);
+                double g = -0.1007 - 0.358 * f + 0.0261 * f * f;
+                double k0 = exp(g);
+                double z = log(stokes / (k0 + 1.e-15));
+                double h = 0.1465 + 1.302 * z - 0.607 * z * z + 0.293 * z * z * z;
+                double yc0 = 2.0 / pi * atan(h);
+                double ef_sw = (yc0 + pp) * (yc0 + pp) / ((1. + pp) * (1. + pp));
+                EFSW(i, j) = dmax(0.0, dmin(ef_sw, 0.95));
+            }
+        }
+    }
+}
+
+/* calc_gamma_series / calc_gamma_cf / calc_gamma_p, src/module_mp_tempo_utils.F90:27-145 */
+static double gamma_series(double a, double x)
+{
+    double aj = a, sum_term = 1.0 / tgamma(aj + 1.0), sum = sum_term;
+    for (int k = 1; k <= 100; ++k) {
+        aj = aj + 1.0;
+        sum_term = sum_term * x / aj;
+        sum = sum + sum_term;
+        if (fabs(sum_term) < (fabs(sum) * 1.e-7)) break;
+    }
+    return sum * pow(x, a) * exp(-x);
+}
+
+static double gamma_cf(double a, double x)
+{
+    const double offset = 1.e-30;
+    double f0 = offset, b = 1.0 - a + x, d = 1.0 / b, c = b + (1.0 / f0), delta = c * d;
+    double f = f0 * delta;
+    for (int k = 1; k <= 100; ++k) {
+        double aj = k * (a - k);
+        b = b + 2.0;
+        d = b + aj * d;
+        if (fabs(d) < offset) d = offset;
+        c = b + aj / c;
+        if (fabs(c) < offset) c = offset;
+        d = 1.0 / d;
+        delta = c * d;
+        f = f * delta;
+        if (fabs(delta - 1.0) < 1.e-7) break;
+    }
+    return exp(-x + a * log(x)) * f / tgamma(a);
+}
+
+double oracle_gamma_p(double a, double x)
+{
+    if ((x < 0.0) || (a <= 0.0)) return NAN;
+    if (x < (a + 1.0)) return gamma_series(a, x);
+    return 1.0 - gamma_cf(a, x);
+}
+
+static void qi_aut_qs(const params_t *p) /* :443-490 */
+{
+    for (int j = 1; j <= 55; ++j)
+        for (int i = 1; i <= 64; ++i) {
+            double lami = pow(p->am_i * p->cig[1] * p->oig1 * nt_i[j - 1] / r_i[i - 1], p->obmi);
+            double di_mean = (bm_i + mu_i + 1.) / lami;
+            double n0_i = nt_i[j - 1] * p->oig1 * pow(lami, p->cie[0]);
+            double t1 = 0., t2 = 0.;
+            if (di_mean > 5. * d0s) {
+                t1 = r_i[i - 1];
+                t2 = nt_i[j - 1];
+                tpi_ide[j - 1][i - 1] = 0.;
+            } else if (di_mean < p->d0i) {
+                t1 = 0.;
+                t2 = 0.;
+                tpi_ide[j - 1][i - 1] = 1.;
+            } else {
+                double xlimit_intg = lami * d0s;
+                tpi_ide[j - 1][i - 1] = oracle_gamma_p(mu_i + 2.0, xlimit_intg);
+                for (int n2 = 1; n2 <= 100; ++n2) {
+                    double n_i = n0_i * pow(p->di[n2 - 1], mu_i) * exp(-lami * p->di[n2 - 1]) * p->dti[n2 - 1];
+                    if (p->di[n2 - 1] >= d0s) {
+                        t1 = t1 + n_i * p->am_i * pow(p->di[n2 - 1], bm_i);
+                        t2 = t2 + n_i;
+                    }
+                }
+            }
+            tps_iaus[j - 1][i - 1] = t1;
+            tni_iaus[j - 1][i - 1] = t2;
+        }
+}
+
+static void compute_drop_evap(const params_t *p) /* :493-527 (only tnc_wev is used) */
+{
+    double massc[100], n_c[100];
+    for (int n = 0; n < 100; ++n) massc[n] = p->am_r * pow(p->dc[n], bm_r);
+    for (int k = 1; k <= 100; ++k) {
+        int nu_c = get_nuc(p->t_nc[k - 1]);
+        for (int j = 1; j <= 37; ++j) {
+            double lamc = pow(p->t_nc[k - 1] * p->am_r * CCG(2, nu_c) * p->ocg1[nu_c - 1] / r_c[j - 1], p->obmr);
+            double n0_c = p->t_nc[k - 1] * p->ocg1[nu_c - 1] * pow(lamc, CCE(1, nu_c));
+            for (int i = 1; i <= 100; ++i) {
+                n_c[i - 1] = n0_c * powi(p->dc[i - 1], nu_c) * exp(-lamc * p->dc[i - 1]) * p->dtc[i - 1];
+                double summ2 = 0.0;
+                for (int n = 1; n <= i; ++n) summ2 = summ2 + n_c[n - 1];
+                tnc_wev[k - 1][j - 1][i - 1] = summ2;
+            }
+        }
+    }
+    (void)massc;
+}
+
+/* ---- big tables (reference file order, Fortran column-major) ---- */
+static float g_ccn[8820];
+static const double *g_fz, *g_qs, *g_qg;
+static int g_flags[18];
+
+#define CCN(i, j, k, l, m) g_ccn[((i) - 1) + 7 * (((j) - 1) + 9 * (((k) - 1) + 7 * (((l) - 1) + 5 * ((m) - 1))))]
+#define FZR(a, i, j, k, m) g_fz[(size_t)(a) * 3388275 + ((i) - 1) + 37 * (((j) - 1) + 37 * (((k) - 1) + 45 * ((m) - 1)))]
+#define FZC(a, i, j, k, m) g_fz[(size_t)4 * 3388275 + (size_t)(a) * 9157500 + ((i) - 1) + 37 * (((j) - 1) + 100 * (((k) - 1) + 45 * ((m) - 1)))]
+#define QS(a, i, j, k, m) g_qs[(size_t)(a) * 455877 + ((i) - 1) + 37 * (((j) - 1) + 9 * (((k) - 1) + 37 * ((m) - 1)))]
+#define QG(a, i, j, n3, k, m) g_qg[(size_t)(a) * 16867449 + ((i) - 1) + 37 * (((j) - 1) + 37 * (((n3) - 1) + 9 * (((k) - 1) + 37 * ((m) - 1))))]
+enum { TCS_RACS1, TMR_RACS1, TCS_RACS2, TMR_RACS2, TCR_SACR1, TMS_SACR1, TCR_SACR2, TMS_SACR2,
+       TNR_RACS1, TNR_RACS2, TNR_SACR1, TNR_SACR2 };
+enum { TCG_RACG, TMR_RACG, TCR_GACR, TNR_RACG, TNR_GACR };
+enum { TPI_QRFZ, TNI_QRFZ, TPG_QRFZ, TNR_QRFZ };
+
+static void init_bin_literals(void)
+{
+    decade_table(r_c, 37, -6);
+    decade_table(r_i, 64, -10);
+    decade_table(r_r, 37, -6);
+    decade_table(r_s, 37, -6);
+    decade_table(r_g, 37, -6);
+    decade_table(n0r_exp, 37, 6);
+    decade_table(n0g_exp, 37, 2);
+    decade_table(nt_i, 55, 0);
+    decade_table(nt_in, 55, 0);
+}
+
+int oracle_init(const int *flags, const float *ccn, const double *fz, const double *qs,
+                const double *qg)
+{
+    memcpy(g_flags, flags, sizeof(g_flags));
+    if (flags[2] || flags[3] || flags[14] || flags[15]) return -1;
+    init_bin_literals();
+    init_params(&P, flags[1] != 0);
+    init_params(&PT, 1);
+    compute_efrw(&P);
+    compute_efsw(&P);
+    compute_drop_evap(&P);
+    qi_aut_qs(&P);
+    if (ccn) memcpy(g_ccn, ccn, sizeof(g_ccn));
+    g_fz = fz;
+    g_qs = qs;
+    g_qg = qg;
+    return 0;
+}
+
+/* ---- per-point utilities ---- */
+double oracle_rslf(double p, double t) /* src/module_mp_tempo_utils.F90:285-309 */
+{
+    const double c0 = .611583699E03, c1 = .444606896E02, c2 = .143177157e01, c3 = .264224321e-1;
+    const double c4 = .299291081e-3, c5 = .203154182e-5, c6 = .702620698e-8, c7 = .379534310e-11;
+    const double c8 = -.321582393e-13;
+    double x = dmax(-80.0, t - 273.16);
+    double esl = c0 + x * (c1 + x * (c2 + x * (c3 + x * (c4 + x * (c5 + x * (c6 + x * (c7 + x * c8)))))));
+    esl = dmin(esl, p * 0.15);
+    return .622 * esl / (p - esl);
+}
+
+double oracle_rsif(double p, double t) /* :312-331 */
+{
+    const double c0 = .609868993e03, c1 = .499320233e02, c2 = .184672631e01, c3 = .402737184e-1;
+    const double c4 = .565392987e-3, c5 = .521693933e-5, c6 = .307839583e-7, c7 = .105785160e-9;
+    const double c8 = .161444444e-12;
+    double x = dmax(-80.0, t - 273.16);
+    double esi = c0 + x * (c1 + x * (c2 + x * (c3 + x * (c4 + x * (c5 + x * (c6 + x * (c7 + x * c8)))))));
+    esi = dmin(esi, p * 0.15);
+    return .622 * esi / dmax(1.e-4, (p - esi));
+}
+
+static double poly_loga(double tc, double cc)
+{
+    return sa[0] + sa[1] * tc + sa[2] * cc + sa[3] * tc * cc + sa[4] * tc * tc + sa[5] * cc * cc +
+           sa[6] * tc * tc * cc + sa[7] * tc * cc * cc + sa[8] * tc * tc * tc + sa[9] * cc * cc * cc;
+}
+static double poly_b(double tc, double cc)
+{
+    return sb[0] + sb[1] * tc + sb[2] * cc + sb[3] * tc * cc + sb[4] * tc * tc + sb[5] * cc * cc +
+           sb[6] * tc * tc * cc + sb[7] * tc * cc * cc + sb[8] * tc * tc * tc + sb[9] * cc * cc * cc;
+}
+
+/* snow_moments, src/module_mp_tempo_utils.F90:148-282.  want: bitmask of
+ * optional outputs (1 ns, 2 smo0, 4 smo1, 8 smo2, 16 smoe, 32 smof, 64 smog,
+ * 128 smoz).  out: smob, smoc, ns, smo0, smo1, smo2, smoe, smof, smog, smoz */
+static void snow_moments(const params_t *p, double rs, double tc, int want, double *o)
+{
+    double loga_, a_, b_, smo2_;
+    o[0] = rs * p->oams;
+    /* bm_s == 2 branch */
+    loga_ = sa[0] + sa[1] * tc + sa[2] * bm_s + sa[3] * tc * bm_s + sa[4] * tc * tc + sa[5] * bm_s * bm_s +
+            sa[6] * tc * tc * bm_s + sa[7] * tc * bm_s * bm_s + sa[8] * tc * tc * tc + sa[9] * bm_s * bm_s * bm_s;
+    a_ = pow(10.0, loga_);
+    b_ = sb[0] + sb[1] * tc + sb[2] * bm_s + sb[3] * tc * bm_s + sb[4] * tc * tc + sb[5] * bm_s * bm_s +
+         sb[6] * tc * tc * bm_s + sb[7] * tc * bm_s * bm_s + sb[8] * tc * tc * tc + sb[9] * bm_s * bm_s * bm_s;
+    smo2_ = pow(o[0] / a_, 1.0 / b_);
+    if (want & 8) o[5] = smo2_;
+    /* bm+1 moment */
+    double c1 = p->cse[0];
+    loga_ = poly_loga(tc, c1);
+    a_ = pow(10.0, loga_);
+    b_ = poly_b(tc, c1);
+    o[1] = a_ * pow(smo2_, b_);
+    /* 0th moment */
+    loga_ = sa[0] + sa[1] * tc + sa[4] * tc * tc + sa[8] * tc * tc * tc;
+    a_ = pow(10.0, loga_);
+    b_ = sb[0] + sb[1] * tc + sb[4] * tc * tc + sb[8] * tc * tc * tc;
+    if (want & 2) o[3] = a_ * pow(smo2_, b_);
+    /* 1st moment */
+    loga_ = sa[0] + sa[1] * tc + sa[2] + sa[3] * tc + sa[4] * tc * tc + sa[5] + sa[6] * tc * tc + sa[7] * tc +
+            sa[8] * tc * tc * tc + sa[9];
+    a_ = pow(10.0, loga_);
+    b_ = sb[0] + sb[1] * tc + sb[2] + sb[3] * tc + sb[4] * tc * tc + sb[5] + sb[6] * tc * tc + sb[7] * tc +
+         sb[8] * tc * tc * tc + sb[9];
+    if (want & 4) o[4] = a_ * pow(smo2_, b_);
+    /* ns */
+    double m0 = o[0] / o[1];
+    double mrat = o[0] * m0 * m0 * m0;
+    double slam1 = m0 * lam0, slam2 = m0 * lam1;
+    if (want & 1)
+        o[2] = mrat * kap0 / slam1 + mrat * kap1 * pow(m0, mu_s) * p->csg[14] / pow(slam2, p->cse[14]);
+    /* bv_s+2 */
+    double c13 = p->cse[12];
+    loga_ = poly_loga(tc, c13);
+    a_ = pow(10.0, loga_);
+    b_ = poly_b(tc, c13);
+    if (want & 16) o[6] = a_ * pow(smo2_, b_);
+    double c16 = p->cse[15];
+    loga_ = poly_loga(tc, c16);
+    a_ = pow(10.0, loga_);
+    b_ = poly_b(tc, c16);
+    if (want & 32) o[7] = a_ * pow(smo2_, b_);
+    double c17 = p->cse[16];
+    loga_ = poly_loga(tc, c17);
+    a_ = pow(10.0, loga_);
+    b_ = poly_b(tc, c17);
+    if (want & 64) o[8] = a_ * pow(smo2_, b_);
+    double c3 = p->cse[2];
+    loga_ = poly_loga(tc, c3);
+    a_ = pow(10.0, loga_);
+    b_ = poly_b(tc, c3);
+    /* the reference raises the smo2 *output* here (:280); it is only
+     * requested together with smo2, so it equals smo2_ */
+    if (want & 128) o[9] = a_ * pow(smo2_, b_);
+}
+
+void oracle_snow_moments(double rs, double tc, double *out10)
+{
+    snow_moments(&P, rs, tc, 255, out10);
+}
+
+/* ---- table index helpers: src/module_mp_tempo_main.F90:2493-2635, 2839-2858 ---- */
+static int decade(double x, int *n_out)
+{
+    int nx = nint_(log10(x)), n = nx + 1;
+    for (int nn = nx - 1; nn <= nx + 1; ++nn) {
+        n = nn;
+        if ((x / powi(10.0, nn)) >= 1.0 && (x / powi(10.0, nn)) < 10.0) break;
+    }
+    *n_out = n;
+    return n;
+}
+static int decade_index(double x, int base, int nmax)
+{
+    int n;
+    decade(x, &n);
+    int idx = (int)(x / powi(10.0, n)) + 10 * (n - base) - (n - base);
+    return imax(1, imin(idx, nmax));
+}
+
+static int temperature_index(double tempk)
+{
+    double tempc = tempk - t0;
+    int idx_t = (int)((tempc - 2.5) / 5.0) - 1;
+    idx_t = imax(1, -idx_t);
+    return imin(idx_t, 9);
+}
+
+static void rain_index(const params_t *p, double rr, double ilamr, int *idx_r, int *idx_r1)
+{
+    *idx_r = decade_index(rr, p->nir2, 37);
+    double lamr = 1. / ilamr;
+    double lam_exp = lamr * pow(p->crg[2] * p->org2 * p->org1, bm_r);
+    double n0_exp = p->org1 * rr / p->am_r * pow(lam_exp, p->cre[0]);
+    *idx_r1 = decade_index(n0_exp, p->nir3, 37);
+}
+
+static void graupel_index(const params_t *p, double rg, double ilamg, int idx, int *idx_g, int *idx_g1)
+{
+    *idx_g = decade_index(rg, p->nig2, 37);
+    double lamg = 1. / ilamg;
+    double lam_exp = lamg * pow(CGG(3, 1) * p->ogg2 * p->ogg1, bm_g);
+    double n0_exp = p->ogg1 * rg / p->am_g[idx - 1] * pow(lam_exp, CGE(1, 1));
+    *idx_g1 = decade_index(n0_exp, p->nig3, 37);
+}
+
+static void cloud_index(const params_t *p, double rc, double nc, int *idx_c, int *idx_n)
+{
+    *idx_c = decade_index(rc, p->nic2, 37);
+    int n = nint_(1.0 + (double)100 * log(nc / p->t_nc[0]) / p->nic1);
+    *idx_n = imax(1, imin(n, 100));
+}
+
+static void ice_index(const params_t *p, double ri, double ni, int *idx_i, int *idx_i1)
+{
+    *idx_i = decade_index(ri, p->nii2, 64);
+    *idx_i1 = decade_index(ni, p->nii3, 55);
+}
+
+static int in_index(const params_t *p, double xni)
+{
+    if (xni > nt_in[0]) return decade_index(xni, p->niin2, 55);
+    return 1;
+}
+
+/* ---- process-rate tendencies: src/module_mp_tempo_main.F90:47-64 ---- */
+enum {
+    prr_wau, pnr_wau, pnc_wau, prr_rcw, pnc_rcw, pnr_rcr,
+    prs_scw, pnc_scw, png_scw, pbg_scw, prg_gcw, pnc_gcw, pbg_gcw, pri_ihm, pni_ihm, prs_ihm, prg_ihm, prg_scw,
+    prr_rcs, pnr_rcs, prg_rcs, png_rcs, prs_rcs, pbg_rcs,
+    prr_rcg, pnr_rcg, prg_rcg, png_rcg, pbg_rcg,
+    pri_inu, pni_inu, pri_iha, pni_iha,
+    pri_wfz, pni_wfz,
+    prg_rfz, png_rfz, pnr_rfz, pri_rfz, pni_rfz, pbg_rfz,
+    prs_sde, pri_ide, pni_ide, prs_ide, prg_gde, png_gde,
+    pni_iau, prs_iau,
+    prr_sml, prr_gml, pbg_sml, pbg_gml, pnr_sml, pnr_gml,
+    prr_rci, pnr_rci, pri_rci, pni_rci, prg_rci, png_rci, pbg_rci,
+    pni_sci, prs_sci,
+    prw_vcd, pnc_wcd, prv_rev, pnr_rev,
+    pna_rca, pna_sca, pna_gca, pnd_rcd, pnd_scd, pnd_gcd,
+    NTEND
+};
+static double tendw[NTEND][NZMAX];
+#define TD(name, k) tendw[name][(k)]
+
+/* ---- per-column state ---- */
+typedef struct {
+    int nz;
+    /* presence of optional arguments (tempo_main's present()) */
+    int has_nc, has_nwfa, has_nifa, has_ng_qb, has_qb, has_land, land;
+    double dt, odt;
+    int first_step;
+    /* 1-d arrays: in/out state */
+    double *t1d, *p1d, *w1d, *dz1d, *qv1d, *qc1d, *qr1d, *qi1d, *qs1d, *qg1d, *ni1d, *nr1d;
+    double *nc1d, *nwfa1d, *nifa1d, *ng1d, *qb1d;
+    /* tendencies */
+    double tten[NZMAX], qvten[NZMAX], qcten[NZMAX], qiten[NZMAX], qrten[NZMAX], qsten[NZMAX];
+    double qgten[NZMAX], qbten[NZMAX], niten[NZMAX], nrten[NZMAX], ncten[NZMAX], ngten[NZMAX];
+    double nwfaten[NZMAX], nifaten[NZMAX];
+    int l_qc[NZMAX], l_qi[NZMAX], l_qr[NZMAX], l_qs[NZMAX], l_qg[NZMAX], idx_bg[NZMAX];
+    double temp[NZMAX], pres[NZMAX], qv[NZMAX], rho[NZMAX], rhof[NZMAX], rhof2[NZMAX];
+    double qvs[NZMAX], qvsi[NZMAX], delqvs[NZMAX], satw[NZMAX], sati[NZMAX], ssatw[NZMAX], ssati[NZMAX];
+    double diffu[NZMAX], visco[NZMAX], vsc2[NZMAX], tcond[NZMAX], lvap[NZMAX], ocp[NZMAX], lvt2[NZMAX];
+    double rc[NZMAX], ri[NZMAX], rr[NZMAX], rs[NZMAX], rg[NZMAX], rb[NZMAX];
+    double ni[NZMAX], nr[NZMAX], nc[NZMAX], ng[NZMAX], nwfa[NZMAX], nifa[NZMAX];
+    double ilamc[NZMAX], ilami[NZMAX], ilamr[NZMAX], ilamg[NZMAX];
+    double mvd_r[NZMAX], mvd_c[NZMAX], mvd_g[NZMAX];
+    double smob[NZMAX], smo2[NZMAX], smo1[NZMAX], smo0[NZMAX], smoc[NZMAX], smoe[NZMAX];
+    double smof[NZMAX], smog[NZMAX], ns[NZMAX], smoz[NZMAX];
+    double xrx[NZMAX], xnx[NZMAX], xncx[NZMAX], xngx[NZMAX], xqbx[NZMAX], ncsave[NZMAX];
+    double vtrr[NZMAX + 1], vtnr[NZMAX + 1], vtrs[NZMAX + 1], vtri[NZMAX + 1], vtni[NZMAX + 1];
+    double vtrg[NZMAX + 1], vtng[NZMAX + 1], vtrc[NZMAX + 1], vtnc[NZMAX + 1], vtboost[NZMAX];
+    int supersaturated;
+    /* diagnostics */
+    double rain_precip, cloud_precip, ice_precip, snow_precip, graupel_precip, frozen_fraction, frz_rain;
+    double refl[NZMAX], re_c[NZMAX], re_i[NZMAX], re_s[NZMAX], max_hail[NZMAX];
+    int early_exit;
+} col_t;
+
+static col_t C_;
+
+/* aerosol_check_and_update, src/module_mp_tempo_main.F90:867-888 */
+static void aerosol_check_and_update(col_t *c)
+{
+    for (int k = 0; k < c->nz; ++k) {
+        if (c->has_nwfa) c->nwfa[k] = (c->nwfa1d[k] + c->nwfaten[k] * c->dt) * c->rho[k];
+        if (c->has_nifa) c->nifa[k] = (c->nifa1d[k] + c->nifaten[k] * c->dt) * c->rho[k];
+        c->nwfa[k] = dmax(nwfa_default * c->rho[k], dmin(aero_max * c->rho[k], c->nwfa[k]));
+        c->nifa[k] = dmax(nifa_default_ * c->rho[k], dmin(aero_max * c->rho[k], c->nifa[k]));
+    }
+}
+
+/* init_water/ice_friendly_aerosols, src/module_mp_tempo_aerosols.F90:14-73 */
+static void init_friendly_aerosols(const col_t *c, double *out, double n0, double n1)
+{
+    double hgt[NZMAX] = {0.0}, h_01, ni3;
+    int nz = c->nz;
+    for (int k = 1; k < nz; ++k) hgt[k] = hgt[k - 1] + c->dz1d[k];
+    if (hgt[0] <= 1000.0) h_01 = 0.8;
+    else if (hgt[0] >= 2500.0) h_01 = 0.01;
+    else h_01 = 0.8 * cos(hgt[0] * 0.001 - 1.0);
+    ni3 = -1.0 * log(n1 / n0) / h_01;
+    out[0] = n1 + n0 * exp(-((hgt[1] - hgt[0]) / 1000.0) * ni3);
+    for (int k = 1; k < nz; ++k) out[k] = n1 + n0 * exp(-((hgt[k] - hgt[0]) / 1000.0) * ni3);
+}
+
+/* cloud_check_and_update, :891-973.  nc1d may be NULL (absent); ncsave used
+ * when use_ncsave. */
+static void cloud_check_and_update(col_t *c, double *qc1d, double *nc1d, int use_ncsave)
+{
+    const params_t *p = &P;
+    for (int k = 0; k < c->nz; ++k) {
+        int hit_limit = 0, nu_c;
+        double lamc, xdc;
+        if (qc1d[k] + c->qcten[k] * c->dt > r1) {
+            c->l_qc[k] = 1;
+            c->rc[k] = (qc1d[k] + c->qcten[k] * c->dt) * c->rho[k];
+            qc1d[k] = qc1d[k] + c->qcten[k] * c->dt;
+            if (nc1d) {
+                c->nc[k] = dmax(nt_c_min, (nc1d[k] + c->ncten[k] * c->dt) * c->rho[k]);
+                if (c->nc[k] <= nt_c_min) { hit_limit = 1; c->nc[k] = nt_c_min; }
+                if (c->nc[k] > nt_c_max) { hit_limit = 1; c->nc[k] = nt_c_max; }
+                nu_c = get_nuc(c->nc[k]);
+                lamc = pow(c->nc[k] * p->am_r * CCG(2, nu_c) * p->ocg1[nu_c - 1] / c->rc[k], p->obmr);
+                xdc = (bm_r + nu_c + 1.0) / lamc;
+                if (xdc < d0c) {
+                    lamc = CCE(2, nu_c) / d0c;
+                    hit_limit = 1;
+                } else if (xdc > d0r * 2.0) {
+                    lamc = CCE(2, nu_c) / (d0r * 2.0);
+                    hit_limit = 1;
+                }
+                c->nc[k] = CCG(1, nu_c) * p->ocg2[nu_c - 1] * c->rc[k] / p->am_r * pow(lamc, bm_r);
+                if (hit_limit) c->ncten[k] = (c->nc[k] / c->rho[k] - nc1d[k]) * c->odt;
+                nc1d[k] = dmax(nt_c_min / c->rho[k],
+                               dmin(CCG(1, nu_c) * p->ocg2[nu_c - 1] * qc1d[k] / p->am_r * pow(lamc, bm_r),
+                                    nt_c_max / c->rho[k]));
+            } else {
+                if (use_ncsave) c->nc[k] = c->ncsave[k];
+                else c->nc[k] = nt_c_l;
+            }
+            nu_c = get_nuc(c->nc[k]);
+            lamc = pow(c->nc[k] * p->am_r * CCG(2, nu_c) * p->ocg1[nu_c - 1] / c->rc[k], p->obmr);
+            c->ilamc[k] = 1.0 / lamc;
+            c->mvd_c[k] = dmax(dmin((3.0 + nu_c + 0.672) * c->ilamc[k], d0r), d0c);
+        } else {
+            c->l_qc[k] = 0;
+            c->rc[k] = r1;
+            c->nc[k] = nt_c_min;
+            c->mvd_c[k] = d0c;
+            c->ilamc[k] = 0.0;
+            c->qcten[k] = -qc1d[k] * c->odt;
+            qc1d[k] = 0.0;
+            if (nc1d) {
+                c->ncten[k] = -nc1d[k] * c->odt;
+                nc1d[k] = 0.0;
+            }
+        }
+    }
+}
+
+/* rain_check_and_update, :976-1042 */
+static void rain_check_and_update(col_t *c, double *qr1d, double *nr1d)
+{
+    const params_t *p = &P;
+    for (int k = 0; k < c->nz; ++k) {
+        int hit_limit = 0;
+        double lamr;
+        if (qr1d[k] + c->qrten[k] * c->dt > r1) {
+            c->l_qr[k] = 1;
+            c->rr[k] = (qr1d[k] + c->qrten[k] * c->dt) * c->rho[k];
+            qr1d[k] = qr1d[k] + c->qrten[k] * c->dt;
+            c->nr[k] = dmax(r2, (nr1d[k] + c->nrten[k] * c->dt) * c->rho[k]);
+            if (c->nr[k] <= r2) {
+                hit_limit = 1;
+                c->mvd_r[k] = 1.0e-3;
+                lamr = (3.0 + mu_r + 0.672) / c->mvd_r[k];
+                c->nr[k] = p->crg[1] * p->org3 * c->rr[k] * pow(lamr, bm_r) / p->am_r;
+            }
+            lamr = pow(p->am_r * p->crg[2] * p->org2 * c->nr[k] / c->rr[k], p->obmr);
+            c->mvd_r[k] = (3.0 + mu_r + 0.672) / lamr;
+            if (c->mvd_r[k] > d0r_max) {
+                hit_limit = 1;
+                c->mvd_r[k] = d0r_max;
+                lamr = (3.0 + mu_r + 0.672) / c->mvd_r[k];
+                c->nr[k] = p->crg[1] * p->org3 * c->rr[k] * pow(lamr, bm_r) / p->am_r;
+            } else if (c->mvd_r[k] < d0r * 0.75) {
+                hit_limit = 1;
+                c->mvd_r[k] = d0r * 0.75;
+                lamr = (3.0 + mu_r + 0.672) / c->mvd_r[k];
+                c->nr[k] = p->crg[1] * p->org3 * c->rr[k] * pow(lamr, bm_r) / p->am_r;
+            }
+            if (hit_limit) c->nrten[k] = (c->nr[k] / c->rho[k] - nr1d[k]) * c->odt;
+            nr1d[k] = p->crg[1] * p->org3 * qr1d[k] * pow(lamr, bm_r) / p->am_r;
+            c->ilamr[k] = 1.0 / lamr;
+        } else {
+            c->l_qr[k] = 0;
+            c->rr[k] = r1;
+            c->nr[k] = r2;
+            c->mvd_r[k] = d0r;
+            c->ilamr[k] = 0.0;
+            c->qrten[k] = -qr1d[k] * c->odt;
+            c->nrten[k] = -nr1d[k] * c->odt;
+            qr1d[k] = 0.0;
+            nr1d[k] = 0.0;
+        }
+    }
+}
+
+/* ice_check_and_update, :1045-1107 */
+static void ice_check_and_update(col_t *c, double *qi1d, double *ni1d)
+{
+    const params_t *p = &P;
+    for (int k = 0; k < c->nz; ++k) {
+        int hit_limit = 0;
+        double lami, xdi;
+        if (qi1d[k] + c->qiten[k] * c->dt > r1) {
+            c->l_qi[k] = 1;
+            c->ri[k] = (qi1d[k] + c->qiten[k] * c->dt) * c->rho[k];
+            qi1d[k] = qi1d[k] + c->qiten[k] * c->dt;
+            c->ni[k] = dmax(r2, (ni1d[k] + c->niten[k] * c->dt) * c->rho[k]);
+            if (c->ni[k] <= r2) {
+                hit_limit = 1;
+                lami = p->cie[1] / 5.e-6;
+                c->ni[k] = dmin(max_ni, p->cig[0] * p->oig2 * c->ri[k] / p->am_i * pow(lami, bm_i));
+            }
+            lami = pow(p->am_i * p->cig[1] * p->oig1 * c->ni[k] / c->ri[k], p->obmi);
+            xdi = (bm_i + mu_i + 1.0) / lami;
+            if (xdi < 5.e-6) {
+                hit_limit = 1;
+                lami = p->cie[1] / 5.e-6;
+                c->ni[k] = dmin(max_ni, p->cig[0] * p->oig2 * c->ri[k] / p->am_i * pow(lami, bm_i));
+            } else if (xdi > d0s) {
+                hit_limit = 1;
+                lami = p->cie[1] / d0s;
+                c->ni[k] = p->cig[0] * p->oig2 * c->ri[k] / p->am_i * pow(lami, bm_i);
+            }
+            if (hit_limit) c->niten[k] = (c->ni[k] / c->rho[k] - ni1d[k]) * c->odt;
+            ni1d[k] = dmax(r2 / c->rho[k],
+                           dmin(p->cig[0] * p->oig2 * qi1d[k] / p->am_i * pow(lami, bm_i), max_ni / c->rho[k]));
+            c->ilami[k] = 1.0 / lami;
+        } else {
+            c->l_qi[k] = 0;
+            c->ri[k] = r1;
+            c->ni[k] = r2;
+            c->ilami[k] = 0.0;
+            c->qiten[k] = -qi1d[k] * c->odt;
+            c->niten[k] = -ni1d[k] * c->odt;
+            qi1d[k] = 0.0;
+            ni1d[k] = 0.0;
+        }
+    }
+}
+
+/* snow_check_and_update, :1110-1134 */
+static void snow_check_and_update(col_t *c, double *qs1d)
+{
+    for (int k = 0; k < c->nz; ++k) {
+        if (qs1d[k] + c->qsten[k] * c->dt > r1) {
+            c->l_qs[k] = 1;
+            c->rs[k] = (qs1d[k] + c->qsten[k] * c->dt) * c->rho[k];
+            qs1d[k] = qs1d[k] + c->qsten[k] * c->dt;
+        } else {
+            c->l_qs[k] = 0;
+            c->rs[k] = r1;
+            c->qsten[k] = -qs1d[k] * c->odt;
+            qs1d[k] = 0.0;
+        }
+    }
+}
+
+/* graupel_check_and_update, :1137-1231 (ng1d/qb1d NULL when absent) */
+static void graupel_check_and_update(col_t *c, double *qg1d, double *ng1d, double *qb1d)
+{
+    const params_t *p = &P;
+    for (int k = 0; k < c->nz; ++k) {
+        int hit_limit = 0;
+        double lamg = 0.0;
+        if (qg1d[k] + c->qgten[k] * c->dt > r1) {
+            c->l_qg[k] = 1;
+            c->rg[k] = (qg1d[k] + c->qgten[k] * c->dt) * c->rho[k];
+            qg1d[k] = qg1d[k] + c->qgten[k] * c->dt;
+            if (ng1d && qb1d) {
+                c->ng[k] = dmax(r2, (ng1d[k] + c->ngten[k] * c->dt) * c->rho[k]);
+                c->rb[k] = dmin(dmax(c->rg[k] * meters3_to_liters / rho_g[NRHG - 1],
+                                     (qb1d[k] + c->qbten[k] * c->dt) * c->rho[k]),
+                                c->rg[k] * meters3_to_liters / rho_g[0]);
+                c->idx_bg[k] = imax(1, imin(nint_(10.0 * c->rg[k] / c->rb[k]) + 1, NRHG));
+                if (c->ng[k] <= r2) {
+                    hit_limit = 1;
+                    c->mvd_g[k] = 1.5e-3;
+                    lamg = (3.0 + mu_g + 0.672) / c->mvd_g[k];
+                    c->ng[k] = CGG(2, 1) * p->ogg3 * c->rg[k] * pow(lamg, bm_g) / p->am_g[c->idx_bg[k] - 1];
+                }
+                lamg = pow(p->am_g[c->idx_bg[k] - 1] * CGG(3, 1) * p->ogg2 * c->ng[k] / c->rg[k], p->obmg);
+                c->mvd_g[k] = (3.0 + mu_g + 0.672) / lamg;
+                if (c->mvd_g[k] > 25.4e-3) {
+                    hit_limit = 1;
+                    c->mvd_g[k] = 25.4e-3;
+                    lamg = (3.0 + mu_g + 0.672) / c->mvd_g[k];
+                    c->ng[k] = CGG(2, 1) * p->ogg3 * c->rg[k] * pow(lamg, bm_g) / p->am_g[c->idx_bg[k] - 1];
+                } else if (c->mvd_g[k] < d0r) {
+                    hit_limit = 1;
+                    c->mvd_g[k] = d0r;
+                    lamg = (3.0 + mu_g + 0.672) / c->mvd_g[k];
+                    c->ng[k] = CGG(2, 1) * p->ogg3 * c->rg[k] * pow(lamg, bm_g) / p->am_g[c->idx_bg[k] - 1];
+                }
+                if (hit_limit) c->ngten[k] = (c->ng[k] / c->rho[k] - ng1d[k]) * c->odt;
+                ng1d[k] = CGG(2, 1) * p->ogg3 * qg1d[k] * pow(lamg, bm_g) / p->am_g[c->idx_bg[k] - 1];
+                qb1d[k] = dmin(dmax(qg1d[k] * meters3_to_liters / rho_g[NRHG - 1], qb1d[k] + c->qbten[k] * c->dt),
+                               meters3_to_liters * qg1d[k] / rho_g[0]);
+                c->idx_bg[k] = imax(1, imin(nint_(10.0 * qg1d[k] / qb1d[k]) + 1, NRHG));
+            } else {
+                c->idx_bg[k] = IDX_BG1;
+                double ygra1 = log10(dmax(1.e-9, c->rg[k]));
+                double zans1 = 3.4 + 2.0 / 7.0 * (ygra1 + 8.0);
+                double n0_exp = dmax(gonv_min, dmin(pow(10.0, zans1), gonv_max));
+                double lam_exp = pow(n0_exp * p->am_g[c->idx_bg[k] - 1] * CGG(1, 1) / c->rg[k], p->oge1);
+                lamg = lam_exp * pow(CGG(3, 1) * p->ogg2 * p->ogg1, p->obmg);
+                c->ng[k] = CGG(2, 1) * p->ogg3 * c->rg[k] * pow(lamg, bm_g) / p->am_g[c->idx_bg[k] - 1];
+                c->rb[k] = meters3_to_liters * c->rg[k] / rho_g[c->idx_bg[k] - 1];
+            }
+            c->ilamg[k] = 1.0 / lamg;
+            c->mvd_g[k] = (3.0 + mu_g + 0.672) * c->ilamg[k];
+        } else {
+            c->l_qg[k] = 0;
+            c->rg[k] = r1;
+            c->ng[k] = r2;
+            c->mvd_g[k] = d0g;
+            c->ilamg[k] = 0.0;
+            c->idx_bg[k] = IDX_BG1;
+            c->rb[k] = meters3_to_liters * r1 / rho_g[c->idx_bg[k] - 1];
+            c->qgten[k] = -qg1d[k] * c->odt;
+            qg1d[k] = 0.0;
+            if (ng1d && qb1d) {
+                c->ngten[k] = -ng1d[k] * c->odt;
+                c->qbten[k] = -qb1d[k] * c->odt;
+                ng1d[k] = 0.0;
+                qb1d[k] = 0.0;
+            }
+        }
+    }
+}
+
+/* graupel_init, :1234-1260 */
+static void graupel_init(col_t *c)
+{
+    const params_t *p = &P;
+    for (int k = 0; k < c->nz; ++k) {
+        if (c->qg1d[k] > r1) {
+            double rg = c->qg1d[k] * c->rho[k];
+            double rb = rg * meters3_to_liters / rho_g[IDX_BG1 - 1];
+            int idx = imax(1, imin(nint_(10.0 * rg / rb) + 1, NRHG));
+            double mvd_g = 5.e-3;
+            double lamg = (3.0 + mu_g + 0.672) / mvd_g;
+            double ng = CGG(2, 1) * p->ogg3 * rg * pow(lamg, bm_g) / p->am_g[idx - 1];
+            c->ng1d[k] = ng / c->rho[k];
+            c->qb1d[k] = rb / c->rho[k];
+        }
+    }
+}
+
+/* thermo_vars, :1263-1310 */
+static void thermo_vars(col_t *c)
+{
+    const params_t *p = &P;
+    c->supersaturated = 0;
+    for (int k = 0; k < c->nz; ++k) {
+        double otemp = 1.0 / c->temp[k], tempc = c->temp[k] - t0;
+        c->rhof[k] = sqrt(p->rho_not / c->rho[k]);
+        c->rhof2[k] = sqrt(c->rhof[k]);
+        c->qvs[k] = oracle_rslf(c->pres[k], c->temp[k]);
+        c->delqvs[k] = dmax(0.0, oracle_rslf(c->pres[k], t0) - c->qv[k]);
+        if (tempc <= 0.0) c->qvsi[k] = oracle_rsif(c->pres[k], c->temp[k]);
+        else c->qvsi[k] = c->qvs[k];
+        c->satw[k] = c->qv[k] / c->qvs[k];
+        c->sati[k] = c->qv[k] / c->qvsi[k];
+        c->ssatw[k] = c->satw[k] - 1.0;
+        c->ssati[k] = c->sati[k] - 1.0;
+        if (fabs(c->ssatw[k]) < eps) c->ssatw[k] = 0.0;
+        if (fabs(c->ssati[k]) < eps) c->ssati[k] = 0.0;
+        if (c->ssati[k] > 0.0) c->supersaturated = 1;
+        c->diffu[k] = 2.11e-5 * pow(c->temp[k] / t0, 1.94) * (101325.0 / c->pres[k]);
+        if (tempc >= 0.0) c->visco[k] = (1.718 + 0.0049 * tempc) * 1.0e-5;
+        else c->visco[k] = (1.718 + 0.0049 * tempc - 1.2e-5 * tempc * tempc) * 1.0e-5;
+        c->ocp[k] = 1.0 / (cp * (1.0 + 0.887 * c->qv[k]));
+        c->vsc2[k] = sqrt(c->rho[k] / c->visco[k]);
+        c->lvap[k] = lvap0 + (2106.0 - 4218.0) * tempc;
+        c->tcond[k] = (5.69 + 0.0168 * tempc) * 1.0e-5 * 418.936;
+        c->lvt2[k] = c->lvap[k] * c->lvap[k] * c->ocp[k] * p->orv * otemp * otemp;
+    }
+}
+
+/* check_over_depletion, :1313-1438 */
+static void check_over_depletion(col_t *c)
+{
+    for (int k = 0; k < c->nz; ++k) {
+        double sump, rate_max, ratio;
+        sump = TD(pri_inu, k) + TD(pri_ide, k) + TD(prs_ide, k) + TD(prs_sde, k) + TD(prg_gde, k) + TD(pri_iha, k);
+        rate_max = (c->qv[k] - c->qvsi[k]) * c->rho[k] * c->odt * 0.999;
+        if ((sump > eps && sump > rate_max) || (sump < -eps && sump < rate_max)) {
+            ratio = rate_max / sump;
+            TD(pri_inu, k) *= ratio;
+            TD(pri_ide, k) *= ratio;
+            TD(pni_ide, k) *= ratio;
+            TD(prs_ide, k) *= ratio;
+            TD(prs_sde, k) *= ratio;
+            TD(prg_gde, k) *= ratio;
+            TD(pri_iha, k) *= ratio;
+        }
+        sump = -TD(prr_wau, k) - TD(pri_wfz, k) - TD(prr_rcw, k) - TD(prs_scw, k) - TD(prg_scw, k) - TD(prg_gcw, k);
+        rate_max = -c->rc[k] * c->odt;
+        if (c->l_qc[k] && sump < rate_max) {
+            ratio = rate_max / sump;
+            TD(prr_wau, k) *= ratio;
+            TD(pri_wfz, k) *= ratio;
+            TD(prr_rcw, k) *= ratio;
+            TD(prs_scw, k) *= ratio;
+            TD(prg_scw, k) *= ratio;
+            TD(prg_gcw, k) *= ratio;
+        }
+        sump = TD(pri_ide, k) - TD(prs_iau, k) - TD(prs_sci, k) - TD(pri_rci, k);
+        rate_max = -c->ri[k] * c->odt;
+        if (c->l_qi[k] && sump < rate_max) {
+            ratio = rate_max / sump;
+            TD(pri_ide, k) *= ratio;
+            TD(prs_iau, k) *= ratio;
+            TD(prs_sci, k) *= ratio;
+            TD(pri_rci, k) *= ratio;
+        }
+        sump = -TD(prg_rfz, k) - TD(pri_rfz, k) - TD(prr_rci, k) + TD(prr_rcs, k) + TD(prr_rcg, k);
+        rate_max = -c->rr[k] * c->odt;
+        if (c->l_qr[k] && sump < rate_max) {
+            ratio = rate_max / sump;
+            TD(prg_rfz, k) *= ratio;
+            TD(pbg_rfz, k) *= ratio;
+            TD(pri_rfz, k) *= ratio;
+            TD(prr_rci, k) *= ratio;
+            TD(prr_rcs, k) *= ratio;
+            TD(prr_rcg, k) *= ratio;
+        }
+        sump = TD(prs_sde, k) - TD(prs_ihm, k) - TD(prr_sml, k) + TD(prs_rcs, k);
+        rate_max = -c->rs[k] * c->odt;
+        if (c->l_qs[k] && sump < rate_max) {
+            ratio = rate_max / sump;
+            TD(prs_sde, k) *= ratio;
+            TD(prs_ihm, k) *= ratio;
+            TD(prr_sml, k) *= ratio;
+            TD(prs_rcs, k) *= ratio;
+        }
+        sump = TD(prg_gde, k) - TD(prg_ihm, k) - TD(prr_gml, k) + TD(prg_rcg, k);
+        rate_max = -c->rg[k] * c->odt;
+        if (c->l_qg[k] && sump < rate_max) {
+            ratio = rate_max / sump;
+            TD(prg_gde, k) *= ratio;
+            TD(prg_ihm, k) *= ratio;
+            TD(prr_gml, k) *= ratio;
+            TD(prg_rcg, k) *= ratio;
+            TD(pbg_rcg, k) *= ratio;
+        }
+        TD(pri_ihm, k) = TD(prs_ihm, k) + TD(prg_ihm, k);
+        ratio = dmin(fabs(TD(prr_rcg, k)), fabs(TD(prg_rcg, k)));
+        TD(prr_rcg, k) = ratio * copysign(1.0, TD(prr_rcg, k));
+        TD(prg_rcg, k) = -TD(prr_rcg, k);
+        TD(pbg_rcg, k) = meters3_to_liters * TD(prg_rcg, k) / rho_i;
+        if (c->temp[k] > t0) {
+            ratio = dmin(fabs(TD(prr_rcs, k)), fabs(TD(prs_rcs, k)));
+            TD(prr_rcs, k) = ratio * copysign(1.0, TD(prr_rcs, k));
+            TD(prs_rcs, k) = -TD(prr_rcs, k);
+        }
+    }
+}
+
+/* sum_tendencies, :1441-1513 */
+static void sum_tendencies(col_t *c)
+{
+    const params_t *p = &P;
+    for (int k = 0; k < c->nz; ++k) {
+        double orho = 1. / c->rho[k], lfus2 = lsub - c->lvap[k];
+        c->qvten[k] = c->qvten[k] + (-TD(pri_inu, k) - TD(pri_iha, k) - TD(pri_ide, k) - TD(prs_ide, k) -
+                                     TD(prs_sde, k) - TD(prg_gde, k)) * orho;
+        c->qcten[k] = c->qcten[k] + (-TD(prr_wau, k) - TD(pri_wfz, k) - TD(prr_rcw, k) - TD(prs_scw, k) -
+                                     TD(prg_scw, k) - TD(prg_gcw, k)) * orho;
+        c->ncten[k] = c->ncten[k] + (-TD(pnc_wau, k) - TD(pnc_rcw, k) - TD(pni_wfz, k) - TD(pnc_scw, k) -
+                                     TD(pnc_gcw, k)) * orho;
+        c->qiten[k] = c->qiten[k] + (TD(pri_inu, k) + TD(pri_iha, k) + TD(pri_ihm, k) + TD(pri_wfz, k) +
+                                     TD(pri_rfz, k) + TD(pri_ide, k) - TD(prs_iau, k) - TD(prs_sci, k) -
+                                     TD(pri_rci, k)) * orho;
+        c->niten[k] = c->niten[k] + (TD(pni_inu, k) + TD(pni_iha, k) + TD(pni_ihm, k) + TD(pni_wfz, k) +
+                                     TD(pni_rfz, k) + TD(pni_ide, k) - TD(pni_iau, k) - TD(pni_sci, k) -
+                                     TD(pni_rci, k)) * orho;
+        c->qrten[k] = c->qrten[k] + (TD(prr_wau, k) + TD(prr_rcw, k) + TD(prr_sml, k) + TD(prr_gml, k) +
+                                     TD(prr_rcs, k) + TD(prr_rcg, k) - TD(prg_rfz, k) - TD(pri_rfz, k) -
+                                     TD(prr_rci, k)) * orho;
+        c->nrten[k] = c->nrten[k] + (TD(pnr_wau, k) + TD(pnr_sml, k) + TD(pnr_gml, k) -
+                                     (TD(pnr_rfz, k) + TD(pnr_rcr, k) + TD(pnr_rcg, k) + TD(pnr_rcs, k) +
+                                      TD(pnr_rci, k) + TD(pni_rfz, k))) * orho;
+        c->qsten[k] = c->qsten[k] + (TD(prs_iau, k) + TD(prs_sde, k) + TD(prs_sci, k) + TD(prs_scw, k) +
+                                     TD(prs_rcs, k) + TD(prs_ide, k) - TD(prs_ihm, k) - TD(prr_sml, k)) * orho;
+        c->qgten[k] = c->qgten[k] + (TD(prg_scw, k) + TD(prg_rfz, k) + TD(prg_gde, k) + TD(prg_rcg, k) +
+                                     TD(prg_gcw, k) + TD(prg_rci, k) + TD(prg_rcs, k) - TD(prg_ihm, k) -
+                                     TD(prr_gml, k)) * orho;
+        c->ngten[k] = c->ngten[k] + (TD(png_scw, k) + TD(png_rfz, k) - TD(png_rcg, k) + TD(png_rci, k) +
+                                     TD(png_rcs, k) + TD(png_gde, k) - TD(pnr_gml, k)) * orho;
+        c->qbten[k] = c->qbten[k] + (TD(pbg_scw, k) + TD(pbg_rfz, k) + TD(pbg_gcw, k) + TD(pbg_rci, k) +
+                                     TD(pbg_rcs, k) + TD(pbg_rcg, k) + TD(pbg_sml, k) - TD(pbg_gml, k) +
+                                     meters3_to_liters * (TD(prg_gde, k) - TD(prg_ihm, k)) /
+                                         rho_g[c->idx_bg[k] - 1]) * orho;
+        if (c->temp[k] < t0) {
+            c->tten[k] = c->tten[k] +
+                         (lsub * c->ocp[k] * (TD(pri_inu, k) + TD(pri_ide, k) + TD(prs_ide, k) + TD(prs_sde, k) +
+                                              TD(prg_gde, k) + TD(pri_iha, k)) +
+                          lfus2 * c->ocp[k] * (TD(pri_wfz, k) + TD(pri_rfz, k) + TD(prg_rfz, k) + TD(prs_scw, k) +
+                                               TD(prg_scw, k) + TD(prg_gcw, k) + TD(prg_rcs, k) + TD(prs_rcs, k) +
+                                               TD(prr_rci, k) + TD(prg_rcg, k))) * orho;
+        } else {
+            c->tten[k] = c->tten[k] +
+                         (p->lfus * c->ocp[k] * (-TD(prr_sml, k) - TD(prr_gml, k) - TD(prr_rcg, k) - TD(prr_rcs, k)) +
+                          lsub * c->ocp[k] * (TD(prs_sde, k) + TD(prg_gde, k))) * orho;
+        }
+    }
+}
+
+/* sedimentation, :1516-1559 */
+static void sedimentation(col_t *c, double *xr, const double *vt, double *xten, double limit, int steps,
+                          int ktop_sedi, double *precip)
+{
+    int nz = c->nz, ktop = ktop_sedi; /* ktop_sedi always passed by tempo_main */
+    double sed_r[NZMAX + 1];
+    for (int k = nz - 1; k >= 0; --k) sed_r[k] = vt[k] * xr[k];
+    int k = nz - 1;
+    double odz = 1.0 / c->dz1d[k], orho = 1.0 / c->rho[k];
+    xten[k] = xten[k] - sed_r[k] * odz * (1.0 / (double)steps) * orho;
+    xr[k] = dmax(limit, xr[k] - sed_r[k] * odz * c->dt * (1.0 / (double)steps));
+    for (k = ktop - 1; k >= 0; --k) {
+        odz = 1.0 / c->dz1d[k];
+        orho = 1.0 / c->rho[k];
+        xten[k] = xten[k] + (sed_r[k + 1] - sed_r[k]) * odz * (1.0 / (double)steps) * orho;
+        xr[k] = dmax(limit, xr[k] + (sed_r[k + 1] - sed_r[k]) * odz * c->dt * (1.0 / (double)steps));
+    }
+    if (precip) {
+        if (xr[0] > low_limit_mass_for_precip) *precip = *precip + sed_r[0] * c->dt * (1.0 / (double)steps);
+    }
+}
+
+/* rain_fallspeed, :1752-1790 */
+static void rain_fallspeed(col_t *c, int *substeps, int *ktop)
+{
+    const params_t *p = &P;
+    int nz = c->nz;
+    *ktop = 1;
+    *substeps = 1;
+    for (int k = nz - 1; k >= 0; --k) {
+        if (c->rr[k] > r1) {
+            double lamr = 1.0 / c->ilamr[k];
+            c->vtrr[k] = c->rhof[k] * av_r * p->crg[5] * p->org3 * pow(lamr, p->cre[2]) * pow(lamr + fv_r, -p->cre[5]);
+            c->vtnr[k] = c->rhof[k] * av_r * p->crg[6] / p->crg[11] * pow(lamr, p->cre[11]) *
+                         pow(lamr + fv_r, -p->cre[6]);
+        } else {
+            c->vtrr[k] = c->vtrr[k + 1];
+            c->vtnr[k] = c->vtnr[k + 1];
+        }
+        if (dmax(c->vtrr[k], c->vtnr[k]) > 1.e-3) {
+            *ktop = imax(*ktop, k + 1);
+            double dz_by_vt = c->dz1d[k] / (dmax(c->vtrr[k], c->vtnr[k]));
+            *substeps = imax(*substeps, (int)(c->dt / dz_by_vt + 1.0));
+        }
+    }
+    if (*ktop == nz) *ktop = nz - 1;
+}
+
+/* graupel_fallspeed, :1793-1848 */
+static void graupel_fallspeed(col_t *c, int *substeps, int *ktop)
+{
+    const params_t *p = &P;
+    int nz = c->nz;
+    *ktop = 1;
+    *substeps = 1;
+    for (int k = nz - 1; k >= 0; --k) {
+        if (c->rg[k] > r1) {
+            double afall, bfall;
+            if (c->has_qb) {
+                double dens_g = dmax(rho_g[0], dmin(meters3_to_liters * c->rg[k] / c->rb[k], rho_g[NRHG - 1]));
+                afall = a_coeff * pow((4.0 * dens_g * earth_gravity) / (3.0 * c->rho[k]), b_coeff);
+                afall = afall * pow(c->visco[k], 1.0 - 2.0 * b_coeff);
+                bfall = 3.0 * b_coeff - 1.0;
+            } else {
+                afall = av_g_old;
+                bfall = bv_g_old;
+            }
+            int idx = c->idx_bg[k];
+            c->vtrg[k] = c->rhof[k] * afall * CGG(6, idx) * p->ogg3 * pow(c->ilamg[k], bfall);
+            c->vtng[k] = c->rhof[k] * afall * CGG(7, idx) / CGG(12, idx) * pow(c->ilamg[k], bfall);
+        } else {
+            c->vtrg[k] = c->vtrg[k + 1];
+            c->vtng[k] = c->vtng[k + 1];
+        }
+        if (c->vtrg[k] > 1.e-3) {
+            *ktop = imax(*ktop, k + 1);
+            double dz_by_vt = c->dz1d[k] / c->vtrg[k];
+            *substeps = imax(*substeps, (int)(c->dt / dz_by_vt + 1.0));
+        }
+    }
+    if (*ktop == nz) *ktop = nz - 1;
+}
+
+/* snow_fallspeed, :1851-1906 */
+static void snow_fallspeed(col_t *c, int *substeps, int *ktop)
+{
+    const params_t *p = &P;
+    int nz = c->nz;
+    *ktop = 1;
+    *substeps = 1;
+    for (int k = nz - 1; k >= 0; --k) {
+        if (c->rs[k] > r1) {
+            double xds = c->smoc[k] / c->smob[k];
+            double mrat = 1.0 / xds;
+            double ils1 = 1.0 / (mrat * lam0 + fv_s), ils2 = 1.0 / (mrat * lam1 + fv_s);
+            double t1_vts = kap0 * p->csg[3] * pow(ils1, p->cse[3]);
+            double t2_vts = kap1 * pow(mrat, mu_s) * p->csg[9] * pow(ils2, p->cse[9]);
+            ils1 = 1.0 / (mrat * lam0);
+            ils2 = 1.0 / (mrat * lam1);
+            double t3_vts = kap0 * p->csg[0] * pow(ils1, p->cse[0]);
+            double t4_vts = kap1 * pow(mrat, mu_s) * p->csg[6] * pow(ils2, p->cse[6]);
+            double vts = c->rhof[k] * av_s * (t1_vts + t2_vts) / (t3_vts + t4_vts);
+            if (TD(prr_sml, k) > 0.0) {
+                double sr = c->rs[k] / (c->rs[k] + c->rr[k]);
+                c->vtrs[k] = vts * sr + (1.0 - sr) * c->vtrr[k];
+            } else {
+                c->vtrs[k] = vts * c->vtboost[k];
+            }
+        } else {
+            c->vtrs[k] = c->vtrs[k + 1];
+        }
+        if (c->vtrs[k] > 1.e-3) {
+            *ktop = imax(*ktop, k + 1);
+            double dz_by_vt = c->dz1d[k] / c->vtrs[k];
+            *substeps = imax(*substeps, (int)(c->dt / dz_by_vt + 1.0));
+        }
+    }
+    if (*ktop == nz) *ktop = nz - 1;
+}
+
+/* ice_fallspeed, :1909-1942 */
+static void ice_fallspeed(col_t *c, int *substeps, int *ktop)
+{
+    const params_t *p = &P;
+    int nz = c->nz;
+    *ktop = 1;
+    *substeps = 1;
+    for (int k = nz - 1; k >= 0; --k) {
+        if (c->ri[k] > r1) {
+            c->vtri[k] = c->rhof[k] * av_i * p->cig[2] * p->oig2 * pow(c->ilami[k], bv_i);
+            c->vtni[k] = c->rhof[k] * av_i * p->cig[5] / p->cig[6] * pow(c->ilami[k], bv_i);
+        } else {
+            c->vtri[k] = c->vtri[k + 1];
+            c->vtni[k] = c->vtni[k + 1];
+        }
+        if (c->vtri[k] > 1.e-3) {
+            *ktop = imax(*ktop, k + 1);
+            double dz_by_vt = c->dz1d[k] / c->vtri[k];
+            *substeps = imax(*substeps, (int)(c->dt / dz_by_vt + 1.0));
+        }
+    }
+    if (*ktop == nz) *ktop = nz - 1;
+}
+
+/* cloud_fallspeed, :1945-1977 */
+static void cloud_fallspeed(col_t *c, int *ktop)
+{
+    const params_t *p = &P;
+    int nz = c->nz;
+    double hgt = 0.0;
+    *ktop = 1;
+    for (int k = 0; k < nz - 1; ++k) {
+        if (c->rc[k] > r2) *ktop = k + 1;
+        hgt = hgt + c->dz1d[k];
+        if (hgt > 500.0) break;
+    }
+    for (int k = *ktop - 1; k >= 0; --k) {
+        if (c->rc[k] > r1 && c->w1d[k] < 0.1) {
+            int nu_c = get_nuc(c->nc[k]);
+            c->vtrc[k] = c->rhof[k] * av_c * CCG(5, nu_c) * p->ocg2[nu_c - 1] * pow(c->ilamc[k], bv_c);
+            c->vtnc[k] = c->rhof[k] * av_c * CCG(4, nu_c) * p->ocg1[nu_c - 1] * pow(c->ilamc[k], bv_c);
+        }
+    }
+}
+
+/* activate_cloud_number, :2198-2276 (land never passed, Q5) */
+static double activate_cloud_number(double temp, double w1d, double nwfa)
+{
+    double n_local = nwfa * 1.e-6, w_local;
+    int n, i, j, k;
+    if (n_local >= ta_na[6]) n_local = ta_na[6] - 1.0;
+    else if (n_local <= ta_na[0]) n_local = ta_na[0] + 1.0;
+    for (n = 2; n <= 7; ++n)
+        if (n_local >= ta_na[n - 2] && n_local < ta_na[n - 1]) break;
+    i = n;
+    double x1 = log(ta_na[i - 2]), x2 = log(ta_na[i - 1]);
+    w_local = w1d;
+    if (w_local >= ta_ww[8]) w_local = ta_ww[8] - 1.0;
+    else if (w_local <= ta_ww[0]) w_local = ta_ww[0] + 0.001;
+    for (n = 2; n <= 9; ++n)
+        if (w_local >= ta_ww[n - 2] && w_local < ta_ww[n - 1]) break;
+    j = n;
+    double y1 = log(ta_ww[j - 2]), y2 = log(ta_ww[j - 1]);
+    k = imax(1, imin(nint_((temp - ta_tk[0]) * 0.1) + 1, 7));
+    const int l = 3, m = 2;
+    double lower_lim_nuc_frac = 0.;
+    double a = CCN(i - 1, j - 1, k, l, m), b = CCN(i, j - 1, k, l, m), cc = CCN(i, j, k, l, m),
+           d = CCN(i - 1, j, k, l, m);
+    double nx = log(n_local), wy = log(w_local);
+    double t = (nx - x1) / (x2 - x1), u = (wy - y1) / (y2 - y1);
+    double fraction = (1.0 - t) * (1.0 - u) * a + t * (1.0 - u) * b + t * u * cc + (1.0 - t) * u * d;
+    fraction = dmax(fraction, lower_lim_nuc_frac);
+    return nwfa * fraction;
+}
+
+/* cloud_condensation, :1980-2056 */
+static void cloud_condensation(col_t *c)
+{
+    const params_t *p = &P;
+    for (int k = 0; k < c->nz; ++k) {
+        if (fabs(c->ssatw[k]) < eps) continue;
+        double orho = 1.0 / c->rho[k];
+        double clap = (c->qv[k] - c->qvs[k]) / (1.0 + c->lvt2[k] * c->qvs[k]);
+        for (int n = 1; n <= 3; ++n) {
+            double fcd = c->qvs[k] * exp(c->lvt2[k] * clap) - c->qv[k] + clap;
+            double dfcd = c->qvs[k] * c->lvt2[k] * exp(c->lvt2[k] * clap) + 1.0;
+            clap = clap - fcd / dfcd;
+        }
+        double xrc = c->rc[k] + clap * c->rho[k], xnc = 0.0;
+        if (xrc > r1) {
+            TD(prw_vcd, k) = clap * c->odt;
+            if (clap > eps) {
+                if (c->has_nwfa) xnc = dmax(nt_c_min, activate_cloud_number(c->temp[k], c->w1d[k], c->nwfa[k]));
+                else if (!c->has_nc) xnc = c->ncsave[k];
+                else xnc = nt_c_l;
+                TD(pnc_wcd, k) = 0.5 * (xnc - c->nc[k] + fabs(xnc - c->nc[k])) * c->odt * orho;
+            } else if (c->l_qc[k] && c->ssatw[k] < -1.e-6 && clap < -eps) {
+                double otemp = 1.0 / c->temp[k];
+                double rvs = c->rho[k] * c->qvs[k];
+                double rvs_p = rvs * otemp * (c->lvap[k] * otemp * p->orv - 1.0);
+                double rvs_pp = rvs * (otemp * (c->lvap[k] * otemp * p->orv - 1.0) * otemp *
+                                           (c->lvap[k] * otemp * p->orv - 1.0) +
+                                       (-2.0 * c->lvap[k] * otemp * otemp * otemp * p->orv) + otemp * otemp);
+                double gamsc = c->lvap[k] * c->diffu[k] / c->tcond[k] * rvs_p;
+                double alphsc = 0.5 * (gamsc / (1.0 + gamsc)) * (gamsc / (1.0 + gamsc)) * rvs_pp / rvs_p * rvs / rvs_p;
+                alphsc = dmax(1.e-9, alphsc);
+                double xsat = c->ssatw[k];
+                if (fabs(xsat) < 1.e-9) xsat = 0.0;
+                double t1_evap = 2.0 * pi *
+                                 (1.0 - alphsc * xsat + 2.0 * alphsc * alphsc * xsat * xsat -
+                                  5.0 * alphsc * alphsc * alphsc * xsat * xsat * xsat) /
+                                 (1.0 + gamsc);
+                double dc_star = sqrt(-2.0 * c->dt * t1_evap / (2.0 * pi) * 4.0 * c->diffu[k] * c->ssatw[k] * rvs / rho_w);
+                int idx_d = imax(1, imin((int)(1.e6 * dc_star), 100));
+                int idx_c, idx_n;
+                cloud_index(p, c->rc[k], c->nc[k], &idx_c, &idx_n);
+                TD(prw_vcd, k) = dmax(-c->rc[k] * 0.99 * orho * c->odt, TD(prw_vcd, k));
+                TD(pnc_wcd, k) = dmax(-c->nc[k] * 0.99 * orho * c->odt,
+                                      -tnc_wev[idx_n - 1][idx_c - 1][idx_d - 1] * orho * c->odt);
+            }
+        } else {
+            TD(prw_vcd, k) = -c->rc[k] * orho * c->odt;
+            TD(pnc_wcd, k) = -c->nc[k] * orho * c->odt;
+        }
+    }
+}
+
+/* rain_evaporation, :2059-2121 */
+static void rain_evaporation(col_t *c)
+{
+    const params_t *p = &P;
+    for (int k = 0; k < c->nz; ++k) {
+        if (!c->l_qr[k]) continue;
+        if (!((c->ssatw[k] < -eps) && TD(prw_vcd, k) <= 0.0)) continue;
+        double orho = 1.0 / c->rho[k], tempc = c->temp[k] - t0, otemp = 1.0 / c->temp[k];
+        double rvs = c->rho[k] * c->qvs[k];
+        double rvs_p = rvs * otemp * (c->lvap[k] * otemp * p->orv - 1.0);
+        double rvs_pp = rvs * (otemp * (c->lvap[k] * otemp * p->orv - 1.0) * otemp * (c->lvap[k] * otemp * p->orv - 1.0) +
+                               (-2.0 * c->lvap[k] * otemp * otemp * otemp * p->orv) + otemp * otemp);
+        double gamsc = c->lvap[k] * c->diffu[k] / c->tcond[k] * rvs_p;
+        double alphsc = 0.5 * (gamsc / (1.0 + gamsc)) * (gamsc / (1.0 + gamsc)) * rvs_pp / rvs_p * rvs / rvs_p;
+        alphsc = dmax(1.e-9, alphsc);
+        double xsat = dmin(-1.e-9, c->ssatw[k]);
+        double t1_evap = 2.0 * pi *
+                         (1.0 - alphsc * xsat + 2.0 * alphsc * alphsc * xsat * xsat -
+                          5.0 * alphsc * alphsc * alphsc * xsat * xsat * xsat) /
+                         (1.0 + gamsc);
+        if (c->qv[k] / c->qvs[k] < 0.95 && c->rr[k] * orho <= 1.e-8) {
+            TD(prv_rev, k) = c->rr[k] * orho * c->odt;
+        } else {
+            double lamr = 1.0 / c->ilamr[k];
+            double n0_r = c->nr[k] * p->org2 * pow(lamr, p->cre[1]);
+            TD(prv_rev, k) = t1_evap * c->diffu[k] * (-c->ssatw[k]) * n0_r * rvs *
+                             (p->t1_qr_ev * pow(c->ilamr[k], p->cre[9]) +
+                              p->t2_qr_ev * c->vsc2[k] * c->rhof2[k] * (pow(lamr + 0.5 * fv_r, -p->cre[10])));
+            double rate_max = dmin((c->rr[k] * orho * c->odt), (c->qvs[k] - c->qv[k]) * c->odt);
+            TD(prv_rev, k) = dmin(rate_max, TD(prv_rev, k) * orho);
+            if (TD(prr_gml, k) > 0.0) {
+                double eva_factor = dmin(1.0, 0.01 + (0.99 - 0.01) * (tempc / 20.0));
+                TD(prv_rev, k) = TD(prv_rev, k) * eva_factor;
+            }
+        }
+        TD(pnr_rev, k) = dmin(c->nr[k] * 0.99 * orho * c->odt, TD(prv_rev, k) * c->nr[k] / c->rr[k]);
+    }
+}
+
+/* freeze_cloud_melt_ice, :2124-2165 */
+static void freeze_cloud_melt_ice(col_t *c)
+{
+    const params_t *p = &P;
+    for (int k = 0; k < c->nz; ++k) {
+        double xri = dmax(0.0, c->qi1d[k] + c->qiten[k] * c->dt);
+        if ((c->temp[k] > t0) && (xri > 0.0)) {
+            c->qcten[k] = c->qcten[k] + xri * c->odt;
+            c->ncten[k] = c->ncten[k] + c->ni1d[k] * c->odt;
+            c->qiten[k] = c->qiten[k] - xri * c->odt;
+            c->niten[k] = -c->ni1d[k] * c->odt;
+            c->tten[k] = c->tten[k] - p->lfus * c->ocp[k] * xri * c->odt;
+        }
+        double xrc = dmax(0.0, c->qc1d[k] + c->qcten[k] * c->dt);
+        if ((c->temp[k] < hgfrz) && (xrc > 0.0)) {
+            double lfus2 = lsub - c->lvap[k], xnc;
+            if (c->has_nc) xnc = c->nc1d[k] + c->ncten[k] * c->dt;
+            else xnc = c->ncsave[k] / c->rho[k] + c->ncten[k] * c->dt;
+            c->qiten[k] = c->qiten[k] + xrc * c->odt;
+            c->niten[k] = c->niten[k] + xnc * c->odt;
+            c->qcten[k] = c->qcten[k] - xrc * c->odt;
+            c->ncten[k] = c->ncten[k] - xnc * c->odt;
+            c->tten[k] = c->tten[k] + lfus2 * c->ocp[k] * xrc * c->odt;
+        }
+    }
+}
+
+/* koop_nucleation, :2168-2195 */
+static double koop_nucleation(double temp, double satw, double naero, double dt)
+{
+    const params_t *p = &P;
+    double xni = 0.0;
+    double mu_diff = 210368.0 + (131.438 * temp) - (3.32373e6 / temp) - (41729.1 * log(temp));
+    double a_w_i = exp(mu_diff / (r_uni * temp));
+    double delta_aw = satw - a_w_i;
+    double log_j_rate = -906.7 + (8502.0 * delta_aw) - (26924.0 * delta_aw * delta_aw) +
+                        (29180.0 * delta_aw * delta_aw * delta_aw);
+    log_j_rate = dmin(20.0, log_j_rate);
+    double j_rate = pow(10.0, log_j_rate);
+    double prob_h = dmin(1.0 - exp(-j_rate * p->ar_volume * dt), 1.0);
+    if (prob_h > 0.0) xni = dmin(prob_h * naero, 1000.e3);
+    return dmax(0.0, xni);
+}
+
+/* demott_nucleation, :2823-2836 */
+static double demott_nucleation(double tempc, double rho, double nifa)
+{
+    const params_t *p = &P;
+    double nifa_cc = dmax(0.5, nifa * p->rho_not0 * 1.e-6 / rho);
+    double xni = (5.94e-5 * pow(-tempc, 3.33)) * (pow(nifa_cc, (-0.0264 * (tempc)) + 0.0033));
+    xni = xni * rho / p->rho_not0 * 1000.0;
+    return dmax(0.0, xni);
+}
+
+/* warm_rain, :2279-2355 */
+static void warm_rain(col_t *c)
+{
+    const params_t *p = &P;
+    for (int k = 0; k < c->nz; ++k) {
+        if (c->l_qr[k] && c->mvd_r[k] > d0r) {
+            double ef_rr = dmax(-0.1, 1.0 - exp(2300.0 * (c->mvd_r[k] - 1950.0e-6)));
+            TD(pnr_rcr, k) = ef_rr * 2.0 * c->nr[k] * c->rr[k];
+        }
+        if (c->l_qc[k] && c->rc[k] > 0.01e-3) {
+            int nu_c = get_nuc(c->nc[k]);
+            double lamc = 1.0 / c->ilamc[k];
+            double xdc = dmax(d0c * 1.e6, (pow(c->rc[k] / (p->am_r * c->nc[k]), p->obmr)) * 1.e6);
+            double dc_g = (pow(CCG(3, nu_c) * p->ocg2[nu_c - 1], p->obmr) / lamc) * 1.e6;
+            double dc_b = pow(xdc * xdc * xdc * dc_g * dc_g * dc_g - xdc * xdc * xdc * xdc * xdc * xdc, 1.0 / 6.0);
+            double zeta1 = 0.5 * ((6.25e-6 * xdc * dc_b * dc_b * dc_b - 0.4) + fabs(6.25e-6 * xdc * dc_b * dc_b * dc_b - 0.4));
+            double zeta = 0.027 * c->rc[k] * zeta1;
+            double taud = 0.5 * ((0.5 * dc_b - 7.5) + fabs(0.5 * dc_b - 7.5)) + r1;
+            double tau = 3.72 / (c->rc[k] * taud);
+            TD(prr_wau, k) = zeta / tau;
+            TD(prr_wau, k) = dmin(c->rc[k] * c->odt, TD(prr_wau, k));
+            TD(pnr_wau, k) = TD(prr_wau, k) / (p->am_r * nu_c * autocon_nr_factor * d0r * d0r * d0r);
+            TD(pnc_wau, k) = dmin(c->nc[k] * c->odt, TD(prr_wau, k) / (p->am_r * c->mvd_c[k] * c->mvd_c[k] * c->mvd_c[k]));
+        }
+        if (c->l_qr[k] && c->l_qc[k] && c->mvd_r[k] > d0r && c->mvd_c[k] > d0c) {
+            double lamr = (3.0 + mu_r + 0.672) / c->mvd_r[k];
+            int idx = 1 + (int)(100 * log(c->mvd_r[k] / p->dr[0]) / log(p->dr[99] / p->dr[0]));
+            idx = imin(idx, 100);
+            double ef_rw = EFRW(idx, (int)(c->mvd_c[k] * 1.e6));
+            double n0_r = c->nr[k] * p->org2 * pow(lamr, p->cre[1]);
+            TD(prr_rcw, k) = c->rhof[k] * p->t1_qr_qc * ef_rw * c->rc[k] * n0_r * (pow(lamr + fv_r, -p->cre[8]));
+            TD(prr_rcw, k) = dmin(c->rc[k] * c->odt, TD(prr_rcw, k));
+            TD(pnc_rcw, k) = c->rhof[k] * p->t1_qr_qc * ef_rw * c->nc[k] * n0_r * (pow(lamr + fv_r, -p->cre[8]));
+            TD(pnc_rcw, k) = dmin(c->nc[k] * c->odt, TD(pnc_rcw, k));
+        }
+    }
+}
+
+/* riming, :2358-2490 */
+static void riming(col_t *c)
+{
+    const params_t *p = &P;
+    for (int k = 0; k < c->nz; ++k) {
+        double tempc = c->temp[k] - t0;
+        c->vtboost[k] = 1.0;
+        if (c->l_qc[k] && c->l_qs[k]) {
+            double xds = c->smoc[k] / c->smob[k];
+            if ((c->mvd_c[k] > d0c) && (xds > d0s)) {
+                int idxs = 1 + (int)(100 * log(xds / p->ds[0]) / log(p->ds[99] / p->ds[0]));
+                idxs = imin(idxs, 100);
+                double ef_sw = EFSW(idxs, (int)(c->mvd_c[k] * 1.e6));
+                TD(prs_scw, k) = c->rhof[k] * p->t1_qs_qc * ef_sw * c->rc[k] * c->smoe[k];
+                TD(prs_scw, k) = dmin(c->rc[k] * c->odt, TD(prs_scw, k));
+                TD(pnc_scw, k) = c->rhof[k] * p->t1_qs_qc * ef_sw * c->nc[k] * c->smoe[k];
+                TD(pnc_scw, k) = dmin(c->nc[k] * c->odt, TD(pnc_scw, k));
+                if (c->temp[k] < t0) {
+                    if (TD(prs_scw, k) > rime_threshold * TD(prs_sde, k) && TD(prs_sde, k) > eps) {
+                        double r_frac = dmin(30.0, TD(prs_scw, k) / TD(prs_sde, k));
+                        double g_frac = dmin(rime_conversion, 0.15 + (r_frac - 2.0) * .028);
+                        c->vtboost[k] = dmin(1.5, 1.1 + (r_frac - 2.) * .014);
+                        TD(prg_scw, k) = g_frac * TD(prs_scw, k);
+                        TD(png_scw, k) = TD(prg_scw, k) * c->smo0[k] / c->rs[k];
+                        double vts = av_s * pow(xds, bv_s) * exp(-fv_s * xds);
+                        double const_ri = -1.0 * (c->mvd_c[k] * 0.5e6) * vts / dmin(-0.1, tempc);
+                        const_ri = dmax(0.1, dmin(const_ri, 10.0));
+                        double rime_dens = (0.051 + 0.114 * const_ri - 0.0055 * const_ri * const_ri) * 1000.0;
+                        if (rime_dens < 150.0) {
+                            g_frac = 0.0;
+                            TD(prg_scw, k) = 0.0;
+                            TD(png_scw, k) = 0.0;
+                        }
+                        double snow_dens_frac =
+                            dmin(1.0, dmax(0.0, c->rs[k] * c->odt / (c->rs[k] * c->odt + TD(prg_scw, k))));
+                        TD(pbg_scw, k) = meters3_to_liters * TD(prg_scw, k) /
+                                         (rho_s * snow_dens_frac + rime_dens * (1.0 - snow_dens_frac));
+                        TD(prs_scw, k) = (1.0 - g_frac) * TD(prs_scw, k);
+                    }
+                }
+            }
+        }
+        if (c->l_qc[k] && c->l_qg[k]) {
+            if (c->rg[k] >= r_g[0] && c->mvd_c[k] > d0c) {
+                int idx = c->idx_bg[k];
+                double xdg = (bm_g + mu_g + 1.0) * c->ilamg[k];
+                double vtg = c->rhof[k] * p->av_g[idx - 1] * CGG(6, idx) * p->ogg3 * pow(c->ilamg[k], p->bv_g[idx - 1]);
+                double stoke_g = c->mvd_c[k] * c->mvd_c[k] * vtg * rho_w / (9.0 * c->visco[k] * xdg);
+                double const_ri = -1.0 * (c->mvd_c[k] * 0.5e6) * vtg / dmin(-0.1, tempc);
+                const_ri = dmax(0.1, dmin(const_ri, 10.0));
+                double rime_dens = (0.051 + 0.114 * const_ri - 0.0055 * const_ri * const_ri) * 1000.0;
+                if (xdg > d0g) {
+                    double ef_gw = 0.0;
+                    if (stoke_g >= 0.4 && stoke_g <= 10.0) ef_gw = 0.55 * log10(2.51 * stoke_g);
+                    else if (stoke_g < 0.4) ef_gw = 0.0;
+                    else if (stoke_g > 10.0) ef_gw = 0.77;
+                    if (c->temp[k] > t0) ef_gw = ef_gw * 0.1;
+                    double t1_qg_qc = pi * .25 * p->av_g[idx - 1] * CGG(9, idx);
+                    double n0_g = c->ng[k] * p->ogg2 * pow(1.0 / c->ilamg[k], CGE(2, 1));
+                    TD(prg_gcw, k) = c->rhof[k] * t1_qg_qc * ef_gw * c->rc[k] * n0_g * pow(c->ilamg[k], CGE(9, idx));
+                    TD(pnc_gcw, k) = c->rhof[k] * t1_qg_qc * ef_gw * c->nc[k] * n0_g * pow(c->ilamg[k], CGE(9, idx));
+                    TD(pnc_gcw, k) = dmin(c->nc[k] * c->odt, TD(pnc_gcw, k));
+                    if (c->temp[k] < t0) TD(pbg_gcw, k) = meters3_to_liters * TD(prg_gcw, k) / rime_dens;
+                    if (c->temp[k] < t0) {
+                        if (TD(prg_gcw, k) > eps && tempc > -8.0) {
+                            double tf = 0.0;
+                            if (tempc >= -5.0 && tempc < -3.0) tf = 0.5 * (-3.0 - tempc);
+                            else if (tempc > -8.0 && tempc < -5.0) tf = 0.33333333 * (8.0 + tempc);
+                            TD(pni_ihm, k) = 3.5e8 * tf * TD(prg_gcw, k);
+                            TD(pri_ihm, k) = xm0i * TD(pni_ihm, k);
+                            TD(prs_ihm, k) = TD(prs_scw, k) / (TD(prs_scw, k) + TD(prg_gcw, k)) * TD(pri_ihm, k);
+                            TD(prg_ihm, k) = TD(prg_gcw, k) / (TD(prs_scw, k) + TD(prg_gcw, k)) * TD(pri_ihm, k);
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* rain_snow_rain_graupel, :2638-2727 */
+static void rain_snow_rain_graupel(col_t *c)
+{
+    const params_t *p = &P;
+    for (int k = 0; k < c->nz; ++k) {
+        if (c->l_qr[k] && c->l_qs[k] && c->rr[k] >= r_r[0] && c->rs[k] >= r_s[0]) {
+            int it = temperature_index(c->temp[k]), ir, ir1, is;
+            rain_index(p, c->rr[k], c->ilamr[k], &ir, &ir1);
+            is = decade_index(c->rs[k], p->nis2, 37);
+#define Q(a) QS(a, is, it, ir1, ir)
+            if (c->temp[k] < t0) {
+                TD(prr_rcs, k) = -(Q(TMR_RACS2) + Q(TCR_SACR2) + Q(TMR_RACS1) + Q(TCR_SACR1));
+                TD(prs_rcs, k) = Q(TMR_RACS2) + Q(TCR_SACR2) - Q(TCS_RACS1) - Q(TMS_SACR1);
+                TD(prg_rcs, k) = Q(TMR_RACS1) + Q(TCR_SACR1) + Q(TCS_RACS1) + Q(TMS_SACR1);
+                TD(prr_rcs, k) = dmax(-c->rr[k] * c->odt, TD(prr_rcs, k));
+                TD(prs_rcs, k) = dmax(-c->rs[k] * c->odt, TD(prs_rcs, k));
+                TD(prg_rcs, k) = dmin((c->rr[k] + c->rs[k]) * c->odt, TD(prg_rcs, k));
+                TD(pnr_rcs, k) = Q(TNR_RACS1) + Q(TNR_RACS2) + Q(TNR_SACR1) + Q(TNR_SACR2);
+                TD(pnr_rcs, k) = dmin(c->nr[k] * c->odt, TD(pnr_rcs, k));
+                TD(png_rcs, k) = TD(pnr_rcs, k);
+                TD(pbg_rcs, k) = meters3_to_liters * TD(prg_rcs, k) / rho_i;
+            } else {
+                TD(prs_rcs, k) = -Q(TCS_RACS1) - Q(TMS_SACR1) + Q(TMR_RACS2) + Q(TCR_SACR2);
+                TD(prs_rcs, k) = dmax(-c->rs[k] * c->odt, TD(prs_rcs, k));
+                TD(prr_rcs, k) = -TD(prs_rcs, k);
+            }
+#undef Q
+        }
+        if (c->l_qr[k] && c->l_qg[k] && c->rr[k] >= r_r[0] && c->rg[k] >= r_g[0]) {
+            int ir, ir1, ig, ig1, idx = c->idx_bg[k];
+            (void)temperature_index(c->temp[k]);
+            rain_index(p, c->rr[k], c->ilamr[k], &ir, &ir1);
+            graupel_index(p, c->rg[k], c->ilamg[k], idx, &ig, &ig1);
+#define G(a) QG(a, ig1, ig, idx, ir1, ir)
+            if (c->temp[k] < t0) {
+                TD(prg_rcg, k) = G(TMR_RACG) + G(TCR_GACR);
+                TD(prg_rcg, k) = dmin(c->rr[k] * c->odt, TD(prg_rcg, k));
+                TD(prr_rcg, k) = -TD(prg_rcg, k);
+                TD(pnr_rcg, k) = G(TNR_RACG) + G(TNR_GACR);
+                TD(pnr_rcg, k) = dmin(c->nr[k] * c->odt, TD(pnr_rcg, k));
+                TD(pbg_rcg, k) = meters3_to_liters * TD(prg_rcg, k) / rho_i;
+            } else {
+                TD(prr_rcg, k) = G(TCG_RACG);
+                TD(prr_rcg, k) = dmin(c->rg[k] * c->odt, TD(prr_rcg, k));
+                TD(prg_rcg, k) = -TD(prr_rcg, k);
+                TD(png_rcg, k) = G(TNR_RACG);
+                TD(png_rcg, k) = dmin(c->ng[k] * c->odt, TD(png_rcg, k));
+                TD(pbg_rcg, k) = meters3_to_liters * TD(prg_rcg, k) / rho_g[idx - 1];
+                TD(pnr_rcg, k) = -1.5 * G(TNR_GACR);
+            }
+#undef G
+        }
+    }
+}
+
+/* ice_nucleation, :2730-2820 */
+static void ice_nucleation(col_t *c)
+{
+    const params_t *p = &P;
+    for (int k = 0; k < c->nz; ++k) {
+        if (!(c->temp[k] < t0)) continue;
+        double tempc = c->temp[k] - t0, xni, xnc;
+        int idx_tc = imax(1, imin(nint_(-tempc), 45));
+        double rate_max = (c->qv[k] - c->qvsi[k]) * c->rho[k] * c->odt * 0.999;
+        if (c->has_nifa) xni = demott_nucleation(tempc, c->rho[k], c->nifa[k]);
+        else xni = 1.0 * 1000.0;
+        int idx_in = in_index(p, xni);
+        if (c->rr[k] > r_r[0]) {
+            int ir, ir1;
+            rain_index(p, c->rr[k], c->ilamr[k], &ir, &ir1);
+            TD(prg_rfz, k) = FZR(TPG_QRFZ, ir, ir1, idx_tc, idx_in) * c->odt;
+            TD(pri_rfz, k) = FZR(TPI_QRFZ, ir, ir1, idx_tc, idx_in) * c->odt;
+            TD(pni_rfz, k) = FZR(TNI_QRFZ, ir, ir1, idx_tc, idx_in) * c->odt;
+            TD(pnr_rfz, k) = FZR(TNR_QRFZ, ir, ir1, idx_tc, idx_in) * c->odt;
+            TD(prg_rfz, k) = dmin(c->rr[k] * c->odt, TD(prg_rfz, k));
+            TD(pnr_rfz, k) = dmin(c->nr[k] * c->odt, TD(pnr_rfz, k));
+            TD(png_rfz, k) = TD(pnr_rfz, k) * dmax(dmin((pow(10.0, -0.1 * c->w1d[k]) + 0.1), 1.0), 0.1);
+        } else if (c->rr[k] > r1 && c->temp[k] < hgfrz) {
+            TD(pri_rfz, k) = c->rr[k] * c->odt;
+            TD(pni_rfz, k) = c->nr[k] * c->odt;
+        }
+        TD(pbg_rfz, k) = meters3_to_liters * TD(prg_rfz, k) / rho_i;
+        if (c->rc[k] > r_c[0]) {
+            int ic, in_;
+            cloud_index(p, c->rc[k], c->nc[k], &ic, &in_);
+            TD(pri_wfz, k) = FZC(0, ic, in_, idx_tc, idx_in) * c->odt;
+            TD(pri_wfz, k) = dmin(c->rc[k] * c->odt, TD(pri_wfz, k));
+            TD(pni_wfz, k) = FZC(1, ic, in_, idx_tc, idx_in) * c->odt;
+            TD(pni_wfz, k) = dmin(dmin(c->nc[k] * c->odt, TD(pri_wfz, k) / (2.0 * xm0i)), TD(pni_wfz, k));
+        } else if (c->rc[k] > r1 && c->temp[k] < hgfrz) {
+            TD(pri_wfz, k) = c->rc[k] * c->odt;
+            TD(pni_wfz, k) = c->nc[k] * c->odt;
+        }
+        if ((c->ssati[k] >= demott_nuc_ssati) || (c->ssatw[k] > eps && tempc < demott_nuc_tempc)) {
+            if (c->has_nifa) xnc = demott_nucleation(tempc, c->rho[k], c->nifa[k]);
+            else xnc = dmin(icenuc_max, tno * exp(ato * (t0 - c->temp[k])));
+            xni = c->ni[k] + (TD(pni_rfz, k) + TD(pni_wfz, k)) * c->dt;
+            TD(pni_inu, k) = 0.5 * (xnc - xni + fabs(xnc - xni)) * c->odt;
+            TD(pri_inu, k) = dmin(rate_max, xm0i * TD(pni_inu, k));
+            TD(pni_inu, k) = TD(pri_inu, k) / xm0i;
+        }
+        xni = c->smo0[k] + c->ni[k] + (TD(pni_rfz, k) + TD(pni_wfz, k) + TD(pni_inu, k)) * c->dt;
+        if (c->has_nwfa) {
+            if ((xni <= max_ni) && (c->temp[k] < 238.0) && (c->ssati[k] >= 0.4)) {
+                xnc = koop_nucleation(c->temp[k], c->ssatw[k], c->nwfa[k], c->dt);
+                TD(pni_iha, k) = xnc * c->odt;
+                TD(pri_iha, k) = dmin(rate_max, xm0i * 0.1 * TD(pni_iha, k));
+                TD(pni_iha, k) = TD(pri_iha, k) / (xm0i * 0.1);
+            }
+        }
+    }
+}
+
+/* get_t1_subl, :2861-2888 */
+static void get_t1_subl(const col_t *c, double *t1_subl)
+{
+    const params_t *p = &P;
+    for (int k = 0; k < c->nz; ++k) {
+        double otemp = 1.0 / c->temp[k], rvs = c->rho[k] * c->qvsi[k];
+        double rvs_p = rvs * otemp * (lsub * otemp * p->orv - 1.0);
+        double rvs_pp = rvs * (otemp * (lsub * otemp * p->orv - 1.0) * otemp * (lsub * otemp * p->orv - 1.0) +
+                               (-2. * lsub * otemp * otemp * otemp * p->orv) + otemp * otemp);
+        double gamsc = lsub * c->diffu[k] / c->tcond[k] * rvs_p;
+        double alphsc = 0.5 * (gamsc / (1.0 + gamsc)) * (gamsc / (1.0 + gamsc)) * rvs_pp / rvs_p * rvs / rvs_p;
+        alphsc = dmax(1.e-9, alphsc);
+        double xsat = c->ssati[k];
+        if (fabs(xsat) < 1.e-9) xsat = 0.0;
+        t1_subl[k] = 4.0 * pi *
+                     (1.0 - alphsc * xsat + 2.0 * alphsc * alphsc * xsat * xsat -
+                      5.0 * alphsc * alphsc * alphsc * xsat * xsat * xsat) /
+                     (1.0 + gamsc);
+    }
+}
+
+/* ice_processes, :2891-3018 */
+static void ice_processes(col_t *c)
+{
+    const params_t *p = &P;
+    double t1_subl[NZMAX];
+    get_t1_subl(c, t1_subl);
+    for (int k = 0; k < c->nz; ++k) {
+        double rvs = c->rho[k] * c->qvsi[k];
+        double rate_max = (c->qv[k] - c->qvsi[k]) * c->rho[k] * c->odt * 0.999;
+        if (!(c->temp[k] < t0)) continue;
+        if (c->l_qi[k]) {
+            int ii, ii1;
+            ice_index(p, c->ri[k], c->ni[k], &ii, &ii1);
+            double xdi = dmax(p->d0i, (bm_i + mu_i + 1.) * c->ilami[k]);
+            double xmi = p->am_i * pow(xdi, bm_i), oxmi = 1.0 / xmi;
+            TD(pri_ide, k) = c_cube * t1_subl[k] * c->diffu[k] * c->ssati[k] * rvs * p->oig1 * p->cig[4] * c->ni[k] *
+                             c->ilami[k];
+            if (TD(pri_ide, k) < 0.0) {
+                TD(pri_ide, k) = dmax(dmax(-c->ri[k] * c->odt, TD(pri_ide, k)), rate_max);
+                TD(pni_ide, k) = TD(pri_ide, k) * oxmi;
+                TD(pni_ide, k) = dmax(-c->ni[k] * c->odt, TD(pni_ide, k));
+            } else {
+                TD(pri_ide, k) = dmin(TD(pri_ide, k), rate_max);
+                TD(prs_ide, k) = (1.0 - tpi_ide[ii1 - 1][ii - 1]) * TD(pri_ide, k);
+                TD(pri_ide, k) = tpi_ide[ii1 - 1][ii - 1] * TD(pri_ide, k);
+            }
+            if ((ii == 64) || (xdi > 5.0 * d0s)) {
+                TD(prs_iau, k) = c->ri[k] * .99 * c->odt;
+                TD(pni_iau, k) = c->ni[k] * .95 * c->odt;
+            } else if (xdi < 0.1 * d0s) {
+                TD(prs_iau, k) = 0.0;
+                TD(pni_iau, k) = 0.0;
+            } else {
+                TD(prs_iau, k) = tps_iaus[ii1 - 1][ii - 1] * c->odt;
+                TD(prs_iau, k) = dmin(c->ri[k] * .99 * c->odt, TD(prs_iau, k));
+                TD(pni_iau, k) = tni_iaus[ii1 - 1][ii - 1] * c->odt;
+                TD(pni_iau, k) = dmin(c->ni[k] * .95 * c->odt, TD(pni_iau, k));
+            }
+            xdi = dmax(p->d0i, (bm_i + mu_i + 1.) * c->ilami[k]);
+            xmi = p->am_i * pow(xdi, bm_i);
+            oxmi = 1. / xmi;
+            if (c->rs[k] >= r_s[0]) {
+                TD(prs_sci, k) = p->t1_qs_qi * c->rhof[k] * ef_si * c->ri[k] * c->smoe[k];
+                TD(pni_sci, k) = TD(prs_sci, k) * oxmi;
+            }
+            if (c->rr[k] >= r_r[0] && c->mvd_r[k] > 4.0 * xdi) {
+                double lamr = 1.0 / c->ilamr[k];
+                double n0_r = c->nr[k] * p->org2 * pow(lamr, p->cre[1]);
+                TD(pri_rci, k) = c->rhof[k] * p->t1_qr_qi * ef_ri * c->ri[k] * n0_r * (pow(lamr + fv_r, -p->cre[8]));
+                TD(pnr_rci, k) = c->rhof[k] * p->t1_qr_qi * ef_ri * c->ni[k] * n0_r * (pow(lamr + fv_r, -p->cre[8]));
+                TD(pnr_rci, k) = dmin(c->nr[k] * c->odt, TD(pnr_rci, k));
+                TD(png_rci, k) = TD(pnr_rci, k) * dmax(dmin((pow(10.0, -0.1 * c->w1d[k]) + 0.1), 1.0), 0.1);
+                TD(pni_rci, k) = TD(pri_rci, k) * oxmi;
+                TD(prr_rci, k) = c->rhof[k] * p->t2_qr_qi * ef_ri * c->ni[k] * n0_r * (pow(lamr + fv_r, -p->cre[7]));
+                TD(prr_rci, k) = dmin(c->rr[k] * c->odt, TD(prr_rci, k));
+                TD(prg_rci, k) = TD(pri_rci, k) + TD(prr_rci, k);
+                TD(pbg_rci, k) = TD(prg_rci, k) / rho_i;
+            }
+        }
+        if (c->l_qs[k]) {
+            double c_snow = c_sqrd + (c->temp[k] - t0 + 1.5) * (c_cube - c_sqrd) / (-30.0 + 1.5);
+            c_snow = dmax(c_sqrd, dmin(c_snow, c_cube));
+            TD(prs_sde, k) = c_snow * t1_subl[k] * c->diffu[k] * c->ssati[k] * rvs *
+                             (p->t1_qs_sd * c->smo1[k] + p->t2_qs_sd * c->rhof2[k] * c->vsc2[k] * c->smof[k]);
+            if (TD(prs_sde, k) < 0.0) TD(prs_sde, k) = dmax(dmax(-c->rs[k] * c->odt, TD(prs_sde, k)), rate_max);
+            else TD(prs_sde, k) = dmin(TD(prs_sde, k), rate_max);
+        }
+        if (c->l_qg[k] && c->ssati[k] < -eps) {
+            int idx = c->idx_bg[k];
+            double n0_g = c->ng[k] * p->ogg2 * pow(1.0 / c->ilamg[k], CGE(2, 1));
+            double t2_qg_sd = 0.28 * p->sc3 * sqrt(p->av_g[idx - 1]) * CGG(11, idx);
+            TD(prg_gde, k) = c_cube * t1_subl[k] * c->diffu[k] * c->ssati[k] * rvs * n0_g *
+                             (p->t1_qg_sd * pow(c->ilamg[k], CGE(10, 1)) +
+                              t2_qg_sd * c->vsc2[k] * c->rhof2[k] * pow(c->ilamg[k], CGE(11, idx)));
+            if (TD(prg_gde, k) < 0.0) {
+                TD(prg_gde, k) = dmax(dmax(-c->rg[k] * c->odt, TD(prg_gde, k)), rate_max);
+                TD(png_gde, k) = TD(prg_gde, k) * c->ng[k] / c->rg[k];
+            } else {
+                TD(prg_gde, k) = dmin(TD(prg_gde, k), rate_max);
+            }
+        }
+    }
+}
+
+/* melting, :3021-3119 */
+static void melting(col_t *c)
+{
+    const params_t *p = &P;
+    double t1_subl[NZMAX];
+    get_t1_subl(c, t1_subl);
+    for (int k = 0; k < c->nz; ++k) {
+        double tempc = c->temp[k] - t0, rvs = c->rho[k] * c->qvsi[k];
+        if (!(c->temp[k] > t0)) continue;
+        if (c->l_qs[k]) {
+            TD(prr_sml, k) = (tempc * c->tcond[k] - lvap0 * c->diffu[k] * c->delqvs[k]) *
+                             (p->t1_qs_me * c->smo1[k] + p->t2_qs_me * c->rhof2[k] * c->vsc2[k] * c->smof[k]);
+            if (TD(prr_sml, k) > 0.0) {
+                TD(prr_sml, k) = TD(prr_sml, k) + 4218.0 * p->olfus * tempc * (TD(prr_rcs, k) + TD(prs_scw, k));
+                TD(prr_sml, k) = dmin(c->rs[k] * c->odt, dmax(0.0, TD(prr_sml, k)));
+                TD(pnr_sml, k) = c->smo0[k] / c->rs[k] * TD(prr_sml, k) * pow(10.0, -0.25 * tempc);
+                TD(pnr_sml, k) = dmin(c->smo0[k] * c->odt, TD(pnr_sml, k));
+            } else {
+                TD(prr_sml, k) = 0.0;
+                TD(pnr_sml, k) = 0.0;
+                if (c->ssati[k] < 0.0) {
+                    TD(prs_sde, k) = c_cube * t1_subl[k] * c->diffu[k] * c->ssati[k] * rvs *
+                                     (p->t1_qs_sd * c->smo1[k] + p->t2_qs_sd * c->rhof2[k] * c->vsc2[k] * c->smof[k]);
+                    TD(prs_sde, k) = dmax(-c->rs[k] * c->odt, TD(prs_sde, k));
+                }
+            }
+        }
+        if (c->l_qg[k]) {
+            int idx = c->idx_bg[k];
+            double n0_g = c->ng[k] * p->ogg2 * pow(1.0 / c->ilamg[k], CGE(2, 1));
+            double n0_melt = c->ng[k] * p->ogg2 * pow(1.0 / c->ilamg[k], CGE(2, 1));
+            if ((c->rg[k] * c->ng[k]) < 1.e-4) {
+                double lamg = 1. / c->ilamg[k];
+                n0_melt = (1.e-4 / c->rg[k]) * p->ogg2 * pow(lamg, CGE(2, 1));
+            }
+            double t2_qg_me = pi * 4.0 * c_cube * p->olfus * 0.28 * p->sc3 * sqrt(p->av_g[idx - 1]) * CGG(11, idx);
+            TD(prr_gml, k) = (tempc * c->tcond[k] - lvap0 * c->diffu[k] * c->delqvs[k]) * n0_melt *
+                             (p->t1_qg_me * pow(c->ilamg[k], CGE(10, 1)) +
+                              t2_qg_me * c->rhof2[k] * c->vsc2[k] * pow(c->ilamg[k], CGE(11, idx)));
+            TD(prr_gml, k) = dmin(c->rg[k] * c->odt, dmax(0.0, TD(prr_gml, k)));
+            if (TD(prr_gml, k) > 0.0) {
+                double melt_f = dmax(0.05, dmin(TD(prr_gml, k) * c->dt / c->rg[k], 1.0));
+                TD(pbg_gml, k) = meters3_to_liters * TD(prr_gml, k) / dmax(dmin(melt_f * rho_g[idx - 1], rho_w), 50.0);
+                TD(pnr_gml, k) = TD(prr_gml, k) * c->ng[k] / c->rg[k] * pow(10.0, -0.33 * (c->temp[k] - t0));
+            } else {
+                TD(prr_gml, k) = 0.0;
+                TD(pnr_gml, k) = 0.0;
+                TD(pbg_gml, k) = 0.0;
+                if (c->ssati[k] < 0.0) {
+                    double t2_qg_sd = 0.28 * p->sc3 * sqrt(p->av_g[idx - 1]) * CGG(11, idx);
+                    TD(prg_gde, k) = c_cube * t1_subl[k] * c->diffu[k] * c->ssati[k] * rvs * n0_g *
+                                     (p->t1_qg_sd * pow(c->ilamg[k], CGE(10, 1)) +
+                                      t2_qg_sd * c->vsc2[k] * c->rhof2[k] * pow(c->ilamg[k], CGE(11, idx)));
+                    TD(prg_gde, k) = dmax(-c->rg[k] * c->odt, TD(prg_gde, k));
+                    TD(png_gde, k) = TD(prg_gde, k) * c->ng[k] / c->rg[k];
+                }
+            }
+        }
+        if (c->dt > timestep_conversion_rime_to_rain) {
+            TD(prr_rcw, k) = TD(prr_rcw, k) + TD(prs_scw, k) + TD(prg_gcw, k);
+            TD(prs_scw, k) = 0.0;
+            TD(prg_gcw, k) = 0.0;
+        }
+    }
+}
+
+/* aerosol_collection_efficiency, src/module_mp_tempo_aerosols.F90:76-118 */
+static double aerosol_collection_efficiency(double d, double da, double visc, double rhoa, double temp, char species)
+{
+    const double boltzman = 1.3806503e-23, meanpath = 0.0256e-6;
+    double vt = 1.0, rho_p = rho_w;
+    if (species == 'r') {
+        vt = -0.1021 + 4.932e3 * d - 0.9551e6 * d * d + 0.07934e9 * d * d * d - 0.002362e12 * d * d * d * d;
+        rho_p = rho_w;
+    } else if (species == 's') {
+        vt = av_s * pow(d, bv_s);
+        rho_p = rho_s;
+    } else if (species == 'g') {
+        vt = P.av_g[IDX_BG1 - 1] * pow(d, P.bv_g[IDX_BG1 - 1]);
+        rho_p = rho_g[IDX_BG1 - 1];
+    }
+    double cc = 1.0 + 2.0 * meanpath / da * (1.257 + 0.4 * exp(-0.55 * da / meanpath));
+    double diff = boltzman * temp * cc / (3.0 * pi * visc * da);
+    double re = 0.5 * rhoa * d * vt / visc;
+    double sc_ = visc / (rhoa * diff);
+    double st = (rho_p - rhoa) * da * da * vt * cc / (9.0 * visc * d);
+    double aval = log(1.0 + re);
+    double st2 = (1.2 + 1.0 / 12.0 * aval) / (1.0 + aval);
+    double eff = 4.0 / (re * sc_) * (1.0 + 0.4 * sqrt(re) * pow(sc_, 0.3333) + 0.16 * sqrt(re) * sqrt(sc_)) +
+                 4.0 * da / d * (0.02 + da / d * (1.0 + 2.0 * sqrt(re)));
+    if (st > st2) eff = eff + pow((st - st2) / (st - st2 + 0.666667), 1.5);
+    return dmax(1.e-5, dmin(eff, 1.0));
+}
+
+/* aerosol_scavenging, src/module_mp_tempo_main.F90:3122-3193 */
+static void aerosol_scavenging(col_t *c)
+{
+    const params_t *p = &P;
+    const double wf = 0.04e-6, iff = 0.8e-6;
+    for (int k = 0; k < c->nz; ++k) {
+        if (c->l_qr[k] && c->mvd_r[k] > d0r) {
+            double ef_ra = aerosol_collection_efficiency(c->mvd_r[k], wf, c->visco[k], c->rho[k], c->temp[k], 'r');
+            double lamr = 1.0 / c->ilamr[k];
+            double n0_r = c->nr[k] * p->org2 * pow(lamr, p->cre[1]);
+            TD(pna_rca, k) = c->rhof[k] * p->t1_qr_qc * ef_ra * c->nwfa[k] * n0_r * (pow(lamr + fv_r, -p->cre[8]));
+            TD(pna_rca, k) = dmin(c->nwfa[k] * c->odt, TD(pna_rca, k));
+            ef_ra = aerosol_collection_efficiency(c->mvd_r[k], iff, c->visco[k], c->rho[k], c->temp[k], 'r');
+            TD(pnd_rcd, k) = c->rhof[k] * p->t1_qr_qc * ef_ra * c->nifa[k] * n0_r * (pow(lamr + fv_r, -p->cre[8]));
+            TD(pnd_rcd, k) = dmin(c->nifa[k] * c->odt, TD(pnd_rcd, k));
+        }
+        if (c->l_qs[k] && c->rs[k] > r_s[0]) {
+            double xds = c->smoc[k] / c->smob[k];
+            double ef_sa = aerosol_collection_efficiency(xds, wf, c->visco[k], c->rho[k], c->temp[k], 's');
+            TD(pna_sca, k) = c->rhof[k] * p->t1_qs_qc * ef_sa * c->nwfa[k] * c->smoe[k];
+            TD(pna_sca, k) = dmin(c->nwfa[k] * c->odt, TD(pna_sca, k));
+            ef_sa = aerosol_collection_efficiency(xds, iff, c->visco[k], c->rho[k], c->temp[k], 's');
+            TD(pnd_scd, k) = c->rhof[k] * p->t1_qs_qc * ef_sa * c->nifa[k] * c->smoe[k];
+            TD(pnd_scd, k) = dmin(c->nifa[k] * c->odt, TD(pnd_scd, k));
+        }
+        if (c->l_qg[k] && c->rg[k] > r_g[0]) {
+            int idx = c->idx_bg[k];
+            double xdg = (bm_g + mu_g + 1.0) * c->ilamg[k];
+            double ef_ga = aerosol_collection_efficiency(xdg, wf, c->visco[k], c->rho[k], c->temp[k], 'g');
+            double t1_qg_qc = pi * .25 * p->av_g[idx - 1] * CGG(9, idx);
+            double n0_g = c->ng[k] * p->ogg2 * pow(1.0 / c->ilamg[k], CGE(2, 1));
+            TD(pna_gca, k) = c->rhof[k] * t1_qg_qc * ef_ga * c->nwfa[k] * n0_g * pow(c->ilamg[k], CGE(9, idx));
+            TD(pna_gca, k) = dmin(c->nwfa[k] * c->odt, TD(pna_gca, k));
+            ef_ga = aerosol_collection_efficiency(xdg, iff, c->visco[k], c->rho[k], c->temp[k], 'g');
+            TD(pnd_gcd, k) = c->rhof[k] * t1_qg_qc * ef_ga * c->nifa[k] * n0_g * pow(c->ilamg[k], CGE(9, idx));
+            TD(pnd_gcd, k) = dmin(c->nifa[k] * c->odt, TD(pnd_gcd, k));
+        }
+    }
+}
+
+/* snow moments for every snowy level (:353-361, :495-511, :760-777) */
+static void all_snow_moments(col_t *c, int want)
+{
+    const params_t *p = &P;
+    for (int k = 0; k < c->nz; ++k) {
+        if (!c->l_qs[k]) continue;
+        double o[10] = {0};
+        double tc0 = dmin(-0.1, c->temp[k] - t0);
+        snow_moments(p, c->rs[k], tc0, want, o);
+        c->smob[k] = o[0];
+        c->smoc[k] = o[1];
+        if (want & 1) c->ns[k] = o[2];
+        if (want & 2) c->smo0[k] = o[3];
+        if (want & 4) c->smo1[k] = o[4];
+        if (want & 8) c->smo2[k] = o[5];
+        if (want & 16) c->smoe[k] = o[6];
+        if (want & 32) c->smof[k] = o[7];
+        if (want & 64) c->smog[k] = o[8];
+        if (want & 128) c->smoz[k] = o[9];
+    }
+}
+
+static void zero_snow_moments(col_t *c, int with_smoz)
+{
+    for (int k = 0; k < c->nz; ++k) {
+        c->smo0[k] = c->smo1[k] = c->smo2[k] = c->smob[k] = c->smoc[k] = 0.0;
+        c->smoe[k] = c->smof[k] = c->smog[k] = c->ns[k] = 0.0;
+        if (with_smoz) c->smoz[k] = 0.0;
+    }
+}
+
+static void update_rho(col_t *c)
+{
+    const params_t *p = &P;
+    for (int k = 0; k < c->nz; ++k) {
+        c->qv[k] = dmax(min_qv, c->qv1d[k] + c->qvten[k] * c->dt);
+        c->temp[k] = c->t1d[k] + c->tten[k] * c->dt;
+        c->rho[k] = p->roverrv * c->pres[k] / (rdry * c->temp[k] * (c->qv[k] + p->roverrv));
+    }
+}
+
+static int any(const int *l, int nz)
+{
+    for (int k = 0; k < nz; ++k)
+        if (l[k]) return 1;
+    return 0;
+}
+
+/* tempo_main, src/module_mp_tempo_main.F90:73-864 */
+static void tempo_main(col_t *c)
+{
+    const params_t *p = &P;
+    int nz = c->nz, substeps, ktop;
+    const int *F = g_flags;
+    memset(tendw, 0, sizeof(tendw));
+    for (int k = 0; k < nz; ++k) {
+        c->tten[k] = c->qvten[k] = c->qcten[k] = c->qiten[k] = c->qrten[k] = c->qsten[k] = 0.0;
+        c->qgten[k] = c->ngten[k] = c->qbten[k] = c->niten[k] = c->nrten[k] = c->ncten[k] = 0.0;
+        c->nwfaten[k] = c->nifaten[k] = 0.0;
+        c->smoz[k] = 0.0;
+        c->vtboost[k] = 1.0;
+    }
+    zero_snow_moments(c, 1);
+    for (int k = 0; k <= nz; ++k)
+        c->vtrr[k] = c->vtnr[k] = c->vtrs[k] = c->vtri[k] = c->vtni[k] = c->vtrc[k] = c->vtnc[k] = c->vtrg[k] =
+            c->vtng[k] = 0.0;
+    c->rain_precip = c->cloud_precip = c->ice_precip = c->snow_precip = c->graupel_precip = 0.0;
+    c->frz_rain = 0.0;
+    c->frozen_fraction = 0.0;
+    c->early_exit = 0;
+    for (int k = 0; k < nz; ++k) {
+        c->refl[k] = -35.0;
+        c->re_c[k] = c->re_i[k] = c->re_s[k] = c->max_hail[k] = 0.0;
+    }
+
+    for (int k = 0; k < nz; ++k) {
+        c->temp[k] = c->t1d[k];
+        c->qv[k] = dmax(min_qv, c->qv1d[k]);
+        c->pres[k] = c->p1d[k];
+        c->rho[k] = p->roverrv * c->pres[k] / (rdry * c->temp[k] * (c->qv[k] + p->roverrv));
+    }
+    /* aerosol profiles: when the field is present, aerosol_check_and_update
+     * overwrites the local profile with the field itself (:328-345) */
+    if (!c->has_nwfa) init_friendly_aerosols(c, c->nwfa, naccn0, naccn1);
+    if (!c->has_nifa) init_friendly_aerosols(c, c->nifa, nain0, nain1);
+    aerosol_check_and_update(c);
+
+    rain_check_and_update(c, c->qr1d, c->nr1d);
+    ice_check_and_update(c, c->qi1d, c->ni1d);
+    snow_check_and_update(c, c->qs1d);
+    all_snow_moments(c, 1 | 2 | 4 | 8 | 16 | 32 | 64);
+
+    if (!c->has_nc) {
+        double ncv = nt_c_l;
+        if (c->has_land) ncv = (c->land != 1) ? nt_c_o : nt_c_l;
+        else if (F[16] == 1) ncv = nt_c_o;
+        for (int k = 0; k < nz; ++k) c->nc[k] = ncv;
+        for (int k = 0; k < nz; ++k) c->ncsave[k] = c->nc[k];
+    }
+    cloud_check_and_update(c, c->qc1d, c->has_nc ? c->nc1d : NULL, !c->has_nc);
+
+    if (c->first_step && c->has_ng_qb) {
+        double sqg = 0.0, sng = 0.0, sqb = 0.0;
+        for (int k = 0; k < nz; ++k) { sqg += c->qg1d[k]; sng += c->ng1d[k]; sqb += c->qb1d[k]; }
+        if (sqg > r1 && sng < eps && sqb < eps) graupel_init(c);
+    }
+    graupel_check_and_update(c, c->qg1d, c->has_ng_qb ? c->ng1d : NULL, c->has_ng_qb ? c->qb1d : NULL);
+
+    for (int k = 0; k < nz; ++k) {
+        c->qcten[k] = c->qiten[k] = c->qrten[k] = c->qsten[k] = c->qgten[k] = 0.0;
+        c->ngten[k] = c->qbten[k] = c->niten[k] = c->nrten[k] = c->ncten[k] = 0.0;
+    }
+    thermo_vars(c);
+
+    int do_micro = any(c->l_qc, nz) || any(c->l_qr, nz) || any(c->l_qi, nz) || any(c->l_qs, nz) || any(c->l_qg, nz);
+    if (!do_micro && !c->supersaturated) {
+        c->early_exit = 1;
+        return;
+    }
+
+    int micro = !F[4];
+    if (micro) {
+        warm_rain(c);
+        rain_snow_rain_graupel(c);
+        ice_nucleation(c);
+        ice_processes(c);
+        riming(c);
+        melting(c);
+        if (c->has_nwfa || c->has_nifa) aerosol_scavenging(c);
+    }
+    check_over_depletion(c);
+    sum_tendencies(c);
+
+    for (int k = 0; k < nz; ++k) {
+        c->temp[k] = c->t1d[k] + c->tten[k] * c->dt;
+        c->qv[k] = dmax(min_qv, c->qv1d[k] + c->qvten[k] * c->dt);
+        c->rho[k] = p->roverrv * c->pres[k] / (rdry * c->temp[k] * (c->qv[k] + p->roverrv));
+        c->nwfaten[k] = c->nwfaten[k] - (TD(pna_rca, k) + TD(pna_sca, k) + TD(pna_gca, k) + TD(pni_iha, k)) / c->rho[k];
+        c->nifaten[k] = c->nifaten[k] - (TD(pnd_rcd, k) + TD(pnd_scd, k) + TD(pnd_gcd, k)) / c->rho[k];
+    }
+    aerosol_check_and_update(c);
+
+    memcpy(c->xrx, c->qc1d, nz * sizeof(double));
+    if (c->has_nc) memcpy(c->xncx, c->nc1d, nz * sizeof(double));
+    cloud_check_and_update(c, c->xrx, c->has_nc ? c->xncx : NULL, !c->has_nc);
+    memcpy(c->xrx, c->qr1d, nz * sizeof(double));
+    memcpy(c->xnx, c->nr1d, nz * sizeof(double));
+    rain_check_and_update(c, c->xrx, c->xnx);
+    memcpy(c->xrx, c->qi1d, nz * sizeof(double));
+    memcpy(c->xnx, c->ni1d, nz * sizeof(double));
+    ice_check_and_update(c, c->xrx, c->xnx);
+    memcpy(c->xrx, c->qs1d, nz * sizeof(double));
+    snow_check_and_update(c, c->xrx);
+    zero_snow_moments(c, 0);
+    all_snow_moments(c, 8);
+    memcpy(c->xrx, c->qg1d, nz * sizeof(double));
+    if (c->has_ng_qb) {
+        memcpy(c->xngx, c->ng1d, nz * sizeof(double));
+        memcpy(c->xqbx, c->qb1d, nz * sizeof(double));
+    }
+    graupel_check_and_update(c, c->xrx, c->has_ng_qb ? c->xngx : NULL, c->has_ng_qb ? c->xqbx : NULL);
+    thermo_vars(c);
+
+    if (micro && F[5]) {
+        cloud_condensation(c);
+        for (int k = 0; k < nz; ++k) {
+            c->qvten[k] = c->qvten[k] - TD(prw_vcd, k);
+            c->qcten[k] = c->qcten[k] + TD(prw_vcd, k);
+            c->ncten[k] = c->ncten[k] + TD(pnc_wcd, k);
+            c->nwfaten[k] = c->nwfaten[k] - TD(pnc_wcd, k);
+            c->tten[k] = c->tten[k] + c->lvap[k] * c->ocp[k] * TD(prw_vcd, k);
+        }
+        memcpy(c->xrx, c->qc1d, nz * sizeof(double));
+        if (c->has_nc) memcpy(c->xncx, c->nc1d, nz * sizeof(double));
+        cloud_check_and_update(c, c->xrx, c->has_nc ? c->xncx : NULL, !c->has_nc);
+        update_rho(c);
+        thermo_vars(c);
+    }
+    if (micro) {
+        rain_evaporation(c);
+        for (int k = 0; k < nz; ++k) {
+            c->qrten[k] = c->qrten[k] - TD(prv_rev, k);
+            c->qvten[k] = c->qvten[k] + TD(prv_rev, k);
+            c->nrten[k] = c->nrten[k] - TD(pnr_rev, k);
+            c->nwfaten[k] = c->nwfaten[k] + TD(pnr_rev, k);
+            c->tten[k] = c->tten[k] - c->lvap[k] * c->ocp[k] * TD(prv_rev, k);
+        }
+        memcpy(c->xrx, c->qr1d, nz * sizeof(double));
+        memcpy(c->xnx, c->nr1d, nz * sizeof(double));
+        rain_check_and_update(c, c->xrx, c->xnx);
+        update_rho(c);
+        thermo_vars(c);
+    }
+
+    /* sedimentation (flux form, :590-727) */
+    if (any(c->l_qr, nz)) {
+        rain_fallspeed(c, &substeps, &ktop);
+        for (int n = 1; n <= substeps; ++n) {
+            sedimentation(c, c->rr, c->vtrr, c->qrten, r1, substeps, ktop, &c->rain_precip);
+            sedimentation(c, c->nr, c->vtnr, c->nrten, r2, substeps, ktop, NULL);
+        }
+    }
+    if (any(c->l_qg, nz)) {
+        graupel_fallspeed(c, &substeps, &ktop);
+        for (int n = 1; n <= substeps; ++n) {
+            sedimentation(c, c->rg, c->vtrg, c->qgten, r1, substeps, ktop, &c->graupel_precip);
+            sedimentation(c, c->ng, c->vtng, c->ngten, r2, substeps, ktop, NULL);
+            sedimentation(c, c->rb, c->vtrg, c->qbten, meters3_to_liters * r1 / rho_g[NRHG - 1], substeps, ktop, NULL);
+        }
+    }
+    if (any(c->l_qs, nz)) {
+        snow_fallspeed(c, &substeps, &ktop);
+        for (int n = 1; n <= substeps; ++n)
+            sedimentation(c, c->rs, c->vtrs, c->qsten, r1, substeps, ktop, &c->snow_precip);
+    }
+    if (any(c->l_qi, nz)) {
+        ice_fallspeed(c, &substeps, &ktop);
+        sedimentation(c, c->ri, c->vtri, c->qiten, r1, substeps, ktop, &c->ice_precip);
+        sedimentation(c, c->ni, c->vtni, c->niten, r2, substeps, ktop, NULL);
+    }
+    if (any(c->l_qc, nz)) {
+        cloud_fallspeed(c, &ktop);
+        sedimentation(c, c->rc, c->vtrc, c->qcten, r1, 1, ktop, &c->cloud_precip);
+        sedimentation(c, c->nc, c->vtnc, c->ncten, r2, 1, ktop, NULL);
+    }
+
+    if (micro) freeze_cloud_melt_ice(c);
+
+    /* final update (:738-781) */
+    for (int k = 0; k < nz; ++k) {
+        c->t1d[k] = c->t1d[k] + c->tten[k] * c->dt;
+        c->qv1d[k] = dmax(min_qv, (c->qv1d[k] + c->qvten[k] * c->dt));
+        c->rho[k] = p->roverrv * c->pres[k] / (rdry * c->temp[k] * (c->qv[k] + p->roverrv));
+        if (c->has_nwfa) c->nwfa1d[k] = dmax(nwfa_default, dmin(aero_max, (c->nwfa1d[k] + c->nwfaten[k] * c->dt)));
+        if (c->has_nifa) c->nifa1d[k] = dmax(nifa_default_, dmin(aero_max, (c->nifa1d[k] + c->nifaten[k] * c->dt)));
+    }
+    cloud_check_and_update(c, c->qc1d, c->has_nc ? c->nc1d : NULL, !c->has_nc);
+    rain_check_and_update(c, c->qr1d, c->nr1d);
+    ice_check_and_update(c, c->qi1d, c->ni1d);
+    snow_check_and_update(c, c->qs1d);
+    zero_snow_moments(c, 1);
+    all_snow_moments(c, 8 | 128);
+    graupel_check_and_update(c, c->qg1d, c->has_ng_qb ? c->ng1d : NULL, c->has_ng_qb ? c->qb1d : NULL);
+
+    /* diagnostics (:783-863) */
+    c->frozen_fraction = (c->ice_precip + c->snow_precip + c->graupel_precip) /
+                         (c->ice_precip + c->snow_precip + c->graupel_precip + c->rain_precip + r1);
+    c->frz_rain = 0.0;
+    if ((c->temp[0] - t0) < -0.5) c->frz_rain = c->rain_precip + c->cloud_precip;
+    if (F[10]) { /* max_hail_diam, src/module_mp_tempo_diags.F90:398-438 */
+        for (int k = 0; k < nz; ++k) {
+            c->max_hail[k] = 0.0;
+            if (!(c->rg[k] / c->rho[k] >= 1.e-6)) continue;
+            if (rho_g[c->idx_bg[k] - 1] < 350.0) continue;
+            double lamg = 1.0 / c->ilamg[k];
+            double n0_g = c->ng[k] * p->ogg2 * pow(lamg, CGE(2, 1));
+            double sum_nh = 0.0, sum_t = 0.0, hail_max;
+            const double thr = 0.0005;
+            int n;
+            for (n = 50; n >= 1; --n) {
+                double f_d = n0_g * pow(p->hbins[n - 1], mu_g) * exp(-lamg * p->hbins[n - 1]) * p->dhbins[n - 1];
+                sum_nh = sum_nh + f_d;
+                if (sum_nh > thr) break;
+                sum_t = sum_nh;
+            }
+            if (n >= 50) hail_max = p->hbins[49];
+            else if (p->hbins[n] > 1.e-3)
+                hail_max = p->hbins[n - 1] - (sum_nh - thr) / (sum_nh - sum_t) * (p->hbins[n - 1] - p->hbins[n]);
+            else hail_max = 1.e-4;
+            c->max_hail[k] = 1000.0 * hail_max;
+        }
+    }
+    if (F[6]) { /* reflectivity_10cm, src/module_mp_tempo_diags.F90:59-121 */
+        for (int k = nz - 1; k >= 0; --k) {
+            double ze_rain = 1.e-22, ze_snow = 1.e-22, ze_graupel = 1.e-22;
+            if (c->l_qr[k]) {
+                double lamr = 1.0 / c->ilamr[k];
+                double n0_r = c->nr[k] * p->org2 * pow(lamr, p->cre[1]);
+                ze_rain = n0_r * p->crg[3] * pow(c->ilamr[k], p->cre[3]);
+            }
+            if (c->l_qs[k])
+                ze_snow = (0.176 / 0.93) * (6.0 / pi) * (6.0 / pi) * (am_s / 900.0) * (am_s / 900.0) * c->smoz[k];
+            if (c->l_qg[k]) {
+                int idx = c->idx_bg[k];
+                double n0_g = c->ng[k] * p->ogg2 * pow(1.0 / c->ilamg[k], CGE(2, 1));
+                ze_graupel = (0.176 / 0.93) * (6.0 / pi) * (6.0 / pi) * (p->am_g[idx - 1] / 900.0) *
+                             (p->am_g[idx - 1] / 900.0) * n0_g * CGG(4, 1) * pow(c->ilamg[k], CGE(4, 1));
+            }
+            c->refl[k] = dmax(-35.0, 10.0 * log10((ze_rain + ze_snow + ze_graupel) * 1.e18));
+        }
+    }
+    if (F[7] && F[8] && F[9]) { /* effective_radius, src/module_mp_tempo_diags.F90:13-56 */
+        for (int k = 0; k < nz; ++k) {
+            c->re_c[k] = c->re_i[k] = c->re_s[k] = 0.0;
+            if (c->l_qc[k]) {
+                int nu_c = get_nuc(c->nc[k]);
+                c->re_c[k] = dmax(2.51e-6, dmin(0.5 * (3.0 + (double)nu_c) * c->ilamc[k], 50.e-6));
+            }
+            if (c->l_qi[k]) c->re_i[k] = dmax(2.51e-6, dmin(0.5 * (3.0 + mu_i) * c->ilami[k], 125.e-6));
+            if (c->l_qs[k]) {
+                double o[10];
+                double tc0 = dmin(-0.1, c->temp[k] - t0);
+                snow_moments(p, c->rs[k], tc0, 0, o);
+                c->re_s[k] = dmax(5.01e-6, dmin(0.5 * (o[1] / o[0]), 999.e-6));
+            }
+        }
+    }
+}
+
+/* tempo_run, src/module_mp_tempo_driver.F90:187-513 (one column at a time) */
+int oracle_run(int ncol, int nz, double dt, int itimestep, double *const *state, const int *land,
+               const int *nlev, double *const *diag2, double *const *diag3)
+{
+    col_t *c = &C_;
+    if (nz > NZMAX - 1 || nz < 2) return -2;
+    if (!state[1] || !state[2] || !state[3] || !state[4] || !state[5] || !state[6] || !state[7] || !state[8] ||
+        !state[9] || !state[10] || !state[11])
+        return -3;
+    if (!state[0] && !(state[17] && state[18])) return -4;
+    double col[19][NZMAX];
+    double *ptr[19];
+    for (int f = 0; f < 19; ++f) ptr[f] = state[f] ? col[f] : NULL;
+    for (int i = 0; i < ncol; ++i) {
+        int nzc = nlev ? nlev[i] : nz;
+        for (int f = 0; f < 19; ++f)
+            if (state[f])
+                for (int k = 0; k < nzc; ++k) col[f][k] = state[f][(size_t)k * ncol + i];
+        memset(c, 0, sizeof(*c));
+        c->nz = nzc;
+        c->dt = dt;
+        c->odt = 1.0 / dt;
+        c->first_step = (itimestep == 1);
+        double t1d[NZMAX];
+        if (state[0]) for (int k = 0; k < nzc; ++k) t1d[k] = col[0][k];
+        else for (int k = 0; k < nzc; ++k) t1d[k] = col[17][k] * col[18][k];
+        c->t1d = t1d;
+        c->p1d = ptr[1]; c->w1d = ptr[2]; c->dz1d = ptr[3]; c->qv1d = ptr[4]; c->qc1d = ptr[5];
+        c->qr1d = ptr[6]; c->qi1d = ptr[7]; c->qs1d = ptr[8]; c->qg1d = ptr[9]; c->ni1d = ptr[10];
+        c->nr1d = ptr[11];
+        c->has_nc = state[12] != NULL; c->nc1d = ptr[12];
+        c->has_nwfa = state[13] != NULL; c->nwfa1d = ptr[13];
+        c->has_nifa = state[14] != NULL; c->nifa1d = ptr[14];
+        c->has_ng_qb = state[15] != NULL && state[16] != NULL;
+        c->has_qb = state[16] != NULL;
+        c->ng1d = ptr[15]; c->qb1d = ptr[16];
+        c->has_land = land != NULL;
+        c->land = land ? land[i] : 1;
+        tempo_main(c);
+        if (state[0]) for (int k = 0; k < nzc; ++k) col[0][k] = t1d[k];
+        else for (int k = 0; k < nzc; ++k) col[17][k] = t1d[k] / col[18][k];
+        static const int outf[] = {0, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17};
+        for (unsigned q = 0; q < sizeof(outf) / sizeof(outf[0]); ++q) {
+            int f = outf[q];
+            if (state[f])
+                for (int k = 0; k < nzc; ++k) state[f][(size_t)k * ncol + i] = col[f][k];
+        }
+        /* driver diagnostics: reset values unless tempo_main reached them */
+        double d2[8] = {c->rain_precip, c->ice_precip, c->snow_precip, c->graupel_precip,
+                        c->early_exit ? 0.0 : c->frozen_fraction, c->frz_rain, 0.0, 0.0};
+        if (!c->early_exit && g_flags[10]) {
+            double mx = c->max_hail[0];
+            for (int k = 1; k < nzc; ++k) mx = dmax(mx, c->max_hail[k]);
+            d2[6] = c->max_hail[0];
+            d2[7] = mx;
+        }
+        if (diag2)
+            for (int f = 0; f < 8; ++f)
+                if (diag2[f]) diag2[f][i] = d2[f];
+        if (diag3) {
+            for (int k = 0; k < nzc; ++k) {
+                size_t o = (size_t)k * ncol + i;
+                int live = !c->early_exit;
+                if (diag3[0]) diag3[0][o] = (live && g_flags[6]) ? c->refl[k] : -35.0;
+                if (diag3[1]) diag3[1][o] = (live && g_flags[7] && g_flags[8] && g_flags[9]) ? c->re_c[k] : 0.0;
+                if (diag3[2]) diag3[2][o] = (live && g_flags[7] && g_flags[8] && g_flags[9]) ? c->re_i[k] : 0.0;
+                if (diag3[3]) diag3[3][o] = (live && g_flags[7] && g_flags[8] && g_flags[9]) ? c->re_s[k] : 0.0;
+                if (diag3[4]) diag3[4][o] = (live && g_flags[11]) ? c->mvd_r[k] : 0.0;
+                if (diag3[5]) diag3[5][o] = (live && g_flags[12]) ? c->mvd_g[k] : 0.0;
+                if (diag3[6]) diag3[6][o] = (live && g_flags[13]) ? c->nc[k] * c->rho[k] : 0.0;
+            }
+        }
+    }
+    return 0;
+}
+
+/* ---- single table entries: src/module_mp_tempo_tables.F90:493-805 ---- */
+static void rain_dist(const params_t *p, int k, int m, double *n_r)
+{
+    double lam_exp = pow(n0r_exp[k - 1] * p->am_r * p->crg[0] / r_r[m - 1], p->ore1);
+    double lamr = lam_exp * pow(p->crg[2] * p->org2 * p->org1, p->obmr);
+    double n0_r = n0r_exp[k - 1] / (p->crg[1] * lam_exp) * pow(lamr, p->cre[1]);
+    for (int n2 = 0; n2 < 100; ++n2) n_r[n2] = n0_r * pow(p->dr[n2], mu_r) * exp(-lamr * p->dr[n2]) * p->dtr[n2];
+}
+
+static double rain_vt(double d)
+{
+    return -0.1021 + 4.932e3 * d - 0.9551e6 * d * d + 0.07934e9 * d * d * d - 0.002362e12 * d * d * d * d;
+}
+
+void oracle_qg_entry(int i, int j, int n3, int k, int m, double *out)
+{
+    const params_t *p = &PT;
+    double n_r[100], n_g[100], vr[100], vg[100];
+    for (int n2 = 0; n2 < 100; ++n2) vr[n2] = rain_vt(p->dr[n2]);
+    for (int n = 0; n < 100; ++n) vg[n] = p->av_g[n3 - 1] * pow(p->dg[n], p->bv_g[n3 - 1]);
+    rain_dist(p, k, m, n_r);
+    double lam_exp = pow(n0g_exp[i - 1] * p->am_g[n3 - 1] * CGG(1, 1) / r_g[j - 1], p->oge1);
+    double lamg = lam_exp * pow(CGG(3, 1) * p->ogg2 * p->ogg1, p->obmg);
+    double n0_g = n0g_exp[i - 1] / (CGG(2, 1) * lam_exp) * pow(lamg, CGE(2, 1));
+    for (int n = 0; n < 100; ++n) n_g[n] = n0_g * pow(p->dg[n], mu_g) * exp(-lamg * p->dg[n]) * p->dtg[n];
+    double t1 = 0, t2 = 0, z1 = 0, z2 = 0, y1 = 0, y2 = 0;
+    for (int n2 = 0; n2 < 100; ++n2) {
+        double massr = p->am_r * pow(p->dr[n2], bm_r);
+        for (int n = 0; n < 100; ++n) {
+            double massg = p->am_g[n3 - 1] * pow(p->dg[n], bm_g);
+            double dvg = 0.5 * ((vr[n2] - vg[n]) + fabs(vr[n2] - vg[n]));
+            double dvr = 0.5 * ((vg[n] - vr[n2]) + fabs(vg[n] - vr[n2]));
+            double s = p->dg[n] + p->dr[n2];
+            t1 = t1 + pi * .25 * ef_rg * s * s * dvg * massg * n_g[n] * n_r[n2];
+            z1 = z1 + pi * .25 * ef_rg * s * s * dvg * massr * n_g[n] * n_r[n2];
+            y1 = y1 + pi * .25 * ef_rg * s * s * dvg * n_g[n] * n_r[n2];
+            t2 = t2 + pi * .25 * ef_rg * s * s * dvr * massr * n_g[n] * n_r[n2];
+            y2 = y2 + pi * .25 * ef_rg * s * s * dvr * n_g[n] * n_r[n2];
+            z2 = z2 + pi * .25 * ef_rg * s * s * dvr * massg * n_g[n] * n_r[n2];
+        }
+    }
+    out[0] = t1;
+    out[1] = dmin(z1, r_r[m - 1]);
+    out[2] = t2;
+    out[3] = y1;
+    out[4] = y2;
+    (void)z2;
+}
+
+void oracle_qs_entry(int i, int j, int k, int m, double *out)
+{
+    const params_t *p = &PT;
+    double n_r[100], n_s[100], vr[100], vs[100];
+    for (int n2 = 0; n2 < 100; ++n2) vr[n2] = rain_vt(p->dr[n2]);
+    for (int n = 0; n < 100; ++n) vs[n] = 1.5 * av_s * pow(p->ds[n], bv_s) * exp(-fv_s * p->ds[n]);
+    rain_dist(p, k, m, n_r);
+    double o[10];
+    snow_moments(p, r_s[i - 1], tc_tab[j - 1], 0, o);
+    double m2 = o[0], m3 = o[1];
+    double om3 = 1.0 / m3;
+    double mrat = m2 * (m2 * om3) * (m2 * om3) * (m2 * om3);
+    double m0 = pow(m2 * om3, mu_s);
+    double slam1 = m2 * om3 * lam0, slam2 = m2 * om3 * lam1;
+    for (int n = 0; n < 100; ++n)
+        n_s[n] = mrat * (kap0 * exp(-slam1 * p->ds[n]) + kap1 * m0 * pow(p->ds[n], mu_s) * exp(-slam2 * p->ds[n])) *
+                 p->dts[n];
+    double t1 = 0, t2 = 0, t3 = 0, t4 = 0, z1 = 0, z2 = 0, z3 = 0, z4 = 0, y1 = 0, y2 = 0, y3 = 0, y4 = 0;
+    for (int n2 = 0; n2 < 100; ++n2) {
+        double massr = p->am_r * pow(p->dr[n2], bm_r);
+        for (int n = 0; n < 100; ++n) {
+            double masss = am_s * pow(p->ds[n], bm_s);
+            double dvs = 0.5 * ((vr[n2] - vs[n]) + fabs(vr[n2] - vs[n]));
+            double dvr = 0.5 * ((vs[n] - vr[n2]) + fabs(vs[n] - vr[n2]));
+            double s = p->ds[n] + p->dr[n2];
+            if (massr > 1.5 * masss) {
+                t1 = t1 + pi * .25 * ef_rs * s * s * dvs * masss * n_s[n] * n_r[n2];
+                z1 = z1 + pi * .25 * ef_rs * s * s * dvs * massr * n_s[n] * n_r[n2];
+                y1 = y1 + pi * .25 * ef_rs * s * s * dvs * n_s[n] * n_r[n2];
+            } else {
+                t3 = t3 + pi * .25 * ef_rs * s * s * dvs * masss * n_s[n] * n_r[n2];
+                z3 = z3 + pi * .25 * ef_rs * s * s * dvs * massr * n_s[n] * n_r[n2];
+                y3 = y3 + pi * .25 * ef_rs * s * s * dvs * n_s[n] * n_r[n2];
+            }
+            if (massr > 1.5 * masss) {
+                t2 = t2 + pi * .25 * ef_rs * s * s * dvr * massr * n_s[n] * n_r[n2];
+                y2 = y2 + pi * .25 * ef_rs * s * s * dvr * n_s[n] * n_r[n2];
+                z2 = z2 + pi * .25 * ef_rs * s * s * dvr * masss * n_s[n] * n_r[n2];
+            } else {
+                t4 = t4 + pi * .25 * ef_rs * s * s * dvr * massr * n_s[n] * n_r[n2];
+                y4 = y4 + pi * .25 * ef_rs * s * s * dvr * n_s[n] * n_r[n2];
+                z4 = z4 + pi * .25 * ef_rs * s * s * dvr * masss * n_s[n] * n_r[n2];
+            }
+        }
+    }
+    double v[12] = {t1, dmin(z1, r_r[m - 1]), t3, z3, t2, z2, t4, z4, y1, y3, y2, y4};
+    memcpy(out, v, sizeof(v));
+}
+
+static double fz_texp(int k, int m)
+{
+    double t_adjust = dmax(-3.0, dmin(3.0 - log10(nt_in[m - 1]), 3.0));
+    return exp((double)k - t_adjust) - 1.0;
+}
+
+void oracle_fz_rain_entry(int i, int j, int k, int m, double *out)
+{
+    const params_t *p = &PT;
+    double orho_w = 1.0 / rho_w, texp = fz_texp(k, m);
+    double lam_exp = pow(n0r_exp[j - 1] * p->am_r * p->crg[0] / r_r[i - 1], p->ore1);
+    double lamr = lam_exp * pow(p->crg[2] * p->org2 * p->org1, p->obmr);
+    double n0_r = n0r_exp[j - 1] / (p->crg[1] * lam_exp) * pow(lamr, p->cre[1]);
+    double sum1 = 0, sum2 = 0, sumn1 = 0, sumn2 = 0;
+    for (int n2 = 100; n2 >= 1; --n2) {
+        double massr = p->am_r * pow(p->dr[n2 - 1], bm_r);
+        double n_r = n0_r * pow(p->dr[n2 - 1], mu_r) * exp(-lamr * p->dr[n2 - 1]) * p->dtr[n2 - 1];
+        double vol = massr * orho_w;
+        double prob = dmax(0.0, 1.0 - exp(-120.0 * vol * 5.2e-4 * texp));
+        if (massr < p->xm0g) {
+            sumn1 = sumn1 + prob * n_r;
+            sum1 = sum1 + prob * n_r * massr;
+        } else {
+            sumn2 = sumn2 + prob * n_r;
+            sum2 = sum2 + prob * n_r * massr;
+        }
+        if ((sum1 + sum2) >= r_r[i - 1]) break;
+    }
+    out[0] = sum1;  /* tpi_qrfz */
+    out[1] = sumn1; /* tni_qrfz */
+    out[2] = sum2;  /* tpg_qrfz */
+    out[3] = sumn2; /* tnr_qrfz */
+}
+
+void oracle_fz_cloud_entry(int i, int j, int k, int m, double *out)
+{
+    const params_t *p = &PT;
+    double orho_w = 1.0 / rho_w, texp = fz_texp(k, m);
+    int nu_c = imin(15, nint_(nu_c_scale / p->t_nc[j - 1]) + 2);
+    double lamc = pow(p->t_nc[j - 1] * p->am_r * CCG(2, nu_c) * p->ocg1[nu_c - 1] / r_c[i - 1], p->obmr);
+    double n0_c = p->t_nc[j - 1] * p->ocg1[nu_c - 1] * pow(lamc, CCE(1, nu_c));
+    double sum1 = 0, sumn2 = 0;
+    for (int n = 100; n >= 1; --n) {
+        double massc = p->am_r * pow(p->dc[n - 1], bm_r);
+        double vol = massc * orho_w;
+        double prob = dmax(0.0, 1.0 - exp(-120.0 * vol * 5.2e-4 * texp));
+        double n_c = n0_c * powi(p->dc[n - 1], nu_c) * exp(-lamc * p->dc[n - 1]) * p->dtc[n - 1];
+        sumn2 = dmin(p->t_nc[j - 1], sumn2 + prob * n_c);
+        sum1 = sum1 + prob * n_c * massc;
+        if (sum1 >= r_c[i - 1]) break;
+    }
+    out[0] = sum1;
+    out[1] = sumn2;
+}
+
+int oracle_get_array(int which, double *out, int n)
+{
+    const params_t *p = &P;
+    switch (which) {
+    case 1: for (int j = 0; j < 100; ++j) for (int i = 0; i < 100; ++i) out[i + 100 * j] = t_efrw[j][i]; break;
+    case 2: for (int j = 0; j < 100; ++j) for (int i = 0; i < 100; ++i) out[i + 100 * j] = t_efsw[j][i]; break;
+    case 3:
+        for (int k = 0; k < 100; ++k) for (int j = 0; j < 37; ++j) for (int i = 0; i < 100; ++i)
+            out[i + 100 * (j + 37 * k)] = tnc_wev[k][j][i];
+        break;
+    case 4: for (int j = 0; j < 55; ++j) for (int i = 0; i < 64; ++i) out[i + 64 * j] = tps_iaus[j][i]; break;
+    case 5: for (int j = 0; j < 55; ++j) for (int i = 0; i < 64; ++i) out[i + 64 * j] = tni_iaus[j][i]; break;
+    case 6: for (int j = 0; j < 55; ++j) for (int i = 0; i < 64; ++i) out[i + 64 * j] = tpi_ide[j][i]; break;
+    case 7: {
+        const double *v[11] = {p->dc, p->dtc, p->di, p->dti, p->dr, p->dtr, p->ds, p->dts, p->dg, p->dtg, p->t_nc};
+        for (int a = 0; a < 11; ++a) memcpy(out + 100 * a, v[a], 100 * sizeof(double));
+        break;
+    }
+    case 8:
+        for (int nn = 0; nn < 15; ++nn) for (int a = 0; a < 5; ++a) out[a + 5 * nn] = p->ccg[a][nn];
+        memcpy(out + 75, p->cig, 7 * sizeof(double));
+        memcpy(out + 82, p->crg, 13 * sizeof(double));
+        memcpy(out + 95, p->csg, 17 * sizeof(double));
+        for (int m = 0; m < 9; ++m) for (int a = 0; a < 12; ++a) out[112 + a + 12 * m] = p->cgg[a][m];
+        break;
+    default: return 1;
+    }
+    (void)n;
+    return 0;
+}

@@ -1,0 +1,164 @@
+"""Seeded, counter-based synthetic column generator for the TEMPO column step.
+
+Every value is a pure function of (seed, column index, level, field id), so
+any subset of columns can be regenerated bit-for-bit by any consumer (tests,
+benchmark, golden-vector scripts) without generating the others.
+
+Arrays are float64, shape (nz, ncol), C order: element (k, i) at k*ncol + i,
+i.e. the Fortran (i, k) layout of the reference driver
+(src/module_mp_tempo_driver.F90:207-241) with k = 0 at the surface.
+
+Workloads (k = 1 is the surface, dz uniform):
+
+  plain      1,024 x 60,  dz 250 m, dt 30 s, aerosol- and hail-aware off
+  aerosol    100,000 x 90, dz 200 m, dt 20 s, aerosol-aware (nc, nwfa, nifa)
+  ragged     131,072 columns of U{30..128} active levels, dz 150 m, dt 20 s,
+             hail-aware (ng, qb), 25 % convective columns
+  adjoint    16,384 x 64, dz 250 m, dt 20 s, aerosol-aware
+  multistep  262,144 x 90, dz 200 m, dt 20 s, aerosol- and hail-aware, ragged
+             generator at 90 levels
+"""
+import numpy as np
+
+MASK64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+FIELD_IDS = {
+    "layer1_base": 1, "layer1_thick": 2, "layer2_on": 3, "layer2_base": 4, "layer2_thick": 5,
+    "rh": 6, "qc": 7, "qr": 8, "qi": 9, "qs": 10, "qg": 11, "w1": 12, "w2": 13,
+    "nwfa1": 14, "nwfa2": 15, "nifa1": 16, "nifa2": 17, "rhog": 18, "conv": 19, "wmax": 20,
+    "qr_sc": 21, "nlev": 22, "q1": 23, "q2": 24,
+}
+
+WORKLOADS = {
+    "plain": dict(ncol=1024, nz=60, dz=250.0, dt=30.0, aerosol=False, hail=False, seed=1001,
+                  ragged=False, itimestep=2),
+    "aerosol": dict(ncol=100_000, nz=90, dz=200.0, dt=20.0, aerosol=True, hail=False, seed=1002,
+                    ragged=False, itimestep=2),
+    "ragged": dict(ncol=131_072, nz=128, dz=150.0, dt=20.0, aerosol=False, hail=True, seed=1003,
+                   ragged=True, itimestep=2, convective=0.25),
+    "adjoint": dict(ncol=16_384, nz=64, dz=250.0, dt=20.0, aerosol=True, hail=False, seed=1004,
+                    ragged=False, itimestep=2),
+    "multistep": dict(ncol=262_144, nz=90, dz=200.0, dt=20.0, aerosol=True, hail=True, seed=1005,
+                      ragged=False, itimestep=2, convective=0.25),
+}
+
+
+def _splitmix(x):
+    x = (x + np.uint64(0x9E3779B97F4A7C15)) & MASK64
+    x = ((x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & MASK64
+    x = ((x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & MASK64
+    return x ^ (x >> np.uint64(31))
+
+
+def uniform(seed, col, level, field):
+    """U[0,1) from the counter (seed, column, level, field): 53-bit mantissa."""
+    col = np.asarray(col, dtype=np.uint64)
+    level = np.asarray(level, dtype=np.uint64)
+    key = (np.uint64(seed) << np.uint64(48)) ^ (np.uint64(FIELD_IDS[field]) << np.uint64(40)) \
+        ^ (level << np.uint64(24)) ^ col
+    with np.errstate(over="ignore"):
+        h = _splitmix(_splitmix(key) ^ np.uint64(seed))
+    return (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def normal(seed, col, level, field_a, field_b):
+    """Standard normal via Box-Muller over two counter streams."""
+    u1 = uniform(seed, col, level, field_a)
+    u2 = uniform(seed, col, level, field_b)
+    return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def rslf(p, t):
+    """Liquid saturation mixing ratio polynomial used to set relative humidity
+    (same polynomial as src/module_mp_tempo_utils.F90:285-309)."""
+    c = (.611583699e03, .444606896e02, .143177157e01, .264224321e-1, .299291081e-3,
+         .203154182e-5, .702620698e-8, .379534310e-11, -.321582393e-13)
+    x = np.maximum(-80.0, t - 273.16)
+    esl = c[8]
+    for ci in c[7::-1]:
+        esl = ci + x * esl
+    esl = np.minimum(esl, p * 0.15)
+    return 0.622 * esl / (p - esl)
+
+
+def generate(workload, ncol=None, col0=0, seed=None):
+    """Generate inputs for ``ncol`` columns starting at global column ``col0``.
+
+    Returns (state, meta): state maps field name -> (nz, ncol) float64 array
+    (absent optional fields are omitted), meta holds dt, itimestep and, for
+    ragged workloads, the per-column active level count 'nlev' (int32).
+    """
+    cfg = dict(WORKLOADS[workload])
+    seed = cfg["seed"] if seed is None else seed
+    n = cfg["ncol"] if ncol is None else ncol
+    nz, dz = cfg["nz"], cfg["dz"]
+    cols = np.arange(col0, col0 + n, dtype=np.uint64)[None, :]
+    lev = np.arange(nz, dtype=np.uint64)[:, None]
+    col1 = cols[0]
+    zero = np.zeros(1, dtype=np.uint64)
+
+    z = (np.arange(nz, dtype=np.float64) + 0.5) * dz                     # level mid-points [m]
+    zc = np.repeat(z[:, None], n, axis=1)
+    t = np.maximum(300.0 - 6.5 * zc * 1e-3, 210.0)
+    # hydrostatic pressure from 1000 hPa at the surface with layer-mean temperature
+    g, rd = 9.8, 287.04
+    zi = np.concatenate([[0.0], z])
+    ti = np.maximum(300.0 - 6.5 * zi * 1e-3, 210.0)
+    lnp = np.log(1.0e5) - np.cumsum(g * np.diff(zi) / (rd * 0.5 * (ti[1:] + ti[:-1])))
+    p = np.repeat(np.exp(lnp)[:, None], n, axis=1)
+
+    # one or two cloud layers per column: base U[0.5,8] km, thickness U[2,4] km
+    def layer(bf, tf):
+        base = 500.0 + 7500.0 * uniform(seed, col1, zero, bf)
+        thick = 2000.0 + 2000.0 * uniform(seed, col1, zero, tf)
+        return (zc >= base[None, :]) & (zc <= (base + thick)[None, :])
+    incloud = layer("layer1_base", "layer1_thick")
+    two = uniform(seed, col1, zero, "layer2_on") < 0.5
+    incloud |= layer("layer2_base", "layer2_thick") & two[None, :]
+
+    rh = np.where(incloud, 0.6 + 0.4 * uniform(seed, cols, lev, "rh"), 0.3)
+    qv = rh * rslf(p, t)
+
+    def lognormal(field, median, sigma=1.0, fb=None):
+        return median * np.exp(sigma * normal(seed, cols, lev, field, fb or "q2"))
+    hyd = {}
+    for name in ("qc", "qr", "qi", "qs", "qg"):
+        hyd[name] = np.where(incloud, lognormal(name, 1e-4, fb="q1"), 0.0)
+
+    def number(q, rho_x, d):
+        return q / (np.pi / 6.0 * rho_x * d ** 3)
+    w = 0.5 * normal(seed, cols, lev, "w1", "w2")
+    dzf = np.full((nz, n), dz)
+    state = dict(t=t, p=p, w=w, dz=dzf, qv=qv, qc=hyd["qc"], qr=hyd["qr"], qi=hyd["qi"],
+                 qs=hyd["qs"], qg=hyd["qg"],
+                 ni=number(hyd["qi"], 890.0, 50e-6), nr=number(hyd["qr"], 1000.0, 1e-3))
+
+    conv_frac = cfg.get("convective", 0.0)
+    if conv_frac > 0.0:
+        conv = uniform(seed, col1, zero, "conv") < conv_frac
+        wmax = 5.0 + 25.0 * uniform(seed, col1, zero, "wmax")
+        zconv = 12000.0
+        wconv = np.where(zc < zconv, wmax[None, :] * np.sin(np.pi * zc / zconv), 0.0)
+        state["w"] = np.where(conv[None, :], wconv, state["w"])
+        qr_sc = 1e-4 + (5e-3 - 1e-4) * uniform(seed, cols, lev, "qr_sc")
+        sc = conv[None, :] & (t < 273.15) & (zc < zconv)
+        state["qr"] = np.where(sc, qr_sc, state["qr"])
+        state["nr"] = number(state["qr"], 1000.0, 1e-3)
+
+    if cfg["aerosol"]:
+        decay = np.exp(-zc / 2000.0)
+        state["nc"] = number(hyd["qc"], 1000.0, 10e-6)
+        state["nwfa"] = 3e8 * np.exp(normal(seed, cols, lev, "nwfa1", "nwfa2")) * decay
+        state["nifa"] = 1e6 * np.exp(normal(seed, cols, lev, "nifa1", "nifa2")) * decay
+    if cfg["hail"]:
+        rhog = 300.0 + 600.0 * uniform(seed, cols, lev, "rhog")
+        state["qb"] = 1000.0 * state["qg"] / rhog                       # L/kg
+        state["ng"] = number(state["qg"], rhog, 2e-3)
+
+    meta = dict(dt=cfg["dt"], itimestep=cfg["itimestep"], nz=nz, ncol=n, workload=workload,
+                aerosol=cfg["aerosol"], hail=cfg["hail"])
+    if cfg["ragged"]:
+        nlev = 30 + np.floor(99.0 * uniform(seed, col1, zero, "nlev")).astype(np.int32)
+        meta["nlev"] = nlev.astype(np.int32)
+    state = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in state.items()}
+    return state, meta
