@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Throughput of the TEMPO column microphysics step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32|fp64]
+
+Workload: 100,000 columns x 90 levels, aerosol-aware, one step of dt = 20 s
+per timed step (the state is stepped in place, itimestep advancing), inputs
+synthesised in device memory by the seeded column generator
+(tempo_amd/generator.py).  Every step runs the whole tempo_run replacement
+(tempo_hip_run: pre-pass, level physics, sedimentation, final update and
+diagnostics refl10cm, re_cloud, re_ice, re_snow and the 2-D precipitation
+and hail fields).  Lookup tables are built on the GPU at init (reported,
+not timed).
+
+Prints one JSON line with the metric, the roofline of the dominant kernel
+(algorithmic HBM bytes over its event-timed average duration) and the
+reference CPU step timed on this host's cores over a bounded sample of the
+same workload (oracle/cpu_baseline.py, run in a child process).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+HBM_PEAK_GBS = 8000.0
+PHASES = ("k_prepass", "k_phase_a", "k_phase_b", "k_phase_c")
+# 3-D diagnostics the default configuration computes
+DIAG3 = ("refl10cm", "re_cloud", "re_ice", "re_snow")
+
+
+def algorithmic_bytes(nz, ncol, width, fields_in, fields_out):
+    """Compulsory HBM bytes of one step, per kernel, for the aerosol-aware
+    state: the pre-pass reads nothing but the per-column inputs it scans
+    (t, p, qv and the hydrometeors, all re-read later: counted once, in the
+    level physics); the level physics reads every input field once; the
+    sedimentation reads and writes only scratch; the final update writes every
+    output field, the 3-D diagnostics and the 2-D diagnostics once."""
+    lv = nz * ncol
+    return {
+        "k_prepass": 0,
+        "k_phase_a": fields_in * width * lv,
+        "k_phase_b": 0,
+        "k_phase_c": (fields_out + len(DIAG3)) * width * lv + 8 * width * ncol,
+        "step": (fields_in + fields_out + len(DIAG3)) * width * lv + 8 * width * ncol,
+    }
+
+
+def cpu_baseline(ctx, cols, workers):
+    """The reference -O2 build on the host cores, fed the GPU-built tables."""
+    with tempfile.TemporaryDirectory(prefix="tempo_tables_") as d:
+        ctx.write_tables(d)
+        cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--tables", d, "--workload", "aerosol",
+               "--cols", str(cols)]
+        if workers:
+            cmd += ["--workers", str(workers)]
+        r = subprocess.run(cmd, cwd=HERE, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline failed:\n{r.stdout}\n{r.stderr}")
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    return {k: out[k] for k in ("value", "unit", "cores", "kind", "sample")}, out
+
+
+def copy_probe(gib):
+    """A plain device copy of known size, for scaling rocprof's FETCH_SIZE and
+    WRITE_SIZE counters."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_each_way": n * 4, "launches": 3}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
+    ap.add_argument("--ncol", type=int, default=100_000)
+    ap.add_argument("--cpu-cols", type=int, default=65536, help="columns in the CPU baseline sample")
+    ap.add_argument("--cpu-workers", type=int, default=0, help="0: the host's cores (at most 16)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=float, default=0.0, help="run only a copy of this many GiB")
+    a = ap.parse_args()
+
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the column step shards by columns with no per-step exchange, "
+              "and this build runs on a single MI355X; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if a.copy_probe > 0:
+        copy_probe(a.copy_probe)
+        return
+
+    import torch
+    from tempo_amd import generator as gen
+    from tempo_amd import host
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    dtype = torch.float32 if a.precision == "fp32" else torch.float64
+    width = 4 if dtype == torch.float32 else 8
+
+    ctx = host.TempoHip(aerosolaware_flag=1, hailaware_flag=0)
+    state, meta = gen.generate_torch("aerosol", ncol=a.ncol, device=dev, dtype=dtype)
+    nz, ncol = state["p"].shape
+    diags = {n: torch.zeros(ncol, dtype=dtype, device=dev) for n in host.DIAG2_NAMES}
+    diags.update({n: torch.zeros((nz, ncol), dtype=dtype, device=dev) for n in DIAG3})
+    ctx.reserve(ncol, nz)
+    stream = torch.cuda.current_stream(dev)
+    it = meta["itimestep"]
+
+    def step():
+        nonlocal it
+        ctx.run(state, meta["dt"], it, diags=diags)
+        it += 1
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for _ in range(a.steps):
+        step()
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / a.steps
+
+    # per-kernel durations: HIP events recorded by the library around each
+    # launch, on the stream the kernels run on, over a second set of steps
+    ctx.profile(True)
+    for _ in range(a.steps):
+        step()
+    phase_ms, runs = ctx.profile_read()
+    ctx.profile(False)
+    phase_ms = phase_ms / max(runs, 1)
+    for v in state.values():
+        if not torch.isfinite(v).all():
+            raise RuntimeError("non-finite state after the timed steps")
+
+    levels = nz * ncol
+    value = levels / (ms * 1e-3) / 1e6
+    fields_in = sum(1 for k in state if k != "th" and k != "pii")
+    fields_out = sum(1 for k in state if k not in ("p", "w", "dz"))
+    ab = algorithmic_bytes(nz, ncol, width, fields_in, fields_out)
+    dom = int(max(range(4), key=lambda i: phase_ms[i]))
+    dom_name = PHASES[dom]
+    dom_bytes = ab[dom_name] if ab[dom_name] > 0 else ab["step"]
+    achieved = dom_bytes / (phase_ms[dom] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(HERE, "profiles", f"traffic_{dom_name}_aerosol_{a.precision}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    out = {
+        "metric": "Mcolumn-levels/s per step (forward)",
+        "value": round(value, 3),
+        "unit": "Mcolumn-levels/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if width == 4 else "f64",
+        "data": "synthetic (seeded counter-based column generator, device-side)",
+        "config": {"workload": f"aerosol-aware step, {ncol} columns x {nz} levels, dt {meta['dt']:g} s",
+                   "columns": ncol, "levels": nz, "dt": meta["dt"], "precision": a.precision,
+                   "table_build_ms": round(ctx.table_build_ms, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "kernel": dom_name, "kernel_ms": round(float(phase_ms[dom]), 4),
+                     "algorithmic_bytes": dom_bytes},
+        "phase_ms": {PHASES[i]: round(float(phase_ms[i]), 4) for i in range(4)},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu_baseline:
+        cb, raw = cpu_baseline(ctx, a.cpu_cols, a.cpu_workers)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = round(value / cb["value"], 2)
+        out["cpu_baseline_wall_s"] = round(raw["wall_s"], 3)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,164 @@
+/* tempo_hip.h -- C ABI of the MI355X-native TEMPO column microphysics step.
+ *
+ * Drop-in for the reference driver's public procedures
+ * (src/module_mp_tempo_driver.F90:14): tempo_init, tempo_run and the
+ * lookup-table files they read.  All entry points are plain C: pointers,
+ * sizes and status codes, no C++ or framework types.  A NULL pointer for an
+ * optional field reproduces an absent optional argument of the reference
+ * (present() tests, src/module_mp_tempo_main.F90:328,364,388,446,...).
+ *
+ * Thread safety: one context per device; calls on one context are
+ * stream-ordered and must not overlap from several host threads.
+ */
+#ifndef TEMPO_HIP_H
+#define TEMPO_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (the reference has none: it aborts with `error stop`) */
+#define TEMPO_HIP_OK 0
+#define TEMPO_HIP_E_ARG -1        /* invalid argument or missing required field */
+#define TEMPO_HIP_E_UNSUPPORTED -2 /* option not implemented by this library */
+#define TEMPO_HIP_E_TABLE -3      /* table file missing or wrong size */
+#define TEMPO_HIP_E_HIP -4        /* HIP runtime error */
+#define TEMPO_HIP_E_NOMEM -5      /* device allocation failed */
+
+/* working precision of the state arrays passed to tempo_hip_run */
+#define TEMPO_HIP_FP32 4
+#define TEMPO_HIP_FP64 8
+
+/* table identifiers for tempo_hip_table_export */
+#define TEMPO_HIP_TABLE_QR_ACR_QG 1   /* 5 x (37,37,9,37,37) real64, file order */
+#define TEMPO_HIP_TABLE_QR_ACR_QS 2   /* 12 x (37,9,37,37) real64 */
+#define TEMPO_HIP_TABLE_FREEZEWATER 3 /* 4 x (37,37,45,55) + 2 x (37,100,45,55) real64 */
+
+typedef struct tempo_hip_ctx tempo_hip_ctx;
+
+/* Run configuration: the fields of ty_tempo_cfgs
+ * (src/module_mp_tempo_cfgs.F90:14-38) plus the tempo_init flag overrides
+ * (src/module_mp_tempo_driver.F90:42-79).  Fill with tempo_hip_cfg_default. */
+typedef struct {
+  int aerosolaware_flag;          /* selects the aerosol-aware constants; physics follows field presence */
+  int hailaware_flag;             /* hail-aware graupel fall-speed constants */
+  int ml_for_bl_nc_flag;          /* not supported (must be 0) */
+  int ml_for_nc_flag;             /* not supported (must be 0) */
+  int semi_sedi_flag;             /* not supported (must be 0) */
+  int refl10cm_from_melting_flag; /* not supported (must be 0) */
+  int turn_off_micro_flag;
+  int cloud_condensation_flag;
+  int verbose;
+  int check_tables;               /* verify table file sizes before reading */
+  int nc_option;                  /* single_moment_nc_opt: 0 'land', 1 'ocean', 2 numeric */
+  double nc_value;                /* used when nc_option == 2 [m^-3] */
+  int refl10cm_flag;
+  int re_cloud_flag;
+  int re_ice_flag;
+  int re_snow_flag;
+  int max_hail_diameter_flag;
+  int rain_med_vol_diam_flag;
+  int graupel_med_vol_diam_flag;
+  int cloud_number_mixing_ratio_flag;
+} tempo_hip_cfg;
+
+/* State of tempo_run (src/module_mp_tempo_driver.F90:207-241), device
+ * pointers in the precision given to tempo_hip_run.  Either t, or th and pii,
+ * must be given.  p, w, dz, qv, qc, qr, qi, qs, qg, ni, nr are required;
+ * nc, nwfa, nifa, ng, qb and land (int per column) are optional (NULL). */
+typedef struct {
+  void *t, *th, *pii, *p, *w, *dz;
+  void *qv, *qc, *qr, *qi, *qs, *qg, *ni, *nr;
+  void *nc, *nwfa, *nifa, *ng, *qb;
+  const int *land;
+} tempo_hip_fields;
+
+/* ty_tempo_driver_diags (src/module_mp_tempo_driver.F90:18-34); any entry may
+ * be NULL.  2-D entries have one value per column, 3-D one per level. */
+typedef struct {
+  void *rain_precip, *ice_liquid_equiv_precip, *snow_liquid_equiv_precip, *graupel_liquid_equiv_precip;
+  void *frozen_fraction, *frz_rain_precip, *max_hail_diameter_sfc, *max_hail_diameter_column;
+  void *refl10cm, *re_cloud, *re_ice, *re_snow, *rain_med_vol_diam, *graupel_med_vol_diam;
+  void *cloud_number_mixing_ratio;
+} tempo_hip_diags;
+
+/* Array geometry.  Element (i, k, j) of a state array is at
+ * i + k*ldk + j*ldj (Fortran (ims:ime, kms:kme, jms:jme) with the base
+ * pointer at the tile origin); of a 3-D diagnostic at i + k*dldk + j*dldj;
+ * of a 2-D diagnostic at i + j*d2ldj.  Column index c = i + j*ni.
+ * nlev (device, ni*nj ints, may be NULL) gives each column's active level
+ * count (levels k >= nlev[c] are not touched). */
+typedef struct {
+  int ni, nj, nz;
+  long long ldk, ldj;
+  long long dldk, dldj;
+  long long d2ldj;
+  const int *nlev;
+} tempo_hip_dims;
+
+/* Fills the defaults of ty_tempo_cfgs (src/module_mp_tempo_cfgs.F90:15-34). */
+void tempo_hip_cfg_default(tempo_hip_cfg *cfg);
+
+/* Replaces tempo_init (src/module_mp_tempo_driver.F90:42-182): computes the
+ * constants and init-time tables, and provides the three collection/freezing
+ * tables -- read from the reference files in table_dir
+ * (read_table_*, :532-622), or, when table_dir is NULL, built on the GPU in
+ * FP64 (replacing tools/build_tables.F90).  ccn_path names the CCN
+ * activation table (read_table_ccn, :625-649).  Uses the current device. */
+int tempo_hip_init(const tempo_hip_cfg *cfg, const char *table_dir, const char *ccn_path,
+                   tempo_hip_ctx **ctx);
+
+/* Pre-allocates the per-point scratch for batches of up to ncol columns of
+ * nz levels, so that tempo_hip_run allocates nothing (capture-safe). */
+int tempo_hip_reserve(tempo_hip_ctx *ctx, int ncol, int nz);
+
+/* Replaces tempo_run (src/module_mp_tempo_driver.F90:187-513): one
+ * microphysics step of every column, state updated in place, diagnostics
+ * written; stream-ordered on `stream` (a hipStream_t, NULL = default). */
+int tempo_hip_run(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_diags *d,
+                  const tempo_hip_dims *dims, double dt, int itimestep, int precision, void *stream);
+
+/* As tempo_hip_run with host arrays (FP64, Fortran layout of `dims`): copies
+ * to the device, runs and copies back; for hosts that keep state on the CPU.
+ * land and nlev are host pointers here. */
+int tempo_hip_run_host(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_diags *d,
+                       const tempo_hip_dims *dims, double dt, int itimestep);
+
+/* Copies a lookup table (FP64, reference file order, native endian) to host
+ * memory; n must equal the table's element count. */
+int tempo_hip_table_export(tempo_hip_ctx *ctx, int table, double *host_out, size_t n);
+
+/* Writes the three tables in the reference file format (big-endian stream,
+ * src/module_mp_tempo_tables.F90:96-444) into dir. */
+int tempo_hip_table_write(tempo_hip_ctx *ctx, const char *dir);
+
+/* Milliseconds the GPU table build took at init (0 when read from files). */
+double tempo_hip_table_build_ms(tempo_hip_ctx *ctx);
+
+/* Per-column substep counts and sedimentation tops of the last run:
+ * out[q*ncol + c], q = rain steps, rain ktop, graupel steps, graupel ktop,
+ * snow steps, snow ktop, ice steps, ice ktop, cloud steps, cloud ktop. */
+int tempo_hip_substeps(tempo_hip_ctx *ctx, int *host_out, size_t n);
+
+/* Per-phase timing of subsequent runs (no counterpart in the reference):
+ * enable != 0 records HIP events around the four launches of each run
+ * (pre-pass, level-local physics, sedimentation, final update); read
+ * returns the accumulated milliseconds per phase (ms[0..3]) and the number
+ * of runs, synchronising on the recorded events. */
+int tempo_hip_profile(tempo_hip_ctx *ctx, int enable);
+int tempo_hip_profile_read(tempo_hip_ctx *ctx, double *ms, int n, int *runs);
+
+/* Message of the last failing call on this thread. */
+const char *tempo_hip_last_error(void);
+
+/* Releases the context and its device memory (replaces nothing: the
+ * reference never frees its tables). */
+void tempo_hip_finalize(tempo_hip_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TEMPO_HIP_H */

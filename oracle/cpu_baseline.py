@@ -1,0 +1,93 @@
+"""CPU baseline: the reference TEMPO step timed on the host cores.
+
+Test and benchmark infrastructure only.  Runs the reference's own FP64 build
+at -O2 (oracle/_ref/libtempo_ref_o2.so, compiled from the reference sources
+by ``make -C oracle ref``) over disjoint column slices of a workload, one
+process per core (the reference has no threading; its parallel runs are one
+MPI rank per tile, src/module_mp_tempo_driver.F90:400-512).  The processes
+start their timed call together behind a barrier; throughput is total
+column-levels over the slowest process's wall time.
+
+    python -m oracle.cpu_baseline --tables DIR --workload aerosol --cols N --workers C
+
+DIR holds the three lookup tables in the reference file format (the
+benchmark writes the GPU-built ones there).  Prints one JSON line.  The
+parent process never touches a GPU, so it forks its workers.
+"""
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+from oracle import ref  # noqa: E402
+from tempo_amd import generator as gen  # noqa: E402
+from tempo_amd import tables_io as tio  # noqa: E402
+
+FLAGS = {
+    "aerosol": dict(aerosolaware=True, hailaware=False),
+    "plain": dict(aerosolaware=False, hailaware=False),
+    "adjoint": dict(aerosolaware=True, hailaware=False),
+    "multistep": dict(aerosolaware=True, hailaware=True),
+    "ragged": dict(aerosolaware=False, hailaware=True),
+}
+
+_TABLES = None
+
+
+def _worker(args):
+    workload, col0, ncol, barrier, q = args
+    ref.select(optimised=True)
+    ref.init(_TABLES, **FLAGS[workload])
+    st, m = gen.generate(workload, ncol=ncol, col0=col0)
+    if "nlev" in m:
+        raise SystemExit("ragged workloads are timed per column by the caller")
+    levels = st["p"].size
+    barrier.wait()
+    t0 = time.perf_counter()
+    ref.run(st, m["dt"], m["itimestep"])
+    q.put((levels, time.perf_counter() - t0))
+
+
+def main(argv=None):
+    global _TABLES
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--tables", required=True)
+    ap.add_argument("--workload", default="aerosol")
+    ap.add_argument("--cols", type=int, default=65536)
+    ap.add_argument("--workers", type=int, default=0)
+    ap.add_argument("--col0", type=int, default=0)
+    a = ap.parse_args(argv)
+    if not ref.available(optimised=True):
+        raise SystemExit("reference -O2 build missing: make -C oracle ref")
+    workers = a.workers or min(16, len(os.sched_getaffinity(0)))
+    _TABLES = tio.read_dir(a.tables)
+    ctx = mp.get_context("fork")
+    barrier = ctx.Barrier(workers)
+    q = ctx.Queue()
+    per = [a.cols // workers + (1 if i < a.cols % workers else 0) for i in range(workers)]
+    starts = [a.col0 + sum(per[:i]) for i in range(workers)]
+    procs = [ctx.Process(target=_worker, args=((a.workload, s, n, barrier, q),)) for s, n in zip(starts, per)]
+    for p in procs:
+        p.start()
+    res = [q.get() for _ in procs]
+    for p in procs:
+        p.join()
+        if p.exitcode != 0:
+            raise SystemExit(f"worker failed with {p.exitcode}")
+    levels = sum(r[0] for r in res)
+    wall = max(r[1] for r in res)
+    cpu = sum(r[1] for r in res)
+    print(json.dumps(dict(value=levels / wall / 1e6, unit="Mcolumn-levels/s", cores=workers, kind="reference",
+                          sample=f"{a.workload} workload, {a.cols} columns x {levels // a.cols} levels "
+                                 f"(columns {a.col0}..{a.col0 + a.cols - 1}), one step, reference FP64 -O2 "
+                                 f"build, {workers} processes over disjoint column slices",
+                          wall_s=wall, cpu_s=cpu, per_core=levels / cpu / 1e6)))
+
+
+if __name__ == "__main__":
+    main()

@@ -46,6 +46,11 @@ def lib():
         _lib.oracle_qs_entry.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_double)]
         _lib.oracle_fz_rain_entry.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_double)]
         _lib.oracle_fz_cloud_entry.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_double)]
+        _lib.oracle_trace.restype = ctypes.c_longlong
+        _lib.oracle_trace.argtypes = [vp, ctypes.c_longlong]
+        _lib.oracle_index.restype = ctypes.c_int
+        _lib.oracle_index.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     return _lib
 
 
@@ -143,3 +148,49 @@ def fz_cloud_entry(i, j, k, m):
     out = (ctypes.c_double * 2)()
     lib().oracle_fz_cloud_entry(i, j, k, m, out)
     return np.array(out[:])
+
+
+INDEX_KINDS = dict(cloud=1, snow=2, temperature=3, rain=4, graupel=5, ice=6, ice_nuclei=7)
+
+
+def table_index(kind, a, b=0.0):
+    """Reference table-index helper (src/module_mp_tempo_main.F90:2493-2635)."""
+    i1, i2 = ctypes.c_int(), ctypes.c_int()
+    rc = lib().oracle_index(INDEX_KINDS[kind], float(a), float(b), ctypes.byref(i1), ctypes.byref(i2))
+    if rc:
+        raise ValueError(kind)
+    return i1.value, i2.value
+
+
+class Trace:
+    """Records the big-table elements read while the block runs:
+    dict table -> sorted unique flat indices (table 'fz', 'qs', 'qg')."""
+
+    def __init__(self, cap=1 << 24):
+        self.buf = np.zeros(cap, dtype=np.int64)
+
+    def __enter__(self):
+        lib().oracle_trace(self.buf.ctypes.data, self.buf.size)
+        return self
+
+    def __exit__(self, *exc):
+        n = lib().oracle_trace(None, 0)
+        if n >= self.buf.size:
+            raise RuntimeError("trace buffer overflow")
+        raw = self.buf[:n]
+        ids = raw >> 40
+        off = raw & ((1 << 40) - 1)
+        self.touched = {name: np.unique(off[ids == i]) for i, name in ((1, "fz"), (2, "qs"), (3, "qg"))}
+        return False
+
+
+def sparse_tables(fixture_arrays):
+    """Full-size tables holding only the entries a golden fixture recorded
+    (NaN elsewhere): any read outside them poisons the result."""
+    from tempo_amd import tables_io as tio
+    out = dict(ccn=tio.read_ccn())
+    for kind, n in (("fz", tio.N_FZ), ("qs", tio.N_QS), ("qg", tio.N_QG)):
+        t = np.full(n, np.nan)
+        t[fixture_arrays[f"tab_{kind}_idx"]] = fixture_arrays[f"tab_{kind}_val"]
+        out[kind] = t
+    return out

@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-REF_SO = os.path.join(HERE, "_ref", "libtempo_ref.so")
+REF_SO = os.path.join(HERE, "_ref", "libtempo_ref.so")          # -O0: parity oracle
+REF_SO_O2 = os.path.join(HERE, "_ref", "libtempo_ref_o2.so")    # -O2: CPU timing baseline
 
 # order of the state pointer array in ref_run (oracle/ref_bridge.F90)
 STATE_FIELDS = ("t", "p", "w", "dz", "qv", "qc", "qr", "qi", "qs", "qg", "ni", "nr",
@@ -32,18 +33,28 @@ FLAG_ORDER = ("aerosolaware", "hailaware", "semi_sedi", "refl_from_melting", "tu
               "mvd_r", "mvd_g", "nc_mixing_ratio", "ml_for_bl_nc", "ml_for_nc", "nc_opt")
 
 _lib = None
+_so = REF_SO
 
 
-def available():
-    return os.path.exists(REF_SO)
+def select(optimised):
+    """Choose the -O2 build (timing) or the -O0 build (parity, the default)
+    before the first call."""
+    global _so
+    if _lib is not None:
+        raise RuntimeError("reference library already loaded")
+    _so = REF_SO_O2 if optimised else REF_SO
+
+
+def available(optimised=False):
+    return os.path.exists(REF_SO_O2 if optimised else REF_SO)
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not available():
+        if not os.path.exists(_so):
             raise RuntimeError("reference build missing: run `make -C oracle ref`")
-        _lib = ctypes.CDLL(REF_SO)
+        _lib = ctypes.CDLL(_so)
         _lib.ref_init.restype = ctypes.c_int
         _lib.ref_init.argtypes = [ctypes.POINTER(ctypes.c_int)] + [ctypes.c_void_p] * 4
         _lib.ref_run.restype = ctypes.c_int

@@ -476,10 +476,29 @@ static const double *g_fz, *g_qs, *g_qg;
 static int g_flags[18];
 
 #define CCN(i, j, k, l, m) g_ccn[((i) - 1) + 7 * (((j) - 1) + 9 * (((k) - 1) + 7 * (((l) - 1) + 5 * ((m) - 1))))]
-#define FZR(a, i, j, k, m) g_fz[(size_t)(a) * 3388275 + ((i) - 1) + 37 * (((j) - 1) + 37 * (((k) - 1) + 45 * ((m) - 1)))]
-#define FZC(a, i, j, k, m) g_fz[(size_t)4 * 3388275 + (size_t)(a) * 9157500 + ((i) - 1) + 37 * (((j) - 1) + 100 * (((k) - 1) + 45 * ((m) - 1)))]
-#define QS(a, i, j, k, m) g_qs[(size_t)(a) * 455877 + ((i) - 1) + 37 * (((j) - 1) + 9 * (((k) - 1) + 37 * ((m) - 1)))]
-#define QG(a, i, j, n3, k, m) g_qg[(size_t)(a) * 16867449 + ((i) - 1) + 37 * (((j) - 1) + 37 * (((n3) - 1) + 9 * (((k) - 1) + 37 * ((m) - 1))))]
+
+/* optional trace of every big-table element read: (table id << 40) | flat
+ * element index, table id 1 fz, 2 qs, 3 qg.  Used to cut golden fixtures
+ * down to the entries a case touches. */
+static long long *g_trace = NULL;
+static long long g_trace_cap = 0, g_trace_n = 0;
+static double tab_at(const double *base, int id, size_t off)
+{
+    if (g_trace && g_trace_n < g_trace_cap) g_trace[g_trace_n++] = ((long long)id << 40) | (long long)off;
+    return base[off];
+}
+long long oracle_trace(long long *buf, long long cap)
+{
+    long long n = g_trace_n;
+    g_trace = buf;
+    g_trace_cap = cap;
+    g_trace_n = 0;
+    return n;
+}
+#define FZR(a, i, j, k, m) tab_at(g_fz, 1, (size_t)(a) * 3388275 + ((i) - 1) + 37 * (((j) - 1) + 37 * (((k) - 1) + 45 * ((m) - 1))))
+#define FZC(a, i, j, k, m) tab_at(g_fz, 1, (size_t)4 * 3388275 + (size_t)(a) * 9157500 + ((i) - 1) + 37 * (((j) - 1) + 100 * (((k) - 1) + 45 * ((m) - 1))))
+#define QS(a, i, j, k, m) tab_at(g_qs, 2, (size_t)(a) * 455877 + ((i) - 1) + 37 * (((j) - 1) + 9 * (((k) - 1) + 37 * ((m) - 1))))
+#define QG(a, i, j, n3, k, m) tab_at(g_qg, 3, (size_t)(a) * 16867449 + ((i) - 1) + 37 * (((j) - 1) + 37 * (((n3) - 1) + 9 * (((k) - 1) + 37 * ((m) - 1)))))
 enum { TCS_RACS1, TMR_RACS1, TCS_RACS2, TMR_RACS2, TCR_SACR1, TMS_SACR1, TCR_SACR2, TMS_SACR2,
        TNR_RACS1, TNR_RACS2, TNR_SACR1, TNR_SACR2 };
 enum { TCG_RACG, TMR_RACG, TCR_GACR, TNR_RACG, TNR_GACR };
@@ -2559,5 +2578,25 @@ int oracle_get_array(int which, double *out, int n)
     default: return 1;
     }
     (void)n;
+    return 0;
+}
+
+/* table-index helpers (src/module_mp_tempo_main.F90:2493-2635, 2839-2858),
+ * kind: 1 cloud(rc,nc) 2 snow(rs) 3 temperature(T) 4 rain(rr,ilamr)
+ * 5 graupel(rg,ilamg; idx_bg1) 6 ice(ri,ni) 7 ice nuclei(xni) */
+int oracle_index(int kind, double a, double b, int *i1, int *i2)
+{
+    const params_t *p = &P;
+    *i2 = 0;
+    switch (kind) {
+    case 1: cloud_index(p, a, b, i1, i2); break;
+    case 2: *i1 = decade_index(a, p->nis2, 37); break;
+    case 3: *i1 = temperature_index(a); break;
+    case 4: rain_index(p, a, b, i1, i2); break;
+    case 5: graupel_index(p, a, b, IDX_BG1, i1, i2); break;
+    case 6: ice_index(p, a, b, i1, i2); break;
+    case 7: *i1 = in_index(p, a); break;
+    default: return 1;
+    }
     return 0;
 }

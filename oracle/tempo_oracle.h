@@ -53,6 +53,13 @@ void oracle_qs_entry(int i, int j, int k, int m, double *out12);
 void oracle_fz_rain_entry(int i, int j, int k, int m, double *out4);
 void oracle_fz_cloud_entry(int i, int j, int k, int m, double *out2);
 
+/* Trace big-table reads into buf (NULL stops tracing); returns the number of
+ * entries recorded since the previous call. */
+long long oracle_trace(long long *buf, long long cap);
+
+/* Table-index helpers for the known-answer tests; see tempo_oracle.c. */
+int oracle_index(int kind, double a, double b, int *i1, int *i2);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,588 @@
+// C ABI of the TEMPO column step (declared in include/tempo_hip.h).
+#include "tempo_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tempo_host.hpp"
+#include "tempo_step.hpp"
+
+namespace tempo {
+// defined in tempo_step_f32.hip / tempo_step_f64.hip / tempo_tables.hip
+hipError_t upload_const(const Const<float>& c, hipStream_t s);
+hipError_t upload_const(const Const<double>& c, hipStream_t s);
+hipError_t launch_step(const StepArgs<float>& a, double* vtbuf, hipStream_t s, hipEvent_t* ev);
+hipError_t launch_step(const StepArgs<double>& a, double* vtbuf, hipStream_t s, hipEvent_t* ev);
+hipError_t launch_table_qg(const double*, const double*, const double*, const double*, double*, hipStream_t);
+hipError_t launch_table_qs(const double*, const double*, const double*, const double*, double*, hipStream_t);
+hipError_t launch_table_fz(const double*, const double*, const double*, const double*, double, const double*,
+                           const double*, const double*, const double*, const double*, double*, hipStream_t);
+template <class T>
+hipError_t launch_pack(const double* src, size_t n, int narr, T* dst, hipStream_t s);
+}  // namespace tempo
+
+using namespace tempo;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) return fail(TEMPO_HIP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t ensure(size_t n) {
+    if (n <= bytes) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      if (e != hipSuccess) return e;
+      p = nullptr;
+      bytes = 0;
+    }
+    hipError_t e = hipMalloc(&p, n);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+template <class T>
+hipError_t upload(DevBuf& b, const std::vector<T>& v) {
+  hipError_t e = b.ensure(v.size() * sizeof(T));
+  if (e != hipSuccess) return e;
+  return hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+// Const<double> -> Const<float>: the R-typed members form one contiguous
+// prefix followed by the int members.
+void narrow(const Const<double>& d, Const<float>& f) {
+  constexpr size_t nr = offsetof(Const<double>, nic2) / sizeof(double);
+  static_assert(offsetof(Const<float>, nic2) == nr * sizeof(float), "Const layout");
+  const double* src = reinterpret_cast<const double*>(&d);
+  float* dst = reinterpret_cast<float*>(&f);
+  for (size_t i = 0; i < nr; ++i) dst[i] = float(src[i]);
+  std::memcpy(&f.nic2, &d.nic2, sizeof(int) * 10);
+}
+
+bool read_file(const std::string& path, std::vector<char>& out, std::string& err) {
+  FILE* fp = std::fopen(path.c_str(), "rb");
+  if (!fp) {
+    err = "cannot open " + path;
+    return false;
+  }
+  std::fseek(fp, 0, SEEK_END);
+  long n = std::ftell(fp);
+  std::fseek(fp, 0, SEEK_SET);
+  out.resize(size_t(n));
+  size_t got = n > 0 ? std::fread(out.data(), 1, size_t(n), fp) : 0;
+  std::fclose(fp);
+  if (got != size_t(n)) {
+    err = "short read " + path;
+    return false;
+  }
+  return true;
+}
+
+template <class T>
+T bswap(T v) {
+  T r;
+  const unsigned char* s = reinterpret_cast<const unsigned char*>(&v);
+  unsigned char* d = reinterpret_cast<unsigned char*>(&r);
+  for (size_t i = 0; i < sizeof(T); ++i) d[i] = s[sizeof(T) - 1 - i];
+  return r;
+}
+
+}  // namespace
+
+struct tempo_hip_ctx {
+  tempo_hip_cfg cfg;
+  int device = 0;
+  Const<double> cd;
+  Const<float> cf;
+  // reference-order FP64 tables
+  DevBuf qg_ref, qs_ref, fz_ref;
+  // gather layouts
+  DevBuf qg64, qs64, fzr64, fzc64, qg32, qs32, fzr32, fzc32;
+  DevBuf ccn;  // float slice (7,9,7)
+  DevBuf efrw64, efsw64, wev64, iaus64, efrw32, efsw32, wev32, iaus32;
+  // scratch
+  DevBuf sr, sd, pf, cf_, precip, ksub, vtbuf;
+  size_t cap_plane = 0, cap_ncol = 0;
+  int last_ncol = 0;
+  // host-array staging
+  std::vector<DevBuf> stage;
+  DevBuf stage_land, stage_nlev;
+  double build_ms = 0.0;
+  // per-phase timing: 5 events per profiled run
+  bool profile = false;
+  std::vector<hipEvent_t> events;
+  ~tempo_hip_ctx() {
+    for (auto e : events) (void)hipEventDestroy(e);
+  }
+};
+
+extern "C" {
+
+void tempo_hip_cfg_default(tempo_hip_cfg* c) {
+  std::memset(c, 0, sizeof(*c));
+  c->aerosolaware_flag = 1;
+  c->hailaware_flag = 1;
+  c->cloud_condensation_flag = 1;
+  c->nc_option = 0;
+  c->nc_value = 100.e6;
+  c->refl10cm_flag = 1;
+  c->re_cloud_flag = 1;
+  c->re_ice_flag = 1;
+  c->re_snow_flag = 1;
+  c->max_hail_diameter_flag = 1;
+}
+
+const char* tempo_hip_last_error(void) { return g_err.c_str(); }
+
+void tempo_hip_finalize(tempo_hip_ctx* ctx) { delete ctx; }
+
+double tempo_hip_table_build_ms(tempo_hip_ctx* ctx) { return ctx ? ctx->build_ms : 0.0; }
+
+}  // extern "C"
+
+namespace {
+
+int upload_small_tables(tempo_hip_ctx* x) {
+  HostBins hb;
+  Const<double> c;
+  build_const(c, hb, x->cfg.hailaware_flag != 0);
+  SmallTables t;
+  build_small_tables(c, hb, t);
+  auto both = [&](const std::vector<double>& v, DevBuf& d64, DevBuf& d32) -> hipError_t {
+    hipError_t e = upload(d64, v);
+    if (e != hipSuccess) return e;
+    std::vector<float> f(v.begin(), v.end());
+    return upload(d32, f);
+  };
+  HIP_TRY(both(t.efrw, x->efrw64, x->efrw32));
+  HIP_TRY(both(t.efsw, x->efsw64, x->efsw32));
+  HIP_TRY(both(t.tnc_wev, x->wev64, x->wev32));
+  std::vector<double> iaus(t.tps_iaus);
+  iaus.insert(iaus.end(), t.tni_iaus.begin(), t.tni_iaus.end());
+  iaus.insert(iaus.end(), t.tpi_ide.begin(), t.tpi_ide.end());
+  HIP_TRY(both(iaus, x->iaus64, x->iaus32));
+  return TEMPO_HIP_OK;
+}
+
+int build_tables_gpu(tempo_hip_ctx* x) {
+  TableFactors f;
+  build_table_factors(f);
+  DevBuf nr, qga, qgng, qsa, qsns, massr, massc, probr, probc, nc, tnc, rr, rc;
+  HIP_TRY(upload(nr, f.n_r));
+  HIP_TRY(upload(qga, f.qg_a));
+  HIP_TRY(upload(qgng, f.qg_ng));
+  HIP_TRY(upload(qsa, f.qs_a));
+  HIP_TRY(upload(qsns, f.qs_ns));
+  HIP_TRY(upload(massr, f.fz_massr));
+  HIP_TRY(upload(massc, f.fz_massc));
+  HIP_TRY(upload(probr, f.fz_prob_r));
+  HIP_TRY(upload(probc, f.fz_prob_c));
+  HIP_TRY(upload(nc, f.fz_nc));
+  HIP_TRY(upload(tnc, f.fz_t_nc));
+  HIP_TRY(upload(rr, f.r_r));
+  HIP_TRY(upload(rc, f.r_c));
+  HIP_TRY(x->qg_ref.ensure(kNQgArr * kNQg1 * sizeof(double)));
+  HIP_TRY(x->qs_ref.ensure(kNQsArr * kNQs1 * sizeof(double)));
+  HIP_TRY(x->fz_ref.ensure((kNFzRArr * kNFzR1 + kNFzCArr * kNFzC1) * sizeof(double)));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, nullptr));
+  HIP_TRY(launch_table_qg(qga.as<double>(), qgng.as<double>(), nr.as<double>(), rr.as<double>(),
+                          x->qg_ref.as<double>(), nullptr));
+  HIP_TRY(launch_table_qs(qsa.as<double>(), qsns.as<double>(), nr.as<double>(), rr.as<double>(),
+                          x->qs_ref.as<double>(), nullptr));
+  HIP_TRY(launch_table_fz(nr.as<double>(), probr.as<double>(), massr.as<double>(), rr.as<double>(), f.fz_xm0g,
+                          nc.as<double>(), probc.as<double>(), massc.as<double>(), tnc.as<double>(), rc.as<double>(),
+                          x->fz_ref.as<double>(), nullptr));
+  HIP_TRY(hipEventRecord(e1, nullptr));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  x->build_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return TEMPO_HIP_OK;
+}
+
+int read_tables(tempo_hip_ctx* x, const std::string& dir) {
+  struct Item {
+    const char* name;
+    DevBuf* dst;
+    size_t n;
+  } items[3] = {{"freeze_water_data_tempo_v3", &x->fz_ref, kNFzRArr * kNFzR1 + kNFzCArr * kNFzC1},
+                {"qr_acr_qs_data_tempo_v3", &x->qs_ref, kNQsArr * kNQs1},
+                {"qr_acr_qg_data_tempo_v3", &x->qg_ref, kNQgArr * kNQg1}};
+  for (auto& it : items) {
+    std::vector<char> raw;
+    std::string err;
+    if (!read_file(dir + "/" + it.name, raw, err)) return fail(TEMPO_HIP_E_TABLE, err);
+    if (raw.size() != it.n * sizeof(double))
+      return fail(TEMPO_HIP_E_TABLE, std::string(it.name) + ": size " + std::to_string(raw.size()) +
+                                         " bytes, expected " + std::to_string(it.n * sizeof(double)));
+    std::vector<double> v(it.n);
+    const double* src = reinterpret_cast<const double*>(raw.data());
+    for (size_t i = 0; i < it.n; ++i) v[i] = bswap(src[i]);
+    HIP_TRY(upload(*it.dst, v));
+  }
+  return TEMPO_HIP_OK;
+}
+
+int read_ccn(tempo_hip_ctx* x, const std::string& path) {
+  std::vector<char> raw;
+  std::string err;
+  if (!read_file(path, raw, err)) return fail(TEMPO_HIP_E_TABLE, err);
+  if (raw.size() != kNCcn * 4 + 8) return fail(TEMPO_HIP_E_TABLE, path + ": wrong size");
+  uint32_t head;
+  std::memcpy(&head, raw.data(), 4);
+  if (bswap(head) != kNCcn * 4) return fail(TEMPO_HIP_E_TABLE, path + ": bad record marker");
+  const float* v = reinterpret_cast<const float*>(raw.data() + 4);
+  // slice tnccn_act(:,:,:,3,2) as [k][j][i] (src/module_mp_tempo_main.F90:2246-2266)
+  std::vector<float> s(7 * 9 * 7);
+  for (int k = 0; k < 7; ++k)
+    for (int j = 0; j < 9; ++j)
+      for (int i = 0; i < 7; ++i) {
+        size_t fo = size_t(i) + 7 * (j + 9 * (k + 7 * (2 + 5 * 1)));
+        s[i + 7 * (j + 9 * k)] = bswap(v[fo]);
+      }
+  HIP_TRY(upload(x->ccn, s));
+  return TEMPO_HIP_OK;
+}
+
+int pack_tables(tempo_hip_ctx* x) {
+  HIP_TRY(x->qg64.ensure(kNQgArr * kNQg1 * sizeof(double)));
+  HIP_TRY(x->qs64.ensure(kNQsArr * kNQs1 * sizeof(double)));
+  HIP_TRY(x->fzr64.ensure(kNFzRArr * kNFzR1 * sizeof(double)));
+  HIP_TRY(x->fzc64.ensure(kNFzCArr * kNFzC1 * sizeof(double)));
+  HIP_TRY(x->qg32.ensure(kNQgArr * kNQg1 * sizeof(float)));
+  HIP_TRY(x->qs32.ensure(kNQsArr * kNQs1 * sizeof(float)));
+  HIP_TRY(x->fzr32.ensure(kNFzRArr * kNFzR1 * sizeof(float)));
+  HIP_TRY(x->fzc32.ensure(kNFzCArr * kNFzC1 * sizeof(float)));
+  const double* fz = x->fz_ref.as<double>();
+  HIP_TRY(launch_pack<double>(x->qg_ref.as<double>(), kNQg1, kNQgArr, x->qg64.as<double>(), nullptr));
+  HIP_TRY(launch_pack<double>(x->qs_ref.as<double>(), kNQs1, kNQsArr, x->qs64.as<double>(), nullptr));
+  HIP_TRY(launch_pack<double>(fz, kNFzR1, kNFzRArr, x->fzr64.as<double>(), nullptr));
+  HIP_TRY(launch_pack<double>(fz + kNFzRArr * kNFzR1, kNFzC1, kNFzCArr, x->fzc64.as<double>(), nullptr));
+  HIP_TRY(launch_pack<float>(x->qg_ref.as<double>(), kNQg1, kNQgArr, x->qg32.as<float>(), nullptr));
+  HIP_TRY(launch_pack<float>(x->qs_ref.as<double>(), kNQs1, kNQsArr, x->qs32.as<float>(), nullptr));
+  HIP_TRY(launch_pack<float>(fz, kNFzR1, kNFzRArr, x->fzr32.as<float>(), nullptr));
+  HIP_TRY(launch_pack<float>(fz + kNFzRArr * kNFzR1, kNFzC1, kNFzCArr, x->fzc32.as<float>(), nullptr));
+  HIP_TRY(hipDeviceSynchronize());
+  return TEMPO_HIP_OK;
+}
+
+int ensure_scratch(tempo_hip_ctx* x, size_t ncol, size_t nz) {
+  const size_t plane = ncol * nz;
+  if (plane > x->cap_plane) {
+    HIP_TRY(x->sr.ensure(SR_N * plane * sizeof(double)));
+    HIP_TRY(x->sd.ensure(SD_N * plane * sizeof(double)));
+    HIP_TRY(x->pf.ensure(plane * sizeof(unsigned)));
+    HIP_TRY(x->vtbuf.ensure(3 * plane * sizeof(double)));
+    x->cap_plane = plane;
+  }
+  if (ncol > x->cap_ncol) {
+    HIP_TRY(x->cf_.ensure(ncol * sizeof(unsigned)));
+    HIP_TRY(x->precip.ensure(5 * ncol * sizeof(double)));
+    HIP_TRY(x->ksub.ensure(10 * ncol * sizeof(int)));
+    x->cap_ncol = ncol;
+  }
+  return TEMPO_HIP_OK;
+}
+
+template <class R>
+int run_impl(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags* d, const tempo_hip_dims* dm,
+             double dt, int itimestep, hipStream_t s) {
+  StepArgs<R> a;
+  std::memset(&a, 0, sizeof(a));
+  void* const fp[F_NFIELDS] = {f->t,  f->p,  f->w,  f->dz, f->qv, f->qc,   f->qr,   f->qi, f->qs, f->qg,
+                               f->ni, f->nr, f->nc, f->nwfa, f->nifa, f->ng, f->qb, f->th, f->pii};
+  for (int i = 0; i < F_NFIELDS; ++i) a.f[i] = static_cast<R*>(fp[i]);
+  if (d) {
+    void* const d2[D2_N] = {d->rain_precip, d->ice_liquid_equiv_precip, d->snow_liquid_equiv_precip,
+                            d->graupel_liquid_equiv_precip, d->frozen_fraction, d->frz_rain_precip,
+                            d->max_hail_diameter_sfc, d->max_hail_diameter_column};
+    void* const d3[D3_N] = {d->refl10cm, d->re_cloud, d->re_ice, d->re_snow, d->rain_med_vol_diam,
+                            d->graupel_med_vol_diam, d->cloud_number_mixing_ratio};
+    for (int i = 0; i < D2_N; ++i) a.d2[i] = static_cast<R*>(d2[i]);
+    for (int i = 0; i < D3_N; ++i) a.d3[i] = static_cast<R*>(d3[i]);
+  }
+  a.land = f->land;
+  a.nlev = dm->nlev;
+  a.ni = dm->ni;
+  a.nj = dm->nj;
+  a.nz = dm->nz;
+  a.ncol = dm->ni * dm->nj;
+  a.ldk = dm->ldk;
+  a.ldj = dm->ldj;
+  a.dldk = dm->dldk;
+  a.dldj = dm->dldj;
+  a.d2ldj = dm->d2ldj;
+  int rc = ensure_scratch(x, size_t(a.ncol), size_t(a.nz));
+  if (rc) return rc;
+  a.sr = x->sr.as<R>();
+  a.sd = x->sd.as<double>();
+  a.pf = x->pf.as<unsigned>();
+  a.cf = x->cf_.as<unsigned>();
+  a.precip = x->precip.as<double>();
+  a.ksub = x->ksub.as<int>();
+  x->last_ncol = a.ncol;
+  const bool r64 = sizeof(R) == 8;
+  a.tab.qg = static_cast<const R*>(r64 ? x->qg64.p : x->qg32.p);
+  a.tab.qs = static_cast<const R*>(r64 ? x->qs64.p : x->qs32.p);
+  a.tab.fzr = static_cast<const R*>(r64 ? x->fzr64.p : x->fzr32.p);
+  a.tab.fzc = static_cast<const R*>(r64 ? x->fzc64.p : x->fzc32.p);
+  a.tab.ccn = x->ccn.as<float>();
+  a.tab.efrw = static_cast<const R*>(r64 ? x->efrw64.p : x->efrw32.p);
+  a.tab.efsw = static_cast<const R*>(r64 ? x->efsw64.p : x->efsw32.p);
+  a.tab.tnc_wev = static_cast<const R*>(r64 ? x->wev64.p : x->wev32.p);
+  const R* iaus = static_cast<const R*>(r64 ? x->iaus64.p : x->iaus32.p);
+  a.tab.tps_iaus = iaus;
+  a.tab.tni_iaus = iaus + kNtbI * kNtbI1;
+  a.tab.tpi_ide = iaus + 2 * kNtbI * kNtbI1;
+  const tempo_hip_cfg& c = x->cfg;
+  a.fl.aerosolaware = c.aerosolaware_flag;
+  a.fl.hailaware = c.hailaware_flag;
+  a.fl.turn_off_micro = c.turn_off_micro_flag;
+  a.fl.cloud_condensation = c.cloud_condensation_flag;
+  a.fl.refl10cm = c.refl10cm_flag;
+  a.fl.re_cloud = c.re_cloud_flag;
+  a.fl.re_ice = c.re_ice_flag;
+  a.fl.re_snow = c.re_snow_flag;
+  a.fl.max_hail = c.max_hail_diameter_flag;
+  a.fl.mvd_r = c.rain_med_vol_diam_flag;
+  a.fl.mvd_g = c.graupel_med_vol_diam_flag;
+  a.fl.nc_mixing_ratio = c.cloud_number_mixing_ratio_flag;
+  a.fl.nc_const = c.nc_option == 1 ? kNtCO : (c.nc_option == 2 ? c.nc_value : kNtCL);
+  a.dt = dt;
+  a.first_step = itimestep == 1;
+  hipEvent_t* ev = nullptr;
+  if (x->profile) {
+    for (int i = 0; i < 5; ++i) {
+      hipEvent_t e;
+      HIP_TRY(hipEventCreate(&e));
+      x->events.push_back(e);
+    }
+    ev = &x->events[x->events.size() - 5];
+  }
+  HIP_TRY(launch_step(a, x->vtbuf.as<double>(), s, ev));
+  return TEMPO_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tempo_hip_init(const tempo_hip_cfg* cfg, const char* table_dir, const char* ccn_path, tempo_hip_ctx** out) {
+  if (!cfg || !out) return fail(TEMPO_HIP_E_ARG, "tempo_hip_init: NULL argument");
+  *out = nullptr;
+  if (cfg->semi_sedi_flag) return fail(TEMPO_HIP_E_UNSUPPORTED, "semi-Lagrangian sedimentation is not supported");
+  if (cfg->refl10cm_from_melting_flag)
+    return fail(TEMPO_HIP_E_UNSUPPORTED, "melting-layer reflectivity is not supported");
+  if (cfg->ml_for_nc_flag || cfg->ml_for_bl_nc_flag)
+    return fail(TEMPO_HIP_E_UNSUPPORTED, "ML cloud-number prediction is not supported");
+  auto* x = new tempo_hip_ctx();
+  x->cfg = *cfg;
+  int rc;
+  auto bail = [&](int r) {
+    delete x;
+    return r;
+  };
+  if (hipGetDevice(&x->device) != hipSuccess) return bail(fail(TEMPO_HIP_E_HIP, "no HIP device"));
+  HostBins hb;
+  build_const(x->cd, hb, cfg->hailaware_flag != 0);
+  narrow(x->cd, x->cf);
+  if (upload_const(x->cd, nullptr) != hipSuccess || upload_const(x->cf, nullptr) != hipSuccess)
+    return bail(fail(TEMPO_HIP_E_HIP, "constant upload failed"));
+  if ((rc = upload_small_tables(x))) return bail(rc);
+  std::string ccn = ccn_path ? ccn_path : (table_dir ? std::string(table_dir) + "/ccn_activate.bin" : "");
+  if (ccn.empty()) return bail(fail(TEMPO_HIP_E_ARG, "no CCN table path"));
+  if ((rc = read_ccn(x, ccn))) return bail(rc);
+  if (table_dir) {
+    if ((rc = read_tables(x, table_dir))) return bail(rc);
+  } else {
+    if ((rc = build_tables_gpu(x))) return bail(rc);
+  }
+  if ((rc = pack_tables(x))) return bail(rc);
+  *out = x;
+  return TEMPO_HIP_OK;
+}
+
+int tempo_hip_reserve(tempo_hip_ctx* x, int ncol, int nz) {
+  if (!x || ncol <= 0 || nz <= 0) return fail(TEMPO_HIP_E_ARG, "tempo_hip_reserve: bad arguments");
+  return ensure_scratch(x, size_t(ncol), size_t(nz));
+}
+
+static int check_run_args(const tempo_hip_fields* f, const tempo_hip_dims* dm, double dt) {
+  if (!f || !dm) return fail(TEMPO_HIP_E_ARG, "NULL fields/dims");
+  if (dm->ni <= 0 || dm->nj <= 0 || dm->nz < 2) return fail(TEMPO_HIP_E_ARG, "bad dims (need ni,nj >= 1, nz >= 2)");
+  if (dm->ldk < dm->ni) return fail(TEMPO_HIP_E_ARG, "ldk smaller than ni");
+  if (dm->nj > 1 && dm->ldj < dm->ldk * dm->nz) return fail(TEMPO_HIP_E_ARG, "ldj too small");
+  if (!(dt > 0.0)) return fail(TEMPO_HIP_E_ARG, "dt must be positive");
+  if (!f->p || !f->w || !f->dz || !f->qv || !f->qc || !f->qr || !f->qi || !f->qs || !f->qg || !f->ni || !f->nr)
+    return fail(TEMPO_HIP_E_ARG, "missing required field (p, w, dz, qv, qc, qr, qi, qs, qg, ni, nr)");
+  if (!f->t && !(f->th && f->pii)) return fail(TEMPO_HIP_E_ARG, "need t, or th and pii");
+  return TEMPO_HIP_OK;
+}
+
+int tempo_hip_run(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags* d, const tempo_hip_dims* dm,
+                  double dt, int itimestep, int precision, void* stream) {
+  if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
+  int rc = check_run_args(f, dm, dt);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (precision == TEMPO_HIP_FP32) return run_impl<float>(x, f, d, dm, dt, itimestep, s);
+  if (precision == TEMPO_HIP_FP64) return run_impl<double>(x, f, d, dm, dt, itimestep, s);
+  return fail(TEMPO_HIP_E_ARG, "precision must be TEMPO_HIP_FP32 or TEMPO_HIP_FP64");
+}
+
+int tempo_hip_run_host(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags* d,
+                       const tempo_hip_dims* dm, double dt, int itimestep) {
+  if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
+  int rc = check_run_args(f, dm, dt);
+  if (rc) return rc;
+  const size_t n3 = size_t(dm->ldj) * (dm->nj - 1) + size_t(dm->ldk) * (dm->nz - 1) + dm->ni;
+  const size_t nd3 = size_t(dm->dldj) * (dm->nj - 1) + size_t(dm->dldk) * (dm->nz - 1) + dm->ni;
+  const size_t nd2 = size_t(dm->d2ldj) * (dm->nj - 1) + dm->ni;
+  void* const hf[19] = {f->t,  f->th, f->pii, f->p,  f->w,  f->dz, f->qv, f->qc,   f->qr,   f->qi,
+                        f->qs, f->qg, f->ni,  f->nr, f->nc, f->nwfa, f->nifa, f->ng, f->qb};
+  tempo_hip_diags dd{};
+  void** hd = reinterpret_cast<void**>(const_cast<tempo_hip_diags*>(d));
+  void** ddp = reinterpret_cast<void**>(&dd);
+  const int nd = sizeof(tempo_hip_diags) / sizeof(void*);
+  if (x->stage.size() < size_t(19 + nd)) x->stage.resize(19 + nd);
+  tempo_hip_fields df = *f;
+  void** dfp[19] = {&df.t,  &df.th, &df.pii, &df.p,  &df.w,  &df.dz, &df.qv, &df.qc,   &df.qr,   &df.qi,
+                    &df.qs, &df.qg, &df.ni,  &df.nr, &df.nc, &df.nwfa, &df.nifa, &df.ng, &df.qb};
+  for (int i = 0; i < 19; ++i) {
+    if (!hf[i]) continue;
+    HIP_TRY(x->stage[i].ensure(n3 * sizeof(double)));
+    HIP_TRY(hipMemcpy(x->stage[i].p, hf[i], n3 * sizeof(double), hipMemcpyHostToDevice));
+    *dfp[i] = x->stage[i].p;
+  }
+  for (int i = 0; d && i < nd; ++i) {
+    if (!hd[i]) continue;
+    const size_t n = (i < 8 ? nd2 : nd3) * sizeof(double);
+    HIP_TRY(x->stage[19 + i].ensure(n));
+    ddp[i] = x->stage[19 + i].p;
+  }
+  tempo_hip_dims dmd = *dm;
+  const int ncol = dm->ni * dm->nj;
+  if (f->land) {
+    HIP_TRY(x->stage_land.ensure(ncol * sizeof(int)));
+    HIP_TRY(hipMemcpy(x->stage_land.p, f->land, ncol * sizeof(int), hipMemcpyHostToDevice));
+    df.land = x->stage_land.as<int>();
+  }
+  if (dm->nlev) {
+    HIP_TRY(x->stage_nlev.ensure(ncol * sizeof(int)));
+    HIP_TRY(hipMemcpy(x->stage_nlev.p, dm->nlev, ncol * sizeof(int), hipMemcpyHostToDevice));
+    dmd.nlev = x->stage_nlev.as<int>();
+  }
+  rc = run_impl<double>(x, &df, d ? &dd : nullptr, &dmd, dt, itimestep, nullptr);
+  if (rc) return rc;
+  HIP_TRY(hipDeviceSynchronize());
+  for (int i = 0; i < 19; ++i) {
+    if (!hf[i]) continue;
+    HIP_TRY(hipMemcpy(hf[i], x->stage[i].p, n3 * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  for (int i = 0; d && i < nd; ++i) {
+    if (!hd[i]) continue;
+    const size_t n = (i < 8 ? nd2 : nd3) * sizeof(double);
+    HIP_TRY(hipMemcpy(hd[i], ddp[i], n, hipMemcpyDeviceToHost));
+  }
+  return TEMPO_HIP_OK;
+}
+
+int tempo_hip_table_export(tempo_hip_ctx* x, int table, double* out, size_t n) {
+  if (!x || !out) return fail(TEMPO_HIP_E_ARG, "NULL argument");
+  const DevBuf* b = nullptr;
+  size_t want = 0;
+  switch (table) {
+    case TEMPO_HIP_TABLE_QR_ACR_QG: b = &x->qg_ref; want = kNQgArr * kNQg1; break;
+    case TEMPO_HIP_TABLE_QR_ACR_QS: b = &x->qs_ref; want = kNQsArr * kNQs1; break;
+    case TEMPO_HIP_TABLE_FREEZEWATER: b = &x->fz_ref; want = kNFzRArr * kNFzR1 + kNFzCArr * kNFzC1; break;
+    default: return fail(TEMPO_HIP_E_ARG, "unknown table id");
+  }
+  if (n != want) return fail(TEMPO_HIP_E_ARG, "table size mismatch: expected " + std::to_string(want));
+  HIP_TRY(hipMemcpy(out, b->p, n * sizeof(double), hipMemcpyDeviceToHost));
+  return TEMPO_HIP_OK;
+}
+
+int tempo_hip_table_write(tempo_hip_ctx* x, const char* dir) {
+  if (!x || !dir) return fail(TEMPO_HIP_E_ARG, "NULL argument");
+  struct Item {
+    const char* name;
+    int id;
+    size_t n;
+  } items[3] = {{"freeze_water_data_tempo_v3", TEMPO_HIP_TABLE_FREEZEWATER, kNFzRArr * kNFzR1 + kNFzCArr * kNFzC1},
+                {"qr_acr_qs_data_tempo_v3", TEMPO_HIP_TABLE_QR_ACR_QS, kNQsArr * kNQs1},
+                {"qr_acr_qg_data_tempo_v3", TEMPO_HIP_TABLE_QR_ACR_QG, kNQgArr * kNQg1}};
+  for (auto& it : items) {
+    std::vector<double> v(it.n);
+    int rc = tempo_hip_table_export(x, it.id, v.data(), it.n);
+    if (rc) return rc;
+    for (auto& e : v) e = bswap(e);
+    std::string path = std::string(dir) + "/" + it.name;
+    FILE* fp = std::fopen(path.c_str(), "wb");
+    if (!fp) return fail(TEMPO_HIP_E_TABLE, "cannot write " + path);
+    size_t w = std::fwrite(v.data(), sizeof(double), it.n, fp);
+    std::fclose(fp);
+    if (w != it.n) return fail(TEMPO_HIP_E_TABLE, "short write " + path);
+  }
+  return TEMPO_HIP_OK;
+}
+
+int tempo_hip_profile(tempo_hip_ctx* x, int enable) {
+  if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
+  for (auto e : x->events) (void)hipEventDestroy(e);
+  x->events.clear();
+  x->profile = enable != 0;
+  return TEMPO_HIP_OK;
+}
+
+int tempo_hip_profile_read(tempo_hip_ctx* x, double* ms, int n, int* runs) {
+  if (!x || !ms || n < 4) return fail(TEMPO_HIP_E_ARG, "need room for 4 phases");
+  for (int p = 0; p < 4; ++p) ms[p] = 0.0;
+  const size_t nr = x->events.size() / 5;
+  for (size_t r = 0; r < nr; ++r) {
+    HIP_TRY(hipEventSynchronize(x->events[5 * r + 4]));
+    for (int p = 0; p < 4; ++p) {
+      float t = 0.f;
+      HIP_TRY(hipEventElapsedTime(&t, x->events[5 * r + p], x->events[5 * r + p + 1]));
+      ms[p] += t;
+    }
+  }
+  if (runs) *runs = int(nr);
+  return TEMPO_HIP_OK;
+}
+
+int tempo_hip_substeps(tempo_hip_ctx* x, int* out, size_t n) {
+  if (!x || !out) return fail(TEMPO_HIP_E_ARG, "NULL argument");
+  if (n != size_t(10) * x->last_ncol) return fail(TEMPO_HIP_E_ARG, "need 10*ncol entries");
+  HIP_TRY(hipMemcpy(out, x->ksub.p, n * sizeof(int), hipMemcpyDeviceToHost));
+  return TEMPO_HIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,679 @@
+// Launchable kernels of the column step (see tempo_step.hpp for the
+// phase split).  Included by tempo_step_f32.hip and tempo_step_f64.hip.
+#pragma once
+
+#include "tempo_step.hpp"
+
+namespace tempo {
+
+template <class R>
+__device__ __forceinline__ R load_t(const StepArgs<R>& a, size_t o) {
+  return a.f[F_T] ? a.f[F_T][o] : a.f[F_TH][o] * a.f[F_PII][o];
+}
+
+template <class R>
+__device__ __forceinline__ R column_nc(const StepArgs<R>& a, int c) {
+  // one-moment cloud number when nc is absent (src/module_mp_tempo_main.F90:363-380,
+  // src/module_mp_tempo_utils.F90:12-24)
+  if (a.land) return a.land[c] != 1 ? R(kNtCO) : R(kNtCL);
+  return R(a.fl.nc_const);
+}
+
+// ------------------------------------------------------------ pre-pass ----
+template <class R>
+__global__ void __launch_bounds__(256) k_prepass(StepArgs<R> a) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.ncol) return;
+  const int nzc = a.nlev ? a.nlev[c] : a.nz;
+  unsigned cf = 0;
+  // early exit (:416-418): any hydrometeor above r1 after the initial checks
+  // (tendencies are still zero, so l_q == q1d > r1), or ice supersaturation
+  bool any_q = false;
+  for (int k = 0; k < nzc && !any_q; ++k) {
+    const size_t o = TEMPO_IDX_STATE(a, c, k);
+    any_q = a.f[F_QC][o] > R(kR1) || a.f[F_QR][o] > R(kR1) || a.f[F_QI][o] > R(kR1) || a.f[F_QS][o] > R(kR1) ||
+            a.f[F_QG][o] > R(kR1);
+  }
+  if (!any_q) {
+    bool sup = false;
+    for (int k = 0; k < nzc && !sup; ++k) {
+      const size_t o = TEMPO_IDX_STATE(a, c, k);
+      const R temp = load_t(a, o), pres = a.f[F_P][o], qv = mx(R(kMinQv), a.f[F_QV][o]);
+      const R qvs = rslf(pres, temp);
+      const R qvsi = (temp - R(kT0) <= R(0.0)) ? rsif(pres, temp) : qvs;
+      R ssati = qv / qvsi - R(1.0);
+      if (mabs(ssati) < R(kEps)) ssati = R(0.0);
+      sup = ssati > R(0.0);
+    }
+    if (!sup) cf |= CF_EXIT;
+  }
+  // first-step graupel initialisation test (:387-393)
+  if (a.first_step && a.f[F_NG] && a.f[F_QB]) {
+    R sqg = R(0), sng = R(0), sqb = R(0);
+    for (int k = 0; k < nzc; ++k) {
+      const size_t o = TEMPO_IDX_STATE(a, c, k);
+      sqg += a.f[F_QG][o];
+      sng += a.f[F_NG][o];
+      sqb += a.f[F_QB][o];
+    }
+    if (sqg > R(kR1) && sng < R(kEps) && sqb < R(kEps)) cf |= CF_GINIT;
+  }
+  // height prefix of the default aerosol profile (src/module_mp_tempo_aerosols.F90:24-28)
+  if (!a.f[F_NWFA] || !a.f[F_NIFA]) {
+    R h = R(0);
+    for (int k = 0; k < nzc; ++k) {
+      if (k > 0) h = h + a.f[F_DZ][TEMPO_IDX_STATE(a, c, k)];
+      a.sr[size_t(SR_HGT) * a.nz * a.ncol + TEMPO_IDX_SCR(a, c, k)] = h;
+    }
+  }
+  a.cf[c] = cf;
+}
+
+// ----------------------------------------------------------- phase A ----
+template <class R>
+__device__ __forceinline__ R default_aerosol(const StepArgs<R>& a, int c, int k, R n0, R n1) {
+  // init_water/ice_friendly_aerosols (src/module_mp_tempo_aerosols.F90:14-73):
+  // hgt(1) = 0, so h_01 = 0.8; level 1 uses hgt(2)
+  const R* hgt = a.sr + size_t(SR_HGT) * a.nz * a.ncol;
+  const R h1 = hgt[TEMPO_IDX_SCR(a, c, 0)];
+  const R hk = hgt[TEMPO_IDX_SCR(a, c, (k == 0 ? 1 : k))];
+  const R ni3 = R(-1.0) * mlog(n1 / n0) / R(0.8);
+  return n1 + n0 * mexp(-((hk - h1) / R(1000.0)) * ni3);
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_phase_a(StepArgs<R> a) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (c >= a.ncol) return;
+  if (a.nlev && k >= a.nlev[c]) return;
+  const Const<R>& C = cst<R>();
+  const size_t o = TEMPO_IDX_STATE(a, c, k);
+  const size_t s = TEMPO_IDX_SCR(a, c, k);
+  const size_t plane = size_t(a.nz) * a.ncol;
+  const R dt = R(a.dt), odt = R(1.0) / dt;
+  const unsigned cf = a.cf[c];
+  const bool has_nc = a.f[F_NC], has_nwfa = a.f[F_NWFA], has_nifa = a.f[F_NIFA];
+  const bool has_ngqb = a.f[F_NG] && a.f[F_QB];
+
+  Lvl<R> L;
+  const R t1d = load_t(a, o), qv1d = a.f[F_QV][o], w = a.f[F_W][o];
+  R qc1d = a.f[F_QC][o], qr1d = a.f[F_QR][o], qi1d = a.f[F_QI][o], qs1d = a.f[F_QS][o], qg1d = a.f[F_QG][o];
+  R ni1d = a.f[F_NI][o], nr1d = a.f[F_NR][o];
+  R nc1d = has_nc ? a.f[F_NC][o] : R(0), ng1d = has_ngqb ? a.f[F_NG][o] : R(0), qb1d = has_ngqb ? a.f[F_QB][o] : R(0);
+  const R nwfa1d = has_nwfa ? a.f[F_NWFA][o] : R(0), nifa1d = has_nifa ? a.f[F_NIFA][o] : R(0);
+
+  L.tten = L.qvten = L.qcten = L.qiten = L.qrten = L.qsten = L.qgten = L.qbten = R(0);
+  L.niten = L.nrten = L.ncten = L.ngten = L.nwfaten = L.nifaten = R(0);
+  L.smob = L.smoc = L.smo0 = L.smo1 = L.smo2 = L.smoe = L.smof = L.smoz = R(0);
+  L.temp = t1d;
+  L.qv = mx(R(kMinQv), qv1d);
+  L.pres = a.f[F_P][o];
+  L.rho = rho_of(C, L.pres, L.temp, L.qv);
+  // aerosols: a present field overrides the default profile (:328-345)
+  L.nwfa = has_nwfa ? nwfa1d * L.rho : default_aerosol(a, c, k, R(kNaccn0), R(kNaccn1));
+  L.nifa = has_nifa ? nifa1d * L.rho : default_aerosol(a, c, k, R(kNain0), R(kNain1));
+  L.nwfa = mx(R(kNwfaDefault) * L.rho, mn(R(kAeroMax) * L.rho, L.nwfa));
+  L.nifa = mx(R(kNifaDefault) * L.rho, mn(R(kAeroMax) * L.rho, L.nifa));
+
+  // initial checks (:347-396)
+  rain_check(C, L, qr1d, nr1d, dt, odt);
+  ice_check(C, L, qi1d, ni1d, dt, odt);
+  snow_check(L, qs1d, dt, odt);
+  if (L.l_qs) snow_moments(C, L.rs, mn(R(-0.1), L.temp - R(kT0)), 1 | 2 | 4 | 8 | 16, L);
+  const R ncv = column_nc(a, c);
+  L.ncsave = ncv;
+  cloud_check(C, L, qc1d, has_nc ? &nc1d : nullptr, ncv, dt, odt);
+  if ((cf & CF_GINIT) && qg1d > R(kR1)) {  // graupel_init (:1234-1260)
+    const R rg = qg1d * L.rho;
+    const R rb = rg * R(kM3ToL) / rho_g<R>(kIdxBg1);
+    const int idx = max(1, min(nint_exact(R(10.0) * rg / rb) + 1, kNrhg));
+    const R lamg = (R(3.0) + R(kMuG) + R(0.672)) / R(5.e-3);
+    const R ng = C.cgg[1][0] * C.ogg3 * rg * mpow(lamg, R(kBmG)) / C.am_g[idx - 1];
+    ng1d = ng / L.rho;
+    qb1d = rb / L.rho;
+  }
+  graupel_check(C, L, qg1d, has_ngqb ? &ng1d : nullptr, has_ngqb ? &qb1d : nullptr, dt, odt);
+  // the post-check state is the 1-d state for the rest of the step
+  a.f[F_QC][o] = qc1d;
+  a.f[F_QR][o] = qr1d;
+  a.f[F_NR][o] = nr1d;
+  a.f[F_QI][o] = qi1d;
+  a.f[F_NI][o] = ni1d;
+  a.f[F_QS][o] = qs1d;
+  a.f[F_QG][o] = qg1d;
+  if (has_nc) a.f[F_NC][o] = nc1d;
+  if (has_ngqb) {
+    a.f[F_NG][o] = ng1d;
+    a.f[F_QB][o] = qb1d;
+  }
+  if (cf & CF_EXIT) return;
+  L.qcten = L.qiten = L.qrten = L.qsten = L.qgten = L.ngten = L.qbten = L.niten = L.nrten = L.ncten = R(0);
+  thermo(C, L);
+
+  const bool micro = !a.fl.turn_off_micro;
+  Tend<R> t;
+  {
+    R* tp = reinterpret_cast<R*>(&t);
+    for (unsigned q = 0; q < sizeof(Tend<R>) / sizeof(R); ++q) tp[q] = R(0);
+  }
+  R vtboost = R(1.0);
+  if (micro) {
+    warm_rain(C, a.tab, L, t, odt);
+    rain_snow_graupel(C, a.tab, L, t, odt);
+    ice_nucleation(C, a.tab, L, t, has_nwfa, has_nifa, w, dt, odt);
+    const R t1s = t1_subl(C, L);
+    ice_processes(C, a.tab, L, t, t1s, w, odt);
+    vtboost = riming(C, a.tab, L, t, odt);
+    melting(C, L, t, t1s, dt, odt);
+    if (has_nwfa || has_nifa) aerosol_scavenging(C, L, t, odt);
+  }
+  check_over_depletion(L, t, odt);
+  sum_tendencies(C, L, t);
+
+  // provisional update and re-checks on copies (:459-526)
+  L.temp = t1d + L.tten * dt;
+  L.qv = mx(R(kMinQv), qv1d + L.qvten * dt);
+  L.rho = rho_of(C, L.pres, L.temp, L.qv);
+  L.nwfaten = L.nwfaten - (t.pna_rca + t.pna_sca + t.pna_gca + t.pni_iha) / L.rho;
+  L.nifaten = L.nifaten - (t.pnd_rcd + t.pnd_scd + t.pnd_gcd) / L.rho;
+  if (has_nwfa) L.nwfa = (nwfa1d + L.nwfaten * dt) * L.rho;
+  if (has_nifa) L.nifa = (nifa1d + L.nifaten * dt) * L.rho;
+  L.nwfa = mx(R(kNwfaDefault) * L.rho, mn(R(kAeroMax) * L.rho, L.nwfa));
+  L.nifa = mx(R(kNifaDefault) * L.rho, mn(R(kAeroMax) * L.rho, L.nifa));
+  {
+    R x = qc1d, xn = nc1d;
+    cloud_check(C, L, x, has_nc ? &xn : nullptr, ncv, dt, odt);
+    x = qr1d;
+    xn = nr1d;
+    rain_check(C, L, x, xn, dt, odt);
+    x = qi1d;
+    xn = ni1d;
+    ice_check(C, L, x, xn, dt, odt);
+    x = qs1d;
+    snow_check(L, x, dt, odt);
+    L.smob = L.smoc = L.smo2 = R(0);
+    if (L.l_qs) snow_moments(C, L.rs, mn(R(-0.1), L.temp - R(kT0)), 4, L);
+    x = qg1d;
+    R xg = ng1d, xb = qb1d;
+    graupel_check(C, L, x, has_ngqb ? &xg : nullptr, has_ngqb ? &xb : nullptr, dt, odt);
+  }
+  thermo(C, L);
+  if (micro && a.fl.cloud_condensation) {  // (:528-560)
+    cloud_condensation(C, a.tab, L, t, has_nwfa, has_nc, w, dt, odt);
+    L.qvten = L.qvten - t.prw_vcd;
+    L.qcten = L.qcten + t.prw_vcd;
+    L.ncten = L.ncten + t.pnc_wcd;
+    L.nwfaten = L.nwfaten - t.pnc_wcd;
+    L.tten = L.tten + L.lvap * L.ocp * t.prw_vcd;
+    R x = qc1d, xn = nc1d;
+    cloud_check(C, L, x, has_nc ? &xn : nullptr, ncv, dt, odt);
+    L.qv = mx(R(kMinQv), qv1d + L.qvten * dt);
+    L.temp = t1d + L.tten * dt;
+    L.rho = rho_of(C, L.pres, L.temp, L.qv);
+    thermo(C, L);
+  }
+  if (micro) {  // (:562-588)
+    rain_evaporation(C, L, t, odt);
+    L.qrten = L.qrten - t.prv_rev;
+    L.qvten = L.qvten + t.prv_rev;
+    L.nrten = L.nrten - t.pnr_rev;
+    L.nwfaten = L.nwfaten + t.pnr_rev;
+    L.tten = L.tten - L.lvap * L.ocp * t.prv_rev;
+    R x = qr1d, xn = nr1d;
+    rain_check(C, L, x, xn, dt, odt);
+    L.qv = mx(R(kMinQv), qv1d + L.qvten * dt);
+    L.temp = t1d + L.tten * dt;
+    L.rho = rho_of(C, L.pres, L.temp, L.qv);
+    thermo(C, L);
+  }
+
+  // level-local fall speeds (the top-down fill is done in phase B)
+  R vtrr = R(0), vtnr = R(0), vtrg = R(0), vtng = R(0), vts = R(0), vtri = R(0), vtni = R(0), vtrc = R(0), vtnc = R(0);
+  unsigned pf = (L.l_qr ? PF_QR : 0u) | (L.l_qg ? PF_QG : 0u) | (L.l_qs ? PF_QS : 0u) | (L.l_qi ? PF_QI : 0u) |
+                (L.l_qc ? PF_QC : 0u);
+  if (L.rr > R(kR1)) {  // rain_fallspeed (:1771-1774)
+    const R lamr = R(1.0) / L.ilamr;
+    vtrr = L.rhof * R(kAvR) * C.crg[5] * C.org3 * mpow(lamr, C.cre[2]) * mpow(lamr + R(kFvR), -C.cre[5]);
+    vtnr = L.rhof * R(kAvR) * C.crg[6] / C.crg[11] * mpow(lamr, C.cre[11]) * mpow(lamr + R(kFvR), -C.cre[6]);
+  }
+  if (L.rg > R(kR1)) {  // graupel_fallspeed (:1815-1832)
+    R afall, bfall;
+    if (a.f[F_QB]) {
+      const R dens_g = mx(rho_g<R>(1), mn(R(kM3ToL) * L.rg / L.rb, rho_g<R>(kNrhg)));
+      afall = R(kACoeff) * mpow((R(4.0) * dens_g * R(kGravity)) / (R(3.0) * L.rho), R(kBCoeff));
+      afall = afall * mpow(L.visco, R(1.0) - R(2.0) * R(kBCoeff));
+      bfall = R(3.0) * R(kBCoeff) - R(1.0);
+    } else {
+      afall = R(kAvGOld);
+      bfall = R(kBvGOld);
+    }
+    const int idx = L.idx_bg;
+    vtrg = L.rhof * afall * C.cgg[5][idx - 1] * C.ogg3 * mpow(L.ilamg, bfall);
+    vtng = L.rhof * afall * C.cgg[6][idx - 1] / C.cgg[11][idx - 1] * mpow(L.ilamg, bfall);
+  }
+  if (L.rs > R(kR1)) {  // snow_fallspeed (:1873-1891)
+    const R xds = L.smoc / L.smob;
+    const R mrat = R(1.0) / xds;
+    R ils1 = R(1.0) / (mrat * R(kLam0) + R(kFvS)), ils2 = R(1.0) / (mrat * R(kLam1) + R(kFvS));
+    const R t1v = R(kKap0) * C.csg[3] * mpow(ils1, C.cse[3]);
+    const R mm = mpow(mrat, R(kMuS));
+    const R t2v = R(kKap1) * mm * C.csg[9] * mpow(ils2, C.cse[9]);
+    ils1 = R(1.0) / (mrat * R(kLam0));
+    ils2 = R(1.0) / (mrat * R(kLam1));
+    const R t3v = R(kKap0) * C.csg[0] * mpow(ils1, C.cse[0]);
+    const R t4v = R(kKap1) * mm * C.csg[6] * mpow(ils2, C.cse[6]);
+    vts = L.rhof * R(kAvS) * (t1v + t2v) / (t3v + t4v);
+    if (t.prr_sml > R(0.0))
+      pf |= PF_SNOWMIX;
+    else
+      vts = vts * vtboost;
+  }
+  if (L.ri > R(kR1)) {  // ice_fallspeed (:1928-1930)
+    vtri = L.rhof * R(kAvI) * C.cig[2] * C.oig2 * mpow(L.ilami, R(kBvI));
+    vtni = L.rhof * R(kAvI) * C.cig[5] / C.cig[6] * mpow(L.ilami, R(kBvI));
+  }
+  if (L.rc > R(kR1) && w < R(0.1)) {  // cloud_fallspeed (:1971-1974)
+    const int nu = get_nuc(L.nc);
+    vtrc = L.rhof * R(kAvC) * C.ccg[4][nu] * C.ocg2[nu] * mpow(L.ilamc, R(kBvC));
+    vtnc = L.rhof * R(kAvC) * C.ccg[3][nu] * C.ocg1[nu] * mpow(L.ilamc, R(kBvC));
+  }
+
+  R* sr = a.sr;
+  sr[SR_TEMP * plane + s] = L.temp;
+  sr[SR_QV * plane + s] = L.qv;
+  sr[SR_RHO * plane + s] = L.rho;
+  sr[SR_TTEN * plane + s] = L.tten;
+  sr[SR_QVTEN * plane + s] = L.qvten;
+  sr[SR_NWFATEN * plane + s] = L.nwfaten;
+  sr[SR_NIFATEN * plane + s] = L.nifaten;
+  sr[SR_VTRR * plane + s] = vtrr;
+  sr[SR_VTNR * plane + s] = vtnr;
+  sr[SR_VTRG * plane + s] = vtrg;
+  sr[SR_VTNG * plane + s] = vtng;
+  sr[SR_VTS * plane + s] = vts;
+  sr[SR_VTRI * plane + s] = vtri;
+  sr[SR_VTNI * plane + s] = vtni;
+  sr[SR_VTRC * plane + s] = vtrc;
+  sr[SR_VTNC * plane + s] = vtnc;
+  double* sd = a.sd;
+  sd[SD_RR * plane + s] = L.rr;
+  sd[SD_NR * plane + s] = L.nr;
+  sd[SD_RG * plane + s] = L.rg;
+  sd[SD_NG * plane + s] = L.ng;
+  sd[SD_RB * plane + s] = L.rb;
+  sd[SD_RS * plane + s] = L.rs;
+  sd[SD_RI * plane + s] = L.ri;
+  sd[SD_NI * plane + s] = L.ni;
+  sd[SD_RC * plane + s] = L.rc;
+  sd[SD_NC * plane + s] = L.nc;
+  sd[SD_QRTEN * plane + s] = L.qrten;
+  sd[SD_NRTEN * plane + s] = L.nrten;
+  sd[SD_QGTEN * plane + s] = L.qgten;
+  sd[SD_NGTEN * plane + s] = L.ngten;
+  sd[SD_QBTEN * plane + s] = L.qbten;
+  sd[SD_QSTEN * plane + s] = L.qsten;
+  sd[SD_QITEN * plane + s] = L.qiten;
+  sd[SD_NITEN * plane + s] = L.niten;
+  sd[SD_QCTEN * plane + s] = L.qcten;
+  sd[SD_NCTEN * plane + s] = L.ncten;
+  a.pf[s] = pf;
+}
+
+// ----------------------------------------------------------- phase B ----
+// flux-form sedimentation of one field over one substep (:1516-1559), FP64
+template <class R>
+__device__ void sediment(const StepArgs<R>& a, int c, int nzc, double* xr, const double* vt, double* xten,
+                         double limit, int steps, int ktop, double* precip) {
+  const size_t plane = size_t(a.nz) * a.ncol;
+  const R* rho = a.sr + SR_RHO * plane;
+  const double ost = 1.0 / double(steps), dt = a.dt;
+  double above = 0.0, s0 = 0.0;
+  for (int k = nzc - 1; k >= 0; --k) {
+    const size_t s = TEMPO_IDX_SCR(a, c, k);
+    const double sed = vt[s] * xr[s];
+    if (k == nzc - 1 || k <= ktop - 1) {
+      const double odz = 1.0 / double(a.f[F_DZ][TEMPO_IDX_STATE(a, c, k)]);
+      const double orho = 1.0 / double(rho[s]);
+      const double flux = (k == nzc - 1) ? -sed : (above - sed);
+      xten[s] = xten[s] + flux * odz * ost * orho;
+      const double v = xr[s] + flux * odz * dt * ost;
+      xr[s] = v > limit ? v : limit;
+    }
+    above = sed;
+    if (k == 0) s0 = sed;
+  }
+  if (precip && xr[TEMPO_IDX_SCR(a, c, 0)] > kLowLimitPrecip) *precip = *precip + s0 * dt * ost;
+}
+
+// top-down fill of a fall-speed profile with CFL substeps and ktop
+// (:1768-1789 and the species variants); vt raw values are valid where the
+// mass exceeds r1.  Writes the filled profile to out (FP64).
+template <class R>
+__device__ void fill_fallspeed(const StepArgs<R>& a, int c, int nzc, const double* mass, const R* raw1, const R* raw2,
+                               double* out1, double* out2, bool use_max, int& substeps, int& ktop) {
+  const double dt = a.dt;
+  double v1 = 0.0, v2 = 0.0;
+  substeps = 1;
+  ktop = 1;
+  for (int k = nzc - 1; k >= 0; --k) {
+    const size_t s = TEMPO_IDX_SCR(a, c, k);
+    if (mass[s] > kR1) {
+      v1 = raw1[s];
+      if (raw2) v2 = raw2[s];
+    }
+    out1[s] = v1;
+    if (out2) out2[s] = v2;
+    // the CFL test runs in the working precision, as the reference's vt is wp
+    const R vr1 = R(v1), vr2 = R(v2);
+    const R vmax = use_max ? mx(vr1, vr2) : vr1;
+    if (vmax > R(1.e-3)) {
+      ktop = max(ktop, k + 1);
+      const R dz_by_vt = a.f[F_DZ][TEMPO_IDX_STATE(a, c, k)] / vmax;
+      substeps = max(substeps, int(R(dt) / dz_by_vt + R(1.0)));
+    }
+  }
+  if (ktop == nzc) ktop = nzc - 1;
+}
+
+template <class R>
+__global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, double* vtbuf) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.ncol) return;
+  const int nzc = a.nlev ? a.nlev[c] : a.nz;
+  const size_t plane = size_t(a.nz) * a.ncol;
+  double pr[5] = {0, 0, 0, 0, 0};  // rain graupel snow ice cloud
+  const unsigned cf = a.cf[c];
+  int kst[10] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+  if (!(cf & CF_EXIT)) {
+    unsigned any = 0;
+    for (int k = 0; k < nzc; ++k) any |= a.pf[TEMPO_IDX_SCR(a, c, k)];
+    double* sd = a.sd;
+    const R* sr = a.sr;
+    double* v1 = vtbuf;           // filled fall speeds, FP64
+    double* v2 = vtbuf + plane;
+    double* vr = vtbuf + 2 * plane;  // filled rain mass fall speed kept for snow
+    int steps, ktop;
+    if (any & PF_QR) {
+      fill_fallspeed(a, c, nzc, sd + SD_RR * plane, sr + SR_VTRR * plane, sr + SR_VTNR * plane, vr, v2, true, steps, ktop);
+      for (int n = 0; n < steps; ++n) {
+        sediment(a, c, nzc, sd + SD_RR * plane, vr, sd + SD_QRTEN * plane, kR1, steps, ktop, &pr[0]);
+        sediment(a, c, nzc, sd + SD_NR * plane, v2, sd + SD_NRTEN * plane, kR2, steps, ktop, (double*)nullptr);
+      }
+      kst[0] = steps;
+      kst[1] = ktop;
+    } else {
+      for (int k = 0; k < nzc; ++k) vr[TEMPO_IDX_SCR(a, c, k)] = 0.0;
+    }
+    if (any & PF_QG) {
+      fill_fallspeed(a, c, nzc, sd + SD_RG * plane, sr + SR_VTRG * plane, sr + SR_VTNG * plane, v1, v2, false, steps, ktop);
+      const double rb_lim = kM3ToL * kR1 / 800.0;
+      for (int n = 0; n < steps; ++n) {
+        sediment(a, c, nzc, sd + SD_RG * plane, v1, sd + SD_QGTEN * plane, kR1, steps, ktop, &pr[1]);
+        sediment(a, c, nzc, sd + SD_NG * plane, v2, sd + SD_NGTEN * plane, kR2, steps, ktop, (double*)nullptr);
+        sediment(a, c, nzc, sd + SD_RB * plane, v1, sd + SD_QBTEN * plane, rb_lim, steps, ktop, (double*)nullptr);
+      }
+      kst[2] = steps;
+      kst[3] = ktop;
+    }
+    if (any & PF_QS) {
+      // snow_fallspeed mixes in rain where snow melts (:1886-1891), using the
+      // rain mass after rain sedimentation and the filled rain fall speed
+      const double dt = a.dt;
+      double vprev = 0.0;
+      steps = 1;
+      ktop = 1;
+      for (int k = nzc - 1; k >= 0; --k) {
+        const size_t s = TEMPO_IDX_SCR(a, c, k);
+        const double rs = sd[SD_RS * plane + s];
+        if (rs > kR1) {
+          R v = sr[SR_VTS * plane + s];
+          if (a.pf[s] & PF_SNOWMIX) {
+            const R sr_ = R(rs / (rs + sd[SD_RR * plane + s]));
+            v = v * sr_ + (R(1.0) - sr_) * R(vr[s]);
+          }
+          vprev = v;
+        }
+        v1[s] = vprev;
+        const R vv = R(vprev);
+        if (vv > R(1.e-3)) {
+          ktop = max(ktop, k + 1);
+          const R dz_by_vt = a.f[F_DZ][TEMPO_IDX_STATE(a, c, k)] / vv;
+          steps = max(steps, int(R(dt) / dz_by_vt + R(1.0)));
+        }
+      }
+      if (ktop == nzc) ktop = nzc - 1;
+      for (int n = 0; n < steps; ++n)
+        sediment(a, c, nzc, sd + SD_RS * plane, v1, sd + SD_QSTEN * plane, kR1, steps, ktop, &pr[2]);
+      kst[4] = steps;
+      kst[5] = ktop;
+    }
+    if (any & PF_QI) {
+      // one call with steps = substeps (the reference applies 1/steps of the flux)
+      fill_fallspeed(a, c, nzc, sd + SD_RI * plane, sr + SR_VTRI * plane, sr + SR_VTNI * plane, v1, v2, false, steps, ktop);
+      sediment(a, c, nzc, sd + SD_RI * plane, v1, sd + SD_QITEN * plane, kR1, steps, ktop, &pr[3]);
+      sediment(a, c, nzc, sd + SD_NI * plane, v2, sd + SD_NITEN * plane, kR2, steps, ktop, (double*)nullptr);
+      kst[6] = steps;
+      kst[7] = ktop;
+    }
+    if (any & PF_QC) {
+      // cloud_fallspeed (:1945-1977): settling only below 500 m
+      int kt = 1;
+      R hgt = R(0.0);
+      for (int k = 0; k < nzc - 1; ++k) {
+        if (sd[SD_RC * plane + TEMPO_IDX_SCR(a, c, k)] > kR2) kt = k + 1;
+        hgt = hgt + a.f[F_DZ][TEMPO_IDX_STATE(a, c, k)];
+        if (hgt > R(500.0)) break;
+      }
+      for (int k = 0; k < nzc; ++k) {
+        const size_t s = TEMPO_IDX_SCR(a, c, k);
+        v1[s] = (k < kt) ? double(sr[SR_VTRC * plane + s]) : 0.0;
+        v2[s] = (k < kt) ? double(sr[SR_VTNC * plane + s]) : 0.0;
+      }
+      sediment(a, c, nzc, sd + SD_RC * plane, v1, sd + SD_QCTEN * plane, kR1, 1, kt, &pr[4]);
+      sediment(a, c, nzc, sd + SD_NC * plane, v2, sd + SD_NCTEN * plane, kR2, 1, kt, (double*)nullptr);
+      kst[8] = 1;
+      kst[9] = kt;
+    }
+  }
+  for (int q = 0; q < 5; ++q) a.precip[size_t(q) * a.ncol + c] = pr[q];
+  if (a.ksub)
+    for (int q = 0; q < 10; ++q) a.ksub[size_t(q) * a.ncol + c] = kst[q];
+  // column diagnostics (src/module_mp_tempo_main.F90:784-794, driver :452-457)
+  const size_t d2 = size_t(c % a.ni) + size_t(c / a.ni) * a.d2ldj;
+  const bool live = !(cf & CF_EXIT);
+  const R rain = R(pr[0]), graupel = R(pr[1]), snow = R(pr[2]), ice = R(pr[3]), cloud = R(pr[4]);
+  if (a.d2[D2_RAIN]) a.d2[D2_RAIN][d2] = rain;
+  if (a.d2[D2_ICE]) a.d2[D2_ICE][d2] = ice;
+  if (a.d2[D2_SNOW]) a.d2[D2_SNOW][d2] = snow;
+  if (a.d2[D2_GRAUPEL]) a.d2[D2_GRAUPEL][d2] = graupel;
+  if (a.d2[D2_FROZEN])
+    a.d2[D2_FROZEN][d2] = live ? (ice + snow + graupel) / (ice + snow + graupel + rain + R(kR1)) : R(0);
+  if (a.d2[D2_FRZRAIN]) {
+    R frz = R(0);
+    if (live && (a.sr[SR_TEMP * plane + TEMPO_IDX_SCR(a, c, 0)] - R(kT0)) < R(-0.5)) frz = rain + cloud;
+    a.d2[D2_FRZRAIN][d2] = frz;
+  }
+  if (a.d2[D2_HAIL_SFC]) a.d2[D2_HAIL_SFC][d2] = R(0);
+  if (a.d2[D2_HAIL_COL]) a.d2[D2_HAIL_COL][d2] = R(0);
+}
+
+// ----------------------------------------------------------- phase C ----
+__device__ __forceinline__ void atomic_max_pos(float* p, float v) {
+  atomicMax(reinterpret_cast<unsigned int*>(p), __float_as_uint(v));
+}
+__device__ __forceinline__ void atomic_max_pos(double* p, double v) {
+  atomicMax(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_phase_c(StepArgs<R> a) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (c >= a.ncol) return;
+  if (a.nlev && k >= a.nlev[c]) return;
+  const Const<R>& C = cst<R>();
+  const size_t o = TEMPO_IDX_STATE(a, c, k);
+  const size_t s = TEMPO_IDX_SCR(a, c, k);
+  const size_t plane = size_t(a.nz) * a.ncol;
+  const size_t d3 = size_t(c % a.ni) + size_t(k) * a.dldk + size_t(c / a.ni) * a.dldj;
+  const R dt = R(a.dt), odt = R(1.0) / dt;
+  const bool live = !(a.cf[c] & CF_EXIT);
+  const bool has_nc = a.f[F_NC], has_nwfa = a.f[F_NWFA], has_nifa = a.f[F_NIFA];
+  const bool has_ngqb = a.f[F_NG] && a.f[F_QB];
+  if (!live) {  // early exit: the driver's reset values (src/module_mp_tempo_driver.F90:304-386)
+    if (a.d3[D3_REFL]) a.d3[D3_REFL][d3] = R(-35.0);
+    for (int q = D3_RE_C; q < D3_N; ++q)
+      if (a.d3[q]) a.d3[q][d3] = R(0);
+    return;
+  }
+  Lvl<R> L;
+  const R* sr = a.sr;
+  const double* sd = a.sd;
+  L.temp = sr[SR_TEMP * plane + s];
+  L.qv = sr[SR_QV * plane + s];
+  L.rho = sr[SR_RHO * plane + s];
+  L.tten = sr[SR_TTEN * plane + s];
+  L.qvten = sr[SR_QVTEN * plane + s];
+  L.nwfaten = sr[SR_NWFATEN * plane + s];
+  L.nifaten = sr[SR_NIFATEN * plane + s];
+  L.qrten = R(sd[SD_QRTEN * plane + s]);
+  L.nrten = R(sd[SD_NRTEN * plane + s]);
+  L.qgten = R(sd[SD_QGTEN * plane + s]);
+  L.ngten = R(sd[SD_NGTEN * plane + s]);
+  L.qbten = R(sd[SD_QBTEN * plane + s]);
+  L.qsten = R(sd[SD_QSTEN * plane + s]);
+  L.qiten = R(sd[SD_QITEN * plane + s]);
+  L.niten = R(sd[SD_NITEN * plane + s]);
+  L.qcten = R(sd[SD_QCTEN * plane + s]);
+  L.ncten = R(sd[SD_NCTEN * plane + s]);
+  L.pres = a.f[F_P][o];
+  const R lvap = R(kLvap0) + R(2106.0 - 4218.0) * (L.temp - R(kT0));
+  const R ocp = R(1.0) / (R(kCp) * (R(1.0) + R(0.887) * L.qv));
+  R qc1d = a.f[F_QC][o], qr1d = a.f[F_QR][o], qi1d = a.f[F_QI][o], qs1d = a.f[F_QS][o], qg1d = a.f[F_QG][o];
+  R ni1d = a.f[F_NI][o], nr1d = a.f[F_NR][o];
+  R nc1d = has_nc ? a.f[F_NC][o] : R(0), ng1d = has_ngqb ? a.f[F_NG][o] : R(0), qb1d = has_ngqb ? a.f[F_QB][o] : R(0);
+  const R ncv = column_nc(a, c);
+
+  if (!a.fl.turn_off_micro) {  // freeze_cloud_melt_ice (:2124-2165)
+    const R xri = mx(R(0.0), qi1d + L.qiten * dt);
+    if ((L.temp > R(kT0)) && (xri > R(0.0))) {
+      L.qcten = L.qcten + xri * odt;
+      L.ncten = L.ncten + ni1d * odt;
+      L.qiten = L.qiten - xri * odt;
+      L.niten = -ni1d * odt;
+      L.tten = L.tten - C.lfus * ocp * xri * odt;
+    }
+    const R xrc = mx(R(0.0), qc1d + L.qcten * dt);
+    if ((L.temp < R(kHgFrz)) && (xrc > R(0.0))) {
+      const R lfus2 = R(kLsub) - lvap;
+      const R xnc = has_nc ? nc1d + L.ncten * dt : ncv / L.rho + L.ncten * dt;
+      L.qiten = L.qiten + xrc * odt;
+      L.niten = L.niten + xnc * odt;
+      L.qcten = L.qcten - xrc * odt;
+      L.ncten = L.ncten - xnc * odt;
+      L.tten = L.tten + lfus2 * ocp * xrc * odt;
+    }
+  }
+  // final update (:738-748)
+  const R t_new = load_t(a, o) + L.tten * dt;
+  const R qv_new = mx(R(kMinQv), (a.f[F_QV][o] + L.qvten * dt));
+  L.rho = rho_of(C, L.pres, L.temp, L.qv);
+  if (has_nwfa) a.f[F_NWFA][o] = mx(R(kNwfaDefault), mn(R(kAeroMax), (a.f[F_NWFA][o] + L.nwfaten * dt)));
+  if (has_nifa) a.f[F_NIFA][o] = mx(R(kNifaDefault), mn(R(kAeroMax), (a.f[F_NIFA][o] + L.nifaten * dt)));
+  // final checks (:750-781)
+  cloud_check(C, L, qc1d, has_nc ? &nc1d : nullptr, ncv, dt, odt);
+  rain_check(C, L, qr1d, nr1d, dt, odt);
+  ice_check(C, L, qi1d, ni1d, dt, odt);
+  snow_check(L, qs1d, dt, odt);
+  L.smob = L.smoc = L.smo2 = L.smoz = R(0);
+  if (L.l_qs) snow_moments(C, L.rs, mn(R(-0.1), L.temp - R(kT0)), 4 | 32, L);
+  graupel_check(C, L, qg1d, has_ngqb ? &ng1d : nullptr, has_ngqb ? &qb1d : nullptr, dt, odt);
+
+  if (a.f[F_T])
+    a.f[F_T][o] = t_new;
+  else
+    a.f[F_TH][o] = t_new / a.f[F_PII][o];
+  a.f[F_QV][o] = qv_new;
+  a.f[F_QC][o] = qc1d;
+  a.f[F_QR][o] = qr1d;
+  a.f[F_NR][o] = nr1d;
+  a.f[F_QI][o] = qi1d;
+  a.f[F_NI][o] = ni1d;
+  a.f[F_QS][o] = qs1d;
+  a.f[F_QG][o] = qg1d;
+  if (has_nc) a.f[F_NC][o] = nc1d;
+  if (has_ngqb) {
+    a.f[F_NG][o] = ng1d;
+    a.f[F_QB][o] = qb1d;
+  }
+
+  // diagnostics (:796-863, src/module_mp_tempo_diags.F90)
+  if (a.d3[D3_NC_MIX]) a.d3[D3_NC_MIX][d3] = a.fl.nc_mixing_ratio ? L.nc * L.rho : R(0);
+  if (a.d3[D3_MVD_R]) a.d3[D3_MVD_R][d3] = a.fl.mvd_r ? L.mvd_r : R(0);
+  if (a.d3[D3_MVD_G]) a.d3[D3_MVD_G][d3] = a.fl.mvd_g ? L.mvd_g : R(0);
+  if (a.fl.max_hail) {  // max_hail_diam (:398-438)
+    R hail = R(0.0);
+    if (L.rg / L.rho >= R(1.e-6) && rho_g<R>(L.idx_bg) >= R(350.0)) {
+      const R lamg = R(1.0) / L.ilamg;
+      const R n0_g = L.ng * C.ogg2 * mpow(lamg, C.cge[1][0]);
+      R sum_nh = R(0), sum_t = R(0);
+      const R thr = R(0.0005);
+      int n;
+      for (n = kNhbins; n >= 1; --n) {
+        const R f_d = n0_g * mpow(C.hbins[n - 1], R(kMuG)) * mexp(-lamg * C.hbins[n - 1]) * C.dhbins[n - 1];
+        sum_nh = sum_nh + f_d;
+        if (sum_nh > thr) break;
+        sum_t = sum_nh;
+      }
+      R hmax;
+      if (n >= kNhbins)
+        hmax = C.hbins[kNhbins - 1];
+      else if (C.hbins[n] > R(1.e-3))
+        hmax = C.hbins[n - 1] - (sum_nh - thr) / (sum_nh - sum_t) * (C.hbins[n - 1] - C.hbins[n]);
+      else
+        hmax = R(1.e-4);
+      hail = R(1000.0) * hmax;
+    }
+    const size_t d2 = size_t(c % a.ni) + size_t(c / a.ni) * a.d2ldj;
+    if (k == 0 && a.d2[D2_HAIL_SFC]) a.d2[D2_HAIL_SFC][d2] = hail;
+    if (a.d2[D2_HAIL_COL] && hail > R(0)) atomic_max_pos(&a.d2[D2_HAIL_COL][d2], hail);
+  }
+  if (a.d3[D3_REFL]) {  // reflectivity_10cm (:59-121)
+    R dbz = R(-35.0);
+    if (a.fl.refl10cm) {
+      R ze_r = R(1.e-22), ze_s = R(1.e-22), ze_g = R(1.e-22);
+      if (L.l_qr) {
+        const R lamr = R(1.0) / L.ilamr;
+        const R n0_r = L.nr * C.org2 * mpow(lamr, C.cre[1]);
+        ze_r = n0_r * C.crg[3] * mpow(L.ilamr, C.cre[3]);
+      }
+      const R k6 = R(0.176 / 0.93) * (R(6.0) / R(kPi)) * (R(6.0) / R(kPi));
+      if (L.l_qs) ze_s = k6 * (R(kAmS) / R(900.0)) * (R(kAmS) / R(900.0)) * L.smoz;
+      if (L.l_qg) {
+        const int idx = L.idx_bg;
+        const R n0_g = L.ng * C.ogg2 * mpow(R(1.0) / L.ilamg, C.cge[1][0]);
+        ze_g = k6 * (C.am_g[idx - 1] / R(900.0)) * (C.am_g[idx - 1] / R(900.0)) * n0_g * C.cgg[3][0] *
+               mpow(L.ilamg, C.cge[3][0]);
+      }
+      dbz = mx(R(-35.0), R(10.0) * mlog10((ze_r + ze_s + ze_g) * R(1.e18)));
+    }
+    a.d3[D3_REFL][d3] = dbz;
+  }
+  const bool re = a.fl.re_cloud && a.fl.re_ice && a.fl.re_snow;
+  R re_c = R(0), re_i = R(0), re_s = R(0);
+  if (re) {  // effective_radius (:13-56)
+    if (L.l_qc) re_c = mx(R(2.51e-6), mn(R(0.5) * (R(3.0) + R(get_nuc(L.nc))) * L.ilamc, R(50.e-6)));
+    if (L.l_qi) re_i = mx(R(2.51e-6), mn(R(0.5) * (R(3.0) + R(kMuI)) * L.ilami, R(125.e-6)));
+    if (L.l_qs) {
+      Lvl<R> M;
+      snow_moments(C, L.rs, mn(R(-0.1), L.temp - R(kT0)), 0, M);
+      re_s = mx(R(5.01e-6), mn(R(0.5) * (M.smoc / M.smob), R(999.e-6)));
+    }
+  }
+  if (a.d3[D3_RE_C]) a.d3[D3_RE_C][d3] = re_c;
+  if (a.d3[D3_RE_I]) a.d3[D3_RE_I][d3] = re_i;
+  if (a.d3[D3_RE_S]) a.d3[D3_RE_S][d3] = re_s;
+}
+
+}  // namespace tempo

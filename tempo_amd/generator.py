@@ -162,3 +162,129 @@ def generate(workload, ncol=None, col0=0, seed=None):
         meta["nlev"] = nlev.astype(np.int32)
     state = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in state.items()}
     return state, meta
+
+
+# ---- the same generator on a torch device -----------------------------------
+# Bit-identical counters (int64 arithmetic wraps like uint64; logical shifts
+# are masked), float64 transcendental math from the device's libm, so values
+# agree with generate() to a few ulp.  Used to synthesise large batches
+# directly in device memory.
+
+def _s64(u):
+    return u - (1 << 64) if u >= (1 << 63) else u
+
+
+_C1, _C2, _C3 = _s64(0x9E3779B97F4A7C15), _s64(0xBF58476D1CE4E5B9), _s64(0x94D049BB133111EB)
+
+
+def _shr(x, s):
+    return (x >> s) & ((1 << (64 - s)) - 1)
+
+
+def _splitmix_t(x):
+    x = x + _C1
+    x = (x ^ _shr(x, 30)) * _C2
+    x = (x ^ _shr(x, 27)) * _C3
+    return x ^ _shr(x, 31)
+
+
+def uniform_t(seed, col, level, field):
+    key = (seed << 48) ^ (FIELD_IDS[field] << 40) ^ (level << 24) ^ col
+    h = _splitmix_t(_splitmix_t(key) ^ seed)
+    return _shr(h, 11).double() * (1.0 / 9007199254740992.0)
+
+
+def normal_t(seed, col, level, field_a, field_b):
+    import torch
+    u1 = uniform_t(seed, col, level, field_a)
+    u2 = uniform_t(seed, col, level, field_b)
+    return torch.sqrt(-2.0 * torch.log1p(-u1)) * torch.cos(2.0 * np.pi * u2)
+
+
+def generate_torch(workload, ncol=None, col0=0, seed=None, device="cuda", dtype=None):
+    """generate() evaluated with torch on ``device``; returns (state, meta) with
+    (nz, ncol) tensors of ``dtype`` (default float64) and meta['nlev'] as an
+    int32 tensor for ragged workloads."""
+    import torch
+    dtype = dtype or torch.float64
+    cfg = dict(WORKLOADS[workload])
+    seed = cfg["seed"] if seed is None else seed
+    n = cfg["ncol"] if ncol is None else ncol
+    nz, dz = cfg["nz"], cfg["dz"]
+    f64 = dict(dtype=torch.float64, device=device)
+    cols = torch.arange(col0, col0 + n, dtype=torch.int64, device=device)[None, :]
+    lev = torch.arange(nz, dtype=torch.int64, device=device)[:, None]
+    col1 = cols[0]
+    zero = torch.zeros(1, dtype=torch.int64, device=device)
+
+    z = (torch.arange(nz, **f64) + 0.5) * dz
+    zc = z[:, None].expand(nz, n)
+    t = torch.clamp(300.0 - 6.5 * zc * 1e-3, min=210.0)
+    g, rd = 9.8, 287.04
+    zi = np.concatenate([[0.0], (np.arange(nz, dtype=np.float64) + 0.5) * dz])
+    ti = np.maximum(300.0 - 6.5 * zi * 1e-3, 210.0)
+    lnp = np.log(1.0e5) - np.cumsum(g * np.diff(zi) / (rd * 0.5 * (ti[1:] + ti[:-1])))
+    p = torch.tensor(np.exp(lnp), **f64)[:, None].expand(nz, n)
+
+    def layer(bf, tf):
+        base = 500.0 + 7500.0 * uniform_t(seed, col1, zero, bf)
+        thick = 2000.0 + 2000.0 * uniform_t(seed, col1, zero, tf)
+        return (zc >= base[None, :]) & (zc <= (base + thick)[None, :])
+    incloud = layer("layer1_base", "layer1_thick")
+    two = uniform_t(seed, col1, zero, "layer2_on") < 0.5
+    incloud = incloud | (layer("layer2_base", "layer2_thick") & two[None, :])
+
+    def rslf_t(pp, tt):
+        c = (.611583699e03, .444606896e02, .143177157e01, .264224321e-1, .299291081e-3,
+             .203154182e-5, .702620698e-8, .379534310e-11, -.321582393e-13)
+        x = torch.clamp(tt - 273.16, min=-80.0)
+        esl = torch.full_like(x, c[8])
+        for ci in c[7::-1]:
+            esl = ci + x * esl
+        esl = torch.minimum(esl, pp * 0.15)
+        return 0.622 * esl / (pp - esl)
+
+    rh = torch.where(incloud, 0.6 + 0.4 * uniform_t(seed, cols, lev, "rh"), torch.tensor(0.3, **f64))
+    qv = rh * rslf_t(p, t)
+
+    def lognormal(field, median, fb):
+        return median * torch.exp(normal_t(seed, cols, lev, field, fb))
+    zero_f = torch.tensor(0.0, **f64)
+    hyd = {name: torch.where(incloud, lognormal(name, 1e-4, "q1"), zero_f)
+           for name in ("qc", "qr", "qi", "qs", "qg")}
+
+    def number(q, rho_x, d):
+        return q / (np.pi / 6.0 * rho_x * d ** 3)
+    w = 0.5 * normal_t(seed, cols, lev, "w1", "w2")
+    state = dict(t=t, p=p, w=w, dz=torch.full((nz, n), dz, **f64), qv=qv, qc=hyd["qc"], qr=hyd["qr"],
+                 qi=hyd["qi"], qs=hyd["qs"], qg=hyd["qg"],
+                 ni=number(hyd["qi"], 890.0, 50e-6), nr=number(hyd["qr"], 1000.0, 1e-3))
+
+    conv_frac = cfg.get("convective", 0.0)
+    if conv_frac > 0.0:
+        conv = uniform_t(seed, col1, zero, "conv") < conv_frac
+        wmax = 5.0 + 25.0 * uniform_t(seed, col1, zero, "wmax")
+        zconv = 12000.0
+        wconv = torch.where(zc < zconv, wmax[None, :] * torch.sin(np.pi * zc / zconv), zero_f)
+        state["w"] = torch.where(conv[None, :], wconv, state["w"])
+        qr_sc = 1e-4 + (5e-3 - 1e-4) * uniform_t(seed, cols, lev, "qr_sc")
+        sc = conv[None, :] & (t < 273.15) & (zc < zconv)
+        state["qr"] = torch.where(sc, qr_sc, state["qr"])
+        state["nr"] = number(state["qr"], 1000.0, 1e-3)
+
+    if cfg["aerosol"]:
+        decay = torch.exp(-zc / 2000.0)
+        state["nc"] = number(hyd["qc"], 1000.0, 10e-6)
+        state["nwfa"] = 3e8 * torch.exp(normal_t(seed, cols, lev, "nwfa1", "nwfa2")) * decay
+        state["nifa"] = 1e6 * torch.exp(normal_t(seed, cols, lev, "nifa1", "nifa2")) * decay
+    if cfg["hail"]:
+        rhog = 300.0 + 600.0 * uniform_t(seed, cols, lev, "rhog")
+        state["qb"] = 1000.0 * state["qg"] / rhog
+        state["ng"] = number(state["qg"], rhog, 2e-3)
+
+    meta = dict(dt=cfg["dt"], itimestep=cfg["itimestep"], nz=nz, ncol=n, workload=workload,
+                aerosol=cfg["aerosol"], hail=cfg["hail"])
+    if cfg["ragged"]:
+        meta["nlev"] = (30 + torch.floor(99.0 * uniform_t(seed, col1, zero, "nlev"))).to(torch.int32)
+    state = {k: v.to(dtype).contiguous() for k, v in state.items()}
+    return state, meta

@@ -1,0 +1,46 @@
+import glob
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+def golden_cases():
+    return sorted(os.path.basename(p)[5:-4] for p in glob.glob(os.path.join(GOLDEN, "step_*.npz")))
+
+
+def load_case(name):
+    z = np.load(os.path.join(GOLDEN, f"step_{name}.npz"))
+    arrays = {k: z[k] for k in z.files}
+    meta = json.loads(str(arrays.pop("meta")))
+    inp = {k[3:]: np.ascontiguousarray(v) for k, v in arrays.items() if k.startswith("in_")}
+    out = {k[4:]: v for k, v in arrays.items() if k.startswith("out_")}
+    d2 = {k[3:]: v for k, v in arrays.items() if k.startswith("d2_")}
+    d3 = {k[3:]: v for k, v in arrays.items() if k.startswith("d3_")}
+    return dict(meta=meta, inp=inp, out=out, d2=d2, d3=d3, nlev=arrays.get("nlev"), land=arrays.get("land"),
+                arrays=arrays)
+
+
+@pytest.fixture(scope="session")
+def golden_tables():
+    """Full-size tables holding every entry any golden case reads (NaN elsewhere)."""
+    from tempo_amd import tables_io as tio
+    tabs = dict(ccn=tio.read_ccn())
+    for kind, n in (("fz", tio.N_FZ), ("qs", tio.N_QS), ("qg", tio.N_QG)):
+        tabs[kind] = np.full(n, np.nan)
+    for name in golden_cases():
+        z = np.load(os.path.join(GOLDEN, f"step_{name}.npz"))
+        for kind in ("fz", "qs", "qg"):
+            tabs[kind][z[f"tab_{kind}_idx"]] = z[f"tab_{kind}_val"]
+    return tabs

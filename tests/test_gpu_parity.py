@@ -1,0 +1,295 @@
+"""Parity of the HIP path (through the C ABI) with the reference golden
+vectors and the C oracle, on an MI355X.
+
+Tolerances (stated per test):
+  FP64 twin vs reference / oracle: relative 1e-12 per field, normalised by
+      the field's largest magnitude (|gpu - ref|_inf / |ref|_inf), and at
+      most 0.1 % of points off by more than 1e-9 relative (threshold flips);
+  FP32 production path vs FP64 twin: relative L2 1e-4 per tendency, 1e-5 on
+      surface precipitation, with the fraction of branch-flipped points
+      reported.
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import golden_cases, load_case
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+STATE_OUT = ("t", "th", "qv", "qc", "qr", "qi", "qs", "qg", "ni", "nr", "nc", "nwfa", "nifa", "ng", "qb")
+D2_MAP = dict(rain_precip="rain_precip", ice_precip="ice_liquid_equiv_precip",
+              snow_precip="snow_liquid_equiv_precip", graupel_precip="graupel_liquid_equiv_precip",
+              frozen_fraction="frozen_fraction", frz_rain_precip="frz_rain_precip",
+              max_hail_sfc="max_hail_diameter_sfc", max_hail_column="max_hail_diameter_column")
+D3_MAP = dict(refl10cm="refl10cm", re_cloud="re_cloud", re_ice="re_ice", re_snow="re_snow",
+              mvd_r="rain_med_vol_diam", mvd_g="graupel_med_vol_diam", nc_mixing_ratio="cloud_number_mixing_ratio")
+FLAG_TO_CFG = dict(aerosolaware="aerosolaware_flag", hailaware="hailaware_flag", turn_off_micro="turn_off_micro_flag",
+                   cloud_condensation="cloud_condensation_flag", refl10cm="refl10cm_flag", re_cloud="re_cloud_flag",
+                   re_ice="re_ice_flag", re_snow="re_snow_flag", max_hail="max_hail_diameter_flag",
+                   mvd_r="rain_med_vol_diam_flag", mvd_g="graupel_med_vol_diam_flag",
+                   nc_mixing_ratio="cloud_number_mixing_ratio_flag")
+
+
+def cfg_of(flags):
+    cfg = {}
+    for k, v in flags.items():
+        if k == "nc_opt":
+            cfg["single_moment_nc_opt"] = "ocean" if v == 1 else "land"
+        elif k in FLAG_TO_CFG:
+            cfg[FLAG_TO_CFG[k]] = v
+    return cfg
+
+
+@pytest.fixture(scope="module")
+def hip():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from tempo_amd import host
+    host.lib()  # fails loudly if the library is missing
+    return host
+
+
+_ctx_cache = {}
+
+
+def context(hip, flags):
+    key = tuple(sorted(cfg_of(flags).items()))
+    if key not in _ctx_cache:
+        _ctx_cache[key] = hip.TempoHip(**cfg_of(flags))
+    return _ctx_cache[key]
+
+
+@pytest.fixture(scope="module")
+def gpu_tables(hip):
+    ctx = context(hip, dict(aerosolaware=True, hailaware=True))
+    return dict(ccn=__import__("tempo_amd.tables_io", fromlist=["x"]).read_ccn(),
+                fz=ctx.export_table(hip.TABLE_FZ), qs=ctx.export_table(hip.TABLE_QS),
+                qg=ctx.export_table(hip.TABLE_QG), build_ms=ctx.table_build_ms)
+
+
+def to_dev(state, dtype):
+    return {k: torch.tensor(v, dtype=dtype, device="cuda") for k, v in state.items()}
+
+
+def gpu_run(ctx, case_inp, dt, itimestep, steps=1, dtype=torch.float64, nlev=None, land=None):
+    st = to_dev(case_inp, dtype)
+    nl = None if nlev is None else torch.tensor(nlev, dtype=torch.int32, device="cuda")
+    ld = None if land is None else torch.tensor(land, dtype=torch.int32, device="cuda")
+    for s in range(steps):
+        d = ctx.run(st, dt, itimestep + s, nlev=nl, land=ld)
+    torch.cuda.synchronize()
+    return {k: v.double().cpu().numpy() for k, v in st.items()}, {k: v.double().cpu().numpy() for k, v in d.items()}
+
+
+def rel_inf(a, b):
+    scale = np.abs(b).max()
+    if scale == 0:
+        return float(np.abs(a).max())
+    return float(np.abs(a - b).max() / scale)
+
+
+def flip_fraction(a, b, tol=1e-9):
+    rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+    return float(np.mean(rel > tol))
+
+
+def test_gpu_tables_match_reference_builder(hip, gpu_tables):
+    """FP64 GPU table build vs entries of the tables produced by the
+    reference's own builder: identical summation order, so the entries match
+    to the last bit except where the host libm rounds a factor differently."""
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "tables_sample.npz"))
+    from tempo_amd import tables_io as tio
+    sizes = dict(qg=tio.N_QG1, qs=tio.N_QS1, fzr=tio.N_FZ_R, fzc=tio.N_FZ_C)
+    src = dict(qg=gpu_tables["qg"], qs=gpu_tables["qs"], fzr=gpu_tables["fz"][:4 * tio.N_FZ_R],
+               fzc=gpu_tables["fz"][4 * tio.N_FZ_R:])
+    report = {}
+    for kind in ("qg", "qs", "fzr", "fzc"):
+        idx, val = z[f"{kind}_idx"], z[f"{kind}_val"]
+        n1 = sizes[kind]
+        got = np.stack([src[kind][a * n1 + idx] for a in range(val.shape[1])], axis=1)
+        rel = np.abs(got - val) / np.maximum(np.abs(val), 1e-300)
+        report[kind] = (float(rel.max()), float(np.mean(got == val)))
+        assert rel.max() <= 1e-13, (kind, rel.max())
+    print("table parity (max rel, exact fraction):", report, "build ms", gpu_tables["build_ms"])
+
+
+def test_gpu_tables_match_oracle_entries(hip, gpu_tables):
+    from oracle import oracle as orc
+    from tempo_amd import tables_io as tio
+    orc.init(gpu_tables, aerosolaware=False, hailaware=False)
+    rng = np.random.default_rng(5)
+    qg = tio.view(gpu_tables, "qg")
+    for _ in range(8):
+        i, j, n3, k, m = (int(x) for x in rng.integers(1, [38, 38, 10, 38, 38]))
+        ref = orc.qg_entry(i, j, n3, k, m)
+        got = np.array([qg[n][i - 1, j - 1, n3 - 1, k - 1, m - 1] for n in tio.QG_NAMES])
+        np.testing.assert_allclose(got, ref, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_gpu_fp64_matches_reference_golden(hip, name):
+    """FP64 twin vs the reference FP64 build on the golden columns."""
+    case = load_case(name)
+    meta = case["meta"]
+    ctx = context(hip, meta["flags"])
+    st, d = gpu_run(ctx, case["inp"], meta["dt"], meta["itimestep"], meta["steps"], nlev=case["nlev"],
+                    land=case["land"])
+    has_q = np.zeros(case["inp"]["p"].shape[1], bool)
+    for q in ("qc", "qr", "qi", "qs", "qg"):
+        has_q |= (case["inp"][q] > 1e-12).any(axis=0)
+    worst = {}
+    for k, v in case["out"].items():
+        worst[k] = rel_inf(st[k], v)
+        assert worst[k] <= 1e-12, (name, k, worst[k])
+        assert flip_fraction(st[k], v) <= 1e-3, (name, k)
+    for k, v in case["d2"].items():
+        g = d[D2_MAP[k]]
+        if k == "frozen_fraction":  # undefined in the reference after an early exit
+            g, v = g[has_q], v[has_q]
+        worst[k] = rel_inf(g, v)
+        assert worst[k] <= 1e-12, (name, k, worst[k])
+    for k, v in case["d3"].items():
+        worst[k] = rel_inf(d[D3_MAP[k]], v)
+        assert worst[k] <= 1e-12, (name, k, worst[k])
+    print(name, "max rel", max(worst.values()))
+
+
+@pytest.fixture(scope="module")
+def big_case(hip, gpu_tables):
+    """aerosol- and hail-aware columns at a size the oracle finishes in seconds"""
+    from oracle import oracle as orc
+    from tempo_amd import generator as gen
+    st, m = gen.generate("multistep", ncol=2048, col0=50_000)
+    orc.init(gpu_tables, aerosolaware=True, hailaware=True)
+    ref = {k: v.copy() for k, v in st.items()}
+    d2, d3 = orc.run(ref, m["dt"], m["itimestep"])
+    return st, m, ref, d2, d3
+
+
+def test_gpu_fp64_matches_oracle(hip, big_case):
+    st, m, ref, d2, d3 = big_case
+    ctx = context(hip, dict(aerosolaware=True, hailaware=True))
+    out, d = gpu_run(ctx, st, m["dt"], m["itimestep"])
+    for k in ref:
+        assert rel_inf(out[k], ref[k]) <= 1e-12, (k, rel_inf(out[k], ref[k]))
+        assert flip_fraction(out[k], ref[k]) <= 1e-3, k
+    for k, v in d2.items():
+        assert rel_inf(d[D2_MAP[k]], v) <= 1e-12, k
+    for k, v in d3.items():
+        assert rel_inf(d[D3_MAP[k]], v) <= 1e-12, k
+
+
+def test_gpu_fp32_within_tolerance(hip, big_case):
+    """FP32 production path vs FP64: relative L2 of each tendency <= 1e-4,
+    surface precipitation <= 1e-5; fraction of points whose result moved by
+    more than 1e-3 relative (threshold branch flips) reported."""
+    st, m, ref, d2, d3 = big_case
+    ctx = context(hip, dict(aerosolaware=True, hailaware=True))
+    o64, d64 = gpu_run(ctx, st, m["dt"], m["itimestep"], dtype=torch.float64)
+    o32, d32 = gpu_run(ctx, st, m["dt"], m["itimestep"], dtype=torch.float32)
+    dt = m["dt"]
+    report = {}
+    for k in ("t", "qv", "qc", "qr", "qi", "qs", "qg", "ni", "nr", "nc", "ng", "qb", "nwfa", "nifa"):
+        ten64 = (o64[k] - st[k]) / dt
+        ten32 = (o32[k] - st[k].astype(np.float32).astype(np.float64)) / dt
+        nrm = np.linalg.norm(ten64)
+        rel = np.linalg.norm(ten32 - ten64) / nrm if nrm > 0 else 0.0
+        flips = flip_fraction(o32[k], o64[k], 1e-3)
+        report[k] = (rel, flips)
+    precip = d64["rain_precip"] + d64["snow_liquid_equiv_precip"] + d64["graupel_liquid_equiv_precip"] + \
+        d64["ice_liquid_equiv_precip"]
+    precip32 = d32["rain_precip"] + d32["snow_liquid_equiv_precip"] + d32["graupel_liquid_equiv_precip"] + \
+        d32["ice_liquid_equiv_precip"]
+    prel = np.linalg.norm(precip32 - precip) / np.linalg.norm(precip)
+    print("FP32 vs FP64 (rel L2 tendency, branch-flip fraction):", report, "precip rel L2", prel)
+    for k, (rel, _) in report.items():
+        assert rel <= 1e-4, (k, rel)
+    assert prel <= 1e-5, prel
+
+
+def test_gpu_ragged_matches_oracle(hip, gpu_tables):
+    from oracle import oracle as orc
+    from tempo_amd import generator as gen
+    st, m = gen.generate("ragged", ncol=512, col0=1000)
+    orc.init(gpu_tables, aerosolaware=False, hailaware=True)
+    ref = {k: v.copy() for k, v in st.items()}
+    orc.run(ref, m["dt"], m["itimestep"], nlev=m["nlev"])
+    ctx = context(hip, dict(aerosolaware=False, hailaware=True))
+    out, _ = gpu_run(ctx, st, m["dt"], m["itimestep"], nlev=m["nlev"])
+    for k in ref:
+        assert rel_inf(out[k], ref[k]) <= 1e-12, k
+    # levels above each column's top are untouched
+    for k in ("qr", "t"):
+        for c in range(0, 512, 37):
+            n = m["nlev"][c]
+            np.testing.assert_array_equal(out[k][n:, c], st[k][n:, c])
+
+
+def test_gpu_host_path_equals_device_path(hip):
+    from tempo_amd import generator as gen
+    st, m = gen.generate("aerosol", ncol=300, col0=77)
+    ctx = context(hip, dict(aerosolaware=True, hailaware=False))
+    dev, dd = gpu_run(ctx, st, m["dt"], m["itimestep"])
+    hst = {k: v.copy() for k, v in st.items()}
+    hd = ctx.run_host(hst, m["dt"], m["itimestep"])
+    for k in st:
+        np.testing.assert_array_equal(hst[k], dev[k], err_msg=k)
+    np.testing.assert_array_equal(hd["refl10cm"], dd["refl10cm"])
+
+
+def test_gpu_single_column_and_tiny_grid(hip):
+    from tempo_amd import generator as gen
+    from oracle import oracle as orc
+    st, m = gen.generate("plain", ncol=1, col0=3)
+    ctx = context(hip, dict(aerosolaware=False, hailaware=False))
+    out, _ = gpu_run(ctx, st, m["dt"], m["itimestep"])
+    small = {k: np.ascontiguousarray(v[:2]) for k, v in st.items()}
+    out2, _ = gpu_run(ctx, small, m["dt"], m["itimestep"])
+    assert np.isfinite(out["t"]).all() and np.isfinite(out2["t"]).all()
+
+
+def test_gpu_error_codes(hip):
+    from tempo_amd import generator as gen
+    ctx = context(hip, dict(aerosolaware=False, hailaware=False))
+    st, m = gen.generate("plain", ncol=4)
+    dev = to_dev(st, torch.float64)
+    bad = dict(dev)
+    del bad["qv"]
+    with pytest.raises(hip.TempoError) as e:
+        ctx.run(bad, m["dt"], 2)
+    assert e.value.code == hip.E_ARG
+    with pytest.raises(hip.TempoError) as e:
+        ctx.run(dev, -1.0, 2)
+    assert e.value.code == hip.E_ARG
+    with pytest.raises(hip.TempoError) as e:
+        hip.TempoHip(semi_sedi_flag=1)
+    assert e.value.code == hip.E_UNSUPPORTED
+    with tempfile.TemporaryDirectory() as d:
+        with pytest.raises(hip.TempoError) as e:
+            hip.TempoHip(table_dir=d)
+        assert e.value.code == hip.E_TABLE
+
+
+def test_gpu_table_file_roundtrip(hip):
+    """GPU-built tables written in the reference file format and read back
+    give the identical step."""
+    from tempo_amd import generator as gen
+    from tempo_amd.host import CCN_PATH
+    built = context(hip, dict(aerosolaware=True, hailaware=True))
+    st, m = gen.generate("multistep", ncol=256, col0=9)
+    a, _ = gpu_run(built, st, m["dt"], m["itimestep"])
+    with tempfile.TemporaryDirectory() as d:
+        built.write_tables(d)
+        sizes = {f: os.path.getsize(os.path.join(d, f)) for f in os.listdir(d)}
+        assert sizes == {"freeze_water_data_tempo_v3": 254944800, "qr_acr_qs_data_tempo_v3": 43764192,
+                         "qr_acr_qg_data_tempo_v3": 674697960}
+        read = hip.TempoHip(table_dir=d, ccn_path=CCN_PATH, aerosolaware_flag=1, hailaware_flag=1)
+        b, _ = gpu_run(read, st, m["dt"], m["itimestep"])
+        read.close()
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
