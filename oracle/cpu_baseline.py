@@ -41,7 +41,7 @@ _TABLES = None
 
 def _worker(args):
     workload, col0, ncol, barrier, q = args
-    ref.select(optimised=True)
+    ref.select("o2")
     ref.init(_TABLES, **FLAGS[workload])
     st, m = gen.generate(workload, ncol=ncol, col0=col0)
     if "nlev" in m:
@@ -62,7 +62,7 @@ def main(argv=None):
     ap.add_argument("--workers", type=int, default=0)
     ap.add_argument("--col0", type=int, default=0)
     a = ap.parse_args(argv)
-    if not ref.available(optimised=True):
+    if not ref.available("o2"):
         raise SystemExit("reference -O2 build missing: make -C oracle ref")
     workers = a.workers or min(16, len(os.sched_getaffinity(0)))
     _TABLES = tio.read_dir(a.tables)

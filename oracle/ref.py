@@ -14,8 +14,10 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-REF_SO = os.path.join(HERE, "_ref", "libtempo_ref.so")          # -O0: parity oracle
-REF_SO_O2 = os.path.join(HERE, "_ref", "libtempo_ref_o2.so")    # -O2: CPU timing baseline
+REF_SO = os.path.join(HERE, "_ref", "libtempo_ref.so")          # FP64 -O0: parity oracle
+REF_SO_O2 = os.path.join(HERE, "_ref", "libtempo_ref_o2.so")    # FP64 -O2: CPU timing baseline
+REF_SO_SP = os.path.join(HERE, "_ref", "libtempo_ref_sp.so")    # -DSINGLE_PREC -O2: the reference FP32 path
+VARIANTS = {"o0": REF_SO, "o2": REF_SO_O2, "sp": REF_SO_SP}
 
 # order of the state pointer array in ref_run (oracle/ref_bridge.F90)
 STATE_FIELDS = ("t", "p", "w", "dz", "qv", "qc", "qr", "qi", "qs", "qg", "ni", "nr",
@@ -32,44 +34,46 @@ FLAG_ORDER = ("aerosolaware", "hailaware", "semi_sedi", "refl_from_melting", "tu
               "cloud_condensation", "refl10cm", "re_cloud", "re_ice", "re_snow", "max_hail",
               "mvd_r", "mvd_g", "nc_mixing_ratio", "ml_for_bl_nc", "ml_for_nc", "nc_opt")
 
-_lib = None
-_so = REF_SO
+_libs = {}
+_variant = "o0"
 
 
-def select(optimised):
-    """Choose the -O2 build (timing) or the -O0 build (parity, the default)
-    before the first call."""
-    global _so
-    if _lib is not None:
-        raise RuntimeError("reference library already loaded")
-    _so = REF_SO_O2 if optimised else REF_SO
+def select(variant):
+    """Default build for the calls below: 'o0' (FP64 parity oracle, the
+    default), 'o2' (FP64 timing baseline) or 'sp' (single precision).  Each
+    build is a separate library with its own module state."""
+    global _variant
+    if variant not in VARIANTS:
+        raise ValueError(variant)
+    _variant = variant
 
 
-def available(optimised=False):
-    return os.path.exists(REF_SO_O2 if optimised else REF_SO)
+def available(variant="o0"):
+    return os.path.exists(VARIANTS[variant])
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_so):
+def lib(variant=None):
+    v = variant or _variant
+    if v not in _libs:
+        so = VARIANTS[v]
+        if not os.path.exists(so):
             raise RuntimeError("reference build missing: run `make -C oracle ref`")
-        _lib = ctypes.CDLL(_so)
-        _lib.ref_init.restype = ctypes.c_int
-        _lib.ref_init.argtypes = [ctypes.POINTER(ctypes.c_int)] + [ctypes.c_void_p] * 4
-        _lib.ref_run.restype = ctypes.c_int
-        _lib.ref_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
-                                 ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
-                                 ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]
+        L = ctypes.CDLL(so)
+        L.ref_init.restype = ctypes.c_int
+        L.ref_init.argtypes = [ctypes.POINTER(ctypes.c_int)] + [ctypes.c_void_p] * 4
+        L.ref_run.restype = ctypes.c_int
+        L.ref_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                              ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
+                              ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]
         for fn in ("ref_gamma_p", "ref_rslf", "ref_rsif"):
-            getattr(_lib, fn).restype = ctypes.c_double
-            getattr(_lib, fn).argtypes = [ctypes.c_double, ctypes.c_double]
-        _lib.ref_snow_moments.restype = None
-        _lib.ref_snow_moments.argtypes = [ctypes.c_double, ctypes.c_double,
-                                          ctypes.POINTER(ctypes.c_double)]
-        _lib.ref_get_array.restype = ctypes.c_int
-        _lib.ref_get_array.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
-    return _lib
+            getattr(L, fn).restype = ctypes.c_double
+            getattr(L, fn).argtypes = [ctypes.c_double, ctypes.c_double]
+        L.ref_snow_moments.restype = None
+        L.ref_snow_moments.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+        L.ref_get_array.restype = ctypes.c_int
+        L.ref_get_array.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        _libs[v] = L
+    return _libs[v]
 
 
 def _ptr(a):
@@ -79,7 +83,7 @@ def _ptr(a):
 _keep = []
 
 
-def init(tables, **flags):
+def init(tables, variant=None, **flags):
     """Initialise the reference (tempo_init sequence) with in-memory tables.
 
     ``tables`` holds the flat arrays in reference file order: 'ccn' float32
@@ -92,13 +96,13 @@ def init(tables, **flags):
     fz = np.ascontiguousarray(tables["fz"], dtype=np.float64)
     qs = np.ascontiguousarray(tables["qs"], dtype=np.float64)
     qg = np.ascontiguousarray(tables["qg"], dtype=np.float64)
-    rc = lib().ref_init(arr, _ptr(ccn), _ptr(fz), _ptr(qs), _ptr(qg))
+    rc = lib(variant).ref_init(arr, _ptr(ccn), _ptr(fz), _ptr(qs), _ptr(qg))
     if rc != 0:
         raise RuntimeError("ref_init failed")
     return f
 
 
-def run(state, dt, itimestep, land=None):
+def run(state, dt, itimestep, land=None, variant=None):
     """One tempo_run step over all columns; state arrays are updated in place.
 
     Returns (diag2, diag3) dicts of the driver diagnostics.
@@ -122,7 +126,7 @@ def run(state, dt, itimestep, land=None):
     if land is not None:
         land = np.ascontiguousarray(land, dtype=np.int32)
         landp = land.ctypes.data
-    rc = lib().ref_run(ncol, nz, float(dt), int(itimestep), ptrs, landp, p2, p3)
+    rc = lib(variant).ref_run(ncol, nz, float(dt), int(itimestep), ptrs, landp, p2, p3)
     if rc != 0:
         raise RuntimeError("ref_run failed")
     return d2, d3

@@ -8,6 +8,10 @@
 ! from caller memory instead of files, in the reference file order
 ! (src/module_mp_tempo_driver.F90:532-649).
 !
+! State and diagnostics are exchanged in FP64 and converted to the build's
+! working precision wp at the boundary, so the same bridge serves the FP64
+! build and the -DSINGLE_PREC build (wp = 4).
+!
 ! ref_run calls tempo_run (src/module_mp_tempo_driver.F90:187-513) on ncol
 ! columns laid out (ncol, nz) column-major, i.e. element (i,k) at
 ! i + k*ncol -- one column per contiguous i, levels strided by ncol.
@@ -127,7 +131,7 @@ contains
 
   subroutine copy_in(src, dst, n1, n2)
     type(c_ptr), intent(in) :: src
-    real(c_double), allocatable, intent(inout) :: dst(:,:,:)
+    real(wp), allocatable, intent(inout) :: dst(:,:,:)
     integer, intent(in) :: n1, n2
     real(c_double), pointer :: p(:,:,:)
     if (c_associated(src)) then
@@ -139,7 +143,7 @@ contains
 
   subroutine copy_out(dst, src, n1, n2)
     type(c_ptr), intent(in) :: dst
-    real(c_double), allocatable, intent(in) :: src(:,:,:)
+    real(wp), allocatable, intent(in) :: src(:,:,:)
     integer, intent(in) :: n1, n2
     real(c_double), pointer :: p(:,:,:)
     if (c_associated(dst) .and. allocated(src)) then
@@ -150,7 +154,7 @@ contains
 
   subroutine copy_out2(dst, src, n1)
     type(c_ptr), intent(in) :: dst
-    real(c_double), allocatable, intent(in) :: src(:,:)
+    real(wp), allocatable, intent(in) :: src(:,:)
     integer, intent(in) :: n1
     real(c_double), pointer :: p(:,:)
     if (c_associated(dst) .and. allocated(src)) then
@@ -170,7 +174,7 @@ contains
     real(c_double), value :: dt
     type(c_ptr), intent(in) :: state(19), diag2(8), diag3(7)
     type(c_ptr), value :: land
-    real(c_double), allocatable, dimension(:,:,:) :: t, p, w, dz, qv, qc, qr, qi, qs, qg, &
+    real(wp), allocatable, dimension(:,:,:) :: t, p, w, dz, qv, qc, qr, qi, qs, qg, &
       ni, nr, nc, nwfa, nifa, ng, qb, th, pii
     integer(c_int), pointer :: pland(:,:)
     integer, allocatable :: land_input(:,:)

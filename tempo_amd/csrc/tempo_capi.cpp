@@ -13,8 +13,6 @@
 
 namespace tempo {
 // defined in tempo_step_f32.hip / tempo_step_f64.hip / tempo_tables.hip
-hipError_t upload_const(const Const<float>& c, hipStream_t s);
-hipError_t upload_const(const Const<double>& c, hipStream_t s);
 hipError_t launch_step(const StepArgs<float>& a, double* vtbuf, hipStream_t s, hipEvent_t* ev);
 hipError_t launch_step(const StepArgs<double>& a, double* vtbuf, hipStream_t s, hipEvent_t* ev);
 hipError_t launch_table_qg(const double*, const double*, const double*, const double*, double*, hipStream_t);
@@ -117,6 +115,7 @@ struct tempo_hip_ctx {
   int device = 0;
   Const<double> cd;
   Const<float> cf;
+  DevBuf cd_dev, cf_dev;  // the constants in device memory, one copy per context
   // reference-order FP64 tables
   DevBuf qg_ref, qs_ref, fz_ref;
   // gather layouts
@@ -355,6 +354,7 @@ int run_impl(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags*
   a.tab.fzr = static_cast<const R*>(r64 ? x->fzr64.p : x->fzr32.p);
   a.tab.fzc = static_cast<const R*>(r64 ? x->fzc64.p : x->fzc32.p);
   a.tab.ccn = x->ccn.as<float>();
+  a.cst = static_cast<const Const<R>*>(r64 ? x->cd_dev.p : x->cf_dev.p);
   a.tab.efrw = static_cast<const R*>(r64 ? x->efrw64.p : x->efrw32.p);
   a.tab.efsw = static_cast<const R*>(r64 ? x->efsw64.p : x->efsw32.p);
   a.tab.tnc_wev = static_cast<const R*>(r64 ? x->wev64.p : x->wev32.p);
@@ -414,8 +414,12 @@ int tempo_hip_init(const tempo_hip_cfg* cfg, const char* table_dir, const char* 
   HostBins hb;
   build_const(x->cd, hb, cfg->hailaware_flag != 0);
   narrow(x->cd, x->cf);
-  if (upload_const(x->cd, nullptr) != hipSuccess || upload_const(x->cf, nullptr) != hipSuccess)
-    return bail(fail(TEMPO_HIP_E_HIP, "constant upload failed"));
+  {
+    std::vector<Const<double>> vd(1, x->cd);
+    std::vector<Const<float>> vf(1, x->cf);
+    if (upload(x->cd_dev, vd) != hipSuccess || upload(x->cf_dev, vf) != hipSuccess)
+      return bail(fail(TEMPO_HIP_E_HIP, "constant upload failed"));
+  }
   if ((rc = upload_small_tables(x))) return bail(rc);
   std::string ccn = ccn_path ? ccn_path : (table_dir ? std::string(table_dir) + "/ccn_activate.bin" : "");
   if (ccn.empty()) return bail(fail(TEMPO_HIP_E_ARG, "no CCN table path"));

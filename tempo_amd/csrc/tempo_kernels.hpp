@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(256) k_phase_a(StepArgs<R> a) {
   const int k = blockIdx.y;
   if (c >= a.ncol) return;
   if (a.nlev && k >= a.nlev[c]) return;
-  const Const<R>& C = cst<R>();
+  const Const<R>& C = *a.cst;
   const size_t o = TEMPO_IDX_STATE(a, c, k);
   const size_t s = TEMPO_IDX_SCR(a, c, k);
   const size_t plane = size_t(a.nz) * a.ncol;
@@ -512,7 +512,7 @@ __global__ void __launch_bounds__(256) k_phase_c(StepArgs<R> a) {
   const int k = blockIdx.y;
   if (c >= a.ncol) return;
   if (a.nlev && k >= a.nlev[c]) return;
-  const Const<R>& C = cst<R>();
+  const Const<R>& C = *a.cst;
   const size_t o = TEMPO_IDX_STATE(a, c, k);
   const size_t s = TEMPO_IDX_SCR(a, c, k);
   const size_t plane = size_t(a.nz) * a.ncol;

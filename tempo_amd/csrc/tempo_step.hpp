@@ -25,17 +25,20 @@
 #include <hip/hip_runtime.h>
 
 #include "tempo_const.hpp"
+#include "tempo_crmath.hpp"
 
 namespace tempo {
 
 // ---------------------------------------------------------------- math ----
-__device__ __forceinline__ double mpow(double x, double y) { return pow(x, y); }
+// FP64: double-double evaluations rounded once (tempo_crmath.hpp), so the
+// twin rounds as the reference's host libm does; FP32: the device libm.
+__device__ __forceinline__ double mpow(double x, double y) { return crm::pow_cr(x, y); }
 __device__ __forceinline__ float mpow(float x, float y) { return powf(x, y); }
-__device__ __forceinline__ double mexp(double x) { return exp(x); }
+__device__ __forceinline__ double mexp(double x) { return crm::exp_cr(x); }
 __device__ __forceinline__ float mexp(float x) { return expf(x); }
-__device__ __forceinline__ double mlog(double x) { return log(x); }
+__device__ __forceinline__ double mlog(double x) { return crm::log_cr(x); }
 __device__ __forceinline__ float mlog(float x) { return logf(x); }
-__device__ __forceinline__ double mlog10(double x) { return log10(x); }
+__device__ __forceinline__ double mlog10(double x) { return crm::log10_cr(x); }
 __device__ __forceinline__ float mlog10(float x) { return log10f(x); }
 __device__ __forceinline__ double msqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float msqrt(float x) { return sqrtf(x); }
@@ -120,14 +123,12 @@ struct StepArgs {
   unsigned* cf;      // ncol column flags
   double* precip;    // 5 x ncol: rain, graupel, snow, ice, cloud
   int* ksub;         // 10 x ncol: substeps and ktop per species (diagnostic)
+  const Const<R>* cst;  // derived constants of this context (wave-uniform reads)
   Tables<R> tab;
   RunFlags fl;
   double dt;
   int first_step;
 };
-
-template <class R>
-__device__ __forceinline__ const Const<R>& cst();
 
 #define TEMPO_IDX_STATE(a, c, k) (size_t((c) % (a).ni) + size_t(k) * (a).ldk + size_t((c) / (a).ni) * (a).ldj)
 #define TEMPO_IDX_SCR(a, c, k) (size_t(k) * (a).ncol + size_t(c))

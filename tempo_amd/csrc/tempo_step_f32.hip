@@ -4,17 +4,6 @@
 
 namespace tempo {
 
-__constant__ Const<float> g_const_f32;
-
-template <>
-__device__ __forceinline__ const Const<float>& cst<float>() {
-  return g_const_f32;
-}
-
-hipError_t upload_const(const Const<float>& c, hipStream_t s) {
-  return hipMemcpyToSymbolAsync(HIP_SYMBOL(g_const_f32), &c, sizeof(c), 0, hipMemcpyHostToDevice, s);
-}
-
 // ev (optional): 5 events recorded around the 4 launches for per-phase timing
 hipError_t launch_step(const StepArgs<float>& a, double* vtbuf, hipStream_t s, hipEvent_t* ev) {
   const unsigned gc = unsigned((a.ncol + 255) / 256);

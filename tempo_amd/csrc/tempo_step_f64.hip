@@ -4,17 +4,6 @@
 
 namespace tempo {
 
-__constant__ Const<double> g_const_f64;
-
-template <>
-__device__ __forceinline__ const Const<double>& cst<double>() {
-  return g_const_f64;
-}
-
-hipError_t upload_const(const Const<double>& c, hipStream_t s) {
-  return hipMemcpyToSymbolAsync(HIP_SYMBOL(g_const_f64), &c, sizeof(c), 0, hipMemcpyHostToDevice, s);
-}
-
 // ev (optional): 5 events recorded around the 4 launches for per-phase timing
 hipError_t launch_step(const StepArgs<double>& a, double* vtbuf, hipStream_t s, hipEvent_t* ev) {
   const unsigned gc = unsigned((a.ncol + 255) / 256);

@@ -3,11 +3,15 @@ vectors and the C oracle, on an MI355X.
 
 Tolerances (stated per test):
   FP64 twin vs reference / oracle: relative 1e-12 per field, normalised by
-      the field's largest magnitude (|gpu - ref|_inf / |ref|_inf), and at
-      most 0.1 % of points off by more than 1e-9 relative (threshold flips);
-  FP32 production path vs FP64 twin: relative L2 1e-4 per tendency, 1e-5 on
-      surface precipitation, with the fraction of branch-flipped points
-      reported.
+      the field's largest magnitude (|gpu - ref|_inf / |ref|_inf).
+  FP32 production path vs the FP64 twin, both run on the same
+      FP32-representable inputs: relative L2 of every tendency <= 1e-4 over
+      the points whose branch decisions agree, surface precipitation <= 1e-5
+      over all columns.  A point counts as branch-flipped when its result
+      moved by more than 1e-3 of its value (plus 1e-6 of the field's largest
+      magnitude); the flipped fraction is reported and bounded, and compared
+      with the reference's own single-precision build (-DSINGLE_PREC) on the
+      same inputs when that build is present.
 """
 import os
 import tempfile
@@ -93,9 +97,13 @@ def rel_inf(a, b):
     return float(np.abs(a - b).max() / scale)
 
 
-def flip_fraction(a, b, tol=1e-9):
-    rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
-    return float(np.mean(rel > tol))
+def flipped(a, b, tol=1e-3, floor=1e-6):
+    """points whose result moved by more than rounding can explain"""
+    return np.abs(a - b) > tol * np.abs(b) + floor * np.abs(b).max()
+
+
+def flip_fraction(a, b, tol=1e-3):
+    return float(np.mean(flipped(a, b, tol)))
 
 
 def test_gpu_tables_match_reference_builder(hip, gpu_tables):
@@ -146,7 +154,6 @@ def test_gpu_fp64_matches_reference_golden(hip, name):
     for k, v in case["out"].items():
         worst[k] = rel_inf(st[k], v)
         assert worst[k] <= 1e-12, (name, k, worst[k])
-        assert flip_fraction(st[k], v) <= 1e-3, (name, k)
     for k, v in case["d2"].items():
         g = d[D2_MAP[k]]
         if k == "frozen_fraction":  # undefined in the reference after an early exit
@@ -177,38 +184,59 @@ def test_gpu_fp64_matches_oracle(hip, big_case):
     out, d = gpu_run(ctx, st, m["dt"], m["itimestep"])
     for k in ref:
         assert rel_inf(out[k], ref[k]) <= 1e-12, (k, rel_inf(out[k], ref[k]))
-        assert flip_fraction(out[k], ref[k]) <= 1e-3, k
     for k, v in d2.items():
         assert rel_inf(d[D2_MAP[k]], v) <= 1e-12, k
     for k, v in d3.items():
         assert rel_inf(d[D3_MAP[k]], v) <= 1e-12, k
 
 
-def test_gpu_fp32_within_tolerance(hip, big_case):
-    """FP32 production path vs FP64: relative L2 of each tendency <= 1e-4,
-    surface precipitation <= 1e-5; fraction of points whose result moved by
-    more than 1e-3 relative (threshold branch flips) reported."""
-    st, m, ref, d2, d3 = big_case
+TENDENCY_FIELDS = ("t", "qv", "qc", "qr", "qi", "qs", "qg", "ni", "nr", "nc", "ng", "qb", "nwfa", "nifa")
+
+
+def fp32_errors(inp, out, d, o64, d64, dt):
+    """(rel L2 of each tendency over unflipped points, flipped fraction) and
+    the relative L2 of total surface precipitation."""
+    rep = {}
+    for k in TENDENCY_FIELDS:
+        if k not in inp:
+            continue
+        keep = ~flipped(out[k], o64[k])
+        t64 = ((o64[k] - inp[k]) / dt)[keep]
+        t32 = ((out[k] - inp[k]) / dt)[keep]
+        nrm = np.linalg.norm(t64)
+        rep[k] = (float(np.linalg.norm(t32 - t64) / nrm) if nrm > 0 else 0.0, float(1.0 - keep.mean()))
+    p64 = sum(d64[n] for n in ("rain_precip", "ice_precip", "snow_precip", "graupel_precip"))
+    p32 = sum(d[n] for n in ("rain_precip", "ice_precip", "snow_precip", "graupel_precip"))
+    return rep, float(np.linalg.norm(p32 - p64) / np.linalg.norm(p64))
+
+
+def test_gpu_fp32_within_tolerance(hip, big_case, gpu_tables):
+    """FP32 production path vs the FP64 twin on the same FP32-representable
+    inputs (aerosol- and hail-aware columns)."""
+    st, m, _, _, _ = big_case
+    inp = {k: v.astype(np.float32).astype(np.float64) for k, v in st.items()}
     ctx = context(hip, dict(aerosolaware=True, hailaware=True))
-    o64, d64 = gpu_run(ctx, st, m["dt"], m["itimestep"], dtype=torch.float64)
-    o32, d32 = gpu_run(ctx, st, m["dt"], m["itimestep"], dtype=torch.float32)
-    dt = m["dt"]
-    report = {}
-    for k in ("t", "qv", "qc", "qr", "qi", "qs", "qg", "ni", "nr", "nc", "ng", "qb", "nwfa", "nifa"):
-        ten64 = (o64[k] - st[k]) / dt
-        ten32 = (o32[k] - st[k].astype(np.float32).astype(np.float64)) / dt
-        nrm = np.linalg.norm(ten64)
-        rel = np.linalg.norm(ten32 - ten64) / nrm if nrm > 0 else 0.0
-        flips = flip_fraction(o32[k], o64[k], 1e-3)
-        report[k] = (rel, flips)
-    precip = d64["rain_precip"] + d64["snow_liquid_equiv_precip"] + d64["graupel_liquid_equiv_precip"] + \
-        d64["ice_liquid_equiv_precip"]
-    precip32 = d32["rain_precip"] + d32["snow_liquid_equiv_precip"] + d32["graupel_liquid_equiv_precip"] + \
-        d32["ice_liquid_equiv_precip"]
-    prel = np.linalg.norm(precip32 - precip) / np.linalg.norm(precip)
-    print("FP32 vs FP64 (rel L2 tendency, branch-flip fraction):", report, "precip rel L2", prel)
-    for k, (rel, _) in report.items():
+    o64, d64 = gpu_run(ctx, inp, m["dt"], m["itimestep"], dtype=torch.float64)
+    o32, d32 = gpu_run(ctx, inp, m["dt"], m["itimestep"], dtype=torch.float32)
+    inv = {v: k for k, v in D2_MAP.items()}
+    d64 = {inv[k]: v for k, v in d64.items() if k in inv}
+    d32 = {inv[k]: v for k, v in d32.items() if k in inv}
+    rep, prel = fp32_errors(inp, o32, d32, o64, d64, m["dt"])
+    print("FP32 vs FP64 (rel L2 tendency over unflipped points, flipped fraction):", rep,
+          "precip rel L2", prel)
+    from oracle import ref
+    if ref.available("sp"):
+        # the reference's own single-precision build against the FP64 twin
+        ref.init(gpu_tables, variant="sp", aerosolaware=True, hailaware=True)
+        osp = {k: v.copy() for k, v in inp.items()}
+        dsp, _ = ref.run(osp, m["dt"], m["itimestep"], variant="sp")
+        rsp, psp = fp32_errors(inp, osp, dsp, o64, d64, m["dt"])
+        print("reference -DSINGLE_PREC build vs FP64 twin:", rsp, "precip rel L2", psp)
+        for k, (_, f) in rep.items():
+            assert f <= 2.0 * rsp[k][1] + 1e-4, (k, f, rsp[k][1])
+    for k, (rel, f) in rep.items():
         assert rel <= 1e-4, (k, rel)
+        assert f <= 0.02, (k, f)
     assert prel <= 1e-5, prel
 
 
