@@ -1,0 +1,428 @@
+! Fortran host binding of the MI355X TEMPO column step (libtempo_hip.so).
+!
+! A host model that calls the reference driver
+!     use module_mp_tempo_driver, only : tempo_init, tempo_run, ty_tempo_driver_diags
+! switches to this library with
+!     use tempo_hip_mod, only : tempo_init, tempo_run, ty_tempo_driver_diags, ty_tempo_cfgs
+! and keeps its calls: tempo_init takes the same optional flags
+! (src/module_mp_tempo_driver.F90:42-54), tempo_run the same argument list
+! and memory/tile bounds (src/module_mp_tempo_driver.F90:187-279), and the
+! diagnostics come back in a ty_tempo_driver_diags with the reference's
+! component names (:18-34), allocated on the first call as the reference
+! does (:304-386).  Arrays are the host's real(8) arrays; the library stages
+! them through device memory (tempo_hip_run_host).  Hosts whose state already
+! lives in device memory call the C entry point tempo_hip_run directly (the
+! interface block below).
+!
+! The reference's unused tempo_run arguments (qc_bl, qcfrac_bl, qcfrac,
+! qifrac, thten_*, q*ten_bl, thten_*rad; never read by tempo_main,
+! src/module_mp_tempo_main.F90:76-79,109-118) are accepted and ignored.
+module tempo_hip_mod
+  use iso_c_binding
+  implicit none
+  private
+
+  public :: tempo_init, tempo_run, tempo_finalize, ty_tempo_cfgs, ty_tempo_driver_diags
+  public :: tempo_hip_cfg, tempo_hip_fields, tempo_hip_diags, tempo_hip_dims
+  public :: tempo_hip_cfg_default, tempo_hip_init, tempo_hip_run, tempo_hip_run_host, tempo_hip_last_error
+
+  integer, parameter :: wp = c_double
+
+  ! ---- C ABI (include/tempo_hip.h) -------------------------------------
+  type, bind(c) :: tempo_hip_cfg
+    integer(c_int) :: aerosolaware_flag, hailaware_flag, ml_for_bl_nc_flag, ml_for_nc_flag, semi_sedi_flag
+    integer(c_int) :: refl10cm_from_melting_flag, turn_off_micro_flag, cloud_condensation_flag, verbose
+    integer(c_int) :: check_tables, nc_option
+    real(c_double) :: nc_value
+    integer(c_int) :: refl10cm_flag, re_cloud_flag, re_ice_flag, re_snow_flag, max_hail_diameter_flag
+    integer(c_int) :: rain_med_vol_diam_flag, graupel_med_vol_diam_flag, cloud_number_mixing_ratio_flag
+  end type
+
+  type, bind(c) :: tempo_hip_fields
+    type(c_ptr) :: t = c_null_ptr, th = c_null_ptr, pii = c_null_ptr, p = c_null_ptr, w = c_null_ptr
+    type(c_ptr) :: dz = c_null_ptr, qv = c_null_ptr, qc = c_null_ptr, qr = c_null_ptr, qi = c_null_ptr
+    type(c_ptr) :: qs = c_null_ptr, qg = c_null_ptr, ni = c_null_ptr, nr = c_null_ptr, nc = c_null_ptr
+    type(c_ptr) :: nwfa = c_null_ptr, nifa = c_null_ptr, ng = c_null_ptr, qb = c_null_ptr
+    type(c_ptr) :: land = c_null_ptr
+  end type
+
+  type, bind(c) :: tempo_hip_diags
+    type(c_ptr) :: rain_precip = c_null_ptr, ice_liquid_equiv_precip = c_null_ptr
+    type(c_ptr) :: snow_liquid_equiv_precip = c_null_ptr, graupel_liquid_equiv_precip = c_null_ptr
+    type(c_ptr) :: frozen_fraction = c_null_ptr, frz_rain_precip = c_null_ptr
+    type(c_ptr) :: max_hail_diameter_sfc = c_null_ptr, max_hail_diameter_column = c_null_ptr
+    type(c_ptr) :: refl10cm = c_null_ptr, re_cloud = c_null_ptr, re_ice = c_null_ptr, re_snow = c_null_ptr
+    type(c_ptr) :: rain_med_vol_diam = c_null_ptr, graupel_med_vol_diam = c_null_ptr
+    type(c_ptr) :: cloud_number_mixing_ratio = c_null_ptr
+  end type
+
+  type, bind(c) :: tempo_hip_dims
+    integer(c_int) :: ni, nj, nz
+    integer(c_long_long) :: ldk, ldj, dldk, dldj, d2ldj
+    type(c_ptr) :: nlev = c_null_ptr
+  end type
+
+  interface
+    subroutine tempo_hip_cfg_default(cfg) bind(c, name='tempo_hip_cfg_default')
+      import :: tempo_hip_cfg
+      type(tempo_hip_cfg), intent(out) :: cfg
+    end subroutine
+    integer(c_int) function tempo_hip_init(cfg, table_dir, ccn_path, ctx) bind(c, name='tempo_hip_init')
+      ! table_dir = c_null_ptr: build the tables on the GPU
+      import :: tempo_hip_cfg, c_int, c_ptr
+      type(tempo_hip_cfg), intent(in) :: cfg
+      type(c_ptr), value :: table_dir, ccn_path
+      type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function tempo_hip_run(ctx, f, d, dims, dt, itimestep, precision, stream) &
+        bind(c, name='tempo_hip_run')
+      import :: tempo_hip_fields, tempo_hip_diags, tempo_hip_dims, c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx, stream
+      type(tempo_hip_fields), intent(in) :: f
+      type(tempo_hip_diags), intent(in) :: d
+      type(tempo_hip_dims), intent(in) :: dims
+      real(c_double), value :: dt
+      integer(c_int), value :: itimestep, precision
+    end function
+    integer(c_int) function tempo_hip_run_host(ctx, f, d, dims, dt, itimestep) bind(c, name='tempo_hip_run_host')
+      import :: tempo_hip_fields, tempo_hip_diags, tempo_hip_dims, c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      type(tempo_hip_fields), intent(in) :: f
+      type(tempo_hip_diags), intent(in) :: d
+      type(tempo_hip_dims), intent(in) :: dims
+      real(c_double), value :: dt
+      integer(c_int), value :: itimestep
+    end function
+    type(c_ptr) function tempo_hip_last_error_c() bind(c, name='tempo_hip_last_error')
+      import :: c_ptr
+    end function
+    subroutine tempo_hip_finalize(ctx) bind(c, name='tempo_hip_finalize')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine
+  end interface
+
+  ! ---- reference-shaped types ------------------------------------------
+  ! ty_tempo_cfgs (src/module_mp_tempo_cfgs.F90:14-38) plus the library
+  ! context it owns after tempo_init
+  type :: ty_tempo_cfgs
+    logical :: aerosolaware_flag = .true.
+    logical :: hailaware_flag = .true.
+    logical :: ml_for_bl_nc_flag = .false.
+    logical :: ml_for_nc_flag = .false.
+    logical :: semi_sedi_flag = .false.
+    logical :: refl10cm_from_melting_flag = .false.
+    logical :: turn_off_micro_flag = .false.
+    logical :: cloud_condensation_flag = .true.
+    logical :: verbose = .false.
+    logical :: check_tables = .false.
+    character(len=32) :: single_moment_nc_opt = 'land'
+    logical :: refl10cm_flag = .true.
+    logical :: re_cloud_flag = .true.
+    logical :: re_ice_flag = .true.
+    logical :: re_snow_flag = .true.
+    logical :: max_hail_diameter_flag = .true.
+    logical :: rain_med_vol_diam_flag = .false.
+    logical :: graupel_med_vol_diam_flag = .false.
+    logical :: cloud_number_mixing_ratio_flag = .false.
+    character(len=256) :: table_dir = ''   ! '' = build the tables on the GPU
+    character(len=256) :: ccn_path = ''    ! '' = <table_dir>/ccn_activate.bin
+    type(c_ptr) :: ctx = c_null_ptr
+  end type
+
+  ! ty_tempo_driver_diags (src/module_mp_tempo_driver.F90:18-34)
+  type :: ty_tempo_driver_diags
+    real(wp), dimension(:,:), allocatable :: rain_precip
+    real(wp), dimension(:,:), allocatable :: ice_liquid_equiv_precip
+    real(wp), dimension(:,:), allocatable :: snow_liquid_equiv_precip
+    real(wp), dimension(:,:), allocatable :: graupel_liquid_equiv_precip
+    real(wp), dimension(:,:), allocatable :: frozen_fraction
+    real(wp), dimension(:,:), allocatable :: frz_rain_precip
+    real(wp), dimension(:,:), allocatable :: max_hail_diameter_sfc
+    real(wp), dimension(:,:), allocatable :: max_hail_diameter_column
+    real(wp), dimension(:,:,:), allocatable :: refl10cm
+    real(wp), dimension(:,:,:), allocatable :: re_cloud
+    real(wp), dimension(:,:,:), allocatable :: re_ice
+    real(wp), dimension(:,:,:), allocatable :: re_snow
+    real(wp), dimension(:,:,:), allocatable :: rain_med_vol_diam
+    real(wp), dimension(:,:,:), allocatable :: graupel_med_vol_diam
+    real(wp), dimension(:,:,:), allocatable :: cloud_number_mixing_ratio
+  end type
+
+contains
+
+  function tempo_hip_last_error() result(msg)
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: s(:)
+    integer :: n
+    type(c_ptr) :: p
+    p = tempo_hip_last_error_c()
+    call c_f_pointer(p, s, [4096])
+    n = 0
+    do while (n < 4096)
+      if (s(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    allocate(character(len=n) :: msg)
+    msg = transfer(s(1:n), msg)
+  end function
+
+  subroutine fail(what, rc)
+    character(len=*), intent(in) :: what
+    integer(c_int), intent(in) :: rc
+    write(*, '(a,a,i0,a,a)') what, ' failed with status ', rc, ': ', tempo_hip_last_error()
+    error stop 1
+  end subroutine
+
+  ! NUL-terminated copy of a Fortran string for the C ABI
+  function cstr(str) result(a)
+    character(len=*), intent(in) :: str
+    character(kind=c_char), allocatable :: a(:)
+    integer :: i
+    allocate(a(len(str) + 1))
+    do i = 1, len(str)
+      a(i) = str(i:i)
+    end do
+    a(len(str) + 1) = c_null_char
+  end function
+
+  integer(c_int) function flag(l)
+    logical, intent(in) :: l
+    flag = merge(1_c_int, 0_c_int, l)
+  end function
+
+  !> tempo_init (src/module_mp_tempo_driver.F90:42-182): flags, then the
+  !! library context (constants, init-time tables, collection/freezing tables
+  !! read from tempo_cfgs%table_dir or built on the GPU).  force_init_flag
+  !! re-creates the context.
+  subroutine tempo_init(aerosolaware_flag, hailaware_flag, semi_sedi_flag, cloud_condensation_flag, &
+      refl10cm_from_melting_flag, ml_for_bl_nc_flag, ml_for_nc_flag, force_init_flag, tempo_cfgs)
+    logical, intent(in), optional :: aerosolaware_flag, hailaware_flag, refl10cm_from_melting_flag, &
+      ml_for_bl_nc_flag, ml_for_nc_flag, force_init_flag, semi_sedi_flag, cloud_condensation_flag
+    type(ty_tempo_cfgs), intent(inout) :: tempo_cfgs
+    type(tempo_hip_cfg) :: c
+    integer(c_int) :: rc
+    character(len=:), allocatable :: ccn
+    character(kind=c_char), allocatable, target :: cdir(:), cccn(:)
+
+    if (c_associated(tempo_cfgs%ctx)) then
+      if (.not. present(force_init_flag)) return
+      if (.not. force_init_flag) return
+      call tempo_hip_finalize(tempo_cfgs%ctx)
+      tempo_cfgs%ctx = c_null_ptr
+    endif
+    if (present(aerosolaware_flag)) tempo_cfgs%aerosolaware_flag = aerosolaware_flag
+    if (present(hailaware_flag)) tempo_cfgs%hailaware_flag = hailaware_flag
+    if (present(ml_for_bl_nc_flag)) tempo_cfgs%ml_for_bl_nc_flag = ml_for_bl_nc_flag
+    if (present(ml_for_nc_flag)) tempo_cfgs%ml_for_nc_flag = ml_for_nc_flag
+    if (present(semi_sedi_flag)) tempo_cfgs%semi_sedi_flag = semi_sedi_flag
+    if (present(cloud_condensation_flag)) tempo_cfgs%cloud_condensation_flag = cloud_condensation_flag
+    if (present(refl10cm_from_melting_flag)) tempo_cfgs%refl10cm_from_melting_flag = refl10cm_from_melting_flag
+
+    call tempo_hip_cfg_default(c)
+    c%aerosolaware_flag = flag(tempo_cfgs%aerosolaware_flag)
+    c%hailaware_flag = flag(tempo_cfgs%hailaware_flag)
+    c%ml_for_bl_nc_flag = flag(tempo_cfgs%ml_for_bl_nc_flag)
+    c%ml_for_nc_flag = flag(tempo_cfgs%ml_for_nc_flag)
+    c%semi_sedi_flag = flag(tempo_cfgs%semi_sedi_flag)
+    c%refl10cm_from_melting_flag = flag(tempo_cfgs%refl10cm_from_melting_flag)
+    c%turn_off_micro_flag = flag(tempo_cfgs%turn_off_micro_flag)
+    c%cloud_condensation_flag = flag(tempo_cfgs%cloud_condensation_flag)
+    c%verbose = flag(tempo_cfgs%verbose)
+    c%check_tables = flag(tempo_cfgs%check_tables)
+    ! resolve_nc_value (src/module_mp_tempo_cfgs.F90:50-73)
+    select case (trim(tempo_cfgs%single_moment_nc_opt))
+    case ('land')
+      c%nc_option = 0
+    case ('ocean')
+      c%nc_option = 1
+    case default
+      c%nc_option = 2
+      read(tempo_cfgs%single_moment_nc_opt, *) c%nc_value
+    end select
+    c%refl10cm_flag = flag(tempo_cfgs%refl10cm_flag)
+    c%re_cloud_flag = flag(tempo_cfgs%re_cloud_flag)
+    c%re_ice_flag = flag(tempo_cfgs%re_ice_flag)
+    c%re_snow_flag = flag(tempo_cfgs%re_snow_flag)
+    c%max_hail_diameter_flag = flag(tempo_cfgs%max_hail_diameter_flag)
+    c%rain_med_vol_diam_flag = flag(tempo_cfgs%rain_med_vol_diam_flag)
+    c%graupel_med_vol_diam_flag = flag(tempo_cfgs%graupel_med_vol_diam_flag)
+    c%cloud_number_mixing_ratio_flag = flag(tempo_cfgs%cloud_number_mixing_ratio_flag)
+
+    if (len_trim(tempo_cfgs%ccn_path) > 0) then
+      ccn = trim(tempo_cfgs%ccn_path)
+    else
+      ccn = trim(tempo_cfgs%table_dir) // '/ccn_activate.bin'
+    endif
+    cccn = cstr(ccn)
+    if (len_trim(tempo_cfgs%table_dir) > 0) then
+      cdir = cstr(trim(tempo_cfgs%table_dir))
+      rc = tempo_hip_init(c, c_loc(cdir), c_loc(cccn), tempo_cfgs%ctx)
+    else
+      rc = tempo_hip_init(c, c_null_ptr, c_loc(cccn), tempo_cfgs%ctx)
+    endif
+    if (rc /= 0) call fail('tempo_init', rc)
+  end subroutine
+
+  subroutine tempo_finalize(tempo_cfgs)
+    type(ty_tempo_cfgs), intent(inout) :: tempo_cfgs
+    if (c_associated(tempo_cfgs%ctx)) call tempo_hip_finalize(tempo_cfgs%ctx)
+    tempo_cfgs%ctx = c_null_ptr
+  end subroutine
+
+  !> tempo_run (src/module_mp_tempo_driver.F90:187-513) over the tile
+  !! its:ite, kts:kte, jts:jte of the host's (ims:ime, kms:kme, jms:jme)
+  !! arrays.
+  subroutine tempo_run(tempo_cfgs, dt, itimestep, &
+      t, th, pii, p, w, dz, &
+      qv, qc, qr, qi, qs, qg, ni, nr, &
+      nc, nwfa, nifa, ng, qb, &
+      qc_bl, qcfrac_bl, &
+      qcfrac, qifrac, &
+      thten_bl, qvten_bl, qcten_bl, qiten_bl, &
+      thten_lwrad, thten_swrad, &
+      land_input, &
+      ids, ide, jds, jde, kds, kde, &
+      ims, ime, jms, jme, kms, kme, &
+      its, ite, jts, jte, kts, kte, tempo_diags)
+    type(ty_tempo_cfgs), intent(in) :: tempo_cfgs
+    real(wp), intent(in) :: dt
+    integer, intent(in) :: itimestep
+    integer, intent(in) :: ids, ide, jds, jde, kds, kde
+    integer, intent(in) :: ims, ime, jms, jme, kms, kme
+    integer, intent(in) :: its, ite, jts, jte, kts, kte
+    real(wp), dimension(ims:ime, kms:kme, jms:jme), intent(inout), optional, target :: t, th
+    real(wp), dimension(ims:ime, kms:kme, jms:jme), intent(in), target :: p, w, dz
+    real(wp), dimension(ims:ime, kms:kme, jms:jme), intent(in), optional, target :: pii
+    real(wp), dimension(ims:ime, kms:kme, jms:jme), intent(inout), target :: qv, qc, qr, qi, qs, qg, ni, nr
+    real(wp), dimension(ims:ime, kms:kme, jms:jme), intent(inout), optional, target :: nc, nwfa, nifa, qb, ng
+    integer, dimension(ims:ime, jms:jme), intent(in), optional :: land_input
+    real(wp), dimension(ims:ime, kms:kme, jms:jme), intent(inout), optional :: qcfrac, qifrac
+    real(wp), dimension(ims:ime, kms:kme, jms:jme), intent(in), optional :: qc_bl, qcfrac_bl, thten_bl, &
+      qvten_bl, qcten_bl, qiten_bl, thten_lwrad, thten_swrad
+    type(ty_tempo_driver_diags), intent(inout), target :: tempo_diags
+
+    type(tempo_hip_fields) :: f
+    type(tempo_hip_diags) :: d
+    type(tempo_hip_dims) :: dm
+    integer(c_int), allocatable, target :: land(:)
+    integer(c_int) :: rc
+    integer :: i, j, n
+
+    if (.not. c_associated(tempo_cfgs%ctx)) then
+      write(*, '(a)') 'tempo_run: tempo_init has not been called'
+      error stop 1
+    endif
+    if (.not. (present(t) .or. (present(th) .and. present(pii)))) then
+      write(*, '(a)') 'tempo_run: need t, or th and pii'
+      error stop 1
+    endif
+    call allocate_diags(tempo_cfgs, tempo_diags, ims, ime, jms, jme, kms, kme)
+
+    dm%ni = ite - its + 1
+    dm%nj = jte - jts + 1
+    dm%nz = kte - kts + 1
+    dm%ldk = ime - ims + 1
+    dm%ldj = dm%ldk * (kme - kms + 1)
+    dm%dldk = dm%ldk
+    dm%dldj = dm%ldj
+    dm%d2ldj = dm%ldk
+
+    if (present(t)) f%t = c_loc(t(its, kts, jts))
+    if (present(th)) f%th = c_loc(th(its, kts, jts))
+    if (present(pii)) f%pii = c_loc(pii(its, kts, jts))
+    f%p = c_loc(p(its, kts, jts))
+    f%w = c_loc(w(its, kts, jts))
+    f%dz = c_loc(dz(its, kts, jts))
+    f%qv = c_loc(qv(its, kts, jts))
+    f%qc = c_loc(qc(its, kts, jts))
+    f%qr = c_loc(qr(its, kts, jts))
+    f%qi = c_loc(qi(its, kts, jts))
+    f%qs = c_loc(qs(its, kts, jts))
+    f%qg = c_loc(qg(its, kts, jts))
+    f%ni = c_loc(ni(its, kts, jts))
+    f%nr = c_loc(nr(its, kts, jts))
+    if (present(nc)) f%nc = c_loc(nc(its, kts, jts))
+    if (present(nwfa)) f%nwfa = c_loc(nwfa(its, kts, jts))
+    if (present(nifa)) f%nifa = c_loc(nifa(its, kts, jts))
+    if (present(ng)) f%ng = c_loc(ng(its, kts, jts))
+    if (present(qb)) f%qb = c_loc(qb(its, kts, jts))
+    if (present(land_input)) then
+      allocate(land(dm%ni * dm%nj))
+      n = 0
+      do j = jts, jte
+        do i = its, ite
+          n = n + 1
+          land(n) = land_input(i, j)
+        end do
+      end do
+      f%land = c_loc(land)
+    endif
+
+    d%rain_precip = c_loc(tempo_diags%rain_precip(its, jts))
+    d%ice_liquid_equiv_precip = c_loc(tempo_diags%ice_liquid_equiv_precip(its, jts))
+    d%snow_liquid_equiv_precip = c_loc(tempo_diags%snow_liquid_equiv_precip(its, jts))
+    d%graupel_liquid_equiv_precip = c_loc(tempo_diags%graupel_liquid_equiv_precip(its, jts))
+    d%frozen_fraction = c_loc(tempo_diags%frozen_fraction(its, jts))
+    d%frz_rain_precip = c_loc(tempo_diags%frz_rain_precip(its, jts))
+    if (allocated(tempo_diags%max_hail_diameter_sfc)) then
+      d%max_hail_diameter_sfc = c_loc(tempo_diags%max_hail_diameter_sfc(its, jts))
+      d%max_hail_diameter_column = c_loc(tempo_diags%max_hail_diameter_column(its, jts))
+    endif
+    if (allocated(tempo_diags%refl10cm)) d%refl10cm = c_loc(tempo_diags%refl10cm(its, kts, jts))
+    if (allocated(tempo_diags%re_cloud)) d%re_cloud = c_loc(tempo_diags%re_cloud(its, kts, jts))
+    if (allocated(tempo_diags%re_ice)) d%re_ice = c_loc(tempo_diags%re_ice(its, kts, jts))
+    if (allocated(tempo_diags%re_snow)) d%re_snow = c_loc(tempo_diags%re_snow(its, kts, jts))
+    if (allocated(tempo_diags%rain_med_vol_diam)) &
+      d%rain_med_vol_diam = c_loc(tempo_diags%rain_med_vol_diam(its, kts, jts))
+    if (allocated(tempo_diags%graupel_med_vol_diam)) &
+      d%graupel_med_vol_diam = c_loc(tempo_diags%graupel_med_vol_diam(its, kts, jts))
+    if (allocated(tempo_diags%cloud_number_mixing_ratio)) &
+      d%cloud_number_mixing_ratio = c_loc(tempo_diags%cloud_number_mixing_ratio(its, kts, jts))
+
+    rc = tempo_hip_run_host(tempo_cfgs%ctx, f, d, dm, real(dt, c_double), int(itimestep, c_int))
+    if (rc /= 0) call fail('tempo_run', rc)
+  end subroutine
+
+  ! allocation and reset of the driver diagnostics
+  ! (src/module_mp_tempo_driver.F90:304-386)
+  subroutine allocate_diags(cfg, dg, ims, ime, jms, jme, kms, kme)
+    type(ty_tempo_cfgs), intent(in) :: cfg
+    type(ty_tempo_driver_diags), intent(inout) :: dg
+    integer, intent(in) :: ims, ime, jms, jme, kms, kme
+    if (.not. allocated(dg%rain_precip)) then
+      allocate(dg%rain_precip(ims:ime, jms:jme), dg%ice_liquid_equiv_precip(ims:ime, jms:jme), &
+        dg%snow_liquid_equiv_precip(ims:ime, jms:jme), dg%graupel_liquid_equiv_precip(ims:ime, jms:jme), &
+        dg%frozen_fraction(ims:ime, jms:jme), dg%frz_rain_precip(ims:ime, jms:jme))
+    endif
+    dg%rain_precip = 0._wp
+    dg%ice_liquid_equiv_precip = 0._wp
+    dg%snow_liquid_equiv_precip = 0._wp
+    dg%graupel_liquid_equiv_precip = 0._wp
+    dg%frozen_fraction = 0._wp
+    dg%frz_rain_precip = 0._wp
+    if (cfg%max_hail_diameter_flag) then
+      if (.not. allocated(dg%max_hail_diameter_sfc)) allocate(dg%max_hail_diameter_sfc(ims:ime, jms:jme), &
+        dg%max_hail_diameter_column(ims:ime, jms:jme))
+      dg%max_hail_diameter_sfc = 0._wp
+      dg%max_hail_diameter_column = 0._wp
+    endif
+    call alloc3(cfg%refl10cm_flag, dg%refl10cm, -35._wp)
+    call alloc3(cfg%re_cloud_flag, dg%re_cloud, 0._wp)
+    call alloc3(cfg%re_ice_flag, dg%re_ice, 0._wp)
+    call alloc3(cfg%re_snow_flag, dg%re_snow, 0._wp)
+    call alloc3(cfg%rain_med_vol_diam_flag, dg%rain_med_vol_diam, 0._wp)
+    call alloc3(cfg%graupel_med_vol_diam_flag, dg%graupel_med_vol_diam, 0._wp)
+    call alloc3(cfg%cloud_number_mixing_ratio_flag, dg%cloud_number_mixing_ratio, 0._wp)
+  contains
+    subroutine alloc3(on, a, v)
+      logical, intent(in) :: on
+      real(wp), allocatable, intent(inout) :: a(:,:,:)
+      real(wp), intent(in) :: v
+      if (.not. on) return
+      if (.not. allocated(a)) allocate(a(ims:ime, kms:kme, jms:jme))
+      a = v
+    end subroutine
+  end subroutine
+
+end module tempo_hip_mod

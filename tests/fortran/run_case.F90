@@ -1,0 +1,74 @@
+! Drives the Fortran binding (tempo_amd/fortran/tempo_hip_mod.F90) the way a
+! host model drives the reference: tempo_init, then tempo_run on a tile of
+! arrays with a one-point halo in i and k and two j rows.
+!
+!   run_case <input.bin> <output.bin> <ccn_activate.bin>
+!
+! input.bin (native endian): int32 ni, nj, nz, itimestep; float64 dt; then
+! 17 float64 fields t p w dz qv qc qr qi qs qg ni nr nc nwfa nifa ng qb, each
+! ni*nz*nj values in Fortran order (i, k, j); then int32 land(ni, nj).
+! output.bin: the 14 updated fields (t qv qc qr qi qs qg ni nr nc nwfa nifa
+! ng qb), the 8 two-dimensional and the 4 default three-dimensional
+! diagnostics over the tile, then int32 1 if every halo point kept its
+! sentinel value, else 0.
+program run_case
+  use iso_c_binding
+  use tempo_hip_mod
+  implicit none
+  integer, parameter :: nf = 17
+  integer(c_int) :: hdr(4)
+  real(c_double) :: dt
+  integer :: ni, nj, nz, its, f, u, ios
+  real(c_double), allocatable, target :: st(:,:,:,:)
+  integer, allocatable :: land(:,:)
+  real(c_double), allocatable :: buf(:,:,:)
+  type(ty_tempo_cfgs) :: cfg
+  type(ty_tempo_driver_diags) :: dg
+  character(len=512) :: fin, fout, fccn
+  real(c_double), parameter :: sentinel = -7.25_c_double
+  logical :: halo_ok
+  integer, parameter :: out_fields(14) = [1, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17]
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  call get_command_argument(3, fccn)
+  open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old', iostat=ios)
+  if (ios /= 0) error stop 2
+  read(u) hdr, dt
+  ni = hdr(1); nj = hdr(2); nz = hdr(3)
+  allocate(st(0:ni + 1, 0:nz + 1, 1:nj, nf), buf(ni, nz, nj), land(0:ni + 1, 1:nj))
+  st = sentinel
+  do f = 1, nf
+    read(u) buf
+    st(1:ni, 1:nz, 1:nj, f) = buf
+  end do
+  land = 0
+  read(u) land(1:ni, 1:nj)
+  close(u)
+
+  cfg%ccn_path = trim(fccn)
+  call tempo_init(aerosolaware_flag=.true., hailaware_flag=.true., tempo_cfgs=cfg)
+  its = 1
+  call tempo_run(cfg, dt, int(hdr(4)), &
+    t=st(:,:,:,1), p=st(:,:,:,2), w=st(:,:,:,3), dz=st(:,:,:,4), &
+    qv=st(:,:,:,5), qc=st(:,:,:,6), qr=st(:,:,:,7), qi=st(:,:,:,8), qs=st(:,:,:,9), qg=st(:,:,:,10), &
+    ni=st(:,:,:,11), nr=st(:,:,:,12), nc=st(:,:,:,13), nwfa=st(:,:,:,14), nifa=st(:,:,:,15), &
+    ng=st(:,:,:,16), qb=st(:,:,:,17), land_input=land, &
+    ids=1, ide=ni, jds=1, jde=nj, kds=1, kde=nz, &
+    ims=0, ime=ni + 1, jms=1, jme=nj, kms=0, kme=nz + 1, &
+    its=its, ite=ni, jts=1, jte=nj, kts=1, kte=nz, tempo_diags=dg)
+
+  halo_ok = all(st(0, :, :, :) == sentinel) .and. all(st(ni + 1, :, :, :) == sentinel) .and. &
+    all(st(:, 0, :, :) == sentinel) .and. all(st(:, nz + 1, :, :) == sentinel)
+  open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+  do f = 1, size(out_fields)
+    write(u) st(1:ni, 1:nz, 1:nj, out_fields(f))
+  end do
+  write(u) dg%rain_precip(1:ni, :), dg%ice_liquid_equiv_precip(1:ni, :), dg%snow_liquid_equiv_precip(1:ni, :), &
+    dg%graupel_liquid_equiv_precip(1:ni, :), dg%frozen_fraction(1:ni, :), dg%frz_rain_precip(1:ni, :), &
+    dg%max_hail_diameter_sfc(1:ni, :), dg%max_hail_diameter_column(1:ni, :)
+  write(u) dg%refl10cm(1:ni, 1:nz, :), dg%re_cloud(1:ni, 1:nz, :), dg%re_ice(1:ni, 1:nz, :), dg%re_snow(1:ni, 1:nz, :)
+  write(u) merge(1_c_int, 0_c_int, halo_ok)
+  close(u)
+  call tempo_finalize(cfg)
+end program run_case
