@@ -142,6 +142,19 @@ double tempo_hip_table_build_ms(tempo_hip_ctx *ctx);
  * snow steps, snow ktop, ice steps, ice ktop, cloud steps, cloud ktop. */
 int tempo_hip_substeps(tempo_hip_ctx *ctx, int *host_out, size_t n);
 
+/* Launch-shape tuning of subsequent runs in one precision (no counterpart in
+ * the reference; set by tempo_amd/autotune.py).  Keys and values:
+ *   TEMPO_HIP_TUNE_SED_WAVE  1: sedimentation with one wavefront per column
+ *                               (default; used when nz <= 128), 0: one lane
+ *                               per column
+ *   TEMPO_HIP_TUNE_BLOCK_A   threads per block of the level-local physics
+ *   TEMPO_HIP_TUNE_BLOCK_C   ... and of the final update: 64, 128 or 256
+ * Every setting gives identical results. */
+#define TEMPO_HIP_TUNE_SED_WAVE 1
+#define TEMPO_HIP_TUNE_BLOCK_A 2
+#define TEMPO_HIP_TUNE_BLOCK_C 3
+int tempo_hip_tune(tempo_hip_ctx *ctx, int precision, int key, int value);
+
 /* Per-phase timing of subsequent runs (no counterpart in the reference):
  * enable != 0 records HIP events around the four launches of each run
  * (pre-pass, level-local physics, sedimentation, final update); read

@@ -13,8 +13,8 @@
 
 namespace tempo {
 // defined in tempo_step_f32.hip / tempo_step_f64.hip / tempo_tables.hip
-hipError_t launch_step(const StepArgs<float>& a, double* vtbuf, hipStream_t s, hipEvent_t* ev);
-hipError_t launch_step(const StepArgs<double>& a, double* vtbuf, hipStream_t s, hipEvent_t* ev);
+hipError_t launch_step(const StepArgs<float>& a, const LaunchCfg& lc, double* vtbuf, hipStream_t s, hipEvent_t* ev);
+hipError_t launch_step(const StepArgs<double>& a, const LaunchCfg& lc, double* vtbuf, hipStream_t s, hipEvent_t* ev);
 hipError_t launch_table_qg(const double*, const double*, const double*, const double*, double*, hipStream_t);
 hipError_t launch_table_qs(const double*, const double*, const double*, const double*, double*, hipStream_t);
 hipError_t launch_table_fz(const double*, const double*, const double*, const double*, double, const double*,
@@ -130,6 +130,7 @@ struct tempo_hip_ctx {
   std::vector<DevBuf> stage;
   DevBuf stage_land, stage_nlev;
   double build_ms = 0.0;
+  LaunchCfg launch[2];  // [FP32, FP64]
   // per-phase timing: 5 events per profiled run
   bool profile = false;
   std::vector<hipEvent_t> events;
@@ -293,8 +294,10 @@ int pack_tables(tempo_hip_ctx* x) {
   return TEMPO_HIP_OK;
 }
 
+size_t scratch_plane(size_t ncol, size_t nz) { return (ncol + kScrW - 1) / kScrW * kScrW * nz; }
+
 int ensure_scratch(tempo_hip_ctx* x, size_t ncol, size_t nz) {
-  const size_t plane = ncol * nz;
+  const size_t plane = scratch_plane(ncol, nz);
   if (plane > x->cap_plane) {
     HIP_TRY(x->sr.ensure(SR_N * plane * sizeof(double)));
     HIP_TRY(x->sd.ensure(SD_N * plane * sizeof(double)));
@@ -342,6 +345,7 @@ int run_impl(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags*
   int rc = ensure_scratch(x, size_t(a.ncol), size_t(a.nz));
   if (rc) return rc;
   a.sr = x->sr.as<R>();
+  a.plane = scratch_plane(size_t(a.ncol), size_t(a.nz));
   a.sd = x->sd.as<double>();
   a.pf = x->pf.as<unsigned>();
   a.cf = x->cf_.as<unsigned>();
@@ -387,7 +391,7 @@ int run_impl(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags*
     }
     ev = &x->events[x->events.size() - 5];
   }
-  HIP_TRY(launch_step(a, x->vtbuf.as<double>(), s, ev));
+  HIP_TRY(launch_step(a, x->launch[r64 ? 1 : 0], x->vtbuf.as<double>(), s, ev));
   return TEMPO_HIP_OK;
 }
 
@@ -556,6 +560,25 @@ int tempo_hip_table_write(tempo_hip_ctx* x, const char* dir) {
     if (w != it.n) return fail(TEMPO_HIP_E_TABLE, "short write " + path);
   }
   return TEMPO_HIP_OK;
+}
+
+int tempo_hip_tune(tempo_hip_ctx* x, int precision, int key, int value) {
+  if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
+  if (precision != TEMPO_HIP_FP32 && precision != TEMPO_HIP_FP64) return fail(TEMPO_HIP_E_ARG, "bad precision");
+  LaunchCfg& lc = x->launch[precision == TEMPO_HIP_FP64 ? 1 : 0];
+  switch (key) {
+    case TEMPO_HIP_TUNE_SED_WAVE:
+      if (value != 0 && value != 1) return fail(TEMPO_HIP_E_ARG, "sed_wave must be 0 or 1");
+      lc.sed_wave = value;
+      return TEMPO_HIP_OK;
+    case TEMPO_HIP_TUNE_BLOCK_A:
+    case TEMPO_HIP_TUNE_BLOCK_C:
+      if (value != 64 && value != 128 && value != 256) return fail(TEMPO_HIP_E_ARG, "block must be 64, 128 or 256");
+      (key == TEMPO_HIP_TUNE_BLOCK_A ? lc.block_a : lc.block_c) = value;
+      return TEMPO_HIP_OK;
+    default:
+      return fail(TEMPO_HIP_E_ARG, "unknown tuning key");
+  }
 }
 
 int tempo_hip_profile(tempo_hip_ctx* x, int enable) {

@@ -24,7 +24,7 @@ template <class R>
 __global__ void __launch_bounds__(256) k_prepass(StepArgs<R> a) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.ncol) return;
-  const int nzc = a.nlev ? a.nlev[c] : a.nz;
+  const int nzc = a.nlev ? min(max(a.nlev[c], 1), a.nz) : a.nz;
   unsigned cf = 0;
   // early exit (:416-418): any hydrometeor above r1 after the initial checks
   // (tendencies are still zero, so l_q == q1d > r1), or ice supersaturation
@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(256) k_prepass(StepArgs<R> a) {
     R h = R(0);
     for (int k = 0; k < nzc; ++k) {
       if (k > 0) h = h + a.f[F_DZ][TEMPO_IDX_STATE(a, c, k)];
-      a.sr[size_t(SR_HGT) * a.nz * a.ncol + TEMPO_IDX_SCR(a, c, k)] = h;
+      a.sr[size_t(SR_HGT) * a.plane + TEMPO_IDX_SCR(a, c, k)] = h;
     }
   }
   a.cf[c] = cf;
@@ -74,7 +74,7 @@ template <class R>
 __device__ __forceinline__ R default_aerosol(const StepArgs<R>& a, int c, int k, R n0, R n1) {
   // init_water/ice_friendly_aerosols (src/module_mp_tempo_aerosols.F90:14-73):
   // hgt(1) = 0, so h_01 = 0.8; level 1 uses hgt(2)
-  const R* hgt = a.sr + size_t(SR_HGT) * a.nz * a.ncol;
+  const R* hgt = a.sr + size_t(SR_HGT) * a.plane;
   const R h1 = hgt[TEMPO_IDX_SCR(a, c, 0)];
   const R hk = hgt[TEMPO_IDX_SCR(a, c, (k == 0 ? 1 : k))];
   const R ni3 = R(-1.0) * mlog(n1 / n0) / R(0.8);
@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(256) k_phase_a(StepArgs<R> a) {
   const Const<R>& C = *a.cst;
   const size_t o = TEMPO_IDX_STATE(a, c, k);
   const size_t s = TEMPO_IDX_SCR(a, c, k);
-  const size_t plane = size_t(a.nz) * a.ncol;
+  const size_t plane = a.plane;
   const R dt = R(a.dt), odt = R(1.0) / dt;
   const unsigned cf = a.cf[c];
   const bool has_nc = a.f[F_NC], has_nwfa = a.f[F_NWFA], has_nifa = a.f[F_NIFA];
@@ -325,7 +325,7 @@ __global__ void __launch_bounds__(256) k_phase_a(StepArgs<R> a) {
 template <class R>
 __device__ void sediment(const StepArgs<R>& a, int c, int nzc, double* xr, const double* vt, double* xten,
                          double limit, int steps, int ktop, double* precip) {
-  const size_t plane = size_t(a.nz) * a.ncol;
+  const size_t plane = a.plane;
   const R* rho = a.sr + SR_RHO * plane;
   const double ost = 1.0 / double(steps), dt = a.dt;
   double above = 0.0, s0 = 0.0;
@@ -380,8 +380,8 @@ template <class R>
 __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, double* vtbuf) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.ncol) return;
-  const int nzc = a.nlev ? a.nlev[c] : a.nz;
-  const size_t plane = size_t(a.nz) * a.ncol;
+  const int nzc = a.nlev ? min(max(a.nlev[c], 1), a.nz) : a.nz;
+  const size_t plane = a.plane;
   double pr[5] = {0, 0, 0, 0, 0};  // rain graupel snow ice cloud
   const unsigned cf = a.cf[c];
   int kst[10] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
@@ -515,7 +515,7 @@ __global__ void __launch_bounds__(256) k_phase_c(StepArgs<R> a) {
   const Const<R>& C = *a.cst;
   const size_t o = TEMPO_IDX_STATE(a, c, k);
   const size_t s = TEMPO_IDX_SCR(a, c, k);
-  const size_t plane = size_t(a.nz) * a.ncol;
+  const size_t plane = a.plane;
   const size_t d3 = size_t(c % a.ni) + size_t(k) * a.dldk + size_t(c / a.ni) * a.dldj;
   const R dt = R(a.dt), odt = R(1.0) / dt;
   const bool live = !(a.cf[c] & CF_EXIT);

@@ -1,0 +1,462 @@
+// Phase B on CDNA4: one wavefront per column, levels across lanes.
+//
+// Level k of a column lives in lane k % 64 of register chunk k / 64
+// (kChunks chunks, so nz <= 64 * kChunks).  The column couplings of the
+// reference's sedimentation (src/module_mp_tempo_main.F90:590-727) become
+// wavefront primitives:
+//   top-down fill of a fall-speed profile (:1768-1789 and species variants)
+//       -> ballot of the levels holding mass, lowest set bit at or above
+//          each lane, value fetched by a lane permute;
+//   CFL substep count and ktop -> wavefront max reductions;
+//   one flux-form substep (:1516-1559) -> every level at once, the flux
+//          from the level above by a one-lane shift (all fluxes of a
+//          substep use the fields at its start, so the levels are
+//          independent within it).
+// Every per-level operand is staged through LDS: a block takes one column
+// group of the column-tiled scratch (kScrW columns whose levels are
+// contiguous), reads it as one contiguous run and transposes it to
+// (column, level) rows that one wavefront reads with unit stride; all planes
+// of a species are requested together.
+//
+// The arithmetic per level is the serial kernel's, operation for operation,
+// so both give identical results.
+#pragma once
+
+#include "tempo_step.hpp"
+
+namespace tempo {
+
+constexpr int kSedCols = kScrW;           // columns (wavefronts) per block: one scratch column group
+constexpr int kSedSlots = 8;              // LDS planes staged at once
+constexpr int kSedChunks = 2;             // register chunks of 64 levels
+constexpr int kSedLevels = 64 * kSedChunks;
+constexpr int kSedRow = kSedLevels + 1;   // padded LDS row of one column
+
+namespace wv {
+__device__ __forceinline__ int lane() { return int(threadIdx.x & 63u); }
+__device__ __forceinline__ int imax(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ unsigned uor(unsigned v) {
+  for (int o = 32; o > 0; o >>= 1) v |= unsigned(__shfl_xor(int(v), o, 64));
+  return v;
+}
+// value of x at level k (k uniform or per lane), chunks held in x[]
+template <class T>
+__device__ __forceinline__ T at_level(const T (&x)[kSedChunks], int k) {
+  T v = T(0);
+#pragma unroll
+  for (int j = 0; j < kSedChunks; ++j) {
+    const T t = __shfl(x[j], k & 63, 64);
+    if ((k >> 6) == j) v = t;
+  }
+  return v;
+}
+// value of x at level k+1 for every lane's level k in chunk j (0 above the top chunk)
+template <class T>
+__device__ __forceinline__ T above(const T (&x)[kSedChunks], int j) {
+  const T up = __shfl_down(x[j], 1, 64);
+  T nxt = T(0);
+  if (j + 1 < kSedChunks) nxt = __shfl(x[j + 1 < kSedChunks ? j + 1 : j], 0, 64);
+  return lane() == 63 ? nxt : up;
+}
+}  // namespace wv
+
+// Block-wide staging of (level, column) planes through LDS slots.  Loads
+// and stores issue without barriers so that all planes of a stage are in
+// flight together; the caller separates the phases with __syncthreads().
+struct SedTile {
+  double* lds;  // kSedSlots x kSedCols x kSedRow
+  int c0, ncol, nz;
+
+  __device__ double* slot(int s) const { return lds + s * (kSedCols * kSedRow); }
+  // scratch planes: the block's column group is one contiguous run
+  template <class T>
+  __device__ void load(int s, const T* plane) const {
+    double* d = slot(s);
+    const T* src = plane + size_t(c0) * nz;
+    for (int e = threadIdx.x; e < kSedCols * nz; e += blockDim.x) {
+      const int cc = e % kSedCols, k = e / kSedCols;
+      d[cc * kSedRow + k] = double(src[e]);
+    }
+  }
+  template <class R>
+  __device__ void load_state(int s, const StepArgs<R>& a, const R* f) const {
+    double* d = slot(s);
+    for (int e = threadIdx.x; e < kSedCols * nz; e += blockDim.x) {
+      const int cc = e % kSedCols, k = e / kSedCols, c = c0 + cc;
+      d[cc * kSedRow + k] = (c < ncol) ? double(f[TEMPO_IDX_STATE(a, c, k)]) : 0.0;
+    }
+  }
+  template <class T>
+  __device__ void store(int s, T* plane) const {
+    const double* d = slot(s);
+    T* dst = plane + size_t(c0) * nz;
+    for (int e = threadIdx.x; e < kSedCols * nz; e += blockDim.x) {
+      const int cc = e % kSedCols, k = e / kSedCols;
+      if (c0 + cc < ncol) dst[e] = T(d[cc * kSedRow + k]);
+    }
+  }
+  // this wavefront's column <-> registers
+  template <class T>
+  __device__ void get(int s, int w, T (&x)[kSedChunks]) const {
+    const double* d = slot(s);
+#pragma unroll
+    for (int j = 0; j < kSedChunks; ++j) {
+      const int k = j * 64 + wv::lane();
+      x[j] = (k < nz) ? T(d[w * kSedRow + k]) : T(0);
+    }
+  }
+  template <class T>
+  __device__ void put(int s, int w, const T (&x)[kSedChunks]) const {
+    double* d = slot(s);
+#pragma unroll
+    for (int j = 0; j < kSedChunks; ++j) {
+      const int k = j * 64 + wv::lane();
+      if (k < nz) d[w * kSedRow + k] = double(x[j]);
+    }
+  }
+};
+
+// Per-column level geometry held by every lane.
+struct SedCol {
+  int nzc;                          // active levels
+  double odz[kSedChunks], orho[kSedChunks];
+};
+
+// top-down fill (:1768-1789): out(k) = raw(k') at the lowest k' >= k whose
+// mass exceeds r1 (0 when none); CFL substeps and ktop from the filled
+// speeds in the working precision.
+template <class R>
+__device__ __forceinline__ void wave_fill(const double (&mass)[kSedChunks], const R (&raw1)[kSedChunks],
+                                          const R (&raw2)[kSedChunks], bool two, bool use_max,
+                                          const R (&dz)[kSedChunks], int nzc, double dt, double (&out1)[kSedChunks],
+                                          double (&out2)[kSedChunks], int& substeps, int& ktop) {
+  unsigned long long m[kSedChunks];
+#pragma unroll
+  for (int j = 0; j < kSedChunks; ++j) {
+    const int k = j * 64 + wv::lane();
+    m[j] = __ballot(k < nzc && mass[j] > kR1);
+  }
+  int st = 1, kt = 1;
+#pragma unroll
+  for (int j = 0; j < kSedChunks; ++j) {
+    const int l = wv::lane(), k = j * 64 + l;
+    int src = -1;
+    const unsigned long long mm = m[j] & (~0ull << l);
+    if (mm) {
+      src = j * 64 + __ffsll((long long)mm) - 1;
+    } else {
+#pragma unroll
+      for (int j2 = j + 1; j2 < kSedChunks; ++j2)
+        if (src < 0 && m[j2]) src = j2 * 64 + __ffsll((long long)m[j2]) - 1;
+    }
+    const R r1 = wv::at_level(raw1, src < 0 ? 0 : src);
+    const R r2 = two ? wv::at_level(raw2, src < 0 ? 0 : src) : R(0);
+    const double v1 = src < 0 ? 0.0 : double(r1);
+    const double v2 = src < 0 ? 0.0 : double(r2);
+    out1[j] = v1;
+    out2[j] = v2;
+    if (k < nzc) {
+      const R vr1 = R(v1), vr2 = R(v2);
+      const R vmax = use_max ? mx(vr1, vr2) : vr1;
+      if (vmax > R(1.e-3)) {
+        kt = max(kt, k + 1);
+        const R dz_by_vt = dz[j] / vmax;
+        st = max(st, int(R(dt) / dz_by_vt + R(1.0)));
+      }
+    }
+  }
+  substeps = wv::imax(st);
+  ktop = wv::imax(kt);
+  if (ktop == nzc) ktop = nzc - 1;
+}
+
+// one flux-form substep of one field (:1516-1559); returns the surface
+// precipitation increment (0 when the caller does not accumulate it)
+__device__ __forceinline__ double wave_sediment(const SedCol& g, double (&xr)[kSedChunks],
+                                                const double (&vt)[kSedChunks], double (&xten)[kSedChunks],
+                                                double limit, int steps, int ktop, double dt, bool precip) {
+  const double ost = 1.0 / double(steps);
+  double sed[kSedChunks];
+#pragma unroll
+  for (int j = 0; j < kSedChunks; ++j) sed[j] = vt[j] * xr[j];
+  const double s0 = __shfl(sed[0], 0, 64);
+#pragma unroll
+  for (int j = 0; j < kSedChunks; ++j) {
+    const double up = wv::above(sed, j);
+    const int k = j * 64 + wv::lane();
+    if (k < g.nzc && (k == g.nzc - 1 || k <= ktop - 1)) {
+      const double flux = (k == g.nzc - 1) ? -sed[j] : (up - sed[j]);
+      xten[j] = xten[j] + flux * g.odz[j] * ost * g.orho[j];
+      const double v = xr[j] + flux * g.odz[j] * dt * ost;
+      xr[j] = v > limit ? v : limit;
+    }
+  }
+  if (!precip) return 0.0;
+  const double x0 = __shfl(xr[0], 0, 64);
+  return x0 > kLowLimitPrecip ? s0 * dt * ost : 0.0;
+}
+
+template <class R>
+__global__ void __launch_bounds__(64 * kSedCols) k_phase_b_wave(StepArgs<R> a) {
+  __shared__ double tile_mem[kSedSlots * kSedCols * kSedRow];
+  __shared__ unsigned blk_any;
+  const int w = int(threadIdx.x >> 6);
+  const int c0 = int(blockIdx.x) * kSedCols;
+  const int c = c0 + w;
+  const bool col_ok = c < a.ncol;
+  const size_t plane = a.plane;
+  const SedTile T{tile_mem, c0, a.ncol, a.nz};
+  const double dt = a.dt;
+  double* sd = a.sd;
+  const R* sr = a.sr;
+
+  SedCol g;
+  g.nzc = col_ok ? (a.nlev ? min(max(a.nlev[c], 1), a.nz) : a.nz) : 0;
+  const unsigned cf = col_ok ? a.cf[c] : CF_EXIT;
+  const bool live = col_ok && !(cf & CF_EXIT);
+
+  // column flags (which species hold mass) and the level geometry
+  if (threadIdx.x == 0) blk_any = 0;
+  T.load(0, a.pf);
+  T.load_state(1, a, a.f[F_DZ]);
+  T.load(2, sr + SR_RHO * plane);
+  T.load(3, sd + SD_RR * plane);
+  __syncthreads();
+  unsigned pfl[kSedChunks];
+  R dz[kSedChunks], rho[kSedChunks];
+  double rr[kSedChunks], vr[kSedChunks];
+  T.get(0, w, pfl);
+  T.get(1, w, dz);
+  T.get(2, w, rho);
+  T.get(3, w, rr);
+  unsigned any = 0;
+#pragma unroll
+  for (int j = 0; j < kSedChunks; ++j) {
+    if (j * 64 + wv::lane() < g.nzc) any |= pfl[j];
+    g.odz[j] = 1.0 / double(dz[j]);
+    g.orho[j] = 1.0 / double(rho[j]);
+    vr[j] = 0.0;
+  }
+  any = live ? wv::uor(any) : 0u;
+  if (wv::lane() == 0 && any) atomicOr(&blk_any, any);
+  __syncthreads();
+  const unsigned bany = blk_any;
+
+  double pr[5] = {0, 0, 0, 0, 0};
+  int kst[10] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+
+  // ---- rain (:603-630) ----
+  if (bany & PF_QR) {
+    double xn[kSedChunks], tr[kSedChunks], tn[kSedChunks], v2[kSedChunks];
+    R raw1[kSedChunks], raw2[kSedChunks];
+    T.load(0, sd + SD_NR * plane);
+    T.load(1, sd + SD_QRTEN * plane);
+    T.load(2, sd + SD_NRTEN * plane);
+    T.load(4, sr + SR_VTRR * plane);
+    T.load(5, sr + SR_VTNR * plane);
+    __syncthreads();
+    T.get(0, w, xn);
+    T.get(1, w, tr);
+    T.get(2, w, tn);
+    T.get(4, w, raw1);
+    T.get(5, w, raw2);
+    if (any & PF_QR) {
+      int steps, ktop;
+      wave_fill(rr, raw1, raw2, true, true, dz, g.nzc, dt, vr, v2, steps, ktop);
+      for (int n = 0; n < steps; ++n) {
+        pr[0] += wave_sediment(g, rr, vr, tr, kR1, steps, ktop, dt, true);
+        wave_sediment(g, xn, v2, tn, kR2, steps, ktop, dt, false);
+      }
+      kst[0] = steps;
+      kst[1] = ktop;
+    }
+    T.put(1, w, tr);
+    T.put(2, w, tn);
+    __syncthreads();
+    T.store(1, sd + SD_QRTEN * plane);
+    T.store(2, sd + SD_NRTEN * plane);
+    __syncthreads();
+  }
+
+  // ---- graupel (:632-660) ----
+  if (bany & PF_QG) {
+    double xg[kSedChunks], xn[kSedChunks], xb[kSedChunks], tg[kSedChunks], tn[kSedChunks], tb[kSedChunks];
+    double v1[kSedChunks], v2[kSedChunks];
+    R raw1[kSedChunks], raw2[kSedChunks];
+    T.load(0, sd + SD_RG * plane);
+    T.load(1, sd + SD_NG * plane);
+    T.load(2, sd + SD_RB * plane);
+    T.load(4, sd + SD_QGTEN * plane);
+    T.load(5, sd + SD_NGTEN * plane);
+    T.load(6, sd + SD_QBTEN * plane);
+    T.load(7, sr + SR_VTRG * plane);
+    T.load(3, sr + SR_VTNG * plane);
+    __syncthreads();
+    T.get(0, w, xg);
+    T.get(1, w, xn);
+    T.get(2, w, xb);
+    T.get(4, w, tg);
+    T.get(5, w, tn);
+    T.get(6, w, tb);
+    T.get(7, w, raw1);
+    T.get(3, w, raw2);
+    if (any & PF_QG) {
+      int steps, ktop;
+      wave_fill(xg, raw1, raw2, true, false, dz, g.nzc, dt, v1, v2, steps, ktop);
+      const double rb_lim = kM3ToL * kR1 / 800.0;
+      for (int n = 0; n < steps; ++n) {
+        pr[1] += wave_sediment(g, xg, v1, tg, kR1, steps, ktop, dt, true);
+        wave_sediment(g, xn, v2, tn, kR2, steps, ktop, dt, false);
+        wave_sediment(g, xb, v1, tb, rb_lim, steps, ktop, dt, false);
+      }
+      kst[2] = steps;
+      kst[3] = ktop;
+    }
+    T.put(4, w, tg);
+    T.put(5, w, tn);
+    T.put(6, w, tb);
+    __syncthreads();
+    T.store(4, sd + SD_QGTEN * plane);
+    T.store(5, sd + SD_NGTEN * plane);
+    T.store(6, sd + SD_QBTEN * plane);
+    __syncthreads();
+  }
+
+  // ---- snow (:662-697): mixes in the filled rain speed where snow melts,
+  // with the rain mass after rain sedimentation ----
+  if (bany & PF_QS) {
+    double xs[kSedChunks], ts[kSedChunks], v1[kSedChunks];
+    R raw[kSedChunks];
+    T.load(0, sd + SD_RS * plane);
+    T.load(1, sd + SD_QSTEN * plane);
+    T.load(2, sr + SR_VTS * plane);
+    __syncthreads();
+    T.get(0, w, xs);
+    T.get(1, w, ts);
+    T.get(2, w, raw);
+    if (any & PF_QS) {
+      R vraw[kSedChunks];
+#pragma unroll
+      for (int j = 0; j < kSedChunks; ++j) {
+        R v = raw[j];
+        if (pfl[j] & PF_SNOWMIX) {
+          const R sr_ = R(xs[j] / (xs[j] + rr[j]));
+          v = v * sr_ + (R(1.0) - sr_) * R(vr[j]);
+        }
+        vraw[j] = v;
+      }
+      double unused[kSedChunks];
+      int steps, ktop;
+      wave_fill(xs, vraw, vraw, false, false, dz, g.nzc, dt, v1, unused, steps, ktop);
+      for (int n = 0; n < steps; ++n) pr[2] += wave_sediment(g, xs, v1, ts, kR1, steps, ktop, dt, true);
+      kst[4] = steps;
+      kst[5] = ktop;
+    }
+    T.put(1, w, ts);
+    __syncthreads();
+    T.store(1, sd + SD_QSTEN * plane);
+    __syncthreads();
+  }
+
+  // ---- ice (:699-715): one call with steps = substeps ----
+  if (bany & PF_QI) {
+    double xi[kSedChunks], xn[kSedChunks], ti[kSedChunks], tn[kSedChunks], v1[kSedChunks], v2[kSedChunks];
+    R raw1[kSedChunks], raw2[kSedChunks];
+    T.load(0, sd + SD_RI * plane);
+    T.load(1, sd + SD_NI * plane);
+    T.load(2, sd + SD_QITEN * plane);
+    T.load(4, sd + SD_NITEN * plane);
+    T.load(5, sr + SR_VTRI * plane);
+    T.load(6, sr + SR_VTNI * plane);
+    __syncthreads();
+    T.get(0, w, xi);
+    T.get(1, w, xn);
+    T.get(2, w, ti);
+    T.get(4, w, tn);
+    T.get(5, w, raw1);
+    T.get(6, w, raw2);
+    if (any & PF_QI) {
+      int steps, ktop;
+      wave_fill(xi, raw1, raw2, true, false, dz, g.nzc, dt, v1, v2, steps, ktop);
+      pr[3] += wave_sediment(g, xi, v1, ti, kR1, steps, ktop, dt, true);
+      wave_sediment(g, xn, v2, tn, kR2, steps, ktop, dt, false);
+      kst[6] = steps;
+      kst[7] = ktop;
+    }
+    T.put(2, w, ti);
+    T.put(4, w, tn);
+    __syncthreads();
+    T.store(2, sd + SD_QITEN * plane);
+    T.store(4, sd + SD_NITEN * plane);
+    __syncthreads();
+  }
+
+  // ---- cloud (:717-727, cloud_fallspeed :1945-1977): below 500 m only ----
+  if (bany & PF_QC) {
+    double xc[kSedChunks], xn[kSedChunks], tc[kSedChunks], tn[kSedChunks], v1[kSedChunks], v2[kSedChunks];
+    R raw1[kSedChunks], raw2[kSedChunks];
+    T.load(0, sd + SD_RC * plane);
+    T.load(1, sd + SD_NC * plane);
+    T.load(2, sd + SD_QCTEN * plane);
+    T.load(4, sd + SD_NCTEN * plane);
+    T.load(5, sr + SR_VTRC * plane);
+    T.load(6, sr + SR_VTNC * plane);
+    __syncthreads();
+    T.get(0, w, xc);
+    T.get(1, w, xn);
+    T.get(2, w, tc);
+    T.get(4, w, tn);
+    T.get(5, w, raw1);
+    T.get(6, w, raw2);
+    if (any & PF_QC) {
+      int kt = 1;
+      R hgt = R(0.0);
+      for (int k = 0; k < g.nzc - 1; ++k) {  // uniform loop: every lane follows level k
+        if (wv::at_level(xc, k) > kR2) kt = k + 1;
+        hgt = hgt + wv::at_level(dz, k);
+        if (hgt > R(500.0)) break;
+      }
+#pragma unroll
+      for (int j = 0; j < kSedChunks; ++j) {
+        const int k = j * 64 + wv::lane();
+        v1[j] = (k < kt) ? double(raw1[j]) : 0.0;
+        v2[j] = (k < kt) ? double(raw2[j]) : 0.0;
+      }
+      pr[4] += wave_sediment(g, xc, v1, tc, kR1, 1, kt, dt, true);
+      wave_sediment(g, xn, v2, tn, kR2, 1, kt, dt, false);
+      kst[8] = 1;
+      kst[9] = kt;
+    }
+    T.put(2, w, tc);
+    T.put(4, w, tn);
+    __syncthreads();
+    T.store(2, sd + SD_QCTEN * plane);
+    T.store(4, sd + SD_NCTEN * plane);
+  }
+
+  if (!col_ok || wv::lane() != 0) return;
+  for (int q = 0; q < 5; ++q) a.precip[size_t(q) * a.ncol + c] = pr[q];
+  if (a.ksub)
+    for (int q = 0; q < 10; ++q) a.ksub[size_t(q) * a.ncol + c] = kst[q];
+  // column diagnostics (src/module_mp_tempo_main.F90:784-794, driver :452-457)
+  const size_t d2 = size_t(c % a.ni) + size_t(c / a.ni) * a.d2ldj;
+  const R rain = R(pr[0]), graupel = R(pr[1]), snow = R(pr[2]), ice = R(pr[3]), cloud = R(pr[4]);
+  if (a.d2[D2_RAIN]) a.d2[D2_RAIN][d2] = rain;
+  if (a.d2[D2_ICE]) a.d2[D2_ICE][d2] = ice;
+  if (a.d2[D2_SNOW]) a.d2[D2_SNOW][d2] = snow;
+  if (a.d2[D2_GRAUPEL]) a.d2[D2_GRAUPEL][d2] = graupel;
+  if (a.d2[D2_FROZEN])
+    a.d2[D2_FROZEN][d2] = live ? (ice + snow + graupel) / (ice + snow + graupel + rain + R(kR1)) : R(0);
+  if (a.d2[D2_FRZRAIN]) {
+    R frz = R(0);
+    if (live && (a.sr[SR_TEMP * plane + TEMPO_IDX_SCR(a, c, 0)] - R(kT0)) < R(-0.5)) frz = rain + cloud;
+    a.d2[D2_FRZRAIN][d2] = frz;
+  }
+  if (a.d2[D2_HAIL_SFC]) a.d2[D2_HAIL_SFC][d2] = R(0);
+  if (a.d2[D2_HAIL_COL]) a.d2[D2_HAIL_COL][d2] = R(0);
+}
+
+}  // namespace tempo

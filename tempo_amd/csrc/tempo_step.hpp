@@ -117,8 +117,9 @@ struct StepArgs {
   const int* nlev;   // per column active level count or nullptr
   int ni, nj, nz, ncol;
   long long ldk, ldj, dldk, dldj, d2ldj;
-  R* sr;             // SR_N x nz x ncol
-  double* sd;        // SD_N x nz x ncol
+  R* sr;             // SR_N planes
+  double* sd;        // SD_N planes
+  size_t plane;      // elements per scratch plane: nz x ncol rounded up to kScrW columns
   unsigned* pf;      // nz x ncol point flags
   unsigned* cf;      // ncol column flags
   double* precip;    // 5 x ncol: rain, graupel, snow, ice, cloud
@@ -130,8 +131,21 @@ struct StepArgs {
   int first_step;
 };
 
+// launch shape of one step (host-side choice, see tempo_hip_tune)
+struct LaunchCfg {
+  int sed_wave = 1;   // sedimentation: 1 = one wavefront per column (nz <= 128), 0 = one lane per column
+  int block_a = 256;  // threads per block of the level-local kernels (64, 128 or 256)
+  int block_c = 256;
+};
+
 #define TEMPO_IDX_STATE(a, c, k) (size_t((c) % (a).ni) + size_t(k) * (a).ldk + size_t((c) / (a).ni) * (a).ldj)
-#define TEMPO_IDX_SCR(a, c, k) (size_t(k) * (a).ncol + size_t(c))
+// scratch planes are column-tiled: groups of kScrW consecutive columns hold
+// their nz levels contiguously, (c / kScrW, k, c % kScrW), so that a level-
+// local kernel (lanes over columns) writes whole segments and a column
+// kernel reads one contiguous block per group
+constexpr int kScrW = 8;
+#define TEMPO_IDX_SCR(a, c, k) \
+  ((size_t((c) / kScrW) * size_t((a).nz) + size_t(k)) * kScrW + size_t((c) % kScrW))
 
 // ------------------------------------------------ level-local physics ----
 // Per-level working state: the scalars of one (column, level) point.

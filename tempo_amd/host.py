@@ -26,6 +26,7 @@ CCN_PATH = os.path.join(HERE, "data", "ccn_activate.bin")
 OK, E_ARG, E_UNSUPPORTED, E_TABLE, E_HIP, E_NOMEM = 0, -1, -2, -3, -4, -5
 FP32, FP64 = 4, 8
 TABLE_QG, TABLE_QS, TABLE_FZ = 1, 2, 3
+TUNE_SED_WAVE, TUNE_BLOCK_A, TUNE_BLOCK_C = 1, 2, 3
 TABLE_SIZES = {TABLE_QG: 5 * 16_867_449, TABLE_QS: 12 * 455_877, TABLE_FZ: 4 * 3_388_275 + 2 * 9_157_500}
 
 
@@ -95,6 +96,7 @@ def lib():
         L.tempo_hip_table_build_ms.restype = ctypes.c_double
         L.tempo_hip_substeps.argtypes = [vp, vp, ctypes.c_size_t]
         L.tempo_hip_profile.argtypes = [vp, ctypes.c_int]
+        L.tempo_hip_tune.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.tempo_hip_profile_read.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.tempo_hip_last_error.restype = ctypes.c_char_p
         L.tempo_hip_finalize.argtypes = [vp]
@@ -239,6 +241,10 @@ class TempoHip:
     def write_tables(self, directory):
         os.makedirs(directory, exist_ok=True)
         _check(lib().tempo_hip_table_write(self._ctx, directory.encode()))
+
+    def tune(self, precision, key, value):
+        """Launch-shape setting for one precision (FP32/FP64); results are unchanged."""
+        _check(lib().tempo_hip_tune(self._ctx, int(precision), int(key), int(value)))
 
     def profile(self, enable=True):
         """Record per-phase HIP events on subsequent runs."""

@@ -321,3 +321,48 @@ def test_gpu_table_file_roundtrip(hip):
         read.close()
     for k in a:
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_gpu_launch_shapes_give_identical_results(hip, dtype):
+    """Every tuning setting (sedimentation kernel, block sizes) produces the
+    same bits: the wavefront-per-column sedimentation performs the serial
+    kernel's arithmetic level by level."""
+    from tempo_amd import generator as gen
+    tdt = getattr(torch, dtype)
+    prec = hip.FP32 if dtype == "float32" else hip.FP64
+    for wl, ncol, flags in (("ragged", 300, dict(aerosolaware=False, hailaware=True)),
+                            ("multistep", 257, dict(aerosolaware=True, hailaware=True))):
+        st, m = gen.generate(wl, ncol=ncol, col0=123)
+        ctx = context(hip, flags)
+        nl = None if "nlev" not in m else torch.tensor(m["nlev"], dtype=torch.int32, device="cuda")
+        results = []
+        for sed, ba, bc in ((0, 256, 256), (1, 256, 256), (1, 64, 128), (1, 128, 64)):
+            ctx.tune(prec, hip.TUNE_SED_WAVE, sed)
+            ctx.tune(prec, hip.TUNE_BLOCK_A, ba)
+            ctx.tune(prec, hip.TUNE_BLOCK_C, bc)
+            dev = {k: torch.tensor(v, dtype=tdt, device="cuda") for k, v in st.items()}
+            d = ctx.run(dev, m["dt"], m["itimestep"], nlev=nl)
+            torch.cuda.synchronize()
+            sub = ctx.substeps(ncol)
+            results.append(({k: v.cpu().numpy() for k, v in dev.items()},
+                            {k: v.cpu().numpy() for k, v in d.items()}, sub))
+        ctx.tune(prec, hip.TUNE_SED_WAVE, 1)
+        ctx.tune(prec, hip.TUNE_BLOCK_A, 256)
+        ctx.tune(prec, hip.TUNE_BLOCK_C, 256)
+        base = results[0]
+        for r in results[1:]:
+            for k in base[0]:
+                np.testing.assert_array_equal(r[0][k], base[0][k], err_msg=f"{wl} {k}")
+            for k in base[1]:
+                np.testing.assert_array_equal(r[1][k], base[1][k], err_msg=f"{wl} {k}")
+            np.testing.assert_array_equal(r[2], base[2], err_msg=f"{wl} substeps")
+        print(wl, dtype, "max substeps", base[2][0::2].max(), "identical over", len(results), "settings")
+
+
+def test_gpu_tune_rejects_bad_values(hip):
+    ctx = context(hip, dict(aerosolaware=False, hailaware=False))
+    for key, val in ((hip.TUNE_SED_WAVE, 2), (hip.TUNE_BLOCK_A, 96), (hip.TUNE_BLOCK_C, 512), (99, 1)):
+        with pytest.raises(hip.TempoError) as e:
+            ctx.tune(hip.FP32, key, val)
+        assert e.value.code == hip.E_ARG
