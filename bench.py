@@ -64,6 +64,50 @@ def cpu_baseline(ctx, cols, workers):
     return {k: out[k] for k in ("value", "unit", "cores", "kind", "sample")}, out
 
 
+def adjoint_timing(steps):
+    """Differentiable step, 16,384 columns x 64 levels, FP32: forward alone
+    vs forward + adjoint of the loss (weighted surface precipitation +
+    reflectivity at levels 5, 20, 40) through torch.autograd."""
+    import torch
+    from tempo_amd import generator as gen
+    from tempo_amd import host
+    from tempo_amd.autograd import tempo_step
+    ctx = host.TempoHip(aerosolaware_flag=1, hailaware_flag=0)
+    st, m = gen.generate_torch("adjoint", device="cuda", dtype=torch.float32)
+    nz, ncol = st["p"].shape
+    g = torch.Generator(device="cuda").manual_seed(3)
+    aw = torch.rand(ncol, device="cuda", generator=g)
+    bw = torch.rand((3, ncol), device="cuda", generator=g)
+    xn = ("t", "qv", "qc", "qr", "qi", "qs", "qg", "nc", "nr")
+
+    def fwd(grad):
+        x = {k: (v.clone().requires_grad_(True) if grad and k in xn else v) for k, v in st.items()}
+        out = tempo_step(ctx, m["dt"], m["itimestep"], x)
+        p = out["rain_precip"] + out["snow_liquid_equiv_precip"] + out["graupel_liquid_equiv_precip"] + \
+            out["ice_liquid_equiv_precip"]
+        loss = (aw * p).sum() + (bw * out["refl10cm"][[4, 19, 39]]).sum()
+        if grad:
+            loss.backward()
+        return loss
+
+    def timed(grad):
+        fwd(grad)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            with torch.set_grad_enabled(grad):
+                fwd(grad)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+    f_ms = timed(False)
+    fa_ms = timed(True)
+    return {"workload": f"{ncol} columns x {nz} levels, fp32, loss = weighted precipitation + dBZ at 3 levels",
+            "forward_ms": round(f_ms, 3), "forward_adjoint_ms": round(fa_ms, 3), "ratio": round(fa_ms / f_ms, 2),
+            "mcl_per_s_forward_adjoint": round(ncol * nz / (fa_ms * 1e-3) / 1e6, 2)}
+
+
 def copy_probe(gib):
     """A plain device copy of known size, for scaling rocprof's FETCH_SIZE and
     WRITE_SIZE counters."""
@@ -88,6 +132,8 @@ def main():
     ap.add_argument("--cpu-workers", type=int, default=0, help="0: the host's cores (at most 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=float, default=0.0, help="run only a copy of this many GiB")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint timing")
+    ap.add_argument("--adjoint-steps", type=int, default=3)
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -182,6 +228,8 @@ def main():
         "phase_ms": {PHASES[i]: round(float(phase_ms[i]), 4) for i in range(4)},
         "cpu_baseline": None,
     }
+    if not a.no_adjoint:
+        out["adjoint"] = adjoint_timing(a.adjoint_steps)
     if not a.no_cpu_baseline:
         cb, raw = cpu_baseline(ctx, a.cpu_cols, a.cpu_workers)
         out["cpu_baseline"] = cb

@@ -126,6 +126,31 @@ int tempo_hip_run(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip
 int tempo_hip_run_host(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_diags *d,
                        const tempo_hip_dims *dims, double dt, int itimestep);
 
+/* Tangent-linear step (no counterpart in the reference): the step of
+ * tempo_hip_run on f (updated in place) together with its derivative in the
+ * direction f_tl -- device arrays laid out like f, holding the tangents of
+ * the inputs (NULL entries: zero tangent) and overwritten with the tangents
+ * of the outputs; d_tl (may be NULL) receives the tangents of the requested
+ * diagnostics.  Thresholds, clamps, table indices and substep counts take
+ * their forward branches (branch-frozen derivatives); in FP64 the values
+ * are those of tempo_hip_run bit for bit. */
+int tempo_hip_tlm(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_fields *f_tl,
+                  const tempo_hip_diags *d, const tempo_hip_diags *d_tl, const tempo_hip_dims *dims, double dt,
+                  int itimestep, int precision, void *stream);
+
+/* Adjoint of one step (no counterpart in the reference).  f: the input state
+ * (device, not modified; t must be given).  ybar: cotangents of the outputs
+ * t, qv, qc, qr, qi, qs, qg, ni, nr, nc, nwfa, nifa, ng, qb (NULL entries:
+ * zero); dbar: cotangents of rain_precip, ice_liquid_equiv_precip,
+ * snow_liquid_equiv_precip, graupel_liquid_equiv_precip and refl10cm (other
+ * entries ignored).  Writes dL/d(input) of t, qv, qc, qr, qi, qs, qg, nc and
+ * nr into the matching entries of xbar (NULL entries are skipped), with the
+ * branches frozen as in tempo_hip_tlm.  Working memory (about 3 kB per
+ * column-level) is allocated on first use. */
+int tempo_hip_adjoint(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_fields *ybar,
+                      const tempo_hip_diags *dbar, const tempo_hip_fields *xbar, const tempo_hip_dims *dims, double dt,
+                      int itimestep, int precision, void *stream);
+
 /* Copies a lookup table (FP64, reference file order, native endian) to host
  * memory; n must equal the table's element count. */
 int tempo_hip_table_export(tempo_hip_ctx *ctx, int table, double *host_out, size_t n);

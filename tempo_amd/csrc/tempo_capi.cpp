@@ -10,11 +10,26 @@
 
 #include "tempo_host.hpp"
 #include "tempo_step.hpp"
+#include "tempo_adjoint.hpp"
 
 namespace tempo {
 // defined in tempo_step_f32.hip / tempo_step_f64.hip / tempo_tables.hip
 hipError_t launch_step(const StepArgs<float>& a, const LaunchCfg& lc, double* vtbuf, hipStream_t s, hipEvent_t* ev);
 hipError_t launch_step(const StepArgs<double>& a, const LaunchCfg& lc, double* vtbuf, hipStream_t s, hipEvent_t* ev);
+hipError_t launch_step_tlm(const StepArgs<Dual<float>>& a, Dual<double>* vtbuf, hipStream_t s);
+hipError_t launch_step_tlm(const StepArgs<Dual<double>>& a, Dual<double>* vtbuf, hipStream_t s);
+template <class T>
+hipError_t launch_dual_pack(const T* v, const T* t, const DualGeom& g, Dual<T>* out, hipStream_t s);
+template <class T>
+hipError_t launch_dual_unpack(const Dual<T>* in, const DualGeom& g, T* v, T* t, hipStream_t s);
+hipError_t launch_adjoint(const StepArgs<Dual<float>>& a, const AdjArgs& g, Dual<double>* vtbuf, const void* inputs,
+                          void* work_state, size_t state_bytes, const bool* have_x, hipStream_t s);
+hipError_t launch_adjoint(const StepArgs<Dual<double>>& a, const AdjArgs& g, Dual<double>* vtbuf, const void* inputs,
+                          void* work_state, size_t state_bytes, const bool* have_x, hipStream_t s);
+template <class T>
+hipError_t launch_to_compact(const T* src, const DualGeom& g, double* dst, hipStream_t s);
+template <class T>
+hipError_t launch_from_compact(const double* src, const DualGeom& g, T* dst, hipStream_t s);
 hipError_t launch_table_qg(const double*, const double*, const double*, const double*, double*, hipStream_t);
 hipError_t launch_table_qs(const double*, const double*, const double*, const double*, double*, hipStream_t);
 hipError_t launch_table_fz(const double*, const double*, const double*, const double*, double, const double*,
@@ -124,7 +139,8 @@ struct tempo_hip_ctx {
   DevBuf efrw64, efsw64, wev64, iaus64, efrw32, efsw32, wev32, iaus32;
   // scratch
   DevBuf sr, sd, pf, cf_, precip, ksub, vtbuf;
-  size_t cap_plane = 0, cap_ncol = 0;
+  DevBuf tl_state, tl_diag;  // dual (value, tangent) copies for the tangent-linear step
+  DevBuf adj_in, adj_mem;    // adjoint: packed inputs, cotangent/Jacobian planes
   int last_ncol = 0;
   // host-array staging
   std::vector<DevBuf> stage;
@@ -296,73 +312,46 @@ int pack_tables(tempo_hip_ctx* x) {
 
 size_t scratch_plane(size_t ncol, size_t nz) { return (ncol + kScrW - 1) / kScrW * kScrW * nz; }
 
-int ensure_scratch(tempo_hip_ctx* x, size_t ncol, size_t nz) {
+// per-point scratch for ncol columns of nz levels; br / bw: bytes of one
+// working-precision / wide value (8 for the plain step, 16 for duals)
+int ensure_scratch(tempo_hip_ctx* x, size_t ncol, size_t nz, size_t br = 8, size_t bw = 8) {
   const size_t plane = scratch_plane(ncol, nz);
-  if (plane > x->cap_plane) {
-    HIP_TRY(x->sr.ensure(SR_N * plane * sizeof(double)));
-    HIP_TRY(x->sd.ensure(SD_N * plane * sizeof(double)));
-    HIP_TRY(x->pf.ensure(plane * sizeof(unsigned)));
-    HIP_TRY(x->vtbuf.ensure(3 * plane * sizeof(double)));
-    x->cap_plane = plane;
-  }
-  if (ncol > x->cap_ncol) {
-    HIP_TRY(x->cf_.ensure(ncol * sizeof(unsigned)));
-    HIP_TRY(x->precip.ensure(5 * ncol * sizeof(double)));
-    HIP_TRY(x->ksub.ensure(10 * ncol * sizeof(int)));
-    x->cap_ncol = ncol;
-  }
+  HIP_TRY(x->sr.ensure(SR_N * plane * br));
+  HIP_TRY(x->sd.ensure(SD_N * plane * bw));
+  HIP_TRY(x->pf.ensure(plane * sizeof(unsigned)));
+  HIP_TRY(x->vtbuf.ensure(3 * plane * bw));
+  HIP_TRY(x->cf_.ensure(ncol * sizeof(unsigned)));
+  HIP_TRY(x->precip.ensure(5 * ncol * bw));
+  HIP_TRY(x->ksub.ensure(10 * ncol * sizeof(int)));
   return TEMPO_HIP_OK;
 }
 
+// scratch, tables, constants and flags of one launch (a.ncol, a.nz set)
 template <class R>
-int run_impl(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags* d, const tempo_hip_dims* dm,
-             double dt, int itimestep, hipStream_t s) {
-  StepArgs<R> a;
-  std::memset(&a, 0, sizeof(a));
-  void* const fp[F_NFIELDS] = {f->t,  f->p,  f->w,  f->dz, f->qv, f->qc,   f->qr,   f->qi, f->qs, f->qg,
-                               f->ni, f->nr, f->nc, f->nwfa, f->nifa, f->ng, f->qb, f->th, f->pii};
-  for (int i = 0; i < F_NFIELDS; ++i) a.f[i] = static_cast<R*>(fp[i]);
-  if (d) {
-    void* const d2[D2_N] = {d->rain_precip, d->ice_liquid_equiv_precip, d->snow_liquid_equiv_precip,
-                            d->graupel_liquid_equiv_precip, d->frozen_fraction, d->frz_rain_precip,
-                            d->max_hail_diameter_sfc, d->max_hail_diameter_column};
-    void* const d3[D3_N] = {d->refl10cm, d->re_cloud, d->re_ice, d->re_snow, d->rain_med_vol_diam,
-                            d->graupel_med_vol_diam, d->cloud_number_mixing_ratio};
-    for (int i = 0; i < D2_N; ++i) a.d2[i] = static_cast<R*>(d2[i]);
-    for (int i = 0; i < D3_N; ++i) a.d3[i] = static_cast<R*>(d3[i]);
-  }
-  a.land = f->land;
-  a.nlev = dm->nlev;
-  a.ni = dm->ni;
-  a.nj = dm->nj;
-  a.nz = dm->nz;
-  a.ncol = dm->ni * dm->nj;
-  a.ldk = dm->ldk;
-  a.ldj = dm->ldj;
-  a.dldk = dm->dldk;
-  a.dldj = dm->dldj;
-  a.d2ldj = dm->d2ldj;
-  int rc = ensure_scratch(x, size_t(a.ncol), size_t(a.nz));
+int bind_common(tempo_hip_ctx* x, StepArgs<R>& a, double dt, int itimestep) {
+  using S = S_t<R>;
+  using W = W_t<R>;
+  int rc = ensure_scratch(x, size_t(a.ncol), size_t(a.nz), sizeof(R), sizeof(W));
   if (rc) return rc;
   a.sr = x->sr.as<R>();
   a.plane = scratch_plane(size_t(a.ncol), size_t(a.nz));
-  a.sd = x->sd.as<double>();
+  a.sd = x->sd.as<W>();
   a.pf = x->pf.as<unsigned>();
   a.cf = x->cf_.as<unsigned>();
-  a.precip = x->precip.as<double>();
+  a.precip = x->precip.as<W>();
   a.ksub = x->ksub.as<int>();
   x->last_ncol = a.ncol;
-  const bool r64 = sizeof(R) == 8;
-  a.tab.qg = static_cast<const R*>(r64 ? x->qg64.p : x->qg32.p);
-  a.tab.qs = static_cast<const R*>(r64 ? x->qs64.p : x->qs32.p);
-  a.tab.fzr = static_cast<const R*>(r64 ? x->fzr64.p : x->fzr32.p);
-  a.tab.fzc = static_cast<const R*>(r64 ? x->fzc64.p : x->fzc32.p);
+  const bool r64 = sizeof(S) == 8;
+  a.tab.qg = static_cast<const S*>(r64 ? x->qg64.p : x->qg32.p);
+  a.tab.qs = static_cast<const S*>(r64 ? x->qs64.p : x->qs32.p);
+  a.tab.fzr = static_cast<const S*>(r64 ? x->fzr64.p : x->fzr32.p);
+  a.tab.fzc = static_cast<const S*>(r64 ? x->fzc64.p : x->fzc32.p);
   a.tab.ccn = x->ccn.as<float>();
-  a.cst = static_cast<const Const<R>*>(r64 ? x->cd_dev.p : x->cf_dev.p);
-  a.tab.efrw = static_cast<const R*>(r64 ? x->efrw64.p : x->efrw32.p);
-  a.tab.efsw = static_cast<const R*>(r64 ? x->efsw64.p : x->efsw32.p);
-  a.tab.tnc_wev = static_cast<const R*>(r64 ? x->wev64.p : x->wev32.p);
-  const R* iaus = static_cast<const R*>(r64 ? x->iaus64.p : x->iaus32.p);
+  a.cst = static_cast<const Const<S>*>(r64 ? x->cd_dev.p : x->cf_dev.p);
+  a.tab.efrw = static_cast<const S*>(r64 ? x->efrw64.p : x->efrw32.p);
+  a.tab.efsw = static_cast<const S*>(r64 ? x->efsw64.p : x->efsw32.p);
+  a.tab.tnc_wev = static_cast<const S*>(r64 ? x->wev64.p : x->wev32.p);
+  const S* iaus = static_cast<const S*>(r64 ? x->iaus64.p : x->iaus32.p);
   a.tab.tps_iaus = iaus;
   a.tab.tni_iaus = iaus + kNtbI * kNtbI1;
   a.tab.tpi_ide = iaus + 2 * kNtbI * kNtbI1;
@@ -382,6 +371,52 @@ int run_impl(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags*
   a.fl.nc_const = c.nc_option == 1 ? kNtCO : (c.nc_option == 2 ? c.nc_value : kNtCL);
   a.dt = dt;
   a.first_step = itimestep == 1;
+  return TEMPO_HIP_OK;
+}
+
+void field_ptrs(const tempo_hip_fields* f, void* (&fp)[F_NFIELDS]) {
+  void* const v[F_NFIELDS] = {f->t,  f->p,  f->w,  f->dz, f->qv, f->qc,   f->qr,   f->qi, f->qs, f->qg,
+                              f->ni, f->nr, f->nc, f->nwfa, f->nifa, f->ng, f->qb, f->th, f->pii};
+  for (int i = 0; i < F_NFIELDS; ++i) fp[i] = v[i];
+}
+
+void diag_ptrs(const tempo_hip_diags* d, void* (&d2)[D2_N], void* (&d3)[D3_N]) {
+  void* const v2[D2_N] = {d->rain_precip, d->ice_liquid_equiv_precip, d->snow_liquid_equiv_precip,
+                          d->graupel_liquid_equiv_precip, d->frozen_fraction, d->frz_rain_precip,
+                          d->max_hail_diameter_sfc, d->max_hail_diameter_column};
+  void* const v3[D3_N] = {d->refl10cm, d->re_cloud, d->re_ice, d->re_snow, d->rain_med_vol_diam,
+                          d->graupel_med_vol_diam, d->cloud_number_mixing_ratio};
+  for (int i = 0; i < D2_N; ++i) d2[i] = v2[i];
+  for (int i = 0; i < D3_N; ++i) d3[i] = v3[i];
+}
+
+template <class R>
+int run_impl(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags* d, const tempo_hip_dims* dm,
+             double dt, int itimestep, hipStream_t s) {
+  StepArgs<R> a;
+  std::memset(&a, 0, sizeof(a));
+  void* fp[F_NFIELDS];
+  field_ptrs(f, fp);
+  for (int i = 0; i < F_NFIELDS; ++i) a.f[i] = static_cast<R*>(fp[i]);
+  if (d) {
+    void *d2[D2_N], *d3[D3_N];
+    diag_ptrs(d, d2, d3);
+    for (int i = 0; i < D2_N; ++i) a.d2[i] = static_cast<R*>(d2[i]);
+    for (int i = 0; i < D3_N; ++i) a.d3[i] = static_cast<R*>(d3[i]);
+  }
+  a.land = f->land;
+  a.nlev = dm->nlev;
+  a.ni = dm->ni;
+  a.nj = dm->nj;
+  a.nz = dm->nz;
+  a.ncol = dm->ni * dm->nj;
+  a.ldk = dm->ldk;
+  a.ldj = dm->ldj;
+  a.dldk = dm->dldk;
+  a.dldj = dm->dldj;
+  a.d2ldj = dm->d2ldj;
+  int rc = bind_common(x, a, dt, itimestep);
+  if (rc) return rc;
   hipEvent_t* ev = nullptr;
   if (x->profile) {
     for (int i = 0; i < 5; ++i) {
@@ -391,7 +426,139 @@ int run_impl(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags*
     }
     ev = &x->events[x->events.size() - 5];
   }
-  HIP_TRY(launch_step(a, x->launch[r64 ? 1 : 0], x->vtbuf.as<double>(), s, ev));
+  HIP_TRY(launch_step(a, x->launch[sizeof(R) == 8 ? 1 : 0], x->vtbuf.as<double>(), s, ev));
+  return TEMPO_HIP_OK;
+}
+
+// Tangent-linear step: fields and tangents are interleaved into dual
+// arrays (compact layout, column c = i + j*ni), stepped, and split back.
+template <class T>
+int run_tlm(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fields* ft, const tempo_hip_diags* d,
+            const tempo_hip_diags* dt_, const tempo_hip_dims* dm, double dt, int itimestep, hipStream_t s) {
+  using D = Dual<T>;
+  const int ncol = dm->ni * dm->nj, nz = dm->nz;
+  const size_t n3 = size_t(ncol) * nz;
+  HIP_TRY(x->tl_state.ensure(F_NFIELDS * n3 * sizeof(D)));
+  HIP_TRY(x->tl_diag.ensure((D2_N * size_t(ncol) + D3_N * n3) * sizeof(D)));
+  StepArgs<D> a;
+  std::memset(&a, 0, sizeof(a));
+  void *fp[F_NFIELDS], *tp[F_NFIELDS];
+  field_ptrs(f, fp);
+  field_ptrs(ft, tp);
+  D* st = x->tl_state.as<D>();
+  DualGeom g3{dm->ni, dm->nj, nz, dm->ldk, dm->ldj};
+  for (int i = 0; i < F_NFIELDS; ++i) {
+    if (!fp[i]) continue;
+    a.f[i] = st + size_t(i) * n3;
+    HIP_TRY(launch_dual_pack(static_cast<const T*>(fp[i]), static_cast<const T*>(tp[i]), g3, a.f[i], s));
+  }
+  void *d2[D2_N] = {}, *d3[D3_N] = {}, *t2[D2_N] = {}, *t3[D3_N] = {};
+  if (d) diag_ptrs(d, d2, d3);
+  if (dt_) diag_ptrs(dt_, t2, t3);
+  D* dg = x->tl_diag.as<D>();
+  for (int i = 0; i < D2_N; ++i)
+    if (d2[i] || t2[i]) a.d2[i] = dg + size_t(i) * ncol;
+  for (int i = 0; i < D3_N; ++i)
+    if (d3[i] || t3[i]) a.d3[i] = dg + D2_N * size_t(ncol) + size_t(i) * n3;
+  a.land = f->land;
+  a.nlev = dm->nlev;
+  a.ni = ncol;
+  a.nj = 1;
+  a.nz = nz;
+  a.ncol = ncol;
+  a.ldk = ncol;
+  a.ldj = (long long)n3;
+  a.dldk = ncol;
+  a.dldj = (long long)n3;
+  a.d2ldj = ncol;
+  int rc = bind_common(x, a, dt, itimestep);
+  if (rc) return rc;
+  HIP_TRY(launch_step_tlm(a, x->vtbuf.as<Dual<double>>(), s));
+  for (int i = 0; i < F_NFIELDS; ++i)
+    if (fp[i]) HIP_TRY(launch_dual_unpack(a.f[i], g3, static_cast<T*>(fp[i]), static_cast<T*>(tp[i]), s));
+  DualGeom g2{dm->ni, dm->nj, 1, 0, dm->d2ldj};
+  DualGeom gd3{dm->ni, dm->nj, nz, dm->dldk, dm->dldj};
+  for (int i = 0; i < D2_N; ++i)
+    if (a.d2[i]) HIP_TRY(launch_dual_unpack(a.d2[i], g2, static_cast<T*>(d2[i]), static_cast<T*>(t2[i]), s));
+  for (int i = 0; i < D3_N; ++i)
+    if (a.d3[i]) HIP_TRY(launch_dual_unpack(a.d3[i], gd3, static_cast<T*>(d3[i]), static_cast<T*>(t3[i]), s));
+  return TEMPO_HIP_OK;
+}
+
+// Adjoint step: tangent passes of the dual step plus S^T (tempo_adjoint.hpp).
+template <class T>
+int run_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fields* yb, const tempo_hip_diags* db,
+                const tempo_hip_fields* xb, const tempo_hip_dims* dm, double dt, int itimestep, hipStream_t s) {
+  using D = Dual<T>;
+  if (!f->t) return fail(TEMPO_HIP_E_UNSUPPORTED, "the adjoint takes temperature input t (not th/pii)");
+  const int ncol = dm->ni * dm->nj, nz = dm->nz;
+  const size_t n3 = size_t(ncol) * nz, sb = F_NFIELDS * n3 * sizeof(D);
+  const size_t plane = scratch_plane(size_t(ncol), size_t(nz));
+  const size_t nd = (AY_N + AX_N + AX_N * AZ_N + 10 + AZ_N) * n3 + 4 * size_t(ncol) + AW_N * plane;
+  HIP_TRY(x->tl_state.ensure(sb));
+  HIP_TRY(x->adj_in.ensure(sb));
+  HIP_TRY(x->tl_diag.ensure((D2_N * size_t(ncol) + D3_N * n3) * sizeof(D)));
+  HIP_TRY(x->adj_mem.ensure(nd * sizeof(double)));
+  double* mem = x->adj_mem.as<double>();
+  AdjArgs g;
+  double* ybar = mem;
+  g.ybar = ybar;
+  g.xbar = ybar + AY_N * n3;
+  g.ja = g.xbar + AX_N * n3;
+  g.ubar = g.ja + AX_N * AZ_N * n3;
+  g.zbar = g.ubar + 10 * n3;
+  double* pbar = g.zbar + AZ_N * n3;
+  g.pbar = pbar;
+  g.work = pbar + 4 * size_t(ncol);
+
+  StepArgs<D> a;
+  std::memset(&a, 0, sizeof(a));
+  void* fp[F_NFIELDS];
+  field_ptrs(f, fp);
+  D* in = x->adj_in.as<D>();
+  D* st = x->tl_state.as<D>();
+  const DualGeom g3{dm->ni, dm->nj, nz, dm->ldk, dm->ldj};
+  for (int i = 0; i < F_NFIELDS; ++i) {
+    if (!fp[i]) continue;
+    HIP_TRY(launch_dual_pack(static_cast<const T*>(fp[i]), static_cast<const T*>(nullptr), g3, in + size_t(i) * n3, s));
+    a.f[i] = st + size_t(i) * n3;
+  }
+  // output cotangents
+  void* yp[F_NFIELDS] = {};
+  if (yb) field_ptrs(yb, yp);
+  for (int q = 0; q < AY_REFL; ++q)
+    HIP_TRY(launch_to_compact(static_cast<const T*>(a.f[adj_y_field(q)] ? yp[adj_y_field(q)] : nullptr), g3,
+                              ybar + size_t(q) * n3, s));
+  void *d2[D2_N] = {}, *d3[D3_N] = {};
+  if (db) diag_ptrs(db, d2, d3);
+  const DualGeom gd3{dm->ni, dm->nj, nz, dm->dldk, dm->dldj};
+  const DualGeom g2{dm->ni, dm->nj, 1, 0, dm->d2ldj};
+  HIP_TRY(launch_to_compact(static_cast<const T*>(d3[D3_REFL]), gd3, ybar + size_t(AY_REFL) * n3, s));
+  const int pmap[4] = {D2_RAIN, D2_GRAUPEL, D2_SNOW, D2_ICE};  // S^T order: rain, graupel, snow, ice
+  for (int q = 0; q < 4; ++q)
+    HIP_TRY(launch_to_compact(static_cast<const T*>(d2[pmap[q]]), g2, pbar + size_t(q) * ncol, s));
+  a.d3[D3_REFL] = x->tl_diag.as<D>() + D2_N * size_t(ncol);
+  a.land = f->land;
+  a.nlev = dm->nlev;
+  a.ni = ncol;
+  a.nj = 1;
+  a.nz = nz;
+  a.ncol = ncol;
+  a.ldk = ncol;
+  a.ldj = (long long)n3;
+  a.dldk = ncol;
+  a.dldj = (long long)n3;
+  a.d2ldj = ncol;
+  int rc = bind_common(x, a, dt, itimestep);
+  if (rc) return rc;
+  bool have_x[AX_N];
+  for (int i = 0; i < AX_N; ++i) have_x[i] = fp[adj_x_field(i)] != nullptr;
+  HIP_TRY(launch_adjoint(a, g, x->vtbuf.as<Dual<double>>(), in, st, sb, have_x, s));
+  void* xp[F_NFIELDS] = {};
+  if (xb) field_ptrs(xb, xp);
+  for (int i = 0; i < AX_N; ++i)
+    if (xp[adj_x_field(i)])
+      HIP_TRY(launch_from_compact(g.xbar + size_t(i) * n3, g3, static_cast<T*>(xp[adj_x_field(i)]), s));
   return TEMPO_HIP_OK;
 }
 
@@ -560,6 +727,32 @@ int tempo_hip_table_write(tempo_hip_ctx* x, const char* dir) {
     if (w != it.n) return fail(TEMPO_HIP_E_TABLE, "short write " + path);
   }
   return TEMPO_HIP_OK;
+}
+
+int tempo_hip_tlm(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fields* f_tl, const tempo_hip_diags* d,
+                  const tempo_hip_diags* d_tl, const tempo_hip_dims* dm, double dt, int itimestep, int precision,
+                  void* stream) {
+  if (!x || !f_tl) return fail(TEMPO_HIP_E_ARG, "NULL context or tangent fields");
+  int rc = check_run_args(f, dm, dt);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (precision == TEMPO_HIP_FP32) return run_tlm<float>(x, f, f_tl, d, d_tl, dm, dt, itimestep, s);
+  if (precision == TEMPO_HIP_FP64) return run_tlm<double>(x, f, f_tl, d, d_tl, dm, dt, itimestep, s);
+  return fail(TEMPO_HIP_E_ARG, "precision must be TEMPO_HIP_FP32 or TEMPO_HIP_FP64");
+}
+
+int tempo_hip_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fields* ybar,
+                      const tempo_hip_diags* dbar, const tempo_hip_fields* xbar, const tempo_hip_dims* dm, double dt,
+                      int itimestep, int precision, void* stream) {
+  if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
+  int rc = check_run_args(f, dm, dt);
+  if (rc) return rc;
+  if (!x->cfg.refl10cm_flag && dbar && dbar->refl10cm)
+    return fail(TEMPO_HIP_E_ARG, "reflectivity cotangent given but refl10cm_flag is off");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (precision == TEMPO_HIP_FP32) return run_adjoint<float>(x, f, ybar, dbar, xbar, dm, dt, itimestep, s);
+  if (precision == TEMPO_HIP_FP64) return run_adjoint<double>(x, f, ybar, dbar, xbar, dm, dt, itimestep, s);
+  return fail(TEMPO_HIP_E_ARG, "precision must be TEMPO_HIP_FP32 or TEMPO_HIP_FP64");
 }
 
 int tempo_hip_tune(tempo_hip_ctx* x, int precision, int key, int value) {

@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(256) k_phase_a(StepArgs<R> a) {
   const int k = blockIdx.y;
   if (c >= a.ncol) return;
   if (a.nlev && k >= a.nlev[c]) return;
-  const Const<R>& C = *a.cst;
+  const Const<S_t<R>>& C = *a.cst;
   const size_t o = TEMPO_IDX_STATE(a, c, k);
   const size_t s = TEMPO_IDX_SCR(a, c, k);
   const size_t plane = a.plane;
@@ -296,49 +296,51 @@ __global__ void __launch_bounds__(256) k_phase_a(StepArgs<R> a) {
   sr[SR_VTNI * plane + s] = vtni;
   sr[SR_VTRC * plane + s] = vtrc;
   sr[SR_VTNC * plane + s] = vtnc;
-  double* sd = a.sd;
-  sd[SD_RR * plane + s] = L.rr;
-  sd[SD_NR * plane + s] = L.nr;
-  sd[SD_RG * plane + s] = L.rg;
-  sd[SD_NG * plane + s] = L.ng;
-  sd[SD_RB * plane + s] = L.rb;
-  sd[SD_RS * plane + s] = L.rs;
-  sd[SD_RI * plane + s] = L.ri;
-  sd[SD_NI * plane + s] = L.ni;
-  sd[SD_RC * plane + s] = L.rc;
-  sd[SD_NC * plane + s] = L.nc;
-  sd[SD_QRTEN * plane + s] = L.qrten;
-  sd[SD_NRTEN * plane + s] = L.nrten;
-  sd[SD_QGTEN * plane + s] = L.qgten;
-  sd[SD_NGTEN * plane + s] = L.ngten;
-  sd[SD_QBTEN * plane + s] = L.qbten;
-  sd[SD_QSTEN * plane + s] = L.qsten;
-  sd[SD_QITEN * plane + s] = L.qiten;
-  sd[SD_NITEN * plane + s] = L.niten;
-  sd[SD_QCTEN * plane + s] = L.qcten;
-  sd[SD_NCTEN * plane + s] = L.ncten;
+  using W = W_t<R>;
+  W* sd = a.sd;
+  sd[SD_RR * plane + s] = W(L.rr);
+  sd[SD_NR * plane + s] = W(L.nr);
+  sd[SD_RG * plane + s] = W(L.rg);
+  sd[SD_NG * plane + s] = W(L.ng);
+  sd[SD_RB * plane + s] = W(L.rb);
+  sd[SD_RS * plane + s] = W(L.rs);
+  sd[SD_RI * plane + s] = W(L.ri);
+  sd[SD_NI * plane + s] = W(L.ni);
+  sd[SD_RC * plane + s] = W(L.rc);
+  sd[SD_NC * plane + s] = W(L.nc);
+  sd[SD_QRTEN * plane + s] = W(L.qrten);
+  sd[SD_NRTEN * plane + s] = W(L.nrten);
+  sd[SD_QGTEN * plane + s] = W(L.qgten);
+  sd[SD_NGTEN * plane + s] = W(L.ngten);
+  sd[SD_QBTEN * plane + s] = W(L.qbten);
+  sd[SD_QSTEN * plane + s] = W(L.qsten);
+  sd[SD_QITEN * plane + s] = W(L.qiten);
+  sd[SD_NITEN * plane + s] = W(L.niten);
+  sd[SD_QCTEN * plane + s] = W(L.qcten);
+  sd[SD_NCTEN * plane + s] = W(L.ncten);
   a.pf[s] = pf;
 }
 
 // ----------------------------------------------------------- phase B ----
 // flux-form sedimentation of one field over one substep (:1516-1559), FP64
 template <class R>
-__device__ void sediment(const StepArgs<R>& a, int c, int nzc, double* xr, const double* vt, double* xten,
-                         double limit, int steps, int ktop, double* precip) {
+__device__ void sediment(const StepArgs<R>& a, int c, int nzc, W_t<R>* xr, const W_t<R>* vt, W_t<R>* xten,
+                         double limit, int steps, int ktop, W_t<R>* precip) {
+  using W = W_t<R>;
   const size_t plane = a.plane;
   const R* rho = a.sr + SR_RHO * plane;
   const double ost = 1.0 / double(steps), dt = a.dt;
-  double above = 0.0, s0 = 0.0;
+  W above = W(0.0), s0 = W(0.0);
   for (int k = nzc - 1; k >= 0; --k) {
     const size_t s = TEMPO_IDX_SCR(a, c, k);
-    const double sed = vt[s] * xr[s];
+    const W sed = vt[s] * xr[s];
     if (k == nzc - 1 || k <= ktop - 1) {
-      const double odz = 1.0 / double(a.f[F_DZ][TEMPO_IDX_STATE(a, c, k)]);
-      const double orho = 1.0 / double(rho[s]);
-      const double flux = (k == nzc - 1) ? -sed : (above - sed);
+      const W odz = 1.0 / W(a.f[F_DZ][TEMPO_IDX_STATE(a, c, k)]);
+      const W orho = 1.0 / W(rho[s]);
+      const W flux = (k == nzc - 1) ? -sed : (above - sed);
       xten[s] = xten[s] + flux * odz * ost * orho;
-      const double v = xr[s] + flux * odz * dt * ost;
-      xr[s] = v > limit ? v : limit;
+      const W v = xr[s] + flux * odz * dt * ost;
+      xr[s] = v > limit ? v : W(limit);
     }
     above = sed;
     if (k == 0) s0 = sed;
@@ -350,17 +352,18 @@ __device__ void sediment(const StepArgs<R>& a, int c, int nzc, double* xr, const
 // (:1768-1789 and the species variants); vt raw values are valid where the
 // mass exceeds r1.  Writes the filled profile to out (FP64).
 template <class R>
-__device__ void fill_fallspeed(const StepArgs<R>& a, int c, int nzc, const double* mass, const R* raw1, const R* raw2,
-                               double* out1, double* out2, bool use_max, int& substeps, int& ktop) {
+__device__ void fill_fallspeed(const StepArgs<R>& a, int c, int nzc, const W_t<R>* mass, const R* raw1, const R* raw2,
+                               W_t<R>* out1, W_t<R>* out2, bool use_max, int& substeps, int& ktop) {
+  using W = W_t<R>;
   const double dt = a.dt;
-  double v1 = 0.0, v2 = 0.0;
+  W v1 = W(0.0), v2 = W(0.0);
   substeps = 1;
   ktop = 1;
   for (int k = nzc - 1; k >= 0; --k) {
     const size_t s = TEMPO_IDX_SCR(a, c, k);
     if (mass[s] > kR1) {
-      v1 = raw1[s];
-      if (raw2) v2 = raw2[s];
+      v1 = W(raw1[s]);
+      if (raw2) v2 = W(raw2[s]);
     }
     out1[s] = v1;
     if (out2) out2[s] = v2;
@@ -377,41 +380,42 @@ __device__ void fill_fallspeed(const StepArgs<R>& a, int c, int nzc, const doubl
 }
 
 template <class R>
-__global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, double* vtbuf) {
+__global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, W_t<R>* vtbuf) {
+  using W = W_t<R>;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.ncol) return;
   const int nzc = a.nlev ? min(max(a.nlev[c], 1), a.nz) : a.nz;
   const size_t plane = a.plane;
-  double pr[5] = {0, 0, 0, 0, 0};  // rain graupel snow ice cloud
+  W pr[5] = {W(0.0), W(0.0), W(0.0), W(0.0), W(0.0)};  // rain graupel snow ice cloud
   const unsigned cf = a.cf[c];
   int kst[10] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
   if (!(cf & CF_EXIT)) {
     unsigned any = 0;
     for (int k = 0; k < nzc; ++k) any |= a.pf[TEMPO_IDX_SCR(a, c, k)];
-    double* sd = a.sd;
+    W* sd = a.sd;
     const R* sr = a.sr;
-    double* v1 = vtbuf;           // filled fall speeds, FP64
-    double* v2 = vtbuf + plane;
-    double* vr = vtbuf + 2 * plane;  // filled rain mass fall speed kept for snow
+    W* v1 = vtbuf;           // filled fall speeds, FP64
+    W* v2 = vtbuf + plane;
+    W* vr = vtbuf + 2 * plane;  // filled rain mass fall speed kept for snow
     int steps, ktop;
     if (any & PF_QR) {
       fill_fallspeed(a, c, nzc, sd + SD_RR * plane, sr + SR_VTRR * plane, sr + SR_VTNR * plane, vr, v2, true, steps, ktop);
       for (int n = 0; n < steps; ++n) {
         sediment(a, c, nzc, sd + SD_RR * plane, vr, sd + SD_QRTEN * plane, kR1, steps, ktop, &pr[0]);
-        sediment(a, c, nzc, sd + SD_NR * plane, v2, sd + SD_NRTEN * plane, kR2, steps, ktop, (double*)nullptr);
+        sediment(a, c, nzc, sd + SD_NR * plane, v2, sd + SD_NRTEN * plane, kR2, steps, ktop, (W*)nullptr);
       }
       kst[0] = steps;
       kst[1] = ktop;
     } else {
-      for (int k = 0; k < nzc; ++k) vr[TEMPO_IDX_SCR(a, c, k)] = 0.0;
+      for (int k = 0; k < nzc; ++k) vr[TEMPO_IDX_SCR(a, c, k)] = W(0.0);
     }
     if (any & PF_QG) {
       fill_fallspeed(a, c, nzc, sd + SD_RG * plane, sr + SR_VTRG * plane, sr + SR_VTNG * plane, v1, v2, false, steps, ktop);
       const double rb_lim = kM3ToL * kR1 / 800.0;
       for (int n = 0; n < steps; ++n) {
         sediment(a, c, nzc, sd + SD_RG * plane, v1, sd + SD_QGTEN * plane, kR1, steps, ktop, &pr[1]);
-        sediment(a, c, nzc, sd + SD_NG * plane, v2, sd + SD_NGTEN * plane, kR2, steps, ktop, (double*)nullptr);
-        sediment(a, c, nzc, sd + SD_RB * plane, v1, sd + SD_QBTEN * plane, rb_lim, steps, ktop, (double*)nullptr);
+        sediment(a, c, nzc, sd + SD_NG * plane, v2, sd + SD_NGTEN * plane, kR2, steps, ktop, (W*)nullptr);
+        sediment(a, c, nzc, sd + SD_RB * plane, v1, sd + SD_QBTEN * plane, rb_lim, steps, ktop, (W*)nullptr);
       }
       kst[2] = steps;
       kst[3] = ktop;
@@ -420,19 +424,19 @@ __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, double* vtbuf) {
       // snow_fallspeed mixes in rain where snow melts (:1886-1891), using the
       // rain mass after rain sedimentation and the filled rain fall speed
       const double dt = a.dt;
-      double vprev = 0.0;
+      W vprev = W(0.0);
       steps = 1;
       ktop = 1;
       for (int k = nzc - 1; k >= 0; --k) {
         const size_t s = TEMPO_IDX_SCR(a, c, k);
-        const double rs = sd[SD_RS * plane + s];
+        const W rs = sd[SD_RS * plane + s];
         if (rs > kR1) {
           R v = sr[SR_VTS * plane + s];
           if (a.pf[s] & PF_SNOWMIX) {
             const R sr_ = R(rs / (rs + sd[SD_RR * plane + s]));
             v = v * sr_ + (R(1.0) - sr_) * R(vr[s]);
           }
-          vprev = v;
+          vprev = W(v);
         }
         v1[s] = vprev;
         const R vv = R(vprev);
@@ -452,7 +456,7 @@ __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, double* vtbuf) {
       // one call with steps = substeps (the reference applies 1/steps of the flux)
       fill_fallspeed(a, c, nzc, sd + SD_RI * plane, sr + SR_VTRI * plane, sr + SR_VTNI * plane, v1, v2, false, steps, ktop);
       sediment(a, c, nzc, sd + SD_RI * plane, v1, sd + SD_QITEN * plane, kR1, steps, ktop, &pr[3]);
-      sediment(a, c, nzc, sd + SD_NI * plane, v2, sd + SD_NITEN * plane, kR2, steps, ktop, (double*)nullptr);
+      sediment(a, c, nzc, sd + SD_NI * plane, v2, sd + SD_NITEN * plane, kR2, steps, ktop, (W*)nullptr);
       kst[6] = steps;
       kst[7] = ktop;
     }
@@ -467,11 +471,11 @@ __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, double* vtbuf) {
       }
       for (int k = 0; k < nzc; ++k) {
         const size_t s = TEMPO_IDX_SCR(a, c, k);
-        v1[s] = (k < kt) ? double(sr[SR_VTRC * plane + s]) : 0.0;
-        v2[s] = (k < kt) ? double(sr[SR_VTNC * plane + s]) : 0.0;
+        v1[s] = (k < kt) ? W(sr[SR_VTRC * plane + s]) : W(0.0);
+        v2[s] = (k < kt) ? W(sr[SR_VTNC * plane + s]) : W(0.0);
       }
       sediment(a, c, nzc, sd + SD_RC * plane, v1, sd + SD_QCTEN * plane, kR1, 1, kt, &pr[4]);
-      sediment(a, c, nzc, sd + SD_NC * plane, v2, sd + SD_NCTEN * plane, kR2, 1, kt, (double*)nullptr);
+      sediment(a, c, nzc, sd + SD_NC * plane, v2, sd + SD_NCTEN * plane, kR2, 1, kt, (W*)nullptr);
       kst[8] = 1;
       kst[9] = kt;
     }
@@ -505,6 +509,11 @@ __device__ __forceinline__ void atomic_max_pos(float* p, float v) {
 __device__ __forceinline__ void atomic_max_pos(double* p, double v) {
   atomicMax(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
 }
+// tangent-linear runs: the column maximum of the value only (no derivative)
+template <class T>
+__device__ __forceinline__ void atomic_max_pos(Dual<T>* p, Dual<T> v) {
+  atomic_max_pos(&p->v, v.v);
+}
 
 template <class R>
 __global__ void __launch_bounds__(256) k_phase_c(StepArgs<R> a) {
@@ -512,7 +521,7 @@ __global__ void __launch_bounds__(256) k_phase_c(StepArgs<R> a) {
   const int k = blockIdx.y;
   if (c >= a.ncol) return;
   if (a.nlev && k >= a.nlev[c]) return;
-  const Const<R>& C = *a.cst;
+  const Const<S_t<R>>& C = *a.cst;
   const size_t o = TEMPO_IDX_STATE(a, c, k);
   const size_t s = TEMPO_IDX_SCR(a, c, k);
   const size_t plane = a.plane;
@@ -529,7 +538,7 @@ __global__ void __launch_bounds__(256) k_phase_c(StepArgs<R> a) {
   }
   Lvl<R> L;
   const R* sr = a.sr;
-  const double* sd = a.sd;
+  const W_t<R>* sd = a.sd;
   L.temp = sr[SR_TEMP * plane + s];
   L.qv = sr[SR_QV * plane + s];
   L.rho = sr[SR_RHO * plane + s];
@@ -674,6 +683,57 @@ __global__ void __launch_bounds__(256) k_phase_c(StepArgs<R> a) {
   if (a.d3[D3_RE_C]) a.d3[D3_RE_C][d3] = re_c;
   if (a.d3[D3_RE_I]) a.d3[D3_RE_I][d3] = re_i;
   if (a.d3[D3_RE_S]) a.d3[D3_RE_S][d3] = re_s;
+}
+
+// ------------------------------------------------ tangent-linear I/O ----
+// caller arrays (value, tangent) <-> one compact dual array, element (c, k)
+// at c + k*ncol
+template <class T>
+__global__ void __launch_bounds__(256) k_dual_pack(const T* __restrict__ v, const T* __restrict__ t, DualGeom g,
+                                                   Dual<T>* __restrict__ out) {
+  const size_t ncol = size_t(g.ni) * g.nj;
+  const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= ncol * size_t(g.nk)) return;
+  const size_t c = e % ncol, k = e / ncol;
+  const size_t o = (c % g.ni) + k * g.ldk + (c / g.ni) * g.ldj;
+  out[e] = Dual<T>(v[o], t ? t[o] : T(0));
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_dual_unpack(const Dual<T>* __restrict__ in, DualGeom g, T* __restrict__ v,
+                                                     T* __restrict__ t) {
+  const size_t ncol = size_t(g.ni) * g.nj;
+  const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= ncol * size_t(g.nk)) return;
+  const size_t c = e % ncol, k = e / ncol;
+  const size_t o = (c % g.ni) + k * g.ldk + (c / g.ni) * g.ldj;
+  if (v) v[o] = in[e].v;
+  if (t) t[o] = in[e].d;
+}
+
+template <class T>
+hipError_t launch_dual_pack(const T* v, const T* t, const DualGeom& g, Dual<T>* out, hipStream_t s) {
+  const size_t n = size_t(g.ni) * g.nj * g.nk;
+  hipLaunchKernelGGL(k_dual_pack<T>, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, v, t, g, out);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_dual_unpack(const Dual<T>* in, const DualGeom& g, T* v, T* t, hipStream_t s) {
+  const size_t n = size_t(g.ni) * g.nj * g.nk;
+  hipLaunchKernelGGL(k_dual_unpack<T>, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, in, g, v, t);
+  return hipGetLastError();
+}
+
+// the four launches of one tangent-linear step (sedimentation one lane per column)
+template <class D>
+hipError_t launch_step_tlm_impl(const StepArgs<D>& a, Dual<double>* vtbuf, hipStream_t s) {
+  const unsigned gc = unsigned((a.ncol + 255) / 256);
+  hipLaunchKernelGGL(k_prepass<D>, dim3(gc), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_phase_a<D>, dim3(gc, unsigned(a.nz)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_phase_b<D>, dim3(unsigned((a.ncol + 127) / 128)), dim3(128), 0, s, a, vtbuf);
+  hipLaunchKernelGGL(k_phase_c<D>, dim3(gc, unsigned(a.nz)), dim3(256), 0, s, a);
+  return hipGetLastError();
 }
 
 }  // namespace tempo

@@ -26,6 +26,7 @@
 
 #include "tempo_const.hpp"
 #include "tempo_crmath.hpp"
+#include "tempo_dual.hpp"
 
 namespace tempo {
 
@@ -55,6 +56,48 @@ template <class T>
 __device__ __forceinline__ int nint_(T x) { return int(x < T(0) ? x - T(0.5) : x + T(0.5)); }
 template <class T>
 __device__ __forceinline__ int nint_exact(T x) { return int(round(double(x))); }
+template <class T>
+__device__ __forceinline__ int nint_exact(Dual<T> x) { return nint_exact(x.v); }
+using ::copysign;
+template <class T>
+__device__ __forceinline__ Dual<T> copysign(Dual<T> a, Dual<T> b) {
+  const T v = copysign(a.v, b.v);
+  return {v, (v == a.v) ? a.d : -a.d};
+}
+
+// derivatives of the library functions (tangent-linear step)
+template <class T>
+__device__ __forceinline__ Dual<T> mpow(Dual<T> x, Dual<T> y) {
+  const T p = mpow(x.v, y.v);
+  T d = T(0);
+  if (x.v > T(0)) {
+    d = p * (y.v * x.d / x.v);
+    if (y.d != T(0)) d = d + p * mlog(x.v) * y.d;
+  }
+  return {p, d};
+}
+template <class T, class U>
+__device__ __forceinline__ Dual<T> mpow(Dual<T> x, U y) { return mpow(x, Dual<T>(T(y))); }
+template <class T, class U>
+__device__ __forceinline__ Dual<T> mpow(U x, Dual<T> y) { return mpow(Dual<T>(T(x)), y); }
+template <class T>
+__device__ __forceinline__ Dual<T> mexp(Dual<T> x) {
+  const T e = mexp(x.v);
+  return {e, e * x.d};
+}
+template <class T>
+__device__ __forceinline__ Dual<T> mlog(Dual<T> x) { return {mlog(x.v), x.d / x.v}; }
+template <class T>
+__device__ __forceinline__ Dual<T> mlog10(Dual<T> x) {
+  return {mlog10(x.v), x.d / (x.v * T(2.302585092994045684))};
+}
+template <class T>
+__device__ __forceinline__ Dual<T> msqrt(Dual<T> x) {
+  const T r = msqrt(x.v);
+  return {r, r > T(0) ? x.d / (T(2) * r) : T(0)};
+}
+template <class T>
+__device__ __forceinline__ Dual<T> mcos(Dual<T> x) { return {mcos(x.v), -sin(x.v) * x.d}; }
 
 // real**integer as flang lowers it (repeated squaring, reciprocal for n<0)
 template <class R>
@@ -118,17 +161,24 @@ struct StepArgs {
   int ni, nj, nz, ncol;
   long long ldk, ldj, dldk, dldj, d2ldj;
   R* sr;             // SR_N planes
-  double* sd;        // SD_N planes
+  W_t<R>* sd;        // SD_N planes (FP64, or FP64 duals)
   size_t plane;      // elements per scratch plane: nz x ncol rounded up to kScrW columns
   unsigned* pf;      // nz x ncol point flags
   unsigned* cf;      // ncol column flags
-  double* precip;    // 5 x ncol: rain, graupel, snow, ice, cloud
+  W_t<R>* precip;    // 5 x ncol: rain, graupel, snow, ice, cloud
   int* ksub;         // 10 x ncol: substeps and ktop per species (diagnostic)
-  const Const<R>* cst;  // derived constants of this context (wave-uniform reads)
-  Tables<R> tab;
+  const Const<S_t<R>>* cst;  // derived constants of this context (wave-uniform reads)
+  Tables<S_t<R>> tab;
   RunFlags fl;
   double dt;
   int first_step;
+};
+
+// geometry of a caller array: element (i, k, j) at i + k*ldk + j*ldj,
+// column c = i + j*ni (2-D arrays: nk = 1, ldk = 0)
+struct DualGeom {
+  int ni, nj, nk;
+  long long ldk, ldj;
 };
 
 // launch shape of one step (host-side choice, see tempo_hip_tune)
@@ -183,7 +233,7 @@ struct Tend {
 };
 
 template <class R>
-__device__ __forceinline__ R rho_of(const Const<R>& C, R pres, R temp, R qv) {
+__device__ __forceinline__ R rho_of(const Const<S_t<R>>& C, R pres, R temp, R qv) {
   return C.roverrv * pres / (R(kRdry) * temp * (qv + C.roverrv));
 }
 
@@ -222,16 +272,16 @@ __device__ __forceinline__ int get_nuc(R nc) {  // src/module_mp_tempo_utils.F90
 }
 
 // snow_moments polynomial (src/module_mp_tempo_utils.F90:148-282)
-template <class R>
-__device__ __forceinline__ R snow_poly(const R* s, R tc, R cc) {
+template <class R, class S>
+__device__ __forceinline__ R snow_poly(const S* s, R tc, R cc) {
   return s[0] + s[1] * tc + s[2] * cc + s[3] * tc * cc + s[4] * tc * tc + s[5] * cc * cc + s[6] * tc * tc * cc +
          s[7] * tc * cc * cc + s[8] * tc * tc * tc + s[9] * cc * cc * cc;
 }
 // want: 1 smo0, 2 smo1, 4 smo2, 8 smoe, 16 smof, 32 smoz ; smob, smoc always
 template <class R>
-__device__ void snow_moments(const Const<R>& C, R rs, R tc, int want, Lvl<R>& L) {
-  const R* sa = C.sa;
-  const R* sb = C.sb;
+__device__ void snow_moments(const Const<S_t<R>>& C, R rs, R tc, int want, Lvl<R>& L) {
+  const S_t<R>* sa = C.sa;
+  const S_t<R>* sb = C.sb;
   L.smob = rs * C.oams;
   const R bms = R(kBmS);
   R loga = sa[0] + sa[1] * tc + sa[2] * bms + sa[3] * tc * bms + sa[4] * tc * tc + sa[5] * bms * bms +
@@ -241,8 +291,8 @@ __device__ void snow_moments(const Const<R>& C, R rs, R tc, int want, Lvl<R>& L)
         sb[6] * tc * tc * bms + sb[7] * tc * bms * bms + sb[8] * tc * tc * tc + sb[9] * bms * bms * bms;
   const R smo2 = mpow(L.smob / a, R(1.0) / b);
   if (want & 4) L.smo2 = smo2;
-  a = mpow(R(10.0), snow_poly(sa, tc, C.cse[0]));
-  L.smoc = a * mpow(smo2, snow_poly(sb, tc, C.cse[0]));
+  a = mpow(R(10.0), snow_poly(sa, tc, R(C.cse[0])));
+  L.smoc = a * mpow(smo2, snow_poly(sb, tc, R(C.cse[0])));
   if (want & 1) {
     R la = sa[0] + sa[1] * tc + sa[4] * tc * tc + sa[8] * tc * tc * tc;
     R lb = sb[0] + sb[1] * tc + sb[4] * tc * tc + sb[8] * tc * tc * tc;
@@ -255,14 +305,14 @@ __device__ void snow_moments(const Const<R>& C, R rs, R tc, int want, Lvl<R>& L)
            sb[8] * tc * tc * tc + sb[9];
     L.smo1 = mpow(R(10.0), la) * mpow(smo2, lb);
   }
-  if (want & 8) L.smoe = mpow(R(10.0), snow_poly(sa, tc, C.cse[12])) * mpow(smo2, snow_poly(sb, tc, C.cse[12]));
-  if (want & 16) L.smof = mpow(R(10.0), snow_poly(sa, tc, C.cse[15])) * mpow(smo2, snow_poly(sb, tc, C.cse[15]));
-  if (want & 32) L.smoz = mpow(R(10.0), snow_poly(sa, tc, C.cse[2])) * mpow(smo2, snow_poly(sb, tc, C.cse[2]));
+  if (want & 8) L.smoe = mpow(R(10.0), snow_poly(sa, tc, R(C.cse[12]))) * mpow(smo2, snow_poly(sb, tc, R(C.cse[12])));
+  if (want & 16) L.smof = mpow(R(10.0), snow_poly(sa, tc, R(C.cse[15]))) * mpow(smo2, snow_poly(sb, tc, R(C.cse[15])));
+  if (want & 32) L.smoz = mpow(R(10.0), snow_poly(sa, tc, R(C.cse[2]))) * mpow(smo2, snow_poly(sb, tc, R(C.cse[2])));
 }
 
 // thermo_vars, one level (src/module_mp_tempo_main.F90:1263-1310)
 template <class R>
-__device__ void thermo(const Const<R>& C, Lvl<R>& L) {
+__device__ void thermo(const Const<S_t<R>>& C, Lvl<R>& L) {
   const R otemp = R(1.0) / L.temp, tempc = L.temp - R(kT0);
   L.rhof = msqrt(C.rho_not / L.rho);
   L.rhof2 = msqrt(L.rhof);
@@ -290,7 +340,7 @@ __device__ void thermo(const Const<R>& C, Lvl<R>& L) {
 // ---- *_check_and_update for one level (src/module_mp_tempo_main.F90:891-1231) ----
 // ncv: the column's one-moment cloud number (used when nc1d is absent)
 template <class R>
-__device__ void cloud_check(const Const<R>& C, Lvl<R>& L, R& qc1d, R* nc1d, R ncv, R dt, R odt) {
+__device__ void cloud_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qc1d, R* nc1d, R ncv, R dt, R odt) {
   bool hit = false;
   if (qc1d + L.qcten * dt > R(kR1)) {
     L.l_qc = true;
@@ -339,7 +389,7 @@ __device__ void cloud_check(const Const<R>& C, Lvl<R>& L, R& qc1d, R* nc1d, R nc
 }
 
 template <class R>
-__device__ void rain_check(const Const<R>& C, Lvl<R>& L, R& qr1d, R& nr1d, R dt, R odt) {
+__device__ void rain_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qr1d, R& nr1d, R dt, R odt) {
   bool hit = false;
   if (qr1d + L.qrten * dt > R(kR1)) {
     L.l_qr = true;
@@ -384,7 +434,7 @@ __device__ void rain_check(const Const<R>& C, Lvl<R>& L, R& qr1d, R& nr1d, R dt,
 }
 
 template <class R>
-__device__ void ice_check(const Const<R>& C, Lvl<R>& L, R& qi1d, R& ni1d, R dt, R odt) {
+__device__ void ice_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qi1d, R& ni1d, R dt, R odt) {
   bool hit = false;
   if (qi1d + L.qiten * dt > R(kR1)) {
     L.l_qi = true;
@@ -443,7 +493,7 @@ __device__ R rho_g(int idx) {  // densities of graupel (src/module_mp_tempo_para
 }
 
 template <class R>
-__device__ void graupel_check(const Const<R>& C, Lvl<R>& L, R& qg1d, R* ng1d, R* qb1d, R dt, R odt) {
+__device__ void graupel_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qg1d, R* ng1d, R* qb1d, R dt, R odt) {
   bool hit = false;
   if (qg1d + L.qgten * dt > R(kR1)) {
     L.l_qg = true;
@@ -533,7 +583,7 @@ __device__ __forceinline__ int temperature_index(R tempk) {
 }
 
 template <class R>
-__device__ void rain_index(const Const<R>& C, R rr, R ilamr, int& ir, int& ir1) {
+__device__ void rain_index(const Const<S_t<R>>& C, R rr, R ilamr, int& ir, int& ir1) {
   ir = decade_index(rr, C.nir2, kNtbR);
   const R lamr = R(1.) / ilamr;
   const R lam_exp = lamr * mpow(C.crg[2] * C.org2 * C.org1, R(kBmR));
@@ -542,7 +592,7 @@ __device__ void rain_index(const Const<R>& C, R rr, R ilamr, int& ir, int& ir1) 
 }
 
 template <class R>
-__device__ void graupel_index(const Const<R>& C, R rg, R ilamg, int idx, int& ig, int& ig1) {
+__device__ void graupel_index(const Const<S_t<R>>& C, R rg, R ilamg, int idx, int& ig, int& ig1) {
   ig = decade_index(rg, C.nig2, kNtbG);
   const R lamg = R(1.) / ilamg;
   const R lam_exp = lamg * mpow(C.cgg[2][0] * C.ogg2 * C.ogg1, R(kBmG));
@@ -551,7 +601,7 @@ __device__ void graupel_index(const Const<R>& C, R rg, R ilamg, int idx, int& ig
 }
 
 template <class R>
-__device__ void cloud_index(const Const<R>& C, R rc, R nc, int& ic, int& in) {
+__device__ void cloud_index(const Const<S_t<R>>& C, R rc, R nc, int& ic, int& in) {
   ic = decade_index(rc, C.nic2, kNtbC);
   const int n = nint_exact(R(1.0) + R(kNbins) * mlog(nc / C.t_nc1) / C.nic1);
   in = max(1, min(n, kNbins));
@@ -560,7 +610,7 @@ __device__ void cloud_index(const Const<R>& C, R rc, R nc, int& ic, int& in) {
 // ------------------------------------------------------ process rates ----
 // warm_rain (src/module_mp_tempo_main.F90:2279-2355)
 template <class R>
-__device__ void warm_rain(const Const<R>& C, const Tables<R>& T, const Lvl<R>& L, Tend<R>& t, R odt) {
+__device__ void warm_rain(const Const<S_t<R>>& C, const Tables<S_t<R>>& T, const Lvl<R>& L, Tend<R>& t, R odt) {
   if (L.l_qr && L.mvd_r > R(kD0R)) {
     const R ef_rr = mx(R(-0.1), R(1.0) - mexp(R(2300.0) * (L.mvd_r - R(1950.0e-6))));
     t.pnr_rcr = ef_rr * R(2.0) * L.nr * L.rr;
@@ -599,14 +649,14 @@ __device__ void warm_rain(const Const<R>& C, const Tables<R>& T, const Lvl<R>& L
 
 // rain_snow_rain_graupel (:2638-2727)
 template <class R>
-__device__ void rain_snow_graupel(const Const<R>& C, const Tables<R>& T, const Lvl<R>& L, Tend<R>& t, R odt) {
+__device__ void rain_snow_graupel(const Const<S_t<R>>& C, const Tables<S_t<R>>& T, const Lvl<R>& L, Tend<R>& t, R odt) {
   if (L.l_qr && L.l_qs && L.rr >= R(1.e-6) && L.rs >= R(1.e-6)) {
     const int it = temperature_index(L.temp);
     int ir, ir1;
     rain_index(C, L.rr, L.ilamr, ir, ir1);
     const int is = decade_index(L.rs, C.nis2, kNtbS);
     const size_t e = size_t(is - 1) + kNtbS * (size_t(it - 1) + kNtbT * (size_t(ir1 - 1) + kNtbR1 * size_t(ir - 1)));
-    const R* q = T.qs + e * 12;
+    const S_t<R>* q = T.qs + e * 12;
     // file order: 0 tcs_racs1 1 tmr_racs1 2 tcs_racs2 3 tmr_racs2 4 tcr_sacr1 5 tms_sacr1
     //             6 tcr_sacr2 7 tms_sacr2 8 tnr_racs1 9 tnr_racs2 10 tnr_sacr1 11 tnr_sacr2
     if (L.temp < R(kT0)) {
@@ -633,7 +683,7 @@ __device__ void rain_snow_graupel(const Const<R>& C, const Tables<R>& T, const L
     graupel_index(C, L.rg, L.ilamg, idx, ig, ig1);
     const size_t e = size_t(ig1 - 1) +
                      kNtbG1 * (size_t(ig - 1) + kNtbG * (size_t(idx - 1) + kNrhg * (size_t(ir1 - 1) + kNtbR1 * size_t(ir - 1))));
-    const R* q = T.qg + e * 5;  // tcg_racg tmr_racg tcr_gacr tnr_racg tnr_gacr
+    const S_t<R>* q = T.qg + e * 5;  // tcg_racg tmr_racg tcr_gacr tnr_racg tnr_gacr
     if (L.temp < R(kT0)) {
       t.prg_rcg = q[1] + q[2];
       t.prg_rcg = mn(L.rr * odt, t.prg_rcg);
@@ -655,14 +705,14 @@ __device__ void rain_snow_graupel(const Const<R>& C, const Tables<R>& T, const L
 
 // demott_nucleation (:2823-2836) and koop_nucleation (:2168-2195)
 template <class R>
-__device__ R demott(const Const<R>& C, R tempc, R rho, R nifa) {
+__device__ R demott(const Const<S_t<R>>& C, R tempc, R rho, R nifa) {
   const R nifa_cc = mx(R(0.5), nifa * C.rho_not0 * R(1.e-6) / rho);
   R xni = (R(5.94e-5) * mpow(-tempc, R(3.33))) * (mpow(nifa_cc, (R(-0.0264) * (tempc)) + R(0.0033)));
   xni = xni * rho / C.rho_not0 * R(1000.0);
   return mx(R(0.0), xni);
 }
 template <class R>
-__device__ R koop(const Const<R>& C, R temp, R satw, R naero, R dt) {
+__device__ R koop(const Const<S_t<R>>& C, R temp, R satw, R naero, R dt) {
   R xni = R(0.0);
   const R mu_diff = R(210368.0) + (R(131.438) * temp) - (R(3.32373e6) / temp) - (R(41729.1) * mlog(temp));
   const R a_w_i = mexp(mu_diff / (R(kRUni) * temp));
@@ -677,7 +727,7 @@ __device__ R koop(const Const<R>& C, R temp, R satw, R naero, R dt) {
 
 // ice_nucleation (:2730-2820)
 template <class R>
-__device__ void ice_nucleation(const Const<R>& C, const Tables<R>& T, const Lvl<R>& L, Tend<R>& t, bool has_nwfa,
+__device__ void ice_nucleation(const Const<S_t<R>>& C, const Tables<S_t<R>>& T, const Lvl<R>& L, Tend<R>& t, bool has_nwfa,
                                bool has_nifa, R w, R dt, R odt) {
   if (!(L.temp < R(kT0))) return;
   const R tempc = L.temp - R(kT0);
@@ -689,7 +739,7 @@ __device__ void ice_nucleation(const Const<R>& C, const Tables<R>& T, const Lvl<
     int ir, ir1;
     rain_index(C, L.rr, L.ilamr, ir, ir1);
     const size_t e = size_t(ir - 1) + kNtbR * (size_t(ir1 - 1) + kNtbR1 * (size_t(idx_tc - 1) + kNtbT1 * size_t(idx_in - 1)));
-    const R* q = T.fzr + e * 4;  // tpi_qrfz tni_qrfz tpg_qrfz tnr_qrfz
+    const S_t<R>* q = T.fzr + e * 4;  // tpi_qrfz tni_qrfz tpg_qrfz tnr_qrfz
     t.prg_rfz = q[2] * odt;
     t.pri_rfz = q[0] * odt;
     t.pni_rfz = q[1] * odt;
@@ -706,7 +756,7 @@ __device__ void ice_nucleation(const Const<R>& C, const Tables<R>& T, const Lvl<
     int ic, in;
     cloud_index(C, L.rc, L.nc, ic, in);
     const size_t e = size_t(ic - 1) + kNtbC * (size_t(in - 1) + kNbins * (size_t(idx_tc - 1) + kNtbT1 * size_t(idx_in - 1)));
-    const R* q = T.fzc + e * 2;  // tpi_qcfz tni_qcfz
+    const S_t<R>* q = T.fzc + e * 2;  // tpi_qcfz tni_qcfz
     t.pri_wfz = q[0] * odt;
     t.pri_wfz = mn(L.rc * odt, t.pri_wfz);
     t.pni_wfz = q[1] * odt;
@@ -734,7 +784,7 @@ __device__ void ice_nucleation(const Const<R>& C, const Tables<R>& T, const Lvl<
 
 // get_t1_subl (:2861-2888)
 template <class R>
-__device__ R t1_subl(const Const<R>& C, const Lvl<R>& L) {
+__device__ R t1_subl(const Const<S_t<R>>& C, const Lvl<R>& L) {
   const R otemp = R(1.0) / L.temp, rvs = L.rho * L.qvsi;
   const R a = R(kLsub) * otemp * C.orv - R(1.0);
   const R rvs_p = rvs * otemp * a;
@@ -751,7 +801,7 @@ __device__ R t1_subl(const Const<R>& C, const Lvl<R>& L) {
 
 // ice_processes (:2891-3018)
 template <class R>
-__device__ void ice_processes(const Const<R>& C, const Tables<R>& T, const Lvl<R>& L, Tend<R>& t, R t1s, R w,
+__device__ void ice_processes(const Const<S_t<R>>& C, const Tables<S_t<R>>& T, const Lvl<R>& L, Tend<R>& t, R t1s, R w,
                               R odt) {
   if (!(L.temp < R(kT0))) return;
   const R rvs = L.rho * L.qvsi;
@@ -759,7 +809,7 @@ __device__ void ice_processes(const Const<R>& C, const Tables<R>& T, const Lvl<R
   if (L.l_qi) {
     const int ii = decade_index(L.ri, C.nii2, kNtbI);
     const int ii1 = decade_index(L.ni, C.nii3, kNtbI1);
-    const R xdi = mx(C.d0i, (R(kBmI) + R(kMuI) + R(1.)) * L.ilami);
+    const R xdi = mx(R(C.d0i), (R(kBmI) + R(kMuI) + R(1.)) * L.ilami);
     const R xmi = C.am_i * mpow(xdi, R(kBmI));
     const R oxmi = R(1.0) / xmi;
     const size_t it = size_t(ii - 1) + size_t(kNtbI) * (ii1 - 1);
@@ -830,7 +880,7 @@ __device__ void ice_processes(const Const<R>& C, const Tables<R>& T, const Lvl<R
 
 // riming (:2358-2490); returns vtboost
 template <class R>
-__device__ R riming(const Const<R>& C, const Tables<R>& T, const Lvl<R>& L, Tend<R>& t, R odt) {
+__device__ R riming(const Const<S_t<R>>& C, const Tables<S_t<R>>& T, const Lvl<R>& L, Tend<R>& t, R odt) {
   const R tempc = L.temp - R(kT0);
   R vtboost = R(1.0);
   if (L.l_qc && L.l_qs) {
@@ -907,7 +957,7 @@ __device__ R riming(const Const<R>& C, const Tables<R>& T, const Lvl<R>& L, Tend
 
 // melting (:3021-3119)
 template <class R>
-__device__ void melting(const Const<R>& C, const Lvl<R>& L, Tend<R>& t, R t1s, R dt, R odt) {
+__device__ void melting(const Const<S_t<R>>& C, const Lvl<R>& L, Tend<R>& t, R t1s, R dt, R odt) {
   if (!(L.temp > R(kT0))) return;
   const R tempc = L.temp - R(kT0), rvs = L.rho * L.qvsi;
   const R drive = tempc * L.tcond - R(kLvap0) * L.diffu * L.delqvs;
@@ -964,7 +1014,7 @@ __device__ void melting(const Const<R>& C, const Lvl<R>& L, Tend<R>& t, R t1s, R
 
 // aerosol_collection_efficiency (src/module_mp_tempo_aerosols.F90:76-118)
 template <class R>
-__device__ R aero_eff(const Const<R>& C, R d, R da, R visc, R rhoa, R temp, char species) {
+__device__ R aero_eff(const Const<S_t<R>>& C, R d, R da, R visc, R rhoa, R temp, char species) {
   const R boltzman = R(1.3806503e-23), meanpath = R(0.0256e-6);
   R vt = R(1.0), rho_p = R(kRhoW);
   if (species == 'r') {
@@ -992,7 +1042,7 @@ __device__ R aero_eff(const Const<R>& C, R d, R da, R visc, R rhoa, R temp, char
 
 // aerosol_scavenging (src/module_mp_tempo_main.F90:3122-3193)
 template <class R>
-__device__ void aerosol_scavenging(const Const<R>& C, const Lvl<R>& L, Tend<R>& t, R odt) {
+__device__ void aerosol_scavenging(const Const<S_t<R>>& C, const Lvl<R>& L, Tend<R>& t, R odt) {
   const R wf = R(0.04e-6), iff = R(0.8e-6);
   if (L.l_qr && L.mvd_r > R(kD0R)) {
     R ef = aero_eff(C, L.mvd_r, wf, L.visco, L.rho, L.temp, 'r');
@@ -1086,7 +1136,7 @@ __device__ void check_over_depletion(const Lvl<R>& L, Tend<R>& t, R odt) {
 
 // sum_tendencies (:1441-1513)
 template <class R>
-__device__ void sum_tendencies(const Const<R>& C, Lvl<R>& L, const Tend<R>& t) {
+__device__ void sum_tendencies(const Const<S_t<R>>& C, Lvl<R>& L, const Tend<R>& t) {
   const R orho = R(1.) / L.rho, lfus2 = R(kLsub) - L.lvap;
   L.qvten = L.qvten + (-t.pri_inu - t.pri_iha - t.pri_ide - t.prs_ide - t.prs_sde - t.prg_gde) * orho;
   L.qcten = L.qcten + (-t.prr_wau - t.pri_wfz - t.prr_rcw - t.prs_scw - t.prg_scw - t.prg_gcw) * orho;
@@ -1118,7 +1168,7 @@ __device__ void sum_tendencies(const Const<R>& C, Lvl<R>& L, const Tend<R>& t) {
 
 // activate_cloud_number (:2198-2276), land never passed by the caller
 template <class R>
-__device__ R activate_cloud_number(const Tables<R>& T, R temp, R w1d, R nwfa) {
+__device__ R activate_cloud_number(const Tables<S_t<R>>& T, R temp, R w1d, R nwfa) {
   const R na[7] = {R(10.), R(31.6), R(100.), R(316.), R(1000.), R(3160.), R(10000.)};
   const R ww[9] = {R(0.01), R(0.0316), R(0.1), R(0.316), R(1.), R(3.16), R(10.), R(31.6), R(100.)};
   R n_local = nwfa * R(1.e-6);
@@ -1145,7 +1195,7 @@ __device__ R activate_cloud_number(const Tables<R>& T, R temp, R w1d, R nwfa) {
 
 // cloud_condensation (:1980-2056)
 template <class R>
-__device__ void cloud_condensation(const Const<R>& C, const Tables<R>& T, const Lvl<R>& L, Tend<R>& t, bool has_nwfa,
+__device__ void cloud_condensation(const Const<S_t<R>>& C, const Tables<S_t<R>>& T, const Lvl<R>& L, Tend<R>& t, bool has_nwfa,
                                    bool has_nc, R w, R dt, R odt) {
   if (mabs(L.ssatw) < R(kEps)) return;
   const R orho = R(1.0) / L.rho;
@@ -1195,7 +1245,7 @@ __device__ void cloud_condensation(const Const<R>& C, const Tables<R>& T, const 
 
 // rain_evaporation (:2059-2121)
 template <class R>
-__device__ void rain_evaporation(const Const<R>& C, const Lvl<R>& L, Tend<R>& t, R odt) {
+__device__ void rain_evaporation(const Const<S_t<R>>& C, const Lvl<R>& L, Tend<R>& t, R odt) {
   if (!L.l_qr || !((L.ssatw < R(-kEps)) && t.prw_vcd <= R(0.0))) return;
   const R orho = R(1.0) / L.rho, tempc = L.temp - R(kT0), otemp = R(1.0) / L.temp;
   const R rvs = L.rho * L.qvs;

@@ -2,6 +2,7 @@
 // fluxes carried in FP64 inside phase B).
 #include "tempo_kernels.hpp"
 #include "tempo_sedi.hpp"
+#include "tempo_adjoint.hpp"
 
 namespace tempo {
 
@@ -25,5 +26,19 @@ hipError_t launch_step(const StepArgs<float>& a, const LaunchCfg& lc, double* vt
   if (ev) (void)hipEventRecord(ev[4], s);
   return hipGetLastError();
 }
+
+// tangent-linear step in this precision (same contraction flags as the step)
+hipError_t launch_step_tlm(const StepArgs<Dual<float>>& a, Dual<double>* vtbuf, hipStream_t s) {
+  return launch_step_tlm_impl(a, vtbuf, s);
+}
+template hipError_t launch_dual_pack<float>(const float*, const float*, const DualGeom&, Dual<float>*, hipStream_t);
+template hipError_t launch_dual_unpack<float>(const Dual<float>*, const DualGeom&, float*, float*, hipStream_t);
+
+hipError_t launch_adjoint(const StepArgs<Dual<float>>& a, const AdjArgs& g, Dual<double>* vtbuf, const void* inputs,
+                          void* work_state, size_t state_bytes, const bool* have_x, hipStream_t s) {
+  return adjoint_passes(a, g, vtbuf, inputs, work_state, state_bytes, have_x, s);
+}
+template hipError_t launch_to_compact<float>(const float*, const DualGeom&, double*, hipStream_t);
+template hipError_t launch_from_compact<float>(const double*, const DualGeom&, float*, hipStream_t);
 
 }  // namespace tempo

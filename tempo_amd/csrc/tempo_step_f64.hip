@@ -2,6 +2,7 @@
 // every operation rounds as in the reference FP64 build.
 #include "tempo_kernels.hpp"
 #include "tempo_sedi.hpp"
+#include "tempo_adjoint.hpp"
 
 namespace tempo {
 
@@ -25,5 +26,19 @@ hipError_t launch_step(const StepArgs<double>& a, const LaunchCfg& lc, double* v
   if (ev) (void)hipEventRecord(ev[4], s);
   return hipGetLastError();
 }
+
+// tangent-linear step in this precision (same contraction flags as the step)
+hipError_t launch_step_tlm(const StepArgs<Dual<double>>& a, Dual<double>* vtbuf, hipStream_t s) {
+  return launch_step_tlm_impl(a, vtbuf, s);
+}
+template hipError_t launch_dual_pack<double>(const double*, const double*, const DualGeom&, Dual<double>*, hipStream_t);
+template hipError_t launch_dual_unpack<double>(const Dual<double>*, const DualGeom&, double*, double*, hipStream_t);
+
+hipError_t launch_adjoint(const StepArgs<Dual<double>>& a, const AdjArgs& g, Dual<double>* vtbuf, const void* inputs,
+                          void* work_state, size_t state_bytes, const bool* have_x, hipStream_t s) {
+  return adjoint_passes(a, g, vtbuf, inputs, work_state, state_bytes, have_x, s);
+}
+template hipError_t launch_to_compact<double>(const double*, const DualGeom&, double*, hipStream_t);
+template hipError_t launch_from_compact<double>(const double*, const DualGeom&, double*, hipStream_t);
 
 }  // namespace tempo

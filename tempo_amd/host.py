@@ -97,6 +97,12 @@ def lib():
         L.tempo_hip_substeps.argtypes = [vp, vp, ctypes.c_size_t]
         L.tempo_hip_profile.argtypes = [vp, ctypes.c_int]
         L.tempo_hip_tune.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.tempo_hip_tlm.argtypes = [vp, ctypes.POINTER(Fields), ctypes.POINTER(Fields), ctypes.POINTER(Diags),
+                                    ctypes.POINTER(Diags), ctypes.POINTER(Dims), ctypes.c_double, ctypes.c_int,
+                                    ctypes.c_int, vp]
+        L.tempo_hip_adjoint.argtypes = [vp, ctypes.POINTER(Fields), ctypes.POINTER(Fields), ctypes.POINTER(Diags),
+                                        ctypes.POINTER(Fields), ctypes.POINTER(Dims), ctypes.c_double, ctypes.c_int,
+                                        ctypes.c_int, vp]
         L.tempo_hip_profile_read.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.tempo_hip_last_error.restype = ctypes.c_char_p
         L.tempo_hip_finalize.argtypes = [vp]
@@ -209,6 +215,96 @@ class TempoHip:
         _check(lib().tempo_hip_run(self._ctx, ctypes.byref(f), ctypes.byref(d), ctypes.byref(dims), float(dt),
                                    int(itimestep), prec, ctypes.c_void_p(s)))
         return out
+
+    # ---- shared argument marshalling for device tensors ----
+    def _fields(self, state, nz, ncol, dtype, what="field"):
+        f = Fields()
+        for name in FIELD_NAMES:
+            a = state.get(name) if state else None
+            if a is None:
+                continue
+            if a.dtype != dtype or tuple(a.shape) != (nz, ncol) or not a.is_contiguous() or not a.is_cuda:
+                raise ValueError(f"{what} {name}: need a contiguous {dtype} cuda tensor of shape {(nz, ncol)}")
+            setattr(f, name, a.data_ptr())
+        return f
+
+    def _diags(self, diags, nz, ncol, dtype):
+        d = Diags()
+        for n, a in (diags or {}).items():
+            if n not in DIAG2_NAMES + DIAG3_NAMES:
+                raise ValueError(f"unknown diagnostic {n}")
+            want = (ncol,) if n in DIAG2_NAMES else (nz, ncol)
+            if a.dtype != dtype or tuple(a.shape) != want or not a.is_contiguous() or not a.is_cuda:
+                raise ValueError(f"diagnostic {n}: need a contiguous {dtype} cuda tensor of shape {want}")
+            setattr(d, n, a.data_ptr())
+        return d
+
+    def tlm(self, state, tangent, dt, itimestep, nlev=None, land=None, stream=None):
+        """Tangent-linear step (tempo_hip_tlm): ``state`` is stepped in place and
+        ``tangent`` (dict of tensors like the state; missing = zero) is
+        overwritten with the output tangents.  Returns (diags, diag_tangents)."""
+        import torch
+        ref = state["p"]
+        nz, ncol = ref.shape
+        dtype = ref.dtype
+        tangent = dict(tangent)
+        for k, v in state.items():  # every output needs a tangent buffer
+            if k not in tangent:
+                tangent[k] = torch.zeros_like(v)
+        f = self._fields(state, nz, ncol, dtype)
+        ft = self._fields(tangent, nz, ncol, dtype, "tangent")
+        out = {n: torch.zeros(ncol, dtype=dtype, device=ref.device) for n in DIAG2_NAMES}
+        out.update({n: torch.zeros((nz, ncol), dtype=dtype, device=ref.device) for n in DIAG3_NAMES})
+        tout = {n: torch.zeros_like(v) for n, v in out.items()}
+        keep = []
+        if land is not None:
+            land = land.to(device=ref.device, dtype=torch.int32).contiguous()
+            keep.append(land)
+            f.land = land.data_ptr()
+        dims = Dims(ncol, 1, nz, ncol, nz * ncol, ncol, nz * ncol, ncol, None)
+        if nlev is not None:
+            nlev = nlev.to(device=ref.device, dtype=torch.int32).contiguous()
+            keep.append(nlev)
+            dims.nlev = nlev.data_ptr()
+        s = stream if stream is not None else torch.cuda.current_stream(ref.device).cuda_stream
+        prec = FP32 if dtype == torch.float32 else FP64
+        _check(lib().tempo_hip_tlm(self._ctx, ctypes.byref(f), ctypes.byref(ft),
+                                   ctypes.byref(self._diags(out, nz, ncol, dtype)),
+                                   ctypes.byref(self._diags(tout, nz, ncol, dtype)), ctypes.byref(dims), float(dt),
+                                   int(itimestep), prec, ctypes.c_void_p(s)))
+        return out, tout, tangent
+
+    def adjoint(self, state, ybar, dbar, dt, itimestep, nlev=None, land=None, stream=None):
+        """Adjoint step (tempo_hip_adjoint).  ``state``: the step's inputs (not
+        modified); ``ybar``: cotangents of output fields; ``dbar``:
+        cotangents of the precipitation diagnostics and refl10cm.  Returns
+        the input cotangents of t, qv, qc, qr, qi, qs, qg, nc, nr."""
+        import torch
+        ref = state["p"]
+        nz, ncol = ref.shape
+        dtype = ref.dtype
+        f = self._fields(state, nz, ncol, dtype)
+        yb = self._fields(ybar, nz, ncol, dtype, "cotangent")
+        db = self._diags(dbar, nz, ncol, dtype)
+        xbar = {k: torch.zeros((nz, ncol), dtype=dtype, device=ref.device)
+                for k in ("t", "qv", "qc", "qr", "qi", "qs", "qg", "nc", "nr") if k in state}
+        xb = self._fields(xbar, nz, ncol, dtype)
+        keep = []
+        if land is not None:
+            land = land.to(device=ref.device, dtype=torch.int32).contiguous()
+            keep.append(land)
+            f.land = land.data_ptr()
+        dims = Dims(ncol, 1, nz, ncol, nz * ncol, ncol, nz * ncol, ncol, None)
+        if nlev is not None:
+            nlev = nlev.to(device=ref.device, dtype=torch.int32).contiguous()
+            keep.append(nlev)
+            dims.nlev = nlev.data_ptr()
+        s = stream if stream is not None else torch.cuda.current_stream(ref.device).cuda_stream
+        prec = FP32 if dtype == torch.float32 else FP64
+        _check(lib().tempo_hip_adjoint(self._ctx, ctypes.byref(f), ctypes.byref(yb), ctypes.byref(db),
+                                       ctypes.byref(xb), ctypes.byref(dims), float(dt), int(itimestep), prec,
+                                       ctypes.c_void_p(s)))
+        return xbar
 
     def run_host(self, state, dt, itimestep, nlev=None, land=None):
         """tempo_run on host float64 numpy arrays (staged through device memory)."""
