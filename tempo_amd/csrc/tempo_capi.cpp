@@ -23,9 +23,11 @@ hipError_t launch_dual_pack(const T* v, const T* t, const DualGeom& g, Dual<T>* 
 template <class T>
 hipError_t launch_dual_unpack(const Dual<T>* in, const DualGeom& g, T* v, T* t, hipStream_t s);
 hipError_t launch_adjoint(const StepArgs<Dual<float>>& a, const AdjArgs& g, Dual<double>* vtbuf, const void* inputs,
-                          void* work_state, size_t state_bytes, const bool* have_x, hipStream_t s);
+                          void* work_state, size_t state_bytes, size_t sr_bytes, size_t sd_bytes, AdjSnap snap_a,
+                          AdjSnap snap_b, const bool* have_x, hipStream_t s);
 hipError_t launch_adjoint(const StepArgs<Dual<double>>& a, const AdjArgs& g, Dual<double>* vtbuf, const void* inputs,
-                          void* work_state, size_t state_bytes, const bool* have_x, hipStream_t s);
+                          void* work_state, size_t state_bytes, size_t sr_bytes, size_t sd_bytes, AdjSnap snap_a,
+                          AdjSnap snap_b, const bool* have_x, hipStream_t s);
 template <class T>
 hipError_t launch_to_compact(const T* src, const DualGeom& g, double* dst, hipStream_t s);
 template <class T>
@@ -141,6 +143,7 @@ struct tempo_hip_ctx {
   DevBuf sr, sd, pf, cf_, precip, ksub, vtbuf;
   DevBuf tl_state, tl_diag;  // dual (value, tangent) copies for the tangent-linear step
   DevBuf adj_in, adj_mem;    // adjoint: packed inputs, cotangent/Jacobian planes
+  DevBuf adj_snap;           // adjoint: state and scratch after phases A and B
   int last_ncol = 0;
   // host-array staging
   std::vector<DevBuf> stage;
@@ -553,7 +556,12 @@ int run_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fie
   if (rc) return rc;
   bool have_x[AX_N];
   for (int i = 0; i < AX_N; ++i) have_x[i] = fp[adj_x_field(i)] != nullptr;
-  HIP_TRY(launch_adjoint(a, g, x->vtbuf.as<Dual<double>>(), in, st, sb, have_x, s));
+  const size_t srb = SR_N * a.plane * sizeof(D), sdb = SD_N * a.plane * sizeof(Dual<double>);
+  HIP_TRY(x->adj_snap.ensure(2 * (sb + srb + sdb)));
+  char* sp = static_cast<char*>(x->adj_snap.p);
+  const AdjSnap sa{sp, sp + sb, sp + sb + srb};
+  const AdjSnap sbb{sp + sb + srb + sdb, sp + 2 * sb + srb + sdb, sp + 2 * sb + 2 * srb + sdb};
+  HIP_TRY(launch_adjoint(a, g, x->vtbuf.as<Dual<double>>(), in, st, sb, srb, sdb, sa, sbb, have_x, s));
   void* xp[F_NFIELDS] = {};
   if (xb) field_ptrs(xb, xp);
   for (int i = 0; i < AX_N; ++i)
