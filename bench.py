@@ -108,6 +108,55 @@ def adjoint_timing(steps):
             "mcl_per_s_forward_adjoint": round(ncol * nz / (fa_ms * 1e-3) / 1e6, 2)}
 
 
+def ragged_graph(steps_per_graph=10, replays=3, tune=True):
+    """Uneven columns: 131,072 columns of U{30..128} active levels, hail-aware,
+    25 % convective; 10 steps captured in one HIP graph (torch.cuda.CUDAGraph
+    over the library's stream-ordered launches) and replayed."""
+    import torch
+    from tempo_amd import autotune as at
+    from tempo_amd import generator as gen
+    from tempo_amd import host
+    ctx = host.TempoHip(aerosolaware_flag=0, hailaware_flag=1)
+    st0, m = gen.generate_torch("ragged", device="cuda", dtype=torch.float32)
+    nlev = m["nlev"]
+    nz, ncol = st0["p"].shape
+    levels = int(nlev.sum())
+    best = at.autotune(ctx, st0, m["dt"], m["itimestep"], nlev=nlev) if tune else None
+    ctx.reserve(ncol, nz)
+    state = {k: v.clone() for k, v in st0.items()}
+    diags = {n: torch.zeros(ncol, dtype=torch.float32, device="cuda") for n in host.DIAG2_NAMES}
+    diags.update({n: torch.zeros((nz, ncol), dtype=torch.float32, device="cuda") for n in DIAG3})
+
+    def ten_steps():
+        for i in range(steps_per_graph):
+            ctx.run(state, m["dt"], m["itimestep"] + i, nlev=nlev, diags=diags)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        ten_steps()  # warm-up on the capture stream
+    torch.cuda.current_stream().wait_stream(s)
+    g = torch.cuda.CUDAGraph()
+    for k in state:
+        state[k].copy_(st0[k])
+    with torch.cuda.graph(g):
+        ten_steps()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for k in state:
+        state[k].copy_(st0[k])
+    g.replay()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(replays):
+        g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / replays
+    return {"workload": f"{ncol} columns of 30-128 active levels ({levels} column-levels), hail-aware, "
+                        f"25 % convective, {steps_per_graph} steps per HIP graph, fp32",
+            "ms_per_graph": round(ms, 3), "mcl_per_s": round(levels * steps_per_graph / (ms * 1e-3) / 1e6, 2),
+            "tuned": None if best is None else {k: best[k] for k in ("block_a", "block_c", "sed_wave")}}
+
+
 def copy_probe(gib):
     """A plain device copy of known size, for scaling rocprof's FETCH_SIZE and
     WRITE_SIZE counters."""
@@ -134,6 +183,7 @@ def main():
     ap.add_argument("--copy-probe", type=float, default=0.0, help="run only a copy of this many GiB")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint timing")
     ap.add_argument("--adjoint-steps", type=int, default=3)
+    ap.add_argument("--no-ragged", action="store_true", help="skip the uneven-column HIP-graph run")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -230,6 +280,8 @@ def main():
     }
     if not a.no_adjoint:
         out["adjoint"] = adjoint_timing(a.adjoint_steps)
+    if not a.no_ragged:
+        out["ragged_graph"] = ragged_graph()
     if not a.no_cpu_baseline:
         cb, raw = cpu_baseline(ctx, a.cpu_cols, a.cpu_workers)
         out["cpu_baseline"] = cb

@@ -1,0 +1,75 @@
+"""Launch-shape autotuner for the column step.
+
+Times the step's kernels under each candidate launch shape with the
+library's own per-phase HIP events (tempo_hip_profile) and keeps the fastest
+choice per phase:
+
+  * level-local physics (phase A) and final update (phase C): columns per
+    block, 64 / 128 / 256 (one lane per column at one level);
+  * sedimentation: one wavefront per column with LDS-staged column groups,
+    or one lane per column.
+
+Every candidate computes identical results (the kernels do the same
+arithmetic per point), so the choice affects time only.  Results are cached
+per (precision, nz, column-count bucket, flags) for the life of the process
+and applied to the context with tempo_hip_tune.
+"""
+import itertools
+
+from . import host
+
+BLOCKS = (64, 128, 256)
+SED = (1, 0)
+
+_cache = {}
+
+
+def _key(ctx, precision, nz, ncol):
+    bucket = 1 << max(0, int(ncol - 1).bit_length())
+    c = ctx.cfg
+    return (precision, nz, bucket, c.aerosolaware_flag, c.hailaware_flag)
+
+
+def _phase_ms(ctx, state, dt, itimestep, nlev, reps):
+    import torch
+    work = {k: v.clone() for k, v in state.items()}
+    ctx.run(work, dt, itimestep, nlev=nlev, diags=False)  # warm-up with this shape
+    ctx.profile(True)
+    for _ in range(reps):
+        for k in work:
+            work[k].copy_(state[k])
+        ctx.run(work, dt, itimestep, nlev=nlev, diags=False)
+    ms, runs = ctx.profile_read()
+    ctx.profile(False)
+    torch.cuda.synchronize()
+    return ms / max(runs, 1)
+
+
+def autotune(ctx, state, dt, itimestep, nlev=None, reps=3, verbose=False):
+    """Pick and apply the fastest launch shapes for ``state``'s shape and
+    precision; returns {"block_a": .., "block_c": .., "sed_wave": .., "ms": {...}}."""
+    import torch
+    nz, ncol = state["p"].shape
+    precision = host.FP32 if state["p"].dtype == torch.float32 else host.FP64
+    key = _key(ctx, precision, nz, ncol)
+    if key not in _cache:
+        timings = {}
+        for ba, bc, sw in itertools.product(BLOCKS, BLOCKS, SED):
+            if ba != bc and sw == 0:
+                continue  # the sedimentation choice is independent of the block sizes
+            ctx.tune(precision, host.TUNE_BLOCK_A, ba)
+            ctx.tune(precision, host.TUNE_BLOCK_C, bc)
+            ctx.tune(precision, host.TUNE_SED_WAVE, sw)
+            timings[(ba, bc, sw)] = _phase_ms(ctx, state, dt, itimestep, nlev, reps)
+            if verbose:
+                print("autotune", (ba, bc, sw), [round(float(x), 4) for x in timings[(ba, bc, sw)]])
+        best_a = min(BLOCKS, key=lambda b: min(t[1] for (ba, bc, sw), t in timings.items() if ba == b))
+        best_c = min(BLOCKS, key=lambda b: min(t[3] for (ba, bc, sw), t in timings.items() if bc == b))
+        best_s = min(SED, key=lambda s: min(t[2] for (ba, bc, sw), t in timings.items() if sw == s))
+        _cache[key] = dict(block_a=best_a, block_c=best_c, sed_wave=best_s,
+                           ms={f"{k[0]}/{k[1]}/{k[2]}": [round(float(x), 4) for x in v] for k, v in timings.items()})
+    best = _cache[key]
+    ctx.tune(precision, host.TUNE_BLOCK_A, best["block_a"])
+    ctx.tune(precision, host.TUNE_BLOCK_C, best["block_c"])
+    ctx.tune(precision, host.TUNE_SED_WAVE, best["sed_wave"])
+    return best
