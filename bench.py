@@ -157,6 +157,53 @@ def ragged_graph(steps_per_graph=10, replays=3, tune=True):
             "tuned": None if best is None else {k: best[k] for k in ("block_a", "block_c", "sed_wave")}}
 
 
+def multistep_adjoint(nsteps=12, ncol=262_144):
+    """Multi-step adjoint: nsteps chained steps of the hail- and aerosol-aware
+    workload (ncol x 90, FP32), every prognostic field differentiated, each
+    step's inputs kept as its checkpoint (the autograd graph saves them) and
+    the sedimentation substeps recomputed inside each step's adjoint."""
+    import torch
+    from tempo_amd import generator as gen
+    from tempo_amd import host
+    from tempo_amd.autograd import Y_NAMES, tempo_step
+    ctx = host.TempoHip(aerosolaware_flag=1, hailaware_flag=1)
+    st, m = gen.generate_torch("multistep", ncol=ncol, device="cuda", dtype=torch.float32)
+    prog = [k for k in Y_NAMES if k in st]
+    free0, total = torch.cuda.mem_get_info()
+
+    def run(grad):
+        x = {k: (v.clone().requires_grad_(grad and k in prog)) for k, v in st.items()}
+        cur = dict(x)
+        loss = 0.0
+        for n in range(nsteps):
+            out = tempo_step(ctx, m["dt"], m["itimestep"] + n, cur)
+            loss = loss + out["rain_precip"].sum() + out["refl10cm"][[4, 19, 39]].sum()
+            cur = {k: (out[k] if k in out else cur[k]) for k in cur}
+        if grad:
+            loss.backward()
+        return loss
+
+    with torch.no_grad():
+        run(False)
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    e[0].record()
+    with torch.no_grad():
+        run(False)
+    e[1].record()
+    torch.cuda.reset_peak_memory_stats()
+    e[2].record()
+    run(True)
+    e[3].record()
+    torch.cuda.synchronize()
+    free1, _ = torch.cuda.mem_get_info()
+    f_ms, fa_ms = e[0].elapsed_time(e[1]), e[2].elapsed_time(e[3])
+    return {"workload": f"{nsteps} steps x {ncol} columns x {st['p'].shape[0]} levels, fp32, aerosol- and hail-aware",
+            "forward_ms": round(f_ms, 2), "forward_adjoint_ms": round(fa_ms, 2), "ratio": round(fa_ms / f_ms, 2),
+            "torch_peak_gib": round(torch.cuda.max_memory_allocated() / 2**30, 2),
+            "device_in_use_gib": round((total - free1) / 2**30, 2)}
+
+
 def copy_probe(gib):
     """A plain device copy of known size, for scaling rocprof's FETCH_SIZE and
     WRITE_SIZE counters."""
@@ -184,6 +231,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint timing")
     ap.add_argument("--adjoint-steps", type=int, default=3)
     ap.add_argument("--no-ragged", action="store_true", help="skip the uneven-column HIP-graph run")
+    ap.add_argument("--multistep-adjoint", type=int, default=0,
+                    help="also time a multi-step adjoint over this many columns (262144 for the full run)")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -282,6 +331,8 @@ def main():
         out["adjoint"] = adjoint_timing(a.adjoint_steps)
     if not a.no_ragged:
         out["ragged_graph"] = ragged_graph()
+    if a.multistep_adjoint:
+        out["multistep_adjoint"] = multistep_adjoint(ncol=a.multistep_adjoint)
     if not a.no_cpu_baseline:
         cb, raw = cpu_baseline(ctx, a.cpu_cols, a.cpu_workers)
         out["cpu_baseline"] = cb

@@ -143,10 +143,11 @@ int tempo_hip_tlm(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip
  * t, qv, qc, qr, qi, qs, qg, ni, nr, nc, nwfa, nifa, ng, qb (NULL entries:
  * zero); dbar: cotangents of rain_precip, ice_liquid_equiv_precip,
  * snow_liquid_equiv_precip, graupel_liquid_equiv_precip and refl10cm (other
- * entries ignored).  Writes dL/d(input) of t, qv, qc, qr, qi, qs, qg, nc and
- * nr into the matching entries of xbar (NULL entries are skipped), with the
- * branches frozen as in tempo_hip_tlm.  Working memory (about 3 kB per
- * column-level) is allocated on first use. */
+ * entries ignored).  Writes dL/d(input) of t, qv, qc, qr, qi, qs, qg, nc,
+ * nr, ni, nwfa, nifa, ng and qb into the matching entries of xbar (NULL
+ * entries are skipped and cost nothing), with the branches frozen as in
+ * tempo_hip_tlm; p, w, dz enter as constants.  Working memory (up to about
+ * 4 kB per column-level) is allocated on first use. */
 int tempo_hip_adjoint(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_fields *ybar,
                       const tempo_hip_diags *dbar, const tempo_hip_fields *xbar, const tempo_hip_dims *dims, double dt,
                       int itimestep, int precision, void *stream);

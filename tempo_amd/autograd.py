@@ -6,15 +6,18 @@ backward through the adjoint kernels (tempo_hip_adjoint).
     loss = (a * (out["rain_precip"] + out["snow_liquid_equiv_precip"] + ...)).sum() + (b * out["refl10cm"][levels]).sum()
     loss.backward()                               # grads on state["t"], state["qv"], ... (requires_grad inputs)
 
-Differentiated inputs: t, qv, qc, qr, qi, qs, qg, nc, nr; the other fields
-(p, w, dz, ni, nwfa, nifa, ng, qb) enter as constants.  Differentiated
+Differentiable inputs: t, qv, qc, qr, qi, qs, qg, nc, nr, ni, nwfa, nifa,
+ng, qb (those with requires_grad get gradients, each costs one tangent pass
+of the adjoint); p, w and dz enter as constants.  The outputs cover every
+prognostic field, so steps chain into a multi-step adjoint, each step's
+inputs being its checkpoint.  Differentiated
 outputs: the updated t, qv, qc, qr, qi, qs, qg, ni, nr, nc, nwfa, nifa, ng,
 qb, the four precipitation diagnostics and refl10cm.  Derivatives are
 branch-frozen: every threshold keeps the branch the forward step took.
 """
 import torch
 
-X_NAMES = ("t", "qv", "qc", "qr", "qi", "qs", "qg", "nc", "nr")
+X_NAMES = ("t", "qv", "qc", "qr", "qi", "qs", "qg", "nc", "nr", "ni", "nwfa", "nifa", "ng", "qb")
 Y_NAMES = ("t", "qv", "qc", "qr", "qi", "qs", "qg", "ni", "nr", "nc", "nwfa", "nifa", "ng", "qb")
 D_NAMES = ("rain_precip", "ice_liquid_equiv_precip", "snow_liquid_equiv_precip", "graupel_liquid_equiv_precip",
            "refl10cm")
@@ -40,7 +43,9 @@ class _TempoStep(torch.autograd.Function):
         ny = len(fctx.y_keys)
         ybar = {k: g.contiguous() for k, g in zip(fctx.y_keys, grads[:ny]) if g is not None}
         dbar = {k: g.contiguous() for k, g in zip(D_NAMES, grads[ny:]) if g is not None}
-        xbar = fctx.tempo.adjoint(saved, ybar, dbar, fctx.dt, fctx.itimestep, nlev=fctx.nlev, land=fctx.land)
+        wrt = tuple(k for k, need in zip(fctx.keys, fctx.needs_input_grad[6:]) if need and k in X_NAMES)
+        xbar = fctx.tempo.adjoint(saved, ybar, dbar, fctx.dt, fctx.itimestep, nlev=fctx.nlev, land=fctx.land,
+                                  wrt=wrt) if wrt else {}
         out = [None] * 6 + [xbar.get(k) if k in X_NAMES else None for k in fctx.keys]
         return tuple(out)
 

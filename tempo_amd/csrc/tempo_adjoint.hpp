@@ -33,12 +33,13 @@ enum AdjW { AW_XR, AW_XBAR, AW_FBAR, AW_VB1, AW_VB2, AW_VR, AW_RRPOST, AW_VRBAR,
 // cotangent array handed to tempo_hip_adjoint)
 enum AdjY { AY_T, AY_QV, AY_QC, AY_QR, AY_QI, AY_QS, AY_QG, AY_NI, AY_NR, AY_NC, AY_NWFA, AY_NIFA, AY_NG, AY_QB,
             AY_REFL, AY_N };
-// differentiated inputs
-enum AdjX { AX_T, AX_QV, AX_QC, AX_QR, AX_QI, AX_QS, AX_QG, AX_NC, AX_NR, AX_N };
+// differentiable inputs (every prognostic field, so that steps chain)
+enum AdjX { AX_T, AX_QV, AX_QC, AX_QR, AX_QI, AX_QS, AX_QG, AX_NC, AX_NR, AX_NI, AX_NWFA, AX_NIFA, AX_NG, AX_QB, AX_N };
 
 __host__ __device__ constexpr int adj_x_field(int i) {
   return i == AX_T ? F_T : i == AX_QV ? F_QV : i == AX_QC ? F_QC : i == AX_QR ? F_QR : i == AX_QI ? F_QI
-       : i == AX_QS ? F_QS : i == AX_QG ? F_QG : i == AX_NC ? F_NC : F_NR;
+       : i == AX_QS ? F_QS : i == AX_QG ? F_QG : i == AX_NC ? F_NC : i == AX_NR ? F_NR : i == AX_NI ? F_NI
+       : i == AX_NWFA ? F_NWFA : i == AX_NIFA ? F_NIFA : i == AX_NG ? F_NG : F_QB;
 }
 __host__ __device__ constexpr int adj_y_field(int i) {
   return i == AY_T ? F_T : i == AY_QV ? F_QV : i == AY_QC ? F_QC : i == AY_QR ? F_QR : i == AY_QI ? F_QI

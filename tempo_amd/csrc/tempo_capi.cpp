@@ -554,18 +554,18 @@ int run_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fie
   a.d2ldj = ncol;
   int rc = bind_common(x, a, dt, itimestep);
   if (rc) return rc;
+  void* xp[F_NFIELDS] = {};
+  if (xb) field_ptrs(xb, xp);
   bool have_x[AX_N];
-  for (int i = 0; i < AX_N; ++i) have_x[i] = fp[adj_x_field(i)] != nullptr;
+  for (int i = 0; i < AX_N; ++i) have_x[i] = fp[adj_x_field(i)] && xp[adj_x_field(i)];
   const size_t srb = SR_N * a.plane * sizeof(D), sdb = SD_N * a.plane * sizeof(Dual<double>);
   HIP_TRY(x->adj_snap.ensure(2 * (sb + srb + sdb)));
   char* sp = static_cast<char*>(x->adj_snap.p);
   const AdjSnap sa{sp, sp + sb, sp + sb + srb};
   const AdjSnap sbb{sp + sb + srb + sdb, sp + 2 * sb + srb + sdb, sp + 2 * sb + 2 * srb + sdb};
   HIP_TRY(launch_adjoint(a, g, x->vtbuf.as<Dual<double>>(), in, st, sb, srb, sdb, sa, sbb, have_x, s));
-  void* xp[F_NFIELDS] = {};
-  if (xb) field_ptrs(xb, xp);
   for (int i = 0; i < AX_N; ++i)
-    if (xp[adj_x_field(i)])
+    if (have_x[i])
       HIP_TRY(launch_from_compact(g.xbar + size_t(i) * n3, g3, static_cast<T*>(xp[adj_x_field(i)]), s));
   return TEMPO_HIP_OK;
 }

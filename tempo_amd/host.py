@@ -274,11 +274,12 @@ class TempoHip:
                                    int(itimestep), prec, ctypes.c_void_p(s)))
         return out, tout, tangent
 
-    def adjoint(self, state, ybar, dbar, dt, itimestep, nlev=None, land=None, stream=None):
+    def adjoint(self, state, ybar, dbar, dt, itimestep, nlev=None, land=None, stream=None, wrt=None):
         """Adjoint step (tempo_hip_adjoint).  ``state``: the step's inputs (not
         modified); ``ybar``: cotangents of output fields; ``dbar``:
         cotangents of the precipitation diagnostics and refl10cm.  Returns
-        the input cotangents of t, qv, qc, qr, qi, qs, qg, nc, nr."""
+        the input cotangents of the fields named in ``wrt`` (default t, qv,
+        qc, qr, qi, qs, qg, nc, nr; also ni, nwfa, nifa, ng, qb)."""
         import torch
         ref = state["p"]
         nz, ncol = ref.shape
@@ -286,8 +287,8 @@ class TempoHip:
         f = self._fields(state, nz, ncol, dtype)
         yb = self._fields(ybar, nz, ncol, dtype, "cotangent")
         db = self._diags(dbar, nz, ncol, dtype)
-        xbar = {k: torch.zeros((nz, ncol), dtype=dtype, device=ref.device)
-                for k in ("t", "qv", "qc", "qr", "qi", "qs", "qg", "nc", "nr") if k in state}
+        names = wrt or ("t", "qv", "qc", "qr", "qi", "qs", "qg", "nc", "nr")
+        xbar = {k: torch.zeros((nz, ncol), dtype=dtype, device=ref.device) for k in names if k in state}
         xb = self._fields(xbar, nz, ncol, dtype)
         keep = []
         if land is not None:
