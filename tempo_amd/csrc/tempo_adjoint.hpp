@@ -479,81 +479,4 @@ hipError_t launch_from_compact(const double* src, const DualGeom& g, T* dst, hip
   return hipGetLastError();
 }
 
-// All tangent passes and S^T of one adjoint step.  a: dual step arguments
-// over the working state copy (whose fields point into `work_state`);
-// `inputs` is the packed input state.  snap_a / snap_b receive the working
-// state and scratch after phase A and after phase B of a tangent-free run,
-// so that the later passes restart from them instead of recomputing.
-struct AdjSnap {
-  void* state;       // state_bytes
-  void* sr;          // sr_bytes
-  void* sd;          // sd_bytes
-};
-
-template <class D>
-hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* vtbuf, const void* inputs,
-                          void* work_state, size_t state_bytes, size_t sr_bytes, size_t sd_bytes, AdjSnap snap_a,
-                          AdjSnap snap_b, const bool* have_x, hipStream_t s) {
-  const unsigned gc = unsigned((a.ncol + 255) / 256);
-  const dim3 g2(gc, unsigned(a.nz));
-  const size_t n3 = size_t(a.ncol) * a.nz;
-  hipError_t e;
-  auto cp = [&](void* dst, const void* src, size_t n) { return hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, s); };
-  auto save = [&](const AdjSnap& sn) -> hipError_t {
-    hipError_t r = cp(sn.state, work_state, state_bytes);
-    if (r == hipSuccess) r = cp(sn.sr, a.sr, sr_bytes);
-    if (r == hipSuccess) r = cp(sn.sd, a.sd, sd_bytes);
-    return r;
-  };
-  auto load = [&](const AdjSnap& sn) -> hipError_t {
-    hipError_t r = cp(work_state, sn.state, state_bytes);
-    if (r == hipSuccess) r = cp(a.sr, sn.sr, sr_bytes);
-    if (r == hipSuccess) r = cp(a.sd, sn.sd, sd_bytes);
-    return r;
-  };
-  auto phase_pre_a = [&]() {
-    hipLaunchKernelGGL(k_prepass<D>, dim3(gc), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_phase_a<D>, g2, dim3(256), 0, s, a);
-  };
-  auto phase_c = [&]() { hipLaunchKernelGGL(k_phase_c<D>, g2, dim3(256), 0, s, a); };
-
-  // tangent-free forward: snapshots after A and after B
-  if ((e = cp(work_state, inputs, state_bytes)) != hipSuccess) return e;
-  phase_pre_a();
-  if ((e = save(snap_a)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_phase_b<D>, dim3(unsigned((a.ncol + 127) / 128)), dim3(128), 0, s, a, vtbuf);
-  if ((e = save(snap_b)) != hipSuccess) return e;
-  // passes seeded on the post-sedimentation tendencies
-  for (int m = 0; m < 10; ++m) {
-    if (m > 0 && (e = load(snap_b)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_adj_seed_ten<D>, g2, dim3(256), 0, s, a, m);
-    phase_c();
-    hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, a, g, g.ubar + size_t(m) * n3, 0);
-  }
-  // S^T on the values after phase A
-  if ((e = load(snap_a)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_adj_sedimentation<D>, dim3(unsigned((a.ncol + 63) / 64)), dim3(64), 0, s, a, g);
-  // local passes, one per differentiated input field; phase B is replaced by
-  // the forward's post-sedimentation tendency values
-  const Dual<double>* post_sd = static_cast<const Dual<double>*>(snap_b.sd);
-  for (int i = 0; i < AX_N; ++i) {
-    double* xb = g.xbar + size_t(i) * n3;
-    if (!have_x[i]) {
-      hipLaunchKernelGGL(k_fill_zero, dim3(unsigned((n3 + 255) / 256)), dim3(256), 0, s, xb, n3);
-      hipLaunchKernelGGL(k_fill_zero, dim3(unsigned((AZ_N * n3 + 255) / 256)), dim3(256), 0, s,
-                         g.ja + size_t(i) * AZ_N * n3, AZ_N * n3);
-      continue;
-    }
-    if ((e = cp(work_state, inputs, state_bytes)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_adj_seed_input<D>, g2, dim3(256), 0, s, a, adj_x_field(i));
-    phase_pre_a();
-    hipLaunchKernelGGL(k_adj_capture_z<D>, g2, dim3(256), 0, s, a, g, i);
-    hipLaunchKernelGGL(k_adj_merge_ten<D>, g2, dim3(256), 0, s, a, post_sd);
-    phase_c();
-    hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, a, g, xb, 0);
-  }
-  hipLaunchKernelGGL(k_adj_finish, dim3(unsigned((n3 + 255) / 256)), dim3(256), 0, s, g, n3);
-  return hipGetLastError();
-}
-
 }  // namespace tempo

@@ -11,6 +11,7 @@
 #include "tempo_host.hpp"
 #include "tempo_step.hpp"
 #include "tempo_adjoint.hpp"
+#include "tempo_adjoint_wave.hpp"
 
 namespace tempo {
 // defined in tempo_step_f32.hip / tempo_step_f64.hip / tempo_tables.hip

@@ -2,7 +2,7 @@
 // fluxes carried in FP64 inside phase B).
 #include "tempo_kernels.hpp"
 #include "tempo_sedi.hpp"
-#include "tempo_adjoint.hpp"
+#include "tempo_adjoint_wave.hpp"
 
 namespace tempo {
 

@@ -2,7 +2,7 @@
 // every operation rounds as in the reference FP64 build.
 #include "tempo_kernels.hpp"
 #include "tempo_sedi.hpp"
-#include "tempo_adjoint.hpp"
+#include "tempo_adjoint_wave.hpp"
 
 namespace tempo {
 
