@@ -101,12 +101,15 @@ __global__ void __launch_bounds__(256) k_adj_rho_tangent(StepArgs<D> a, AdjArgs 
   x.d = restore ? decltype(x.d)(g.ja[(size_t(i) * AZ_N + AZ_RHO) * n3 + e]) : decltype(x.d)(0);
 }
 
-// seed the tangent of one pre-update tendency plane (u pass)
+// seed the tangent of one post-sedimentation tendency plane (u pass),
+// clearing the previous pass's seed (m_prev < 0: none)
 template <class D>
-__global__ void __launch_bounds__(256) k_adj_seed_ten(StepArgs<D> a, int m) {
+__global__ void __launch_bounds__(256) k_adj_seed_ten(StepArgs<D> a, int m, int m_prev) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
   if (c >= a.ncol) return;
-  a.sd[size_t(SD_QRTEN + m) * a.plane + TEMPO_IDX_SCR(a, c, k)].d = 1;
+  const size_t s = TEMPO_IDX_SCR(a, c, k);
+  if (m_prev >= 0) a.sd[size_t(SD_QRTEN + m_prev) * a.plane + s].d = 0;
+  a.sd[size_t(SD_QRTEN + m) * a.plane + s].d = 1;
 }
 
 // local pass without phase B: the post-sedimentation tendency values from

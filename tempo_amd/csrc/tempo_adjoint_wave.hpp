@@ -488,9 +488,10 @@ hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* 
   hipLaunchKernelGGL(k_phase_b<D>, dim3(unsigned((a.ncol + 127) / 128)), dim3(128), 0, s, a, vtbuf);
   if ((e = save(snap_b)) != hipSuccess) return e;
   // passes seeded on the post-sedimentation tendencies
+  // (phase C rewrites only the state: restore it, move the seed)
   for (int m = 0; m < 10; ++m) {
-    if (m > 0 && (e = load(snap_b)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_adj_seed_ten<D>, g2, dim3(256), 0, s, a, m);
+    if (m > 0 && (e = cp(work_state, snap_b.state, state_bytes)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_adj_seed_ten<D>, g2, dim3(256), 0, s, a, m, m - 1);
     phase_c();
     hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, a, g, g.ubar + size_t(m) * n3, 0);
   }
