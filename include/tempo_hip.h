@@ -175,10 +175,18 @@ int tempo_hip_substeps(tempo_hip_ctx *ctx, int *host_out, size_t n);
  *                               per column
  *   TEMPO_HIP_TUNE_BLOCK_A   threads per block of the level-local physics
  *   TEMPO_HIP_TUNE_BLOCK_C   ... and of the final update: 64, 128 or 256
+ *   TEMPO_HIP_TUNE_ORDER     point grouping of the level-local kernels, a
+ *                               bit mask: bit 0 (level-local physics), bit 1
+ *                               (final update) set = each block hands its
+ *                               points with microphysics to do to its first
+ *                               wavefronts, so that they do not share
+ *                               wavefronts with idle points; clear = lane i
+ *                               takes column i (default 2)
  * Every setting gives identical results. */
 #define TEMPO_HIP_TUNE_SED_WAVE 1
 #define TEMPO_HIP_TUNE_BLOCK_A 2
 #define TEMPO_HIP_TUNE_BLOCK_C 3
+#define TEMPO_HIP_TUNE_ORDER 4
 int tempo_hip_tune(tempo_hip_ctx *ctx, int precision, int key, int value);
 
 /* Per-phase timing of subsequent runs (no counterpart in the reference):

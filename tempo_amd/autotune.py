@@ -5,7 +5,9 @@ library's own per-phase HIP events (tempo_hip_profile) and keeps the fastest
 choice per phase:
 
   * level-local physics (phase A) and final update (phase C): columns per
-    block, 64 / 128 / 256 (one lane per column at one level);
+    block, 64 / 128 / 256 (one lane per column at one level), and whether
+    each block groups its points with microphysics to do into its first
+    wavefronts;
   * sedimentation: one wavefront per column with LDS-staged column groups,
     or one lane per column.
 
@@ -20,6 +22,7 @@ from . import host
 
 BLOCKS = (64, 128, 256)
 SED = (1, 0)
+ORDERS = (0, 3)  # point grouping off / on in both level-local kernels
 
 _cache = {}
 
@@ -47,29 +50,36 @@ def _phase_ms(ctx, state, dt, itimestep, nlev, reps):
 
 def autotune(ctx, state, dt, itimestep, nlev=None, reps=3, verbose=False):
     """Pick and apply the fastest launch shapes for ``state``'s shape and
-    precision; returns {"block_a": .., "block_c": .., "sed_wave": .., "ms": {...}}."""
+    precision; returns {"block_a": .., "block_c": .., "order": .., "sed_wave": .., "ms": {...}}."""
     import torch
     nz, ncol = state["p"].shape
     precision = host.FP32 if state["p"].dtype == torch.float32 else host.FP64
     key = _key(ctx, precision, nz, ncol)
     if key not in _cache:
+        # phase A's time depends only on (block_a, order bit 0), phase C's on
+        # (block_c, order bit 1) and sedimentation's on sed_wave, so one
+        # sweep of (block, order) with equal blocks covers both
         timings = {}
-        for ba, bc, sw in itertools.product(BLOCKS, BLOCKS, SED):
-            if ba != bc and sw == 0:
-                continue  # the sedimentation choice is independent of the block sizes
-            ctx.tune(precision, host.TUNE_BLOCK_A, ba)
-            ctx.tune(precision, host.TUNE_BLOCK_C, bc)
+        for b, order, sw in itertools.product(BLOCKS, ORDERS, SED):
+            if sw == 0 and (b != BLOCKS[-1] or order != ORDERS[-1]):
+                continue
+            ctx.tune(precision, host.TUNE_BLOCK_A, b)
+            ctx.tune(precision, host.TUNE_BLOCK_C, b)
+            ctx.tune(precision, host.TUNE_ORDER, order)
             ctx.tune(precision, host.TUNE_SED_WAVE, sw)
-            timings[(ba, bc, sw)] = _phase_ms(ctx, state, dt, itimestep, nlev, reps)
+            timings[(b, order, sw)] = _phase_ms(ctx, state, dt, itimestep, nlev, reps)
             if verbose:
-                print("autotune", (ba, bc, sw), [round(float(x), 4) for x in timings[(ba, bc, sw)]])
-        best_a = min(BLOCKS, key=lambda b: min(t[1] for (ba, bc, sw), t in timings.items() if ba == b))
-        best_c = min(BLOCKS, key=lambda b: min(t[3] for (ba, bc, sw), t in timings.items() if bc == b))
-        best_s = min(SED, key=lambda s: min(t[2] for (ba, bc, sw), t in timings.items() if sw == s))
-        _cache[key] = dict(block_a=best_a, block_c=best_c, sed_wave=best_s,
+                print("autotune", (b, order, sw), [round(float(x), 4) for x in timings[(b, order, sw)]])
+        block_a, oa = min(((k[0], k[1] & 1) for k in timings),
+                          key=lambda p: min(t[1] for k, t in timings.items() if (k[0], k[1] & 1) == p))
+        block_c, oc = min(((k[0], k[1] & 2) for k in timings),
+                          key=lambda p: min(t[3] for k, t in timings.items() if (k[0], k[1] & 2) == p))
+        best_s = min(SED, key=lambda s: min(t[2] for k, t in timings.items() if k[2] == s))
+        _cache[key] = dict(block_a=block_a, block_c=block_c, order=oa | oc, sed_wave=best_s,
                            ms={f"{k[0]}/{k[1]}/{k[2]}": [round(float(x), 4) for x in v] for k, v in timings.items()})
     best = _cache[key]
     ctx.tune(precision, host.TUNE_BLOCK_A, best["block_a"])
     ctx.tune(precision, host.TUNE_BLOCK_C, best["block_c"])
+    ctx.tune(precision, host.TUNE_ORDER, best["order"])
     ctx.tune(precision, host.TUNE_SED_WAVE, best["sed_wave"])
     return best

@@ -477,9 +477,9 @@ hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* 
   };
   auto phase_pre_a = [&]() {
     hipLaunchKernelGGL(k_prepass<D>, dim3(gc), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_phase_a<D>, g2, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_phase_a<D>, level_grid(a, 256), dim3(256), 0, s, a);
   };
-  auto phase_c = [&]() { hipLaunchKernelGGL(k_phase_c<D>, g2, dim3(256), 0, s, a); };
+  auto phase_c = [&]() { hipLaunchKernelGGL(k_phase_c<D>, level_grid(a, 256), dim3(256), 0, s, a); };
 
   // tangent-free forward: snapshots after A and after B
   if ((e = cp(work_state, inputs, state_bytes)) != hipSuccess) return e;

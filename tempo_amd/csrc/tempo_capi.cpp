@@ -314,7 +314,7 @@ int pack_tables(tempo_hip_ctx* x) {
   return TEMPO_HIP_OK;
 }
 
-size_t scratch_plane(size_t ncol, size_t nz) { return (ncol + kScrW - 1) / kScrW * kScrW * nz; }
+size_t scratch_plane(size_t ncol, size_t nz) { return (ncol + kScrW - 1) / kScrW * kScrW * size_t(scratch_levels(int(nz))); }
 
 // per-point scratch for ncol columns of nz levels; br / bw: bytes of one
 // working-precision / wide value (8 for the plain step, 16 for duals)
@@ -339,6 +339,7 @@ int bind_common(tempo_hip_ctx* x, StepArgs<R>& a, double dt, int itimestep) {
   if (rc) return rc;
   a.sr = x->sr.as<R>();
   a.plane = scratch_plane(size_t(a.ncol), size_t(a.nz));
+  a.nzs = scratch_levels(a.nz);
   a.sd = x->sd.as<W>();
   a.pf = x->pf.as<unsigned>();
   a.cf = x->cf_.as<unsigned>();
@@ -346,6 +347,7 @@ int bind_common(tempo_hip_ctx* x, StepArgs<R>& a, double dt, int itimestep) {
   a.ksub = x->ksub.as<int>();
   x->last_ncol = a.ncol;
   const bool r64 = sizeof(S) == 8;
+  a.compact = x->launch[r64 ? 1 : 0].order;
   a.tab.qg = static_cast<const S*>(r64 ? x->qg64.p : x->qg32.p);
   a.tab.qs = static_cast<const S*>(r64 ? x->qs64.p : x->qs32.p);
   a.tab.fzr = static_cast<const S*>(r64 ? x->fzr64.p : x->fzr32.p);
@@ -777,6 +779,10 @@ int tempo_hip_tune(tempo_hip_ctx* x, int precision, int key, int value) {
     case TEMPO_HIP_TUNE_BLOCK_C:
       if (value != 64 && value != 128 && value != 256) return fail(TEMPO_HIP_E_ARG, "block must be 64, 128 or 256");
       (key == TEMPO_HIP_TUNE_BLOCK_A ? lc.block_a : lc.block_c) = value;
+      return TEMPO_HIP_OK;
+    case TEMPO_HIP_TUNE_ORDER:
+      if (value < 0 || value > 3) return fail(TEMPO_HIP_E_ARG, "order must be 0..3");
+      lc.order = value;
       return TEMPO_HIP_OK;
     default:
       return fail(TEMPO_HIP_E_ARG, "unknown tuning key");

@@ -4,6 +4,15 @@
 
 #include "tempo_step.hpp"
 
+// register-budget attributes of the level-local kernels (occupancy against
+// spills), chosen at build time
+#ifndef TEMPO_ATTR_A
+#define TEMPO_ATTR_A
+#endif
+#ifndef TEMPO_ATTR_C
+#define TEMPO_ATTR_C
+#endif
+
 namespace tempo {
 
 template <class R>
@@ -69,6 +78,80 @@ __global__ void __launch_bounds__(256) k_prepass(StepArgs<R> a) {
   a.cf[c] = cf;
 }
 
+// ------------------------------------------------ point grouping ----
+// The level-local kernels give one lane per point, and at one level a wave
+// of 64 consecutive columns usually mixes points that run the microphysics
+// (hydrometeors or supersaturated vapour) with points that only pass the
+// checks: the wave then pays the full path at a fraction of its lanes.  A
+// block (up to 256 consecutive columns of one level) therefore partitions
+// its points, active ones first, each group in column order, and lane i
+// takes the i-th point of that order.  The block's loads and stores stay in
+// its own window of columns (shared by its waves through the CU's caches);
+// every point is still computed on its own, so results do not depend on the
+// grouping.
+
+// phase A: hydrometeors, or vapour above liquid or (below 0 C) ice saturation
+template <class R>
+__device__ bool point_active_a(const StepArgs<R>& a, int c, int k) {
+  if ((a.nlev && k >= a.nlev[c]) || (a.cf[c] & CF_EXIT)) return false;
+  const size_t o = TEMPO_IDX_STATE(a, c, k);
+  if (a.f[F_QC][o] > R(kR1) || a.f[F_QR][o] > R(kR1) || a.f[F_QI][o] > R(kR1) || a.f[F_QS][o] > R(kR1) ||
+      a.f[F_QG][o] > R(kR1))
+    return true;
+  const R temp = load_t(a, o), pres = a.f[F_P][o], qv = a.f[F_QV][o];
+  if (qv > rslf(pres, temp)) return true;
+  return temp < R(kT0) && qv > rsif(pres, temp);
+}
+
+// final update: a hydrometeor present after the tendencies of phases A and B
+template <class R>
+__device__ bool point_active_c(const StepArgs<R>& a, int c, int k) {
+  if ((a.nlev && k >= a.nlev[c]) || (a.cf[c] & CF_EXIT)) return false;
+  const size_t o = TEMPO_IDX_STATE(a, c, k), s = TEMPO_IDX_SCR(a, c, k);
+  const R dt = R(a.dt);
+  auto present = [&](int f, int ten) { return a.f[f][o] + R(a.sd[size_t(ten) * a.plane + s]) * dt > R(kR1); };
+  return present(F_QC, SD_QCTEN) || present(F_QR, SD_QRTEN) || present(F_QI, SD_QITEN) ||
+         present(F_QS, SD_QSTEN) || present(F_QG, SD_QGTEN);
+}
+
+// The level-local kernels run blocks of 64 columns x blockDim.x/64 levels,
+// one level per wavefront (blockDim.x = 64, 128 or 256), on a grid of
+// ceil(ncol/64) x ceil(nz/(blockDim.x/64)) blocks.  Point (column, level) of
+// this thread; every thread of the block calls it, and the point may lie
+// outside the batch (c >= ncol or k >= nz) in the last blocks.
+template <class R, bool kFinal>
+__device__ int2 block_point(const StepArgs<R>& a) {
+  const int lane = int(threadIdx.x & 63u), w = int(threadIdx.x >> 6);
+  const int c0 = int(blockIdx.x) * 64, k0 = int(blockIdx.y) * int(blockDim.x >> 6);
+  if (!(a.compact & (kFinal ? 2 : 1))) return make_int2(c0 + lane, k0 + w);
+  __shared__ unsigned short slot_pt[256];
+  __shared__ unsigned wave_act[4];
+  const int c = c0 + lane, k = k0 + w;
+  const bool act = c < a.ncol && k < a.nz && (kFinal ? point_active_c(a, c, k) : point_active_a(a, c, k));
+  const unsigned long long b = __ballot(act);
+  const unsigned below = __builtin_amdgcn_mbcnt_hi(unsigned(b >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(b), 0u));
+  const unsigned nw = blockDim.x >> 6;
+  if (lane == 0) wave_act[w] = unsigned(__popcll(b));
+  __syncthreads();
+  unsigned before = below, nact = 0;  // active points before this thread, and in the block
+  for (unsigned i = 0; i < nw; ++i) {
+    if (int(i) < w) before += wave_act[i];
+    nact += wave_act[i];
+  }
+  const unsigned pos = act ? before : nact + (threadIdx.x - before);
+  slot_pt[pos] = static_cast<unsigned short>(threadIdx.x);
+  __syncthreads();
+  const unsigned t = slot_pt[threadIdx.x];
+  return make_int2(c0 + int(t & 63u), k0 + int(t >> 6));
+}
+
+// grid of a level-local kernel for blocks of `threads` threads
+template <class R>
+dim3 level_grid(const StepArgs<R>& a, int threads) {
+  const int lv = threads / 64;
+  return dim3(unsigned((a.ncol + 63) / 64), unsigned((a.nz + lv - 1) / lv));
+}
+
 // ----------------------------------------------------------- phase A ----
 template <class R>
 __device__ __forceinline__ R default_aerosol(const StepArgs<R>& a, int c, int k, R n0, R n1) {
@@ -82,10 +165,10 @@ __device__ __forceinline__ R default_aerosol(const StepArgs<R>& a, int c, int k,
 }
 
 template <class R>
-__global__ void __launch_bounds__(256) k_phase_a(StepArgs<R> a) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = blockIdx.y;
-  if (c >= a.ncol) return;
+__global__ void __launch_bounds__(256) TEMPO_ATTR_A k_phase_a(StepArgs<R> a) {
+  const int2 pt = block_point<R, false>(a);
+  const int c = pt.x, k = pt.y;
+  if (c >= a.ncol || k >= a.nz) return;
   if (a.nlev && k >= a.nlev[c]) return;
   const Const<S_t<R>>& C = *a.cst;
   const size_t o = TEMPO_IDX_STATE(a, c, k);
@@ -123,7 +206,7 @@ __global__ void __launch_bounds__(256) k_phase_a(StepArgs<R> a) {
   if (L.l_qs) snow_moments(C, L.rs, mn(R(-0.1), L.temp - R(kT0)), 1 | 2 | 4 | 8 | 16, L);
   const R ncv = column_nc(a, c);
   L.ncsave = ncv;
-  cloud_check(C, L, qc1d, has_nc ? &nc1d : nullptr, ncv, dt, odt);
+  cloud_check(C, L, qc1d, nc1d, has_nc, ncv, dt, odt);
   if ((cf & CF_GINIT) && qg1d > R(kR1)) {  // graupel_init (:1234-1260)
     const R rg = qg1d * L.rho;
     const R rb = rg * R(kM3ToL) / rho_g<R>(kIdxBg1);
@@ -133,7 +216,7 @@ __global__ void __launch_bounds__(256) k_phase_a(StepArgs<R> a) {
     ng1d = ng / L.rho;
     qb1d = rb / L.rho;
   }
-  graupel_check(C, L, qg1d, has_ngqb ? &ng1d : nullptr, has_ngqb ? &qb1d : nullptr, dt, odt);
+  graupel_check(C, L, qg1d, ng1d, qb1d, has_ngqb, dt, odt);
   // the post-check state is the 1-d state for the rest of the step
   a.f[F_QC][o] = qc1d;
   a.f[F_QR][o] = qr1d;
@@ -152,11 +235,7 @@ __global__ void __launch_bounds__(256) k_phase_a(StepArgs<R> a) {
   thermo(C, L);
 
   const bool micro = !a.fl.turn_off_micro;
-  Tend<R> t;
-  {
-    R* tp = reinterpret_cast<R*>(&t);
-    for (unsigned q = 0; q < sizeof(Tend<R>) / sizeof(R); ++q) tp[q] = R(0);
-  }
+  Tend<R> t{};  // every process rate starts at zero
   R vtboost = R(1.0);
   if (micro) {
     warm_rain(C, a.tab, L, t, odt);
@@ -183,7 +262,7 @@ __global__ void __launch_bounds__(256) k_phase_a(StepArgs<R> a) {
   L.nifa = mx(R(kNifaDefault) * L.rho, mn(R(kAeroMax) * L.rho, L.nifa));
   {
     R x = qc1d, xn = nc1d;
-    cloud_check(C, L, x, has_nc ? &xn : nullptr, ncv, dt, odt);
+    cloud_check(C, L, x, xn, has_nc, ncv, dt, odt);
     x = qr1d;
     xn = nr1d;
     rain_check(C, L, x, xn, dt, odt);
@@ -196,7 +275,7 @@ __global__ void __launch_bounds__(256) k_phase_a(StepArgs<R> a) {
     if (L.l_qs) snow_moments(C, L.rs, mn(R(-0.1), L.temp - R(kT0)), 4, L);
     x = qg1d;
     R xg = ng1d, xb = qb1d;
-    graupel_check(C, L, x, has_ngqb ? &xg : nullptr, has_ngqb ? &xb : nullptr, dt, odt);
+    graupel_check(C, L, x, xg, xb, has_ngqb, dt, odt);
   }
   thermo(C, L);
   if (micro && a.fl.cloud_condensation) {  // (:528-560)
@@ -207,7 +286,7 @@ __global__ void __launch_bounds__(256) k_phase_a(StepArgs<R> a) {
     L.nwfaten = L.nwfaten - t.pnc_wcd;
     L.tten = L.tten + L.lvap * L.ocp * t.prw_vcd;
     R x = qc1d, xn = nc1d;
-    cloud_check(C, L, x, has_nc ? &xn : nullptr, ncv, dt, odt);
+    cloud_check(C, L, x, xn, has_nc, ncv, dt, odt);
     L.qv = mx(R(kMinQv), qv1d + L.qvten * dt);
     L.temp = t1d + L.tten * dt;
     L.rho = rho_of(C, L.pres, L.temp, L.qv);
@@ -516,10 +595,10 @@ __device__ __forceinline__ void atomic_max_pos(Dual<T>* p, Dual<T> v) {
 }
 
 template <class R>
-__global__ void __launch_bounds__(256) k_phase_c(StepArgs<R> a) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = blockIdx.y;
-  if (c >= a.ncol) return;
+__global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
+  const int2 pt = block_point<R, true>(a);
+  const int c = pt.x, k = pt.y;
+  if (c >= a.ncol || k >= a.nz) return;
   if (a.nlev && k >= a.nlev[c]) return;
   const Const<S_t<R>>& C = *a.cst;
   const size_t o = TEMPO_IDX_STATE(a, c, k);
@@ -591,13 +670,13 @@ __global__ void __launch_bounds__(256) k_phase_c(StepArgs<R> a) {
   if (has_nwfa) a.f[F_NWFA][o] = mx(R(kNwfaDefault), mn(R(kAeroMax), (a.f[F_NWFA][o] + L.nwfaten * dt)));
   if (has_nifa) a.f[F_NIFA][o] = mx(R(kNifaDefault), mn(R(kAeroMax), (a.f[F_NIFA][o] + L.nifaten * dt)));
   // final checks (:750-781)
-  cloud_check(C, L, qc1d, has_nc ? &nc1d : nullptr, ncv, dt, odt);
+  cloud_check(C, L, qc1d, nc1d, has_nc, ncv, dt, odt);
   rain_check(C, L, qr1d, nr1d, dt, odt);
   ice_check(C, L, qi1d, ni1d, dt, odt);
   snow_check(L, qs1d, dt, odt);
   L.smob = L.smoc = L.smo2 = L.smoz = R(0);
   if (L.l_qs) snow_moments(C, L.rs, mn(R(-0.1), L.temp - R(kT0)), 4 | 32, L);
-  graupel_check(C, L, qg1d, has_ngqb ? &ng1d : nullptr, has_ngqb ? &qb1d : nullptr, dt, odt);
+  graupel_check(C, L, qg1d, ng1d, qb1d, has_ngqb, dt, odt);
 
   if (a.f[F_T])
     a.f[F_T][o] = t_new;
@@ -730,9 +809,9 @@ template <class D>
 hipError_t launch_step_tlm_impl(const StepArgs<D>& a, Dual<double>* vtbuf, hipStream_t s) {
   const unsigned gc = unsigned((a.ncol + 255) / 256);
   hipLaunchKernelGGL(k_prepass<D>, dim3(gc), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(k_phase_a<D>, dim3(gc, unsigned(a.nz)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_phase_a<D>, level_grid(a, 256), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_phase_b<D>, dim3(unsigned((a.ncol + 127) / 128)), dim3(128), 0, s, a, vtbuf);
-  hipLaunchKernelGGL(k_phase_c<D>, dim3(gc, unsigned(a.nz)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_phase_c<D>, level_grid(a, 256), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

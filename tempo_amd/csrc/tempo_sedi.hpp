@@ -69,13 +69,14 @@ __device__ __forceinline__ T above(const T (&x)[kSedChunks], int j) {
 struct SedTile {
   double* lds;  // kSedSlots x kSedCols x kSedRow
   int c0, ncol, nz;
+  int nzs;      // level stride of a scratch column group
 
   __device__ double* slot(int s) const { return lds + s * (kSedCols * kSedRow); }
   // scratch planes: the block's column group is one contiguous run
   template <class T>
   __device__ void load(int s, const T* plane) const {
     double* d = slot(s);
-    const T* src = plane + size_t(c0) * nz;
+    const T* src = plane + size_t(c0) * nzs;
     for (int e = threadIdx.x; e < kSedCols * nz; e += blockDim.x) {
       const int cc = e % kSedCols, k = e / kSedCols;
       d[cc * kSedRow + k] = double(src[e]);
@@ -92,7 +93,7 @@ struct SedTile {
   template <class T>
   __device__ void store(int s, T* plane) const {
     const double* d = slot(s);
-    T* dst = plane + size_t(c0) * nz;
+    T* dst = plane + size_t(c0) * nzs;
     for (int e = threadIdx.x; e < kSedCols * nz; e += blockDim.x) {
       const int cc = e % kSedCols, k = e / kSedCols;
       if (c0 + cc < ncol) dst[e] = T(d[cc * kSedRow + k]);
@@ -208,7 +209,7 @@ __global__ void __launch_bounds__(64 * kSedCols) k_phase_b_wave(StepArgs<R> a) {
   const int c = c0 + w;
   const bool col_ok = c < a.ncol;
   const size_t plane = a.plane;
-  const SedTile T{tile_mem, c0, a.ncol, a.nz};
+  const SedTile T{tile_mem, c0, a.ncol, a.nz, a.nzs};
   const double dt = a.dt;
   double* sd = a.sd;
   const R* sr = a.sr;

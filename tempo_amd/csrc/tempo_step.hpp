@@ -159,15 +159,20 @@ struct StepArgs {
   const int* land;   // per column land flag or nullptr
   const int* nlev;   // per column active level count or nullptr
   int ni, nj, nz, ncol;
+  int nzs;           // level stride of a scratch column group (scratch_levels(nz))
   long long ldk, ldj, dldk, dldj, d2ldj;
   R* sr;             // SR_N planes
   W_t<R>* sd;        // SD_N planes (FP64, or FP64 duals)
-  size_t plane;      // elements per scratch plane: nz x ncol rounded up to kScrW columns
+  size_t plane;      // elements per scratch plane: scratch_levels(nz) x ncol rounded up to kScrW columns
   unsigned* pf;      // nz x ncol point flags
   unsigned* cf;      // ncol column flags
   W_t<R>* precip;    // 5 x ncol: rain, graupel, snow, ice, cloud
   int* ksub;         // 10 x ncol: substeps and ktop per species (diagnostic)
   const Const<S_t<R>>* cst;  // derived constants of this context (wave-uniform reads)
+  // level-local kernels: bit 0 (phase A) / bit 1 (final update) set = each
+  // block hands its level's points with microphysics to do to its first
+  // lanes, clear = lane i takes column i
+  int compact;
   Tables<S_t<R>> tab;
   RunFlags fl;
   double dt;
@@ -184,18 +189,24 @@ struct DualGeom {
 // launch shape of one step (host-side choice, see tempo_hip_tune)
 struct LaunchCfg {
   int sed_wave = 1;   // sedimentation: 1 = one wavefront per column (nz <= 128), 0 = one lane per column
+  int order = 2;      // point grouping within blocks: bit 0 phase A, bit 1 final update
   int block_a = 256;  // threads per block of the level-local kernels (64, 128 or 256)
   int block_c = 256;
 };
 
 #define TEMPO_IDX_STATE(a, c, k) (size_t((c) % (a).ni) + size_t(k) * (a).ldk + size_t((c) / (a).ni) * (a).ldj)
 // scratch planes are column-tiled: groups of kScrW consecutive columns hold
-// their nz levels contiguously, (c / kScrW, k, c % kScrW), so that a level-
-// local kernel (lanes over columns) writes whole segments and a column
-// kernel reads one contiguous block per group
+// their levels contiguously, (c / kScrW, k, c % kScrW), so that a column
+// kernel reads one contiguous block per group.  The level stride of a group
+// is nz rounded up to kScrLevels, so that groups start on 128-byte lines and
+// the 128-byte line holding levels 4m..4m+3 (FP32) or 2m, 2m+1 (FP64) of a
+// group is used whole by one block of a level-local kernel (64 columns x 2-4
+// levels).
 constexpr int kScrW = 8;
+constexpr int kScrLevels = 16;
+__host__ __device__ constexpr int scratch_levels(int nz) { return (nz + kScrLevels - 1) / kScrLevels * kScrLevels; }
 #define TEMPO_IDX_SCR(a, c, k) \
-  ((size_t((c) / kScrW) * size_t((a).nz) + size_t(k)) * kScrW + size_t((c) % kScrW))
+  ((size_t((c) / kScrW) * size_t((a).nzs) + size_t(k)) * kScrW + size_t((c) % kScrW))
 
 // ------------------------------------------------ level-local physics ----
 // Per-level working state: the scalars of one (column, level) point.
@@ -338,9 +349,9 @@ __device__ void thermo(const Const<S_t<R>>& C, Lvl<R>& L) {
 }
 
 // ---- *_check_and_update for one level (src/module_mp_tempo_main.F90:891-1231) ----
-// ncv: the column's one-moment cloud number (used when nc1d is absent)
+// ncv: the column's one-moment cloud number (used when nc is absent, has_nc false)
 template <class R>
-__device__ void cloud_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qc1d, R* nc1d, R ncv, R dt, R odt) {
+__device__ void cloud_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qc1d, R& nc1d, bool has_nc, R ncv, R dt, R odt) {
   bool hit = false;
   if (qc1d + L.qcten * dt > R(kR1)) {
     L.l_qc = true;
@@ -348,8 +359,8 @@ __device__ void cloud_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qc1d, R* nc1d,
     qc1d = qc1d + L.qcten * dt;
     int nu;
     R lamc;
-    if (nc1d) {
-      L.nc = mx(R(kNtCMin), (*nc1d + L.ncten * dt) * L.rho);
+    if (has_nc) {
+      L.nc = mx(R(kNtCMin), (nc1d + L.ncten * dt) * L.rho);
       if (L.nc <= R(kNtCMin)) { hit = true; L.nc = R(kNtCMin); }
       if (L.nc > R(kNtCMax)) { hit = true; L.nc = R(kNtCMax); }
       nu = get_nuc(L.nc);
@@ -363,8 +374,8 @@ __device__ void cloud_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qc1d, R* nc1d,
         hit = true;
       }
       L.nc = C.ccg[0][nu] * C.ocg2[nu] * L.rc / C.am_r * mpow(lamc, R(kBmR));
-      if (hit) L.ncten = (L.nc / L.rho - *nc1d) * odt;
-      *nc1d = mx(R(kNtCMin) / L.rho,
+      if (hit) L.ncten = (L.nc / L.rho - nc1d) * odt;
+      nc1d = mx(R(kNtCMin) / L.rho,
                  mn(C.ccg[0][nu] * C.ocg2[nu] * qc1d / C.am_r * mpow(lamc, R(kBmR)), R(kNtCMax) / L.rho));
     } else {
       L.nc = ncv;
@@ -381,9 +392,9 @@ __device__ void cloud_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qc1d, R* nc1d,
     L.ilamc = R(0.0);
     L.qcten = -qc1d * odt;
     qc1d = R(0.0);
-    if (nc1d) {
-      L.ncten = -*nc1d * odt;
-      *nc1d = R(0.0);
+    if (has_nc) {
+      L.ncten = -nc1d * odt;
+      nc1d = R(0.0);
     }
   }
 }
@@ -493,7 +504,7 @@ __device__ R rho_g(int idx) {  // densities of graupel (src/module_mp_tempo_para
 }
 
 template <class R>
-__device__ void graupel_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qg1d, R* ng1d, R* qb1d, R dt, R odt) {
+__device__ void graupel_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qg1d, R& ng1d, R& qb1d, bool has_ngqb, R dt, R odt) {
   bool hit = false;
   if (qg1d + L.qgten * dt > R(kR1)) {
     L.l_qg = true;
@@ -501,9 +512,9 @@ __device__ void graupel_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qg1d, R* ng1
     qg1d = qg1d + L.qgten * dt;
     R lamg;
     const R c3 = R(3.0) + R(kMuG) + R(0.672);
-    if (ng1d && qb1d) {
-      L.ng = mx(R(kR2), (*ng1d + L.ngten * dt) * L.rho);
-      L.rb = mn(mx(L.rg * R(kM3ToL) / rho_g<R>(kNrhg), (*qb1d + L.qbten * dt) * L.rho),
+    if (has_ngqb) {
+      L.ng = mx(R(kR2), (ng1d + L.ngten * dt) * L.rho);
+      L.rb = mn(mx(L.rg * R(kM3ToL) / rho_g<R>(kNrhg), (qb1d + L.qbten * dt) * L.rho),
                 L.rg * R(kM3ToL) / rho_g<R>(1));
       L.idx_bg = max(1, min(nint_exact(R(10.0) * L.rg / L.rb) + 1, kNrhg));
       if (L.ng <= R(kR2)) {
@@ -525,10 +536,10 @@ __device__ void graupel_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qg1d, R* ng1
         lamg = c3 / L.mvd_g;
         L.ng = C.cgg[1][0] * C.ogg3 * L.rg * mpow(lamg, R(kBmG)) / C.am_g[L.idx_bg - 1];
       }
-      if (hit) L.ngten = (L.ng / L.rho - *ng1d) * odt;
-      *ng1d = C.cgg[1][0] * C.ogg3 * qg1d * mpow(lamg, R(kBmG)) / C.am_g[L.idx_bg - 1];
-      *qb1d = mn(mx(qg1d * R(kM3ToL) / rho_g<R>(kNrhg), *qb1d + L.qbten * dt), R(kM3ToL) * qg1d / rho_g<R>(1));
-      L.idx_bg = max(1, min(nint_exact(R(10.0) * qg1d / *qb1d) + 1, kNrhg));
+      if (hit) L.ngten = (L.ng / L.rho - ng1d) * odt;
+      ng1d = C.cgg[1][0] * C.ogg3 * qg1d * mpow(lamg, R(kBmG)) / C.am_g[L.idx_bg - 1];
+      qb1d = mn(mx(qg1d * R(kM3ToL) / rho_g<R>(kNrhg), qb1d + L.qbten * dt), R(kM3ToL) * qg1d / rho_g<R>(1));
+      L.idx_bg = max(1, min(nint_exact(R(10.0) * qg1d / qb1d) + 1, kNrhg));
     } else {
       L.idx_bg = kIdxBg1;
       const R ygra1 = mlog10(mx(R(1.e-9), L.rg));
@@ -551,11 +562,11 @@ __device__ void graupel_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qg1d, R* ng1
     L.rb = R(kM3ToL) * R(kR1) / rho_g<R>(L.idx_bg);
     L.qgten = -qg1d * odt;
     qg1d = R(0.0);
-    if (ng1d && qb1d) {
-      L.ngten = -*ng1d * odt;
-      L.qbten = -*qb1d * odt;
-      *ng1d = R(0.0);
-      *qb1d = R(0.0);
+    if (has_ngqb) {
+      L.ngten = -ng1d * odt;
+      L.qbten = -qb1d * odt;
+      ng1d = R(0.0);
+      qb1d = R(0.0);
     }
   }
 }

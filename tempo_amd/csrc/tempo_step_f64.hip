@@ -9,12 +9,10 @@ namespace tempo {
 // ev (optional): 5 events recorded around the 4 launches for per-phase timing
 hipError_t launch_step(const StepArgs<double>& a, const LaunchCfg& lc, double* vtbuf, hipStream_t s, hipEvent_t* ev) {
   const unsigned gc = unsigned((a.ncol + 255) / 256);
-  const unsigned ga = unsigned((a.ncol + lc.block_a - 1) / lc.block_a);
-  const unsigned gcc = unsigned((a.ncol + lc.block_c - 1) / lc.block_c);
   if (ev) (void)hipEventRecord(ev[0], s);
   hipLaunchKernelGGL(k_prepass<double>, dim3(gc), dim3(256), 0, s, a);
   if (ev) (void)hipEventRecord(ev[1], s);
-  hipLaunchKernelGGL(k_phase_a<double>, dim3(ga, unsigned(a.nz)), dim3(unsigned(lc.block_a)), 0, s, a);
+  hipLaunchKernelGGL(k_phase_a<double>, level_grid(a, lc.block_a), dim3(unsigned(lc.block_a)), 0, s, a);
   if (ev) (void)hipEventRecord(ev[2], s);
   if (lc.sed_wave && a.nz <= kSedLevels)
     hipLaunchKernelGGL(k_phase_b_wave<double>, dim3(unsigned((a.ncol + kSedCols - 1) / kSedCols)), dim3(64 * kSedCols),
@@ -22,7 +20,7 @@ hipError_t launch_step(const StepArgs<double>& a, const LaunchCfg& lc, double* v
   else
     hipLaunchKernelGGL(k_phase_b<double>, dim3(unsigned((a.ncol + 127) / 128)), dim3(128), 0, s, a, vtbuf);
   if (ev) (void)hipEventRecord(ev[3], s);
-  hipLaunchKernelGGL(k_phase_c<double>, dim3(gcc, unsigned(a.nz)), dim3(unsigned(lc.block_c)), 0, s, a);
+  hipLaunchKernelGGL(k_phase_c<double>, level_grid(a, lc.block_c), dim3(unsigned(lc.block_c)), 0, s, a);
   if (ev) (void)hipEventRecord(ev[4], s);
   return hipGetLastError();
 }

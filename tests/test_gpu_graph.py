@@ -73,7 +73,7 @@ def test_autotune_choice_keeps_results(hip):
     a = {k: v.clone() for k, v in st0.items()}
     ctx.run(a, m["dt"], m["itimestep"])
     best = autotune.autotune(ctx, st0, m["dt"], m["itimestep"])
-    print("autotuned", {k: best[k] for k in ("block_a", "block_c", "sed_wave")})
+    print("autotuned", {k: best[k] for k in ("block_a", "block_c", "order", "sed_wave")})
     b = {k: v.clone() for k, v in st0.items()}
     ctx.run(b, m["dt"], m["itimestep"])
     torch.cuda.synchronize()

@@ -325,9 +325,10 @@ def test_gpu_table_file_roundtrip(hip):
 
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
 def test_gpu_launch_shapes_give_identical_results(hip, dtype):
-    """Every tuning setting (sedimentation kernel, block sizes) produces the
-    same bits: the wavefront-per-column sedimentation performs the serial
-    kernel's arithmetic level by level."""
+    """Every tuning setting (sedimentation kernel, block sizes, point order)
+    produces the same bits: the wavefront-per-column sedimentation performs
+    the serial kernel's arithmetic level by level, and the point order only
+    regroups independent points into wavefronts."""
     from tempo_amd import generator as gen
     tdt = getattr(torch, dtype)
     prec = hip.FP32 if dtype == "float32" else hip.FP64
@@ -337,10 +338,12 @@ def test_gpu_launch_shapes_give_identical_results(hip, dtype):
         ctx = context(hip, flags)
         nl = None if "nlev" not in m else torch.tensor(m["nlev"], dtype=torch.int32, device="cuda")
         results = []
-        for sed, ba, bc in ((0, 256, 256), (1, 256, 256), (1, 64, 128), (1, 128, 64)):
+        for sed, ba, bc, order in ((0, 256, 256, 0), (1, 256, 256, 0), (1, 64, 128, 0), (1, 128, 64, 1),
+                                   (1, 256, 256, 3), (0, 64, 64, 2), (1, 128, 256, 2)):
             ctx.tune(prec, hip.TUNE_SED_WAVE, sed)
             ctx.tune(prec, hip.TUNE_BLOCK_A, ba)
             ctx.tune(prec, hip.TUNE_BLOCK_C, bc)
+            ctx.tune(prec, hip.TUNE_ORDER, order)
             dev = {k: torch.tensor(v, dtype=tdt, device="cuda") for k, v in st.items()}
             d = ctx.run(dev, m["dt"], m["itimestep"], nlev=nl)
             torch.cuda.synchronize()
@@ -350,6 +353,7 @@ def test_gpu_launch_shapes_give_identical_results(hip, dtype):
         ctx.tune(prec, hip.TUNE_SED_WAVE, 1)
         ctx.tune(prec, hip.TUNE_BLOCK_A, 256)
         ctx.tune(prec, hip.TUNE_BLOCK_C, 256)
+        ctx.tune(prec, hip.TUNE_ORDER, 2)
         base = results[0]
         for r in results[1:]:
             for k in base[0]:
@@ -362,7 +366,8 @@ def test_gpu_launch_shapes_give_identical_results(hip, dtype):
 
 def test_gpu_tune_rejects_bad_values(hip):
     ctx = context(hip, dict(aerosolaware=False, hailaware=False))
-    for key, val in ((hip.TUNE_SED_WAVE, 2), (hip.TUNE_BLOCK_A, 96), (hip.TUNE_BLOCK_C, 512), (99, 1)):
+    for key, val in ((hip.TUNE_SED_WAVE, 2), (hip.TUNE_BLOCK_A, 96), (hip.TUNE_BLOCK_C, 512), (hip.TUNE_ORDER, 4),
+                     (99, 1)):
         with pytest.raises(hip.TempoError) as e:
             ctx.tune(hip.FP32, key, val)
         assert e.value.code == hip.E_ARG
