@@ -181,7 +181,7 @@ int tempo_hip_substeps(tempo_hip_ctx *ctx, int *host_out, size_t n);
  *                               points with microphysics to do to its first
  *                               wavefronts, so that they do not share
  *                               wavefronts with idle points; clear = lane i
- *                               takes column i (default 2)
+ *                               takes column i (default 0)
  * Every setting gives identical results. */
 #define TEMPO_HIP_TUNE_SED_WAVE 1
 #define TEMPO_HIP_TUNE_BLOCK_A 2

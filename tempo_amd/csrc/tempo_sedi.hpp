@@ -30,7 +30,10 @@ constexpr int kSedCols = kScrW;           // columns (wavefronts) per block: one
 constexpr int kSedSlots = 8;              // LDS planes staged at once
 constexpr int kSedChunks = 2;             // register chunks of 64 levels
 constexpr int kSedLevels = 64 * kSedChunks;
-constexpr int kSedRow = kSedLevels + 1;   // padded LDS row of one column
+// LDS row of one column: nz + 1 doubles (odd, so that the 8 columns of a
+// level fall in different banks); the block's LDS is sized at launch
+__host__ __device__ constexpr int sed_row(int nz) { return nz + 1; }
+__host__ __device__ constexpr size_t sed_lds_bytes(int nz) { return size_t(kSedSlots) * kSedCols * sed_row(nz) * sizeof(double); }
 
 namespace wv {
 __device__ __forceinline__ int lane() { return int(threadIdx.x & 63u); }
@@ -67,11 +70,12 @@ __device__ __forceinline__ T above(const T (&x)[kSedChunks], int j) {
 // and stores issue without barriers so that all planes of a stage are in
 // flight together; the caller separates the phases with __syncthreads().
 struct SedTile {
-  double* lds;  // kSedSlots x kSedCols x kSedRow
+  double* lds;  // kSedSlots x kSedCols x row
   int c0, ncol, nz;
   int nzs;      // level stride of a scratch column group
+  int row;      // LDS row of one column (sed_row(nz))
 
-  __device__ double* slot(int s) const { return lds + s * (kSedCols * kSedRow); }
+  __device__ double* slot(int s) const { return lds + s * (kSedCols * row); }
   // scratch planes: the block's column group is one contiguous run
   template <class T>
   __device__ void load(int s, const T* plane) const {
@@ -79,7 +83,7 @@ struct SedTile {
     const T* src = plane + size_t(c0) * nzs;
     for (int e = threadIdx.x; e < kSedCols * nz; e += blockDim.x) {
       const int cc = e % kSedCols, k = e / kSedCols;
-      d[cc * kSedRow + k] = double(src[e]);
+      d[cc * row + k] = double(src[e]);
     }
   }
   template <class R>
@@ -87,7 +91,7 @@ struct SedTile {
     double* d = slot(s);
     for (int e = threadIdx.x; e < kSedCols * nz; e += blockDim.x) {
       const int cc = e % kSedCols, k = e / kSedCols, c = c0 + cc;
-      d[cc * kSedRow + k] = (c < ncol) ? double(f[TEMPO_IDX_STATE(a, c, k)]) : 0.0;
+      d[cc * row + k] = (c < ncol) ? double(f[TEMPO_IDX_STATE(a, c, k)]) : 0.0;
     }
   }
   template <class T>
@@ -96,7 +100,7 @@ struct SedTile {
     T* dst = plane + size_t(c0) * nzs;
     for (int e = threadIdx.x; e < kSedCols * nz; e += blockDim.x) {
       const int cc = e % kSedCols, k = e / kSedCols;
-      if (c0 + cc < ncol) dst[e] = T(d[cc * kSedRow + k]);
+      if (c0 + cc < ncol) dst[e] = T(d[cc * row + k]);
     }
   }
   // this wavefront's column <-> registers
@@ -106,7 +110,7 @@ struct SedTile {
 #pragma unroll
     for (int j = 0; j < kSedChunks; ++j) {
       const int k = j * 64 + wv::lane();
-      x[j] = (k < nz) ? T(d[w * kSedRow + k]) : T(0);
+      x[j] = (k < nz) ? T(d[w * row + k]) : T(0);
     }
   }
   template <class T>
@@ -115,7 +119,7 @@ struct SedTile {
 #pragma unroll
     for (int j = 0; j < kSedChunks; ++j) {
       const int k = j * 64 + wv::lane();
-      if (k < nz) d[w * kSedRow + k] = double(x[j]);
+      if (k < nz) d[w * row + k] = double(x[j]);
     }
   }
 };
@@ -201,15 +205,15 @@ __device__ __forceinline__ double wave_sediment(const SedCol& g, double (&xr)[kS
 }
 
 template <class R>
-__global__ void __launch_bounds__(64 * kSedCols) k_phase_b_wave(StepArgs<R> a) {
-  __shared__ double tile_mem[kSedSlots * kSedCols * kSedRow];
+__global__ void __launch_bounds__(64 * kSedCols) __attribute__((amdgpu_waves_per_eu(6))) k_phase_b_wave(StepArgs<R> a) {
+  extern __shared__ double tile_mem[];  // sed_lds_bytes(nz)
   __shared__ unsigned blk_any;
   const int w = int(threadIdx.x >> 6);
   const int c0 = int(blockIdx.x) * kSedCols;
   const int c = c0 + w;
   const bool col_ok = c < a.ncol;
   const size_t plane = a.plane;
-  const SedTile T{tile_mem, c0, a.ncol, a.nz, a.nzs};
+  const SedTile T{tile_mem, c0, a.ncol, a.nz, a.nzs, sed_row(a.nz)};
   const double dt = a.dt;
   double* sd = a.sd;
   const R* sr = a.sr;

@@ -32,9 +32,35 @@ namespace tempo {
 
 // ---------------------------------------------------------------- math ----
 // FP64: double-double evaluations rounded once (tempo_crmath.hpp), so the
-// twin rounds as the reference's host libm does; FP32: the device libm.
+// twin rounds as the reference's host libm does; FP32: the hardware log2 and
+// exp2 behind an exact range reduction for x**y, the device libm otherwise.
+
+// x**y in FP32 (x >= 0, as every power of the step): x = m 2^e with m in
+// [sqrt(1/2), sqrt(2)), so log2 x = e + log2 m where e is exact and the
+// hardware log2 sees |log2 m| < 1/2 (absolute error near 2^-24); y*e is
+// carried as a rounded product plus its exact fma residual, split into an
+// integer n and a fraction |f| <= 1/2 + small, and x**y = 2^f 2^n with the
+// hardware exp2 on f.  Error: a few units in the last place, against the
+// 1-ulp device powf at several times the instruction count.
+__device__ __forceinline__ float pow_f32(float x, float y) {
+  if (y == 0.0f) return 1.0f;
+  int e = __builtin_amdgcn_frexp_expf(x);
+  float m = __builtin_amdgcn_frexp_mantf(x);  // x = m 2^e, m in [1/2, 1) (0 for x = 0)
+  if (m < 0.70710678f) {
+    m = m + m;
+    e = e - 1;
+  }
+  const float lm = __builtin_amdgcn_logf(m);  // log2 m (-inf for x = 0)
+  const float fe = float(e);
+  const float p = y * fe;
+  const float perr = __builtin_fmaf(y, fe, -p);  // y*e = p + perr exactly
+  const float n = __builtin_rintf(p);
+  const float f = (p - n) + __builtin_fmaf(y, lm, perr);  // p - n is exact
+  return __builtin_ldexpf(__builtin_amdgcn_exp2f(f), int(n));
+}
+
 __device__ __forceinline__ double mpow(double x, double y) { return crm::pow_cr(x, y); }
-__device__ __forceinline__ float mpow(float x, float y) { return powf(x, y); }
+__device__ __forceinline__ float mpow(float x, float y) { return pow_f32(x, y); }
 __device__ __forceinline__ double mexp(double x) { return crm::exp_cr(x); }
 __device__ __forceinline__ float mexp(float x) { return expf(x); }
 __device__ __forceinline__ double mlog(double x) { return crm::log_cr(x); }
@@ -189,7 +215,7 @@ struct DualGeom {
 // launch shape of one step (host-side choice, see tempo_hip_tune)
 struct LaunchCfg {
   int sed_wave = 1;   // sedimentation: 1 = one wavefront per column (nz <= 128), 0 = one lane per column
-  int order = 2;      // point grouping within blocks: bit 0 phase A, bit 1 final update
+  int order = 0;      // point grouping within blocks: bit 0 phase A, bit 1 final update
   int block_a = 256;  // threads per block of the level-local kernels (64, 128 or 256)
   int block_c = 256;
 };

@@ -16,7 +16,7 @@ hipError_t launch_step(const StepArgs<float>& a, const LaunchCfg& lc, double* vt
   if (ev) (void)hipEventRecord(ev[2], s);
   if (lc.sed_wave && a.nz <= kSedLevels)
     hipLaunchKernelGGL(k_phase_b_wave<float>, dim3(unsigned((a.ncol + kSedCols - 1) / kSedCols)), dim3(64 * kSedCols),
-                       0, s, a);
+                       sed_lds_bytes(a.nz), s, a);
   else
     hipLaunchKernelGGL(k_phase_b<float>, dim3(unsigned((a.ncol + 127) / 128)), dim3(128), 0, s, a, vtbuf);
   if (ev) (void)hipEventRecord(ev[3], s);

@@ -16,7 +16,7 @@ hipError_t launch_step(const StepArgs<double>& a, const LaunchCfg& lc, double* v
   if (ev) (void)hipEventRecord(ev[2], s);
   if (lc.sed_wave && a.nz <= kSedLevels)
     hipLaunchKernelGGL(k_phase_b_wave<double>, dim3(unsigned((a.ncol + kSedCols - 1) / kSedCols)), dim3(64 * kSedCols),
-                       0, s, a);
+                       sed_lds_bytes(a.nz), s, a);
   else
     hipLaunchKernelGGL(k_phase_b<double>, dim3(unsigned((a.ncol + 127) / 128)), dim3(128), 0, s, a, vtbuf);
   if (ev) (void)hipEventRecord(ev[3], s);
