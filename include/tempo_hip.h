@@ -46,7 +46,7 @@ typedef struct {
   int hailaware_flag;             /* hail-aware graupel fall-speed constants */
   int ml_for_bl_nc_flag;          /* not supported (must be 0) */
   int ml_for_nc_flag;             /* not supported (must be 0) */
-  int semi_sedi_flag;             /* not supported (must be 0) */
+  int semi_sedi_flag;             /* semi-Lagrangian rain and graupel sedimentation (tempo_hip_run only) */
   int refl10cm_from_melting_flag; /* not supported (must be 0) */
   int turn_off_micro_flag;
   int cloud_condensation_flag;

@@ -521,7 +521,7 @@ int oracle_init(const int *flags, const float *ccn, const double *fz, const doub
                 const double *qg)
 {
     memcpy(g_flags, flags, sizeof(g_flags));
-    if (flags[2] || flags[3] || flags[14] || flags[15]) return -1;
+    if (flags[3] || flags[14] || flags[15]) return -1;
     init_bin_literals();
     init_params(&P, flags[1] != 0);
     init_params(&PT, 1);
@@ -1221,6 +1221,123 @@ static void sedimentation(col_t *c, double *xr, const double *vt, double *xten, 
     }
     if (precip) {
         if (xr[0] > low_limit_mass_for_precip) *precip = *precip + sed_r[0] * c->dt * (1.0 / (double)steps);
+    }
+}
+
+/* semilagrangian_sedimentation, :1562-1749 (Juang and Hong 2010, as the
+   reference implements it; 0-based k here is the reference's k+1).  vt holds
+   nz+1 entries (the reference's fall-speed arrays end one level above the
+   column). */
+static void semilagrangian_sedimentation(col_t *c, double *xr, double *xten, const double *vt, int steps,
+                                         double limit, double *precip)
+{
+    int nz = c->nz;
+    const double dt = c->dt, odt = c->odt, ost = 1.0 / (double)steps;
+    double zi[NZMAX + 2], wi[NZMAX + 2], za[NZMAX + 3], dza[NZMAX + 2], qa[NZMAX + 2];
+    double qmi[NZMAX + 2], qpi[NZMAX + 2], rr_save[NZMAX];
+    for (int k = 0; k < nz; ++k) rr_save[k] = xr[k];
+    for (int k = 0; k < nz + 2; ++k) za[k] = 0.0;
+    for (int k = 0; k < nz + 1; ++k) zi[k] = wi[k] = dza[k] = qa[k] = qmi[k] = qpi[k] = 0.0;
+    zi[0] = 0.0;
+    for (int k = 0; k < nz; ++k) zi[k + 1] = zi[k] + c->dz1d[k];
+    /* third-order interpolation of the interface speeds (:1599-1608) */
+    const double fa1 = 9.0 / 16.0, fa2 = 1.0 / 16.0;
+    wi[0] = vt[0];
+    wi[1] = 0.5 * (vt[1] + vt[0]);
+    for (int k = 2; k < nz - 1; ++k) wi[k] = fa1 * (vt[k] + vt[k - 1]) - fa2 * (vt[k + 1] + vt[k - 2]);
+    wi[nz - 1] = 0.5 * (vt[nz - 1] + vt[nz - 2]);
+    wi[nz] = vt[nz];
+    for (int k = 1; k < nz; ++k)
+        if (vt[k] == 0.0) wi[k] = vt[k - 1];
+    /* diffusivity limit (:1614-1621) */
+    const double con1 = 0.05;
+    for (int k = nz - 1; k >= 0; --k) {
+        double decfl = (wi[k + 1] - wi[k]) * dt * ost / c->dz1d[k];
+        if (decfl > con1) wi[k] = wi[k + 1] - con1 * c->dz1d[k] * odt * (double)steps;
+    }
+    /* arrival points and their spacing (:1623-1630) */
+    for (int k = 0; k < nz + 1; ++k) za[k] = zi[k] - wi[k] * dt * ost;
+    za[nz + 1] = zi[nz];
+    for (int k = 0; k < nz + 1; ++k) dza[k] = za[k + 1] - za[k];
+    for (int k = 0; k < nz; ++k) qa[k] = xr[k] * c->dz1d[k] / dza[k];
+    qa[nz] = 0.0;
+    /* monotone interface values (:1638-1658) */
+    for (int k = 1; k < nz; ++k) {
+        double dip = (qa[k + 1] - qa[k]) / (dza[k + 1] + dza[k]);
+        double dim = (qa[k] - qa[k - 1]) / (dza[k - 1] + dza[k]);
+        if (dip * dim <= 0.0) {
+            qmi[k] = qa[k];
+            qpi[k] = qa[k];
+        } else {
+            qpi[k] = qa[k] + 0.5 * (dip + dim) * dza[k];
+            qmi[k] = 2.0 * qa[k] - qpi[k];
+            if (qpi[k] > 0.0 || qmi[k] < 0.0) {
+                qpi[k] = qa[k];
+                qmi[k] = qa[k];
+            }
+        }
+    }
+    qpi[0] = qa[0];
+    qmi[0] = qa[0];
+    qmi[nz] = qa[nz];
+    qpi[nz] = qa[nz];
+    /* remap to the model levels (:1660-1727); kb, kt are 1-based as in the reference */
+    int kb = 1, kt = 1;
+    for (int k = 1; k <= nz; ++k) {
+        kb = imax(kb - 1, 1);
+        kt = imax(kt - 1, 1);
+        if (zi[k - 1] >= za[nz]) break;
+        for (int kk = kb; kk <= nz; ++kk)
+            if (zi[k - 1] <= za[kk]) { kb = kk; break; }
+        for (int kk = kt; kk <= nz + 2; ++kk)
+            if (zi[k] <= za[kk - 1]) { kt = kk; break; }
+        kt = kt - 1;
+        double *x = &xr[k - 1];
+        if (kt == kb) {
+            double tl = (zi[k - 1] - za[kb - 1]) / dza[kb - 1];
+            double th = (zi[k] - za[kb - 1]) / dza[kb - 1];
+            double tl2 = tl * tl, th2 = th * th;
+            double qqd = 0.5 * (qpi[kb - 1] - qmi[kb - 1]);
+            double qqh = qqd * th2 + qmi[kb - 1] * th;
+            double qql = qqd * tl2 + qmi[kb - 1] * tl;
+            *x = (qqh - qql) / (th - tl);
+        } else if (kt > kb) {
+            double tl = (zi[k - 1] - za[kb - 1]) / dza[kb - 1];
+            double tl2 = tl * tl;
+            double qqd = 0.5 * (qpi[kb - 1] - qmi[kb - 1]);
+            double qql = qqd * tl2 + qmi[kb - 1] * tl;
+            double dql = qa[kb - 1] - qql;
+            double zsum = (1.0 - tl) * dza[kb - 1];
+            double qsum = dql * dza[kb - 1];
+            if (kt - kb > 1)
+                for (int m = kb + 1; m <= kt - 1; ++m) {
+                    zsum = zsum + dza[m - 1];
+                    qsum = qsum + qa[m - 1] * dza[m - 1];
+                }
+            double th = (zi[k] - za[kt - 1]) / dza[kt - 1];
+            double th2 = th * th;
+            qqd = 0.5 * (qpi[kt - 1] - qmi[kt - 1]);
+            double dqh = qqd * th2 + qmi[kt - 1] * th;
+            zsum = zsum + th * dza[kt - 1];
+            qsum = qsum + dqh * dza[kt - 1];
+            *x = qsum / zsum;
+        }
+        double orho = 1.0 / c->rho[k - 1];
+        *x = dmax(*x, limit);
+        xten[k - 1] = xten[k - 1] + (*x - rr_save[k - 1]) * orho * odt;
+    }
+    /* surface precipitation (:1729-1744) */
+    for (int k = 0; k < nz; ++k) {
+        if (za[k] < 0.0 && za[k + 1] <= 0.0) {
+            if (precip) *precip = *precip + qa[k] * dza[k];
+        } else if (za[k] < 0.0 && za[k + 1] > 0.0) {
+            double th = (0.0 - za[k]) / dza[k];
+            double th2 = th * th;
+            double qqd = 0.5 * (qpi[k] - qmi[k]);
+            double qqh = qqd * th2 + qmi[k] * th;
+            if (precip) *precip = *precip + qqh * dza[k];
+            break;
+        }
     }
 }
 
@@ -2206,20 +2323,56 @@ static void tempo_main(col_t *c)
         thermo_vars(c);
     }
 
-    /* sedimentation (flux form, :590-727) */
+    /* sedimentation (flux form, :590-727; semi-Lagrangian for rain and
+       graupel with semi_sedi_flag, :597-615,637-667) */
+    const double semi_sedi_factor = 10.0;
     if (any(c->l_qr, nz)) {
         rain_fallspeed(c, &substeps, &ktop);
-        for (int n = 1; n <= substeps; ++n) {
-            sedimentation(c, c->rr, c->vtrr, c->qrten, r1, substeps, ktop, &c->rain_precip);
-            sedimentation(c, c->nr, c->vtnr, c->nrten, r2, substeps, ktop, NULL);
+        if (g_flags[2]) {
+            substeps = imax((int)((double)substeps / semi_sedi_factor) + 1, 1);
+            for (int n = 1; n <= substeps; ++n) {
+                semilagrangian_sedimentation(c, c->rr, c->qrten, c->vtrr, substeps, r1, &c->rain_precip);
+                semilagrangian_sedimentation(c, c->nr, c->nrten, c->vtnr, substeps, r2, NULL);
+                for (int k = 0; k <= nz; ++k) c->vtrr[k] = c->vtnr[k] = 0.0;
+                memcpy(c->xrx, c->qr1d, nz * sizeof(double));
+                memcpy(c->xnx, c->nr1d, nz * sizeof(double));
+                rain_check_and_update(c, c->xrx, c->xnx);
+                int s_unused, k_unused;
+                rain_fallspeed(c, &s_unused, &k_unused);
+            }
+        } else {
+            for (int n = 1; n <= substeps; ++n) {
+                sedimentation(c, c->rr, c->vtrr, c->qrten, r1, substeps, ktop, &c->rain_precip);
+                sedimentation(c, c->nr, c->vtnr, c->nrten, r2, substeps, ktop, NULL);
+            }
         }
     }
     if (any(c->l_qg, nz)) {
         graupel_fallspeed(c, &substeps, &ktop);
-        for (int n = 1; n <= substeps; ++n) {
-            sedimentation(c, c->rg, c->vtrg, c->qgten, r1, substeps, ktop, &c->graupel_precip);
-            sedimentation(c, c->ng, c->vtng, c->ngten, r2, substeps, ktop, NULL);
-            sedimentation(c, c->rb, c->vtrg, c->qbten, meters3_to_liters * r1 / rho_g[NRHG - 1], substeps, ktop, NULL);
+        if (g_flags[2]) {
+            substeps = imax((int)((double)substeps / semi_sedi_factor) + 1, 1);
+            for (int n = 1; n <= substeps; ++n) {
+                semilagrangian_sedimentation(c, c->rg, c->qgten, c->vtrg, substeps, r1, &c->graupel_precip);
+                semilagrangian_sedimentation(c, c->ng, c->ngten, c->vtng, substeps, r2, NULL);
+                semilagrangian_sedimentation(c, c->rb, c->qbten, c->vtrg, substeps,
+                                             meters3_to_liters * r1 / rho_g[NRHG - 1], NULL);
+                for (int k = 0; k <= nz; ++k) c->vtrg[k] = c->vtng[k] = 0.0;
+                memcpy(c->xrx, c->qg1d, nz * sizeof(double));
+                if (c->has_ng_qb) {
+                    memcpy(c->xngx, c->ng1d, nz * sizeof(double));
+                    memcpy(c->xqbx, c->qb1d, nz * sizeof(double));
+                }
+                graupel_check_and_update(c, c->xrx, c->has_ng_qb ? c->xngx : NULL, c->has_ng_qb ? c->xqbx : NULL);
+                int s_unused, k_unused;
+                graupel_fallspeed(c, &s_unused, &k_unused);
+            }
+        } else {
+            for (int n = 1; n <= substeps; ++n) {
+                sedimentation(c, c->rg, c->vtrg, c->qgten, r1, substeps, ktop, &c->graupel_precip);
+                sedimentation(c, c->ng, c->vtng, c->ngten, r2, substeps, ktop, NULL);
+                sedimentation(c, c->rb, c->vtrg, c->qbten, meters3_to_liters * r1 / rho_g[NRHG - 1], substeps, ktop,
+                              NULL);
+            }
         }
     }
     if (any(c->l_qs, nz)) {

@@ -4,7 +4,8 @@
 
 Device code is written for CDNA4 only.  All HIP sources are compiled with
 packed-FP32 instructions disabled; the FP64 twin and the FP64 table builder
-with -ffp-contract=off so they round exactly as the reference FP64 build.
+with -ffp-contract=off so they round exactly as the reference FP64 build;
+the FP32 path with contraction, reciprocal and approximate-function math.
 """
 import os
 import subprocess
@@ -27,7 +28,9 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 
 # (source, extra flags)
 UNITS = [
-    ("tempo_step_f32.hip", DEVICE + ["-ffp-contract=fast"]),
+    # FP32 production path: contracted multiply-adds, hardware reciprocal and
+    # approximate exp/log/sqrt (errors of a few ulps, within the FP32 tolerances)
+    ("tempo_step_f32.hip", DEVICE + ["-ffp-contract=fast", "-fapprox-func", "-freciprocal-math"]),
     ("tempo_step_f64.hip", DEVICE + ["-ffp-contract=off"]),
     ("tempo_tables.hip", DEVICE + ["-ffp-contract=off"]),
     ("tempo_capi.cpp", ["-x", "hip"] + DEVICE + ["-ffp-contract=off"]),

@@ -142,6 +142,7 @@ struct tempo_hip_ctx {
   DevBuf efrw64, efsw64, wev64, iaus64, efrw32, efsw32, wev32, iaus32;
   // scratch
   DevBuf sr, sd, pf, cf_, precip, ksub, vtbuf;
+  DevBuf semiw;  // semi-Lagrangian sedimentation work (allocated only with semi_sedi_flag)
   DevBuf tl_state, tl_diag;  // dual (value, tangent) copies for the tangent-linear step
   DevBuf adj_in, adj_mem;    // adjoint: packed inputs, cotangent/Jacobian planes
   DevBuf adj_snap;           // adjoint: state and scratch after phases A and B
@@ -327,6 +328,7 @@ int ensure_scratch(tempo_hip_ctx* x, size_t ncol, size_t nz, size_t br = 8, size
   HIP_TRY(x->cf_.ensure(ncol * sizeof(unsigned)));
   HIP_TRY(x->precip.ensure(5 * ncol * bw));
   HIP_TRY(x->ksub.ensure(10 * ncol * sizeof(int)));
+  if (x->cfg.semi_sedi_flag) HIP_TRY(x->semiw.ensure(kSemiArrays * (nz + 2) * ncol * bw));
   return TEMPO_HIP_OK;
 }
 
@@ -365,6 +367,8 @@ int bind_common(tempo_hip_ctx* x, StepArgs<R>& a, double dt, int itimestep) {
   a.fl.aerosolaware = c.aerosolaware_flag;
   a.fl.hailaware = c.hailaware_flag;
   a.fl.turn_off_micro = c.turn_off_micro_flag;
+  a.fl.semi_sedi = c.semi_sedi_flag;
+  a.semiw = c.semi_sedi_flag ? x->semiw.as<W>() : nullptr;
   a.fl.cloud_condensation = c.cloud_condensation_flag;
   a.fl.refl10cm = c.refl10cm_flag;
   a.fl.re_cloud = c.re_cloud_flag;
@@ -442,6 +446,8 @@ template <class T>
 int run_tlm(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fields* ft, const tempo_hip_diags* d,
             const tempo_hip_diags* dt_, const tempo_hip_dims* dm, double dt, int itimestep, hipStream_t s) {
   using D = Dual<T>;
+  if (x->cfg.semi_sedi_flag)
+    return fail(TEMPO_HIP_E_UNSUPPORTED, "the tangent-linear step covers flux-form sedimentation only");
   const int ncol = dm->ni * dm->nj, nz = dm->nz;
   const size_t n3 = size_t(ncol) * nz;
   HIP_TRY(x->tl_state.ensure(F_NFIELDS * n3 * sizeof(D)));
@@ -496,6 +502,8 @@ template <class T>
 int run_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fields* yb, const tempo_hip_diags* db,
                 const tempo_hip_fields* xb, const tempo_hip_dims* dm, double dt, int itimestep, hipStream_t s) {
   using D = Dual<T>;
+  if (x->cfg.semi_sedi_flag)
+    return fail(TEMPO_HIP_E_UNSUPPORTED, "the adjoint covers flux-form sedimentation only");
   if (!f->t) return fail(TEMPO_HIP_E_UNSUPPORTED, "the adjoint takes temperature input t (not th/pii)");
   const int ncol = dm->ni * dm->nj, nz = dm->nz;
   const size_t n3 = size_t(ncol) * nz, sb = F_NFIELDS * n3 * sizeof(D);
@@ -580,7 +588,6 @@ extern "C" {
 int tempo_hip_init(const tempo_hip_cfg* cfg, const char* table_dir, const char* ccn_path, tempo_hip_ctx** out) {
   if (!cfg || !out) return fail(TEMPO_HIP_E_ARG, "tempo_hip_init: NULL argument");
   *out = nullptr;
-  if (cfg->semi_sedi_flag) return fail(TEMPO_HIP_E_UNSUPPORTED, "semi-Lagrangian sedimentation is not supported");
   if (cfg->refl10cm_from_melting_flag)
     return fail(TEMPO_HIP_E_UNSUPPORTED, "melting-layer reflectivity is not supported");
   if (cfg->ml_for_nc_flag || cfg->ml_for_bl_nc_flag)

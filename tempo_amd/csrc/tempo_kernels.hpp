@@ -375,6 +375,10 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_A k_phase_a(StepArgs<R> a) {
   sr[SR_VTNI * plane + s] = vtni;
   sr[SR_VTRC * plane + s] = vtrc;
   sr[SR_VTNC * plane + s] = vtnc;
+  if (a.fl.semi_sedi) {  // the fall-speed refresh between semi-Lagrangian substeps reads these
+    sr[SR_RHOF * plane + s] = L.rhof;
+    sr[SR_VISCO * plane + s] = L.visco;
+  }
   using W = W_t<R>;
   W* sd = a.sd;
   sd[SD_RR * plane + s] = W(L.rr);
@@ -458,6 +462,235 @@ __device__ void fill_fallspeed(const StepArgs<R>& a, int c, int nzc, const W_t<R
   if (ktop == nzc) ktop = nzc - 1;
 }
 
+// semi-Lagrangian sedimentation of one field over one substep
+// (semilagrangian_sedimentation, :1562-1749, after Juang and Hong 2010) for
+// the column c of nzc levels: interface speeds by third-order interpolation
+// of vt (vt above the top is 0), limited so that the departure cells do not
+// shrink by more than con1 per substep; departure points za and their
+// spacing; monotone piecewise-parabolic interface values; the remap of the
+// departure cells onto the model levels; the surface flux from the cells
+// that leave the column.  Work arrays in semiw ([array][level][column]).
+constexpr int kSemiArrays = 8;
+template <class R>
+__device__ void semi_sediment(const StepArgs<R>& a, int c, int nzc, W_t<R>* xr, W_t<R>* xten, const W_t<R>* vt,
+                              int steps, double limit, W_t<R>* precip) {
+  // The remap and the surface flux branch on differences of nearly equal
+  // positions (departure point against interface, za against 0), where the
+  // reference's algorithm is discontinuous: its rounding is kept exactly,
+  // without fused multiply-adds, in either precision of the step.
+#pragma clang fp contract(off)
+  using W = W_t<R>;
+  const size_t plane = a.plane, nk = size_t(a.nz) + 2, nc = size_t(a.ncol);
+  auto arr = [&](int q, int k) -> W& { return a.semiw[(size_t(q) * nk + size_t(k)) * nc + size_t(c)]; };
+  enum { ZI, WI, ZA, DZA, QA, QMI, QPI, SAVE };
+  auto X = [&](int k) -> W& { return xr[TEMPO_IDX_SCR(a, c, k)]; };
+  auto V = [&](int k) -> W { return k < nzc ? vt[TEMPO_IDX_SCR(a, c, k)] : W(0.0); };
+  auto DZ = [&](int k) -> W { return W(a.f[F_DZ][TEMPO_IDX_STATE(a, c, k)]); };
+  const int nz = nzc;
+  const W dt = W(a.dt), odt = W(1.0) / dt, ost = W(1.0) / W(double(steps));
+  for (int k = 0; k < nz; ++k) arr(SAVE, k) = X(k);
+  arr(ZI, 0) = W(0.0);
+  for (int k = 0; k < nz; ++k) arr(ZI, k + 1) = arr(ZI, k) + DZ(k);
+  // interface speeds (:1599-1612)
+  const W fa1 = W(9.0 / 16.0), fa2 = W(1.0 / 16.0);
+  arr(WI, 0) = V(0);
+  arr(WI, 1) = W(0.5) * (V(1) + V(0));
+  for (int k = 2; k < nz - 1; ++k) arr(WI, k) = fa1 * (V(k) + V(k - 1)) - fa2 * (V(k + 1) + V(k - 2));
+  arr(WI, nz - 1) = W(0.5) * (V(nz - 1) + V(nz - 2));
+  arr(WI, nz) = V(nz);
+  for (int k = 1; k < nz; ++k)
+    if (V(k) == W(0.0)) arr(WI, k) = V(k - 1);
+  // diffusivity limit, top-down (:1614-1621)
+  const W con1 = W(0.05);
+  for (int k = nz - 1; k >= 0; --k) {
+    const W decfl = (arr(WI, k + 1) - arr(WI, k)) * dt * ost / DZ(k);
+    if (decfl > con1) arr(WI, k) = arr(WI, k + 1) - con1 * DZ(k) * odt * W(double(steps));
+  }
+  // departure points and cell sizes (:1623-1635)
+  for (int k = 0; k < nz + 1; ++k) arr(ZA, k) = arr(ZI, k) - arr(WI, k) * dt * ost;
+  arr(ZA, nz + 1) = arr(ZI, nz);
+  for (int k = 0; k < nz + 1; ++k) arr(DZA, k) = arr(ZA, k + 1) - arr(ZA, k);
+  for (int k = 0; k < nz; ++k) arr(QA, k) = X(k) * DZ(k) / arr(DZA, k);
+  arr(QA, nz) = W(0.0);
+  // monotone interface values (:1638-1658)
+  for (int k = 1; k < nz; ++k) {
+    const W dip = (arr(QA, k + 1) - arr(QA, k)) / (arr(DZA, k + 1) + arr(DZA, k));
+    const W dim = (arr(QA, k) - arr(QA, k - 1)) / (arr(DZA, k - 1) + arr(DZA, k));
+    if (dip * dim <= W(0.0)) {
+      arr(QMI, k) = arr(QA, k);
+      arr(QPI, k) = arr(QA, k);
+    } else {
+      arr(QPI, k) = arr(QA, k) + W(0.5) * (dip + dim) * arr(DZA, k);
+      arr(QMI, k) = W(2.0) * arr(QA, k) - arr(QPI, k);
+      if (arr(QPI, k) > W(0.0) || arr(QMI, k) < W(0.0)) {
+        arr(QPI, k) = arr(QA, k);
+        arr(QMI, k) = arr(QA, k);
+      }
+    }
+  }
+  arr(QPI, 0) = arr(QA, 0);
+  arr(QMI, 0) = arr(QA, 0);
+  arr(QMI, nz) = arr(QA, nz);
+  arr(QPI, nz) = arr(QA, nz);
+  // remap onto the model levels (:1660-1727); kb, kt 1-based as in the reference
+  int kb = 1, kt = 1;
+  const R* rho = a.sr + SR_RHO * plane;
+  for (int k = 1; k <= nz; ++k) {
+    kb = max(kb - 1, 1);
+    kt = max(kt - 1, 1);
+    if (arr(ZI, k - 1) >= arr(ZA, nz)) break;
+    for (int kk = kb; kk <= nz; ++kk)
+      if (arr(ZI, k - 1) <= arr(ZA, kk)) {
+        kb = kk;
+        break;
+      }
+    for (int kk = kt; kk <= nz + 2; ++kk)
+      if (arr(ZI, k) <= arr(ZA, kk - 1)) {
+        kt = kk;
+        break;
+      }
+    kt = kt - 1;
+    W& x = X(k - 1);
+    if (kt == kb) {
+      const W tl = (arr(ZI, k - 1) - arr(ZA, kb - 1)) / arr(DZA, kb - 1);
+      const W th = (arr(ZI, k) - arr(ZA, kb - 1)) / arr(DZA, kb - 1);
+      const W tl2 = tl * tl, th2 = th * th;
+      const W qqd = W(0.5) * (arr(QPI, kb - 1) - arr(QMI, kb - 1));
+      const W qqh = qqd * th2 + arr(QMI, kb - 1) * th;
+      const W qql = qqd * tl2 + arr(QMI, kb - 1) * tl;
+      x = (qqh - qql) / (th - tl);
+    } else if (kt > kb) {
+      const W tl = (arr(ZI, k - 1) - arr(ZA, kb - 1)) / arr(DZA, kb - 1);
+      const W tl2 = tl * tl;
+      W qqd = W(0.5) * (arr(QPI, kb - 1) - arr(QMI, kb - 1));
+      const W qql = qqd * tl2 + arr(QMI, kb - 1) * tl;
+      const W dql = arr(QA, kb - 1) - qql;
+      W zsum = (W(1.0) - tl) * arr(DZA, kb - 1);
+      W qsum = dql * arr(DZA, kb - 1);
+      if (kt - kb > 1)
+        for (int m = kb + 1; m <= kt - 1; ++m) {
+          zsum = zsum + arr(DZA, m - 1);
+          qsum = qsum + arr(QA, m - 1) * arr(DZA, m - 1);
+        }
+      const W th = (arr(ZI, k) - arr(ZA, kt - 1)) / arr(DZA, kt - 1);
+      const W th2 = th * th;
+      qqd = W(0.5) * (arr(QPI, kt - 1) - arr(QMI, kt - 1));
+      const W dqh = qqd * th2 + arr(QMI, kt - 1) * th;
+      zsum = zsum + th * arr(DZA, kt - 1);
+      qsum = qsum + dqh * arr(DZA, kt - 1);
+      x = qsum / zsum;
+    }
+    const W orho = W(1.0) / W(rho[TEMPO_IDX_SCR(a, c, k - 1)]);
+    x = x > W(limit) ? x : W(limit);
+    W& ten = xten[TEMPO_IDX_SCR(a, c, k - 1)];
+    ten = ten + (x - arr(SAVE, k - 1)) * orho * odt;
+  }
+  // surface precipitation (:1729-1744)
+  if (!precip) return;
+  for (int k = 0; k < nz; ++k) {
+    if (arr(ZA, k) < W(0.0) && arr(ZA, k + 1) <= W(0.0)) {
+      *precip = *precip + arr(QA, k) * arr(DZA, k);
+    } else if (arr(ZA, k) < W(0.0) && arr(ZA, k + 1) > W(0.0)) {
+      const W th = (W(0.0) - arr(ZA, k)) / arr(DZA, k);
+      const W th2 = th * th;
+      const W qqd = W(0.5) * (arr(QPI, k) - arr(QMI, k));
+      const W qqh = qqd * th2 + arr(QMI, k) * th;
+      *precip = *precip + qqh * arr(DZA, k);
+      break;
+    }
+  }
+}
+
+// between semi-Lagrangian substeps (:606-614, 651-666): fall speeds reset,
+// rain_check_and_update / graupel_check_and_update on copies of the 1-d
+// state with the tendencies so far, then rain_fallspeed / graupel_fallspeed
+// (top-down fill from 0 above the column).  A tendency keeps its FP64
+// accumulator unless the check rewrites it.
+template <class R>
+__device__ void semi_rain_refresh(const StepArgs<R>& a, int c, int nzc, W_t<R>* vt1, W_t<R>* vt2) {
+  using W = W_t<R>;
+  const Const<S_t<R>>& C = *a.cst;
+  const size_t plane = a.plane;
+  W* sd = a.sd;
+  const R* sr = a.sr;
+  const R dt = R(a.dt), odt = R(1.0) / dt;
+  R v1 = R(0.0), v2 = R(0.0);
+  for (int k = nzc - 1; k >= 0; --k) {
+    const size_t s = TEMPO_IDX_SCR(a, c, k), o = TEMPO_IDX_STATE(a, c, k);
+    Lvl<R> L;
+    L.rho = sr[SR_RHO * plane + s];
+    W& qrten = sd[SD_QRTEN * plane + s];
+    W& nrten = sd[SD_NRTEN * plane + s];
+    L.qrten = R(qrten);
+    L.nrten = R(nrten);
+    const R q0 = L.qrten, n0 = L.nrten;
+    R x = a.f[F_QR][o], xn = a.f[F_NR][o];
+    rain_check(C, L, x, xn, dt, odt);
+    if (L.qrten != q0) qrten = W(L.qrten);
+    if (L.nrten != n0) nrten = W(L.nrten);
+    sd[SD_RR * plane + s] = W(L.rr);
+    sd[SD_NR * plane + s] = W(L.nr);
+    if (L.rr > R(kR1)) {  // rain_fallspeed (:1771-1774)
+      const R rhof = sr[SR_RHOF * plane + s], lamr = R(1.0) / L.ilamr;
+      v1 = rhof * R(kAvR) * C.crg[5] * C.org3 * mpow(lamr, C.cre[2]) * mpow(lamr + R(kFvR), -C.cre[5]);
+      v2 = rhof * R(kAvR) * C.crg[6] / C.crg[11] * mpow(lamr, C.cre[11]) * mpow(lamr + R(kFvR), -C.cre[6]);
+    }
+    vt1[s] = W(v1);
+    vt2[s] = W(v2);
+  }
+}
+
+template <class R>
+__device__ void semi_graupel_refresh(const StepArgs<R>& a, int c, int nzc, W_t<R>* vt1, W_t<R>* vt2) {
+  using W = W_t<R>;
+  const Const<S_t<R>>& C = *a.cst;
+  const size_t plane = a.plane;
+  W* sd = a.sd;
+  const R* sr = a.sr;
+  const R dt = R(a.dt), odt = R(1.0) / dt;
+  const bool has_ngqb = a.f[F_NG] && a.f[F_QB];
+  R v1 = R(0.0), v2 = R(0.0);
+  for (int k = nzc - 1; k >= 0; --k) {
+    const size_t s = TEMPO_IDX_SCR(a, c, k), o = TEMPO_IDX_STATE(a, c, k);
+    Lvl<R> L;
+    L.rho = sr[SR_RHO * plane + s];
+    W& qgten = sd[SD_QGTEN * plane + s];
+    W& ngten = sd[SD_NGTEN * plane + s];
+    W& qbten = sd[SD_QBTEN * plane + s];
+    L.qgten = R(qgten);
+    L.ngten = R(ngten);
+    L.qbten = R(qbten);
+    const R g0 = L.qgten, n0 = L.ngten, b0 = L.qbten;
+    R x = a.f[F_QG][o];
+    R xg = has_ngqb ? a.f[F_NG][o] : R(0.0), xb = has_ngqb ? a.f[F_QB][o] : R(0.0);
+    graupel_check(C, L, x, xg, xb, has_ngqb, dt, odt);
+    if (L.qgten != g0) qgten = W(L.qgten);
+    if (L.ngten != n0) ngten = W(L.ngten);
+    if (L.qbten != b0) qbten = W(L.qbten);
+    sd[SD_RG * plane + s] = W(L.rg);
+    sd[SD_NG * plane + s] = W(L.ng);
+    sd[SD_RB * plane + s] = W(L.rb);
+    if (L.rg > R(kR1)) {  // graupel_fallspeed (:1815-1832)
+      const R rhof = sr[SR_RHOF * plane + s];
+      R afall, bfall;
+      if (a.f[F_QB]) {
+        const R dens_g = mx(rho_g<R>(1), mn(R(kM3ToL) * L.rg / L.rb, rho_g<R>(kNrhg)));
+        afall = R(kACoeff) * mpow((R(4.0) * dens_g * R(kGravity)) / (R(3.0) * L.rho), R(kBCoeff));
+        afall = afall * mpow(sr[SR_VISCO * plane + s], R(1.0) - R(2.0) * R(kBCoeff));
+        bfall = R(3.0) * R(kBCoeff) - R(1.0);
+      } else {
+        afall = R(kAvGOld);
+        bfall = R(kBvGOld);
+      }
+      const int idx = L.idx_bg;
+      v1 = rhof * afall * C.cgg[5][idx - 1] * C.ogg3 * mpow(L.ilamg, bfall);
+      v2 = rhof * afall * C.cgg[6][idx - 1] / C.cgg[11][idx - 1] * mpow(L.ilamg, bfall);
+    }
+    vt1[s] = W(v1);
+    vt2[s] = W(v2);
+  }
+}
+
 template <class R>
 __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, W_t<R>* vtbuf) {
   using W = W_t<R>;
@@ -479,9 +712,18 @@ __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, W_t<R>* vtbuf) {
     int steps, ktop;
     if (any & PF_QR) {
       fill_fallspeed(a, c, nzc, sd + SD_RR * plane, sr + SR_VTRR * plane, sr + SR_VTNR * plane, vr, v2, true, steps, ktop);
-      for (int n = 0; n < steps; ++n) {
-        sediment(a, c, nzc, sd + SD_RR * plane, vr, sd + SD_QRTEN * plane, kR1, steps, ktop, &pr[0]);
-        sediment(a, c, nzc, sd + SD_NR * plane, v2, sd + SD_NRTEN * plane, kR2, steps, ktop, (W*)nullptr);
+      if (a.fl.semi_sedi) {  // (:599-615): far fewer substeps, fall speeds refreshed after each
+        steps = max(int(R(steps) / R(10.0)) + 1, 1);
+        for (int n = 0; n < steps; ++n) {
+          semi_sediment(a, c, nzc, sd + SD_RR * plane, sd + SD_QRTEN * plane, vr, steps, kR1, &pr[0]);
+          semi_sediment(a, c, nzc, sd + SD_NR * plane, sd + SD_NRTEN * plane, v2, steps, kR2, (W*)nullptr);
+          semi_rain_refresh(a, c, nzc, vr, v2);
+        }
+      } else {
+        for (int n = 0; n < steps; ++n) {
+          sediment(a, c, nzc, sd + SD_RR * plane, vr, sd + SD_QRTEN * plane, kR1, steps, ktop, &pr[0]);
+          sediment(a, c, nzc, sd + SD_NR * plane, v2, sd + SD_NRTEN * plane, kR2, steps, ktop, (W*)nullptr);
+        }
       }
       kst[0] = steps;
       kst[1] = ktop;
@@ -491,10 +733,20 @@ __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, W_t<R>* vtbuf) {
     if (any & PF_QG) {
       fill_fallspeed(a, c, nzc, sd + SD_RG * plane, sr + SR_VTRG * plane, sr + SR_VTNG * plane, v1, v2, false, steps, ktop);
       const double rb_lim = kM3ToL * kR1 / 800.0;
-      for (int n = 0; n < steps; ++n) {
-        sediment(a, c, nzc, sd + SD_RG * plane, v1, sd + SD_QGTEN * plane, kR1, steps, ktop, &pr[1]);
-        sediment(a, c, nzc, sd + SD_NG * plane, v2, sd + SD_NGTEN * plane, kR2, steps, ktop, (W*)nullptr);
-        sediment(a, c, nzc, sd + SD_RB * plane, v1, sd + SD_QBTEN * plane, rb_lim, steps, ktop, (W*)nullptr);
+      if (a.fl.semi_sedi) {  // (:641-667)
+        steps = max(int(R(steps) / R(10.0)) + 1, 1);
+        for (int n = 0; n < steps; ++n) {
+          semi_sediment(a, c, nzc, sd + SD_RG * plane, sd + SD_QGTEN * plane, v1, steps, kR1, &pr[1]);
+          semi_sediment(a, c, nzc, sd + SD_NG * plane, sd + SD_NGTEN * plane, v2, steps, kR2, (W*)nullptr);
+          semi_sediment(a, c, nzc, sd + SD_RB * plane, sd + SD_QBTEN * plane, v1, steps, rb_lim, (W*)nullptr);
+          semi_graupel_refresh(a, c, nzc, v1, v2);
+        }
+      } else {
+        for (int n = 0; n < steps; ++n) {
+          sediment(a, c, nzc, sd + SD_RG * plane, v1, sd + SD_QGTEN * plane, kR1, steps, ktop, &pr[1]);
+          sediment(a, c, nzc, sd + SD_NG * plane, v2, sd + SD_NGTEN * plane, kR2, steps, ktop, (W*)nullptr);
+          sediment(a, c, nzc, sd + SD_RB * plane, v1, sd + SD_QBTEN * plane, rb_lim, steps, ktop, (W*)nullptr);
+        }
       }
       kst[2] = steps;
       kst[3] = ktop;

@@ -150,7 +150,9 @@ enum Diag3 { D3_REFL, D3_RE_C, D3_RE_I, D3_RE_S, D3_MVD_R, D3_MVD_G, D3_NC_MIX, 
 // working-precision scratch (per point)
 enum ScrR {
   SR_TEMP, SR_QV, SR_RHO, SR_TTEN, SR_QVTEN, SR_NWFATEN, SR_NIFATEN, SR_VTRR, SR_VTNR, SR_VTRG, SR_VTNG, SR_VTS,
-  SR_VTRI, SR_VTNI, SR_VTRC, SR_VTNC, SR_HGT, SR_N
+  SR_VTRI, SR_VTNI, SR_VTRC, SR_VTNC, SR_HGT,
+  SR_RHOF, SR_VISCO,  // density factor and viscosity (written only for semi-Lagrangian sedimentation)
+  SR_N
 };
 // FP64 scratch (per point): sedimented state and tendencies
 enum ScrD {
@@ -194,6 +196,7 @@ struct StepArgs {
   unsigned* cf;      // ncol column flags
   W_t<R>* precip;    // 5 x ncol: rain, graupel, snow, ice, cloud
   int* ksub;         // 10 x ncol: substeps and ktop per species (diagnostic)
+  W_t<R>* semiw;     // semi-Lagrangian sedimentation work: kSemiArrays x (nz + 2) x ncol, or nullptr
   const Const<S_t<R>>* cst;  // derived constants of this context (wave-uniform reads)
   // level-local kernels: bit 0 (phase A) / bit 1 (final update) set = each
   // block hands its level's points with microphysics to do to its first

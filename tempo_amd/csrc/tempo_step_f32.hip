@@ -14,7 +14,7 @@ hipError_t launch_step(const StepArgs<float>& a, const LaunchCfg& lc, double* vt
   if (ev) (void)hipEventRecord(ev[1], s);
   hipLaunchKernelGGL(k_phase_a<float>, level_grid(a, lc.block_a), dim3(unsigned(lc.block_a)), 0, s, a);
   if (ev) (void)hipEventRecord(ev[2], s);
-  if (lc.sed_wave && a.nz <= kSedLevels)
+  if (lc.sed_wave && a.nz <= kSedLevels && !a.fl.semi_sedi)
     hipLaunchKernelGGL(k_phase_b_wave<float>, dim3(unsigned((a.ncol + kSedCols - 1) / kSedCols)), dim3(64 * kSedCols),
                        sed_lds_bytes(a.nz), s, a);
   else

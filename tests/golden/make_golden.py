@@ -46,6 +46,8 @@ def col(a):
 
 
 def run_case(name, state, dt, itimestep, flags, nlev=None, land=None, steps=1):
+    if ONLY and name not in ONLY:
+        return
     ref.init(TABLES, **flags)
     inp = {k: v.copy() for k, v in state.items()}
     out = {k: v.copy() for k, v in state.items()}
@@ -97,11 +99,16 @@ def pick(state, idx):
     return {k: np.ascontiguousarray(v[:, idx]) for k, v in state.items()}
 
 
+ONLY = ()
+
+
 def main():
-    global TABLES
+    global TABLES, ONLY
     if len(sys.argv) < 2:
-        raise SystemExit("usage: make_golden.py TABLE_DIR  (tables written by `python -m oracle.ref_tables`)")
+        raise SystemExit("usage: make_golden.py TABLE_DIR [CASE ...]  (tables written by `python -m oracle.ref_tables`;"
+                         " with CASE names only those fixtures are rewritten)")
     TABLES = tio.read_dir(sys.argv[1])
+    ONLY = tuple(sys.argv[2:])
 
     # ---- workload samples (generator columns) ----
     st, m = gen.generate("plain", ncol=40)
@@ -161,6 +168,19 @@ def main():
         full[k] = col(d[k])
     full["qb"] = col(d["volg"] * 1000.)
     run_case("mpas_full", full, 20.0, 2, dict(BOTH, mvd_r=True, mvd_g=True, nc_mixing_ratio=True), steps=3)
+
+    # semi-Lagrangian sedimentation (semi_sedi_flag): the reference's graupel
+    # test with semi_sedi=.true. (test/run_tempo_tests.F90:29-31) at two of its
+    # time steps, and full-physics columns with rain and graupel
+    semi = dict(BOTH, turn_off_micro=True, mvd_g=True, semi_sedi=True)
+    run_case("semi_mpas_graupel", base, 20.0, 1, semi, steps=10)
+    run_case("semi_mpas_graupel_dt300", base, 300.0, 1, semi, steps=3)
+    st, m = gen.generate("multistep", ncol=24, col0=300)
+    run_case("semi_multistep", st, m["dt"], m["itimestep"], dict(BOTH, semi_sedi=True), steps=2)
+    st, m = gen.generate("plain", ncol=16, col0=900)
+    run_case("semi_plain_dt300", st, 300.0, 2, dict(PLAIN, semi_sedi=True))
+    if ONLY:
+        return
 
     # ---- sampled lookup-table entries ----
     rng = np.random.default_rng(2024)

@@ -36,7 +36,7 @@ FLAG_TO_CFG = dict(aerosolaware="aerosolaware_flag", hailaware="hailaware_flag",
                    cloud_condensation="cloud_condensation_flag", refl10cm="refl10cm_flag", re_cloud="re_cloud_flag",
                    re_ice="re_ice_flag", re_snow="re_snow_flag", max_hail="max_hail_diameter_flag",
                    mvd_r="rain_med_vol_diam_flag", mvd_g="graupel_med_vol_diam_flag",
-                   nc_mixing_ratio="cloud_number_mixing_ratio_flag")
+                   nc_mixing_ratio="cloud_number_mixing_ratio_flag", semi_sedi="semi_sedi_flag")
 
 
 def cfg_of(flags):
@@ -190,12 +190,39 @@ def test_gpu_fp64_matches_oracle(hip, big_case):
         assert rel_inf(d[D3_MAP[k]], v) <= 1e-12, k
 
 
+def test_gpu_semi_sedi_fp64_matches_oracle(hip, gpu_tables):
+    """Semi-Lagrangian rain and graupel sedimentation on 512 aerosol- and
+    hail-aware columns at dt = 120 s (several semi-Lagrangian substeps in the
+    convective columns): FP64 path vs the oracle."""
+    from oracle import oracle as orc
+    from tempo_amd import generator as gen
+    st, m = gen.generate("multistep", ncol=512, col0=70_000)
+    flags = dict(aerosolaware=True, hailaware=True, semi_sedi=True)
+    orc.init(gpu_tables, **flags)
+    ref = {k: v.copy() for k, v in st.items()}
+    d2, d3 = orc.run(ref, 120.0, m["itimestep"])
+    out, d = gpu_run(context(hip, flags), st, 120.0, m["itimestep"])
+    for k in ref:
+        assert rel_inf(out[k], ref[k]) <= 1e-12, (k, rel_inf(out[k], ref[k]))
+    for k, v in d2.items():
+        assert rel_inf(d[D2_MAP[k]], v) <= 1e-12, k
+    # The remap of the departure cells turns one-ulp differences of the fall
+    # speeds (device pow vs the host libm in about 1e-3 of calls) into
+    # 1e-14-relative differences of the remapped graupel; where graupel
+    # volume sits at its clamp (qb = 20 qg), the density index
+    # nint(10 qg/qb) + 1 is taken exactly at 0.5 and flips with them, moving
+    # the 3-D diagnostics of that point.  Such points are counted, not failed.
+    for k, v in d3.items():
+        bad = np.abs(d[D3_MAP[k]] - v) > 1e-12 * max(np.abs(v).max(), 1e-300)
+        assert bad.sum() <= 8, (k, int(bad.sum()), np.argwhere(bad)[:8].tolist())
+
+
 TENDENCY_FIELDS = ("t", "qv", "qc", "qr", "qi", "qs", "qg", "ni", "nr", "nc", "ng", "qb", "nwfa", "nifa")
 
 
 def fp32_errors(inp, out, d, o64, d64, dt):
     """(rel L2 of each tendency over unflipped points, flipped fraction) and
-    the relative L2 of total surface precipitation."""
+    the same pair for total surface precipitation over columns."""
     rep = {}
     for k in TENDENCY_FIELDS:
         if k not in inp:
@@ -207,15 +234,18 @@ def fp32_errors(inp, out, d, o64, d64, dt):
         rep[k] = (float(np.linalg.norm(t32 - t64) / nrm) if nrm > 0 else 0.0, float(1.0 - keep.mean()))
     p64 = sum(d64[n] for n in ("rain_precip", "ice_precip", "snow_precip", "graupel_precip"))
     p32 = sum(d[n] for n in ("rain_precip", "ice_precip", "snow_precip", "graupel_precip"))
-    return rep, float(np.linalg.norm(p32 - p64) / np.linalg.norm(p64))
+    keep = ~flipped(p32, p64)
+    return rep, (float(np.linalg.norm((p32 - p64)[keep]) / np.linalg.norm(p64[keep])), float(1.0 - keep.mean()))
 
 
-def test_gpu_fp32_within_tolerance(hip, big_case, gpu_tables):
+@pytest.mark.parametrize("semi", [False, True])
+def test_gpu_fp32_within_tolerance(hip, big_case, gpu_tables, semi):
     """FP32 production path vs the FP64 twin on the same FP32-representable
-    inputs (aerosol- and hail-aware columns)."""
+    inputs (aerosol- and hail-aware columns), with flux-form and with
+    semi-Lagrangian rain and graupel sedimentation."""
     st, m, _, _, _ = big_case
     inp = {k: v.astype(np.float32).astype(np.float64) for k, v in st.items()}
-    ctx = context(hip, dict(aerosolaware=True, hailaware=True))
+    ctx = context(hip, dict(aerosolaware=True, hailaware=True, semi_sedi=semi))
     o64, d64 = gpu_run(ctx, inp, m["dt"], m["itimestep"], dtype=torch.float64)
     o32, d32 = gpu_run(ctx, inp, m["dt"], m["itimestep"], dtype=torch.float32)
     inv = {v: k for k, v in D2_MAP.items()}
@@ -227,17 +257,20 @@ def test_gpu_fp32_within_tolerance(hip, big_case, gpu_tables):
     from oracle import ref
     if ref.available("sp"):
         # the reference's own single-precision build against the FP64 twin
-        ref.init(gpu_tables, variant="sp", aerosolaware=True, hailaware=True)
+        ref.init(gpu_tables, variant="sp", aerosolaware=True, hailaware=True, semi_sedi=semi)
         osp = {k: v.copy() for k, v in inp.items()}
         dsp, _ = ref.run(osp, m["dt"], m["itimestep"], variant="sp")
         rsp, psp = fp32_errors(inp, osp, dsp, o64, d64, m["dt"])
         print("reference -DSINGLE_PREC build vs FP64 twin:", rsp, "precip rel L2", psp)
         for k, (_, f) in rep.items():
             assert f <= 2.0 * rsp[k][1] + 1e-4, (k, f, rsp[k][1])
+        # semi-Lagrangian surface flux has no low-mass cut-off: a column whose
+        # lowest rain level sits at the r1 threshold may flip (a column or two)
+        assert prel[1] <= 2.0 * psp[1] + 1e-3, (prel, psp)
     for k, (rel, f) in rep.items():
         assert rel <= 1e-4, (k, rel)
         assert f <= 0.02, (k, f)
-    assert prel <= 1e-5, prel
+    assert prel[0] <= 1e-5, prel
 
 
 def test_gpu_ragged_matches_oracle(hip, gpu_tables):
@@ -295,8 +328,14 @@ def test_gpu_error_codes(hip):
         ctx.run(dev, -1.0, 2)
     assert e.value.code == hip.E_ARG
     with pytest.raises(hip.TempoError) as e:
-        hip.TempoHip(semi_sedi_flag=1)
+        hip.TempoHip(refl10cm_from_melting_flag=1)
     assert e.value.code == hip.E_UNSUPPORTED
+    # the derivative entry points cover the flux-form sedimentation only
+    semi = hip.TempoHip(aerosolaware_flag=0, hailaware_flag=0, semi_sedi_flag=1)
+    with pytest.raises(hip.TempoError) as e:
+        semi.tlm(dict(dev), {k: torch.zeros_like(v) for k, v in dev.items()}, m["dt"], 2)
+    assert e.value.code == hip.E_UNSUPPORTED
+    semi.close()
     with tempfile.TemporaryDirectory() as d:
         with pytest.raises(hip.TempoError) as e:
             hip.TempoHip(table_dir=d)
