@@ -126,6 +126,19 @@ int tempo_hip_run(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip
 int tempo_hip_run_host(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_diags *d,
                        const tempo_hip_dims *dims, double dt, int itimestep);
 
+/* Replaces tempo_aerosol_surface_emissions
+ * (src/module_mp_tempo_driver.F90:516-529): nwfa(i, k, j) += nwfa2d(i, j) * dt
+ * at level k (0-based: kts - kms) of every column described by dims (ni, nj,
+ * ldk, ldj; nz bounds k), the reference looping over the whole memory extent
+ * ims:ime, jms:jme.  nwfa2d element (i, j) at i + j*d2ldj.  Device arrays in
+ * the given precision, stream-ordered. */
+int tempo_hip_aerosol_surface_emissions(tempo_hip_ctx *ctx, double dt, void *nwfa, const void *nwfa2d,
+                                        const tempo_hip_dims *dims, int k, int precision, void *stream);
+
+/* As tempo_hip_aerosol_surface_emissions with FP64 host arrays. */
+int tempo_hip_aerosol_surface_emissions_host(tempo_hip_ctx *ctx, double dt, double *nwfa, const double *nwfa2d,
+                                             const tempo_hip_dims *dims, int k);
+
 /* Tangent-linear step (no counterpart in the reference): the step of
  * tempo_hip_run on f (updated in place) together with its derivative in the
  * direction f_tl -- device arrays laid out like f, holding the tangents of

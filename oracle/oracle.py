@@ -48,6 +48,8 @@ def lib():
         _lib.oracle_fz_cloud_entry.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_double)]
         _lib.oracle_trace.restype = ctypes.c_longlong
         _lib.oracle_trace.argtypes = [vp, ctypes.c_longlong]
+        _lib.oracle_surface_emissions.restype = None
+        _lib.oracle_surface_emissions.argtypes = [ctypes.c_double, vp, vp] + [ctypes.c_int] * 4
         _lib.oracle_index.restype = ctypes.c_int
         _lib.oracle_index.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
@@ -194,3 +196,12 @@ def sparse_tables(fixture_arrays):
         t[fixture_arrays[f"tab_{kind}_idx"]] = fixture_arrays[f"tab_{kind}_val"]
         out[kind] = t
     return out
+
+
+def surface_emissions(dt, nwfa, nwfa2d, kts):
+    """tempo_aerosol_surface_emissions (src/module_mp_tempo_driver.F90:516-529)
+    on nwfa (nj, nk, ni) C-order == (ni, nk, nj) Fortran order, in place;
+    nwfa2d (nj, ni); kts 1-based."""
+    assert nwfa.dtype == np.float64 and nwfa.flags.c_contiguous and nwfa2d.flags.c_contiguous
+    nj, nk, ni = nwfa.shape
+    lib().oracle_surface_emissions(dt, nwfa.ctypes.data, nwfa2d.ctypes.data, ni, nk, nj, kts)

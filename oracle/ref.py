@@ -70,6 +70,8 @@ def lib(variant=None):
             getattr(L, fn).argtypes = [ctypes.c_double, ctypes.c_double]
         L.ref_snow_moments.restype = None
         L.ref_snow_moments.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+        L.ref_surface_emissions.restype = None
+        L.ref_surface_emissions.argtypes = [ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 4
         L.ref_get_array.restype = ctypes.c_int
         L.ref_get_array.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
         _libs[v] = L
@@ -160,3 +162,11 @@ def get_array(which):
     if rc != 0:
         raise RuntimeError("ref_get_array failed")
     return out
+
+
+def surface_emissions(dt, nwfa, nwfa2d, kts, variant=None):
+    """tempo_aerosol_surface_emissions of the reference build on nwfa
+    (nj, nk, ni) C-order (== Fortran (ni, nk, nj)), in place; kts 1-based."""
+    assert nwfa.dtype == np.float64 and nwfa.flags.c_contiguous and nwfa2d.flags.c_contiguous
+    nj, nk, ni = nwfa.shape
+    lib(variant).ref_surface_emissions(dt, _ptr(nwfa), _ptr(nwfa2d), ni, nk, nj, kts)

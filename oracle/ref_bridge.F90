@@ -23,7 +23,7 @@ module ref_bridge
   use module_mp_tempo_params
   use module_mp_tempo_utils, only : compute_efrw, compute_efsw, compute_drop_evap, qi_aut_qs, &
     calc_gamma_p, calc_rslf, calc_rsif, snow_moments
-  use module_mp_tempo_driver, only : tempo_run, ty_tempo_driver_diags
+  use module_mp_tempo_driver, only : tempo_run, ty_tempo_driver_diags, tempo_aerosol_surface_emissions
   implicit none
 
   type(ty_tempo_cfgs), save :: cfgs
@@ -265,6 +265,22 @@ contains
 
   ! Known-answer hooks into public reference routines
   ! (src/module_mp_tempo_utils.F90:27-51, 148-331).
+  ! tempo_aerosol_surface_emissions (src/module_mp_tempo_driver.F90:516-529)
+  ! on nwfa(ni, nk, nj) and nwfa2d(ni, nj), both column-major FP64, with
+  ! ims = jms = kms = 1; kts is 1-based
+  subroutine ref_surface_emissions(dt, nwfa, nwfa2d, ni, nk, nj, kts) bind(c, name='ref_surface_emissions')
+    real(c_double), value :: dt
+    integer(c_int), value :: ni, nk, nj, kts
+    real(c_double), intent(inout) :: nwfa(ni, nk, nj)
+    real(c_double), intent(in) :: nwfa2d(ni, nj)
+    real(wp), allocatable :: w3(:,:,:), w2(:,:)
+    allocate(w3(ni, nk, nj), w2(ni, nj))
+    w3 = real(nwfa, wp)
+    w2 = real(nwfa2d, wp)
+    call tempo_aerosol_surface_emissions(real(dt, wp), w3, w2, 1, ni, 1, nj, 1, nk, kts)
+    nwfa = real(w3, c_double)
+  end subroutine ref_surface_emissions
+
   real(c_double) function ref_gamma_p(a, x) bind(c, name='ref_gamma_p')
     real(c_double), value :: a, x
     ref_gamma_p = calc_gamma_p(real(a, wp), real(x, wp))

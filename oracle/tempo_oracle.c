@@ -2753,3 +2753,16 @@ int oracle_index(int kind, double a, double b, int *i1, int *i2)
     }
     return 0;
 }
+
+/* tempo_aerosol_surface_emissions, src/module_mp_tempo_driver.F90:516-529:
+   nwfa(i, kts, j) += nwfa2d(i, j) * dt over the whole memory extent
+   (ims:ime, jms:jme); nwfa is (ni, nk, nj) and nwfa2d (ni, nj), column-major,
+   kts 1-based. */
+void oracle_surface_emissions(double dt, double *nwfa, const double *nwfa2d, int ni, int nk, int nj, int kts)
+{
+    for (int j = 0; j < nj; ++j)
+        for (int i = 0; i < ni; ++i) {
+            size_t o = (size_t)i + (size_t)(kts - 1) * ni + (size_t)j * ni * nk;
+            nwfa[o] = nwfa[o] + nwfa2d[(size_t)i + (size_t)j * ni] * dt;
+        }
+}

@@ -44,6 +44,7 @@ double oracle_gamma_p(double a, double x);
 double oracle_rslf(double p, double t);
 double oracle_rsif(double p, double t);
 void oracle_snow_moments(double rs, double tc, double *out10);
+void oracle_surface_emissions(double dt, double *nwfa, const double *nwfa2d, int ni, int nk, int nj, int kts);
 int oracle_get_array(int which, double *out, int n);
 
 /* Single lookup-table entries, 1-based reference indices.  Built with the

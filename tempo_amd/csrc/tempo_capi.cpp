@@ -21,6 +21,9 @@ hipError_t launch_step_tlm(const StepArgs<Dual<float>>& a, Dual<double>* vtbuf, 
 hipError_t launch_step_tlm(const StepArgs<Dual<double>>& a, Dual<double>* vtbuf, hipStream_t s);
 template <class T>
 hipError_t launch_dual_pack(const T* v, const T* t, const DualGeom& g, Dual<T>* out, hipStream_t s);
+template <class R>
+hipError_t launch_surface_emissions(R* nwfa, const R* nwfa2d, int ni, int nj, long long ldk, long long ldj,
+                                    long long d2ldj, int k, double dt, hipStream_t s);
 template <class T>
 hipError_t launch_dual_unpack(const Dual<T>* in, const DualGeom& g, T* v, T* t, hipStream_t s);
 hipError_t launch_adjoint(const StepArgs<Dual<float>>& a, const AdjArgs& g, Dual<double>* vtbuf, const void* inputs,
@@ -649,6 +652,51 @@ int tempo_hip_run(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_d
   if (precision == TEMPO_HIP_FP32) return run_impl<float>(x, f, d, dm, dt, itimestep, s);
   if (precision == TEMPO_HIP_FP64) return run_impl<double>(x, f, d, dm, dt, itimestep, s);
   return fail(TEMPO_HIP_E_ARG, "precision must be TEMPO_HIP_FP32 or TEMPO_HIP_FP64");
+}
+
+static int check_emission_args(const void* nwfa, const void* nwfa2d, const tempo_hip_dims* dm, int k) {
+  if (!nwfa || !nwfa2d || !dm) return fail(TEMPO_HIP_E_ARG, "NULL nwfa, nwfa2d or dims");
+  if (dm->ni <= 0 || dm->nj <= 0 || dm->nz <= 0 || k < 0 || k >= dm->nz)
+    return fail(TEMPO_HIP_E_ARG, "bad dims or level (need 0 <= k < nz)");
+  if (dm->ldk < dm->ni || (dm->nj > 1 && (dm->ldj < dm->ldk * dm->nz || dm->d2ldj < dm->ni)))
+    return fail(TEMPO_HIP_E_ARG, "leading dimensions too small");
+  return TEMPO_HIP_OK;
+}
+
+int tempo_hip_aerosol_surface_emissions(tempo_hip_ctx* x, double dt, void* nwfa, const void* nwfa2d,
+                                        const tempo_hip_dims* dm, int k, int precision, void* stream) {
+  if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
+  int rc = check_emission_args(nwfa, nwfa2d, dm, k);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (precision == TEMPO_HIP_FP32)
+    HIP_TRY(launch_surface_emissions(static_cast<float*>(nwfa), static_cast<const float*>(nwfa2d), dm->ni, dm->nj,
+                                     dm->ldk, dm->ldj, dm->d2ldj, k, dt, s));
+  else if (precision == TEMPO_HIP_FP64)
+    HIP_TRY(launch_surface_emissions(static_cast<double*>(nwfa), static_cast<const double*>(nwfa2d), dm->ni, dm->nj,
+                                     dm->ldk, dm->ldj, dm->d2ldj, k, dt, s));
+  else
+    return fail(TEMPO_HIP_E_ARG, "bad precision");
+  return TEMPO_HIP_OK;
+}
+
+int tempo_hip_aerosol_surface_emissions_host(tempo_hip_ctx* x, double dt, double* nwfa, const double* nwfa2d,
+                                             const tempo_hip_dims* dm, int k) {
+  if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
+  int rc = check_emission_args(nwfa, nwfa2d, dm, k);
+  if (rc) return rc;
+  const size_t n3 = size_t(dm->ldj) * (dm->nj - 1) + size_t(dm->ldk) * (dm->nz - 1) + dm->ni;
+  const size_t n2 = size_t(dm->d2ldj) * (dm->nj - 1) + dm->ni;
+  if (x->stage.size() < 2) x->stage.resize(2);
+  HIP_TRY(x->stage[0].ensure(n3 * sizeof(double)));
+  HIP_TRY(x->stage[1].ensure(n2 * sizeof(double)));
+  HIP_TRY(hipMemcpy(x->stage[0].p, nwfa, n3 * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(x->stage[1].p, nwfa2d, n2 * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(launch_surface_emissions(x->stage[0].as<double>(), x->stage[1].as<double>(), dm->ni, dm->nj, dm->ldk,
+                                   dm->ldj, dm->d2ldj, k, dt, nullptr));
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(nwfa, x->stage[0].p, n3 * sizeof(double), hipMemcpyDeviceToHost));
+  return TEMPO_HIP_OK;
 }
 
 int tempo_hip_run_host(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags* d,

@@ -1016,6 +1016,29 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
   if (a.d3[D3_RE_S]) a.d3[D3_RE_S][d3] = re_s;
 }
 
+// -------------------------------------------- surface emissions ----
+// tempo_aerosol_surface_emissions (src/module_mp_tempo_driver.F90:516-529):
+// nwfa(i, k, j) += nwfa2d(i, j) * dt at one level, one lane per column.
+template <class R>
+__global__ void __launch_bounds__(256) k_surface_emissions(R* __restrict__ nwfa, const R* __restrict__ nwfa2d,
+                                                           int ni, int ncol, long long ldj, long long d2ldj,
+                                                           size_t koff, R dt) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  const int i = c % ni, j = c / ni;
+  R& x = nwfa[size_t(i) + koff + size_t(j) * size_t(ldj)];
+  x = x + nwfa2d[size_t(i) + size_t(j) * size_t(d2ldj)] * dt;
+}
+
+template <class R>
+hipError_t launch_surface_emissions(R* nwfa, const R* nwfa2d, int ni, int nj, long long ldk, long long ldj,
+                                    long long d2ldj, int k, double dt, hipStream_t s) {
+  const int ncol = ni * nj;
+  hipLaunchKernelGGL(k_surface_emissions<R>, dim3(unsigned((ncol + 255) / 256)), dim3(256), 0, s, nwfa, nwfa2d, ni,
+                     ncol, ldj, d2ldj, size_t(k) * size_t(ldk), R(dt));
+  return hipGetLastError();
+}
+
 // ------------------------------------------------ tangent-linear I/O ----
 // caller arrays (value, tangent) <-> one compact dual array, element (c, k)
 // at c + k*ncol

@@ -23,6 +23,7 @@ module tempo_hip_mod
   private
 
   public :: tempo_init, tempo_run, tempo_finalize, ty_tempo_cfgs, ty_tempo_driver_diags
+  public :: tempo_aerosol_surface_emissions
   public :: tempo_hip_cfg, tempo_hip_fields, tempo_hip_diags, tempo_hip_dims
   public :: tempo_hip_cfg_default, tempo_hip_init, tempo_hip_run, tempo_hip_run_host, tempo_hip_last_error
 
@@ -100,7 +101,19 @@ module tempo_hip_mod
       import :: c_ptr
       type(c_ptr), value :: ctx
     end subroutine
+    integer(c_int) function tempo_hip_aerosol_surface_emissions_host(ctx, dt, nwfa, nwfa2d, dims, k) &
+        bind(c, name='tempo_hip_aerosol_surface_emissions_host')
+      import :: c_ptr, c_int, c_double, tempo_hip_dims
+      type(c_ptr), value :: ctx, nwfa, nwfa2d
+      real(c_double), value :: dt
+      type(tempo_hip_dims), intent(in) :: dims
+      integer(c_int), value :: k
+    end function
   end interface
+
+  ! context of the last tempo_init, for tempo_aerosol_surface_emissions, whose
+  ! reference signature carries no configuration
+  type(c_ptr), save :: last_ctx = c_null_ptr
 
   ! ---- reference-shaped types ------------------------------------------
   ! ty_tempo_cfgs (src/module_mp_tempo_cfgs.F90:14-38) plus the library
@@ -262,12 +275,38 @@ contains
       rc = tempo_hip_init(c, c_null_ptr, c_loc(cccn), tempo_cfgs%ctx)
     endif
     if (rc /= 0) call fail('tempo_init', rc)
+    last_ctx = tempo_cfgs%ctx
   end subroutine
 
   subroutine tempo_finalize(tempo_cfgs)
     type(ty_tempo_cfgs), intent(inout) :: tempo_cfgs
     if (c_associated(tempo_cfgs%ctx)) call tempo_hip_finalize(tempo_cfgs%ctx)
+    if (c_associated(last_ctx, tempo_cfgs%ctx)) last_ctx = c_null_ptr
     tempo_cfgs%ctx = c_null_ptr
+  end subroutine
+
+  !> tempo_aerosol_surface_emissions (src/module_mp_tempo_driver.F90:516-529):
+  !! nwfa(i, kts, j) += nwfa2d(i, j) * dt over the memory extent, on the
+  !! device of the last tempo_init.
+  subroutine tempo_aerosol_surface_emissions(dt, nwfa, nwfa2d, ims, ime, jms, jme, kms, kme, kts)
+    real(wp), intent(in) :: dt
+    integer, intent(in) :: ims, ime, jms, jme, kms, kme, kts
+    real(wp), dimension(ims:ime, kms:kme, jms:jme), intent(inout), target :: nwfa
+    real(wp), dimension(ims:ime, jms:jme), intent(in), target :: nwfa2d
+    type(tempo_hip_dims) :: dm
+    integer(c_int) :: rc
+    if (.not. c_associated(last_ctx)) call fail('tempo_aerosol_surface_emissions (call tempo_init first)', -1)
+    dm%ni = ime - ims + 1
+    dm%nj = jme - jms + 1
+    dm%nz = kme - kms + 1
+    dm%ldk = dm%ni
+    dm%ldj = int(dm%ni, c_long_long) * dm%nz
+    dm%dldk = dm%ldk
+    dm%dldj = dm%ldj
+    dm%d2ldj = dm%ni
+    rc = tempo_hip_aerosol_surface_emissions_host(last_ctx, real(dt, c_double), c_loc(nwfa), c_loc(nwfa2d), &
+                                                  dm, int(kts - kms, c_int))
+    if (rc /= 0) call fail('tempo_aerosol_surface_emissions', rc)
   end subroutine
 
   !> tempo_run (src/module_mp_tempo_driver.F90:187-513) over the tile

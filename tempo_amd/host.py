@@ -104,6 +104,10 @@ def lib():
                                         ctypes.POINTER(Fields), ctypes.POINTER(Dims), ctypes.c_double, ctypes.c_int,
                                         ctypes.c_int, vp]
         L.tempo_hip_profile_read.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.tempo_hip_aerosol_surface_emissions.argtypes = [vp, ctypes.c_double, vp, vp, ctypes.POINTER(Dims),
+                                                          ctypes.c_int, ctypes.c_int, vp]
+        L.tempo_hip_aerosol_surface_emissions_host.argtypes = [vp, ctypes.c_double, vp, vp, ctypes.POINTER(Dims),
+                                                               ctypes.c_int]
         L.tempo_hip_last_error.restype = ctypes.c_char_p
         L.tempo_hip_finalize.argtypes = [vp]
         L.tempo_hip_finalize.restype = None
@@ -330,6 +334,29 @@ class TempoHip:
                                         float(dt), int(itimestep)))
         return out
 
+    def surface_emissions(self, nwfa, nwfa2d, dt, k=0, stream=None):
+        """tempo_aerosol_surface_emissions (src/module_mp_tempo_driver.F90:516-529):
+        nwfa[k, c] += nwfa2d[c] * dt in place; nwfa a (nz, ncol) device tensor,
+        nwfa2d (ncol,), both float32 or both float64; k is 0-based (kts - kms)."""
+        import torch
+        nz, ncol = nwfa.shape
+        assert nwfa.is_contiguous() and nwfa2d.is_contiguous() and nwfa2d.shape == (ncol,)
+        assert nwfa.dtype == nwfa2d.dtype and nwfa.is_cuda and nwfa2d.is_cuda
+        precision = FP32 if nwfa.dtype == torch.float32 else FP64
+        dims = Dims(ncol, 1, nz, ncol, nz * ncol, ncol, nz * ncol, ncol, None)
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        _check(lib().tempo_hip_aerosol_surface_emissions(self._ctx, float(dt), nwfa.data_ptr(), nwfa2d.data_ptr(),
+                                                         ctypes.byref(dims), int(k), precision, s))
+
+    def surface_emissions_host(self, nwfa, nwfa2d, dt, k=0):
+        """As surface_emissions on host float64 arrays (staged through device memory)."""
+        nz, ncol = nwfa.shape
+        assert nwfa.dtype == np.float64 and nwfa.flags.c_contiguous and nwfa2d.shape == (ncol,)
+        nwfa2d = np.ascontiguousarray(nwfa2d, dtype=np.float64)
+        dims = Dims(ncol, 1, nz, ncol, nz * ncol, ncol, nz * ncol, ncol, None)
+        _check(lib().tempo_hip_aerosol_surface_emissions_host(self._ctx, float(dt), nwfa.ctypes.data,
+                                                              nwfa2d.ctypes.data, ctypes.byref(dims), int(k)))
+
     def export_table(self, which):
         out = np.empty(TABLE_SIZES[which], dtype=np.float64)
         _check(lib().tempo_hip_table_export(self._ctx, which, out.ctypes.data, out.size))
@@ -381,3 +408,9 @@ def tempo_run(ctx, dt, itimestep, t=None, th=None, pii=None, p=None, w=None, dz=
     state = dict(t=t, th=th, pii=pii, p=p, w=w, dz=dz, qv=qv, qc=qc, qr=qr, qi=qi, qs=qs, qg=qg, ni=ni, nr=nr,
                  nc=nc, nwfa=nwfa, nifa=nifa, ng=ng, qb=qb)
     return ctx.run({k: v for k, v in state.items() if v is not None}, dt, itimestep, nlev=nlev, land=land_input)
+
+
+def tempo_aerosol_surface_emissions(ctx, dt, nwfa, nwfa2d, kts=0):
+    """tempo_aerosol_surface_emissions (src/module_mp_tempo_driver.F90:516-529)
+    on an (nz, ncol) device tensor nwfa and (ncol,) nwfa2d; kts 0-based."""
+    ctx.surface_emissions(nwfa, nwfa2d, dt, kts)

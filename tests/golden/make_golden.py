@@ -179,6 +179,20 @@ def main():
     run_case("semi_multistep", st, m["dt"], m["itimestep"], dict(BOTH, semi_sedi=True), steps=2)
     st, m = gen.generate("plain", ncol=16, col0=900)
     run_case("semi_plain_dt300", st, 300.0, 2, dict(PLAIN, semi_sedi=True))
+    # tempo_aerosol_surface_emissions (src/module_mp_tempo_driver.F90:516-529)
+    if not ONLY or "surface_emissions" in ONLY:
+        rng = np.random.default_rng(77)
+        nwfa = np.ascontiguousarray(rng.uniform(1e7, 1e9, (3, 6, 7)))  # (j, k, i): Fortran (7, 6, 3)
+        nwfa2d = np.ascontiguousarray(rng.uniform(0.0, 5e4, (3, 7)))
+        out = nwfa.copy()
+        ref.surface_emissions(20.0, out, nwfa2d, 2)
+        chk = nwfa.copy()
+        orc.surface_emissions(20.0, chk, nwfa2d, 2)
+        if not np.array_equal(chk, out):
+            raise SystemExit("surface_emissions: C restatement differs from the reference")
+        np.savez_compressed(os.path.join(HERE, "surface_emissions.npz"), nwfa=nwfa, nwfa2d=nwfa2d, out=out,
+                            dt=np.float64(20.0), kts=np.int32(2))
+        print("wrote surface_emissions")
     if ONLY:
         return
 

@@ -410,3 +410,35 @@ def test_gpu_tune_rejects_bad_values(hip):
         with pytest.raises(hip.TempoError) as e:
             ctx.tune(hip.FP32, key, val)
         assert e.value.code == hip.E_ARG
+
+
+def test_gpu_surface_emissions_matches_reference_fixture(hip):
+    """tempo_hip_aerosol_surface_emissions (device, FP64 and FP32) and the
+    host-array entry point against the reference's output
+    (src/module_mp_tempo_driver.F90:516-529).  The fixture's nwfa is
+    Fortran (ni, nk, nj); the C ABI takes it through ldk = ni, ldj = ni*nk."""
+    import ctypes
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "surface_emissions.npz"))
+    nj, nk, ni = z["nwfa"].shape
+    dt, k = float(z["dt"]), int(z["kts"]) - 1
+    ctx = context(hip, dict(aerosolaware=True, hailaware=False))
+    dims = hip.Dims(ni, nj, nk, ni, ni * nk, ni, ni * nk, ni, None)
+    for tdt, prec, tol in ((torch.float64, hip.FP64, 0.0), (torch.float32, hip.FP32, 1e-6)):
+        nwfa = torch.tensor(z["nwfa"], dtype=tdt, device="cuda")
+        n2 = torch.tensor(z["nwfa2d"], dtype=tdt, device="cuda")
+        hip._check(hip.lib().tempo_hip_aerosol_surface_emissions(ctx._ctx, dt, nwfa.data_ptr(), n2.data_ptr(),
+                                                                 ctypes.byref(dims), k, prec, None))
+        torch.cuda.synchronize()
+        got = nwfa.double().cpu().numpy()
+        if tol == 0.0:
+            np.testing.assert_array_equal(got, z["out"])
+        else:
+            np.testing.assert_allclose(got, z["out"], rtol=tol)
+    host = z["nwfa"].copy()
+    hip._check(hip.lib().tempo_hip_aerosol_surface_emissions_host(ctx._ctx, dt, host.ctypes.data,
+                                                                  np.ascontiguousarray(z["nwfa2d"]).ctypes.data,
+                                                                  ctypes.byref(dims), k))
+    np.testing.assert_array_equal(host, z["out"])
+    with pytest.raises(hip.TempoError) as e:
+        ctx.surface_emissions(torch.zeros((4, 5), device="cuda"), torch.zeros(5, device="cuda"), dt, k=4)
+    assert e.value.code == hip.E_ARG

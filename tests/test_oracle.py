@@ -2,11 +2,12 @@
 answers of the reference's unit tests, closed forms and -- where the
 reference build is present -- the reference itself."""
 import math
+import os
 
 import numpy as np
 import pytest
 
-from conftest import golden_cases, load_case
+from conftest import GOLDEN, golden_cases, load_case
 from oracle import oracle as orc
 from oracle import ref
 
@@ -202,3 +203,15 @@ def test_oracle_matches_reference_build_directly(golden_tables):
         rb = orc.run(b, m["dt"], m["itimestep"])
         for k in a:
             np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+def test_oracle_surface_emissions_matches_reference_fixture():
+    """tempo_aerosol_surface_emissions (src/module_mp_tempo_driver.F90:516-529):
+    the C restatement against the reference's output on a 7 x 6 x 3 block
+    (kts = 2), and only level kts changes."""
+    z = np.load(os.path.join(GOLDEN, "surface_emissions.npz"))
+    out = z["nwfa"].copy()
+    orc.surface_emissions(float(z["dt"]), out, z["nwfa2d"], int(z["kts"]))
+    np.testing.assert_array_equal(out, z["out"])
+    changed = np.nonzero((z["out"] != z["nwfa"]).any(axis=(0, 2)))[0]
+    assert changed.tolist() == [int(z["kts"]) - 1]
