@@ -58,47 +58,42 @@ struct AdjArgs {
 };
 
 // ---- tangent seeding and collection (one lane per (column, level)) ------
+// seed the tangent of one input field, clearing the previous pass's seed
+// (prev < 0: none)
 template <class D>
-__global__ void __launch_bounds__(256) k_adj_seed_input(StepArgs<D> a, int field) {
+__global__ void __launch_bounds__(256) k_adj_seed_input(StepArgs<D> a, int field, int prev) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
-  if (c >= a.ncol || !a.f[field]) return;
-  a.f[field][TEMPO_IDX_STATE(a, c, k)].d = 1;
+  if (c >= a.ncol) return;
+  const size_t o = TEMPO_IDX_STATE(a, c, k);
+  if (prev >= 0 && a.f[prev]) a.f[prev][o].d = 0;
+  if (a.f[field]) a.f[field][o].d = 1;
 }
 
-// after phase A of a local pass: keep dz/dx_i of the sedimentation inputs,
-// then remove their tangents so that phase B adds no level coupling
+// after phase A of a local pass: keep dz/dx_i of the sedimentation inputs
+// (masses, numbers, raw fall speeds, density) for the S^T term, and replace
+// the values of the pre-sedimentation tendencies by the forward's
+// post-sedimentation ones, keeping their local tangents (S frozen out).  The
+// final update reads neither the sedimenting masses nor the fall speeds, so
+// their tangents need not be cleared.
 template <class D>
-__global__ void __launch_bounds__(256) k_adj_capture_z(StepArgs<D> a, AdjArgs g, int i) {
+__global__ void __launch_bounds__(256) k_adj_capture_merge(StepArgs<D> a, AdjArgs g, int i,
+                                                           const Dual<double>* __restrict__ post_sd) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
   if (c >= a.ncol) return;
   const size_t s = TEMPO_IDX_SCR(a, c, k), e = size_t(c) + size_t(k) * a.ncol, n3 = size_t(a.ncol) * a.nz;
   double* ja = g.ja + size_t(i) * AZ_N * n3;
   const int sdf[10] = {SD_RR, SD_NR, SD_RG, SD_NG, SD_RB, SD_RS, SD_RI, SD_NI, SD_RC, SD_NC};
   const int srf[9] = {SR_VTRR, SR_VTNR, SR_VTRG, SR_VTNG, SR_VTS, SR_VTRI, SR_VTNI, SR_VTRC, SR_VTNC};
-  for (int q = 0; q < 10; ++q) {
-    auto& x = a.sd[size_t(sdf[q]) * a.plane + s];
-    ja[size_t(AZ_RR + q) * n3 + e] = double(x.d);
-    x.d = 0;
-  }
-  for (int q = 0; q < 9; ++q) {
-    auto& x = a.sr[size_t(srf[q]) * a.plane + s];
-    ja[size_t(AZ_VTRR + q) * n3 + e] = double(x.d);
-    x.d = 0;
-  }
-  // density: B reads it, and C reads the same value -- keep C's tangent,
-  // record it for S^T (whose density cotangent is added separately)
+#pragma unroll
+  for (int q = 0; q < 10; ++q) ja[size_t(AZ_RR + q) * n3 + e] = double(a.sd[size_t(sdf[q]) * a.plane + s].d);
+#pragma unroll
+  for (int q = 0; q < 9; ++q) ja[size_t(AZ_VTRR + q) * n3 + e] = double(a.sr[size_t(srf[q]) * a.plane + s].d);
   ja[size_t(AZ_RHO) * n3 + e] = double(a.sr[size_t(SR_RHO) * a.plane + s].d);
-}
-
-// phase B of the local pass must not see the density tangent either: it is
-// restored for phase C from the recorded value
-template <class D>
-__global__ void __launch_bounds__(256) k_adj_rho_tangent(StepArgs<D> a, AdjArgs g, int i, int restore) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
-  if (c >= a.ncol) return;
-  const size_t s = TEMPO_IDX_SCR(a, c, k), e = size_t(c) + size_t(k) * a.ncol, n3 = size_t(a.ncol) * a.nz;
-  auto& x = a.sr[size_t(SR_RHO) * a.plane + s];
-  x.d = restore ? decltype(x.d)(g.ja[(size_t(i) * AZ_N + AZ_RHO) * n3 + e]) : decltype(x.d)(0);
+#pragma unroll
+  for (int m = 0; m < 10; ++m) {
+    const size_t x = size_t(SD_QRTEN + m) * a.plane + s;
+    a.sd[x].v = post_sd[x].v;
+  }
 }
 
 // seed the tangent of one post-sedimentation tendency plane (u pass),
@@ -112,19 +107,6 @@ __global__ void __launch_bounds__(256) k_adj_seed_ten(StepArgs<D> a, int m, int 
   a.sd[size_t(SD_QRTEN + m) * a.plane + s].d = 1;
 }
 
-// local pass without phase B: the post-sedimentation tendency values from
-// the forward run, the tangents of the pre-sedimentation ones (S frozen out)
-template <class D>
-__global__ void __launch_bounds__(256) k_adj_merge_ten(StepArgs<D> a, const Dual<double>* __restrict__ post_sd) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
-  if (c >= a.ncol) return;
-  const size_t s = TEMPO_IDX_SCR(a, c, k);
-  for (int m = 0; m < 10; ++m) {
-    const size_t i = size_t(SD_QRTEN + m) * a.plane + s;
-    a.sd[i].v = post_sd[i].v;
-  }
-}
-
 // out[e] (+)= sum over outputs of ybar * tangent
 template <class D>
 __global__ void __launch_bounds__(256) k_adj_contract(StepArgs<D> a, AdjArgs g, double* out, int accumulate) {
@@ -133,18 +115,19 @@ __global__ void __launch_bounds__(256) k_adj_contract(StepArgs<D> a, AdjArgs g, 
   const size_t o = TEMPO_IDX_STATE(a, c, k), e = size_t(c) + size_t(k) * a.ncol, n3 = size_t(a.ncol) * a.nz;
   double acc = 0.0;
   for (int q = 0; q < AY_REFL; ++q) {
-    const D* f = a.f[adj_y_field(q)];
+    const D* f = a.fo[adj_y_field(q)];
     if (f) acc += g.ybar[size_t(q) * n3 + e] * double(f[o].d);
   }
   if (a.d3[D3_REFL]) acc += g.ybar[size_t(AY_REFL) * n3 + e] * double(a.d3[D3_REFL][o].d);
   out[e] = accumulate ? out[e] + acc : acc;
 }
 
-// xbar_i += (dz/dx_i) . zbar
-static __global__ void __launch_bounds__(256) k_adj_finish(AdjArgs g, size_t n3) {
+// xbar_i += (dz/dx_i) . zbar for the requested fields (bit i of mask)
+static __global__ void __launch_bounds__(256) k_adj_finish(AdjArgs g, size_t n3, unsigned mask) {
   const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= n3) return;
   for (int i = 0; i < AX_N; ++i) {
+    if (!((mask >> i) & 1u)) continue;
     double acc = g.xbar[size_t(i) * n3 + e];
     const double* ja = g.ja + size_t(i) * AZ_N * n3;
     for (int z = 0; z < AZ_N; ++z) acc += ja[size_t(z) * n3 + e] * g.zbar[size_t(z) * n3 + e];
@@ -463,10 +446,6 @@ __global__ void __launch_bounds__(256) k_from_compact(const double* __restrict__
   dst[(c % g.ni) + k * g.ldk + (c / g.ni) * g.ldj] = T(src[e]);
 }
 
-static __global__ void __launch_bounds__(256) k_fill_zero(double* __restrict__ p, size_t n) {
-  const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e < n) p[e] = 0.0;
-}
 
 template <class T>
 hipError_t launch_to_compact(const T* src, const DualGeom& g, double* dst, hipStream_t s) {

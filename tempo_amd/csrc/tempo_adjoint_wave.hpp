@@ -448,79 +448,68 @@ __global__ void __launch_bounds__(64 * kScrW) k_adj_sed_wave(StepArgs<D> a, AdjA
 // `inputs` is the packed input state.  snap_a / snap_b receive the working
 // state and scratch after phase A and after phase B of a tangent-free run,
 // so that the later passes restart from them instead of recomputing.
+// scratch snapshots of the adjoint: working-precision and FP64 planes after
+// phase A, FP64 planes after sedimentation
 struct AdjSnap {
-  void* state;       // state_bytes
-  void* sr;          // sr_bytes
-  void* sd;          // sd_bytes
+  void* sr_a;  // sr_bytes
+  void* sd_a;  // sd_bytes
+  void* sd_b;  // sd_bytes
 };
 
+// The passes of one adjoint.  The state is out of place: a.f holds the
+// packed inputs (duals, tangents zero except the seeded field), a.fa the
+// state after the initial checks (phase A's output), a.fo the new state
+// (the final update's output), so that no pass has to restore the state.
 template <class D>
-hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* vtbuf, const void* inputs,
-                          void* work_state, size_t state_bytes, size_t sr_bytes, size_t sd_bytes, AdjSnap snap_a,
-                          AdjSnap snap_b, const bool* have_x, hipStream_t s) {
+hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* vtbuf, size_t sr_bytes,
+                          size_t sd_bytes, AdjSnap snap, const bool* have_x, hipStream_t s) {
   const unsigned gc = unsigned((a.ncol + 255) / 256);
   const dim3 g2(gc, unsigned(a.nz));
   const size_t n3 = size_t(a.ncol) * a.nz;
   hipError_t e;
   auto cp = [&](void* dst, const void* src, size_t n) { return hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, s); };
-  auto save = [&](const AdjSnap& sn) -> hipError_t {
-    hipError_t r = cp(sn.state, work_state, state_bytes);
-    if (r == hipSuccess) r = cp(sn.sr, a.sr, sr_bytes);
-    if (r == hipSuccess) r = cp(sn.sd, a.sd, sd_bytes);
-    return r;
-  };
-  auto load = [&](const AdjSnap& sn) -> hipError_t {
-    hipError_t r = cp(work_state, sn.state, state_bytes);
-    if (r == hipSuccess) r = cp(a.sr, sn.sr, sr_bytes);
-    if (r == hipSuccess) r = cp(a.sd, sn.sd, sd_bytes);
-    return r;
-  };
   auto phase_pre_a = [&]() {
     hipLaunchKernelGGL(k_prepass<D>, dim3(gc), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_phase_a<D>, level_grid(a, 256), dim3(256), 0, s, a);
   };
   auto phase_c = [&]() { hipLaunchKernelGGL(k_phase_c<D>, level_grid(a, 256), dim3(256), 0, s, a); };
 
-  // tangent-free forward: snapshots after A and after B
-  if ((e = cp(work_state, inputs, state_bytes)) != hipSuccess) return e;
+  // tangent-free forward: scratch snapshots after A and after B
   phase_pre_a();
-  if ((e = save(snap_a)) != hipSuccess) return e;
+  if ((e = cp(snap.sr_a, a.sr, sr_bytes)) != hipSuccess) return e;
+  if ((e = cp(snap.sd_a, a.sd, sd_bytes)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_phase_b<D>, dim3(unsigned((a.ncol + 127) / 128)), dim3(128), 0, s, a, vtbuf);
-  if ((e = save(snap_b)) != hipSuccess) return e;
-  // passes seeded on the post-sedimentation tendencies
-  // (phase C rewrites only the state: restore it, move the seed)
+  if ((e = cp(snap.sd_b, a.sd, sd_bytes)) != hipSuccess) return e;
+  // passes seeded on the post-sedimentation tendencies (the final update
+  // reads the inputs, a.fa and the scratch, and writes only a.fo)
   for (int m = 0; m < 10; ++m) {
-    if (m > 0 && (e = cp(work_state, snap_b.state, state_bytes)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_adj_seed_ten<D>, g2, dim3(256), 0, s, a, m, m - 1);
     phase_c();
     hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, a, g, g.ubar + size_t(m) * n3, 0);
   }
   // S^T on the values after phase A
-  if ((e = load(snap_a)) != hipSuccess) return e;
+  if ((e = cp(a.sr, snap.sr_a, sr_bytes)) != hipSuccess) return e;
+  if ((e = cp(a.sd, snap.sd_a, sd_bytes)) != hipSuccess) return e;
   if (a.nz <= kSedLevels)
     hipLaunchKernelGGL(k_adj_sed_wave<D>, dim3(unsigned((a.ncol + kScrW - 1) / kScrW)), dim3(64 * kScrW), 0, s, a, g);
   else
     hipLaunchKernelGGL(k_adj_sedimentation<D>, dim3(unsigned((a.ncol + 63) / 64)), dim3(64), 0, s, a, g);
-  // local passes, one per differentiated input field; phase B is replaced by
-  // the forward's post-sedimentation tendency values
-  const Dual<double>* post_sd = static_cast<const Dual<double>*>(snap_b.sd);
+  // local passes, one per requested input field; phase B is replaced by the
+  // forward's post-sedimentation tendency values
+  const Dual<double>* post_sd = static_cast<const Dual<double>*>(snap.sd_b);
+  unsigned mask = 0;
+  int prev = -1;
   for (int i = 0; i < AX_N; ++i) {
-    double* xb = g.xbar + size_t(i) * n3;
-    if (!have_x[i]) {
-      hipLaunchKernelGGL(k_fill_zero, dim3(unsigned((n3 + 255) / 256)), dim3(256), 0, s, xb, n3);
-      hipLaunchKernelGGL(k_fill_zero, dim3(unsigned((AZ_N * n3 + 255) / 256)), dim3(256), 0, s,
-                         g.ja + size_t(i) * AZ_N * n3, AZ_N * n3);
-      continue;
-    }
-    if ((e = cp(work_state, inputs, state_bytes)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_adj_seed_input<D>, g2, dim3(256), 0, s, a, adj_x_field(i));
+    if (!have_x[i]) continue;
+    mask |= 1u << i;
+    hipLaunchKernelGGL(k_adj_seed_input<D>, g2, dim3(256), 0, s, a, adj_x_field(i), prev < 0 ? -1 : adj_x_field(prev));
     phase_pre_a();
-    hipLaunchKernelGGL(k_adj_capture_z<D>, g2, dim3(256), 0, s, a, g, i);
-    hipLaunchKernelGGL(k_adj_merge_ten<D>, g2, dim3(256), 0, s, a, post_sd);
+    hipLaunchKernelGGL(k_adj_capture_merge<D>, g2, dim3(256), 0, s, a, g, i, post_sd);
     phase_c();
-    hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, a, g, xb, 0);
+    hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, a, g, g.xbar + size_t(i) * n3, 0);
+    prev = i;
   }
-  hipLaunchKernelGGL(k_adj_finish, dim3(unsigned((n3 + 255) / 256)), dim3(256), 0, s, g, n3);
+  hipLaunchKernelGGL(k_adj_finish, dim3(unsigned((n3 + 255) / 256)), dim3(256), 0, s, g, n3, mask);
   return hipGetLastError();
 }
 

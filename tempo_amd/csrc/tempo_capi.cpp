@@ -26,12 +26,10 @@ hipError_t launch_surface_emissions(R* nwfa, const R* nwfa2d, int ni, int nj, lo
                                     long long d2ldj, int k, double dt, hipStream_t s);
 template <class T>
 hipError_t launch_dual_unpack(const Dual<T>* in, const DualGeom& g, T* v, T* t, hipStream_t s);
-hipError_t launch_adjoint(const StepArgs<Dual<float>>& a, const AdjArgs& g, Dual<double>* vtbuf, const void* inputs,
-                          void* work_state, size_t state_bytes, size_t sr_bytes, size_t sd_bytes, AdjSnap snap_a,
-                          AdjSnap snap_b, const bool* have_x, hipStream_t s);
-hipError_t launch_adjoint(const StepArgs<Dual<double>>& a, const AdjArgs& g, Dual<double>* vtbuf, const void* inputs,
-                          void* work_state, size_t state_bytes, size_t sr_bytes, size_t sd_bytes, AdjSnap snap_a,
-                          AdjSnap snap_b, const bool* have_x, hipStream_t s);
+hipError_t launch_adjoint(const StepArgs<Dual<float>>& a, const AdjArgs& g, Dual<double>* vtbuf, size_t sr_bytes,
+                          size_t sd_bytes, AdjSnap snap, const bool* have_x, hipStream_t s);
+hipError_t launch_adjoint(const StepArgs<Dual<double>>& a, const AdjArgs& g, Dual<double>* vtbuf, size_t sr_bytes,
+                          size_t sd_bytes, AdjSnap snap, const bool* have_x, hipStream_t s);
 template <class T>
 hipError_t launch_to_compact(const T* src, const DualGeom& g, double* dst, hipStream_t s);
 template <class T>
@@ -147,8 +145,8 @@ struct tempo_hip_ctx {
   DevBuf sr, sd, pf, cf_, precip, ksub, vtbuf;
   DevBuf semiw;  // semi-Lagrangian sedimentation work (allocated only with semi_sedi_flag)
   DevBuf tl_state, tl_diag;  // dual (value, tangent) copies for the tangent-linear step
-  DevBuf adj_in, adj_mem;    // adjoint: packed inputs, cotangent/Jacobian planes
-  DevBuf adj_snap;           // adjoint: state and scratch after phases A and B
+  DevBuf adj_in, adj_out, adj_mem;  // adjoint: packed inputs, new state, cotangent/Jacobian planes
+  DevBuf adj_snap;                  // adjoint: scratch after phases A and B
   int last_ncol = 0;
   // host-array staging
   std::vector<DevBuf> stage;
@@ -359,6 +357,7 @@ int bind_common(tempo_hip_ctx* x, StepArgs<R>& a, double dt, int itimestep) {
   a.tab.fzc = static_cast<const S*>(r64 ? x->fzc64.p : x->fzc32.p);
   a.tab.ccn = x->ccn.as<float>();
   a.cst = static_cast<const Const<S>*>(r64 ? x->cd_dev.p : x->cf_dev.p);
+  for (int i = 0; i < F_NFIELDS; ++i) a.fa[i] = a.fo[i] = a.f[i];  // in place
   a.tab.efrw = static_cast<const S*>(r64 ? x->efrw64.p : x->efrw32.p);
   a.tab.efsw = static_cast<const S*>(r64 ? x->efsw64.p : x->efsw32.p);
   a.tab.tnc_wev = static_cast<const S*>(r64 ? x->wev64.p : x->wev32.p);
@@ -514,6 +513,7 @@ int run_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fie
   const size_t nd = (AY_N + AX_N + AX_N * AZ_N + 10 + AZ_N) * n3 + 4 * size_t(ncol) + AW_N * plane;
   HIP_TRY(x->tl_state.ensure(sb));
   HIP_TRY(x->adj_in.ensure(sb));
+  HIP_TRY(x->adj_out.ensure(sb));
   HIP_TRY(x->tl_diag.ensure((D2_N * size_t(ncol) + D3_N * n3) * sizeof(D)));
   HIP_TRY(x->adj_mem.ensure(nd * sizeof(double)));
   double* mem = x->adj_mem.as<double>();
@@ -573,11 +573,17 @@ int run_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fie
   bool have_x[AX_N];
   for (int i = 0; i < AX_N; ++i) have_x[i] = fp[adj_x_field(i)] && xp[adj_x_field(i)];
   const size_t srb = SR_N * a.plane * sizeof(D), sdb = SD_N * a.plane * sizeof(Dual<double>);
-  HIP_TRY(x->adj_snap.ensure(2 * (sb + srb + sdb)));
+  HIP_TRY(x->adj_snap.ensure(srb + 2 * sdb));
   char* sp = static_cast<char*>(x->adj_snap.p);
-  const AdjSnap sa{sp, sp + sb, sp + sb + srb};
-  const AdjSnap sbb{sp + sb + srb + sdb, sp + 2 * sb + srb + sdb, sp + 2 * sb + 2 * srb + sdb};
-  HIP_TRY(launch_adjoint(a, g, x->vtbuf.as<Dual<double>>(), in, st, sb, srb, sdb, sa, sbb, have_x, s));
+  const AdjSnap snap{sp, sp + srb, sp + srb + sdb};
+  // out of place: inputs in adj_in, post-check state in tl_state, new state in adj_out
+  D* out = x->adj_out.as<D>();
+  for (int i = 0; i < F_NFIELDS; ++i) {
+    a.f[i] = fp[i] ? in + size_t(i) * n3 : nullptr;
+    a.fa[i] = fp[i] ? st + size_t(i) * n3 : nullptr;
+    a.fo[i] = fp[i] ? out + size_t(i) * n3 : nullptr;
+  }
+  HIP_TRY(launch_adjoint(a, g, x->vtbuf.as<Dual<double>>(), srb, sdb, snap, have_x, s));
   for (int i = 0; i < AX_N; ++i)
     if (have_x[i])
       HIP_TRY(launch_from_compact(g.xbar + size_t(i) * n3, g3, static_cast<T*>(xp[adj_x_field(i)]), s));

@@ -109,7 +109,7 @@ __device__ bool point_active_c(const StepArgs<R>& a, int c, int k) {
   if ((a.nlev && k >= a.nlev[c]) || (a.cf[c] & CF_EXIT)) return false;
   const size_t o = TEMPO_IDX_STATE(a, c, k), s = TEMPO_IDX_SCR(a, c, k);
   const R dt = R(a.dt);
-  auto present = [&](int f, int ten) { return a.f[f][o] + R(a.sd[size_t(ten) * a.plane + s]) * dt > R(kR1); };
+  auto present = [&](int f, int ten) { return a.fa[f][o] + R(a.sd[size_t(ten) * a.plane + s]) * dt > R(kR1); };
   return present(F_QC, SD_QCTEN) || present(F_QR, SD_QRTEN) || present(F_QI, SD_QITEN) ||
          present(F_QS, SD_QSTEN) || present(F_QG, SD_QGTEN);
 }
@@ -218,17 +218,17 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_A k_phase_a(StepArgs<R> a) {
   }
   graupel_check(C, L, qg1d, ng1d, qb1d, has_ngqb, dt, odt);
   // the post-check state is the 1-d state for the rest of the step
-  a.f[F_QC][o] = qc1d;
-  a.f[F_QR][o] = qr1d;
-  a.f[F_NR][o] = nr1d;
-  a.f[F_QI][o] = qi1d;
-  a.f[F_NI][o] = ni1d;
-  a.f[F_QS][o] = qs1d;
-  a.f[F_QG][o] = qg1d;
-  if (has_nc) a.f[F_NC][o] = nc1d;
+  a.fa[F_QC][o] = qc1d;
+  a.fa[F_QR][o] = qr1d;
+  a.fa[F_NR][o] = nr1d;
+  a.fa[F_QI][o] = qi1d;
+  a.fa[F_NI][o] = ni1d;
+  a.fa[F_QS][o] = qs1d;
+  a.fa[F_QG][o] = qg1d;
+  if (has_nc) a.fa[F_NC][o] = nc1d;
   if (has_ngqb) {
-    a.f[F_NG][o] = ng1d;
-    a.f[F_QB][o] = qb1d;
+    a.fa[F_NG][o] = ng1d;
+    a.fa[F_QB][o] = qb1d;
   }
   if (cf & CF_EXIT) return;
   L.qcten = L.qiten = L.qrten = L.qsten = L.qgten = L.ngten = L.qbten = L.niten = L.nrten = L.ncten = R(0);
@@ -624,7 +624,7 @@ __device__ void semi_rain_refresh(const StepArgs<R>& a, int c, int nzc, W_t<R>* 
     L.qrten = R(qrten);
     L.nrten = R(nrten);
     const R q0 = L.qrten, n0 = L.nrten;
-    R x = a.f[F_QR][o], xn = a.f[F_NR][o];
+    R x = a.fa[F_QR][o], xn = a.fa[F_NR][o];
     rain_check(C, L, x, xn, dt, odt);
     if (L.qrten != q0) qrten = W(L.qrten);
     if (L.nrten != n0) nrten = W(L.nrten);
@@ -661,8 +661,8 @@ __device__ void semi_graupel_refresh(const StepArgs<R>& a, int c, int nzc, W_t<R
     L.ngten = R(ngten);
     L.qbten = R(qbten);
     const R g0 = L.qgten, n0 = L.ngten, b0 = L.qbten;
-    R x = a.f[F_QG][o];
-    R xg = has_ngqb ? a.f[F_NG][o] : R(0.0), xb = has_ngqb ? a.f[F_QB][o] : R(0.0);
+    R x = a.fa[F_QG][o];
+    R xg = has_ngqb ? a.fa[F_NG][o] : R(0.0), xb = has_ngqb ? a.fa[F_QB][o] : R(0.0);
     graupel_check(C, L, x, xg, xb, has_ngqb, dt, odt);
     if (L.qgten != g0) qgten = W(L.qgten);
     if (L.ngten != n0) ngten = W(L.ngten);
@@ -890,9 +890,10 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
   L.pres = a.f[F_P][o];
   const R lvap = R(kLvap0) + R(2106.0 - 4218.0) * (L.temp - R(kT0));
   const R ocp = R(1.0) / (R(kCp) * (R(1.0) + R(0.887) * L.qv));
-  R qc1d = a.f[F_QC][o], qr1d = a.f[F_QR][o], qi1d = a.f[F_QI][o], qs1d = a.f[F_QS][o], qg1d = a.f[F_QG][o];
-  R ni1d = a.f[F_NI][o], nr1d = a.f[F_NR][o];
-  R nc1d = has_nc ? a.f[F_NC][o] : R(0), ng1d = has_ngqb ? a.f[F_NG][o] : R(0), qb1d = has_ngqb ? a.f[F_QB][o] : R(0);
+  R qc1d = a.fa[F_QC][o], qr1d = a.fa[F_QR][o], qi1d = a.fa[F_QI][o], qs1d = a.fa[F_QS][o], qg1d = a.fa[F_QG][o];
+  R ni1d = a.fa[F_NI][o], nr1d = a.fa[F_NR][o];
+  R nc1d = has_nc ? a.fa[F_NC][o] : R(0), ng1d = has_ngqb ? a.fa[F_NG][o] : R(0);
+  R qb1d = has_ngqb ? a.fa[F_QB][o] : R(0);
   const R ncv = column_nc(a, c);
 
   if (!a.fl.turn_off_micro) {  // freeze_cloud_melt_ice (:2124-2165)
@@ -919,8 +920,8 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
   const R t_new = load_t(a, o) + L.tten * dt;
   const R qv_new = mx(R(kMinQv), (a.f[F_QV][o] + L.qvten * dt));
   L.rho = rho_of(C, L.pres, L.temp, L.qv);
-  if (has_nwfa) a.f[F_NWFA][o] = mx(R(kNwfaDefault), mn(R(kAeroMax), (a.f[F_NWFA][o] + L.nwfaten * dt)));
-  if (has_nifa) a.f[F_NIFA][o] = mx(R(kNifaDefault), mn(R(kAeroMax), (a.f[F_NIFA][o] + L.nifaten * dt)));
+  if (has_nwfa) a.fo[F_NWFA][o] = mx(R(kNwfaDefault), mn(R(kAeroMax), (a.f[F_NWFA][o] + L.nwfaten * dt)));
+  if (has_nifa) a.fo[F_NIFA][o] = mx(R(kNifaDefault), mn(R(kAeroMax), (a.f[F_NIFA][o] + L.nifaten * dt)));
   // final checks (:750-781)
   cloud_check(C, L, qc1d, nc1d, has_nc, ncv, dt, odt);
   rain_check(C, L, qr1d, nr1d, dt, odt);
@@ -931,21 +932,21 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
   graupel_check(C, L, qg1d, ng1d, qb1d, has_ngqb, dt, odt);
 
   if (a.f[F_T])
-    a.f[F_T][o] = t_new;
+    a.fo[F_T][o] = t_new;
   else
-    a.f[F_TH][o] = t_new / a.f[F_PII][o];
-  a.f[F_QV][o] = qv_new;
-  a.f[F_QC][o] = qc1d;
-  a.f[F_QR][o] = qr1d;
-  a.f[F_NR][o] = nr1d;
-  a.f[F_QI][o] = qi1d;
-  a.f[F_NI][o] = ni1d;
-  a.f[F_QS][o] = qs1d;
-  a.f[F_QG][o] = qg1d;
-  if (has_nc) a.f[F_NC][o] = nc1d;
+    a.fo[F_TH][o] = t_new / a.f[F_PII][o];
+  a.fo[F_QV][o] = qv_new;
+  a.fo[F_QC][o] = qc1d;
+  a.fo[F_QR][o] = qr1d;
+  a.fo[F_NR][o] = nr1d;
+  a.fo[F_QI][o] = qi1d;
+  a.fo[F_NI][o] = ni1d;
+  a.fo[F_QS][o] = qs1d;
+  a.fo[F_QG][o] = qg1d;
+  if (has_nc) a.fo[F_NC][o] = nc1d;
   if (has_ngqb) {
-    a.f[F_NG][o] = ng1d;
-    a.f[F_QB][o] = qb1d;
+    a.fo[F_NG][o] = ng1d;
+    a.fo[F_QB][o] = qb1d;
   }
 
   // diagnostics (:796-863, src/module_mp_tempo_diags.F90)

@@ -181,7 +181,9 @@ struct Tables {
 
 template <class R>
 struct StepArgs {
-  R* f[F_NFIELDS];   // nullptr = absent (optional fields)
+  R* f[F_NFIELDS];   // nullptr = absent (optional fields); the step's inputs
+  R* fa[F_NFIELDS];  // state after the initial checks: written by phase A, read by the final update
+  R* fo[F_NFIELDS];  // new state written by the final update (fa = fo = f for a step in place)
   R* d2[D2_N];       // per column diagnostics (nullptr = not requested)
   R* d3[D3_N];       // per level diagnostics
   const int* land;   // per column land flag or nullptr

@@ -34,10 +34,9 @@ template hipError_t launch_surface_emissions<double>(double*, const double*, int
 template hipError_t launch_dual_pack<double>(const double*, const double*, const DualGeom&, Dual<double>*, hipStream_t);
 template hipError_t launch_dual_unpack<double>(const Dual<double>*, const DualGeom&, double*, double*, hipStream_t);
 
-hipError_t launch_adjoint(const StepArgs<Dual<double>>& a, const AdjArgs& g, Dual<double>* vtbuf, const void* inputs,
-                          void* work_state, size_t state_bytes, size_t sr_bytes, size_t sd_bytes, AdjSnap snap_a,
-                          AdjSnap snap_b, const bool* have_x, hipStream_t s) {
-  return adjoint_passes(a, g, vtbuf, inputs, work_state, state_bytes, sr_bytes, sd_bytes, snap_a, snap_b, have_x, s);
+hipError_t launch_adjoint(const StepArgs<Dual<double>>& a, const AdjArgs& g, Dual<double>* vtbuf, size_t sr_bytes,
+                          size_t sd_bytes, AdjSnap snap, const bool* have_x, hipStream_t s) {
+  return adjoint_passes(a, g, vtbuf, sr_bytes, sd_bytes, snap, have_x, s);
 }
 template hipError_t launch_to_compact<double>(const double*, const DualGeom&, double*, hipStream_t);
 template hipError_t launch_from_compact<double>(const double*, const DualGeom&, double*, hipStream_t);
