@@ -159,8 +159,8 @@ int tempo_hip_tlm(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip
  * entries ignored).  Writes dL/d(input) of t, qv, qc, qr, qi, qs, qg, nc,
  * nr, ni, nwfa, nifa, ng and qb into the matching entries of xbar (NULL
  * entries are skipped and cost nothing), with the branches frozen as in
- * tempo_hip_tlm; p, w, dz enter as constants.  Working memory (up to about
- * 4 kB per column-level) is allocated on first use. */
+ * tempo_hip_tlm; p, w, dz enter as constants.  Working memory (about 2.5 kB
+ * per column-level) is allocated on first use. */
 int tempo_hip_adjoint(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_fields *ybar,
                       const tempo_hip_diags *dbar, const tempo_hip_fields *xbar, const tempo_hip_dims *dims, double dt,
                       int itimestep, int precision, void *stream);

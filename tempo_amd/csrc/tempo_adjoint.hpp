@@ -51,7 +51,6 @@ struct AdjArgs {
   const double* ybar;   // AY_N planes (compact c + k*ncol), absent outputs all zero
   const double* pbar;   // 4 x ncol: rain, graupel, snow, ice precipitation
   double* xbar;         // AX_N planes (compact)
-  double* ja;           // AX_N x AZ_N planes (compact): dz/dx of the local passes
   double* ubar;         // 10 planes (compact): tendency cotangents, SD_QRTEN order
   double* zbar;         // AZ_N planes (compact)
   double* work;         // AW_N scratch planes (tiled like the step scratch)
@@ -69,26 +68,30 @@ __global__ void __launch_bounds__(256) k_adj_seed_input(StepArgs<D> a, int field
   if (a.f[field]) a.f[field][o].d = 1;
 }
 
-// after phase A of a local pass: keep dz/dx_i of the sedimentation inputs
-// (masses, numbers, raw fall speeds, density) for the S^T term, and replace
-// the values of the pre-sedimentation tendencies by the forward's
-// post-sedimentation ones, keeping their local tangents (S frozen out).  The
-// final update reads neither the sedimenting masses nor the fall speeds, so
-// their tangents need not be cleared.
+// after phase A of a local pass: the S^T part of xbar_i, (dz/dx_i) . zbar,
+// from the tangents of the sedimentation inputs (masses, numbers, raw fall
+// speeds, density) and the cotangents zbar that S^T has already produced;
+// then the values of the pre-sedimentation tendencies are replaced by the
+// forward's post-sedimentation ones, keeping their local tangents (S frozen
+// out).  The final update reads neither the sedimenting masses nor the fall
+// speeds, so their tangents need not be cleared.
 template <class D>
-__global__ void __launch_bounds__(256) k_adj_capture_merge(StepArgs<D> a, AdjArgs g, int i,
+__global__ void __launch_bounds__(256) k_adj_capture_merge(StepArgs<D> a, AdjArgs g, double* __restrict__ xb,
                                                            const Dual<double>* __restrict__ post_sd) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
   if (c >= a.ncol) return;
   const size_t s = TEMPO_IDX_SCR(a, c, k), e = size_t(c) + size_t(k) * a.ncol, n3 = size_t(a.ncol) * a.nz;
-  double* ja = g.ja + size_t(i) * AZ_N * n3;
   const int sdf[10] = {SD_RR, SD_NR, SD_RG, SD_NG, SD_RB, SD_RS, SD_RI, SD_NI, SD_RC, SD_NC};
   const int srf[9] = {SR_VTRR, SR_VTNR, SR_VTRG, SR_VTNG, SR_VTS, SR_VTRI, SR_VTNI, SR_VTRC, SR_VTNC};
+  double acc = 0.0;
 #pragma unroll
-  for (int q = 0; q < 10; ++q) ja[size_t(AZ_RR + q) * n3 + e] = double(a.sd[size_t(sdf[q]) * a.plane + s].d);
+  for (int q = 0; q < 10; ++q)
+    acc += double(a.sd[size_t(sdf[q]) * a.plane + s].d) * g.zbar[size_t(AZ_RR + q) * n3 + e];
 #pragma unroll
-  for (int q = 0; q < 9; ++q) ja[size_t(AZ_VTRR + q) * n3 + e] = double(a.sr[size_t(srf[q]) * a.plane + s].d);
-  ja[size_t(AZ_RHO) * n3 + e] = double(a.sr[size_t(SR_RHO) * a.plane + s].d);
+  for (int q = 0; q < 9; ++q)
+    acc += double(a.sr[size_t(srf[q]) * a.plane + s].d) * g.zbar[size_t(AZ_VTRR + q) * n3 + e];
+  acc += double(a.sr[size_t(SR_RHO) * a.plane + s].d) * g.zbar[size_t(AZ_RHO) * n3 + e];
+  xb[e] = acc;
 #pragma unroll
   for (int m = 0; m < 10; ++m) {
     const size_t x = size_t(SD_QRTEN + m) * a.plane + s;
@@ -122,18 +125,6 @@ __global__ void __launch_bounds__(256) k_adj_contract(StepArgs<D> a, AdjArgs g, 
   out[e] = accumulate ? out[e] + acc : acc;
 }
 
-// xbar_i += (dz/dx_i) . zbar for the requested fields (bit i of mask)
-static __global__ void __launch_bounds__(256) k_adj_finish(AdjArgs g, size_t n3, unsigned mask) {
-  const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= n3) return;
-  for (int i = 0; i < AX_N; ++i) {
-    if (!((mask >> i) & 1u)) continue;
-    double acc = g.xbar[size_t(i) * n3 + e];
-    const double* ja = g.ja + size_t(i) * AZ_N * n3;
-    for (int z = 0; z < AZ_N; ++z) acc += ja[size_t(z) * n3 + e] * g.zbar[size_t(z) * n3 + e];
-    g.xbar[size_t(i) * n3 + e] = acc;
-  }
-}
 
 // ---- S^T: sedimentation adjoint, one lane per column --------------------
 struct SedAdjCol {

@@ -494,22 +494,22 @@ hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* 
     hipLaunchKernelGGL(k_adj_sed_wave<D>, dim3(unsigned((a.ncol + kScrW - 1) / kScrW)), dim3(64 * kScrW), 0, s, a, g);
   else
     hipLaunchKernelGGL(k_adj_sedimentation<D>, dim3(unsigned((a.ncol + 63) / 64)), dim3(64), 0, s, a, g);
-  // local passes, one per requested input field; phase B is replaced by the
-  // forward's post-sedimentation tendency values
+  // local passes, one per requested input field: xbar_i = (dz/dx_i) . zbar
+  // + the local part through A and C, phase B replaced by the forward's
+  // post-sedimentation tendency values.  The pre-pass results (column flags,
+  // heights) depend on values only and stay from the first pass.
   const Dual<double>* post_sd = static_cast<const Dual<double>*>(snap.sd_b);
-  unsigned mask = 0;
   int prev = -1;
   for (int i = 0; i < AX_N; ++i) {
     if (!have_x[i]) continue;
-    mask |= 1u << i;
+    double* xb = g.xbar + size_t(i) * n3;
     hipLaunchKernelGGL(k_adj_seed_input<D>, g2, dim3(256), 0, s, a, adj_x_field(i), prev < 0 ? -1 : adj_x_field(prev));
-    phase_pre_a();
-    hipLaunchKernelGGL(k_adj_capture_merge<D>, g2, dim3(256), 0, s, a, g, i, post_sd);
+    hipLaunchKernelGGL(k_phase_a<D>, level_grid(a, 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_adj_capture_merge<D>, g2, dim3(256), 0, s, a, g, xb, post_sd);
     phase_c();
-    hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, a, g, g.xbar + size_t(i) * n3, 0);
+    hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, a, g, xb, 1);
     prev = i;
   }
-  hipLaunchKernelGGL(k_adj_finish, dim3(unsigned((n3 + 255) / 256)), dim3(256), 0, s, g, n3, mask);
   return hipGetLastError();
 }
 

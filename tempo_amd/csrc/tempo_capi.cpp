@@ -510,7 +510,7 @@ int run_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fie
   const int ncol = dm->ni * dm->nj, nz = dm->nz;
   const size_t n3 = size_t(ncol) * nz, sb = F_NFIELDS * n3 * sizeof(D);
   const size_t plane = scratch_plane(size_t(ncol), size_t(nz));
-  const size_t nd = (AY_N + AX_N + AX_N * AZ_N + 10 + AZ_N) * n3 + 4 * size_t(ncol) + AW_N * plane;
+  const size_t nd = (AY_N + AX_N + 10 + AZ_N) * n3 + 4 * size_t(ncol) + AW_N * plane;
   HIP_TRY(x->tl_state.ensure(sb));
   HIP_TRY(x->adj_in.ensure(sb));
   HIP_TRY(x->adj_out.ensure(sb));
@@ -521,8 +521,7 @@ int run_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fie
   double* ybar = mem;
   g.ybar = ybar;
   g.xbar = ybar + AY_N * n3;
-  g.ja = g.xbar + AX_N * n3;
-  g.ubar = g.ja + AX_N * AZ_N * n3;
+  g.ubar = g.xbar + AX_N * n3;
   g.zbar = g.ubar + 10 * n3;
   double* pbar = g.zbar + AZ_N * n3;
   g.pbar = pbar;
