@@ -41,13 +41,13 @@ def algorithmic_bytes(nz, ncol, width, fields_in, fields_out):
     output and diagnostic once.  The sedimentation kernel touches no state
     field: its bytes are the scratch it must exchange in this three-kernel
     split -- point flags, layer depth, density and the nine raw fall speeds
-    (working precision), the ten sedimenting masses/numbers and ten
-    tendencies read in FP64, the ten tendencies written in FP64."""
+    and the ten sedimenting masses/numbers and ten tendencies read, the ten
+    tendencies written, all in the working precision."""
     lv = nz * ncol
     return {
         "k_prepass": 0,
         "k_phase_a": fields_in * width * lv,
-        "k_phase_b": (4 + 11 * width + 30 * 8) * lv,
+        "k_phase_b": (4 + 41 * width) * lv,
         "k_phase_c": (fields_out + len(DIAG3)) * width * lv + 8 * width * ncol,
         "step": (fields_in + fields_out + len(DIAG3)) * width * lv + 8 * width * ncol,
     }

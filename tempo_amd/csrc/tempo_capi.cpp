@@ -318,14 +318,18 @@ int pack_tables(tempo_hip_ctx* x) {
 
 size_t scratch_plane(size_t ncol, size_t nz) { return (ncol + kScrW - 1) / kScrW * kScrW * size_t(scratch_levels(int(nz))); }
 
-// per-point scratch for ncol columns of nz levels; br / bw: bytes of one
-// working-precision / wide value (8 for the plain step, 16 for duals)
-int ensure_scratch(tempo_hip_ctx* x, size_t ncol, size_t nz, size_t br = 8, size_t bw = 8) {
+// per-point scratch for ncol columns of nz levels; br / bw / bd: bytes of
+// one working-precision / wide / sedimented-state value (8, 8, 4 or 8 for the
+// plain step, 16 for duals); vt_planes: wide planes of the one-lane-per-column
+// sedimentation (3 fall-speed planes, plus a wide copy of the SD_N planes
+// when these are stored narrower).  The defaults cover every step.
+int ensure_scratch(tempo_hip_ctx* x, size_t ncol, size_t nz, size_t br = 8, size_t bw = 8, size_t bd = 8,
+                   size_t vt_planes = 3 + SD_N) {
   const size_t plane = scratch_plane(ncol, nz);
   HIP_TRY(x->sr.ensure(SR_N * plane * br));
-  HIP_TRY(x->sd.ensure(SD_N * plane * bw));
+  HIP_TRY(x->sd.ensure(SD_N * plane * bd));
   HIP_TRY(x->pf.ensure(plane * sizeof(unsigned)));
-  HIP_TRY(x->vtbuf.ensure(3 * plane * bw));
+  HIP_TRY(x->vtbuf.ensure(vt_planes * plane * bw));
   HIP_TRY(x->cf_.ensure(ncol * sizeof(unsigned)));
   HIP_TRY(x->precip.ensure(5 * ncol * bw));
   HIP_TRY(x->ksub.ensure(10 * ncol * sizeof(int)));
@@ -338,12 +342,18 @@ template <class R>
 int bind_common(tempo_hip_ctx* x, StepArgs<R>& a, double dt, int itimestep) {
   using S = S_t<R>;
   using W = W_t<R>;
-  int rc = ensure_scratch(x, size_t(a.ncol), size_t(a.nz), sizeof(R), sizeof(W));
+  using D = Sd_t<R>;
+  // the wide copy is needed where the one-lane-per-column sedimentation runs
+  // on narrow planes
+  const LaunchCfg& lc = x->launch[sizeof(S) == 8 ? 1 : 0];
+  const bool lane_sed = !lc.sed_wave || a.nz > kSedLevels || x->cfg.semi_sedi_flag;
+  const size_t vt_planes = 3 + (sizeof(D) < sizeof(W) && lane_sed ? SD_N : 0);
+  int rc = ensure_scratch(x, size_t(a.ncol), size_t(a.nz), sizeof(R), sizeof(W), sizeof(D), vt_planes);
   if (rc) return rc;
   a.sr = x->sr.as<R>();
   a.plane = scratch_plane(size_t(a.ncol), size_t(a.nz));
   a.nzs = scratch_levels(a.nz);
-  a.sd = x->sd.as<W>();
+  a.sd = x->sd.as<D>();
   a.pf = x->pf.as<unsigned>();
   a.cf = x->cf_.as<unsigned>();
   a.precip = x->precip.as<W>();

@@ -10,6 +10,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace tempo {
 
 template <class T>
@@ -26,21 +28,29 @@ struct Dual {
   __host__ __device__ explicit operator unsigned() const { return unsigned(v); }
 };
 
-// scalar type of a working precision and its wide (FP64 accumulation) type
+// scalar type of a working precision, its wide (FP64 accumulation) type and
+// the type the sedimented state and tendencies are stored in between kernels:
+// the working precision for a plain step (the values are working-precision
+// numbers when phase A writes them and are rounded to it once, when the
+// sedimentation writes its tendencies back), FP64 duals for derivatives
 template <class R>
 struct Num {
   using S = R;
   using W = double;
+  using D = R;
 };
 template <class T>
 struct Num<Dual<T>> {
   using S = T;
   using W = Dual<double>;
+  using D = Dual<double>;
 };
 template <class R>
 using S_t = typename Num<R>::S;
 template <class R>
 using W_t = typename Num<R>::W;
+template <class R>
+using Sd_t = typename Num<R>::D;
 
 template <class R>
 __host__ __device__ __forceinline__ S_t<R> value(const R& x) { return x; }

@@ -164,6 +164,22 @@ __device__ __forceinline__ R default_aerosol(const StepArgs<R>& a, int c, int k,
   return n1 + n0 * mexp(-((hk - h1) / R(1000.0)) * ni3);
 }
 
+__device__ __forceinline__ bool same_bits(float x, float y) { return __float_as_uint(x) == __float_as_uint(y); }
+__device__ __forceinline__ bool same_bits(double x, double y) {
+  return __double_as_longlong(x) == __double_as_longlong(y);
+}
+template <class T>
+__device__ __forceinline__ bool same_bits(const Dual<T>& x, const Dual<T>& y) {
+  return same_bits(x.v, y.v) && same_bits(x.d, y.d);
+}
+// post-check value of field f at o; where the step runs in place the input
+// is still in memory (the read is a cache hit) and an unchanged value is not
+// stored again
+template <class R>
+__device__ __forceinline__ void put_post_check(const StepArgs<R>& a, int f, size_t o, const R& v) {
+  if (a.fa[f] != a.f[f] || !same_bits(a.f[f][o], v)) a.fa[f][o] = v;
+}
+
 template <class R>
 __global__ void __launch_bounds__(256) TEMPO_ATTR_A k_phase_a(StepArgs<R> a) {
   const int2 pt = block_point<R, false>(a);
@@ -217,18 +233,19 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_A k_phase_a(StepArgs<R> a) {
     qb1d = rb / L.rho;
   }
   graupel_check(C, L, qg1d, ng1d, qb1d, has_ngqb, dt, odt);
-  // the post-check state is the 1-d state for the rest of the step
-  a.fa[F_QC][o] = qc1d;
-  a.fa[F_QR][o] = qr1d;
-  a.fa[F_NR][o] = nr1d;
-  a.fa[F_QI][o] = qi1d;
-  a.fa[F_NI][o] = ni1d;
-  a.fa[F_QS][o] = qs1d;
-  a.fa[F_QG][o] = qg1d;
-  if (has_nc) a.fa[F_NC][o] = nc1d;
+  // the post-check state is the 1-d state for the rest of the step; a step
+  // in place writes back only the values the checks changed
+  put_post_check(a, F_QC, o, qc1d);
+  put_post_check(a, F_QR, o, qr1d);
+  put_post_check(a, F_NR, o, nr1d);
+  put_post_check(a, F_QI, o, qi1d);
+  put_post_check(a, F_NI, o, ni1d);
+  put_post_check(a, F_QS, o, qs1d);
+  put_post_check(a, F_QG, o, qg1d);
+  if (has_nc) put_post_check(a, F_NC, o, nc1d);
   if (has_ngqb) {
-    a.fa[F_NG][o] = ng1d;
-    a.fa[F_QB][o] = qb1d;
+    put_post_check(a, F_NG, o, ng1d);
+    put_post_check(a, F_QB, o, qb1d);
   }
   if (cf & CF_EXIT) return;
   L.qcten = L.qiten = L.qrten = L.qsten = L.qgten = L.ngten = L.qbten = L.niten = L.nrten = L.ncten = R(0);
@@ -379,28 +396,28 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_A k_phase_a(StepArgs<R> a) {
     sr[SR_RHOF * plane + s] = L.rhof;
     sr[SR_VISCO * plane + s] = L.visco;
   }
-  using W = W_t<R>;
-  W* sd = a.sd;
-  sd[SD_RR * plane + s] = W(L.rr);
-  sd[SD_NR * plane + s] = W(L.nr);
-  sd[SD_RG * plane + s] = W(L.rg);
-  sd[SD_NG * plane + s] = W(L.ng);
-  sd[SD_RB * plane + s] = W(L.rb);
-  sd[SD_RS * plane + s] = W(L.rs);
-  sd[SD_RI * plane + s] = W(L.ri);
-  sd[SD_NI * plane + s] = W(L.ni);
-  sd[SD_RC * plane + s] = W(L.rc);
-  sd[SD_NC * plane + s] = W(L.nc);
-  sd[SD_QRTEN * plane + s] = W(L.qrten);
-  sd[SD_NRTEN * plane + s] = W(L.nrten);
-  sd[SD_QGTEN * plane + s] = W(L.qgten);
-  sd[SD_NGTEN * plane + s] = W(L.ngten);
-  sd[SD_QBTEN * plane + s] = W(L.qbten);
-  sd[SD_QSTEN * plane + s] = W(L.qsten);
-  sd[SD_QITEN * plane + s] = W(L.qiten);
-  sd[SD_NITEN * plane + s] = W(L.niten);
-  sd[SD_QCTEN * plane + s] = W(L.qcten);
-  sd[SD_NCTEN * plane + s] = W(L.ncten);
+  using D = Sd_t<R>;
+  D* sd = a.sd;
+  sd[SD_RR * plane + s] = D(L.rr);
+  sd[SD_NR * plane + s] = D(L.nr);
+  sd[SD_RG * plane + s] = D(L.rg);
+  sd[SD_NG * plane + s] = D(L.ng);
+  sd[SD_RB * plane + s] = D(L.rb);
+  sd[SD_RS * plane + s] = D(L.rs);
+  sd[SD_RI * plane + s] = D(L.ri);
+  sd[SD_NI * plane + s] = D(L.ni);
+  sd[SD_RC * plane + s] = D(L.rc);
+  sd[SD_NC * plane + s] = D(L.nc);
+  sd[SD_QRTEN * plane + s] = D(L.qrten);
+  sd[SD_NRTEN * plane + s] = D(L.nrten);
+  sd[SD_QGTEN * plane + s] = D(L.qgten);
+  sd[SD_NGTEN * plane + s] = D(L.ngten);
+  sd[SD_QBTEN * plane + s] = D(L.qbten);
+  sd[SD_QSTEN * plane + s] = D(L.qsten);
+  sd[SD_QITEN * plane + s] = D(L.qiten);
+  sd[SD_NITEN * plane + s] = D(L.niten);
+  sd[SD_QCTEN * plane + s] = D(L.qcten);
+  sd[SD_NCTEN * plane + s] = D(L.ncten);
   a.pf[s] = pf;
 }
 
@@ -607,11 +624,10 @@ __device__ void semi_sediment(const StepArgs<R>& a, int c, int nzc, W_t<R>* xr, 
 // (top-down fill from 0 above the column).  A tendency keeps its FP64
 // accumulator unless the check rewrites it.
 template <class R>
-__device__ void semi_rain_refresh(const StepArgs<R>& a, int c, int nzc, W_t<R>* vt1, W_t<R>* vt2) {
+__device__ void semi_rain_refresh(const StepArgs<R>& a, int c, int nzc, W_t<R>* sd, W_t<R>* vt1, W_t<R>* vt2) {
   using W = W_t<R>;
   const Const<S_t<R>>& C = *a.cst;
   const size_t plane = a.plane;
-  W* sd = a.sd;
   const R* sr = a.sr;
   const R dt = R(a.dt), odt = R(1.0) / dt;
   R v1 = R(0.0), v2 = R(0.0);
@@ -641,11 +657,10 @@ __device__ void semi_rain_refresh(const StepArgs<R>& a, int c, int nzc, W_t<R>* 
 }
 
 template <class R>
-__device__ void semi_graupel_refresh(const StepArgs<R>& a, int c, int nzc, W_t<R>* vt1, W_t<R>* vt2) {
+__device__ void semi_graupel_refresh(const StepArgs<R>& a, int c, int nzc, W_t<R>* sd, W_t<R>* vt1, W_t<R>* vt2) {
   using W = W_t<R>;
   const Const<S_t<R>>& C = *a.cst;
   const size_t plane = a.plane;
-  W* sd = a.sd;
   const R* sr = a.sr;
   const R dt = R(a.dt), odt = R(1.0) / dt;
   const bool has_ngqb = a.f[F_NG] && a.f[F_QB];
@@ -704,7 +719,16 @@ __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, W_t<R>* vtbuf) {
   if (!(cf & CF_EXIT)) {
     unsigned any = 0;
     for (int k = 0; k < nzc; ++k) any |= a.pf[TEMPO_IDX_SCR(a, c, k)];
-    W* sd = a.sd;
+    // the column's masses and tendencies in FP64 while they sediment: the
+    // scratch planes themselves when they are wide, else a wide copy
+    constexpr bool narrow = !std::is_same<Sd_t<R>, W>::value;
+    W* sd = narrow ? vtbuf + 3 * plane : reinterpret_cast<W*>(a.sd);
+    if constexpr (narrow)
+      for (int q = 0; q < SD_N; ++q)
+        for (int k = 0; k < nzc; ++k) {
+          const size_t s = size_t(q) * plane + TEMPO_IDX_SCR(a, c, k);
+          sd[s] = W(a.sd[s]);
+        }
     const R* sr = a.sr;
     W* v1 = vtbuf;           // filled fall speeds, FP64
     W* v2 = vtbuf + plane;
@@ -717,7 +741,7 @@ __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, W_t<R>* vtbuf) {
         for (int n = 0; n < steps; ++n) {
           semi_sediment(a, c, nzc, sd + SD_RR * plane, sd + SD_QRTEN * plane, vr, steps, kR1, &pr[0]);
           semi_sediment(a, c, nzc, sd + SD_NR * plane, sd + SD_NRTEN * plane, v2, steps, kR2, (W*)nullptr);
-          semi_rain_refresh(a, c, nzc, vr, v2);
+          semi_rain_refresh(a, c, nzc, sd, vr, v2);
         }
       } else {
         for (int n = 0; n < steps; ++n) {
@@ -739,7 +763,7 @@ __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, W_t<R>* vtbuf) {
           semi_sediment(a, c, nzc, sd + SD_RG * plane, sd + SD_QGTEN * plane, v1, steps, kR1, &pr[1]);
           semi_sediment(a, c, nzc, sd + SD_NG * plane, sd + SD_NGTEN * plane, v2, steps, kR2, (W*)nullptr);
           semi_sediment(a, c, nzc, sd + SD_RB * plane, sd + SD_QBTEN * plane, v1, steps, rb_lim, (W*)nullptr);
-          semi_graupel_refresh(a, c, nzc, v1, v2);
+          semi_graupel_refresh(a, c, nzc, sd, v1, v2);
         }
       } else {
         for (int n = 0; n < steps; ++n) {
@@ -810,6 +834,12 @@ __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, W_t<R>* vtbuf) {
       kst[8] = 1;
       kst[9] = kt;
     }
+    if constexpr (narrow)  // the tendencies, rounded once to the working precision
+      for (int q = SD_QRTEN; q < SD_N; ++q)
+        for (int k = 0; k < nzc; ++k) {
+          const size_t s = size_t(q) * plane + TEMPO_IDX_SCR(a, c, k);
+          a.sd[s] = Sd_t<R>(sd[s]);
+        }
   }
   for (int q = 0; q < 5; ++q) a.precip[size_t(q) * a.ncol + c] = pr[q];
   if (a.ksub)
@@ -869,7 +899,7 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
   }
   Lvl<R> L;
   const R* sr = a.sr;
-  const W_t<R>* sd = a.sd;
+  const Sd_t<R>* sd = a.sd;
   L.temp = sr[SR_TEMP * plane + s];
   L.qv = sr[SR_QV * plane + s];
   L.rho = sr[SR_RHO * plane + s];

@@ -215,7 +215,7 @@ __global__ void __launch_bounds__(64 * kSedCols) __attribute__((amdgpu_waves_per
   const size_t plane = a.plane;
   const SedTile T{tile_mem, c0, a.ncol, a.nz, a.nzs, sed_row(a.nz)};
   const double dt = a.dt;
-  double* sd = a.sd;
+  Sd_t<R>* sd = a.sd;
   const R* sr = a.sr;
 
   SedCol g;

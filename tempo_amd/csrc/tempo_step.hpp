@@ -154,7 +154,7 @@ enum ScrR {
   SR_RHOF, SR_VISCO,  // density factor and viscosity (written only for semi-Lagrangian sedimentation)
   SR_N
 };
-// FP64 scratch (per point): sedimented state and tendencies
+// scratch of the sedimented state and tendencies (per point, Sd_t<R>)
 enum ScrD {
   SD_RR, SD_NR, SD_RG, SD_NG, SD_RB, SD_RS, SD_RI, SD_NI, SD_RC, SD_NC,
   SD_QRTEN, SD_NRTEN, SD_QGTEN, SD_NGTEN, SD_QBTEN, SD_QSTEN, SD_QITEN, SD_NITEN, SD_QCTEN, SD_NCTEN, SD_N
@@ -192,7 +192,7 @@ struct StepArgs {
   int nzs;           // level stride of a scratch column group (scratch_levels(nz))
   long long ldk, ldj, dldk, dldj, d2ldj;
   R* sr;             // SR_N planes
-  W_t<R>* sd;        // SD_N planes (FP64, or FP64 duals)
+  Sd_t<R>* sd;       // SD_N planes (working precision, or FP64 duals)
   size_t plane;      // elements per scratch plane: scratch_levels(nz) x ncol rounded up to kScrW columns
   unsigned* pf;      // nz x ncol point flags
   unsigned* cf;      // ncol column flags
