@@ -44,8 +44,8 @@ typedef struct tempo_hip_ctx tempo_hip_ctx;
 typedef struct {
   int aerosolaware_flag;          /* selects the aerosol-aware constants; physics follows field presence */
   int hailaware_flag;             /* hail-aware graupel fall-speed constants */
-  int ml_for_bl_nc_flag;          /* not supported (must be 0) */
-  int ml_for_nc_flag;             /* not supported (must be 0) */
+  int ml_for_bl_nc_flag;          /* cloud-number network loaded (for qc_bl/qcfrac_bl) */
+  int ml_for_nc_flag;             /* one-moment cloud number from the network (nc absent) */
   int semi_sedi_flag;             /* semi-Lagrangian rain and graupel sedimentation (tempo_hip_run only) */
   int refl10cm_from_melting_flag; /* not supported (must be 0) */
   int turn_off_micro_flag;
@@ -67,12 +67,17 @@ typedef struct {
 /* State of tempo_run (src/module_mp_tempo_driver.F90:207-241), device
  * pointers in the precision given to tempo_hip_run.  Either t, or th and pii,
  * must be given.  p, w, dz, qv, qc, qr, qi, qs, qg, ni, nr are required;
- * nc, nwfa, nifa, ng, qb and land (int per column) are optional (NULL). */
+ * nc, nwfa, nifa, ng, qb and land (int per column) are optional (NULL).
+ * qc_bl and qcfrac_bl (inputs, laid out like the state) are the boundary
+ * layer's cloud water and cloud fraction: given together, with the
+ * cloud-number network on, they enter the cloud effective radius
+ * (src/module_mp_tempo_main.F90:839-857). */
 typedef struct {
   void *t, *th, *pii, *p, *w, *dz;
   void *qv, *qc, *qr, *qi, *qs, *qg, *ni, *nr;
   void *nc, *nwfa, *nifa, *ng, *qb;
   const int *land;
+  const void *qc_bl, *qcfrac_bl;
 } tempo_hip_fields;
 
 /* ty_tempo_driver_diags (src/module_mp_tempo_driver.F90:18-34); any entry may
@@ -100,6 +105,13 @@ typedef struct {
 
 /* Fills the defaults of ty_tempo_cfgs (src/module_mp_tempo_cfgs.F90:15-34). */
 void tempo_hip_cfg_default(tempo_hip_cfg *cfg);
+
+/* The run-time flags of later tempo_hip_run / tempo_hip_run_host calls, as
+ * a host changes its ty_tempo_cfgs between tempo_run calls (tempo_run reads
+ * tempo_cfgs on every call, src/module_mp_tempo_driver.F90:187-205).
+ * aerosolaware_flag and hailaware_flag select constants at tempo_hip_init
+ * and must keep their values (E_ARG otherwise). */
+int tempo_hip_set_run_cfg(tempo_hip_ctx *ctx, const tempo_hip_cfg *cfg);
 
 /* Replaces tempo_init (src/module_mp_tempo_driver.F90:42-182): computes the
  * constants and init-time tables, and provides the three collection/freezing

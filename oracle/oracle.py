@@ -94,7 +94,16 @@ def run(state, dt, itimestep, land=None, nlev=None):
     if nlev is not None:
         nlev = np.ascontiguousarray(nlev, dtype=np.int32)
         nlevp = nlev.ctypes.data
+    # boundary-layer cloud inputs (qc_bl, qcfrac_bl), absent unless given
+    bl = [state.get(n) for n in ("qc_bl", "qcfrac_bl")]
+    for a in bl:
+        assert a is None or (a.shape == (nz, ncol) and a.dtype == np.float64 and a.flags.c_contiguous)
+    f_bl = lib().oracle_set_bl
+    f_bl.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    f_bl.restype = None
+    f_bl(*[None if a is None else a.ctypes.data for a in bl])
     rc = lib().oracle_run(ncol, nz, float(dt), int(itimestep), ptrs, landp, nlevp, p2, p3)
+    f_bl(None, None)
     if rc != 0:
         raise RuntimeError(f"oracle_run failed ({rc})")
     return d2, d3

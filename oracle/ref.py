@@ -128,7 +128,16 @@ def run(state, dt, itimestep, land=None, variant=None):
     if land is not None:
         land = np.ascontiguousarray(land, dtype=np.int32)
         landp = land.ctypes.data
+    # boundary-layer cloud inputs (qc_bl, qcfrac_bl), absent unless given
+    bl = [state.get(n) for n in ("qc_bl", "qcfrac_bl")]
+    for a in bl:
+        assert a is None or (a.shape == (nz, ncol) and a.dtype == np.float64 and a.flags.c_contiguous)
+    f_bl = lib(variant).ref_set_bl
+    f_bl.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    f_bl.restype = None
+    f_bl(*[None if a is None else a.ctypes.data for a in bl])
     rc = lib(variant).ref_run(ncol, nz, float(dt), int(itimestep), ptrs, landp, p2, p3)
+    f_bl(None, None)
     if rc != 0:
         raise RuntimeError("ref_run failed")
     return d2, d3

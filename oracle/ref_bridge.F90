@@ -24,10 +24,14 @@ module ref_bridge
   use module_mp_tempo_utils, only : compute_efrw, compute_efsw, compute_drop_evap, qi_aut_qs, &
     calc_gamma_p, calc_rslf, calc_rsif, snow_moments
   use module_mp_tempo_driver, only : tempo_run, ty_tempo_driver_diags, tempo_aerosol_surface_emissions
+  use module_mp_tempo_ml, only : ty_tempo_ml_data, save_or_read_ml_data, nc_ml_nodes, nc_ml_input, &
+    nc_ml_output, nc_ml_trans_mean, nc_ml_trans_var, nc_ml_w00, nc_ml_w01, nc_ml_b00, nc_ml_b01
   implicit none
 
   type(ty_tempo_cfgs), save :: cfgs
   type(ty_tempo_driver_diags), save :: diags
+  ! boundary-layer cloud inputs of the next ref_run (ref_set_bl)
+  type(c_ptr), save :: bl_qc = c_null_ptr, bl_frac = c_null_ptr
 
 contains
 
@@ -126,8 +130,36 @@ contains
 
     if (cfgs%max_hail_diameter_flag) call initialize_bins_for_hail_size()
     if (cfgs%refl10cm_from_melting_flag) call initialize_bins_for_radar()
+    if (cfgs%ml_for_bl_nc_flag .or. cfgs%ml_for_nc_flag) call load_ml_data()
     ref_init = 0
   end function ref_init
+
+  ! the network parameters handed to the reference as init_ml_data does
+  ! (src/module_mp_tempo_driver.F90:703-730; that routine is private)
+  subroutine load_ml_data()
+    type(ty_tempo_ml_data), dimension(1) :: d
+    d(1)%input_size = nc_ml_input
+    d(1)%node_size = nc_ml_nodes
+    d(1)%output_size = nc_ml_output
+    allocate(d(1)%transform_mean(nc_ml_input), d(1)%transform_var(nc_ml_input))
+    allocate(d(1)%weights00(nc_ml_nodes, nc_ml_input), d(1)%weights01(nc_ml_output, nc_ml_nodes))
+    allocate(d(1)%bias00(nc_ml_nodes), d(1)%bias01(nc_ml_output))
+    d(1)%transform_mean = nc_ml_trans_mean
+    d(1)%transform_var = nc_ml_trans_var
+    d(1)%weights00 = reshape(nc_ml_w00, [nc_ml_nodes, nc_ml_input])
+    d(1)%weights01 = reshape(nc_ml_w01, [nc_ml_output, nc_ml_nodes])
+    d(1)%bias00 = nc_ml_b00
+    d(1)%bias01 = nc_ml_b01
+    call save_or_read_ml_data(ml_data_in=d)
+  end subroutine load_ml_data
+
+  ! qc_bl and qcfrac_bl ((ncol, nz) column-major FP64, NULL = absent) for the
+  ! following ref_run calls
+  subroutine ref_set_bl(qc_bl, qcfrac_bl) bind(c, name='ref_set_bl')
+    type(c_ptr), value :: qc_bl, qcfrac_bl
+    bl_qc = qc_bl
+    bl_frac = qcfrac_bl
+  end subroutine ref_set_bl
 
   subroutine copy_in(src, dst, n1, n2)
     type(c_ptr), intent(in) :: src
@@ -175,7 +207,7 @@ contains
     type(c_ptr), intent(in) :: state(19), diag2(8), diag3(7)
     type(c_ptr), value :: land
     real(wp), allocatable, dimension(:,:,:) :: t, p, w, dz, qv, qc, qr, qi, qs, qg, &
-      ni, nr, nc, nwfa, nifa, ng, qb, th, pii
+      ni, nr, nc, nwfa, nifa, ng, qb, th, pii, qc_bl, qcfrac_bl
     integer(c_int), pointer :: pland(:,:)
     integer, allocatable :: land_input(:,:)
     integer :: n1
@@ -200,6 +232,8 @@ contains
     call copy_in(state(17), qb, n1, nz)
     call copy_in(state(18), th, n1, nz)
     call copy_in(state(19), pii, n1, nz)
+    call copy_in(bl_qc, qc_bl, n1, nz)
+    call copy_in(bl_frac, qcfrac_bl, n1, nz)
     if (c_associated(land)) then
       call c_f_pointer(land, pland, [n1, 1])
       allocate(land_input(n1, 1))
@@ -210,6 +244,7 @@ contains
       t=t, th=th, pii=pii, p=p, w=w, dz=dz, &
       qv=qv, qc=qc, qr=qr, qi=qi, qs=qs, qg=qg, ni=ni, nr=nr, &
       nc=nc, nwfa=nwfa, nifa=nifa, ng=ng, qb=qb, land_input=land_input, &
+      qc_bl=qc_bl, qcfrac_bl=qcfrac_bl, &
       ids=1, ide=n1, jds=1, jde=1, kds=1, kde=int(nz), &
       ims=1, ime=n1, jms=1, jme=1, kms=1, kme=int(nz), &
       its=1, ite=n1, jts=1, jte=1, kts=1, kte=int(nz), tempo_diags=diags)

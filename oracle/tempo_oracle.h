@@ -26,7 +26,7 @@ extern "C" {
  * nc_opt (0 land, 1 ocean), unused.
  * Tables are flat, in reference file order: ccn float32 (8820),
  * fz/qs/qg float64.  Returns 0, or a negative code for an unsupported
- * option (semi-Lagrangian sedimentation, melting-layer reflectivity, ML). */
+ * option (melting-layer reflectivity). */
 int oracle_init(const int *flags, const float *ccn, const double *fz, const double *qs,
                 const double *qg);
 
@@ -39,6 +39,9 @@ int oracle_init(const int *flags, const float *ccn, const double *fz, const doub
  * nc mixing ratio ((nz, ncol), may be NULL). */
 int oracle_run(int ncol, int nz, double dt, int itimestep, double *const *state, const int *land,
                const int *nlev, double *const *diag2, double *const *diag3);
+/* Boundary-layer cloud water and fraction ((nz, ncol), NULL = absent) for the
+ * following oracle_run calls (tempo_run's qc_bl, qcfrac_bl). */
+void oracle_set_bl(const double *qc_bl, const double *qcfrac_bl);
 
 double oracle_gamma_p(double a, double x);
 double oracle_rslf(double p, double t);

@@ -179,6 +179,16 @@ void tempo_hip_cfg_default(tempo_hip_cfg* c) {
 
 const char* tempo_hip_last_error(void) { return g_err.c_str(); }
 
+int tempo_hip_set_run_cfg(tempo_hip_ctx* x, const tempo_hip_cfg* cfg) {
+  if (!x || !cfg) return fail(TEMPO_HIP_E_ARG, "tempo_hip_set_run_cfg: NULL argument");
+  if (cfg->aerosolaware_flag != x->cfg.aerosolaware_flag || cfg->hailaware_flag != x->cfg.hailaware_flag)
+    return fail(TEMPO_HIP_E_ARG, "aerosolaware_flag and hailaware_flag are fixed at tempo_hip_init");
+  if (cfg->refl10cm_from_melting_flag)
+    return fail(TEMPO_HIP_E_UNSUPPORTED, "melting-layer reflectivity is not supported");
+  x->cfg = *cfg;
+  return TEMPO_HIP_OK;
+}
+
 void tempo_hip_finalize(tempo_hip_ctx* ctx) { delete ctx; }
 
 double tempo_hip_table_build_ms(tempo_hip_ctx* ctx) { return ctx ? ctx->build_ms : 0.0; }
@@ -390,6 +400,8 @@ int bind_common(tempo_hip_ctx* x, StepArgs<R>& a, double dt, int itimestep) {
   a.fl.mvd_r = c.rain_med_vol_diam_flag;
   a.fl.mvd_g = c.graupel_med_vol_diam_flag;
   a.fl.nc_mixing_ratio = c.cloud_number_mixing_ratio_flag;
+  a.fl.ml_for_nc = c.ml_for_nc_flag;
+  a.fl.ml_for_bl_nc = c.ml_for_bl_nc_flag;
   a.fl.nc_const = c.nc_option == 1 ? kNtCO : (c.nc_option == 2 ? c.nc_value : kNtCL);
   a.dt = dt;
   a.first_step = itimestep == 1;
@@ -427,6 +439,10 @@ int run_impl(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags*
     for (int i = 0; i < D3_N; ++i) a.d3[i] = static_cast<R*>(d3[i]);
   }
   a.land = f->land;
+  if ((f->qc_bl || f->qcfrac_bl) && !(x->cfg.ml_for_nc_flag || x->cfg.ml_for_bl_nc_flag))
+    return fail(TEMPO_HIP_E_ARG, "qc_bl/qcfrac_bl need the cloud-number network (ml_for_bl_nc_flag or ml_for_nc_flag)");
+  a.qc_bl = static_cast<const R*>(f->qc_bl);
+  a.qcfrac_bl = static_cast<const R*>(f->qcfrac_bl);
   a.nlev = dm->nlev;
   a.ni = dm->ni;
   a.nj = dm->nj;
@@ -460,6 +476,8 @@ int run_tlm(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fields*
   using D = Dual<T>;
   if (x->cfg.semi_sedi_flag)
     return fail(TEMPO_HIP_E_UNSUPPORTED, "the tangent-linear step covers flux-form sedimentation only");
+  if (x->cfg.ml_for_nc_flag || f->qc_bl || f->qcfrac_bl)
+    return fail(TEMPO_HIP_E_UNSUPPORTED, "the tangent-linear step does not cover the cloud-number network");
   const int ncol = dm->ni * dm->nj, nz = dm->nz;
   const size_t n3 = size_t(ncol) * nz;
   HIP_TRY(x->tl_state.ensure(F_NFIELDS * n3 * sizeof(D)));
@@ -516,6 +534,8 @@ int run_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fie
   using D = Dual<T>;
   if (x->cfg.semi_sedi_flag)
     return fail(TEMPO_HIP_E_UNSUPPORTED, "the adjoint covers flux-form sedimentation only");
+  if (x->cfg.ml_for_nc_flag || f->qc_bl || f->qcfrac_bl)
+    return fail(TEMPO_HIP_E_UNSUPPORTED, "the adjoint does not cover the cloud-number network");
   if (!f->t) return fail(TEMPO_HIP_E_UNSUPPORTED, "the adjoint takes temperature input t (not th/pii)");
   const int ncol = dm->ni * dm->nj, nz = dm->nz;
   const size_t n3 = size_t(ncol) * nz, sb = F_NFIELDS * n3 * sizeof(D);
@@ -608,8 +628,6 @@ int tempo_hip_init(const tempo_hip_cfg* cfg, const char* table_dir, const char* 
   *out = nullptr;
   if (cfg->refl10cm_from_melting_flag)
     return fail(TEMPO_HIP_E_UNSUPPORTED, "melting-layer reflectivity is not supported");
-  if (cfg->ml_for_nc_flag || cfg->ml_for_bl_nc_flag)
-    return fail(TEMPO_HIP_E_UNSUPPORTED, "ML cloud-number prediction is not supported");
   auto* x = new tempo_hip_ctx();
   x->cfg = *cfg;
   int rc;
@@ -728,8 +746,17 @@ int tempo_hip_run_host(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_
   void** hd = reinterpret_cast<void**>(const_cast<tempo_hip_diags*>(d));
   void** ddp = reinterpret_cast<void**>(&dd);
   const int nd = sizeof(tempo_hip_diags) / sizeof(void*);
-  if (x->stage.size() < size_t(19 + nd)) x->stage.resize(19 + nd);
+  if (x->stage.size() < size_t(19 + nd + 2)) x->stage.resize(19 + nd + 2);
   tempo_hip_fields df = *f;
+  const void* const hbl[2] = {f->qc_bl, f->qcfrac_bl};
+  const void** dbl[2] = {&df.qc_bl, &df.qcfrac_bl};
+  for (int i = 0; i < 2; ++i) {
+    if (!hbl[i]) continue;
+    DevBuf& b = x->stage[19 + nd + i];
+    HIP_TRY(b.ensure(n3 * sizeof(double)));
+    HIP_TRY(hipMemcpy(b.p, hbl[i], n3 * sizeof(double), hipMemcpyHostToDevice));
+    *dbl[i] = b.p;
+  }
   void** dfp[19] = {&df.t,  &df.th, &df.pii, &df.p,  &df.w,  &df.dz, &df.qv, &df.qc,   &df.qr,   &df.qi,
                     &df.qs, &df.qg, &df.ni,  &df.nr, &df.nc, &df.nwfa, &df.nifa, &df.ng, &df.qb};
   for (int i = 0; i < 19; ++i) {

@@ -220,7 +220,17 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_A k_phase_a(StepArgs<R> a) {
   ice_check(C, L, qi1d, ni1d, dt, odt);
   snow_check(L, qs1d, dt, odt);
   if (L.l_qs) snow_moments(C, L.rs, mn(R(-0.1), L.temp - R(kT0)), 1 | 2 | 4 | 8 | 16, L);
-  const R ncv = column_nc(a, c);
+  // one-moment cloud number (:363-380): the network's prediction from this
+  // point of the step (the final update reads it back), else the column's
+  // constant
+  R ncv;
+  if (a.fl.ml_for_nc && !has_nc) {
+    const R xrx = qc1d <= R(S_t<R>(1.e-12)) ? R(0.0) : qc1d;
+    ncv = ml_cloud_number(xrx, qr1d, qi1d, qs1d, L.pres, L.temp, w) * L.rho;
+    a.sr[SR_NCSAVE * plane + s] = ncv;
+  } else {
+    ncv = column_nc(a, c);
+  }
   L.ncsave = ncv;
   cloud_check(C, L, qc1d, nc1d, has_nc, ncv, dt, odt);
   if ((cf & CF_GINIT) && qg1d > R(kR1)) {  // graupel_init (:1234-1260)
@@ -924,7 +934,7 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
   R ni1d = a.fa[F_NI][o], nr1d = a.fa[F_NR][o];
   R nc1d = has_nc ? a.fa[F_NC][o] : R(0), ng1d = has_ngqb ? a.fa[F_NG][o] : R(0);
   R qb1d = has_ngqb ? a.fa[F_QB][o] : R(0);
-  const R ncv = column_nc(a, c);
+  const R ncv = (a.fl.ml_for_nc && !has_nc) ? sr[SR_NCSAVE * plane + s] : column_nc(a, c);
 
   if (!a.fl.turn_off_micro) {  // freeze_cloud_melt_ice (:2124-2165)
     const R xri = mx(R(0.0), qi1d + L.qiten * dt);
@@ -1033,6 +1043,19 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
   }
   const bool re = a.fl.re_cloud && a.fl.re_ice && a.fl.re_snow;
   R re_c = R(0), re_i = R(0), re_s = R(0);
+  if (re && a.qc_bl && a.qcfrac_bl) {
+    // boundary-layer cloud in the cloud radius (:839-857): in-cloud PBL water
+    // where the resolved cloud is absent, the network's number for the sum,
+    // and the cloud check redone on them (state and earlier diagnostics keep
+    // the resolved cloud)
+    R xrx = qc1d;
+    const R qbl = a.qc_bl[o], fbl = a.qcfrac_bl[o];
+    if (xrx <= R(kR1) && qbl > R(S_t<R>(1.e-9)) && fbl > R(0.0)) xrx = xrx + qbl / fbl;
+    if (xrx <= R(S_t<R>(1.e-12))) xrx = R(0.0);
+    R xnc = ml_cloud_number(xrx, qr1d, qi1d, qs1d, L.pres, L.temp, a.f[F_W][o]);
+    L.qcten = L.ncten = R(0.0);
+    cloud_check(C, L, xrx, xnc, true, ncv, dt, odt);
+  }
   if (re) {  // effective_radius (:13-56)
     if (L.l_qc) re_c = mx(R(2.51e-6), mn(R(0.5) * (R(3.0) + R(get_nuc(L.nc))) * L.ilamc, R(50.e-6)));
     if (L.l_qi) re_i = mx(R(2.51e-6), mn(R(0.5) * (R(3.0) + R(kMuI)) * L.ilami, R(125.e-6)));

@@ -152,6 +152,7 @@ enum ScrR {
   SR_TEMP, SR_QV, SR_RHO, SR_TTEN, SR_QVTEN, SR_NWFATEN, SR_NIFATEN, SR_VTRR, SR_VTNR, SR_VTRG, SR_VTNG, SR_VTS,
   SR_VTRI, SR_VTNI, SR_VTRC, SR_VTNC, SR_HGT,
   SR_RHOF, SR_VISCO,  // density factor and viscosity (written only for semi-Lagrangian sedimentation)
+  SR_NCSAVE,          // one-moment cloud number of the network (written only with ml_for_nc and no nc)
   SR_N
 };
 // scratch of the sedimented state and tendencies (per point, Sd_t<R>)
@@ -188,6 +189,8 @@ struct StepArgs {
   R* d3[D3_N];       // per level diagnostics
   const int* land;   // per column land flag or nullptr
   const int* nlev;   // per column active level count or nullptr
+  const R* qc_bl;      // boundary-layer cloud water and its cloud fraction (state layout), or nullptr
+  const R* qcfrac_bl;
   int ni, nj, nz, ncol;
   int nzs;           // level stride of a scratch column group (scratch_levels(nz))
   long long ldk, ldj, dldk, dldj, d2ldj;
@@ -377,6 +380,49 @@ __device__ void thermo(const Const<S_t<R>>& C, Lvl<R>& L) {
   L.lvap = R(kLvap0) + R(2106.0 - 4218.0) * tempc;
   L.tcond = (R(5.69) + R(0.0168) * tempc) * R(1.0e-5) * R(418.936);
   L.lvt2 = L.lvap * L.lvap * L.ocp * C.orv * otemp * otemp;
+}
+
+// ---- cloud number from the trained network (src/module_mp_tempo_ml.F90) ----
+#include "tempo_ml_weights.inc"
+
+template <class S>
+__device__ __forceinline__ S ml_par(const double* d, const float* f, int i) {
+  if constexpr (sizeof(S) == 4)
+    return f[i];
+  else
+    return d[i];
+}
+
+// tempo_ml_predict_cloud_number (src/module_mp_tempo_ml.F90:104-174) at one
+// level, per kg: standard-scaled inputs (x - mean) / sqrt(var), 24 ReLU
+// nodes, one ReLU output clamped to [-6, 9.3010299957] (log10 of the number),
+// the quintic bias correction on [0, 3), and 10**output.  The products of
+// each layer are summed in input order onto the first, then the bias added,
+// as the reference's matmul and bias arrays do.
+template <class R>
+__device__ R ml_cloud_number(R qc, R qr, R qi, R qs, R pres, R temp, R w) {
+  using S = S_t<R>;
+  const R x[7] = {qc, qr, qi, qs, pres, temp, w};
+  R xt[7];
+#pragma unroll
+  for (int j = 0; j < 7; ++j)
+    xt[j] = (x[j] - R(ml_par<S>(kMlMeanD, kMlMeanF, j))) / msqrt(R(ml_par<S>(kMlVarD, kMlVarF, j)));
+  R out = R(0.0);
+#pragma unroll
+  for (int n = 0; n < 24; ++n) {
+    R h = R(ml_par<S>(kMlW0D, kMlW0F, n)) * xt[0];
+#pragma unroll
+    for (int j = 1; j < 7; ++j) h = h + R(ml_par<S>(kMlW0D, kMlW0F, n + 24 * j)) * xt[j];
+    h = mx(h + R(ml_par<S>(kMlB0D, kMlB0F, n)), R(0.0));
+    out = (n == 0) ? R(ml_par<S>(kMlW1D, kMlW1F, 0)) * h : out + R(ml_par<S>(kMlW1D, kMlW1F, n)) * h;
+  }
+  out = mx(out + R(ml_par<S>(kMlB1D, kMlB1F, 0)), R(0.0));
+  const R p = mn(R(S(9.3010299957)), mx(R(-6.0), out));
+  R corr = R(1.0);
+  if (p >= R(0.0) && p < R(3.0))
+    corr = -R(S(0.2704)) * powi(p, 5) + R(S(1.838)) * powi(p, 4) - R(S(5.127)) * powi(p, 3) +
+           R(S(8.547)) * powi(p, 2) - R(S(8.439)) * p + R(S(4.297));
+  return corr * mpow(R(10.0), p);
 }
 
 // ---- *_check_and_update for one level (src/module_mp_tempo_main.F90:891-1231) ----

@@ -45,6 +45,7 @@ module tempo_hip_mod
     type(c_ptr) :: qs = c_null_ptr, qg = c_null_ptr, ni = c_null_ptr, nr = c_null_ptr, nc = c_null_ptr
     type(c_ptr) :: nwfa = c_null_ptr, nifa = c_null_ptr, ng = c_null_ptr, qb = c_null_ptr
     type(c_ptr) :: land = c_null_ptr
+    type(c_ptr) :: qc_bl = c_null_ptr, qcfrac_bl = c_null_ptr
   end type
 
   type, bind(c) :: tempo_hip_diags
@@ -96,6 +97,11 @@ module tempo_hip_mod
     end function
     type(c_ptr) function tempo_hip_last_error_c() bind(c, name='tempo_hip_last_error')
       import :: c_ptr
+    end function
+    integer(c_int) function tempo_hip_set_run_cfg(ctx, cfg) bind(c, name='tempo_hip_set_run_cfg')
+      import :: c_ptr, c_int, tempo_hip_cfg
+      type(c_ptr), value :: ctx
+      type(tempo_hip_cfg), intent(in) :: cfg
     end function
     subroutine tempo_hip_finalize(ctx) bind(c, name='tempo_hip_finalize')
       import :: c_ptr
@@ -204,6 +210,41 @@ contains
     flag = merge(1_c_int, 0_c_int, l)
   end function
 
+  !> ty_tempo_cfgs -> tempo_hip_cfg, with resolve_nc_value
+  !! (src/module_mp_tempo_cfgs.F90:50-73)
+  subroutine to_c_cfg(tempo_cfgs, c)
+    type(ty_tempo_cfgs), intent(in) :: tempo_cfgs
+    type(tempo_hip_cfg), intent(out) :: c
+    call tempo_hip_cfg_default(c)
+    c%aerosolaware_flag = flag(tempo_cfgs%aerosolaware_flag)
+    c%hailaware_flag = flag(tempo_cfgs%hailaware_flag)
+    c%ml_for_bl_nc_flag = flag(tempo_cfgs%ml_for_bl_nc_flag)
+    c%ml_for_nc_flag = flag(tempo_cfgs%ml_for_nc_flag)
+    c%semi_sedi_flag = flag(tempo_cfgs%semi_sedi_flag)
+    c%refl10cm_from_melting_flag = flag(tempo_cfgs%refl10cm_from_melting_flag)
+    c%turn_off_micro_flag = flag(tempo_cfgs%turn_off_micro_flag)
+    c%cloud_condensation_flag = flag(tempo_cfgs%cloud_condensation_flag)
+    c%verbose = flag(tempo_cfgs%verbose)
+    c%check_tables = flag(tempo_cfgs%check_tables)
+    select case (trim(tempo_cfgs%single_moment_nc_opt))
+    case ('land')
+      c%nc_option = 0
+    case ('ocean')
+      c%nc_option = 1
+    case default
+      c%nc_option = 2
+      read(tempo_cfgs%single_moment_nc_opt, *) c%nc_value
+    end select
+    c%refl10cm_flag = flag(tempo_cfgs%refl10cm_flag)
+    c%re_cloud_flag = flag(tempo_cfgs%re_cloud_flag)
+    c%re_ice_flag = flag(tempo_cfgs%re_ice_flag)
+    c%re_snow_flag = flag(tempo_cfgs%re_snow_flag)
+    c%max_hail_diameter_flag = flag(tempo_cfgs%max_hail_diameter_flag)
+    c%rain_med_vol_diam_flag = flag(tempo_cfgs%rain_med_vol_diam_flag)
+    c%graupel_med_vol_diam_flag = flag(tempo_cfgs%graupel_med_vol_diam_flag)
+    c%cloud_number_mixing_ratio_flag = flag(tempo_cfgs%cloud_number_mixing_ratio_flag)
+  end subroutine to_c_cfg
+
   !> tempo_init (src/module_mp_tempo_driver.F90:42-182): flags, then the
   !! library context (constants, init-time tables, collection/freezing tables
   !! read from tempo_cfgs%table_dir or built on the GPU).  force_init_flag
@@ -232,35 +273,7 @@ contains
     if (present(cloud_condensation_flag)) tempo_cfgs%cloud_condensation_flag = cloud_condensation_flag
     if (present(refl10cm_from_melting_flag)) tempo_cfgs%refl10cm_from_melting_flag = refl10cm_from_melting_flag
 
-    call tempo_hip_cfg_default(c)
-    c%aerosolaware_flag = flag(tempo_cfgs%aerosolaware_flag)
-    c%hailaware_flag = flag(tempo_cfgs%hailaware_flag)
-    c%ml_for_bl_nc_flag = flag(tempo_cfgs%ml_for_bl_nc_flag)
-    c%ml_for_nc_flag = flag(tempo_cfgs%ml_for_nc_flag)
-    c%semi_sedi_flag = flag(tempo_cfgs%semi_sedi_flag)
-    c%refl10cm_from_melting_flag = flag(tempo_cfgs%refl10cm_from_melting_flag)
-    c%turn_off_micro_flag = flag(tempo_cfgs%turn_off_micro_flag)
-    c%cloud_condensation_flag = flag(tempo_cfgs%cloud_condensation_flag)
-    c%verbose = flag(tempo_cfgs%verbose)
-    c%check_tables = flag(tempo_cfgs%check_tables)
-    ! resolve_nc_value (src/module_mp_tempo_cfgs.F90:50-73)
-    select case (trim(tempo_cfgs%single_moment_nc_opt))
-    case ('land')
-      c%nc_option = 0
-    case ('ocean')
-      c%nc_option = 1
-    case default
-      c%nc_option = 2
-      read(tempo_cfgs%single_moment_nc_opt, *) c%nc_value
-    end select
-    c%refl10cm_flag = flag(tempo_cfgs%refl10cm_flag)
-    c%re_cloud_flag = flag(tempo_cfgs%re_cloud_flag)
-    c%re_ice_flag = flag(tempo_cfgs%re_ice_flag)
-    c%re_snow_flag = flag(tempo_cfgs%re_snow_flag)
-    c%max_hail_diameter_flag = flag(tempo_cfgs%max_hail_diameter_flag)
-    c%rain_med_vol_diam_flag = flag(tempo_cfgs%rain_med_vol_diam_flag)
-    c%graupel_med_vol_diam_flag = flag(tempo_cfgs%graupel_med_vol_diam_flag)
-    c%cloud_number_mixing_ratio_flag = flag(tempo_cfgs%cloud_number_mixing_ratio_flag)
+    call to_c_cfg(tempo_cfgs, c)
 
     if (len_trim(tempo_cfgs%ccn_path) > 0) then
       ccn = trim(tempo_cfgs%ccn_path)
@@ -337,13 +350,15 @@ contains
     real(wp), dimension(ims:ime, kms:kme, jms:jme), intent(inout), optional, target :: nc, nwfa, nifa, qb, ng
     integer, dimension(ims:ime, jms:jme), intent(in), optional :: land_input
     real(wp), dimension(ims:ime, kms:kme, jms:jme), intent(inout), optional :: qcfrac, qifrac
-    real(wp), dimension(ims:ime, kms:kme, jms:jme), intent(in), optional :: qc_bl, qcfrac_bl, thten_bl, &
+    real(wp), dimension(ims:ime, kms:kme, jms:jme), intent(in), optional, target :: qc_bl, qcfrac_bl
+    real(wp), dimension(ims:ime, kms:kme, jms:jme), intent(in), optional :: thten_bl, &
       qvten_bl, qcten_bl, qiten_bl, thten_lwrad, thten_swrad
     type(ty_tempo_driver_diags), intent(inout), target :: tempo_diags
 
     type(tempo_hip_fields) :: f
     type(tempo_hip_diags) :: d
     type(tempo_hip_dims) :: dm
+    type(tempo_hip_cfg) :: cc
     integer(c_int), allocatable, target :: land(:)
     integer(c_int) :: rc
     integer :: i, j, n
@@ -356,6 +371,10 @@ contains
       write(*, '(a)') 'tempo_run: need t, or th and pii'
       error stop 1
     endif
+    ! tempo_run reads the run flags of tempo_cfgs on every call
+    call to_c_cfg(tempo_cfgs, cc)
+    rc = tempo_hip_set_run_cfg(tempo_cfgs%ctx, cc)
+    if (rc /= 0) call fail('tempo_run', rc)
     call allocate_diags(tempo_cfgs, tempo_diags, ims, ime, jms, jme, kms, kme)
 
     dm%ni = ite - its + 1
@@ -386,6 +405,10 @@ contains
     if (present(nifa)) f%nifa = c_loc(nifa(its, kts, jts))
     if (present(ng)) f%ng = c_loc(ng(its, kts, jts))
     if (present(qb)) f%qb = c_loc(qb(its, kts, jts))
+    if (present(qc_bl) .and. present(qcfrac_bl)) then
+      f%qc_bl = c_loc(qc_bl(its, kts, jts))
+      f%qcfrac_bl = c_loc(qcfrac_bl(its, kts, jts))
+    endif
     if (present(land_input)) then
       allocate(land(dm%ni * dm%nj))
       n = 0

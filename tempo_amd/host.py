@@ -48,10 +48,14 @@ class Cfg(ctypes.Structure):
 
 FIELD_NAMES = ("t", "th", "pii", "p", "w", "dz", "qv", "qc", "qr", "qi", "qs", "qg", "ni", "nr",
                "nc", "nwfa", "nifa", "ng", "qb")
+# boundary-layer cloud inputs (read only; with the cloud-number network they
+# enter the cloud effective radius)
+BL_NAMES = ("qc_bl", "qcfrac_bl")
 
 
 class Fields(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_void_p) for n in FIELD_NAMES] + [("land", ctypes.c_void_p)]
+    _fields_ = [(n, ctypes.c_void_p) for n in FIELD_NAMES] + [("land", ctypes.c_void_p)] + \
+        [(n, ctypes.c_void_p) for n in BL_NAMES]
 
 
 DIAG2_NAMES = ("rain_precip", "ice_liquid_equiv_precip", "snow_liquid_equiv_precip",
@@ -108,6 +112,7 @@ def lib():
                                                           ctypes.c_int, ctypes.c_int, vp]
         L.tempo_hip_aerosol_surface_emissions_host.argtypes = [vp, ctypes.c_double, vp, vp, ctypes.POINTER(Dims),
                                                                ctypes.c_int]
+        L.tempo_hip_set_run_cfg.argtypes = [vp, ctypes.POINTER(Cfg)]
         L.tempo_hip_last_error.restype = ctypes.c_char_p
         L.tempo_hip_finalize.argtypes = [vp]
         L.tempo_hip_finalize.restype = None
@@ -147,6 +152,19 @@ class TempoHip:
                                     ctypes.byref(ctx)))
         self._ctx = ctx
 
+    def set_run_cfg(self, **changes):
+        """Change run-time flags for later steps (a host editing its
+        ty_tempo_cfgs between tempo_run calls); aerosolaware_flag and
+        hailaware_flag stay as initialised."""
+        c = Cfg()
+        ctypes.memmove(ctypes.byref(c), ctypes.byref(self.cfg), ctypes.sizeof(Cfg))
+        for k, v in changes.items():
+            if not hasattr(c, k):
+                raise TypeError(f"unknown option {k}")
+            setattr(c, k, int(v) if k != "nc_value" else float(v))
+        _check(lib().tempo_hip_set_run_cfg(self._ctx, ctypes.byref(c)))
+        self.cfg = c
+
     def close(self):
         if self._ctx:
             lib().tempo_hip_finalize(self._ctx)
@@ -180,7 +198,7 @@ class TempoHip:
             raise TypeError("fields must be float32 or float64")
         f = Fields()
         keep = []
-        for name in FIELD_NAMES:
+        for name in FIELD_NAMES + BL_NAMES:
             a = state.get(name)
             if a is None:
                 continue
@@ -223,7 +241,7 @@ class TempoHip:
     # ---- shared argument marshalling for device tensors ----
     def _fields(self, state, nz, ncol, dtype, what="field"):
         f = Fields()
-        for name in FIELD_NAMES:
+        for name in FIELD_NAMES + BL_NAMES:
             a = state.get(name) if state else None
             if a is None:
                 continue
@@ -315,7 +333,7 @@ class TempoHip:
         """tempo_run on host float64 numpy arrays (staged through device memory)."""
         nz, ncol = state["p"].shape
         f = Fields()
-        for name in FIELD_NAMES:
+        for name in FIELD_NAMES + BL_NAMES:
             a = state.get(name)
             if a is not None:
                 assert a.dtype == np.float64 and a.flags.c_contiguous and a.shape == (nz, ncol), name
@@ -403,10 +421,10 @@ def tempo_init(aerosolaware_flag=None, hailaware_flag=None, semi_sedi_flag=None,
 
 def tempo_run(ctx, dt, itimestep, t=None, th=None, pii=None, p=None, w=None, dz=None, qv=None, qc=None, qr=None,
               qi=None, qs=None, qg=None, ni=None, nr=None, nc=None, nwfa=None, nifa=None, ng=None, qb=None,
-              land_input=None, nlev=None):
+              land_input=None, nlev=None, qc_bl=None, qcfrac_bl=None):
     """tempo_run (src/module_mp_tempo_driver.F90:187-513) on (nz, ncol) device tensors."""
     state = dict(t=t, th=th, pii=pii, p=p, w=w, dz=dz, qv=qv, qc=qc, qr=qr, qi=qi, qs=qs, qg=qg, ni=ni, nr=nr,
-                 nc=nc, nwfa=nwfa, nifa=nifa, ng=ng, qb=qb)
+                 nc=nc, nwfa=nwfa, nifa=nifa, ng=ng, qb=qb, qc_bl=qc_bl, qcfrac_bl=qcfrac_bl)
     return ctx.run({k: v for k, v in state.items() if v is not None}, dt, itimestep, nlev=nlev, land=land_input)
 
 

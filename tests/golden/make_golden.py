@@ -74,10 +74,15 @@ def run_case(name, state, dt, itimestep, flags, nlev=None, land=None, steps=1):
     chk = {k: v.copy() for k, v in state.items()}
     with orc.Trace() as tr:
         for s in range(steps):
-            orc.run(chk, dt, itimestep + s, land=land, nlev=nlev)
+            c2, c3 = orc.run(chk, dt, itimestep + s, land=land, nlev=nlev)
     for k in chk:
         if not np.array_equal(chk[k], out[k]):
             raise SystemExit(f"{name}: C restatement differs from the reference on {k}")
+    if CHECK_DIAGS:
+        for k in CHECK_DIAGS:
+            got, want = (c2[k], d2[k]) if k in c2 else (c3[k], d3[k])
+            if not np.array_equal(got, want):
+                raise SystemExit(f"{name}: C restatement differs from the reference on diagnostic {k}")
     meta = dict(name=name, dt=dt, itimestep=itimestep, steps=steps, flags=flags)
     arrays = {f"in_{k}": v for k, v in inp.items()}
     arrays.update({f"out_{k}": v for k, v in out.items()})
@@ -100,10 +105,12 @@ def pick(state, idx):
 
 
 ONLY = ()
+# diagnostics the C restatement must reproduce bit for bit in the next cases
+CHECK_DIAGS = ()
 
 
 def main():
-    global TABLES, ONLY
+    global TABLES, ONLY, CHECK_DIAGS
     if len(sys.argv) < 2:
         raise SystemExit("usage: make_golden.py TABLE_DIR [CASE ...]  (tables written by `python -m oracle.ref_tables`;"
                          " with CASE names only those fixtures are rewritten)")
@@ -179,6 +186,33 @@ def main():
     run_case("semi_multistep", st, m["dt"], m["itimestep"], dict(BOTH, semi_sedi=True), steps=2)
     st, m = gen.generate("plain", ncol=16, col0=900)
     run_case("semi_plain_dt300", st, 300.0, 2, dict(PLAIN, semi_sedi=True))
+    # cloud number from the trained network: one-moment cloud (ml_for_nc,
+    # test/tests.F90:380-461) and boundary-layer cloud in the cloud radius
+    # (ml_for_bl_nc with qc_bl/qcfrac_bl, test/tests.F90:464-558), on the
+    # reference's column and on generated columns
+    CHECK_DIAGS = ("re_cloud", "nc_mixing_ratio")
+    one = {k: v for k, v in full.items() if k != "nc"}
+    run_case("ml_nc_mpas_micro_off", one, 20.0, 1, dict(PLAIN, turn_off_micro=True, nc_mixing_ratio=True,
+                                                         ml_for_nc=True), steps=3)
+    run_case("ml_nc_mpas", one, 20.0, 1, dict(PLAIN, nc_mixing_ratio=True, ml_for_nc=True), steps=3)
+    st, m = gen.generate("plain", ncol=16, col0=1300)
+    run_case("ml_nc_plain", st, m["dt"], m["itimestep"], dict(PLAIN, nc_mixing_ratio=True, ml_for_nc=True),
+             steps=2)
+    qc_bl = np.zeros(59)
+    qc_bl[:19] = d["qc"][15:34] * 0.2
+    frac = np.where(qc_bl > 0.0, 0.35, 0.0)
+    bl = dict(full, qc_bl=col(qc_bl), qcfrac_bl=col(frac))
+    run_case("ml_bl_mpas_micro_off", bl, 20.0, 1, dict(PLAIN, turn_off_micro=True, nc_mixing_ratio=True,
+                                                       ml_for_bl_nc=True), steps=3)
+    run_case("ml_bl_mpas", bl, 20.0, 1, dict(BOTH, nc_mixing_ratio=True, ml_for_bl_nc=True), steps=2)
+    st, m = gen.generate("aerosol", ncol=16, col0=700)
+    rng = np.random.default_rng(11)
+    nz, ncol = st["p"].shape
+    qbl = np.where(rng.random((nz, ncol)) < 0.4, rng.uniform(1e-8, 1e-3, (nz, ncol)), 0.0)
+    st["qc_bl"] = np.ascontiguousarray(qbl)
+    st["qcfrac_bl"] = np.ascontiguousarray(np.where(qbl > 0.0, rng.uniform(0.05, 1.0, (nz, ncol)), 0.0))
+    run_case("ml_bl_aerosol", st, m["dt"], m["itimestep"], dict(AERO, ml_for_bl_nc=True), steps=2)
+    CHECK_DIAGS = ()
     # tempo_aerosol_surface_emissions (src/module_mp_tempo_driver.F90:516-529)
     if not ONLY or "surface_emissions" in ONLY:
         rng = np.random.default_rng(77)

@@ -36,7 +36,8 @@ FLAG_TO_CFG = dict(aerosolaware="aerosolaware_flag", hailaware="hailaware_flag",
                    cloud_condensation="cloud_condensation_flag", refl10cm="refl10cm_flag", re_cloud="re_cloud_flag",
                    re_ice="re_ice_flag", re_snow="re_snow_flag", max_hail="max_hail_diameter_flag",
                    mvd_r="rain_med_vol_diam_flag", mvd_g="graupel_med_vol_diam_flag",
-                   nc_mixing_ratio="cloud_number_mixing_ratio_flag", semi_sedi="semi_sedi_flag")
+                   nc_mixing_ratio="cloud_number_mixing_ratio_flag", semi_sedi="semi_sedi_flag",
+                   ml_for_nc="ml_for_nc_flag", ml_for_bl_nc="ml_for_bl_nc_flag")
 
 
 def cfg_of(flags):
@@ -273,6 +274,86 @@ def test_gpu_fp32_within_tolerance(hip, big_case, gpu_tables, semi):
     assert prel[0] <= 1e-5, prel
 
 
+def ml_columns(ncol, col0, with_bl):
+    """generated columns for the cloud-number network: one-moment cloud (no
+    nc) or two-moment cloud with boundary-layer cloud water and fraction"""
+    from tempo_amd import generator as gen
+    if not with_bl:
+        return gen.generate("plain", ncol=ncol, col0=col0)
+    st, m = gen.generate("aerosol", ncol=ncol, col0=col0)
+    rng = np.random.default_rng(col0)
+    nz = st["p"].shape[0]
+    qbl = np.where(rng.random((nz, ncol)) < 0.4, rng.uniform(1e-8, 1e-3, (nz, ncol)), 0.0)
+    st["qc_bl"] = np.ascontiguousarray(qbl)
+    st["qcfrac_bl"] = np.ascontiguousarray(np.where(qbl > 0.0, rng.uniform(0.05, 1.0, (nz, ncol)), 0.0))
+    return st, m
+
+
+ML_CASES = {"ml_for_nc": (False, dict(aerosolaware=False, hailaware=False, nc_mixing_ratio=True, ml_for_nc=True)),
+            "ml_for_bl_nc": (True, dict(aerosolaware=True, hailaware=False, nc_mixing_ratio=True,
+                                        ml_for_bl_nc=True))}
+
+
+@pytest.mark.parametrize("which", sorted(ML_CASES))
+def test_gpu_ml_cloud_number_matches_oracle(hip, gpu_tables, which):
+    """Cloud number from the trained network (src/module_mp_tempo_ml.F90):
+    the one-moment cloud's number at every level (ml_for_nc) and the
+    boundary-layer cloud in the cloud radius (ml_for_bl_nc with qc_bl,
+    qcfrac_bl), FP64 path vs the oracle on 512 columns, two steps; and the
+    FP32 path vs the FP64 twin on the cloud diagnostics the network feeds."""
+    from oracle import oracle as orc
+    with_bl, flags = ML_CASES[which]
+    st, m = ml_columns(512, 4000, with_bl)
+    orc.init(gpu_tables, **flags)
+    ref = {k: v.copy() for k, v in st.items()}
+    for s in range(2):
+        d2, d3 = orc.run(ref, m["dt"], m["itimestep"] + s)
+    ctx = context(hip, flags)
+    out, d = gpu_run(ctx, st, m["dt"], m["itimestep"], steps=2)
+    for k in ref:
+        assert rel_inf(out[k], ref[k]) <= 1e-12, (k, rel_inf(out[k], ref[k]))
+    for k, v in d3.items():
+        assert rel_inf(d[D3_MAP[k]], v) <= 1e-12, (k, rel_inf(d[D3_MAP[k]], v))
+    assert (d["re_cloud"] > 0).sum() > 100
+    # FP32: same FP32-representable inputs, one step, vs the FP64 twin.  Where
+    # cloud freezes completely, the new cloud water is a rounding residual
+    # near the r1 = 1e-12 presence threshold in single precision (the
+    # reference's own -DSINGLE_PREC build leaves such residuals too), and the
+    # cloud radius follows that presence; radius flips at those points are
+    # the state's and are counted apart.
+    inp = {k: v.astype(np.float32).astype(np.float64) for k, v in st.items()}
+    o64, e64 = gpu_run(ctx, inp, m["dt"], m["itimestep"], dtype=torch.float64)
+    o32, e32 = gpu_run(ctx, inp, m["dt"], m["itimestep"], dtype=torch.float32)
+    qc_flip = (o32["qc"] > 1e-12) != (o64["qc"] > 1e-12)
+    for k in ("re_cloud", "cloud_number_mixing_ratio"):
+        flip = flipped(e32[k], e64[k])
+        keep = ~flip
+        err = np.linalg.norm((e32[k] - e64[k])[keep]) / max(np.linalg.norm(e64[k][keep]), 1e-300)
+        other = float((flip & ~qc_flip).mean())
+        print(which, k, "FP32 rel L2", err, "flipped", float(flip.mean()), "of which not at a cloud-presence flip",
+              other, "cloud-presence flips", float(qc_flip.mean()))
+        assert err <= 1e-4, (k, err)
+        assert other <= 1e-3, (k, other)
+
+
+def test_gpu_run_cfg_changes_between_steps(hip):
+    """Flags changed after init (tempo_hip_set_run_cfg) act like a context
+    initialised with them, as a host editing ty_tempo_cfgs between calls."""
+    flags = dict(aerosolaware=False, hailaware=False)
+    st, m = ml_columns(64, 900, False)
+    want, dw = gpu_run(context(hip, dict(flags, ml_for_nc=True, nc_mixing_ratio=True)), st, m["dt"], m["itimestep"])
+    ctx = hip.TempoHip(aerosolaware_flag=0, hailaware_flag=0)
+    ctx.set_run_cfg(ml_for_nc_flag=1, cloud_number_mixing_ratio_flag=1)
+    got, dg = gpu_run(ctx, st, m["dt"], m["itimestep"])
+    for k in want:
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+    np.testing.assert_array_equal(dg["cloud_number_mixing_ratio"], dw["cloud_number_mixing_ratio"])
+    with pytest.raises(hip.TempoError) as e:
+        ctx.set_run_cfg(hailaware_flag=1)
+    assert e.value.code == hip.E_ARG
+    ctx.close()
+
+
 def test_gpu_ragged_matches_oracle(hip, gpu_tables):
     from oracle import oracle as orc
     from tempo_amd import generator as gen
@@ -336,6 +417,16 @@ def test_gpu_error_codes(hip):
         semi.tlm(dict(dev), {k: torch.zeros_like(v) for k, v in dev.items()}, m["dt"], 2)
     assert e.value.code == hip.E_UNSUPPORTED
     semi.close()
+    # boundary-layer cloud without the network; derivatives with the network
+    bl = dict(dev, qc_bl=torch.zeros_like(dev["qc"]), qcfrac_bl=torch.zeros_like(dev["qc"]))
+    with pytest.raises(hip.TempoError) as e:
+        ctx.run(bl, m["dt"], 2)
+    assert e.value.code == hip.E_ARG
+    ml = hip.TempoHip(aerosolaware_flag=0, hailaware_flag=0, ml_for_nc_flag=1)
+    with pytest.raises(hip.TempoError) as e:
+        ml.tlm(dict(dev), {k: torch.zeros_like(v) for k, v in dev.items()}, m["dt"], 2)
+    assert e.value.code == hip.E_UNSUPPORTED
+    ml.close()
     with tempfile.TemporaryDirectory() as d:
         with pytest.raises(hip.TempoError) as e:
             hip.TempoHip(table_dir=d)
