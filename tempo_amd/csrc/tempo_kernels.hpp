@@ -4,13 +4,30 @@
 
 #include "tempo_step.hpp"
 
+namespace tempo {
+
+// Waves per SIMD the level-local kernels are compiled for, per working type.
+// The FP32 level physics needs 130 VGPRs unconstrained (3 waves per SIMD);
+// held to 128 it runs 4 waves with a few bytes more of private memory and
+// is measured faster.  Tighter budgets spill heavily and are slower; the
+// FP64 and dual instantiations keep the compiler's choice.
+template <class R>
+struct WavesPerEu {
+  static constexpr int a = 1, c = 1;
+};
+template <>
+struct WavesPerEu<float> {
+  static constexpr int a = 4, c = 1;
+};
+}  // namespace tempo
+
 // register-budget attributes of the level-local kernels (occupancy against
-// spills), chosen at build time
+// spills); a build may override them
 #ifndef TEMPO_ATTR_A
-#define TEMPO_ATTR_A
+#define TEMPO_ATTR_A __attribute__((amdgpu_waves_per_eu(WavesPerEu<R>::a)))
 #endif
 #ifndef TEMPO_ATTR_C
-#define TEMPO_ATTR_C
+#define TEMPO_ATTR_C __attribute__((amdgpu_waves_per_eu(WavesPerEu<R>::c)))
 #endif
 
 namespace tempo {

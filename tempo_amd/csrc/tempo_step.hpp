@@ -1254,24 +1254,38 @@ __device__ void sum_tendencies(const Const<S_t<R>>& C, Lvl<R>& L, const Tend<R>&
   }
 }
 
+// the activation table's aerosol-number and vertical-velocity breakpoints
+// (ta_na, ta_ww, src/module_mp_tempo_params.F90:266-270) by index, as
+// selects rather than a private array
+template <class R>
+__device__ __forceinline__ R ccn_na(int i) {
+  return i == 0 ? R(10.) : i == 1 ? R(31.6) : i == 2 ? R(100.) : i == 3 ? R(316.) : i == 4 ? R(1000.)
+       : i == 5 ? R(3160.) : R(10000.);
+}
+template <class R>
+__device__ __forceinline__ R ccn_ww(int i) {
+  return i == 0 ? R(0.01) : i == 1 ? R(0.0316) : i == 2 ? R(0.1) : i == 3 ? R(0.316) : i == 4 ? R(1.)
+       : i == 5 ? R(3.16) : i == 6 ? R(10.) : i == 7 ? R(31.6) : R(100.);
+}
+
 // activate_cloud_number (:2198-2276), land never passed by the caller
 template <class R>
 __device__ R activate_cloud_number(const Tables<S_t<R>>& T, R temp, R w1d, R nwfa) {
-  const R na[7] = {R(10.), R(31.6), R(100.), R(316.), R(1000.), R(3160.), R(10000.)};
-  const R ww[9] = {R(0.01), R(0.0316), R(0.1), R(0.316), R(1.), R(3.16), R(10.), R(31.6), R(100.)};
+  auto na = [](int i) { return ccn_na<R>(i); };
+  auto ww = [](int i) { return ccn_ww<R>(i); };
   R n_local = nwfa * R(1.e-6);
-  if (n_local >= na[6]) n_local = na[6] - R(1.0);
-  else if (n_local <= na[0]) n_local = na[0] + R(1.0);
+  if (n_local >= na(6)) n_local = na(6) - R(1.0);
+  else if (n_local <= na(0)) n_local = na(0) + R(1.0);
   int i = 7;
   for (int n = 2; n <= 7; ++n)
-    if (n_local >= na[n - 2] && n_local < na[n - 1]) { i = n; break; }
+    if (n_local >= na(n - 2) && n_local < na(n - 1)) { i = n; break; }
   R w_local = w1d;
-  if (w_local >= ww[8]) w_local = ww[8] - R(1.0);
-  else if (w_local <= ww[0]) w_local = ww[0] + R(0.001);
+  if (w_local >= ww(8)) w_local = ww(8) - R(1.0);
+  else if (w_local <= ww(0)) w_local = ww(0) + R(0.001);
   int j = 9;
   for (int n = 2; n <= 9; ++n)
-    if (w_local >= ww[n - 2] && w_local < ww[n - 1]) { j = n; break; }
-  const R x1 = mlog(na[i - 2]), x2 = mlog(na[i - 1]), y1 = mlog(ww[j - 2]), y2 = mlog(ww[j - 1]);
+    if (w_local >= ww(n - 2) && w_local < ww(n - 1)) { j = n; break; }
+  const R x1 = mlog(na(i - 2)), x2 = mlog(na(i - 1)), y1 = mlog(ww(j - 2)), y2 = mlog(ww(j - 1));
   const int k = max(1, min(nint_exact((temp - R(243.15)) * R(0.1)) + 1, kNtbArt));
   auto at = [&](int ii, int jj) { return R(T.ccn[(ii - 1) + 7 * ((jj - 1) + 9 * (k - 1))]); };
   const R a = at(i - 1, j - 1), b = at(i, j - 1), c = at(i, j), d = at(i - 1, j);
