@@ -47,7 +47,7 @@ typedef struct {
   int ml_for_bl_nc_flag;          /* cloud-number network loaded (for qc_bl/qcfrac_bl) */
   int ml_for_nc_flag;             /* one-moment cloud number from the network (nc absent) */
   int semi_sedi_flag;             /* semi-Lagrangian rain and graupel sedimentation (tempo_hip_run only) */
-  int refl10cm_from_melting_flag; /* not supported (must be 0) */
+  int refl10cm_from_melting_flag; /* reflectivity of melting snow and graupel below the melting level */
   int turn_off_micro_flag;
   int cloud_condensation_flag;
   int verbose;

@@ -5,6 +5,7 @@
  * FP64 build at -O0. */
 #include "tempo_oracle.h"
 
+#include <complex.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -122,6 +123,7 @@ typedef struct {
     double nic1;
     double dc[100], dtc[100], di[100], dti[100], dr[100], dtr[100], ds[100], dts[100];
     double dg[100], dtg[100], t_nc[100], hbins[50], dhbins[50];
+    double sbins_r[50], dsbins_r[50], gbins_r[50], dgbins_r[50]; /* radar bins */
 } params_t;
 
 static params_t P;   /* run-time constants (hail flag of the run) */
@@ -298,6 +300,9 @@ static void init_params(params_t *p, int hail)
     p->nic1 = log(p->t_nc[99] / p->t_nc[0]);
     /* initialize_bins_for_hail_size (:661-671) */
     create_bins(50, 500.e-6, 0.075, p->hbins, p->dhbins);
+    /* initialize_bins_for_radar, src/module_mp_tempo_params.F90:674-692 */
+    create_bins(50, 100.e-6, 0.02, p->sbins_r, p->dsbins_r);
+    create_bins(50, 100.e-6, 0.05, p->gbins_r, p->dgbins_r);
 }
 
 /* ---- init-time tables: src/module_mp_tempo_utils.F90:349-527 ---- */
@@ -524,7 +529,7 @@ int oracle_init(const int *flags, const float *ccn, const double *fz, const doub
                 const double *qg)
 {
     memcpy(g_flags, flags, sizeof(g_flags));
-    if (flags[3]) return -1;
+
     init_bin_literals();
     init_params(&P, flags[1] != 0);
     init_params(&PT, 1);
@@ -2183,6 +2188,171 @@ static int any(const int *l, int nz)
 }
 
 /* tempo_main, src/module_mp_tempo_main.F90:73-864 */
+/* ---- melting-layer reflectivity, src/module_mp_tempo_diags.F90:124-395 ----
+ * Complex arithmetic as the reference's flang build performs it: products
+ * (ac - bd, ad + bc), quotients by the textbook formula over |b|^2, and
+ * libm csqrt, clog and cabs. */
+static double complex cx_mul(double complex a, double complex b)
+{
+    return CMPLX(creal(a) * creal(b) - cimag(a) * cimag(b), creal(a) * cimag(b) + cimag(a) * creal(b));
+}
+static double complex cx_div(double complex a, double complex b)
+{
+    const double d = creal(b) * creal(b) + cimag(b) * cimag(b);
+    return CMPLX((creal(a) * creal(b) + cimag(a) * cimag(b)) / d, (cimag(a) * creal(b) - creal(a) * cimag(b)) / d);
+}
+static double complex cx_real(double x) { return CMPLX(x, 0.0); }
+
+/* find_melting_level, :124-141 (0-based result: the 1-based k_melt) */
+static int find_melting_level(const col_t *c)
+{
+    int k_melt = 1;
+    for (int k = c->nz - 2; k >= 0; --k) {
+        if (c->temp[k] > t0 && c->l_qr[k] && (c->l_qs[k + 1] || c->l_qg[k + 1])) {
+            k_melt = imax(k + 2, k_melt);
+            break;
+        }
+    }
+    return k_melt;
+}
+
+/* complex_water_ray, :144-172 */
+static double complex complex_water_ray(double lambda, double t)
+{
+    const double epsinf = 5.27137 + 0.02164740 * t - 0.00131198 * t * t;
+    const double epss = 78.54 * (1.0 - 4.579e-3 * (t - 25.0) + 1.190e-5 * (t - 25.0) * (t - 25.0) -
+                                 2.800e-8 * (t - 25.0) * (t - 25.0) * (t - 25.0));
+    const double alpha = -16.8129 / (t + 273.16) + 0.0609265;
+    const double lambdas = 0.00033836 * exp(2513.98 / (t + 273.16)) * 1e-2;
+    const double nenner = 1.0 + 2.0 * pow(lambdas / lambda, 1.0 - alpha) * sin(alpha * pi * 0.5) +
+                          pow(lambdas / lambda, 2.0 - 2.0 * alpha);
+    const double epsr = epsinf + ((epss - epsinf) * (pow(lambdas / lambda, 1.0 - alpha) * sin(alpha * pi * 0.5) + 1.0)) /
+                                     nenner;
+    const double epsi = ((epss - epsinf) * (pow(lambdas / lambda, 1.0 - alpha) * cos(alpha * pi * 0.5) + 0.0)) / nenner +
+                        lambda * 1.25664 / 1.88496;
+    return csqrt(CMPLX(epsr, -epsi));
+}
+
+/* complex_ice_maetzler, :175-201 */
+static double complex complex_ice_maetzler(double lambda, double t)
+{
+    const double c = 2.99e8, tk = t + 273.16, f = c / lambda * 1e-9;
+    const double b1 = 0.0207, b2 = 1.16e-11, b = 335.0;
+    const double deltabeta = exp(-10.02 + 0.0364 * (tk - 273.16));
+    const double betam = (b1 / tk) * (exp(b / tk) / powi(exp(b / tk) - 1.0, 2)) + b2 * f * f;
+    const double beta = betam + deltabeta;
+    const double theta = 300.0 / tk - 1.0;
+    const double alfa = (0.00504 + 0.0062 * theta) * exp(-22.1 * theta);
+    double complex ri = cx_real(3.1884 + 9.1e-4 * (tk - 273.16));
+    ri = ri + CMPLX(0.0, alfa / f + beta * f);
+    return csqrt(conj(ri));
+}
+
+/* one Maxwell-Garnett mixing step (:355-363, 372-380) */
+static double complex mg_mix(double complex m1t, double complex m2t, double complex m3t, double vol2, double vol3)
+{
+    const double complex beta2 = cx_mul(cx_div(cx_mul(cx_real(2.0), m1t), m2t - m1t),
+                                        cx_mul(cx_div(m2t, m2t - m1t), clog(cx_div(m2t, m1t))) - 1.0);
+    const double complex beta3 = cx_mul(cx_div(cx_mul(cx_real(2.0), m1t), m3t - m1t),
+                                        cx_mul(cx_div(m3t, m3t - m1t), clog(cx_div(m3t, m1t))) - 1.0);
+    const double complex num = cx_mul(cx_real(1.0 - vol2 - vol3), m1t) + cx_mul(cx_mul(cx_real(vol2), beta2), m2t) +
+                               cx_mul(cx_mul(cx_real(vol3), beta3), m3t);
+    const double complex den = cx_real(1.0 - vol2 - vol3) + cx_mul(cx_real(vol2), beta2) + cx_mul(cx_real(vol3), beta3);
+    return csqrt(cx_div(num, den));
+}
+
+/* rayleigh_soak_wetgraupel, :309-395 */
+static double rayleigh_soak_wetgraupel(double x_g, double a_geo, double b_geo, double fmelt, double lambda_radar,
+                                       double meltratio_outside, double complex m_w, double complex m_i)
+{
+    const double complex m_air = CMPLX(1.0, 0.0);
+    const double fm = dmax(dmin(fmelt, 1.0), 0.0);
+    double mra = dmax(dmin(meltratio_outside, 1.0), 0.0);
+    mra = mra + (1.0 - mra) * fm;
+    const double x_w = x_g * fm;
+    const double d_g = a_geo * pow(x_g, b_geo);
+    if (!(d_g >= r1)) return 0.0;
+    double vg = pi / 6.0 * powi(d_g, 3);
+    const double rhog = dmax(dmin(x_g / vg, 900.0), 10.0);
+    vg = x_g / rhog;
+    const double grenz = 1.0 - rhog / 1000.0;
+    double volg;
+    if (mra <= grenz) {
+        volg = vg * (1.0 - mra * fm);
+    } else {
+        const double fmgrenz = (900.0 - rhog) / (mra * 900.0 - rhog + 900.0 * rhog / 1000.0);
+        if (fm <= fmgrenz) volg = (1.0 - mra * fm) * vg;
+        else volg = (x_g - x_w) / 900.0 + x_w / 1000.0;
+    }
+    const double d_large = pow(6.0 / pi * volg, 1.0 / 3.0);
+    const double volice = (x_g - x_w) / (volg * 900.0);
+    const double volwater = x_w / (1000.0 * volg);
+    const double volair = 1.0 - volice - volwater;
+    const double volmix1 = volice / dmax(volice + volwater, 1e-10);
+    const double complex mt = mg_mix(cx_mul(m_w, m_w), cx_mul(m_air, m_air), cx_mul(m_i, m_i), 0.0, volmix1);
+    const double complex two_air = cx_mul(cx_real(2.0), m_air);
+    const double complex m_core = mg_mix(cx_mul(mt, mt), cx_mul(m_air, m_air), cx_mul(two_air, two_air), volair, 0.0);
+    const double complex mc2 = cx_mul(m_core, m_core);
+    const double q = cabs(cx_div(mc2 - 1.0, mc2 + 2.0));
+    return powi(q, 2) * pi * pi * pi * pi * pi * powi(d_large, 6) /
+           (lambda_radar * lambda_radar * lambda_radar * lambda_radar);
+}
+
+/* Simpson weights of the radar bins (:226-233, 268-275) */
+static double simpson_weight(int n) /* 1-based bin */
+{
+    double w = 0.0;
+    for (int m = 1; m <= 49; m += 2) {
+        if (n == m) w += 1.0 / 3.0;
+        if (n == m + 1) w += 4.0 / 3.0;
+        if (n == m + 2) w += 1.0 / 3.0;
+    }
+    return w;
+}
+
+/* reflectivity_from_melting_graupel, :204-245 */
+static double melting_graupel(const params_t *p, double rg, double ng, double ilamg, int idx, double rr)
+{
+    const double lambda_radar = 0.10;
+    const double complex m_w_0 = complex_water_ray(lambda_radar, 0.0), m_i_0 = complex_ice_maetzler(lambda_radar, 0.0);
+    const double complex mw2 = cx_mul(m_w_0, m_w_0);
+    const double k_w = powi(cabs(cx_div(mw2 - 1.0, mw2 + 2.0)), 2);
+    const double sr = dmax(0.01, dmin(1.0 - rg / dmax(rg + rr, r1), 0.99));
+    const double fmelt = sr * sr;
+    double eta = 0.0;
+    const double lamg = 1.0 / ilamg;
+    const double n0_g = ng * p->ogg2 * pow(lamg, CGE(2, 1));
+    for (int n = 1; n <= 50; ++n) {
+        const double mass = p->am_g[idx - 1] * pow(p->gbins_r[n - 1], bm_g);
+        const double bsc = rayleigh_soak_wetgraupel(mass, p->ocmg[idx - 1], p->obmg, fmelt, lambda_radar, 0.9, m_w_0, m_i_0);
+        const double f_d = n0_g * pow(p->gbins_r[n - 1], mu_g) * exp(-lamg * p->gbins_r[n - 1]);
+        eta = eta + f_d * bsc * simpson_weight(n) * p->dgbins_r[n - 1];
+    }
+    return lambda_radar * lambda_radar * lambda_radar * lambda_radar / (pi * pi * pi * pi * pi * k_w) * eta;
+}
+
+/* reflectivity_from_melting_snow, :248-300 */
+static double melting_snow(const params_t *p, double rs, double smob, double smoc, double rr)
+{
+    const double lambda_radar = 0.10;
+    const double complex m_w_0 = complex_water_ray(lambda_radar, 0.0), m_i_0 = complex_ice_maetzler(lambda_radar, 0.0);
+    const double complex mw2 = cx_mul(m_w_0, m_w_0);
+    const double k_w = powi(cabs(cx_div(mw2 - 1.0, mw2 + 2.0)), 2);
+    const double sr = dmax(0.01, dmin(1.0 - rs / (rs + rr), 0.99));
+    const double fmelt = sr * sr;
+    double eta = 0.0;
+    const double om3 = 1.0 / smoc, m0 = smob * om3, mrat = smob * m0 * m0 * m0;
+    const double slam1 = m0 * lam0, slam2 = m0 * lam1;
+    for (int n = 1; n <= 50; ++n) {
+        const double mass = am_s * pow(p->sbins_r[n - 1], bm_s);
+        const double bsc = rayleigh_soak_wetgraupel(mass, p->ocms, p->obms, fmelt, lambda_radar, 0.9, m_w_0, m_i_0);
+        const double f_d = mrat * (kap0 * exp(-slam1 * p->sbins_r[n - 1]) +
+                                   kap1 * pow(m0 * p->sbins_r[n - 1], mu_s) * exp(-slam2 * p->sbins_r[n - 1]));
+        eta = eta + f_d * bsc * simpson_weight(n) * p->dsbins_r[n - 1];
+    }
+    return lambda_radar * lambda_radar * lambda_radar * lambda_radar / (pi * pi * pi * pi * pi * k_w) * eta;
+}
+
 /* tempo_ml_predict_cloud_number, src/module_mp_tempo_ml.F90:104-174, on
  * the column: inputs (qc, qr, qi, qs, pres, temp, w) scaled by
  * (x - mean) / sqrt(var) (:177-187), first layer = matmul(weights00, x) +
@@ -2482,20 +2652,25 @@ static void tempo_main(col_t *c)
         }
     }
     if (F[6]) { /* reflectivity_10cm, src/module_mp_tempo_diags.F90:59-121 */
+        const int k_melt = F[3] ? find_melting_level(c) : 1;
         for (int k = nz - 1; k >= 0; --k) {
+            const int melt = F[3] && k_melt > 2 && k + 1 < k_melt - 1;
             double ze_rain = 1.e-22, ze_snow = 1.e-22, ze_graupel = 1.e-22;
             if (c->l_qr[k]) {
                 double lamr = 1.0 / c->ilamr[k];
                 double n0_r = c->nr[k] * p->org2 * pow(lamr, p->cre[1]);
                 ze_rain = n0_r * p->crg[3] * pow(c->ilamr[k], p->cre[3]);
             }
-            if (c->l_qs[k])
+            if (c->l_qs[k]) {
                 ze_snow = (0.176 / 0.93) * (6.0 / pi) * (6.0 / pi) * (am_s / 900.0) * (am_s / 900.0) * c->smoz[k];
+                if (melt) ze_snow = melting_snow(p, c->rs[k], c->smob[k], c->smoc[k], c->rr[k]);
+            }
             if (c->l_qg[k]) {
                 int idx = c->idx_bg[k];
                 double n0_g = c->ng[k] * p->ogg2 * pow(1.0 / c->ilamg[k], CGE(2, 1));
                 ze_graupel = (0.176 / 0.93) * (6.0 / pi) * (6.0 / pi) * (p->am_g[idx - 1] / 900.0) *
                              (p->am_g[idx - 1] / 900.0) * n0_g * CGG(4, 1) * pow(c->ilamg[k], CGE(4, 1));
+                if (melt) ze_graupel = melting_graupel(p, c->rg[k], c->ng[k], c->ilamg[k], idx, c->rr[k]);
             }
             c->refl[k] = dmax(-35.0, 10.0 * log10((ze_rain + ze_snow + ze_graupel) * 1.e18));
         }

@@ -142,7 +142,7 @@ struct tempo_hip_ctx {
   DevBuf ccn;  // float slice (7,9,7)
   DevBuf efrw64, efsw64, wev64, iaus64, efrw32, efsw32, wev32, iaus32;
   // scratch
-  DevBuf sr, sd, pf, cf_, precip, ksub, vtbuf;
+  DevBuf sr, sd, pf, cf_, precip, ksub, vtbuf, kmelt, radar;
   DevBuf semiw;  // semi-Lagrangian sedimentation work (allocated only with semi_sedi_flag)
   DevBuf tl_state, tl_diag;  // dual (value, tangent) copies for the tangent-linear step
   DevBuf adj_in, adj_out, adj_mem;  // adjoint: packed inputs, new state, cotangent/Jacobian planes
@@ -183,8 +183,6 @@ int tempo_hip_set_run_cfg(tempo_hip_ctx* x, const tempo_hip_cfg* cfg) {
   if (!x || !cfg) return fail(TEMPO_HIP_E_ARG, "tempo_hip_set_run_cfg: NULL argument");
   if (cfg->aerosolaware_flag != x->cfg.aerosolaware_flag || cfg->hailaware_flag != x->cfg.hailaware_flag)
     return fail(TEMPO_HIP_E_ARG, "aerosolaware_flag and hailaware_flag are fixed at tempo_hip_init");
-  if (cfg->refl10cm_from_melting_flag)
-    return fail(TEMPO_HIP_E_UNSUPPORTED, "melting-layer reflectivity is not supported");
   x->cfg = *cfg;
   return TEMPO_HIP_OK;
 }
@@ -341,6 +339,7 @@ int ensure_scratch(tempo_hip_ctx* x, size_t ncol, size_t nz, size_t br = 8, size
   HIP_TRY(x->pf.ensure(plane * sizeof(unsigned)));
   HIP_TRY(x->vtbuf.ensure(vt_planes * plane * bw));
   HIP_TRY(x->cf_.ensure(ncol * sizeof(unsigned)));
+  HIP_TRY(x->kmelt.ensure(ncol * sizeof(int)));
   HIP_TRY(x->precip.ensure(5 * ncol * bw));
   HIP_TRY(x->ksub.ensure(10 * ncol * sizeof(int)));
   if (x->cfg.semi_sedi_flag) HIP_TRY(x->semiw.ensure(kSemiArrays * (nz + 2) * ncol * bw));
@@ -390,6 +389,9 @@ int bind_common(tempo_hip_ctx* x, StepArgs<R>& a, double dt, int itimestep) {
   a.fl.hailaware = c.hailaware_flag;
   a.fl.turn_off_micro = c.turn_off_micro_flag;
   a.fl.semi_sedi = c.semi_sedi_flag;
+  a.fl.refl_from_melting = c.refl10cm_from_melting_flag;
+  a.radar = x->radar.as<Radar>();
+  a.kmelt = x->kmelt.as<int>();
   a.semiw = c.semi_sedi_flag ? x->semiw.as<W>() : nullptr;
   a.fl.cloud_condensation = c.cloud_condensation_flag;
   a.fl.refl10cm = c.refl10cm_flag;
@@ -404,6 +406,7 @@ int bind_common(tempo_hip_ctx* x, StepArgs<R>& a, double dt, int itimestep) {
   a.fl.ml_for_bl_nc = c.ml_for_bl_nc_flag;
   a.fl.nc_const = c.nc_option == 1 ? kNtCO : (c.nc_option == 2 ? c.nc_value : kNtCL);
   a.dt = dt;
+  a.odt = sizeof(S) == 4 ? double(1.0f / float(dt)) : 1.0 / dt;
   a.first_step = itimestep == 1;
   return TEMPO_HIP_OK;
 }
@@ -478,6 +481,8 @@ int run_tlm(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fields*
     return fail(TEMPO_HIP_E_UNSUPPORTED, "the tangent-linear step covers flux-form sedimentation only");
   if (x->cfg.ml_for_nc_flag || f->qc_bl || f->qcfrac_bl)
     return fail(TEMPO_HIP_E_UNSUPPORTED, "the tangent-linear step does not cover the cloud-number network");
+  if (x->cfg.refl10cm_from_melting_flag)
+    return fail(TEMPO_HIP_E_UNSUPPORTED, "the tangent-linear step does not cover the melting-layer reflectivity");
   const int ncol = dm->ni * dm->nj, nz = dm->nz;
   const size_t n3 = size_t(ncol) * nz;
   HIP_TRY(x->tl_state.ensure(F_NFIELDS * n3 * sizeof(D)));
@@ -536,6 +541,8 @@ int run_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fie
     return fail(TEMPO_HIP_E_UNSUPPORTED, "the adjoint covers flux-form sedimentation only");
   if (x->cfg.ml_for_nc_flag || f->qc_bl || f->qcfrac_bl)
     return fail(TEMPO_HIP_E_UNSUPPORTED, "the adjoint does not cover the cloud-number network");
+  if (x->cfg.refl10cm_from_melting_flag)
+    return fail(TEMPO_HIP_E_UNSUPPORTED, "the adjoint does not cover the melting-layer reflectivity");
   if (!f->t) return fail(TEMPO_HIP_E_UNSUPPORTED, "the adjoint takes temperature input t (not th/pii)");
   const int ncol = dm->ni * dm->nj, nz = dm->nz;
   const size_t n3 = size_t(ncol) * nz, sb = F_NFIELDS * n3 * sizeof(D);
@@ -626,8 +633,6 @@ extern "C" {
 int tempo_hip_init(const tempo_hip_cfg* cfg, const char* table_dir, const char* ccn_path, tempo_hip_ctx** out) {
   if (!cfg || !out) return fail(TEMPO_HIP_E_ARG, "tempo_hip_init: NULL argument");
   *out = nullptr;
-  if (cfg->refl10cm_from_melting_flag)
-    return fail(TEMPO_HIP_E_UNSUPPORTED, "melting-layer reflectivity is not supported");
   auto* x = new tempo_hip_ctx();
   x->cfg = *cfg;
   int rc;
@@ -644,6 +649,9 @@ int tempo_hip_init(const tempo_hip_cfg* cfg, const char* table_dir, const char* 
     std::vector<Const<float>> vf(1, x->cf);
     if (upload(x->cd_dev, vd) != hipSuccess || upload(x->cf_dev, vf) != hipSuccess)
       return bail(fail(TEMPO_HIP_E_HIP, "constant upload failed"));
+    std::vector<Radar> vr(1);
+    build_radar(x->cd, vr[0]);
+    if (upload(x->radar, vr) != hipSuccess) return bail(fail(TEMPO_HIP_E_HIP, "constant upload failed"));
   }
   if ((rc = upload_small_tables(x))) return bail(rc);
   std::string ccn = ccn_path ? ccn_path : (table_dir ? std::string(table_dir) + "/ccn_activate.bin" : "");

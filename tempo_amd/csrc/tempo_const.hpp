@@ -80,6 +80,22 @@ struct Const {
   int hail;  // hail-aware graupel constants in use
 };
 
+constexpr int kRadarBins = 50;
+
+// Constants of the melting reflectivity, built on the host in FP64
+// (tempo_host.cpp, build_radar); used by tempo_radar.hpp.
+struct Radar {
+  double sbins[kRadarBins], dsbins[kRadarBins];  // snow diameters 100 um .. 2 cm and widths
+  double gbins[kRadarBins], dgbins[kRadarBins];  // graupel diameters 100 um .. 5 cm and widths
+  double simpson[kRadarBins];                    // Simpson weights of the bins (:226-233)
+  double ocmg[kNrhg], ocms, obms;                // oamg^obmg, oams^obms, 1/bm_s
+  // the first (ice-water) Maxwell-Garnett mixing's operands: m_w^2, m_i^2
+  // and its two shape factors, fixed by the 10-cm indices of water and ice at
+  // 0 C (complex_water_ray, complex_ice_maetzler, :144-201)
+  double m1_re, m1_im, m3_re, m3_im, b2_re, b2_im, b3_re, b3_im;
+  double k_w;  // |(m_w^2 - 1)/(m_w^2 + 2)|^2
+};
+
 // Per-run configuration flags (ty_tempo_cfgs, src/module_mp_tempo_cfgs.F90:14-38).
 struct RunFlags {
   int aerosolaware, hailaware, semi_sedi, refl_from_melting, turn_off_micro, cloud_condensation;

@@ -8,6 +8,7 @@
 #include "tempo_host.hpp"
 
 #include <cmath>
+#include <complex>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -39,7 +40,80 @@ void log_bins(int numbins, double lowbin, double highbin, double* bins, double* 
 
 int nint(double v) { return int(std::lround(v)); }
 
+// ---- melting-layer reflectivity constants (src/module_mp_tempo_diags.F90) ----
+// complex products and quotients formed as the reference's build forms them
+// ((ac - bd, ad + bc), the quotient over |b|^2); square root and logarithm
+// from the C library, as there
+using cxd = std::complex<double>;
+static cxd cx_mul(cxd a, cxd b) {
+  return cxd(a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real());
+}
+static cxd cx_div(cxd a, cxd b) {
+  const double d = b.real() * b.real() + b.imag() * b.imag();
+  return cxd((a.real() * b.real() + a.imag() * b.imag()) / d, (a.imag() * b.real() - a.real() * b.imag()) / d);
+}
+
+// complex_water_ray (:144-172) and complex_ice_maetzler (:175-201)
+static cxd water_ray(double lambda, double t) {
+  const double pi = kPi;
+  const double epsinf = 5.27137 + 0.02164740 * t - 0.00131198 * t * t;
+  const double epss = 78.54 * (1.0 - 4.579e-3 * (t - 25.0) + 1.190e-5 * (t - 25.0) * (t - 25.0) -
+                               2.800e-8 * (t - 25.0) * (t - 25.0) * (t - 25.0));
+  const double alpha = -16.8129 / (t + 273.16) + 0.0609265;
+  const double lambdas = 0.00033836 * std::exp(2513.98 / (t + 273.16)) * 1e-2;
+  const double p1 = std::pow(lambdas / lambda, 1.0 - alpha);
+  const double nenner = 1.0 + 2.0 * p1 * std::sin(alpha * pi * 0.5) + std::pow(lambdas / lambda, 2.0 - 2.0 * alpha);
+  const double epsr = epsinf + ((epss - epsinf) * (p1 * std::sin(alpha * pi * 0.5) + 1.0)) / nenner;
+  const double epsi = ((epss - epsinf) * (p1 * std::cos(alpha * pi * 0.5) + 0.0)) / nenner + lambda * 1.25664 / 1.88496;
+  return std::sqrt(cxd(epsr, -epsi));
+}
+static cxd ice_maetzler(double lambda, double t) {
+  const double c = 2.99e8, tk = t + 273.16, f = c / lambda * 1e-9;
+  const double b1 = 0.0207, b2 = 1.16e-11, b = 335.0;
+  const double deltabeta = std::exp(-10.02 + 0.0364 * (tk - 273.16));
+  const double em1 = std::exp(b / tk) - 1.0;
+  const double betam = (b1 / tk) * (std::exp(b / tk) / (em1 * em1)) + b2 * f * f;
+  const double beta = betam + deltabeta;
+  const double theta = 300.0 / tk - 1.0;
+  const double alfa = (0.00504 + 0.0062 * theta) * std::exp(-22.1 * theta);
+  cxd ri(3.1884 + 9.1e-4 * (tk - 273.16), 0.0);
+  ri = ri + cxd(0.0, alfa / f + beta * f);
+  return std::sqrt(std::conj(ri));
+}
+
+
 }  // namespace
+
+void build_radar(const Const<double>& c, Radar& r) {
+  std::memset(&r, 0, sizeof(r));
+  // initialize_bins_for_radar (src/module_mp_tempo_params.F90:674-692)
+  log_bins(kRadarBins, 100.e-6, 0.02, r.sbins, r.dsbins);
+  log_bins(kRadarBins, 100.e-6, 0.05, r.gbins, r.dgbins);
+  // Simpson weights (:226-233): 1/3, 4/3, 1/3 added over consecutive pairs
+  double w[kRadarBins + 1] = {};
+  for (int n = 0; n < kRadarBins - 1; n += 2) {
+    w[n] += 1.0 / 3.0;
+    w[n + 1] += 4.0 / 3.0;
+    w[n + 2] += 1.0 / 3.0;
+  }
+  for (int n = 0; n < kRadarBins; ++n) r.simpson[n] = w[n];
+  // obms, ocms = oams**obms, ocmg = oamg**obmg (src/module_mp_tempo_params.F90:468-473)
+  r.obms = 1.0 / kBmS;
+  r.ocms = std::pow(c.oams, r.obms);
+  for (int m = 0; m < kNrhg; ++m) r.ocmg[m] = std::pow(1.0 / c.am_g[m], c.obmg);
+  // the first Maxwell-Garnett mixing: m1t = m_w^2, m2t = m_air^2 = 1, m3t = m_i^2
+  const cxd m_w = water_ray(0.10, 0.0), m_i = ice_maetzler(0.10, 0.0), m_air(1.0, 0.0);
+  const cxd m1t = cx_mul(m_w, m_w), m2t = cx_mul(m_air, m_air), m3t = cx_mul(m_i, m_i);
+  const cxd two(2.0, 0.0), one(1.0, 0.0);
+  const cxd beta2 = cx_mul(cx_div(cx_mul(two, m1t), m2t - m1t), cx_mul(cx_div(m2t, m2t - m1t), std::log(cx_div(m2t, m1t))) - one);
+  const cxd beta3 = cx_mul(cx_div(cx_mul(two, m1t), m3t - m1t), cx_mul(cx_div(m3t, m3t - m1t), std::log(cx_div(m3t, m1t))) - one);
+  r.m1_re = m1t.real(); r.m1_im = m1t.imag();
+  r.m3_re = m3t.real(); r.m3_im = m3t.imag();
+  r.b2_re = beta2.real(); r.b2_im = beta2.imag();
+  r.b3_re = beta3.real(); r.b3_im = beta3.imag();
+  const double kw = std::abs(cx_div(m1t - one, m1t + cxd(2.0, 0.0)));
+  r.k_w = kw * kw;
+}
 
 Bins make_bins() {
   Bins b;

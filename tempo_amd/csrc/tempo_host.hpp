@@ -40,6 +40,8 @@ struct TableFactors {
 Bins make_bins();
 void build_const(Const<double>& c, HostBins& hb, bool hail, bool cube_as_product = false);
 void build_small_tables(const Const<double>& c, const HostBins& hb, SmallTables& t);
+// constants of the melting-layer reflectivity
+void build_radar(const Const<double>& c, Radar& r);
 void build_table_factors(TableFactors& f);
 void snow_m2_m3(const Const<double>& c, double rs, double tc, double& smob, double& smoc);
 double gamma_p_host(double a, double x);

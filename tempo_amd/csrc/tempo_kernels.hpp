@@ -3,6 +3,7 @@
 #pragma once
 
 #include "tempo_step.hpp"
+#include "tempo_radar.hpp"
 
 namespace tempo {
 
@@ -207,7 +208,7 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_A k_phase_a(StepArgs<R> a) {
   const size_t o = TEMPO_IDX_STATE(a, c, k);
   const size_t s = TEMPO_IDX_SCR(a, c, k);
   const size_t plane = a.plane;
-  const R dt = R(a.dt), odt = R(1.0) / dt;
+  const R dt = R(a.dt), odt = R(a.odt);
   const unsigned cf = a.cf[c];
   const bool has_nc = a.f[F_NC], has_nwfa = a.f[F_NWFA], has_nifa = a.f[F_NIFA];
   const bool has_ngqb = a.f[F_NG] && a.f[F_QB];
@@ -656,7 +657,7 @@ __device__ void semi_rain_refresh(const StepArgs<R>& a, int c, int nzc, W_t<R>* 
   const Const<S_t<R>>& C = *a.cst;
   const size_t plane = a.plane;
   const R* sr = a.sr;
-  const R dt = R(a.dt), odt = R(1.0) / dt;
+  const R dt = R(a.dt), odt = R(a.odt);
   R v1 = R(0.0), v2 = R(0.0);
   for (int k = nzc - 1; k >= 0; --k) {
     const size_t s = TEMPO_IDX_SCR(a, c, k), o = TEMPO_IDX_STATE(a, c, k);
@@ -689,7 +690,7 @@ __device__ void semi_graupel_refresh(const StepArgs<R>& a, int c, int nzc, W_t<R
   const Const<S_t<R>>& C = *a.cst;
   const size_t plane = a.plane;
   const R* sr = a.sr;
-  const R dt = R(a.dt), odt = R(1.0) / dt;
+  const R dt = R(a.dt), odt = R(a.odt);
   const bool has_ngqb = a.f[F_NG] && a.f[F_QB];
   R v1 = R(0.0), v2 = R(0.0);
   for (int k = nzc - 1; k >= 0; --k) {
@@ -890,6 +891,97 @@ __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, W_t<R>* vtbuf) {
   if (a.d2[D2_HAIL_COL]) a.d2[D2_HAIL_COL][d2] = R(0);
 }
 
+// ------------------------------------------------------ melting level ----
+// find_melting_level (src/module_mp_tempo_diags.F90:124-141) on the state
+// the final update will check: rain present at a level above freezing with
+// snow or graupel present at the level above; presence as the final
+// checks decide it (mass after the step's tendencies above r1).  One lane
+// per column, before the final update.
+template <class R>
+__global__ void __launch_bounds__(256) k_melt_level(StepArgs<R> a) {
+  const int c = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= a.ncol || (a.cf[c] & CF_EXIT)) return;
+  const int nzc = a.nlev ? min(max(a.nlev[c], 1), a.nz) : a.nz;
+  const R dt = R(a.dt);
+  const size_t plane = a.plane;
+  auto present = [&](int f, int ten, int k) {
+    const size_t o = TEMPO_IDX_STATE(a, c, k), s = TEMPO_IDX_SCR(a, c, k);
+    return a.fa[f][o] + R(a.sd[size_t(ten) * plane + s]) * dt > R(kR1);
+  };
+  int k_melt = 1;
+  for (int k = nzc - 2; k >= 0; --k) {
+    const R temp = a.sr[SR_TEMP * plane + TEMPO_IDX_SCR(a, c, k)];
+    if (temp > R(kT0) && present(F_QR, SD_QRTEN, k) &&
+        (present(F_QS, SD_QSTEN, k + 1) || present(F_QG, SD_QGTEN, k + 1))) {
+      k_melt = max(k + 2, k_melt);
+      break;
+    }
+  }
+  a.kmelt[c] = k_melt;
+}
+
+// a level below the column's melting level, where reflectivity_10cm takes
+// snow and graupel as melting (:92-112; k_melt > 2 and k < k_melt - 1, 1-based)
+template <class R>
+__device__ __forceinline__ bool melt_level(const StepArgs<R>& a, int c, int k) {
+  return a.fl.refl_from_melting && a.fl.refl10cm && a.kmelt[c] > 2 && k + 1 < a.kmelt[c] - 1;
+}
+
+// reflectivity_10cm at the levels with melting snow or graupel: the final
+// rain, snow and graupel checks of the final update (on copies, from the same
+// operands), dry rain and the binned wet-particle reflectivity of snow and
+// graupel.  Its own launch, before the final update (which leaves these
+// levels' reflectivity to it), so that the final update keeps its register
+// budget.
+template <class R>
+__global__ void __launch_bounds__(256) k_melt_refl(StepArgs<R> a) {
+  const int c = int(blockIdx.x) * 64 + int(threadIdx.x & 63u);
+  const int k = int(blockIdx.y) * int(blockDim.x >> 6) + int(threadIdx.x >> 6);
+  if (c >= a.ncol || k >= a.nz) return;
+  if ((a.nlev && k >= a.nlev[c]) || (a.cf[c] & CF_EXIT) || !melt_level(a, c, k)) return;
+  const Const<S_t<R>>& C = *a.cst;
+  const size_t o = TEMPO_IDX_STATE(a, c, k), s = TEMPO_IDX_SCR(a, c, k), plane = a.plane;
+  const R dt = R(a.dt), odt = R(a.odt);
+  const bool has_ngqb = a.f[F_NG] && a.f[F_QB];
+  Lvl<R> L;
+  L.temp = a.sr[SR_TEMP * plane + s];
+  L.qv = a.sr[SR_QV * plane + s];
+  L.pres = a.f[F_P][o];
+  L.rho = rho_of(C, L.pres, L.temp, L.qv);
+  L.qrten = R(a.sd[SD_QRTEN * plane + s]);
+  L.nrten = R(a.sd[SD_NRTEN * plane + s]);
+  L.qsten = R(a.sd[SD_QSTEN * plane + s]);
+  L.qgten = R(a.sd[SD_QGTEN * plane + s]);
+  L.ngten = R(a.sd[SD_NGTEN * plane + s]);
+  L.qbten = R(a.sd[SD_QBTEN * plane + s]);
+  R qr1d = a.fa[F_QR][o], nr1d = a.fa[F_NR][o], qs1d = a.fa[F_QS][o], qg1d = a.fa[F_QG][o];
+  R ng1d = has_ngqb ? a.fa[F_NG][o] : R(0), qb1d = has_ngqb ? a.fa[F_QB][o] : R(0);
+  rain_check(C, L, qr1d, nr1d, dt, odt);
+  snow_check(L, qs1d, dt, odt);
+  L.smob = L.smoc = R(0);
+  if (L.l_qs) snow_moments(C, L.rs, mn(R(-0.1), L.temp - R(kT0)), 0, L);
+  graupel_check(C, L, qg1d, ng1d, qb1d, has_ngqb, dt, odt);
+  if (!(L.l_qs || L.l_qg)) return;
+  constexpr bool kCR = sizeof(S_t<R>) == 8;
+  R ze_r = R(1.e-22), ze_s = R(1.e-22), ze_g = R(1.e-22);
+  if (L.l_qr) {
+    const R lamr = R(1.0) / L.ilamr;
+    const R n0_r = L.nr * C.org2 * mpow(lamr, C.cre[1]);
+    ze_r = n0_r * C.crg[3] * mpow(L.ilamr, C.cre[3]);
+  }
+  if (L.l_qs)
+    ze_s = R(S_t<R>(radar::melting_snow<kCR>(*a.radar, double(value(L.rs)), double(value(L.smob)),
+                                             double(value(L.smoc)), double(value(L.rr)))));
+  if (L.l_qg) {
+    const int idx = L.idx_bg;
+    ze_g = R(S_t<R>(radar::melting_graupel<kCR>(*a.radar, double(C.am_g[idx - 1]), double(C.ogg2),
+                                                double(C.cge[1][0]), double(value(L.rg)), double(value(L.ng)),
+                                                double(value(L.ilamg)), idx, double(value(L.rr)))));
+  }
+  const size_t d3 = size_t(c % a.ni) + size_t(k) * a.dldk + size_t(c / a.ni) * a.dldj;
+  a.d3[D3_REFL][d3] = mx(R(-35.0), R(10.0) * mlog10((ze_r + ze_s + ze_g) * R(1.e18)));
+}
+
 // ----------------------------------------------------------- phase C ----
 __device__ __forceinline__ void atomic_max_pos(float* p, float v) {
   atomicMax(reinterpret_cast<unsigned int*>(p), __float_as_uint(v));
@@ -914,7 +1006,7 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
   const size_t s = TEMPO_IDX_SCR(a, c, k);
   const size_t plane = a.plane;
   const size_t d3 = size_t(c % a.ni) + size_t(k) * a.dldk + size_t(c / a.ni) * a.dldj;
-  const R dt = R(a.dt), odt = R(1.0) / dt;
+  const R dt = R(a.dt), odt = R(a.odt);
   const bool live = !(a.cf[c] & CF_EXIT);
   const bool has_nc = a.f[F_NC], has_nwfa = a.f[F_NWFA], has_nifa = a.f[F_NIFA];
   const bool has_ngqb = a.f[F_NG] && a.f[F_QB];
@@ -954,6 +1046,9 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
   const R ncv = (a.fl.ml_for_nc && !has_nc) ? sr[SR_NCSAVE * plane + s] : column_nc(a, c);
 
   if (!a.fl.turn_off_micro) {  // freeze_cloud_melt_ice (:2124-2165)
+    // uncontracted, as the final checks: a fully frozen or melted mass then
+    // cancels to zero as in the reference
+#pragma clang fp contract(off)
     const R xri = mx(R(0.0), qi1d + L.qiten * dt);
     if ((L.temp > R(kT0)) && (xri > R(0.0))) {
       L.qcten = L.qcten + xri * odt;
@@ -1056,7 +1151,8 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
       }
       dbz = mx(R(-35.0), R(10.0) * mlog10((ze_r + ze_s + ze_g) * R(1.e18)));
     }
-    a.d3[D3_REFL][d3] = dbz;
+    // levels with melting snow or graupel are k_melt_refl's
+    if (!(melt_level(a, c, k) && (L.l_qs || L.l_qg))) a.d3[D3_REFL][d3] = dbz;
   }
   const bool re = a.fl.re_cloud && a.fl.re_ice && a.fl.re_snow;
   R re_c = R(0), re_i = R(0), re_s = R(0);

@@ -191,6 +191,8 @@ struct StepArgs {
   const int* nlev;   // per column active level count or nullptr
   const R* qc_bl;      // boundary-layer cloud water and its cloud fraction (state layout), or nullptr
   const R* qcfrac_bl;
+  const Radar* radar;  // melting-layer reflectivity constants (device copy)
+  int* kmelt;          // per column melting level (1-based), written when the melting reflectivity is on
   int ni, nj, nz, ncol;
   int nzs;           // level stride of a scratch column group (scratch_levels(nz))
   long long ldk, ldj, dldk, dldj, d2ldj;
@@ -210,6 +212,10 @@ struct StepArgs {
   Tables<S_t<R>> tab;
   RunFlags fl;
   double dt;
+  // 1/dt in the working precision, divided on the host: the FP32 unit's
+  // reciprocal arithmetic would leave a one-ulp error that keeps the
+  // x + (-x/dt)*dt of a fully removed mass from cancelling
+  double odt;
   int first_step;
 };
 
@@ -429,6 +435,11 @@ __device__ R ml_cloud_number(R qc, R qr, R qi, R qs, R pres, R temp, R w) {
 // ncv: the column's one-moment cloud number (used when nc is absent, has_nc false)
 template <class R>
 __device__ void cloud_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qc1d, R& nc1d, bool has_nc, R ncv, R dt, R odt) {
+  // x + ten*dt rounded as two operations, as the reference forms it: where the
+  // tendency removes the whole mass the product rounds to -x and nothing is
+  // left, whereas a fused multiply-add would leave the rounding error of
+  // the tendency, above the r1 presence threshold for typical masses
+#pragma clang fp contract(off)
   bool hit = false;
   if (qc1d + L.qcten * dt > R(kR1)) {
     L.l_qc = true;
@@ -478,6 +489,7 @@ __device__ void cloud_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qc1d, R& nc1d,
 
 template <class R>
 __device__ void rain_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qr1d, R& nr1d, R dt, R odt) {
+#pragma clang fp contract(off)
   bool hit = false;
   if (qr1d + L.qrten * dt > R(kR1)) {
     L.l_qr = true;
@@ -523,6 +535,7 @@ __device__ void rain_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qr1d, R& nr1d, 
 
 template <class R>
 __device__ void ice_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qi1d, R& ni1d, R dt, R odt) {
+#pragma clang fp contract(off)
   bool hit = false;
   if (qi1d + L.qiten * dt > R(kR1)) {
     L.l_qi = true;
@@ -563,6 +576,7 @@ __device__ void ice_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qi1d, R& ni1d, R
 
 template <class R>
 __device__ void snow_check(Lvl<R>& L, R& qs1d, R dt, R odt) {
+#pragma clang fp contract(off)
   if (qs1d + L.qsten * dt > R(kR1)) {
     L.l_qs = true;
     L.rs = (qs1d + L.qsten * dt) * L.rho;
@@ -582,6 +596,7 @@ __device__ R rho_g(int idx) {  // densities of graupel (src/module_mp_tempo_para
 
 template <class R>
 __device__ void graupel_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qg1d, R& ng1d, R& qb1d, bool has_ngqb, R dt, R odt) {
+#pragma clang fp contract(off)
   bool hit = false;
   if (qg1d + L.qgten * dt > R(kR1)) {
     L.l_qg = true;

@@ -19,6 +19,10 @@ hipError_t launch_step(const StepArgs<float>& a, const LaunchCfg& lc, double* vt
                        sed_lds_bytes(a.nz), s, a);
   else
     hipLaunchKernelGGL(k_phase_b<float>, dim3(unsigned((a.ncol + 127) / 128)), dim3(128), 0, s, a, vtbuf);
+  if (a.fl.refl_from_melting && a.fl.refl10cm && a.d3[D3_REFL]) {
+    hipLaunchKernelGGL(k_melt_level<float>, dim3(unsigned((a.ncol + 255) / 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_melt_refl<float>, level_grid(a, 256), dim3(256), 0, s, a);
+  }
   if (ev) (void)hipEventRecord(ev[3], s);
   hipLaunchKernelGGL(k_phase_c<float>, level_grid(a, lc.block_c), dim3(unsigned(lc.block_c)), 0, s, a);
   if (ev) (void)hipEventRecord(ev[4], s);

@@ -212,6 +212,14 @@ def main():
     st["qc_bl"] = np.ascontiguousarray(qbl)
     st["qcfrac_bl"] = np.ascontiguousarray(np.where(qbl > 0.0, rng.uniform(0.05, 1.0, (nz, ncol)), 0.0))
     run_case("ml_bl_aerosol", st, m["dt"], m["itimestep"], dict(AERO, ml_for_bl_nc=True), steps=2)
+    # reflectivity with melting snow and graupel below the melting level
+    # (refl10cm_from_melting_flag, src/module_mp_tempo_diags.F90:59-395)
+    CHECK_DIAGS = ("refl10cm",)
+    run_case("melt_mpas", full, 20.0, 2, dict(BOTH, refl_from_melting=True), steps=2)
+    st, m = gen.generate("multistep", ncol=32, col0=2100)
+    run_case("melt_multistep", st, m["dt"], m["itimestep"], dict(BOTH, refl_from_melting=True), steps=2)
+    st, m = gen.generate("ragged", ncol=24, col0=3300)
+    run_case("melt_ragged", st, m["dt"], m["itimestep"], dict(HAIL, refl_from_melting=True), nlev=m["nlev"])
     CHECK_DIAGS = ()
     # tempo_aerosol_surface_emissions (src/module_mp_tempo_driver.F90:516-529)
     if not ONLY or "surface_emissions" in ONLY:

@@ -37,7 +37,8 @@ FLAG_TO_CFG = dict(aerosolaware="aerosolaware_flag", hailaware="hailaware_flag",
                    re_ice="re_ice_flag", re_snow="re_snow_flag", max_hail="max_hail_diameter_flag",
                    mvd_r="rain_med_vol_diam_flag", mvd_g="graupel_med_vol_diam_flag",
                    nc_mixing_ratio="cloud_number_mixing_ratio_flag", semi_sedi="semi_sedi_flag",
-                   ml_for_nc="ml_for_nc_flag", ml_for_bl_nc="ml_for_bl_nc_flag")
+                   ml_for_nc="ml_for_nc_flag", ml_for_bl_nc="ml_for_bl_nc_flag",
+                   refl_from_melting="refl10cm_from_melting_flag")
 
 
 def cfg_of(flags):
@@ -336,6 +337,39 @@ def test_gpu_ml_cloud_number_matches_oracle(hip, gpu_tables, which):
         assert other <= 1e-3, (k, other)
 
 
+def test_gpu_melting_reflectivity_matches_oracle(hip, gpu_tables):
+    """Reflectivity of melting snow and graupel below the melting level
+    (refl10cm_from_melting_flag, src/module_mp_tempo_diags.F90:59-395) on 512
+    hail- and aerosol-aware columns: FP64 path vs the oracle; FP32 path vs the
+    FP64 twin (dBZ)."""
+    from oracle import oracle as orc
+    from tempo_amd import generator as gen
+    st, m = gen.generate("multistep", ncol=512, col0=91_000)
+    flags = dict(aerosolaware=True, hailaware=True, refl_from_melting=True)
+    orc.init(gpu_tables, **flags)
+    ref = {k: v.copy() for k, v in st.items()}
+    d2, d3 = orc.run(ref, m["dt"], m["itimestep"])
+    ctx = context(hip, flags)
+    out, d = gpu_run(ctx, st, m["dt"], m["itimestep"])
+    for k in ref:
+        assert rel_inf(out[k], ref[k]) <= 1e-12, k
+    assert rel_inf(d["refl10cm"], d3["refl10cm"]) <= 1e-12, rel_inf(d["refl10cm"], d3["refl10cm"])
+    # the melting branch is reached
+    orc.init(gpu_tables, **dict(flags, refl_from_melting=False))
+    dry = {k: v.copy() for k, v in st.items()}
+    _, e3 = orc.run(dry, m["dt"], m["itimestep"])
+    assert (np.abs(e3["refl10cm"] - d3["refl10cm"]) > 0.01).sum() > 100
+    inp = {k: v.astype(np.float32).astype(np.float64) for k, v in st.items()}
+    o64, e64 = gpu_run(ctx, inp, m["dt"], m["itimestep"], dtype=torch.float64)
+    o32, e32 = gpu_run(ctx, inp, m["dt"], m["itimestep"], dtype=torch.float32)
+    flip = np.abs(e32["refl10cm"] - e64["refl10cm"]) > 0.01
+    keep = ~flip
+    err = np.linalg.norm((e32["refl10cm"] - e64["refl10cm"])[keep]) / np.linalg.norm(e64["refl10cm"][keep])
+    print("melting refl10cm FP32 rel L2", err, "points moved by more than 0.01 dBZ", flip.mean())
+    assert err <= 1e-4, err
+    assert flip.mean() <= 0.01, flip.mean()
+
+
 def test_gpu_run_cfg_changes_between_steps(hip):
     """Flags changed after init (tempo_hip_set_run_cfg) act like a context
     initialised with them, as a host editing ty_tempo_cfgs between calls."""
@@ -408,9 +442,6 @@ def test_gpu_error_codes(hip):
     with pytest.raises(hip.TempoError) as e:
         ctx.run(dev, -1.0, 2)
     assert e.value.code == hip.E_ARG
-    with pytest.raises(hip.TempoError) as e:
-        hip.TempoHip(refl10cm_from_melting_flag=1)
-    assert e.value.code == hip.E_UNSUPPORTED
     # the derivative entry points cover the flux-form sedimentation only
     semi = hip.TempoHip(aerosolaware_flag=0, hailaware_flag=0, semi_sedi_flag=1)
     with pytest.raises(hip.TempoError) as e:
@@ -422,11 +453,12 @@ def test_gpu_error_codes(hip):
     with pytest.raises(hip.TempoError) as e:
         ctx.run(bl, m["dt"], 2)
     assert e.value.code == hip.E_ARG
-    ml = hip.TempoHip(aerosolaware_flag=0, hailaware_flag=0, ml_for_nc_flag=1)
-    with pytest.raises(hip.TempoError) as e:
-        ml.tlm(dict(dev), {k: torch.zeros_like(v) for k, v in dev.items()}, m["dt"], 2)
-    assert e.value.code == hip.E_UNSUPPORTED
-    ml.close()
+    for opt in ("ml_for_nc_flag", "refl10cm_from_melting_flag"):
+        other = hip.TempoHip(aerosolaware_flag=0, hailaware_flag=0, **{opt: 1})
+        with pytest.raises(hip.TempoError) as e:
+            other.tlm(dict(dev), {k: torch.zeros_like(v) for k, v in dev.items()}, m["dt"], 2)
+        assert e.value.code == hip.E_UNSUPPORTED, opt
+        other.close()
     with tempfile.TemporaryDirectory() as d:
         with pytest.raises(hip.TempoError) as e:
             hip.TempoHip(table_dir=d)
