@@ -151,6 +151,36 @@ int tempo_hip_aerosol_surface_emissions(tempo_hip_ctx *ctx, double dt, void *nwf
 int tempo_hip_aerosol_surface_emissions_host(tempo_hip_ctx *ctx, double dt, double *nwfa, const double *nwfa2d,
                                              const tempo_hip_dims *dims, int k);
 
+/* The public check-and-update routines a host calls outside the step
+ * (src/module_mp_tempo_main.F90:18-20; FV3 recomputes the effective-radius
+ * inputs after advection with them), over n independent points of device
+ * arrays in the given precision.  Arguments follow the reference:
+ *   cloud_check_and_update (:891-973): rho in; qc1d, qcten, ncten inout;
+ *     nc1d inout or NULL (one-moment cloud), ncsave in or NULL (one-moment
+ *     numbers; NULL: the land value); l_qc (0/1), rc, nc, ilamc, mvd_c out;
+ *   ice_check_and_update (:1045-1107): rho in; qi1d, ni1d, qiten, niten
+ *     inout; l_qi, ri, ni, ilami out;
+ *   snow_check_and_update (:1110-1134): rho in; qs1d, qsten inout; l_qs,
+ *     rs out.
+ * ilamc and ilami are in the working precision (FP64 in the reference). */
+int tempo_hip_cloud_check_and_update(tempo_hip_ctx *ctx, int n, const void *rho, int *l_qc, void *qc1d, void *nc1d,
+                                     const void *ncsave, void *rc, void *nc, void *qcten, void *ncten, void *ilamc,
+                                     void *mvd_c, double dt, int precision, void *stream);
+int tempo_hip_ice_check_and_update(tempo_hip_ctx *ctx, int n, const void *rho, int *l_qi, void *qi1d, void *ni1d,
+                                   void *ri, void *ni, void *qiten, void *niten, void *ilami, double dt, int precision,
+                                   void *stream);
+int tempo_hip_snow_check_and_update(tempo_hip_ctx *ctx, int n, const void *rho, int *l_qs, void *qs1d, void *rs,
+                                    void *qsten, double dt, int precision, void *stream);
+/* The same on FP64 host arrays. */
+int tempo_hip_cloud_check_and_update_host(tempo_hip_ctx *ctx, int n, const double *rho, int *l_qc, double *qc1d,
+                                          double *nc1d, const double *ncsave, double *rc, double *nc, double *qcten,
+                                          double *ncten, double *ilamc, double *mvd_c, double dt);
+int tempo_hip_ice_check_and_update_host(tempo_hip_ctx *ctx, int n, const double *rho, int *l_qi, double *qi1d,
+                                        double *ni1d, double *ri, double *ni, double *qiten, double *niten,
+                                        double *ilami, double dt);
+int tempo_hip_snow_check_and_update_host(tempo_hip_ctx *ctx, int n, const double *rho, int *l_qs, double *qs1d,
+                                         double *rs, double *qsten, double dt);
+
 /* Tangent-linear step (no counterpart in the reference): the step of
  * tempo_hip_run on f (updated in place) together with its derivative in the
  * direction f_tl -- device arrays laid out like f, holding the tangents of

@@ -179,3 +179,29 @@ def surface_emissions(dt, nwfa, nwfa2d, kts, variant=None):
     assert nwfa.dtype == np.float64 and nwfa.flags.c_contiguous and nwfa2d.flags.c_contiguous
     nj, nk, ni = nwfa.shape
     lib(variant).ref_surface_emissions(dt, _ptr(nwfa), _ptr(nwfa2d), ni, nk, nj, kts)
+
+
+def check_and_update(which, arrays, dt, variant=None, has_nc=True, has_ncsave=False):
+    """The reference's public cloud/ice/snow_check_and_update
+    (src/module_mp_tempo_main.F90:891-1134) on n points; ``arrays`` maps the
+    argument names to float64 arrays (l_* int32 0/1) updated in place."""
+    L = lib(variant)
+    n = len(arrays["rho"])
+    P = ctypes.c_void_p
+    ptr = lambda k: arrays[k].ctypes.data
+    if which == "cloud":
+        f = L.ref_cloud_check
+        f.argtypes = [ctypes.c_int] + [P] * 11 + [ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        f.restype = None
+        f(n, *[ptr(k) for k in ("rho", "l_qc", "qc1d", "nc1d", "ncsave", "rc", "nc", "qcten", "ncten", "ilamc",
+                                "mvd_c")], float(dt), int(has_nc), int(has_ncsave))
+    elif which == "ice":
+        f = L.ref_ice_check
+        f.argtypes = [ctypes.c_int] + [P] * 9 + [ctypes.c_double]
+        f.restype = None
+        f(n, *[ptr(k) for k in ("rho", "l_qi", "qi1d", "ni1d", "ri", "ni", "qiten", "niten", "ilami")], float(dt))
+    else:
+        f = L.ref_snow_check
+        f.argtypes = [ctypes.c_int] + [P] * 5 + [ctypes.c_double]
+        f.restype = None
+        f(n, *[ptr(k) for k in ("rho", "l_qs", "qs1d", "rs", "qsten")], float(dt))

@@ -24,6 +24,7 @@ module ref_bridge
   use module_mp_tempo_utils, only : compute_efrw, compute_efsw, compute_drop_evap, qi_aut_qs, &
     calc_gamma_p, calc_rslf, calc_rsif, snow_moments
   use module_mp_tempo_driver, only : tempo_run, ty_tempo_driver_diags, tempo_aerosol_surface_emissions
+  use module_mp_tempo_main, only : cloud_check_and_update, ice_check_and_update, snow_check_and_update
   use module_mp_tempo_ml, only : ty_tempo_ml_data, save_or_read_ml_data, nc_ml_nodes, nc_ml_input, &
     nc_ml_output, nc_ml_trans_mean, nc_ml_trans_var, nc_ml_w00, nc_ml_w01, nc_ml_b00, nc_ml_b01
   implicit none
@@ -297,6 +298,68 @@ contains
     if (allocated(diags%cloud_number_mixing_ratio)) deallocate(diags%cloud_number_mixing_ratio)
     ref_run = 0
   end function ref_run
+
+  ! The public check-and-update routines (src/module_mp_tempo_main.F90:18-20,
+  ! 891-1134) on n points, FP64 arrays; logical flags exchanged as 0/1;
+  ! has_nc / has_ncsave select the optional nc1d / ncsave arguments.
+  subroutine ref_cloud_check(n, rho, l_qc, qc1d, nc1d, ncsave, rc, nc, qcten, ncten, ilamc, mvd_c, dt, has_nc, &
+      has_ncsave) bind(c, name='ref_cloud_check')
+    integer(c_int), value :: n, has_nc, has_ncsave
+    real(c_double), value :: dt
+    real(c_double), intent(in) :: rho(n), ncsave(n)
+    real(c_double), intent(inout) :: qc1d(n), nc1d(n), rc(n), nc(n), qcten(n), ncten(n), ilamc(n), mvd_c(n)
+    integer(c_int), intent(inout) :: l_qc(n)
+    real(wp) :: w_rho(n), w_qc(n), w_nc(n), w_ncs(n), w_rc(n), w_ncv(n), w_qt(n), w_nt(n), w_mvd(n)
+    real(dp) :: w_ilam(n)
+    logical :: l(n)
+    w_rho = real(rho, wp); w_qc = real(qc1d, wp); w_nc = real(nc1d, wp); w_ncs = real(ncsave, wp)
+    w_rc = real(rc, wp); w_ncv = real(nc, wp); w_qt = real(qcten, wp); w_nt = real(ncten, wp)
+    l = l_qc /= 0
+    if (has_nc /= 0) then
+      call cloud_check_and_update(rho=w_rho, l_qc=l, qc1d=w_qc, nc1d=w_nc, rc=w_rc, nc=w_ncv, qcten=w_qt, &
+        ncten=w_nt, ilamc=w_ilam, mvd_c=w_mvd, dt=real(dt, wp), odt=1._wp/real(dt, wp))
+    else if (has_ncsave /= 0) then
+      call cloud_check_and_update(rho=w_rho, l_qc=l, qc1d=w_qc, ncsave=w_ncs, rc=w_rc, nc=w_ncv, qcten=w_qt, &
+        ncten=w_nt, ilamc=w_ilam, mvd_c=w_mvd, dt=real(dt, wp), odt=1._wp/real(dt, wp))
+    else
+      call cloud_check_and_update(rho=w_rho, l_qc=l, qc1d=w_qc, rc=w_rc, nc=w_ncv, qcten=w_qt, &
+        ncten=w_nt, ilamc=w_ilam, mvd_c=w_mvd, dt=real(dt, wp), odt=1._wp/real(dt, wp))
+    endif
+    l_qc = merge(1, 0, l)
+    qc1d = w_qc; nc1d = w_nc; rc = w_rc; nc = w_ncv; qcten = w_qt; ncten = w_nt; ilamc = w_ilam; mvd_c = w_mvd
+  end subroutine ref_cloud_check
+
+  subroutine ref_ice_check(n, rho, l_qi, qi1d, ni1d, ri, ni, qiten, niten, ilami, dt) bind(c, name='ref_ice_check')
+    integer(c_int), value :: n
+    real(c_double), value :: dt
+    real(c_double), intent(in) :: rho(n)
+    real(c_double), intent(inout) :: qi1d(n), ni1d(n), ri(n), ni(n), qiten(n), niten(n), ilami(n)
+    integer(c_int), intent(inout) :: l_qi(n)
+    real(wp) :: w_rho(n), w_q(n), w_n(n), w_r(n), w_nv(n), w_qt(n), w_nt(n)
+    real(dp) :: w_ilam(n)
+    logical :: l(n)
+    w_rho = real(rho, wp); w_q = real(qi1d, wp); w_n = real(ni1d, wp); w_r = real(ri, wp); w_nv = real(ni, wp)
+    w_qt = real(qiten, wp); w_nt = real(niten, wp)
+    l = l_qi /= 0
+    call ice_check_and_update(w_rho, l, w_q, w_n, w_r, w_nv, w_qt, w_nt, w_ilam, real(dt, wp), 1._wp/real(dt, wp))
+    l_qi = merge(1, 0, l)
+    qi1d = w_q; ni1d = w_n; ri = w_r; ni = w_nv; qiten = w_qt; niten = w_nt; ilami = w_ilam
+  end subroutine ref_ice_check
+
+  subroutine ref_snow_check(n, rho, l_qs, qs1d, rs, qsten, dt) bind(c, name='ref_snow_check')
+    integer(c_int), value :: n
+    real(c_double), value :: dt
+    real(c_double), intent(in) :: rho(n)
+    real(c_double), intent(inout) :: qs1d(n), rs(n), qsten(n)
+    integer(c_int), intent(inout) :: l_qs(n)
+    real(wp) :: w_rho(n), w_q(n), w_r(n), w_qt(n)
+    logical :: l(n)
+    w_rho = real(rho, wp); w_q = real(qs1d, wp); w_r = real(rs, wp); w_qt = real(qsten, wp)
+    l = l_qs /= 0
+    call snow_check_and_update(w_rho, l, w_q, w_r, w_qt, real(dt, wp), 1._wp/real(dt, wp))
+    l_qs = merge(1, 0, l)
+    qs1d = w_q; rs = w_r; qsten = w_qt
+  end subroutine ref_snow_check
 
   ! Known-answer hooks into public reference routines
   ! (src/module_mp_tempo_utils.F90:27-51, 148-331).

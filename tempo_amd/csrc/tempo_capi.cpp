@@ -22,6 +22,8 @@ hipError_t launch_step_tlm(const StepArgs<Dual<double>>& a, Dual<double>* vtbuf,
 template <class T>
 hipError_t launch_dual_pack(const T* v, const T* t, const DualGeom& g, Dual<T>* out, hipStream_t s);
 template <class R>
+hipError_t launch_check_update(const CheckArgs<R>& g, int which, hipStream_t s);
+template <class R>
 hipError_t launch_surface_emissions(R* nwfa, const R* nwfa2d, int ni, int nj, long long ldk, long long ldj,
                                     long long d2ldj, int k, double dt, hipStream_t s);
 template <class T>
@@ -719,6 +721,123 @@ int tempo_hip_aerosol_surface_emissions(tempo_hip_ctx* x, double dt, void* nwfa,
   else
     return fail(TEMPO_HIP_E_ARG, "bad precision");
   return TEMPO_HIP_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// the public check routines: which = 0 cloud, 1 ice, 2 snow; p holds rho,
+// q, nn, ncsave, r, num, qten, nten, ilam, mvd (working precision)
+struct CheckPtrs {
+  const void* rho;
+  int* l;
+  void *q, *nn;
+  const void* ncsave;
+  void *r, *num, *qten, *nten, *ilam, *mvd;
+};
+
+int check_ptrs_ok(int which, int n, const CheckPtrs& p, double dt) {
+  if (n < 0 || !(dt > 0.0)) return fail(TEMPO_HIP_E_ARG, "check_and_update: bad n or dt");
+  bool ok = p.rho && p.l && p.q && p.r && p.qten;
+  if (which == 0) ok = ok && p.num && p.nten && p.ilam && p.mvd;
+  if (which == 1) ok = ok && p.nn && p.num && p.nten && p.ilam;
+  return ok ? TEMPO_HIP_OK : fail(TEMPO_HIP_E_ARG, "check_and_update: NULL array");
+}
+
+template <class R>
+int run_check(tempo_hip_ctx* x, int which, int n, const CheckPtrs& p, double dt, hipStream_t s) {
+  CheckArgs<R> g;
+  g.n = n;
+  g.dt = R(dt);
+  g.odt = sizeof(R) == 4 ? R(1.0f / float(dt)) : R(1.0 / dt);
+  g.rho = static_cast<const R*>(p.rho);
+  g.l = p.l;
+  g.q = static_cast<R*>(p.q);
+  g.nn = static_cast<R*>(p.nn);
+  g.ncsave = static_cast<const R*>(p.ncsave);
+  g.r = static_cast<R*>(p.r);
+  g.num = static_cast<R*>(p.num);
+  g.qten = static_cast<R*>(p.qten);
+  g.nten = static_cast<R*>(p.nten);
+  g.ilam = static_cast<R*>(p.ilam);
+  g.mvd = static_cast<R*>(p.mvd);
+  g.cst = static_cast<const Const<R>*>(sizeof(R) == 8 ? x->cd_dev.p : x->cf_dev.p);
+  HIP_TRY(launch_check_update(g, which, s));
+  return TEMPO_HIP_OK;
+}
+
+int check_device(tempo_hip_ctx* x, int which, int n, const CheckPtrs& p, double dt, int precision, void* stream) {
+  if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
+  int rc = check_ptrs_ok(which, n, p, dt);
+  if (rc) return rc;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if (precision == TEMPO_HIP_FP32) return run_check<float>(x, which, n, p, dt, s);
+  if (precision == TEMPO_HIP_FP64) return run_check<double>(x, which, n, p, dt, s);
+  return fail(TEMPO_HIP_E_ARG, "precision must be TEMPO_HIP_FP32 or TEMPO_HIP_FP64");
+}
+
+// host arrays: every given array staged through device memory and back
+int check_host(tempo_hip_ctx* x, int which, int n, const CheckPtrs& p, double dt) {
+  if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
+  int rc = check_ptrs_ok(which, n, p, dt);
+  if (rc || n == 0) return rc;
+  const void* host[11] = {p.rho, p.l, p.q, p.nn, p.ncsave, p.r, p.num, p.qten, p.nten, p.ilam, p.mvd};
+  void* dev[11] = {};
+  if (x->stage.size() < 11) x->stage.resize(11);
+  for (int i = 0; i < 11; ++i) {
+    if (!host[i]) continue;
+    const size_t b = size_t(n) * (i == 1 ? sizeof(int) : sizeof(double));
+    HIP_TRY(x->stage[i].ensure(b));
+    HIP_TRY(hipMemcpy(x->stage[i].p, host[i], b, hipMemcpyHostToDevice));
+    dev[i] = x->stage[i].p;
+  }
+  CheckPtrs d{dev[0], static_cast<int*>(dev[1]), dev[2], dev[3], dev[4], dev[5], dev[6], dev[7], dev[8], dev[9], dev[10]};
+  if ((rc = run_check<double>(x, which, n, d, dt, nullptr))) return rc;
+  HIP_TRY(hipDeviceSynchronize());
+  for (int i = 1; i < 11; ++i) {
+    if (!host[i] || i == 4) continue;  // rho and ncsave are inputs only
+    const size_t b = size_t(n) * (i == 1 ? sizeof(int) : sizeof(double));
+    HIP_TRY(hipMemcpy(const_cast<void*>(host[i]), dev[i], b, hipMemcpyDeviceToHost));
+  }
+  return TEMPO_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tempo_hip_cloud_check_and_update(tempo_hip_ctx* x, int n, const void* rho, int* l_qc, void* qc1d, void* nc1d,
+                                     const void* ncsave, void* rc, void* nc, void* qcten, void* ncten, void* ilamc,
+                                     void* mvd_c, double dt, int precision, void* stream) {
+  return check_device(x, 0, n, CheckPtrs{rho, l_qc, qc1d, nc1d, ncsave, rc, nc, qcten, ncten, ilamc, mvd_c}, dt,
+                      precision, stream);
+}
+int tempo_hip_ice_check_and_update(tempo_hip_ctx* x, int n, const void* rho, int* l_qi, void* qi1d, void* ni1d,
+                                   void* ri, void* ni, void* qiten, void* niten, void* ilami, double dt, int precision,
+                                   void* stream) {
+  return check_device(x, 1, n, CheckPtrs{rho, l_qi, qi1d, ni1d, nullptr, ri, ni, qiten, niten, ilami, nullptr}, dt,
+                      precision, stream);
+}
+int tempo_hip_snow_check_and_update(tempo_hip_ctx* x, int n, const void* rho, int* l_qs, void* qs1d, void* rs,
+                                    void* qsten, double dt, int precision, void* stream) {
+  return check_device(x, 2, n, CheckPtrs{rho, l_qs, qs1d, nullptr, nullptr, rs, nullptr, qsten, nullptr, nullptr, nullptr},
+                      dt, precision, stream);
+}
+int tempo_hip_cloud_check_and_update_host(tempo_hip_ctx* x, int n, const double* rho, int* l_qc, double* qc1d,
+                                          double* nc1d, const double* ncsave, double* rc, double* nc, double* qcten,
+                                          double* ncten, double* ilamc, double* mvd_c, double dt) {
+  return check_host(x, 0, n, CheckPtrs{rho, l_qc, qc1d, nc1d, ncsave, rc, nc, qcten, ncten, ilamc, mvd_c}, dt);
+}
+int tempo_hip_ice_check_and_update_host(tempo_hip_ctx* x, int n, const double* rho, int* l_qi, double* qi1d,
+                                        double* ni1d, double* ri, double* ni, double* qiten, double* niten,
+                                        double* ilami, double dt) {
+  return check_host(x, 1, n, CheckPtrs{rho, l_qi, qi1d, ni1d, nullptr, ri, ni, qiten, niten, ilami, nullptr}, dt);
+}
+int tempo_hip_snow_check_and_update_host(tempo_hip_ctx* x, int n, const double* rho, int* l_qs, double* qs1d,
+                                         double* rs, double* qsten, double dt) {
+  return check_host(x, 2, n, CheckPtrs{rho, l_qs, qs1d, nullptr, nullptr, rs, nullptr, qsten, nullptr, nullptr, nullptr},
+                    dt);
 }
 
 int tempo_hip_aerosol_surface_emissions_host(tempo_hip_ctx* x, double dt, double* nwfa, const double* nwfa2d,

@@ -1183,6 +1183,66 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
   if (a.d3[D3_RE_S]) a.d3[D3_RE_S][d3] = re_s;
 }
 
+// ------------------------------------ public check-and-update routines ----
+// cloud_check_and_update (:891-973), ice_check_and_update (:1045-1107) and
+// snow_check_and_update (:1110-1134) on n independent points, the device
+// functions the step uses; which = 0 cloud, 1 ice, 2 snow.
+template <class R>
+__global__ void __launch_bounds__(256) k_check_update(CheckArgs<R> g, int which) {
+  const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= g.n) return;
+  const Const<S_t<R>>& C = *g.cst;
+  Lvl<R> L;
+  L.rho = g.rho[i];
+  R q = g.q[i];
+  if (which == 0) {
+    R nn = g.nn ? g.nn[i] : R(0);
+    L.qcten = g.qten[i];
+    L.ncten = g.nten[i];
+    L.rc = g.r[i];
+    L.nc = g.num[i];
+    const R ncv = g.ncsave ? g.ncsave[i] : R(kNtCL);
+    cloud_check(C, L, q, nn, g.nn != nullptr, ncv, g.dt, g.odt);
+    if (g.nn) g.nn[i] = nn;
+    g.l[i] = L.l_qc ? 1 : 0;
+    g.r[i] = L.rc;
+    g.num[i] = L.nc;
+    g.qten[i] = L.qcten;
+    g.nten[i] = L.ncten;
+    g.ilam[i] = L.ilamc;
+    g.mvd[i] = L.mvd_c;
+  } else if (which == 1) {
+    R nn = g.nn[i];
+    L.qiten = g.qten[i];
+    L.niten = g.nten[i];
+    L.ri = g.r[i];
+    L.ni = g.num[i];
+    ice_check(C, L, q, nn, g.dt, g.odt);
+    g.nn[i] = nn;
+    g.l[i] = L.l_qi ? 1 : 0;
+    g.r[i] = L.ri;
+    g.num[i] = L.ni;
+    g.qten[i] = L.qiten;
+    g.nten[i] = L.niten;
+    g.ilam[i] = L.ilami;
+  } else {
+    L.qsten = g.qten[i];
+    L.rs = g.r[i];
+    snow_check(L, q, g.dt, g.odt);
+    g.l[i] = L.l_qs ? 1 : 0;
+    g.r[i] = L.rs;
+    g.qten[i] = L.qsten;
+  }
+  g.q[i] = q;
+}
+
+template <class R>
+hipError_t launch_check_update(const CheckArgs<R>& g, int which, hipStream_t s) {
+  if (g.n > 0)
+    hipLaunchKernelGGL(k_check_update<R>, dim3(unsigned((g.n + 255) / 256)), dim3(256), 0, s, g, which);
+  return hipGetLastError();
+}
+
 // -------------------------------------------- surface emissions ----
 // tempo_aerosol_surface_emissions (src/module_mp_tempo_driver.F90:516-529):
 // nwfa(i, k, j) += nwfa2d(i, j) * dt at one level, one lane per column.

@@ -219,6 +219,23 @@ struct StepArgs {
   int first_step;
 };
 
+// operands of the public cloud/ice/snow_check_and_update over n points
+// (src/module_mp_tempo_main.F90:18-20): q the mass mixing ratio, nn its
+// number (cloud, ice; nullptr = absent for cloud), r and num the per-volume
+// mass and number, l the presence flag (0/1), ilam the inverse slope, mvd
+// the cloud's median volume diameter
+template <class R>
+struct CheckArgs {
+  int n;
+  R dt, odt;
+  const R* rho;
+  int* l;
+  R *q, *nn;
+  const R* ncsave;  // cloud: one-moment numbers when nn is absent (nullptr = the land value)
+  R *r, *num, *qten, *nten, *ilam, *mvd;
+  const Const<S_t<R>>* cst;
+};
+
 // geometry of a caller array: element (i, k, j) at i + k*ldk + j*ldj,
 // column c = i + j*ni (2-D arrays: nk = 1, ldk = 0)
 struct DualGeom {

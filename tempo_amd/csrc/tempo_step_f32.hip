@@ -33,6 +33,7 @@ hipError_t launch_step(const StepArgs<float>& a, const LaunchCfg& lc, double* vt
 hipError_t launch_step_tlm(const StepArgs<Dual<float>>& a, Dual<double>* vtbuf, hipStream_t s) {
   return launch_step_tlm_impl(a, vtbuf, s);
 }
+template hipError_t launch_check_update<float>(const CheckArgs<float>&, int, hipStream_t);
 template hipError_t launch_surface_emissions<float>(float*, const float*, int, int, long long, long long, long long, int,
                                                      double, hipStream_t);
 template hipError_t launch_dual_pack<float>(const float*, const float*, const DualGeom&, Dual<float>*, hipStream_t);

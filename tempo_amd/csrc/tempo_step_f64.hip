@@ -33,6 +33,7 @@ hipError_t launch_step(const StepArgs<double>& a, const LaunchCfg& lc, double* v
 hipError_t launch_step_tlm(const StepArgs<Dual<double>>& a, Dual<double>* vtbuf, hipStream_t s) {
   return launch_step_tlm_impl(a, vtbuf, s);
 }
+template hipError_t launch_check_update<double>(const CheckArgs<double>&, int, hipStream_t);
 template hipError_t launch_surface_emissions<double>(double*, const double*, int, int, long long, long long, long long, int,
                                                      double, hipStream_t);
 template hipError_t launch_dual_pack<double>(const double*, const double*, const DualGeom&, Dual<double>*, hipStream_t);

@@ -24,6 +24,7 @@ module tempo_hip_mod
 
   public :: tempo_init, tempo_run, tempo_finalize, ty_tempo_cfgs, ty_tempo_driver_diags
   public :: tempo_aerosol_surface_emissions
+  public :: cloud_check_and_update, ice_check_and_update, snow_check_and_update
   public :: tempo_hip_cfg, tempo_hip_fields, tempo_hip_diags, tempo_hip_dims
   public :: tempo_hip_cfg_default, tempo_hip_init, tempo_hip_run, tempo_hip_run_host, tempo_hip_last_error
 
@@ -107,6 +108,27 @@ module tempo_hip_mod
       import :: c_ptr
       type(c_ptr), value :: ctx
     end subroutine
+    integer(c_int) function tempo_hip_cloud_check_and_update_host(ctx, n, rho, l_qc, qc1d, nc1d, ncsave, rc, nc, &
+        qcten, ncten, ilamc, mvd_c, dt) bind(c, name='tempo_hip_cloud_check_and_update_host')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx, rho, l_qc, qc1d, nc1d, ncsave, rc, nc, qcten, ncten, ilamc, mvd_c
+      integer(c_int), value :: n
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function tempo_hip_ice_check_and_update_host(ctx, n, rho, l_qi, qi1d, ni1d, ri, ni, qiten, &
+        niten, ilami, dt) bind(c, name='tempo_hip_ice_check_and_update_host')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx, rho, l_qi, qi1d, ni1d, ri, ni, qiten, niten, ilami
+      integer(c_int), value :: n
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function tempo_hip_snow_check_and_update_host(ctx, n, rho, l_qs, qs1d, rs, qsten, dt) &
+        bind(c, name='tempo_hip_snow_check_and_update_host')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx, rho, l_qs, qs1d, rs, qsten
+      integer(c_int), value :: n
+      real(c_double), value :: dt
+    end function
     integer(c_int) function tempo_hip_aerosol_surface_emissions_host(ctx, dt, nwfa, nwfa2d, dims, k) &
         bind(c, name='tempo_hip_aerosol_surface_emissions_host')
       import :: c_ptr, c_int, c_double, tempo_hip_dims
@@ -320,6 +342,64 @@ contains
     rc = tempo_hip_aerosol_surface_emissions_host(last_ctx, real(dt, c_double), c_loc(nwfa), c_loc(nwfa2d), &
                                                   dm, int(kts - kms, c_int))
     if (rc /= 0) call fail('tempo_aerosol_surface_emissions', rc)
+  end subroutine
+
+  !> cloud_check_and_update (src/module_mp_tempo_main.F90:891-973) with the
+  !! reference's arguments (odt unused: the library forms 1/dt), on the
+  !! context of the last tempo_init
+  subroutine cloud_check_and_update(rho, l_qc, qc1d, nc1d, ncsave, rc, nc, qcten, ncten, ilamc, mvd_c, dt, odt)
+    real(wp), intent(in) :: dt, odt
+    real(wp), dimension(:), intent(in), target :: rho
+    real(wp), dimension(:), intent(inout), target :: qc1d, qcten, ncten, rc, nc
+    real(wp), dimension(:), intent(out), target :: mvd_c, ilamc
+    real(wp), dimension(:), intent(inout), optional, target :: nc1d, ncsave
+    logical, dimension(:), intent(inout) :: l_qc
+    integer(c_int), allocatable, target :: l(:)
+    type(c_ptr) :: pn, ps
+    integer(c_int) :: rc_
+    if (.not. c_associated(last_ctx)) call fail('cloud_check_and_update (call tempo_init first)', -1)
+    allocate(l(size(qc1d)))
+    pn = c_null_ptr
+    ps = c_null_ptr
+    if (present(nc1d)) pn = c_loc(nc1d)
+    if (present(ncsave)) ps = c_loc(ncsave)
+    rc_ = tempo_hip_cloud_check_and_update_host(last_ctx, int(size(qc1d), c_int), c_loc(rho), c_loc(l), c_loc(qc1d), &
+      pn, ps, c_loc(rc), c_loc(nc), c_loc(qcten), c_loc(ncten), c_loc(ilamc), c_loc(mvd_c), real(dt, c_double))
+    if (rc_ /= 0) call fail('cloud_check_and_update', rc_)
+    l_qc = l /= 0
+  end subroutine
+
+  !> ice_check_and_update (src/module_mp_tempo_main.F90:1045-1107); odt unused
+  subroutine ice_check_and_update(rho, l_qi, qi1d, ni1d, ri, ni, qiten, niten, ilami, dt, odt)
+    real(wp), intent(in) :: dt, odt
+    real(wp), dimension(:), intent(in), target :: rho
+    real(wp), dimension(:), intent(inout), target :: qi1d, ni1d, qiten, niten, ri, ni
+    real(wp), dimension(:), intent(out), target :: ilami
+    logical, dimension(:), intent(inout) :: l_qi
+    integer(c_int), allocatable, target :: l(:)
+    integer(c_int) :: rc_
+    if (.not. c_associated(last_ctx)) call fail('ice_check_and_update (call tempo_init first)', -1)
+    allocate(l(size(qi1d)))
+    rc_ = tempo_hip_ice_check_and_update_host(last_ctx, int(size(qi1d), c_int), c_loc(rho), c_loc(l), c_loc(qi1d), &
+      c_loc(ni1d), c_loc(ri), c_loc(ni), c_loc(qiten), c_loc(niten), c_loc(ilami), real(dt, c_double))
+    if (rc_ /= 0) call fail('ice_check_and_update', rc_)
+    l_qi = l /= 0
+  end subroutine
+
+  !> snow_check_and_update (src/module_mp_tempo_main.F90:1110-1134); odt unused
+  subroutine snow_check_and_update(rho, l_qs, qs1d, rs, qsten, dt, odt)
+    real(wp), intent(in) :: dt, odt
+    real(wp), dimension(:), intent(in), target :: rho
+    real(wp), dimension(:), intent(inout), target :: qs1d, rs, qsten
+    logical, dimension(:), intent(inout) :: l_qs
+    integer(c_int), allocatable, target :: l(:)
+    integer(c_int) :: rc_
+    if (.not. c_associated(last_ctx)) call fail('snow_check_and_update (call tempo_init first)', -1)
+    allocate(l(size(qs1d)))
+    rc_ = tempo_hip_snow_check_and_update_host(last_ctx, int(size(qs1d), c_int), c_loc(rho), c_loc(l), c_loc(qs1d), &
+      c_loc(rs), c_loc(qsten), real(dt, c_double))
+    if (rc_ /= 0) call fail('snow_check_and_update', rc_)
+    l_qs = l /= 0
   end subroutine
 
   !> tempo_run (src/module_mp_tempo_driver.F90:187-513) over the tile

@@ -113,6 +113,12 @@ def lib():
         L.tempo_hip_aerosol_surface_emissions_host.argtypes = [vp, ctypes.c_double, vp, vp, ctypes.POINTER(Dims),
                                                                ctypes.c_int]
         L.tempo_hip_set_run_cfg.argtypes = [vp, ctypes.POINTER(Cfg)]
+        L.tempo_hip_cloud_check_and_update.argtypes = [vp, ctypes.c_int] + [vp] * 11 + [ctypes.c_double, ctypes.c_int, vp]
+        L.tempo_hip_ice_check_and_update.argtypes = [vp, ctypes.c_int] + [vp] * 9 + [ctypes.c_double, ctypes.c_int, vp]
+        L.tempo_hip_snow_check_and_update.argtypes = [vp, ctypes.c_int] + [vp] * 5 + [ctypes.c_double, ctypes.c_int, vp]
+        L.tempo_hip_cloud_check_and_update_host.argtypes = [vp, ctypes.c_int] + [vp] * 11 + [ctypes.c_double]
+        L.tempo_hip_ice_check_and_update_host.argtypes = [vp, ctypes.c_int] + [vp] * 9 + [ctypes.c_double]
+        L.tempo_hip_snow_check_and_update_host.argtypes = [vp, ctypes.c_int] + [vp] * 5 + [ctypes.c_double]
         L.tempo_hip_last_error.restype = ctypes.c_char_p
         L.tempo_hip_finalize.argtypes = [vp]
         L.tempo_hip_finalize.restype = None
@@ -351,6 +357,30 @@ class TempoHip:
         _check(lib().tempo_hip_run_host(self._ctx, ctypes.byref(f), ctypes.byref(d), ctypes.byref(dims),
                                         float(dt), int(itimestep)))
         return out
+
+    # argument order of the public check routines (src/module_mp_tempo_main.F90:891-1134)
+    CHECK_ARGS = {"cloud": ("rho", "l_qc", "qc1d", "nc1d", "ncsave", "rc", "nc", "qcten", "ncten", "ilamc", "mvd_c"),
+                  "ice": ("rho", "l_qi", "qi1d", "ni1d", "ri", "ni", "qiten", "niten", "ilami"),
+                  "snow": ("rho", "l_qs", "qs1d", "rs", "qsten")}
+
+    def check_and_update(self, which, arrays, dt, stream=None):
+        """cloud/ice/snow_check_and_update on n points: ``arrays`` maps the
+        argument names of CHECK_ARGS[which] to 1-D device tensors (float32 or
+        float64; l_* int32), updated in place; absent nc1d / ncsave = None.
+        numpy float64 arrays instead run on the host-array entry points."""
+        import torch
+        names = self.CHECK_ARGS[which]
+        n = len(arrays["rho"])
+        on_host = isinstance(arrays["rho"], np.ndarray)
+        ptrs = [None if arrays.get(k) is None else (arrays[k].ctypes.data if on_host else arrays[k].data_ptr())
+                for k in names]
+        fn = getattr(lib(), f"tempo_hip_{which}_check_and_update" + ("_host" if on_host else ""))
+        if on_host:
+            _check(fn(self._ctx, n, *ptrs, float(dt)))
+        else:
+            prec = FP32 if arrays["rho"].dtype == torch.float32 else FP64
+            s = stream if stream is not None else torch.cuda.current_stream(arrays["rho"].device).cuda_stream
+            _check(fn(self._ctx, n, *ptrs, float(dt), prec, ctypes.c_void_p(s)))
 
     def surface_emissions(self, nwfa, nwfa2d, dt, k=0, stream=None):
         """tempo_aerosol_surface_emissions (src/module_mp_tempo_driver.F90:516-529):

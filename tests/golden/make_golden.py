@@ -100,6 +100,43 @@ def run_case(name, state, dt, itimestep, flags, nlev=None, land=None, steps=1):
     print("wrote", name, {k: v.shape for k, v in list(inp.items())[:1]})
 
 
+def check_inputs(rng, n):
+    """points across the branches of the checks: absent, threshold-level and
+    present masses, numbers at and beyond their limits, tendencies of both
+    signs"""
+    def mass():
+        return np.where(rng.random(n) < 0.2, 0.0, 10.0 ** rng.uniform(-13, -2.5, n))
+    def number(lo, hi):
+        return np.where(rng.random(n) < 0.15, 0.0, 10.0 ** rng.uniform(lo, hi, n))
+    def ten(q):
+        return np.where(rng.random(n) < 0.5, 0.0, rng.normal(0.0, 1.0, n) * q / 60.0)
+    rho = rng.uniform(0.3, 1.3, n)
+    qc, qi, qs = mass(), mass(), mass()
+    a = dict(rho=rho, l_qc=np.zeros(n, np.int32), qc1d=qc, nc1d=number(0, 10), ncsave=10.0 ** rng.uniform(7, 9, n),
+             rc=np.zeros(n), nc=np.zeros(n), qcten=ten(qc), ncten=ten(np.full(n, 1e8)), ilamc=np.zeros(n),
+             mvd_c=np.zeros(n), l_qi=np.zeros(n, np.int32), qi1d=qi, ni1d=number(0, 7), ri=np.zeros(n),
+             ni=np.zeros(n), qiten=ten(qi), niten=ten(np.full(n, 1e5)), ilami=np.zeros(n), l_qs=np.zeros(n, np.int32),
+             qs1d=qs, rs=np.zeros(n), qsten=ten(qs))
+    return {k: np.ascontiguousarray(v) for k, v in a.items()}
+
+
+def write_check_fixture():
+    ref.init(TABLES, **BOTH)
+    rng = np.random.default_rng(31)
+    inp = check_inputs(rng, 4000)
+    dt = 20.0
+    out = {}
+    for case, which, kw in (("cloud2", "cloud", dict(has_nc=True)), ("cloud_ncsave", "cloud", dict(has_nc=False, has_ncsave=True)),
+                            ("cloud_land", "cloud", dict(has_nc=False, has_ncsave=False)), ("ice", "ice", {}),
+                            ("snow", "snow", {})):
+        a = {k: v.copy() for k, v in inp.items()}
+        ref.check_and_update(which, a, dt, **kw)
+        out.update({f"{case}__{k}": v for k, v in a.items()})
+    np.savez_compressed(os.path.join(HERE, "check_and_update.npz"), dt=np.float64(dt),
+                        **{f"in__{k}": v for k, v in inp.items()}, **out)
+    print("wrote check_and_update")
+
+
 def pick(state, idx):
     return {k: np.ascontiguousarray(v[:, idx]) for k, v in state.items()}
 
@@ -235,6 +272,9 @@ def main():
         np.savez_compressed(os.path.join(HERE, "surface_emissions.npz"), nwfa=nwfa, nwfa2d=nwfa2d, out=out,
                             dt=np.float64(20.0), kts=np.int32(2))
         print("wrote surface_emissions")
+    # the public cloud/ice/snow_check_and_update (src/module_mp_tempo_main.F90:18-20)
+    if not ONLY or "check_and_update" in ONLY:
+        write_check_fixture()
     if ONLY:
         return
 
