@@ -112,6 +112,50 @@ def adjoint_timing(steps):
             "mcl_per_s_forward_adjoint": round(ncol * nz / (fa_ms * 1e-3) / 1e6, 2)}
 
 
+def fp32_parity(ncol=8192):
+    """The FP32 path against the FP64 path on the same FP32-representable
+    inputs (one step of the bench workload's first ncol columns): relative
+    L2 of each tendency over the points whose result did not move by more
+    than rounding explains (1e-3 of the value + 1e-6 of the field maximum),
+    the fraction of points that did (branch flips), and the same for total
+    surface precipitation per column.  The FP64 path matches the reference's
+    FP64 build to 1e-12 (the parity tests)."""
+    import torch
+    from tempo_amd import generator as gen
+    from tempo_amd import host
+    ctx = host.TempoHip(aerosolaware_flag=1, hailaware_flag=0)
+    st, m = gen.generate_torch("aerosol", ncol=ncol, device="cuda", dtype=torch.float64)
+    inp = {k: v.float().double() for k, v in st.items()}
+    o64 = {k: v.clone() for k, v in inp.items()}
+    o32 = {k: v.float() for k, v in inp.items()}
+    d64 = ctx.run(o64, m["dt"], m["itimestep"])
+    d32 = ctx.run(o32, m["dt"], m["itimestep"])
+    torch.cuda.synchronize()
+
+    def cmp(a, b, a_ten=None, b_ten=None):
+        # flipped: the value moved by more than rounding explains; the error
+        # is measured on the tendency (or the value itself) elsewhere
+        flip = (a - b).abs() > 1e-3 * b.abs() + 1e-6 * b.abs().max()
+        keep = ~flip
+        x, y = (a, b) if a_ten is None else (a_ten, b_ten)
+        nrm = torch.linalg.norm(y[keep])
+        rel = float(torch.linalg.norm((x - y)[keep]) / nrm) if nrm > 0 else 0.0
+        return float(f"{rel:.3g}"), float(f"{float(flip.double().mean()):.3g}")
+
+    ten = {}
+    for k in ("t", "qv", "qc", "qr", "qi", "qs", "qg", "ni", "nr", "nc", "nwfa", "nifa"):
+        if k in inp:
+            a32 = o32[k].double()
+            ten[k] = cmp(a32, o64[k], (a32 - inp[k]) / m["dt"], (o64[k] - inp[k]) / m["dt"])
+    names = ("rain_precip", "ice_liquid_equiv_precip", "snow_liquid_equiv_precip", "graupel_liquid_equiv_precip")
+    p64 = sum(d64[n] for n in names)
+    p32 = sum(d32[n].double() for n in names)
+    # (no surface precipitation yet in the first step of this workload: None)
+    prec = cmp(p32, p64) if bool((p64 > 0).any()) else None
+    return {"sample": f"{ncol} columns of the bench workload, one step", "tendency_rel_l2_and_flipped": ten,
+            "precip_rel_l2_and_flipped": prec, "tolerance": {"tendency_rel_l2": 1e-4, "precip_rel_l2": 1e-5}}
+
+
 def ragged_graph(steps_per_graph=10, replays=3, tune=True):
     """Uneven columns: 131,072 columns of U{30..128} active levels, hail-aware,
     25 % convective; 10 steps captured in one HIP graph (torch.cuda.CUDAGraph
@@ -235,6 +279,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint timing")
     ap.add_argument("--adjoint-steps", type=int, default=3)
     ap.add_argument("--no-ragged", action="store_true", help="skip the uneven-column HIP-graph run")
+    ap.add_argument("--no-parity", action="store_true", help="skip the FP32-against-FP64 report")
     ap.add_argument("--multistep-adjoint", type=int, default=0,
                     help="also time a multi-step adjoint over this many columns (262144 for the full run)")
     a = ap.parse_args()
@@ -333,6 +378,8 @@ def main():
         "phase_ms": {PHASES[i]: round(float(phase_ms[i]), 4) for i in range(4)},
         "cpu_baseline": None,
     }
+    if a.precision == "fp32" and not a.no_parity:
+        out["fp32_parity"] = fp32_parity()
     if not a.no_adjoint:
         out["adjoint"] = adjoint_timing(a.adjoint_steps)
     if not a.no_ragged:

@@ -275,6 +275,35 @@ def test_gpu_fp32_within_tolerance(hip, big_case, gpu_tables, semi):
     assert prel[0] <= 1e-5, prel
 
 
+def test_gpu_fp32_bench_workload(hip, gpu_tables):
+    """FP32 against the FP64 twin on 2,048 columns of the benchmark's
+    aerosol-aware workload: each tendency within 1e-4 relative L2 over the
+    unflipped points, and flipped fractions within twice those of the
+    reference's own -DSINGLE_PREC build on the same inputs."""
+    from tempo_amd import generator as gen
+    from oracle import ref
+    if not ref.available("sp"):
+        pytest.skip("reference FP32 build not built")
+    st, m = gen.generate("aerosol", ncol=2048)
+    inp = {k: v.astype(np.float32).astype(np.float64) for k, v in st.items()}
+    flags = dict(aerosolaware=True, hailaware=False)
+    ctx = context(hip, flags)
+    o64, d64 = gpu_run(ctx, inp, m["dt"], m["itimestep"], dtype=torch.float64)
+    o32, d32 = gpu_run(ctx, inp, m["dt"], m["itimestep"], dtype=torch.float32)
+    inv = {v: k for k, v in D2_MAP.items()}
+    d64 = {inv[k]: v for k, v in d64.items() if k in inv}
+    d32 = {inv[k]: v for k, v in d32.items() if k in inv}
+    rep, prel = fp32_errors(inp, o32, d32, o64, d64, m["dt"])
+    ref.init(gpu_tables, variant="sp", **flags)
+    osp = {k: v.copy() for k, v in inp.items()}
+    dsp, _ = ref.run(osp, m["dt"], m["itimestep"], variant="sp")
+    rsp, psp = fp32_errors(inp, osp, dsp, o64, d64, m["dt"])
+    print("bench workload FP32 vs FP64:", rep, "reference FP32 build:", rsp)
+    for k, (rel, f) in rep.items():
+        assert rel <= 1e-4, (k, rel, rsp[k][0])
+        assert f <= 2.0 * rsp[k][1] + 1e-4, (k, f, rsp[k][1])
+
+
 def ml_columns(ncol, col0, with_bl):
     """generated columns for the cloud-number network: one-moment cloud (no
     nc) or two-moment cloud with boundary-layer cloud water and fraction"""
