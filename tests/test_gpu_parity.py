@@ -275,18 +275,20 @@ def test_gpu_fp32_within_tolerance(hip, big_case, gpu_tables, semi):
     assert prel[0] <= 1e-5, prel
 
 
-def test_gpu_fp32_bench_workload(hip, gpu_tables):
+@pytest.mark.parametrize("workload,aero", [("aerosol", True), ("plain", False)])
+def test_gpu_fp32_bench_workload(hip, gpu_tables, workload, aero):
     """FP32 against the FP64 twin on 2,048 columns of the benchmark's
-    aerosol-aware workload: each tendency within 1e-4 relative L2 over the
-    unflipped points, and flipped fractions within twice those of the
-    reference's own -DSINGLE_PREC build on the same inputs."""
+    aerosol-aware workload and of the one-moment-cloud workload: each
+    tendency within 1e-4 relative L2 over the unflipped points, and flipped
+    fractions within twice those of the reference's own -DSINGLE_PREC build
+    on the same inputs."""
     from tempo_amd import generator as gen
     from oracle import ref
     if not ref.available("sp"):
         pytest.skip("reference FP32 build not built")
-    st, m = gen.generate("aerosol", ncol=2048)
+    st, m = gen.generate(workload, ncol=2048)
     inp = {k: v.astype(np.float32).astype(np.float64) for k, v in st.items()}
-    flags = dict(aerosolaware=True, hailaware=False)
+    flags = dict(aerosolaware=aero, hailaware=False)
     ctx = context(hip, flags)
     o64, d64 = gpu_run(ctx, inp, m["dt"], m["itimestep"], dtype=torch.float64)
     o32, d32 = gpu_run(ctx, inp, m["dt"], m["itimestep"], dtype=torch.float32)
