@@ -53,12 +53,12 @@ def algorithmic_bytes(nz, ncol, width, fields_in, fields_out):
     }
 
 
-def cpu_baseline(ctx, cols, workers):
+def cpu_baseline(ctx, cols, workers, steps):
     """The reference -O2 build on the host cores, fed the GPU-built tables."""
     with tempfile.TemporaryDirectory(prefix="tempo_tables_") as d:
         ctx.write_tables(d)
         cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--tables", d, "--workload", "aerosol",
-               "--cols", str(cols)]
+               "--cols", str(cols), "--steps", str(steps)]
         if workers:
             cmd += ["--workers", str(workers)]
         r = subprocess.run(cmd, cwd=HERE, capture_output=True, text=True, timeout=600)
@@ -273,6 +273,7 @@ def main():
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
     ap.add_argument("--ncol", type=int, default=100_000)
     ap.add_argument("--cpu-cols", type=int, default=65536, help="columns in the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="consecutive steps in the CPU baseline sample")
     ap.add_argument("--cpu-workers", type=int, default=0, help="0: the host's cores (at most 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=float, default=0.0, help="run only a copy of this many GiB")
@@ -387,7 +388,7 @@ def main():
     if a.multistep_adjoint:
         out["multistep_adjoint"] = multistep_adjoint(ncol=a.multistep_adjoint)
     if not a.no_cpu_baseline:
-        cb, raw = cpu_baseline(ctx, a.cpu_cols, a.cpu_workers)
+        cb, raw = cpu_baseline(ctx, a.cpu_cols, a.cpu_workers, a.cpu_steps)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = round(value / cb["value"], 2)
         out["cpu_baseline_wall_s"] = round(raw["wall_s"], 3)

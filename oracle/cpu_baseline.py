@@ -8,7 +8,7 @@ MPI rank per tile, src/module_mp_tempo_driver.F90:400-512).  The processes
 start their timed call together behind a barrier; throughput is total
 column-levels over the slowest process's wall time.
 
-    python -m oracle.cpu_baseline --tables DIR --workload aerosol --cols N --workers C
+    python -m oracle.cpu_baseline --tables DIR --workload aerosol --cols N --workers C --steps S
 
 DIR holds the three lookup tables in the reference file format (the
 benchmark writes the GPU-built ones there).  Prints one JSON line.  The
@@ -40,7 +40,7 @@ _TABLES = None
 
 
 def _worker(args):
-    workload, col0, ncol, barrier, q = args
+    workload, col0, ncol, steps, barrier, q = args
     ref.select("o2")
     ref.init(_TABLES, **FLAGS[workload])
     st, m = gen.generate(workload, ncol=ncol, col0=col0)
@@ -49,8 +49,9 @@ def _worker(args):
     levels = st["p"].size
     barrier.wait()
     t0 = time.perf_counter()
-    ref.run(st, m["dt"], m["itimestep"])
-    q.put((levels, time.perf_counter() - t0))
+    for i in range(steps):
+        ref.run(st, m["dt"], m["itimestep"] + i)
+    q.put((levels * steps, time.perf_counter() - t0))
 
 
 def main(argv=None):
@@ -61,6 +62,7 @@ def main(argv=None):
     ap.add_argument("--cols", type=int, default=65536)
     ap.add_argument("--workers", type=int, default=0)
     ap.add_argument("--col0", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=1, help="consecutive steps of each slice's evolving state")
     a = ap.parse_args(argv)
     if not ref.available("o2"):
         raise SystemExit("reference -O2 build missing: make -C oracle ref")
@@ -71,7 +73,7 @@ def main(argv=None):
     q = ctx.Queue()
     per = [a.cols // workers + (1 if i < a.cols % workers else 0) for i in range(workers)]
     starts = [a.col0 + sum(per[:i]) for i in range(workers)]
-    procs = [ctx.Process(target=_worker, args=((a.workload, s, n, barrier, q),)) for s, n in zip(starts, per)]
+    procs = [ctx.Process(target=_worker, args=((a.workload, s, n, a.steps, barrier, q),)) for s, n in zip(starts, per)]
     for p in procs:
         p.start()
     res = [q.get() for _ in procs]
@@ -83,8 +85,9 @@ def main(argv=None):
     wall = max(r[1] for r in res)
     cpu = sum(r[1] for r in res)
     print(json.dumps(dict(value=levels / wall / 1e6, unit="Mcolumn-levels/s", cores=workers, kind="reference",
-                          sample=f"{a.workload} workload, {a.cols} columns x {levels // a.cols} levels "
-                                 f"(columns {a.col0}..{a.col0 + a.cols - 1}), one step, reference FP64 -O2 "
+                          sample=f"{a.workload} workload, {a.cols} columns x {levels // a.cols // a.steps} levels "
+                                 f"(columns {a.col0}..{a.col0 + a.cols - 1}), {a.steps} consecutive step(s), "
+                                 f"reference FP64 -O2 "
                                  f"build, {workers} processes over disjoint column slices",
                           wall_s=wall, cpu_s=cpu, per_core=levels / cpu / 1e6)))
 

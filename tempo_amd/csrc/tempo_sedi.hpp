@@ -76,22 +76,66 @@ struct SedTile {
   int row;      // LDS row of one column (sed_row(nz))
 
   __device__ double* slot(int s) const { return lds + s * (kSedCols * row); }
-  // scratch planes: the block's column group is one contiguous run
-  template <class T>
-  __device__ void load(int s, const T* plane) const {
-    double* d = slot(s);
-    const T* src = plane + size_t(c0) * nzs;
-    for (int e = threadIdx.x; e < kSedCols * nz; e += blockDim.x) {
-      const int cc = e % kSedCols, k = e / kSedCols;
-      d[cc * row + k] = double(src[e]);
+  // Several scratch planes of one stage (the block's column group of each is
+  // one contiguous run).  Every global load of the stage is issued into
+  // registers before the first LDS write, so the stage waits for one memory
+  // latency instead of one per plane and pass (a load-convert-write loop per
+  // plane waits for each load before its write).  The block has
+  // 64 * kSedCols threads and nz <= kSedLevels, so kSedChunks passes cover
+  // the column group; a pass runs only when the group reaches it (a
+  // block-uniform test), and its elements past the group's end load the last
+  // one (the index is clamped, not branched on per lane, so that the loads
+  // issue back to back) and are not written.
+  template <class T, int N>
+  __device__ void load_many(const int (&s)[N], const T* const (&plane)[N]) const {
+    T v[kSedChunks][N];
+    const int n = kSedCols * nz;
+    const size_t base = size_t(c0) * nzs;
+#pragma unroll
+    for (int it = 0; it < kSedChunks; ++it) {
+      if (it * 64 * kSedCols >= n) break;
+      const int e = min(int(threadIdx.x) + it * 64 * kSedCols, n - 1);
+#pragma unroll
+      for (int p = 0; p < N; ++p) v[it][p] = plane[p][base + e];
+    }
+#pragma unroll
+    for (int it = 0; it < kSedChunks; ++it) {
+      const int e = int(threadIdx.x) + it * 64 * kSedCols;
+      if (e < n) {
+        const int cc = e % kSedCols, k = e / kSedCols;
+#pragma unroll
+        for (int p = 0; p < N; ++p) slot(s[p])[cc * row + k] = double(v[it][p]);
+      }
     }
   }
+  // the first stage in the same way: per-point flags (slot 0), layer depths
+  // from the state (slot 1), density (slot 2) and rain mass (slot 3)
   template <class R>
-  __device__ void load_state(int s, const StepArgs<R>& a, const R* f) const {
-    double* d = slot(s);
-    for (int e = threadIdx.x; e < kSedCols * nz; e += blockDim.x) {
-      const int cc = e % kSedCols, k = e / kSedCols, c = c0 + cc;
-      d[cc * row + k] = (c < ncol) ? double(f[TEMPO_IDX_STATE(a, c, k)]) : 0.0;
+  __device__ void load_head(const StepArgs<R>& a, const unsigned* pf, const R* rho, const R* rr) const {
+    unsigned vf[kSedChunks];
+    R vz[kSedChunks], vh[kSedChunks], vr[kSedChunks];
+    const int n = kSedCols * nz;
+    const size_t base = size_t(c0) * nzs;
+#pragma unroll
+    for (int it = 0; it < kSedChunks; ++it) {
+      if (it * 64 * kSedCols >= n) break;
+      const int e = min(int(threadIdx.x) + it * 64 * kSedCols, n - 1);
+      const int cc = e % kSedCols, k = e / kSedCols, c = min(c0 + cc, ncol - 1);
+      vf[it] = pf[base + e];
+      vh[it] = rho[base + e];
+      vr[it] = rr[base + e];
+      vz[it] = a.f[F_DZ][TEMPO_IDX_STATE(a, c, k)];
+    }
+#pragma unroll
+    for (int it = 0; it < kSedChunks; ++it) {
+      const int e = int(threadIdx.x) + it * 64 * kSedCols;
+      if (e < n) {
+        const int cc = e % kSedCols, k = e / kSedCols, x = cc * row + k;
+        slot(0)[x] = double(vf[it]);
+        slot(1)[x] = c0 + cc < ncol ? double(vz[it]) : 0.0;
+        slot(2)[x] = double(vh[it]);
+        slot(3)[x] = double(vr[it]);
+      }
     }
   }
   template <class T>
@@ -225,10 +269,7 @@ __global__ void __launch_bounds__(64 * kSedCols) __attribute__((amdgpu_waves_per
 
   // column flags (which species hold mass) and the level geometry
   if (threadIdx.x == 0) blk_any = 0;
-  T.load(0, a.pf);
-  T.load_state(1, a, a.f[F_DZ]);
-  T.load(2, sr + SR_RHO * plane);
-  T.load(3, sd + SD_RR * plane);
+  T.load_head(a, a.pf, sr + SR_RHO * plane, sd + SD_RR * plane);
   __syncthreads();
   unsigned pfl[kSedChunks];
   R dz[kSedChunks], rho[kSedChunks];
@@ -257,11 +298,8 @@ __global__ void __launch_bounds__(64 * kSedCols) __attribute__((amdgpu_waves_per
   if (bany & PF_QR) {
     double xn[kSedChunks], tr[kSedChunks], tn[kSedChunks], v2[kSedChunks];
     R raw1[kSedChunks], raw2[kSedChunks];
-    T.load(0, sd + SD_NR * plane);
-    T.load(1, sd + SD_QRTEN * plane);
-    T.load(2, sd + SD_NRTEN * plane);
-    T.load(4, sr + SR_VTRR * plane);
-    T.load(5, sr + SR_VTNR * plane);
+    T.load_many<R, 5>({0, 1, 2, 4, 5}, {sd + SD_NR * plane, sd + SD_QRTEN * plane, sd + SD_NRTEN * plane,
+                                        sr + SR_VTRR * plane, sr + SR_VTNR * plane});
     __syncthreads();
     T.get(0, w, xn);
     T.get(1, w, tr);
@@ -291,14 +329,9 @@ __global__ void __launch_bounds__(64 * kSedCols) __attribute__((amdgpu_waves_per
     double xg[kSedChunks], xn[kSedChunks], xb[kSedChunks], tg[kSedChunks], tn[kSedChunks], tb[kSedChunks];
     double v1[kSedChunks], v2[kSedChunks];
     R raw1[kSedChunks], raw2[kSedChunks];
-    T.load(0, sd + SD_RG * plane);
-    T.load(1, sd + SD_NG * plane);
-    T.load(2, sd + SD_RB * plane);
-    T.load(4, sd + SD_QGTEN * plane);
-    T.load(5, sd + SD_NGTEN * plane);
-    T.load(6, sd + SD_QBTEN * plane);
-    T.load(7, sr + SR_VTRG * plane);
-    T.load(3, sr + SR_VTNG * plane);
+    T.load_many<R, 8>({0, 1, 2, 4, 5, 6, 7, 3},
+                      {sd + SD_RG * plane, sd + SD_NG * plane, sd + SD_RB * plane, sd + SD_QGTEN * plane,
+                       sd + SD_NGTEN * plane, sd + SD_QBTEN * plane, sr + SR_VTRG * plane, sr + SR_VTNG * plane});
     __syncthreads();
     T.get(0, w, xg);
     T.get(1, w, xn);
@@ -335,9 +368,7 @@ __global__ void __launch_bounds__(64 * kSedCols) __attribute__((amdgpu_waves_per
   if (bany & PF_QS) {
     double xs[kSedChunks], ts[kSedChunks], v1[kSedChunks];
     R raw[kSedChunks];
-    T.load(0, sd + SD_RS * plane);
-    T.load(1, sd + SD_QSTEN * plane);
-    T.load(2, sr + SR_VTS * plane);
+    T.load_many<R, 3>({0, 1, 2}, {sd + SD_RS * plane, sd + SD_QSTEN * plane, sr + SR_VTS * plane});
     __syncthreads();
     T.get(0, w, xs);
     T.get(1, w, ts);
@@ -370,12 +401,8 @@ __global__ void __launch_bounds__(64 * kSedCols) __attribute__((amdgpu_waves_per
   if (bany & PF_QI) {
     double xi[kSedChunks], xn[kSedChunks], ti[kSedChunks], tn[kSedChunks], v1[kSedChunks], v2[kSedChunks];
     R raw1[kSedChunks], raw2[kSedChunks];
-    T.load(0, sd + SD_RI * plane);
-    T.load(1, sd + SD_NI * plane);
-    T.load(2, sd + SD_QITEN * plane);
-    T.load(4, sd + SD_NITEN * plane);
-    T.load(5, sr + SR_VTRI * plane);
-    T.load(6, sr + SR_VTNI * plane);
+    T.load_many<R, 6>({0, 1, 2, 4, 5, 6}, {sd + SD_RI * plane, sd + SD_NI * plane, sd + SD_QITEN * plane,
+                                           sd + SD_NITEN * plane, sr + SR_VTRI * plane, sr + SR_VTNI * plane});
     __syncthreads();
     T.get(0, w, xi);
     T.get(1, w, xn);
@@ -403,12 +430,8 @@ __global__ void __launch_bounds__(64 * kSedCols) __attribute__((amdgpu_waves_per
   if (bany & PF_QC) {
     double xc[kSedChunks], xn[kSedChunks], tc[kSedChunks], tn[kSedChunks], v1[kSedChunks], v2[kSedChunks];
     R raw1[kSedChunks], raw2[kSedChunks];
-    T.load(0, sd + SD_RC * plane);
-    T.load(1, sd + SD_NC * plane);
-    T.load(2, sd + SD_QCTEN * plane);
-    T.load(4, sd + SD_NCTEN * plane);
-    T.load(5, sr + SR_VTRC * plane);
-    T.load(6, sr + SR_VTNC * plane);
+    T.load_many<R, 6>({0, 1, 2, 4, 5, 6}, {sd + SD_RC * plane, sd + SD_NC * plane, sd + SD_QCTEN * plane,
+                                           sd + SD_NCTEN * plane, sr + SR_VTRC * plane, sr + SR_VTNC * plane});
     __syncthreads();
     T.get(0, w, xc);
     T.get(1, w, xn);

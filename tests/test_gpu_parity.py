@@ -562,12 +562,14 @@ def test_gpu_launch_shapes_give_identical_results(hip, dtype):
     """Every tuning setting (sedimentation kernel, block sizes, point order)
     produces the same bits: the wavefront-per-column sedimentation performs
     the serial kernel's arithmetic level by level, and the point order only
-    regroups independent points into wavefronts."""
+    regroups independent points into wavefronts.  The 64-level case fills
+    exactly one staging pass of the sedimentation kernel's blocks."""
     from tempo_amd import generator as gen
     tdt = getattr(torch, dtype)
     prec = hip.FP32 if dtype == "float32" else hip.FP64
     for wl, ncol, flags in (("ragged", 300, dict(aerosolaware=False, hailaware=True)),
-                            ("multistep", 257, dict(aerosolaware=True, hailaware=True))):
+                            ("multistep", 257, dict(aerosolaware=True, hailaware=True)),
+                            ("adjoint", 256, dict(aerosolaware=True, hailaware=False))):
         st, m = gen.generate(wl, ncol=ncol, col0=123)
         ctx = context(hip, flags)
         nl = None if "nlev" not in m else torch.tensor(m["nlev"], dtype=torch.int32, device="cuda")
