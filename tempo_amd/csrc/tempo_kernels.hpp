@@ -450,20 +450,26 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_A k_phase_a(StepArgs<R> a) {
 }
 
 // ----------------------------------------------------------- phase B ----
-// flux-form sedimentation of one field over one substep (:1516-1559), FP64
+// flux-form sedimentation of one field over one substep (:1516-1559), FP64.
+// xr, vt and xten are distinct planes (and distinct from the density and the
+// layer depths), so the loads of several levels may be issued ahead of the
+// stores of the level being updated.
 template <class R>
-__device__ void sediment(const StepArgs<R>& a, int c, int nzc, W_t<R>* xr, const W_t<R>* vt, W_t<R>* xten,
-                         double limit, int steps, int ktop, W_t<R>* precip) {
+__device__ void sediment(const StepArgs<R>& a, int c, int nzc, W_t<R>* __restrict__ xr,
+                         const W_t<R>* __restrict__ vt, W_t<R>* __restrict__ xten, double limit, int steps, int ktop,
+                         W_t<R>* precip) {
   using W = W_t<R>;
   const size_t plane = a.plane;
-  const R* rho = a.sr + SR_RHO * plane;
+  const R* __restrict__ rho = a.sr + SR_RHO * plane;
+  const R* __restrict__ dzf = a.f[F_DZ];
   const double ost = 1.0 / double(steps), dt = a.dt;
   W above = W(0.0), s0 = W(0.0);
+#pragma unroll 4
   for (int k = nzc - 1; k >= 0; --k) {
     const size_t s = TEMPO_IDX_SCR(a, c, k);
     const W sed = vt[s] * xr[s];
     if (k == nzc - 1 || k <= ktop - 1) {
-      const W odz = 1.0 / W(a.f[F_DZ][TEMPO_IDX_STATE(a, c, k)]);
+      const W odz = 1.0 / W(dzf[TEMPO_IDX_STATE(a, c, k)]);
       const W orho = 1.0 / W(rho[s]);
       const W flux = (k == nzc - 1) ? -sed : (above - sed);
       xten[s] = xten[s] + flux * odz * ost * orho;
@@ -480,13 +486,15 @@ __device__ void sediment(const StepArgs<R>& a, int c, int nzc, W_t<R>* xr, const
 // (:1768-1789 and the species variants); vt raw values are valid where the
 // mass exceeds r1.  Writes the filled profile to out (FP64).
 template <class R>
-__device__ void fill_fallspeed(const StepArgs<R>& a, int c, int nzc, const W_t<R>* mass, const R* raw1, const R* raw2,
-                               W_t<R>* out1, W_t<R>* out2, bool use_max, int& substeps, int& ktop) {
+__device__ void fill_fallspeed(const StepArgs<R>& a, int c, int nzc, const W_t<R>* __restrict__ mass,
+                               const R* __restrict__ raw1, const R* __restrict__ raw2, W_t<R>* __restrict__ out1,
+                               W_t<R>* __restrict__ out2, bool use_max, int& substeps, int& ktop) {
   using W = W_t<R>;
   const double dt = a.dt;
   W v1 = W(0.0), v2 = W(0.0);
   substeps = 1;
   ktop = 1;
+#pragma unroll 4
   for (int k = nzc - 1; k >= 0; --k) {
     const size_t s = TEMPO_IDX_SCR(a, c, k);
     if (mass[s] > kR1) {
@@ -753,6 +761,7 @@ __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, W_t<R>* vtbuf) {
     W* sd = narrow ? vtbuf + 3 * plane : reinterpret_cast<W*>(a.sd);
     if constexpr (narrow)
       for (int q = 0; q < SD_N; ++q)
+#pragma unroll 8
         for (int k = 0; k < nzc; ++k) {
           const size_t s = size_t(q) * plane + TEMPO_IDX_SCR(a, c, k);
           sd[s] = W(a.sd[s]);
@@ -864,6 +873,7 @@ __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, W_t<R>* vtbuf) {
     }
     if constexpr (narrow)  // the tendencies, rounded once to the working precision
       for (int q = SD_QRTEN; q < SD_N; ++q)
+#pragma unroll 8
         for (int k = 0; k < nzc; ++k) {
           const size_t s = size_t(q) * plane + TEMPO_IDX_SCR(a, c, k);
           a.sd[s] = Sd_t<R>(sd[s]);
