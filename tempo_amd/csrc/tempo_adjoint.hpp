@@ -110,6 +110,14 @@ __global__ void __launch_bounds__(256) k_adj_seed_ten(StepArgs<D> a, int m, int 
   a.sd[size_t(SD_QRTEN + m) * a.plane + s].d = 1;
 }
 
+// out[0, n) = 0 (a kernel rather than a memset, so that the sequence may be
+// captured in a graph)
+template <class T>
+__global__ void __launch_bounds__(256) k_adj_zero(T* __restrict__ out, size_t n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = T(0);
+}
+
 // out[e] (+)= sum over outputs of ybar * tangent
 template <class D>
 __global__ void __launch_bounds__(256) k_adj_contract(StepArgs<D> a, AdjArgs g, double* out, int accumulate) {

@@ -481,11 +481,21 @@ hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* 
   hipLaunchKernelGGL(k_phase_b<D>, dim3(unsigned((a.ncol + 127) / 128)), dim3(128), 0, s, a, vtbuf);
   if ((e = cp(snap.sd_b, a.sd, sd_bytes)) != hipSuccess) return e;
   // passes seeded on the post-sedimentation tendencies (the final update
-  // reads the inputs, a.fa and the scratch, and writes only a.fo)
+  // reads the inputs, a.fa and the scratch, and writes only a.fo).  Without
+  // prognostic graupel number and volume the final update does not read
+  // their tendencies, so those cotangents are zero and need no pass.
+  const bool ngqb = a.f[F_NG] && a.f[F_QB];
+  int m_prev = -1;
   for (int m = 0; m < 10; ++m) {
-    hipLaunchKernelGGL(k_adj_seed_ten<D>, g2, dim3(256), 0, s, a, m, m - 1);
+    double* ub = g.ubar + size_t(m) * n3;
+    if (!ngqb && (SD_QRTEN + m == SD_NGTEN || SD_QRTEN + m == SD_QBTEN)) {
+      hipLaunchKernelGGL(k_adj_zero<double>, dim3(unsigned((n3 + 255) / 256)), dim3(256), 0, s, ub, n3);
+      continue;
+    }
+    hipLaunchKernelGGL(k_adj_seed_ten<D>, g2, dim3(256), 0, s, a, m, m_prev);
     phase_c();
-    hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, a, g, g.ubar + size_t(m) * n3, 0);
+    hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, a, g, ub, 0);
+    m_prev = m;
   }
   // S^T on the values after phase A
   if ((e = cp(a.sr, snap.sr_a, sr_bytes)) != hipSuccess) return e;
