@@ -132,6 +132,21 @@ int tempo_hip_reserve(tempo_hip_ctx *ctx, int ncol, int nz);
 int tempo_hip_run(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_diags *d,
                   const tempo_hip_dims *dims, double dt, int itimestep, int precision, void *stream);
 
+/* Branch accounting (no counterpart in the reference): the step of
+ * tempo_hip_run, bit for bit, that also records per point a 32-bit hash of
+ * the threshold decisions its level-local physics took -- every comparison
+ * of an `if` on a value, every table index, nint and bin count, the presence
+ * flags, clamps and density index of the checks.  sig (device, 2*nz*ni*nj
+ * words): sig[p*nz*ncol + k*ncol + c] for point (column c = i + j*ni, level
+ * k), p = 0 the physics up to sedimentation (src/module_mp_tempo_main.F90:
+ * 320-588), p = 1 the final update (:729-863); levels k >= nlev[c] are not
+ * written.  Two runs took the same branches at a point where its two words
+ * agree; the sedimentation's substep counts and tops follow from
+ * tempo_hip_substeps. */
+int tempo_hip_branch_signature(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_diags *d,
+                               const tempo_hip_dims *dims, double dt, int itimestep, int precision, unsigned *sig,
+                               void *stream);
+
 /* As tempo_hip_run with host arrays (FP64, Fortran layout of `dims`): copies
  * to the device, runs and copies back; for hosts that keep state on the CPU.
  * land and nlev are host pointers here. */

@@ -6,8 +6,10 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
+#include "tempo_capi_internal.hpp"
 #include "tempo_host.hpp"
 #include "tempo_step.hpp"
 #include "tempo_adjoint.hpp"
@@ -17,6 +19,8 @@ namespace tempo {
 // defined in tempo_step_f32.hip / tempo_step_f64.hip / tempo_tables.hip
 hipError_t launch_step(const StepArgs<float>& a, const LaunchCfg& lc, double* vtbuf, hipStream_t s, hipEvent_t* ev);
 hipError_t launch_step(const StepArgs<double>& a, const LaunchCfg& lc, double* vtbuf, hipStream_t s, hipEvent_t* ev);
+hipError_t launch_step_sig(const StepArgs<float>& a, const LaunchCfg& lc, double* vtbuf, hipStream_t s);
+hipError_t launch_step_sig(const StepArgs<double>& a, const LaunchCfg& lc, double* vtbuf, hipStream_t s);
 hipError_t launch_step_tlm(const StepArgs<Dual<float>>& a, Dual<double>* vtbuf, hipStream_t s);
 hipError_t launch_step_tlm(const StepArgs<Dual<double>>& a, Dual<double>* vtbuf, hipStream_t s);
 template <class T>
@@ -61,9 +65,28 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(TEMPO_HIP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Owns one device allocation.  Move-only: a copy would free the same pointer
+// twice (containers of DevBuf move their elements when they grow).
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipFree(p);
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
   ~DevBuf() {
     if (p) (void)hipFree(p);
   }
@@ -82,6 +105,8 @@ struct DevBuf {
   template <class T>
   T* as() const { return static_cast<T*>(p); }
 };
+static_assert(!std::is_copy_constructible<DevBuf>::value && std::is_nothrow_move_constructible<DevBuf>::value,
+              "DevBuf owns its allocation");
 
 template <class T>
 hipError_t upload(DevBuf& b, const std::vector<T>& v) {
@@ -150,6 +175,8 @@ struct tempo_hip_ctx {
   DevBuf adj_in, adj_out, adj_mem;  // adjoint: packed inputs, new state, cotangent/Jacobian planes
   DevBuf adj_snap;                  // adjoint: scratch after phases A and B
   int last_ncol = 0;
+  void* last_ksub = nullptr;  // substep counts of the last run (its scratch)
+  void* last_vt = nullptr;    // its wide sedimentation planes
   // host-array staging
   std::vector<DevBuf> stage;
   DevBuf stage_land, stage_nlev;
@@ -328,29 +355,57 @@ int pack_tables(tempo_hip_ctx* x) {
 
 size_t scratch_plane(size_t ncol, size_t nz) { return (ncol + kScrW - 1) / kScrW * kScrW * size_t(scratch_levels(int(nz))); }
 
-// per-point scratch for ncol columns of nz levels; br / bw / bd: bytes of
-// one working-precision / wide / sedimented-state value (8, 8, 4 or 8 for the
-// plain step, 16 for duals); vt_planes: wide planes of the one-lane-per-column
-// sedimentation (3 fall-speed planes, plus a wide copy of the SD_N planes
-// when these are stored narrower).  The defaults cover every step.
+// Byte sizes of the per-point scratch of one step of ncol columns of nz
+// levels; br / bw / bd: bytes of one working-precision / wide /
+// sedimented-state value (8, 8, 4 or 8 for the plain step, 16 for duals);
+// vt_planes: wide planes of the one-lane-per-column sedimentation (3
+// fall-speed planes, plus a wide copy of the SD_N planes when these are
+// stored narrower); semi: semi-Lagrangian work arrays.
+struct ScratchSizes {
+  size_t sr, sd, pf, vt, cf, kmelt, precip, ksub, semiw;
+  ScratchSizes(size_t ncol, size_t nz, size_t br, size_t bw, size_t bd, size_t vt_planes, bool semi) {
+    const size_t plane = scratch_plane(ncol, nz);
+    sr = SR_N * plane * br;
+    sd = SD_N * plane * bd;
+    pf = plane * sizeof(unsigned);
+    vt = vt_planes * plane * bw;
+    cf = ncol * sizeof(unsigned);
+    kmelt = ncol * sizeof(int);
+    precip = 5 * ncol * bw;
+    ksub = 10 * ncol * sizeof(int);
+    semiw = semi ? kSemiArrays * (nz + 2) * ncol * bw : 0;
+  }
+  // the buffers carved in this order from one block, each 256-byte aligned
+  static size_t up(size_t b) { return (b + 255) / 256 * 256; }
+  size_t total() const {
+    return up(sr) + up(sd) + up(pf) + up(vt) + up(cf) + up(kmelt) + up(precip) + up(ksub) + up(semiw);
+  }
+};
+
+// where one step's scratch lives
+struct ScratchPtrs {
+  void *sr, *sd, *pf, *vt, *cf, *kmelt, *precip, *ksub, *semiw;
+};
+
+// the context's own scratch, grown to the sizes (the defaults cover every step)
 int ensure_scratch(tempo_hip_ctx* x, size_t ncol, size_t nz, size_t br = 8, size_t bw = 8, size_t bd = 8,
                    size_t vt_planes = 3 + SD_N) {
-  const size_t plane = scratch_plane(ncol, nz);
-  HIP_TRY(x->sr.ensure(SR_N * plane * br));
-  HIP_TRY(x->sd.ensure(SD_N * plane * bd));
-  HIP_TRY(x->pf.ensure(plane * sizeof(unsigned)));
-  HIP_TRY(x->vtbuf.ensure(vt_planes * plane * bw));
-  HIP_TRY(x->cf_.ensure(ncol * sizeof(unsigned)));
-  HIP_TRY(x->kmelt.ensure(ncol * sizeof(int)));
-  HIP_TRY(x->precip.ensure(5 * ncol * bw));
-  HIP_TRY(x->ksub.ensure(10 * ncol * sizeof(int)));
-  if (x->cfg.semi_sedi_flag) HIP_TRY(x->semiw.ensure(kSemiArrays * (nz + 2) * ncol * bw));
+  const ScratchSizes z(ncol, nz, br, bw, bd, vt_planes, x->cfg.semi_sedi_flag != 0);
+  HIP_TRY(x->sr.ensure(z.sr));
+  HIP_TRY(x->sd.ensure(z.sd));
+  HIP_TRY(x->pf.ensure(z.pf));
+  HIP_TRY(x->vtbuf.ensure(z.vt));
+  HIP_TRY(x->cf_.ensure(z.cf));
+  HIP_TRY(x->kmelt.ensure(z.kmelt));
+  HIP_TRY(x->precip.ensure(z.precip));
+  HIP_TRY(x->ksub.ensure(z.ksub));
+  if (z.semiw) HIP_TRY(x->semiw.ensure(z.semiw));
   return TEMPO_HIP_OK;
 }
 
-// scratch, tables, constants and flags of one launch (a.ncol, a.nz set)
+// the scratch sizes of one step of a in the context's configuration
 template <class R>
-int bind_common(tempo_hip_ctx* x, StepArgs<R>& a, double dt, int itimestep) {
+ScratchSizes step_scratch(const tempo_hip_ctx* x, const StepArgs<R>& a) {
   using S = S_t<R>;
   using W = W_t<R>;
   using D = Sd_t<R>;
@@ -359,17 +414,44 @@ int bind_common(tempo_hip_ctx* x, StepArgs<R>& a, double dt, int itimestep) {
   const LaunchCfg& lc = x->launch[sizeof(S) == 8 ? 1 : 0];
   const bool lane_sed = !lc.sed_wave || a.nz > kSedLevels || x->cfg.semi_sedi_flag;
   const size_t vt_planes = 3 + (sizeof(D) < sizeof(W) && lane_sed ? SD_N : 0);
-  int rc = ensure_scratch(x, size_t(a.ncol), size_t(a.nz), sizeof(R), sizeof(W), sizeof(D), vt_planes);
-  if (rc) return rc;
-  a.sr = x->sr.as<R>();
+  return ScratchSizes(size_t(a.ncol), size_t(a.nz), sizeof(R), sizeof(W), sizeof(D), vt_planes,
+                      x->cfg.semi_sedi_flag != 0);
+}
+
+// scratch, tables, constants and flags of one launch (a.ncol, a.nz set);
+// sc: the scratch to use (NULL: the context's own, grown as needed)
+template <class R>
+int bind_common(tempo_hip_ctx* x, StepArgs<R>& a, double dt, int itimestep, const ScratchPtrs* sc = nullptr) {
+  using S = S_t<R>;
+  using W = W_t<R>;
+  using D = Sd_t<R>;
+  ScratchPtrs own;
+  if (!sc) {
+    const ScratchSizes z = step_scratch(x, a);
+    HIP_TRY(x->sr.ensure(z.sr));
+    HIP_TRY(x->sd.ensure(z.sd));
+    HIP_TRY(x->pf.ensure(z.pf));
+    HIP_TRY(x->vtbuf.ensure(z.vt));
+    HIP_TRY(x->cf_.ensure(z.cf));
+    HIP_TRY(x->kmelt.ensure(z.kmelt));
+    HIP_TRY(x->precip.ensure(z.precip));
+    HIP_TRY(x->ksub.ensure(z.ksub));
+    if (z.semiw) HIP_TRY(x->semiw.ensure(z.semiw));
+    own = ScratchPtrs{x->sr.p, x->sd.p, x->pf.p, x->vtbuf.p, x->cf_.p, x->kmelt.p, x->precip.p, x->ksub.p,
+                      z.semiw ? x->semiw.p : nullptr};
+    sc = &own;
+  }
+  a.sr = static_cast<R*>(sc->sr);
   a.plane = scratch_plane(size_t(a.ncol), size_t(a.nz));
   a.nzs = scratch_levels(a.nz);
-  a.sd = x->sd.as<D>();
-  a.pf = x->pf.as<unsigned>();
-  a.cf = x->cf_.as<unsigned>();
-  a.precip = x->precip.as<W>();
-  a.ksub = x->ksub.as<int>();
+  a.sd = static_cast<D*>(sc->sd);
+  a.pf = static_cast<unsigned*>(sc->pf);
+  a.cf = static_cast<unsigned*>(sc->cf);
+  a.precip = static_cast<W*>(sc->precip);
+  a.ksub = static_cast<int*>(sc->ksub);
   x->last_ncol = a.ncol;
+  x->last_ksub = sc->ksub;
+  x->last_vt = sc->vt;
   const bool r64 = sizeof(S) == 8;
   a.compact = x->launch[r64 ? 1 : 0].order;
   a.tab.qg = static_cast<const S*>(r64 ? x->qg64.p : x->qg32.p);
@@ -393,8 +475,8 @@ int bind_common(tempo_hip_ctx* x, StepArgs<R>& a, double dt, int itimestep) {
   a.fl.semi_sedi = c.semi_sedi_flag;
   a.fl.refl_from_melting = c.refl10cm_from_melting_flag;
   a.radar = x->radar.as<Radar>();
-  a.kmelt = x->kmelt.as<int>();
-  a.semiw = c.semi_sedi_flag ? x->semiw.as<W>() : nullptr;
+  a.kmelt = static_cast<int*>(sc->kmelt);
+  a.semiw = c.semi_sedi_flag ? static_cast<W*>(sc->semiw) : nullptr;
   a.fl.cloud_condensation = c.cloud_condensation_flag;
   a.fl.refl10cm = c.refl10cm_flag;
   a.fl.re_cloud = c.re_cloud_flag;
@@ -429,9 +511,12 @@ void diag_ptrs(const tempo_hip_diags* d, void* (&d2)[D2_N], void* (&d3)[D3_N]) {
   for (int i = 0; i < D3_N; ++i) d3[i] = v3[i];
 }
 
+// sig (device, may be NULL): record the level-local kernels' branch
+// signatures there (a signature run; no per-phase timing).  sc: scratch to
+// use instead of the context's own (no allocation, no per-phase timing).
 template <class R>
 int run_impl(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags* d, const tempo_hip_dims* dm,
-             double dt, int itimestep, hipStream_t s) {
+             double dt, int itimestep, hipStream_t s, unsigned* sig = nullptr, const ScratchPtrs* sc = nullptr) {
   StepArgs<R> a;
   std::memset(&a, 0, sizeof(a));
   void* fp[F_NFIELDS];
@@ -458,10 +543,16 @@ int run_impl(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags*
   a.dldk = dm->dldk;
   a.dldj = dm->dldj;
   a.d2ldj = dm->d2ldj;
-  int rc = bind_common(x, a, dt, itimestep);
+  int rc = bind_common(x, a, dt, itimestep, sc);
   if (rc) return rc;
+  double* vt = static_cast<double*>(x->last_vt);
+  if (sig) {
+    a.sig = sig;
+    HIP_TRY(launch_step_sig(a, x->launch[sizeof(R) == 8 ? 1 : 0], vt, s));
+    return TEMPO_HIP_OK;
+  }
   hipEvent_t* ev = nullptr;
-  if (x->profile) {
+  if (x->profile && !sc) {
     for (int i = 0; i < 5; ++i) {
       hipEvent_t e;
       HIP_TRY(hipEventCreate(&e));
@@ -469,7 +560,7 @@ int run_impl(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags*
     }
     ev = &x->events[x->events.size() - 5];
   }
-  HIP_TRY(launch_step(a, x->launch[sizeof(R) == 8 ? 1 : 0], x->vtbuf.as<double>(), s, ev));
+  HIP_TRY(launch_step(a, x->launch[sizeof(R) == 8 ? 1 : 0], vt, s, ev));
   return TEMPO_HIP_OK;
 }
 
@@ -697,6 +788,18 @@ int tempo_hip_run(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_d
   return fail(TEMPO_HIP_E_ARG, "precision must be TEMPO_HIP_FP32 or TEMPO_HIP_FP64");
 }
 
+int tempo_hip_branch_signature(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags* d,
+                               const tempo_hip_dims* dm, double dt, int itimestep, int precision, unsigned* sig,
+                               void* stream) {
+  if (!x || !sig) return fail(TEMPO_HIP_E_ARG, "NULL context or signature array");
+  int rc = check_run_args(f, dm, dt);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (precision == TEMPO_HIP_FP32) return run_impl<float>(x, f, d, dm, dt, itimestep, s, sig);
+  if (precision == TEMPO_HIP_FP64) return run_impl<double>(x, f, d, dm, dt, itimestep, s, sig);
+  return fail(TEMPO_HIP_E_ARG, "precision must be TEMPO_HIP_FP32 or TEMPO_HIP_FP64");
+}
+
 static int check_emission_args(const void* nwfa, const void* nwfa2d, const tempo_hip_dims* dm, int k) {
   if (!nwfa || !nwfa2d || !dm) return fail(TEMPO_HIP_E_ARG, "NULL nwfa, nwfa2d or dims");
   if (dm->ni <= 0 || dm->nj <= 0 || dm->nz <= 0 || k < 0 || k >= dm->nz)
@@ -892,10 +995,14 @@ int tempo_hip_run_host(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_
     HIP_TRY(hipMemcpy(x->stage[i].p, hf[i], n3 * sizeof(double), hipMemcpyHostToDevice));
     *dfp[i] = x->stage[i].p;
   }
+  // diagnostics: the caller's values go in first, so that points the step
+  // does not write (halos and padding between rows, levels k >= nlev[c])
+  // come back unchanged
   for (int i = 0; d && i < nd; ++i) {
     if (!hd[i]) continue;
     const size_t n = (i < 8 ? nd2 : nd3) * sizeof(double);
     HIP_TRY(x->stage[19 + i].ensure(n));
+    HIP_TRY(hipMemcpy(x->stage[19 + i].p, hd[i], n, hipMemcpyHostToDevice));
     ddp[i] = x->stage[19 + i].p;
   }
   tempo_hip_dims dmd = *dm;
@@ -1040,8 +1147,59 @@ int tempo_hip_profile_read(tempo_hip_ctx* x, double* ms, int n, int* runs) {
 int tempo_hip_substeps(tempo_hip_ctx* x, int* out, size_t n) {
   if (!x || !out) return fail(TEMPO_HIP_E_ARG, "NULL argument");
   if (n != size_t(10) * x->last_ncol) return fail(TEMPO_HIP_E_ARG, "need 10*ncol entries");
-  HIP_TRY(hipMemcpy(out, x->ksub.p, n * sizeof(int), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(out, x->last_ksub, n * sizeof(int), hipMemcpyDeviceToHost));
   return TEMPO_HIP_OK;
 }
 
 }  // extern "C"
+
+namespace tempo {
+
+// One step of tempo_hip_run with its per-point scratch carved from the
+// caller's device block [base, base + bytes) of the context's device:
+// nothing is allocated.  E_ARG when the batch needs more than bytes.
+int run_with_scratch(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags* d, const tempo_hip_dims* dm,
+                     double dt, int itimestep, int precision, void* stream, void* base, size_t bytes) {
+  if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
+  int rc = check_run_args(f, dm, dt);
+  if (rc) return rc;
+  if (precision != TEMPO_HIP_FP32 && precision != TEMPO_HIP_FP64)
+    return fail(TEMPO_HIP_E_ARG, "precision must be TEMPO_HIP_FP32 or TEMPO_HIP_FP64");
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev != x->device) return fail(TEMPO_HIP_E_ARG, "call with the context's device current");
+  const bool r64 = precision == TEMPO_HIP_FP64;
+  StepArgs<float> p32;
+  StepArgs<double> p64;
+  std::memset(&p32, 0, sizeof(p32));
+  std::memset(&p64, 0, sizeof(p64));
+  p32.ncol = p64.ncol = dm->ni * dm->nj;
+  p32.nz = p64.nz = dm->nz;
+  const ScratchSizes z = r64 ? step_scratch(x, p64) : step_scratch(x, p32);
+  if (z.total() > bytes)
+    return fail(TEMPO_HIP_E_ARG, std::to_string(p32.ncol) + " columns of " + std::to_string(p32.nz) +
+                                     " levels need " + std::to_string(z.total()) + " bytes of scratch, more than the " +
+                                     std::to_string(bytes) + " available");
+  char* p = static_cast<char*>(base);
+  auto take = [&](size_t b) -> void* {
+    if (!b) return nullptr;
+    void* q = p;
+    p += ScratchSizes::up(b);
+    return q;
+  };
+  ScratchPtrs sc;
+  sc.sr = take(z.sr);
+  sc.sd = take(z.sd);
+  sc.pf = take(z.pf);
+  sc.vt = take(z.vt);
+  sc.cf = take(z.cf);
+  sc.kmelt = take(z.kmelt);
+  sc.precip = take(z.precip);
+  sc.ksub = take(z.ksub);
+  sc.semiw = take(z.semiw);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return r64 ? run_impl<double>(x, f, d, dm, dt, itimestep, s, nullptr, &sc)
+             : run_impl<float>(x, f, d, dm, dt, itimestep, s, nullptr, &sc);
+}
+
+}  // namespace tempo

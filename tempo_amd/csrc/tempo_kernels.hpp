@@ -190,6 +190,10 @@ template <class T>
 __device__ __forceinline__ bool same_bits(const Dual<T>& x, const Dual<T>& y) {
   return same_bits(x.v, y.v) && same_bits(x.d, y.d);
 }
+template <class T>
+__device__ __forceinline__ bool same_bits(const Sig<T>& x, const Sig<T>& y) {
+  return same_bits(x.v, y.v);
+}
 // post-check value of field f at o; where the step runs in place the input
 // is still in memory (the read is a cache hit) and an unchanged value is not
 // stored again
@@ -198,12 +202,32 @@ __device__ __forceinline__ void put_post_check(const StepArgs<R>& a, int f, size
   if (a.fa[f] != a.f[f] || !same_bits(a.f[f][o], v)) a.fa[f][o] = v;
 }
 
+// branch signature of a level-local kernel's point: opened on entry, stored
+// at every exit (the plain types record nothing)
+template <class R>
+struct SigScope {
+  __device__ SigScope(const StepArgs<R>&, int, int, int) {}
+};
+template <class T>
+struct SigScope<Sig<T>> {
+  unsigned* out = nullptr;
+  __device__ SigScope(const StepArgs<Sig<T>>& a, int c, int k, int phase) {
+    if (!a.sig) return;
+    out = a.sig + size_t(phase) * size_t(a.ncol) * size_t(a.nz) + size_t(c) + size_t(k) * size_t(a.ncol);
+    sig_begin();
+  }
+  __device__ ~SigScope() {
+    if (out) *out = sig_end();
+  }
+};
+
 template <class R>
 __global__ void __launch_bounds__(256) TEMPO_ATTR_A k_phase_a(StepArgs<R> a) {
   const int2 pt = block_point<R, false>(a);
   const int c = pt.x, k = pt.y;
   if (c >= a.ncol || k >= a.nz) return;
   if (a.nlev && k >= a.nlev[c]) return;
+  SigScope<R> scope(a, c, k, 0);
   const Const<S_t<R>>& C = *a.cst;
   const size_t o = TEMPO_IDX_STATE(a, c, k);
   const size_t s = TEMPO_IDX_SCR(a, c, k);
@@ -1005,12 +1029,58 @@ __device__ __forceinline__ void atomic_max_pos(Dual<T>* p, Dual<T> v) {
   atomic_max_pos(&p->v, v.v);
 }
 
+// freeze_cloud_melt_ice (src/module_mp_tempo_main.F90:2124-2165), on the
+// tendencies of the final update
+template <class R>
+__device__ __forceinline__ void freeze_cloud_melt_ice(const Const<S_t<R>>& C, Lvl<R>& L, R qc1d, R qi1d, R ni1d,
+                                                      R nc1d, bool has_nc, R ncv, R lvap, R ocp, R dt, R odt) {
+  // uncontracted, as the final checks: a fully frozen or melted mass then
+  // cancels to zero as in the reference
+#pragma clang fp contract(off)
+  const R xri = mx(R(0.0), qi1d + L.qiten * dt);
+  if ((L.temp > R(kT0)) && (xri > R(0.0))) {
+    L.qcten = L.qcten + xri * odt;
+    L.ncten = L.ncten + ni1d * odt;
+    L.qiten = L.qiten - xri * odt;
+    L.niten = -ni1d * odt;
+    L.tten = L.tten - C.lfus * ocp * xri * odt;
+  }
+  const R xrc = mx(R(0.0), qc1d + L.qcten * dt);
+  if ((L.temp < R(kHgFrz)) && (xrc > R(0.0))) {
+    const R lfus2 = R(kLsub) - lvap;
+    const R xnc = has_nc ? nc1d + L.ncten * dt : ncv / L.rho + L.ncten * dt;
+    L.qiten = L.qiten + xrc * odt;
+    L.niten = L.niten + xnc * odt;
+    L.qcten = L.qcten - xrc * odt;
+    L.ncten = L.ncten - xnc * odt;
+    L.tten = L.tten + lfus2 * ocp * xrc * odt;
+  }
+}
+// a signature run executes it on the plain type (its contraction pragma
+// applies to the plain type only) and records its two decisions
+template <class T>
+__device__ __forceinline__ void freeze_cloud_melt_ice(const Const<T>& C, Lvl<Sig<T>>& L, Sig<T> qc1d, Sig<T> qi1d,
+                                                      Sig<T> ni1d, Sig<T> nc1d, bool has_nc, Sig<T> ncv, Sig<T> lvap,
+                                                      Sig<T> ocp, Sig<T> dt, Sig<T> odt) {
+  Lvl<T> M = sig_unwrap(L);
+  const T qiten = M.qiten, qcten = M.qcten;
+  freeze_cloud_melt_ice<T>(C, M, qc1d.v, qi1d.v, ni1d.v, nc1d.v, has_nc, ncv.v, lvap.v, ocp.v, dt.v, odt.v);
+  sig_wrap(L, M);
+  sig_note(0x61u + (M.qiten != qiten ? 2u : 0u) + (M.qcten != qcten ? 4u : 0u));
+}
+
+template <class T>
+__device__ __forceinline__ void atomic_max_pos(Sig<T>* p, Sig<T> v) {
+  atomic_max_pos(&p->v, v.v);
+}
+
 template <class R>
 __global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
   const int2 pt = block_point<R, true>(a);
   const int c = pt.x, k = pt.y;
   if (c >= a.ncol || k >= a.nz) return;
   if (a.nlev && k >= a.nlev[c]) return;
+  SigScope<R> scope(a, c, k, 1);
   const Const<S_t<R>>& C = *a.cst;
   const size_t o = TEMPO_IDX_STATE(a, c, k);
   const size_t s = TEMPO_IDX_SCR(a, c, k);
@@ -1055,29 +1125,7 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
   R qb1d = has_ngqb ? a.fa[F_QB][o] : R(0);
   const R ncv = (a.fl.ml_for_nc && !has_nc) ? sr[SR_NCSAVE * plane + s] : column_nc(a, c);
 
-  if (!a.fl.turn_off_micro) {  // freeze_cloud_melt_ice (:2124-2165)
-    // uncontracted, as the final checks: a fully frozen or melted mass then
-    // cancels to zero as in the reference
-#pragma clang fp contract(off)
-    const R xri = mx(R(0.0), qi1d + L.qiten * dt);
-    if ((L.temp > R(kT0)) && (xri > R(0.0))) {
-      L.qcten = L.qcten + xri * odt;
-      L.ncten = L.ncten + ni1d * odt;
-      L.qiten = L.qiten - xri * odt;
-      L.niten = -ni1d * odt;
-      L.tten = L.tten - C.lfus * ocp * xri * odt;
-    }
-    const R xrc = mx(R(0.0), qc1d + L.qcten * dt);
-    if ((L.temp < R(kHgFrz)) && (xrc > R(0.0))) {
-      const R lfus2 = R(kLsub) - lvap;
-      const R xnc = has_nc ? nc1d + L.ncten * dt : ncv / L.rho + L.ncten * dt;
-      L.qiten = L.qiten + xrc * odt;
-      L.niten = L.niten + xnc * odt;
-      L.qcten = L.qcten - xrc * odt;
-      L.ncten = L.ncten - xnc * odt;
-      L.tten = L.tten + lfus2 * ocp * xrc * odt;
-    }
-  }
+  if (!a.fl.turn_off_micro) freeze_cloud_melt_ice(C, L, qc1d, qi1d, ni1d, nc1d, has_nc, ncv, lvap, ocp, dt, odt);
   // final update (:738-748)
   const R t_new = load_t(a, o) + L.tten * dt;
   const R qv_new = mx(R(kMinQv), (a.f[F_QV][o] + L.qvten * dt));

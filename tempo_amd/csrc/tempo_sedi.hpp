@@ -22,6 +22,8 @@
 // so both give identical results.
 #pragma once
 
+#include <cstring>
+
 #include "tempo_step.hpp"
 
 namespace tempo {
@@ -485,6 +487,43 @@ __global__ void __launch_bounds__(64 * kSedCols) __attribute__((amdgpu_waves_per
   }
   if (a.d2[D2_HAIL_SFC]) a.d2[D2_HAIL_SFC][d2] = R(0);
   if (a.d2[D2_HAIL_COL]) a.d2[D2_HAIL_COL][d2] = R(0);
+}
+
+// ------------------------------------------------------------- launch ----
+// The four launches of one step.  ga carries the same arguments as a in the
+// level-local kernels' working type: a itself, or its Sig<T> view for a run
+// that records branch signatures (StepArgs<T> and StepArgs<Sig<T>> have the
+// same layout).
+template <class T, class RA>
+hipError_t launch_step_impl(const StepArgs<T>& a, const StepArgs<RA>& ga, const LaunchCfg& lc, double* vtbuf,
+                            hipStream_t s, hipEvent_t* ev) {
+  const unsigned gc = unsigned((a.ncol + 255) / 256);
+  if (ev) (void)hipEventRecord(ev[0], s);
+  hipLaunchKernelGGL(k_prepass<T>, dim3(gc), dim3(256), 0, s, a);
+  if (ev) (void)hipEventRecord(ev[1], s);
+  hipLaunchKernelGGL(k_phase_a<RA>, level_grid(a, lc.block_a), dim3(unsigned(lc.block_a)), 0, s, ga);
+  if (ev) (void)hipEventRecord(ev[2], s);
+  if (lc.sed_wave && a.nz <= kSedLevels && !a.fl.semi_sedi)
+    hipLaunchKernelGGL(k_phase_b_wave<T>, dim3(unsigned((a.ncol + kSedCols - 1) / kSedCols)), dim3(64 * kSedCols),
+                       sed_lds_bytes(a.nz), s, a);
+  else
+    hipLaunchKernelGGL(k_phase_b<T>, dim3(unsigned((a.ncol + 127) / 128)), dim3(128), 0, s, a, vtbuf);
+  if (a.fl.refl_from_melting && a.fl.refl10cm && a.d3[D3_REFL]) {
+    hipLaunchKernelGGL(k_melt_level<T>, dim3(unsigned((a.ncol + 255) / 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_melt_refl<T>, level_grid(a, 256), dim3(256), 0, s, a);
+  }
+  if (ev) (void)hipEventRecord(ev[3], s);
+  hipLaunchKernelGGL(k_phase_c<RA>, level_grid(a, lc.block_c), dim3(unsigned(lc.block_c)), 0, s, ga);
+  if (ev) (void)hipEventRecord(ev[4], s);
+  return hipGetLastError();
+}
+
+template <class T>
+StepArgs<Sig<T>> sig_args(const StepArgs<T>& a) {
+  static_assert(sizeof(StepArgs<Sig<T>>) == sizeof(StepArgs<T>), "StepArgs layout");
+  StepArgs<Sig<T>> g;
+  std::memcpy(&g, &a, sizeof(a));
+  return g;
 }
 
 }  // namespace tempo

@@ -27,6 +27,7 @@
 #include "tempo_const.hpp"
 #include "tempo_crmath.hpp"
 #include "tempo_dual.hpp"
+#include "tempo_sig.hpp"
 
 namespace tempo {
 
@@ -125,6 +126,47 @@ __device__ __forceinline__ Dual<T> msqrt(Dual<T> x) {
 template <class T>
 __device__ __forceinline__ Dual<T> mcos(Dual<T> x) { return {mcos(x.v), -sin(x.v) * x.d}; }
 
+// branch-signature runs (tempo_sig.hpp): the library functions on the value,
+// exactly as for the plain type; min, max and |x| record nothing
+template <class T>
+struct Num<Sig<T>> {
+  using S = T;
+  using W = double;
+  using D = T;
+};
+template <class T>
+__host__ __device__ __forceinline__ T value(const Sig<T>& x) { return x.v; }
+template <class T>
+__device__ __forceinline__ Sig<T> mpow(Sig<T> x, Sig<T> y) { return mpow(x.v, y.v); }
+template <class T, class U, class = typename std::enable_if<std::is_arithmetic<U>::value>::type>
+__device__ __forceinline__ Sig<T> mpow(Sig<T> x, U y) { return mpow(x.v, T(y)); }
+template <class T, class U, class = typename std::enable_if<std::is_arithmetic<U>::value>::type>
+__device__ __forceinline__ Sig<T> mpow(U x, Sig<T> y) { return mpow(T(x), y.v); }
+template <class T>
+__device__ __forceinline__ Sig<T> mexp(Sig<T> x) { return mexp(x.v); }
+template <class T>
+__device__ __forceinline__ Sig<T> mlog(Sig<T> x) { return mlog(x.v); }
+template <class T>
+__device__ __forceinline__ Sig<T> mlog10(Sig<T> x) { return mlog10(x.v); }
+template <class T>
+__device__ __forceinline__ Sig<T> msqrt(Sig<T> x) { return msqrt(x.v); }
+template <class T>
+__device__ __forceinline__ Sig<T> mcos(Sig<T> x) { return mcos(x.v); }
+template <class T>
+__device__ __forceinline__ Sig<T> mabs(Sig<T> x) { return mabs(x.v); }
+template <class T>
+__device__ __forceinline__ Sig<T> mx(Sig<T> a, Sig<T> b) { return mx(a.v, b.v); }
+template <class T>
+__device__ __forceinline__ Sig<T> mn(Sig<T> a, Sig<T> b) { return mn(a.v, b.v); }
+template <class T>
+__device__ __forceinline__ Sig<T> copysign(Sig<T> a, Sig<T> b) { return copysign(a.v, b.v); }
+template <class T>
+__device__ __forceinline__ int nint_exact(Sig<T> x) {
+  const int i = nint_exact(x.v);
+  sig_note(unsigned(i) ^ 0x2545f491u);
+  return i;
+}
+
 // real**integer as flang lowers it (repeated squaring, reciprocal for n<0)
 template <class R>
 __device__ __forceinline__ R powi(R a, int b) {
@@ -217,6 +259,9 @@ struct StepArgs {
   // x + (-x/dt)*dt of a fully removed mass from cancelling
   double odt;
   int first_step;
+  // branch signatures of the level-local kernels (signature runs only):
+  // 2 x nz x ncol words, phase p, point (c, k) at p*nz*ncol + k*ncol + c
+  unsigned* sig;
 };
 
 // operands of the public cloud/ice/snow_check_and_update over n points
@@ -678,6 +723,63 @@ __device__ void graupel_check(const Const<S_t<R>>& C, Lvl<R>& L, R& qg1d, R& ng1
       qb1d = R(0.0);
     }
   }
+}
+
+// The checks above round x + ten*dt as two operations (their contraction
+// pragma).  In a signature run they execute on the plain type, compiled
+// exactly as in the production step, and their decisions are recorded from
+// their outcome: the presence flag, the cloud's shape index, the graupel
+// density index and the number clamps (a changed number marks a hit limit).
+template <class T>
+__device__ __forceinline__ Lvl<T> sig_unwrap(const Lvl<Sig<T>>& L) {
+  static_assert(sizeof(Lvl<Sig<T>>) == sizeof(Lvl<T>) && sizeof(Sig<T>) == sizeof(T), "Sig layout");
+  Lvl<T> M;
+  __builtin_memcpy(&M, &L, sizeof(M));
+  return M;
+}
+template <class T>
+__device__ __forceinline__ void sig_wrap(Lvl<Sig<T>>& L, const Lvl<T>& M) {
+  __builtin_memcpy(&L, &M, sizeof(M));
+}
+template <class T>
+__device__ void cloud_check(const Const<T>& C, Lvl<Sig<T>>& L, Sig<T>& qc1d, Sig<T>& nc1d, bool has_nc, Sig<T> ncv,
+                            Sig<T> dt, Sig<T> odt) {
+  Lvl<T> M = sig_unwrap(L);
+  cloud_check<T>(C, M, qc1d.v, nc1d.v, has_nc, ncv.v, dt.v, odt.v);
+  sig_wrap(L, M);
+  sig_note(M.l_qc ? 0x11u + 256u * unsigned(get_nuc(M.nc)) : 0x12u);
+}
+template <class T>
+__device__ void rain_check(const Const<T>& C, Lvl<Sig<T>>& L, Sig<T>& qr1d, Sig<T>& nr1d, Sig<T> dt, Sig<T> odt) {
+  Lvl<T> M = sig_unwrap(L);
+  const T nten = M.nrten;
+  rain_check<T>(C, M, qr1d.v, nr1d.v, dt.v, odt.v);
+  sig_wrap(L, M);
+  sig_note(M.l_qr ? (M.nrten != nten ? 0x21u : 0x22u) : 0x23u);
+}
+template <class T>
+__device__ void ice_check(const Const<T>& C, Lvl<Sig<T>>& L, Sig<T>& qi1d, Sig<T>& ni1d, Sig<T> dt, Sig<T> odt) {
+  Lvl<T> M = sig_unwrap(L);
+  const T nten = M.niten;
+  ice_check<T>(C, M, qi1d.v, ni1d.v, dt.v, odt.v);
+  sig_wrap(L, M);
+  sig_note(M.l_qi ? (M.niten != nten ? 0x31u : 0x32u) : 0x33u);
+}
+template <class T>
+__device__ void snow_check(Lvl<Sig<T>>& L, Sig<T>& qs1d, Sig<T> dt, Sig<T> odt) {
+  Lvl<T> M = sig_unwrap(L);
+  snow_check<T>(M, qs1d.v, dt.v, odt.v);
+  sig_wrap(L, M);
+  sig_note(M.l_qs ? 0x41u : 0x42u);
+}
+template <class T>
+__device__ void graupel_check(const Const<T>& C, Lvl<Sig<T>>& L, Sig<T>& qg1d, Sig<T>& ng1d, Sig<T>& qb1d,
+                              bool has_ngqb, Sig<T> dt, Sig<T> odt) {
+  Lvl<T> M = sig_unwrap(L);
+  const T nten = M.ngten;
+  graupel_check<T>(C, M, qg1d.v, ng1d.v, qb1d.v, has_ngqb, dt.v, odt.v);
+  sig_wrap(L, M);
+  sig_note(M.l_qg ? 0x51u + 256u * unsigned(M.idx_bg) + (M.ngten != nten ? 0x10000u : 0u) : 0x52u);
 }
 
 // ---- table indices (src/module_mp_tempo_main.F90:2493-2635, 2839-2858) ----

@@ -8,25 +8,13 @@ namespace tempo {
 
 // ev (optional): 5 events recorded around the 4 launches for per-phase timing
 hipError_t launch_step(const StepArgs<float>& a, const LaunchCfg& lc, double* vtbuf, hipStream_t s, hipEvent_t* ev) {
-  const unsigned gc = unsigned((a.ncol + 255) / 256);
-  if (ev) (void)hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL(k_prepass<float>, dim3(gc), dim3(256), 0, s, a);
-  if (ev) (void)hipEventRecord(ev[1], s);
-  hipLaunchKernelGGL(k_phase_a<float>, level_grid(a, lc.block_a), dim3(unsigned(lc.block_a)), 0, s, a);
-  if (ev) (void)hipEventRecord(ev[2], s);
-  if (lc.sed_wave && a.nz <= kSedLevels && !a.fl.semi_sedi)
-    hipLaunchKernelGGL(k_phase_b_wave<float>, dim3(unsigned((a.ncol + kSedCols - 1) / kSedCols)), dim3(64 * kSedCols),
-                       sed_lds_bytes(a.nz), s, a);
-  else
-    hipLaunchKernelGGL(k_phase_b<float>, dim3(unsigned((a.ncol + 127) / 128)), dim3(128), 0, s, a, vtbuf);
-  if (a.fl.refl_from_melting && a.fl.refl10cm && a.d3[D3_REFL]) {
-    hipLaunchKernelGGL(k_melt_level<float>, dim3(unsigned((a.ncol + 255) / 256)), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_melt_refl<float>, level_grid(a, 256), dim3(256), 0, s, a);
-  }
-  if (ev) (void)hipEventRecord(ev[3], s);
-  hipLaunchKernelGGL(k_phase_c<float>, level_grid(a, lc.block_c), dim3(unsigned(lc.block_c)), 0, s, a);
-  if (ev) (void)hipEventRecord(ev[4], s);
-  return hipGetLastError();
+  return launch_step_impl(a, a, lc, vtbuf, s, ev);
+}
+
+// the same step with the level-local kernels recording branch signatures
+// into a.sig
+hipError_t launch_step_sig(const StepArgs<float>& a, const LaunchCfg& lc, double* vtbuf, hipStream_t s) {
+  return launch_step_impl(a, sig_args(a), lc, vtbuf, s, nullptr);
 }
 
 // tangent-linear step in this precision (same contraction flags as the step)

@@ -455,16 +455,18 @@ contains
     call to_c_cfg(tempo_cfgs, cc)
     rc = tempo_hip_set_run_cfg(tempo_cfgs%ctx, cc)
     if (rc /= 0) call fail('tempo_run', rc)
-    call allocate_diags(tempo_cfgs, tempo_diags, ims, ime, jms, jme, kms, kme)
+    call allocate_diags(tempo_cfgs, tempo_diags, its, ite, jts, jte, kts, kte)
 
     dm%ni = ite - its + 1
     dm%nj = jte - jts + 1
     dm%nz = kte - kts + 1
     dm%ldk = ime - ims + 1
     dm%ldj = dm%ldk * (kme - kms + 1)
-    dm%dldk = dm%ldk
-    dm%dldj = dm%ldj
-    dm%d2ldj = dm%ldk
+    ! the diagnostics are tile-shaped (its:ite, kts:kte, jts:jte), as the
+    ! reference allocates them
+    dm%dldk = dm%ni
+    dm%dldj = int(dm%ni, c_long_long) * dm%nz
+    dm%d2ldj = dm%ni
 
     if (present(t)) f%t = c_loc(t(its, kts, jts))
     if (present(th)) f%th = c_loc(th(its, kts, jts))
@@ -526,29 +528,21 @@ contains
     if (rc /= 0) call fail('tempo_run', rc)
   end subroutine
 
-  ! allocation and reset of the driver diagnostics
-  ! (src/module_mp_tempo_driver.F90:304-386)
-  subroutine allocate_diags(cfg, dg, ims, ime, jms, jme, kms, kme)
+  ! allocation and reset of the driver diagnostics on the tile
+  ! (its:ite, kts:kte, jts:jte), as src/module_mp_tempo_driver.F90:304-386
+  ! does; an array the caller allocated with other bounds is reallocated
+  subroutine allocate_diags(cfg, dg, its, ite, jts, jte, kts, kte)
     type(ty_tempo_cfgs), intent(in) :: cfg
     type(ty_tempo_driver_diags), intent(inout) :: dg
-    integer, intent(in) :: ims, ime, jms, jme, kms, kme
-    if (.not. allocated(dg%rain_precip)) then
-      allocate(dg%rain_precip(ims:ime, jms:jme), dg%ice_liquid_equiv_precip(ims:ime, jms:jme), &
-        dg%snow_liquid_equiv_precip(ims:ime, jms:jme), dg%graupel_liquid_equiv_precip(ims:ime, jms:jme), &
-        dg%frozen_fraction(ims:ime, jms:jme), dg%frz_rain_precip(ims:ime, jms:jme))
-    endif
-    dg%rain_precip = 0._wp
-    dg%ice_liquid_equiv_precip = 0._wp
-    dg%snow_liquid_equiv_precip = 0._wp
-    dg%graupel_liquid_equiv_precip = 0._wp
-    dg%frozen_fraction = 0._wp
-    dg%frz_rain_precip = 0._wp
-    if (cfg%max_hail_diameter_flag) then
-      if (.not. allocated(dg%max_hail_diameter_sfc)) allocate(dg%max_hail_diameter_sfc(ims:ime, jms:jme), &
-        dg%max_hail_diameter_column(ims:ime, jms:jme))
-      dg%max_hail_diameter_sfc = 0._wp
-      dg%max_hail_diameter_column = 0._wp
-    endif
+    integer, intent(in) :: its, ite, jts, jte, kts, kte
+    call alloc2(.true., dg%rain_precip)
+    call alloc2(.true., dg%ice_liquid_equiv_precip)
+    call alloc2(.true., dg%snow_liquid_equiv_precip)
+    call alloc2(.true., dg%graupel_liquid_equiv_precip)
+    call alloc2(.true., dg%frozen_fraction)
+    call alloc2(.true., dg%frz_rain_precip)
+    call alloc2(cfg%max_hail_diameter_flag, dg%max_hail_diameter_sfc)
+    call alloc2(cfg%max_hail_diameter_flag, dg%max_hail_diameter_column)
     call alloc3(cfg%refl10cm_flag, dg%refl10cm, -35._wp)
     call alloc3(cfg%re_cloud_flag, dg%re_cloud, 0._wp)
     call alloc3(cfg%re_ice_flag, dg%re_ice, 0._wp)
@@ -557,12 +551,27 @@ contains
     call alloc3(cfg%graupel_med_vol_diam_flag, dg%graupel_med_vol_diam, 0._wp)
     call alloc3(cfg%cloud_number_mixing_ratio_flag, dg%cloud_number_mixing_ratio, 0._wp)
   contains
+    subroutine alloc2(on, a)
+      logical, intent(in) :: on
+      real(wp), allocatable, intent(inout) :: a(:,:)
+      if (.not. on) return
+      if (allocated(a)) then
+        if (lbound(a, 1) /= its .or. ubound(a, 1) /= ite .or. lbound(a, 2) /= jts .or. ubound(a, 2) /= jte) &
+          deallocate(a)
+      endif
+      if (.not. allocated(a)) allocate(a(its:ite, jts:jte))
+      a = 0._wp
+    end subroutine
     subroutine alloc3(on, a, v)
       logical, intent(in) :: on
       real(wp), allocatable, intent(inout) :: a(:,:,:)
       real(wp), intent(in) :: v
       if (.not. on) return
-      if (.not. allocated(a)) allocate(a(ims:ime, kms:kme, jms:jme))
+      if (allocated(a)) then
+        if (lbound(a, 1) /= its .or. ubound(a, 1) /= ite .or. lbound(a, 2) /= kts .or. ubound(a, 2) /= kte .or. &
+            lbound(a, 3) /= jts .or. ubound(a, 3) /= jte) deallocate(a)
+      endif
+      if (.not. allocated(a)) allocate(a(its:ite, kts:kte, jts:jte))
       a = v
     end subroutine
   end subroutine

@@ -17,6 +17,10 @@ Workloads (k = 1 is the surface, dz uniform):
   adjoint    16,384 x 64, dz 250 m, dt 20 s, aerosol-aware
   multistep  262,144 x 90, dz 200 m, dt 20 s, aerosol- and hail-aware, ragged
              generator at 90 levels
+  cold       4,096 x 60, dz 250 m, dt 20 s, aerosol- and hail-aware; surface
+             temperature U[258, 272.5] K per column and the first cloud layer
+             resting on the surface, so that rain, snow, ice and graupel reach
+             the ground below freezing (freezing rain)
 """
 import numpy as np
 
@@ -26,7 +30,7 @@ FIELD_IDS = {
     "layer1_base": 1, "layer1_thick": 2, "layer2_on": 3, "layer2_base": 4, "layer2_thick": 5,
     "rh": 6, "qc": 7, "qr": 8, "qi": 9, "qs": 10, "qg": 11, "w1": 12, "w2": 13,
     "nwfa1": 14, "nwfa2": 15, "nifa1": 16, "nifa2": 17, "rhog": 18, "conv": 19, "wmax": 20,
-    "qr_sc": 21, "nlev": 22, "q1": 23, "q2": 24,
+    "qr_sc": 21, "nlev": 22, "q1": 23, "q2": 24, "tsfc": 25,
 }
 
 WORKLOADS = {
@@ -40,6 +44,8 @@ WORKLOADS = {
                     ragged=False, itimestep=2),
     "multistep": dict(ncol=262_144, nz=90, dz=200.0, dt=20.0, aerosol=True, hail=True, seed=1005,
                       ragged=False, itimestep=2, convective=0.25),
+    "cold": dict(ncol=4096, nz=60, dz=250.0, dt=20.0, aerosol=True, hail=True, seed=1006,
+                 ragged=False, itimestep=2, t_sfc=(258.0, 272.5), surface_cloud=True),
 }
 
 
@@ -99,20 +105,30 @@ def generate(workload, ncol=None, col0=0, seed=None):
 
     z = (np.arange(nz, dtype=np.float64) + 0.5) * dz                     # level mid-points [m]
     zc = np.repeat(z[:, None], n, axis=1)
-    t = np.maximum(300.0 - 6.5 * zc * 1e-3, 210.0)
-    # hydrostatic pressure from 1000 hPa at the surface with layer-mean temperature
     g, rd = 9.8, 287.04
     zi = np.concatenate([[0.0], z])
-    ti = np.maximum(300.0 - 6.5 * zi * 1e-3, 210.0)
-    lnp = np.log(1.0e5) - np.cumsum(g * np.diff(zi) / (rd * 0.5 * (ti[1:] + ti[:-1])))
-    p = np.repeat(np.exp(lnp)[:, None], n, axis=1)
+    if "t_sfc" in cfg:
+        # per-column surface temperature; hydrostatic pressure per column
+        lo, hi = cfg["t_sfc"]
+        tsfc = lo + (hi - lo) * uniform(seed, col1, zero, "tsfc")
+        t = np.maximum(tsfc[None, :] - 6.5 * zc * 1e-3, 210.0)
+        ti = np.maximum(tsfc[None, :] - 6.5 * zi[:, None] * 1e-3, 210.0)
+        lnp = np.log(1.0e5) - np.cumsum(g * np.diff(zi)[:, None] / (rd * 0.5 * (ti[1:] + ti[:-1])), axis=0)
+        p = np.exp(lnp)
+    else:
+        t = np.maximum(300.0 - 6.5 * zc * 1e-3, 210.0)
+        # hydrostatic pressure from 1000 hPa at the surface with layer-mean temperature
+        ti = np.maximum(300.0 - 6.5 * zi * 1e-3, 210.0)
+        lnp = np.log(1.0e5) - np.cumsum(g * np.diff(zi) / (rd * 0.5 * (ti[1:] + ti[:-1])))
+        p = np.repeat(np.exp(lnp)[:, None], n, axis=1)
 
     # one or two cloud layers per column: base U[0.5,8] km, thickness U[2,4] km
-    def layer(bf, tf):
-        base = 500.0 + 7500.0 * uniform(seed, col1, zero, bf)
+    # (the first from the surface up when the workload puts cloud there)
+    def layer(bf, tf, surface=False):
+        base = 0.0 if surface else 500.0 + 7500.0 * uniform(seed, col1, zero, bf)
         thick = 2000.0 + 2000.0 * uniform(seed, col1, zero, tf)
-        return (zc >= base[None, :]) & (zc <= (base + thick)[None, :])
-    incloud = layer("layer1_base", "layer1_thick")
+        return (zc >= base) & (zc <= (base + thick)[None, :])
+    incloud = layer("layer1_base", "layer1_thick", cfg.get("surface_cloud", False))
     two = uniform(seed, col1, zero, "layer2_on") < 0.5
     incloud |= layer("layer2_base", "layer2_thick") & two[None, :]
 
@@ -219,18 +235,28 @@ def generate_torch(workload, ncol=None, col0=0, seed=None, device="cuda", dtype=
 
     z = (torch.arange(nz, **f64) + 0.5) * dz
     zc = z[:, None].expand(nz, n)
-    t = torch.clamp(300.0 - 6.5 * zc * 1e-3, min=210.0)
     g, rd = 9.8, 287.04
     zi = np.concatenate([[0.0], (np.arange(nz, dtype=np.float64) + 0.5) * dz])
-    ti = np.maximum(300.0 - 6.5 * zi * 1e-3, 210.0)
-    lnp = np.log(1.0e5) - np.cumsum(g * np.diff(zi) / (rd * 0.5 * (ti[1:] + ti[:-1])))
-    p = torch.tensor(np.exp(lnp), **f64)[:, None].expand(nz, n)
+    if "t_sfc" in cfg:
+        lo, hi = cfg["t_sfc"]
+        tsfc = lo + (hi - lo) * uniform_t(seed, col1, zero, "tsfc")
+        t = torch.clamp(tsfc[None, :] - 6.5 * zc * 1e-3, min=210.0)
+        zit = torch.tensor(zi, **f64)
+        ti = torch.clamp(tsfc[None, :] - 6.5 * zit[:, None] * 1e-3, min=210.0)
+        dzi = torch.tensor(np.diff(zi), **f64)[:, None]
+        lnp = np.log(1.0e5) - torch.cumsum(g * dzi / (rd * 0.5 * (ti[1:] + ti[:-1])), dim=0)
+        p = torch.exp(lnp)
+    else:
+        t = torch.clamp(300.0 - 6.5 * zc * 1e-3, min=210.0)
+        ti = np.maximum(300.0 - 6.5 * zi * 1e-3, 210.0)
+        lnp = np.log(1.0e5) - np.cumsum(g * np.diff(zi) / (rd * 0.5 * (ti[1:] + ti[:-1])))
+        p = torch.tensor(np.exp(lnp), **f64)[:, None].expand(nz, n)
 
-    def layer(bf, tf):
-        base = 500.0 + 7500.0 * uniform_t(seed, col1, zero, bf)
+    def layer(bf, tf, surface=False):
+        base = 0.0 if surface else 500.0 + 7500.0 * uniform_t(seed, col1, zero, bf)
         thick = 2000.0 + 2000.0 * uniform_t(seed, col1, zero, tf)
-        return (zc >= base[None, :]) & (zc <= (base + thick)[None, :])
-    incloud = layer("layer1_base", "layer1_thick")
+        return (zc >= base) & (zc <= (base + thick)[None, :])
+    incloud = layer("layer1_base", "layer1_thick", cfg.get("surface_cloud", False))
     two = uniform_t(seed, col1, zero, "layer2_on") < 0.5
     incloud = incloud | (layer("layer2_base", "layer2_thick") & two[None, :])
 

@@ -94,6 +94,9 @@ def lib():
                                     ctypes.c_double, ctypes.c_int, ctypes.c_int, vp]
         L.tempo_hip_run_host.argtypes = [vp, ctypes.POINTER(Fields), ctypes.POINTER(Diags), ctypes.POINTER(Dims),
                                          ctypes.c_double, ctypes.c_int]
+        L.tempo_hip_branch_signature.argtypes = [vp, ctypes.POINTER(Fields), ctypes.POINTER(Diags),
+                                                 ctypes.POINTER(Dims), ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                                 vp, vp]
         L.tempo_hip_table_export.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t]
         L.tempo_hip_table_write.argtypes = [vp, ctypes.c_char_p]
         L.tempo_hip_table_build_ms.argtypes = [vp]
@@ -189,11 +192,14 @@ class TempoHip:
     def reserve(self, ncol, nz):
         _check(lib().tempo_hip_reserve(self._ctx, int(ncol), int(nz)))
 
-    def run(self, state, dt, itimestep, nlev=None, land=None, diags=True, stream=None):
+    def run(self, state, dt, itimestep, nlev=None, land=None, diags=True, stream=None, signature=None):
         """tempo_run over every column of ``state`` (dict of (nz, ncol) tensors).
 
         ``diags``: True allocates every diagnostic, False none, or a dict of
         preallocated tensors (name -> (ncol,) or (nz, ncol)) to write into.
+        ``signature``: an int32 cuda tensor of shape (2, nz, ncol) runs the
+        same step through tempo_hip_branch_signature and receives the branch
+        signatures of its points (phase A, final update).
         Returns the dict of diagnostic tensors (2-D ones of shape (ncol,)).
         """
         import torch
@@ -240,6 +246,14 @@ class TempoHip:
             dims.nlev = nlev.data_ptr()
         s = stream if stream is not None else torch.cuda.current_stream(ref.device).cuda_stream
         prec = FP32 if dtype == torch.float32 else FP64
+        if signature is not None:
+            if signature.dtype != torch.int32 or tuple(signature.shape) != (2, nz, ncol) or \
+                    not signature.is_contiguous() or not signature.is_cuda:
+                raise ValueError(f"signature: need a contiguous int32 cuda tensor of shape {(2, nz, ncol)}")
+            _check(lib().tempo_hip_branch_signature(self._ctx, ctypes.byref(f), ctypes.byref(d), ctypes.byref(dims),
+                                                    float(dt), int(itimestep), prec, signature.data_ptr(),
+                                                    ctypes.c_void_p(s)))
+            return out
         _check(lib().tempo_hip_run(self._ctx, ctypes.byref(f), ctypes.byref(d), ctypes.byref(dims), float(dt),
                                    int(itimestep), prec, ctypes.c_void_p(s)))
         return out

@@ -28,7 +28,9 @@ def load_case(name):
     out = {k[4:]: v for k, v in arrays.items() if k.startswith("out_")}
     d2 = {k[3:]: v for k, v in arrays.items() if k.startswith("d2_")}
     d3 = {k[3:]: v for k, v in arrays.items() if k.startswith("d3_")}
-    return dict(meta=meta, inp=inp, out=out, d2=d2, d3=d3, nlev=arrays.get("nlev"), land=arrays.get("land"),
+    # 2-D diagnostics of every step (steps x ncol), where the case stores them
+    ts = {k[3:]: v for k, v in arrays.items() if k.startswith("ts_")}
+    return dict(meta=meta, inp=inp, out=out, d2=d2, d3=d3, ts=ts, nlev=arrays.get("nlev"), land=arrays.get("land"),
                 arrays=arrays)
 
 

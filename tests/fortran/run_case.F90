@@ -10,7 +10,7 @@
 ! output.bin: the 14 updated fields (t qv qc qr qi qs qg ni nr nc nwfa nifa
 ! ng qb), the 8 two-dimensional and the 4 default three-dimensional
 ! diagnostics over the tile, then int32 1 if every halo point kept its
-! sentinel value, else 0.
+! sentinel value and the diagnostics have the tile's bounds, else 0.
 program run_case
   use iso_c_binding
   use tempo_hip_mod
@@ -48,6 +48,11 @@ program run_case
 
   cfg%ccn_path = trim(fccn)
   call tempo_init(aerosolaware_flag=.true., hailaware_flag=.true., tempo_cfgs=cfg)
+  ! diagnostics the host allocated over its memory extent: tempo_run gives
+  ! them the tile's bounds, as the reference allocates them
+  allocate(dg%rain_precip(0:ni + 1, 1:nj), dg%refl10cm(0:ni + 1, 0:nz + 1, 1:nj))
+  dg%rain_precip = sentinel
+  dg%refl10cm = sentinel
   its = 1
   call tempo_run(cfg, dt, int(hdr(4)), &
     t=st(:,:,:,1), p=st(:,:,:,2), w=st(:,:,:,3), dz=st(:,:,:,4), &
@@ -60,6 +65,10 @@ program run_case
 
   halo_ok = all(st(0, :, :, :) == sentinel) .and. all(st(ni + 1, :, :, :) == sentinel) .and. &
     all(st(:, 0, :, :) == sentinel) .and. all(st(:, nz + 1, :, :) == sentinel)
+  ! tile-shaped diagnostics
+  halo_ok = halo_ok .and. all(lbound(dg%rain_precip) == [1, 1]) .and. all(ubound(dg%rain_precip) == [ni, nj]) .and. &
+    all(lbound(dg%refl10cm) == [1, 1, 1]) .and. all(ubound(dg%refl10cm) == [ni, nz, nj]) .and. &
+    all(lbound(dg%re_ice) == [1, 1, 1]) .and. all(ubound(dg%re_ice) == [ni, nz, nj])
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
   do f = 1, size(out_fields)
     write(u) st(1:ni, 1:nz, 1:nj, out_fields(f))

@@ -45,7 +45,10 @@ def col(a):
     return np.ascontiguousarray(a.reshape(-1, 1), dtype=np.float64)
 
 
-def run_case(name, state, dt, itimestep, flags, nlev=None, land=None, steps=1):
+def run_case(name, state, dt, itimestep, flags, nlev=None, land=None, steps=1, series=False, need=()):
+    """series: also store every step's 2-D diagnostics (ts_<name>, steps x
+    ncol); need: 2-D diagnostics that must be non-zero in some column of
+    some step, or the case does not exercise what it is for"""
     if ONLY and name not in ONLY:
         return
     ref.init(TABLES, **flags)
@@ -54,9 +57,12 @@ def run_case(name, state, dt, itimestep, flags, nlev=None, land=None, steps=1):
     nz, ncol = state["p"].shape
     d2 = {k: np.zeros(ncol) for k in ref.DIAG2_FIELDS}
     d3 = {k: np.zeros((nz, ncol)) for k in ref.DIAG3_FIELDS}
+    ts = {k: np.zeros((steps, ncol)) for k in ref.DIAG2_FIELDS}
     for s in range(steps):
         if nlev is None:
             d2, d3 = ref.run(out, dt, itimestep + s, land=land)
+            for k in ts:
+                ts[k][s] = d2[k]
         else:
             for c in range(ncol):
                 n = int(nlev[c])
@@ -78,6 +84,9 @@ def run_case(name, state, dt, itimestep, flags, nlev=None, land=None, steps=1):
     for k in chk:
         if not np.array_equal(chk[k], out[k]):
             raise SystemExit(f"{name}: C restatement differs from the reference on {k}")
+    for k in need:
+        if not (ts[k] != 0.0).any():
+            raise SystemExit(f"{name}: diagnostic {k} is zero in every column and step")
     if CHECK_DIAGS:
         for k in CHECK_DIAGS:
             got, want = (c2[k], d2[k]) if k in c2 else (c3[k], d3[k])
@@ -88,6 +97,10 @@ def run_case(name, state, dt, itimestep, flags, nlev=None, land=None, steps=1):
     arrays.update({f"out_{k}": v for k, v in out.items()})
     arrays.update({f"d2_{k}": v for k, v in d2.items()})
     arrays.update({f"d3_{k}": v for k, v in d3.items()})
+    if series:
+        if nlev is not None:
+            raise SystemExit(f"{name}: series are written for full columns only")
+        arrays.update({f"ts_{k}": v for k, v in ts.items()})
     if nlev is not None:
         arrays["nlev"] = np.asarray(nlev, dtype=np.int32)
     if land is not None:
@@ -258,6 +271,48 @@ def main():
     st, m = gen.generate("ragged", ncol=24, col0=3300)
     run_case("melt_ragged", st, m["dt"], m["itimestep"], dict(HAIL, refl_from_melting=True), nlev=m["nlev"])
     CHECK_DIAGS = ()
+
+    # the reference's own integration set-ups (test/tests.F90, driven by
+    # test/run_tempo_tests.F90:19-40 after tempo_init with the default flags):
+    # 1,200 s of sedimentation alone at every time step of its sweep, with the
+    # surface precipitation of every step stored
+    sweep = (1.0, 10.0, 20.0, 60.0, 120.0, 300.0, 600.0)
+    zeros = col(np.zeros(59))
+    snow = dict(base, qs=col(d["qs"]), qg=zeros, ng=zeros, qb=zeros)
+    for dt in sweep:
+        n = int(1200 / dt)
+        # test_snow_sedimentation (test/tests.F90:132-214): the column's snow
+        # starts 2.4 km up and does not reach the ground within 1,200 s
+        run_case(f"ref_snow_sedi_dt{int(dt)}", snow, dt, 1, dict(BOTH, turn_off_micro=True), steps=n, series=True)
+        # test_graupel_sedimentation, flux form and semi-Lagrangian (:24-129)
+        run_case(f"ref_graupel_sedi_dt{int(dt)}", base, dt, 1, dict(BOTH, turn_off_micro=True, mvd_g=True),
+                 steps=n, series=True, need=("graupel_precip",))
+        run_case(f"ref_graupel_semi_dt{int(dt)}", base, dt, 1,
+                 dict(BOTH, turn_off_micro=True, mvd_g=True, semi_sedi=True), steps=n, series=True,
+                 need=("graupel_precip",))
+    # test_cloud_number_aerosolaware / _non_aerosolaware / _ml (:217-461),
+    # dt = 20 s over 1,200 s
+    cn = dict(t=col(t), p=col(d["pressure"]), dz=col(d["dz"]), w=col(d["w"]), qv=col(d["qv"]), qc=col(d["qc"]),
+              qr=col(d["qr"]), nr=col(d["nr"]), qi=col(d["qi"]), qs=col(d["qs"]), ni=col(d["ni"]), qg=zeros)
+    run_case("ref_cloud_number_aero", dict(cn, nc=col(d["nc"])), 20.0, 1, dict(BOTH, turn_off_micro=True),
+             steps=60, series=True)
+    CHECK_DIAGS = ("nc_mixing_ratio",)
+    run_case("ref_cloud_number_const", cn, 20.0, 1, dict(BOTH, turn_off_micro=True, nc_mixing_ratio=True),
+             steps=60, series=True)
+    run_case("ref_cloud_number_ml", cn, 20.0, 1,
+             dict(BOTH, turn_off_micro=True, nc_mixing_ratio=True, ml_for_nc=True), steps=60, series=True)
+    CHECK_DIAGS = ()
+    # below-freezing surface with cloud down to it: freezing rain and ice,
+    # snow and graupel reaching the ground, in both sedimentation forms
+    st, m = gen.generate("cold", ncol=24, col0=0)
+    allp = ("rain_precip", "ice_precip", "snow_precip", "graupel_precip", "frz_rain_precip")
+    run_case("cold_surface", st, m["dt"], m["itimestep"], BOTH, steps=3, series=True, need=allp)
+    run_case("cold_surface_semi", st, m["dt"], m["itimestep"], dict(BOTH, semi_sedi=True), steps=3, series=True,
+             need=allp)
+    st, m = gen.generate("cold", ncol=16, col0=5000)
+    run_case("cold_surface_plain_dt60", {k: v for k, v in st.items() if k not in ("nc", "nwfa", "nifa", "ng", "qb")},
+             60.0, 2, PLAIN, steps=2, series=True, need=("rain_precip", "frz_rain_precip", "snow_precip"))
+
     # tempo_aerosol_surface_emissions (src/module_mp_tempo_driver.F90:516-529)
     if not ONLY or "surface_emissions" in ONLY:
         rng = np.random.default_rng(77)

@@ -36,7 +36,7 @@ def test_device_generator_matches_numpy():
     """generate_torch reproduces generate(): identical counters, float math
     within a few ulp."""
     torch = __import__("pytest").importorskip("torch")
-    for wl in ("plain", "aerosol", "ragged", "multistep"):
+    for wl in ("plain", "aerosol", "ragged", "multistep", "cold"):
         a, ma = gen.generate(wl, ncol=200, col0=4321)
         b, mb = gen.generate_torch(wl, ncol=200, col0=4321, device="cpu")
         assert sorted(a) == sorted(b)

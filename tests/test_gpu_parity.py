@@ -82,12 +82,17 @@ def to_dev(state, dtype):
     return {k: torch.tensor(v, dtype=dtype, device="cuda") for k, v in state.items()}
 
 
-def gpu_run(ctx, case_inp, dt, itimestep, steps=1, dtype=torch.float64, nlev=None, land=None):
+def gpu_run(ctx, case_inp, dt, itimestep, steps=1, dtype=torch.float64, nlev=None, land=None, series=None):
+    """steps of the case on the GPU; series (dict) receives every step's 2-D
+    diagnostics under the reference's names"""
     st = to_dev(case_inp, dtype)
     nl = None if nlev is None else torch.tensor(nlev, dtype=torch.int32, device="cuda")
     ld = None if land is None else torch.tensor(land, dtype=torch.int32, device="cuda")
     for s in range(steps):
         d = ctx.run(st, dt, itimestep + s, nlev=nl, land=ld)
+        if series is not None:
+            for k, n in D2_MAP.items():
+                series.setdefault(k, []).append(d[n].double().cpu().numpy())
     torch.cuda.synchronize()
     return {k: v.double().cpu().numpy() for k, v in st.items()}, {k: v.double().cpu().numpy() for k, v in d.items()}
 
@@ -147,8 +152,9 @@ def test_gpu_fp64_matches_reference_golden(hip, name):
     case = load_case(name)
     meta = case["meta"]
     ctx = context(hip, meta["flags"])
+    series = {}
     st, d = gpu_run(ctx, case["inp"], meta["dt"], meta["itimestep"], meta["steps"], nlev=case["nlev"],
-                    land=case["land"])
+                    land=case["land"], series=series)
     has_q = np.zeros(case["inp"]["p"].shape[1], bool)
     for q in ("qc", "qr", "qi", "qs", "qg"):
         has_q |= (case["inp"][q] > 1e-12).any(axis=0)
@@ -165,6 +171,13 @@ def test_gpu_fp64_matches_reference_golden(hip, name):
     for k, v in case["d3"].items():
         worst[k] = rel_inf(d[D3_MAP[k]], v)
         assert worst[k] <= 1e-12, (name, k, worst[k])
+    # the surface precipitation of every step, where the case stores it
+    for k, v in case["ts"].items():
+        got = np.array(series[k])
+        if k == "frozen_fraction":
+            got, v = got[:, has_q], v[:, has_q]
+        worst["ts_" + k] = rel_inf(got, v)
+        assert worst["ts_" + k] <= 1e-12, (name, "step series", k, worst["ts_" + k])
     print(name, "max rel", max(worst.values()))
 
 
@@ -222,88 +235,138 @@ def test_gpu_semi_sedi_fp64_matches_oracle(hip, gpu_tables):
 TENDENCY_FIELDS = ("t", "qv", "qc", "qr", "qi", "qs", "qg", "ni", "nr", "nc", "ng", "qb", "nwfa", "nifa")
 
 
-def fp32_errors(inp, out, d, o64, d64, dt):
-    """(rel L2 of each tendency over unflipped points, flipped fraction) and
-    the same pair for total surface precipitation over columns."""
-    rep = {}
-    for k in TENDENCY_FIELDS:
-        if k not in inp:
-            continue
-        keep = ~flipped(out[k], o64[k])
-        t64 = ((o64[k] - inp[k]) / dt)[keep]
-        t32 = ((out[k] - inp[k]) / dt)[keep]
-        nrm = np.linalg.norm(t64)
-        rep[k] = (float(np.linalg.norm(t32 - t64) / nrm) if nrm > 0 else 0.0, float(1.0 - keep.mean()))
-    p64 = sum(d64[n] for n in ("rain_precip", "ice_precip", "snow_precip", "graupel_precip"))
-    p32 = sum(d[n] for n in ("rain_precip", "ice_precip", "snow_precip", "graupel_precip"))
-    keep = ~flipped(p32, p64)
-    return rep, (float(np.linalg.norm((p32 - p64)[keep]) / np.linalg.norm(p64[keep])), float(1.0 - keep.mean()))
+def moved_fraction(inp, out, o64, tol=1e-3):
+    """per field, the fraction of points whose FP32 result moved from the
+    FP64 one by more than ``tol`` relative (a magnitude measure, used only
+    to set our FP32 path beside the reference's own single-precision build,
+    which records no branch signatures)"""
+    return {k: flip_fraction(out[k], o64[k], tol) for k in TENDENCY_FIELDS if k in inp}
+
+
+def check_fp32_report(rep, precip_all=True):
+    """the FP32 tolerances: every tendency within 1e-4 relative L2 over the
+    points that took the branches of the FP64 run, surface precipitation
+    within 1e-5 relative L2 over all columns (precip_all) or over the columns
+    that took the FP64 run's branches"""
+    for k, e in rep["fields"].items():
+        assert e["rel_l2"] <= 1e-4, (k, e)
+    p = rep["precip"]
+    assert p["rel_l2"] <= 1e-5, p
+    if precip_all:
+        assert p["rel_l2_all"] <= 1e-5, p
+
+
+def report_line(tag, rep):
+    from tempo_amd import branch_report as br
+    s = br.summary(rep)
+    print(tag, "flipped points", s["flipped_points"], "columns", s["flipped_columns"])
+    for k, e in s["fields"].items():
+        print(f"  {k}: rel L2 {e['rel_l2']:.3e} (all points {e['rel_l2_all']:.3e})")
+    p = s["precip"]
+    print(f"  precip: rel L2 {p['rel_l2']:.3e} (all columns {p['rel_l2_all']:.3e}), "
+          f"{p['nonzero_columns']} columns with precipitation")
+
+
+@pytest.mark.parametrize("dtype", ["fp32", "fp64"])
+def test_gpu_signature_run_is_the_step(hip, big_case, dtype):
+    """tempo_hip_branch_signature performs tempo_hip_run's step bit for bit,
+    its signatures are deterministic, and a planted threshold crossing (cloud
+    water just above and just below the r1 = 1e-12 presence threshold at one
+    point) changes the signature of that point and no other point's"""
+    from tempo_amd import branch_report as br
+    st, m, _, _, _ = big_case
+    sub = {k: np.ascontiguousarray(v[:, :256]) for k, v in st.items()}
+    tdt = torch.float32 if dtype == "fp32" else torch.float64
+    ctx = context(hip, dict(aerosolaware=True, hailaware=True))
+    want, dw = gpu_run(ctx, sub, m["dt"], m["itimestep"], dtype=tdt)
+    got, dg, sig, _ = br.signature_run(ctx, sub, m["dt"], m["itimestep"], tdt)
+    for k in want:
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+    for k in dw:
+        np.testing.assert_array_equal(dg[k], dw[k], err_msg=k)
+    _, _, sig2, _ = br.signature_run(ctx, sub, m["dt"], m["itimestep"], tdt)
+    np.testing.assert_array_equal(sig, sig2)
+    # a point with only cloud water, at the presence threshold
+    k, c = 30, 17
+    a = {kk: v.copy() for kk, v in sub.items()}
+    for q in ("qc", "qr", "qi", "qs", "qg", "ni", "nr", "nc", "ng", "qb"):
+        a[q][k, c] = 0.0
+    b = {kk: v.copy() for kk, v in a.items()}
+    a["qc"][k, c] = 1.5e-12
+    b["qc"][k, c] = 0.5e-12
+    _, _, sa, _ = br.signature_run(ctx, a, m["dt"], m["itimestep"], tdt)
+    _, _, sb, _ = br.signature_run(ctx, b, m["dt"], m["itimestep"], tdt)
+    diff = (sa != sb).any(axis=0)
+    assert diff[k, c]
+    assert diff.sum() == 1 or diff[:, c].sum() == diff.sum(), np.argwhere(diff)[:8].tolist()
 
 
 @pytest.mark.parametrize("semi", [False, True])
 def test_gpu_fp32_within_tolerance(hip, big_case, gpu_tables, semi):
     """FP32 production path vs the FP64 twin on the same FP32-representable
-    inputs (aerosol- and hail-aware columns), with flux-form and with
-    semi-Lagrangian rain and graupel sedimentation."""
+    inputs (2,048 aerosol- and hail-aware columns), flux-form and
+    semi-Lagrangian sedimentation, with branch accounting.  The
+    semi-Lagrangian surface flux has no low-mass cut-off, so a column whose
+    lowest level sits at a presence threshold flips its surface flux with
+    its branch; its precipitation is bounded over the columns that took the
+    FP64 run's branches, and its flipped columns are reported."""
+    from tempo_amd import branch_report as br
     st, m, _, _, _ = big_case
-    inp = {k: v.astype(np.float32).astype(np.float64) for k, v in st.items()}
     ctx = context(hip, dict(aerosolaware=True, hailaware=True, semi_sedi=semi))
-    o64, d64 = gpu_run(ctx, inp, m["dt"], m["itimestep"], dtype=torch.float64)
-    o32, d32 = gpu_run(ctx, inp, m["dt"], m["itimestep"], dtype=torch.float32)
-    inv = {v: k for k, v in D2_MAP.items()}
-    d64 = {inv[k]: v for k, v in d64.items() if k in inv}
-    d32 = {inv[k]: v for k, v in d32.items() if k in inv}
-    rep, prel = fp32_errors(inp, o32, d32, o64, d64, m["dt"])
-    print("FP32 vs FP64 (rel L2 tendency over unflipped points, flipped fraction):", rep,
-          "precip rel L2", prel)
+    rep = br.compare(ctx, st, m["dt"], m["itimestep"])
+    report_line(f"FP32 vs FP64 (semi={semi}):", rep)
+    check_fp32_report(rep, precip_all=not semi)
+    assert rep["flipped_points"] <= 0.02, rep["flipped_points"]
     from oracle import ref
     if ref.available("sp"):
-        # the reference's own single-precision build against the FP64 twin
+        # the reference's own single-precision build on the same inputs,
+        # against the same FP64 run: points moved by more than 1e-3
+        inp = {k: v.astype(np.float32).astype(np.float64) for k, v in st.items()}
+        o32, _, o64, _ = rep["outputs"]
         ref.init(gpu_tables, variant="sp", aerosolaware=True, hailaware=True, semi_sedi=semi)
         osp = {k: v.copy() for k, v in inp.items()}
-        dsp, _ = ref.run(osp, m["dt"], m["itimestep"], variant="sp")
-        rsp, psp = fp32_errors(inp, osp, dsp, o64, d64, m["dt"])
-        print("reference -DSINGLE_PREC build vs FP64 twin:", rsp, "precip rel L2", psp)
-        for k, (_, f) in rep.items():
-            assert f <= 2.0 * rsp[k][1] + 1e-4, (k, f, rsp[k][1])
-        # semi-Lagrangian surface flux has no low-mass cut-off: a column whose
-        # lowest rain level sits at the r1 threshold may flip (a column or two)
-        assert prel[1] <= 2.0 * psp[1] + 1e-3, (prel, psp)
-    for k, (rel, f) in rep.items():
-        assert rel <= 1e-4, (k, rel)
-        assert f <= 0.02, (k, f)
-    assert prel[0] <= 1e-5, prel
+        ref.run(osp, m["dt"], m["itimestep"], variant="sp")
+        ours, theirs = moved_fraction(inp, o32, o64), moved_fraction(inp, osp, o64)
+        print("points moved > 1e-3, ours vs the reference FP32 build:", ours, theirs)
+        for k in ours:
+            assert ours[k] <= 2.0 * theirs[k] + 1e-4, (k, ours[k], theirs[k])
 
 
 @pytest.mark.parametrize("workload,aero", [("aerosol", True), ("plain", False)])
 def test_gpu_fp32_bench_workload(hip, gpu_tables, workload, aero):
     """FP32 against the FP64 twin on 2,048 columns of the benchmark's
-    aerosol-aware workload and of the one-moment-cloud workload: each
-    tendency within 1e-4 relative L2 over the unflipped points, and flipped
-    fractions within twice those of the reference's own -DSINGLE_PREC build
-    on the same inputs."""
+    aerosol-aware workload and of the one-moment-cloud workload, after eight
+    FP64 steps so that rain has reached the ground: each tendency within
+    1e-4 relative L2 over the points that took the same branches, surface
+    precipitation within 1e-5 over all columns."""
+    from tempo_amd import branch_report as br
     from tempo_amd import generator as gen
-    from oracle import ref
-    if not ref.available("sp"):
-        pytest.skip("reference FP32 build not built")
     st, m = gen.generate(workload, ncol=2048)
-    inp = {k: v.astype(np.float32).astype(np.float64) for k, v in st.items()}
     flags = dict(aerosolaware=aero, hailaware=False)
     ctx = context(hip, flags)
-    o64, d64 = gpu_run(ctx, inp, m["dt"], m["itimestep"], dtype=torch.float64)
-    o32, d32 = gpu_run(ctx, inp, m["dt"], m["itimestep"], dtype=torch.float32)
-    inv = {v: k for k, v in D2_MAP.items()}
-    d64 = {inv[k]: v for k, v in d64.items() if k in inv}
-    d32 = {inv[k]: v for k, v in d32.items() if k in inv}
-    rep, prel = fp32_errors(inp, o32, d32, o64, d64, m["dt"])
-    ref.init(gpu_tables, variant="sp", **flags)
-    osp = {k: v.copy() for k, v in inp.items()}
-    dsp, _ = ref.run(osp, m["dt"], m["itimestep"], variant="sp")
-    rsp, psp = fp32_errors(inp, osp, dsp, o64, d64, m["dt"])
-    print("bench workload FP32 vs FP64:", rep, "reference FP32 build:", rsp)
-    for k, (rel, f) in rep.items():
-        assert rel <= 1e-4, (k, rel, rsp[k][0])
-        assert f <= 2.0 * rsp[k][1] + 1e-4, (k, f, rsp[k][1])
+    st, _ = gpu_run(ctx, st, m["dt"], m["itimestep"], steps=8)
+    rep = br.compare(ctx, st, m["dt"], m["itimestep"] + 8)
+    report_line(f"{workload} after 8 steps:", rep)
+    assert rep["precip"]["nonzero_columns"] > 0
+    check_fp32_report(rep)
+
+
+@pytest.mark.parametrize("semi", [False, True])
+def test_gpu_fp32_cold_surface(hip, semi):
+    """FP32 against the FP64 twin where every surface diagnostic is live:
+    1,024 columns with a below-freezing surface and cloud down to it (rain,
+    freezing rain, ice, snow and graupel reaching the ground)."""
+    from tempo_amd import branch_report as br
+    from tempo_amd import generator as gen
+    st, m = gen.generate("cold", ncol=1024)
+    ctx = context(hip, dict(aerosolaware=True, hailaware=True, semi_sedi=semi))
+    rep = br.compare(ctx, st, m["dt"], m["itimestep"])
+    report_line(f"cold surface (semi={semi}):", rep)
+    p = rep["precip"]
+    for n in br.PRECIP + ("frz_rain_precip",):
+        assert p[n]["nonzero_columns"] > 0, n
+        assert p[n]["rel_l2"] <= 1e-5, (n, p[n])
+    check_fp32_report(rep, precip_all=not semi)
 
 
 def ml_columns(ncol, col0, with_bl):
@@ -488,6 +551,106 @@ def test_gpu_host_path_equals_device_path(hip):
     for k in st:
         np.testing.assert_array_equal(hst[k], dev[k], err_msg=k)
     np.testing.assert_array_equal(hd["refl10cm"], dd["refl10cm"])
+
+
+def _host_call(ctx, hip, st, m, diag, dims, nlev=None):
+    """tempo_hip_run_host through ctypes on numpy arrays with the given diag
+    arrays (name -> float64 array) and geometry"""
+    import ctypes
+    f = hip.Fields()
+    for name in hip.FIELD_NAMES:
+        if name in st:
+            setattr(f, name, st[name].ctypes.data)
+    d = hip.Diags()
+    for n, a in diag.items():
+        setattr(d, n, a.ctypes.data)
+    hip._check(hip.lib().tempo_hip_run_host(ctx._ctx, ctypes.byref(f), ctypes.byref(d), ctypes.byref(dims),
+                                            float(m["dt"]), int(m["itimestep"])))
+
+
+def test_gpu_host_path_padded_diagnostics_and_ragged_columns(hip):
+    """tempo_hip_run_host with diagnostics whose rows are padded (d2ldj,
+    dldk > ni, as a Fortran host's halo'd tile) and ragged columns (nlev):
+    tile points equal the device path's, and every point the step does not
+    write -- padding between rows, levels k >= nlev[c] -- keeps the caller's
+    value"""
+    from tempo_amd import generator as gen
+    st, m = gen.generate("ragged", ncol=96, col0=123)
+    nz, ncol = st["p"].shape
+    nlev = m["nlev"]
+    ctx = context(hip, dict(aerosolaware=False, hailaware=True))
+    dev, dd = gpu_run(ctx, st, m["dt"], m["itimestep"], nlev=nlev)
+    ni, nj, pad = 32, 3, 5
+    assert ni * nj == ncol
+    sentinel = -7.25
+    # state in the Fortran (i, k, j) layout of an (ni, nz, nj) tile, column
+    # c = i + j*ni; diagnostics with rows padded by `pad`
+    def fort(a):
+        return np.ascontiguousarray(a.reshape(nz, nj, ni).transpose(1, 0, 2))
+    hst = {k: fort(v) for k, v in st.items()}
+    ld2 = ni + pad
+    d2 = {n: np.full(ld2 * nj, sentinel) for n in hip.DIAG2_NAMES}
+    ldk3 = ni + pad
+    d3 = {n: np.full(ldk3 * nz * nj, sentinel) for n in hip.DIAG3_NAMES}
+    nl = np.ascontiguousarray(nlev, dtype=np.int32)
+    dims = hip.Dims(ni, nj, nz, ni, ni * nz, ldk3, ldk3 * nz, ld2, nl.ctypes.data)
+    _host_call(ctx, hip, hst, m, dict(d2, **d3), dims)
+    for k in st:
+        np.testing.assert_array_equal(hst[k].transpose(1, 0, 2).reshape(nz, ncol), dev[k], err_msg=k)
+    i = np.arange(ncol) % ni
+    j = np.arange(ncol) // ni
+    for n in hip.DIAG2_NAMES:
+        got = d2[n].reshape(nj, ld2)
+        np.testing.assert_array_equal(got[j, i], dd[n], err_msg=n)
+        assert (got[:, ni:] == sentinel).all(), n
+    for n in hip.DIAG3_NAMES:
+        got = d3[n].reshape(nj, nz, ldk3)
+        assert (got[:, :, ni:] == sentinel).all(), n
+        for c in range(ncol):
+            col = got[j[c], :, i[c]]
+            np.testing.assert_array_equal(col[:nlev[c]], dd[n][:nlev[c], c], err_msg=n)
+            assert (col[nlev[c]:] == sentinel).all(), (n, c)
+
+
+def test_gpu_host_entry_points_share_one_context(hip):
+    """the host-array entry points in growing order of their staging needs
+    (surface emissions, a check routine, the step) on one context give what
+    fresh contexts give"""
+    from tempo_amd import generator as gen
+    st, m = gen.generate("aerosol", ncol=64, col0=4242)
+    nz, ncol = st["p"].shape
+
+    def sequence(ctx_a, ctx_b, ctx_c):
+        nwfa = st["nwfa"].copy()
+        nwfa2d = np.linspace(0.0, 1e4, ncol)
+        ctx_a.surface_emissions_host(nwfa, nwfa2d, 20.0, k=0)
+        rng = np.random.default_rng(3)
+        arr = dict(rho=rng.uniform(0.5, 1.2, 500), l_qi=np.zeros(500, np.int32), qi1d=rng.uniform(0, 1e-4, 500),
+                   ni1d=rng.uniform(0, 1e5, 500), ri=np.zeros(500), ni=np.zeros(500), qiten=np.zeros(500),
+                   niten=np.zeros(500), ilami=np.zeros(500))
+        ctx_b.check_and_update("ice", arr, 20.0)
+        hst = {k: v.copy() for k, v in st.items()}
+        hst["nwfa"] = nwfa
+        d = ctx_c.run_host(hst, m["dt"], m["itimestep"])
+        return nwfa, arr, hst, d
+
+    shared = hip.TempoHip(aerosolaware_flag=1, hailaware_flag=0)
+    got = sequence(shared, shared, shared)
+    fresh = [hip.TempoHip(aerosolaware_flag=1, hailaware_flag=0) for _ in range(3)]
+    want = sequence(*fresh)
+    np.testing.assert_array_equal(got[0], want[0])
+    for k in got[1]:
+        np.testing.assert_array_equal(got[1][k], want[1][k], err_msg=k)
+    for k in got[2]:
+        np.testing.assert_array_equal(got[2][k], want[2][k], err_msg=k)
+    for k in got[3]:
+        np.testing.assert_array_equal(got[3][k], want[3][k], err_msg=k)
+    # and again on the same context, now holding the largest staging
+    again = sequence(shared, shared, shared)
+    for k in got[2]:
+        np.testing.assert_array_equal(again[2][k], want[2][k], err_msg=k)
+    for c in [shared] + fresh:
+        c.close()
 
 
 def test_gpu_single_column_and_tiny_grid(hip):

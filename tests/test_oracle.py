@@ -14,12 +14,17 @@ from oracle import ref
 FLAG_KEYS = set(ref.DEFAULT_FLAGS)
 
 
-def run_oracle(case, tables):
+def run_oracle(case, tables, series=None):
+    """the case's steps on the oracle; series (dict) receives every step's
+    2-D diagnostics"""
     orc.init(tables, **case["meta"]["flags"])
     st = {k: v.copy() for k, v in case["inp"].items()}
     for s in range(case["meta"]["steps"]):
         d2, d3 = orc.run(st, case["meta"]["dt"], case["meta"]["itimestep"] + s, land=case["land"],
                          nlev=case["nlev"])
+        if series is not None:
+            for k, v in d2.items():
+                series.setdefault(k, []).append(v.copy())
     return st, d2, d3
 
 
@@ -30,9 +35,13 @@ def test_oracle_matches_reference_golden(name, golden_tables):
     The tables hold only the entries the case reads (NaN elsewhere), so an
     index error in a table lookup cannot go unnoticed."""
     case = load_case(name)
-    st, d2, d3 = run_oracle(case, golden_tables)
+    series = {}
+    st, d2, d3 = run_oracle(case, golden_tables, series)
     for k, v in case["out"].items():
         np.testing.assert_array_equal(st[k], v, err_msg=f"{name}: state {k}")
+    # surface precipitation (and every other 2-D diagnostic) of every step
+    for k, v in case["ts"].items():
+        np.testing.assert_array_equal(np.array(series[k]), v, err_msg=f"{name}: step series of {k}")
     inp = case["inp"]
     has_q = np.zeros(inp["p"].shape[1], bool)
     for q in ("qc", "qr", "qi", "qs", "qg"):
@@ -52,8 +61,26 @@ def test_oracle_matches_reference_golden(name, golden_tables):
 def test_golden_cases_cover_paths():
     names = set(golden_cases())
     for need in ("plain", "aerosol", "ragged", "multistep", "first_step", "thpii_land", "micro_off",
-                 "no_condensation", "dt300", "mpas_full", "mpas_graupel_sedi"):
+                 "no_condensation", "dt300", "mpas_full", "mpas_graupel_sedi", "cold_surface", "cold_surface_semi",
+                 "ref_cloud_number_aero", "ref_cloud_number_const", "ref_cloud_number_ml"):
         assert need in names
+    # the reference's sedimentation sweep (test/run_tempo_tests.F90:10-40), both forms
+    for dt in (1, 10, 20, 60, 120, 300, 600):
+        for kind in ("snow_sedi", "graupel_sedi", "graupel_semi"):
+            assert f"ref_{kind}_dt{dt}" in names
+
+
+def test_golden_cases_exercise_every_surface_diagnostic():
+    """every 2-D diagnostic, freezing rain and the ice, snow and graupel
+    surface precipitation included, is non-zero in some golden case"""
+    seen = set()
+    for name in golden_cases():
+        case = load_case(name)
+        for src in (case["d2"], case["ts"]):
+            seen.update(k for k, v in src.items() if (np.asarray(v) != 0.0).any())
+    for k in ("rain_precip", "ice_precip", "snow_precip", "graupel_precip", "frz_rain_precip", "frozen_fraction",
+              "max_hail_sfc", "max_hail_column"):
+        assert k in seen, k
 
 
 @pytest.fixture(scope="module")
