@@ -133,16 +133,17 @@ int tempo_hip_run(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip
                   const tempo_hip_dims *dims, double dt, int itimestep, int precision, void *stream);
 
 /* Branch accounting (no counterpart in the reference): the step of
- * tempo_hip_run, bit for bit, that also records per point a 32-bit hash of
- * the threshold decisions its level-local physics took -- every comparison
- * of an `if` on a value, every table index, nint and bin count, the presence
- * flags, clamps and density index of the checks.  sig (device, 2*nz*ni*nj
- * words): sig[p*nz*ncol + k*ncol + c] for point (column c = i + j*ni, level
- * k), p = 0 the physics up to sedimentation (src/module_mp_tempo_main.F90:
- * 320-588), p = 1 the final update (:729-863); levels k >= nlev[c] are not
- * written.  Two runs took the same branches at a point where its two words
- * agree; the sedimentation's substep counts and tops follow from
- * tempo_hip_substeps. */
+ * tempo_hip_run, bit for bit, that also records per point 32-bit hashes of
+ * the threshold decisions its level-local physics took, in two classes:
+ * q = 0 every comparison of an `if` condition on a value; q = 1 the discrete
+ * outcomes -- every table index, nint and bin count, and the checks'
+ * presence flags (l_q*), number clamps and graupel density index.  sig
+ * (device, 4*nz*ni*nj words): sig[(2p + q)*nz*ncol + k*ncol + c] for point
+ * (column c = i + j*ni, level k), p = 0 the physics up to sedimentation
+ * (src/module_mp_tempo_main.F90:320-588), p = 1 the final update
+ * (:729-863); levels k >= nlev[c] are not written.  Two runs took the same
+ * decisions of a class at a point where its words of that class agree; the
+ * sedimentation's substep counts and tops follow from tempo_hip_substeps. */
 int tempo_hip_branch_signature(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_diags *d,
                                const tempo_hip_dims *dims, double dt, int itimestep, int precision, unsigned *sig,
                                void *stream);

@@ -50,6 +50,8 @@ def lib():
         _lib.oracle_trace.argtypes = [vp, ctypes.c_longlong]
         _lib.oracle_surface_emissions.restype = None
         _lib.oracle_surface_emissions.argtypes = [ctypes.c_double, vp, vp] + [ctypes.c_int] * 4
+        _lib.oracle_perturb_pow.restype = None
+        _lib.oracle_perturb_pow.argtypes = [ctypes.c_longlong]
         _lib.oracle_index.restype = ctypes.c_int
         _lib.oracle_index.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
@@ -214,3 +216,20 @@ def surface_emissions(dt, nwfa, nwfa2d, kts):
     assert nwfa.dtype == np.float64 and nwfa.flags.c_contiguous and nwfa2d.flags.c_contiguous
     nj, nk, ni = nwfa.shape
     lib().oracle_surface_emissions(dt, nwfa.ctypes.data, nwfa2d.ctypes.data, ni, nk, nj, kts)
+
+
+class PerturbedPow:
+    """Within the block, one in ``rate`` real powers of the oracle is moved up
+    by one ulp (a sensitivity probe of the reference algorithm; results are
+    not the reference's)."""
+
+    def __init__(self, rate=1000):
+        self.rate = rate
+
+    def __enter__(self):
+        lib().oracle_perturb_pow(self.rate)
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_perturb_pow(0)
+        return False

@@ -10,6 +10,32 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Sensitivity probe (off unless oracle_perturb_pow is called): moves the
+ * result of a deterministic pseudo-random fraction of the real powers x**y
+ * up by one unit in the last place, as a libm that rounds those calls the
+ * other way would.  A computation whose outcome moves far more than that
+ * under the probe is ill-conditioned in the reference algorithm itself. */
+static unsigned long long pert_rate = 0;  /* 1 in pert_rate calls; 0 = off */
+static double oracle_pow(double x, double y)
+{
+    double r = pow(x, y);
+    if (pert_rate) {
+        unsigned long long a, b;
+        memcpy(&a, &x, 8);
+        memcpy(&b, &y, 8);
+        unsigned long long h = a * 31u + b;
+        h ^= h >> 33;
+        h *= 0xff51afd7ed558ccdULL;
+        h ^= h >> 33;
+        h *= 0xc4ceb9fe1a85ec53ULL;
+        h ^= h >> 33;
+        if (h % pert_rate == 0) r = nextafter(r, INFINITY);
+    }
+    return r;
+}
+void oracle_perturb_pow(long long rate) { pert_rate = rate > 0 ? (unsigned long long)rate : 0; }
+#define pow(x, y) oracle_pow((x), (y))
+
 /* the network parameters: data generated from src/module_mp_tempo_ml.F90 */
 #include "../tempo_amd/csrc/tempo_ml_weights.inc"
 

@@ -60,6 +60,8 @@ void oracle_fz_cloud_entry(int i, int j, int k, int m, double *out2);
 /* Trace big-table reads into buf (NULL stops tracing); returns the number of
  * entries recorded since the previous call. */
 long long oracle_trace(long long *buf, long long cap);
+/* sensitivity probe: one in `rate` real powers moved up by one ulp (0: off) */
+void oracle_perturb_pow(long long rate);
 
 /* Table-index helpers for the known-answer tests; see tempo_oracle.c. */
 int oracle_index(int kind, double a, double b, int *i1, int *i2);

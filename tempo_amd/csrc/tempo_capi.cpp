@@ -1155,6 +1155,8 @@ int tempo_hip_substeps(tempo_hip_ctx* x, int* out, size_t n) {
 
 namespace tempo {
 
+int set_error(int code, const char* msg) { return fail(code, msg); }
+
 // One step of tempo_hip_run with its per-point scratch carved from the
 // caller's device block [base, base + bytes) of the context's device:
 // nothing is allocated.  E_ARG when the batch needs more than bytes.

@@ -14,4 +14,8 @@ namespace tempo {
 int run_with_scratch(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags* d, const tempo_hip_dims* dm,
                      double dt, int itimestep, int precision, void* stream, void* base, size_t bytes);
 
+// Records msg as the calling thread's last error (tempo_hip_last_error) and
+// returns code.
+int set_error(int code, const char* msg);
+
 }  // namespace tempo

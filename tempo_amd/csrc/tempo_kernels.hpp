@@ -211,13 +211,17 @@ struct SigScope {
 template <class T>
 struct SigScope<Sig<T>> {
   unsigned* out = nullptr;
+  size_t n3 = 0;
   __device__ SigScope(const StepArgs<Sig<T>>& a, int c, int k, int phase) {
     if (!a.sig) return;
-    out = a.sig + size_t(phase) * size_t(a.ncol) * size_t(a.nz) + size_t(c) + size_t(k) * size_t(a.ncol);
+    n3 = size_t(a.ncol) * size_t(a.nz);
+    out = a.sig + size_t(2 * phase) * n3 + size_t(c) + size_t(k) * size_t(a.ncol);
     sig_begin();
   }
   __device__ ~SigScope() {
-    if (out) *out = sig_end();
+    if (!out) return;
+    out[0] = sig_end(0);
+    out[n3] = sig_end(1);
   }
 };
 

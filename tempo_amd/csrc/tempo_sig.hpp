@@ -10,9 +10,12 @@
 // agree took the same branches there.  min/max and |x| are continuous in
 // their operands, so the selections they make are not recorded.
 //
-// The hash lives in LDS, one word per thread of the block (at most 256
-// threads); a kernel instantiated with Sig opens it with sig_begin() and
-// reads it with sig_end().
+// Two hashes per point separate the kinds of decision: class 0 the
+// comparisons of `if` conditions, class 1 the discrete outcomes -- integer
+// conversions (table indices, nint, bin counts) and the checks' presence
+// flags, clamps and density index.  They live in LDS, one pair of words per
+// thread of the block (at most 256 threads); a kernel instantiated with Sig
+// opens them with sig_begin() and reads them with sig_end(cls).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,24 +24,27 @@
 
 namespace tempo {
 
-__device__ __forceinline__ unsigned& sig_word() {
-  __shared__ unsigned acc[256];
-  return acc[threadIdx.x];
+__device__ __forceinline__ unsigned& sig_word(int cls) {
+  __shared__ unsigned acc[2][256];
+  return acc[cls][threadIdx.x];
 }
-__device__ __forceinline__ void sig_begin() { sig_word() = 2166136261u; }
-__device__ __forceinline__ unsigned sig_end() { return sig_word(); }
+__device__ __forceinline__ void sig_begin() { sig_word(0) = sig_word(1) = 2166136261u; }
+__device__ __forceinline__ unsigned sig_end(int cls) { return sig_word(cls); }
 // FNV-1a step: order-sensitive, so a different sequence of decisions (a
 // flipped branch changes which decisions follow) changes the hash too
-__device__ __forceinline__ void sig_note(unsigned x) {
-  unsigned h = sig_word();
+__device__ __forceinline__ void sig_mix(int cls, unsigned x) {
+  unsigned h = sig_word(cls);
   h = (h ^ (x & 0xffu)) * 16777619u;
   h = (h ^ ((x >> 8) & 0xffu)) * 16777619u;
   h = (h ^ ((x >> 16) & 0xffu)) * 16777619u;
   h = (h ^ (x >> 24)) * 16777619u;
-  sig_word() = h;
+  sig_word(cls) = h;
 }
+// a discrete outcome (class 1)
+__device__ __forceinline__ void sig_note(unsigned x) { sig_mix(1, x); }
+// a comparison (class 0)
 __device__ __forceinline__ bool sig_cmp(bool r) {
-  sig_note(r ? 0x5bd1e995u : 0x1b873593u);
+  sig_mix(0, r ? 0x5bd1e995u : 0x1b873593u);
   return r;
 }
 

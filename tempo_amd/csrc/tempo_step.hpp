@@ -260,7 +260,8 @@ struct StepArgs {
   double odt;
   int first_step;
   // branch signatures of the level-local kernels (signature runs only):
-  // 2 x nz x ncol words, phase p, point (c, k) at p*nz*ncol + k*ncol + c
+  // 4 x nz x ncol words, phase p, class q (0 comparisons, 1 discrete
+  // outcomes), point (c, k) at (2p + q)*nz*ncol + k*ncol + c
   unsigned* sig;
 };
 

@@ -197,9 +197,10 @@ class TempoHip:
 
         ``diags``: True allocates every diagnostic, False none, or a dict of
         preallocated tensors (name -> (ncol,) or (nz, ncol)) to write into.
-        ``signature``: an int32 cuda tensor of shape (2, nz, ncol) runs the
-        same step through tempo_hip_branch_signature and receives the branch
-        signatures of its points (phase A, final update).
+        ``signature``: an int32 cuda tensor of shape (2, 2, nz, ncol) runs
+        the same step through tempo_hip_branch_signature and receives the
+        branch signatures of its points ([phase A, final update][comparisons,
+        discrete outcomes]).
         Returns the dict of diagnostic tensors (2-D ones of shape (ncol,)).
         """
         import torch
@@ -247,9 +248,9 @@ class TempoHip:
         s = stream if stream is not None else torch.cuda.current_stream(ref.device).cuda_stream
         prec = FP32 if dtype == torch.float32 else FP64
         if signature is not None:
-            if signature.dtype != torch.int32 or tuple(signature.shape) != (2, nz, ncol) or \
+            if signature.dtype != torch.int32 or tuple(signature.shape) != (2, 2, nz, ncol) or \
                     not signature.is_contiguous() or not signature.is_cuda:
-                raise ValueError(f"signature: need a contiguous int32 cuda tensor of shape {(2, nz, ncol)}")
+                raise ValueError(f"signature: need a contiguous int32 cuda tensor of shape {(2, 2, nz, ncol)}")
             _check(lib().tempo_hip_branch_signature(self._ctx, ctypes.byref(f), ctypes.byref(d), ctypes.byref(dims),
                                                     float(dt), int(itimestep), prec, signature.data_ptr(),
                                                     ctypes.c_void_p(s)))

@@ -20,6 +20,17 @@ def golden_cases():
     return sorted(os.path.basename(p)[5:-4] for p in glob.glob(os.path.join(GOLDEN, "step_*.npz")))
 
 
+# Golden cases whose outcome the reference algorithm itself does not
+# determine to working precision: moving one in a thousand real powers by one
+# ulp (oracle.PerturbedPow) moves the result by far more than 1e-12
+# (tests/test_oracle.py::test_ill_conditioned_cases).  The semi-Lagrangian
+# graupel remap at dt = 1 s over 1,200 steps amplifies such differences
+# (density-index and number clamps flip and the columns part ways).  A
+# libm that rounds differently cannot follow the reference along them, so
+# they are checked step by step from the reference's own states.
+ILL_CONDITIONED = ("ref_graupel_semi_dt1",)
+
+
 def load_case(name):
     z = np.load(os.path.join(GOLDEN, f"step_{name}.npz"))
     arrays = {k: z[k] for k in z.files}

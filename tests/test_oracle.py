@@ -58,6 +58,28 @@ def test_oracle_matches_reference_golden(name, golden_tables):
         np.testing.assert_array_equal(d3[k], v, err_msg=f"{name}: diag {k}")
 
 
+def _max_rel(a, b):
+    s = np.abs(b).max()
+    return float(np.abs(a - b).max() / s) if s > 0 else float(np.abs(a).max())
+
+
+def test_ill_conditioned_cases(golden_tables):
+    """One in a thousand real powers moved by one ulp (the probe) moves the
+    ill-conditioned cases' outcome by more than 1e-6 and every other case's
+    by no more than 1e-12: the classification the GPU parity tests rely on"""
+    from conftest import ILL_CONDITIONED
+    for name in golden_cases():
+        case = load_case(name)
+        with orc.PerturbedPow(1000):
+            st, d2, d3 = run_oracle(case, golden_tables)
+        worst = max([_max_rel(st[k], v) for k, v in case["out"].items()] +
+                    [_max_rel(d3[k], v) for k, v in case["d3"].items()])
+        if name in ILL_CONDITIONED:
+            assert worst > 1e-6, (name, worst)
+        else:
+            assert worst <= 1e-12, (name, worst)
+
+
 def test_golden_cases_cover_paths():
     names = set(golden_cases())
     for need in ("plain", "aerosol", "ragged", "multistep", "first_step", "thpii_land", "micro_off",
