@@ -132,6 +132,23 @@ int tempo_hip_reserve(tempo_hip_ctx *ctx, int ncol, int nz);
 int tempo_hip_run(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_diags *d,
                   const tempo_hip_dims *dims, double dt, int itimestep, int precision, void *stream);
 
+/* tempo_hip_run without allocation (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The step's per-point
+ * scratch comes from a static device array of 256 MiB per GPU instead of
+ * the context's buffers; batches needing more are rejected with
+ * TEMPO_HIP_E_ARG (per point, levels rounded up to 16 and columns to 8:
+ * 188 bytes in FP32 -- 348 where the lane-per-column sedimentation runs --
+ * and 348 in FP64, plus 88 per column; about 14,800 FP32 columns of 90
+ * levels).  Results equal tempo_hip_run's bit for bit.
+ * Every call on one GPU shares that array: calls that overlap on a GPU
+ * (several streams or host threads) race, and must be serialised by the
+ * caller.  No scratch needs clearing: every value the step reads from it was
+ * written earlier in the same step.  Call with the context's device
+ * current. */
+int tempo_hip_run_noalloc(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_diags *d,
+                          const tempo_hip_dims *dims, double dt, int itimestep, int precision, void *stream);
+
 /* Branch accounting (no counterpart in the reference): the step of
  * tempo_hip_run, bit for bit, that also records per point 32-bit hashes of
  * the threshold decisions its level-local physics took, in two classes:

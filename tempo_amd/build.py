@@ -34,6 +34,7 @@ UNITS = [
     ("tempo_step_f64.hip", DEVICE + ["-ffp-contract=off"]),
     ("tempo_tables.hip", DEVICE + ["-ffp-contract=off"]),
     ("tempo_capi.cpp", ["-x", "hip"] + DEVICE + ["-ffp-contract=off"]),
+    ("tempo_noalloc.hip", DEVICE),
     ("tempo_host.cpp", ["-ffp-contract=off"]),
 ]
 
