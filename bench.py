@@ -4,18 +4,22 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32|fp64]
 
 Workload: 100,000 columns x 90 levels, aerosol-aware, one step of dt = 20 s
-per timed step (the state is stepped in place, itimestep advancing), inputs
-synthesised in device memory by the seeded column generator
-(tempo_amd/generator.py).  Every step runs the whole tempo_run replacement
-(tempo_hip_run: pre-pass, level physics, sedimentation, final update and
-diagnostics refl10cm, re_cloud, re_ice, re_snow and the 2-D precipitation
-and hail fields).  Lookup tables are built on the GPU at init (reported,
-not timed).
+from the seeded inputs (synthesised in device memory by the seeded column
+generator, tempo_amd/generator.py).  Each timed step starts from those
+inputs, restored by a device copy outside the timed events; the same steps
+run in place (the state evolving, itimestep advancing) are reported beside
+it.  Every step runs the whole tempo_run replacement (tempo_hip_run:
+pre-pass, level physics, sedimentation, final update and diagnostics
+refl10cm, re_cloud, re_ice, re_snow and the 2-D precipitation and hail
+fields).  Lookup tables are built on the GPU at init (reported, not timed).
 
 Prints one JSON line with the metric, the roofline of the dominant kernel
-(algorithmic HBM bytes over its event-timed average duration) and the
-reference CPU step timed on this host's cores over a bounded sample of the
-same workload (oracle/cpu_baseline.py, run in a child process).
+(algorithmic HBM bytes over its event-timed average duration, with its VALU
+and transcendental issue rates from the committed counter pass), the FP32
+against FP64 report with branch accounting, the forward + adjoint timings
+(16,384 x 64 single step; 12 steps x 262,144 x 90) and the reference CPU
+step timed on this host's cores over a bounded sample of the same seeded
+step (oracle/cpu_baseline.py, run in a child process).
 """
 import argparse
 import json
@@ -27,6 +31,10 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue of the chip: 256 CUs x 4 SIMDs, one wave-instruction per cycle
+# at 2.4 GHz; transcendental instructions issue at a quarter of that rate
+VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9
+TRANS_PEAK_WAVE_INSTS = VALU_PEAK_WAVE_INSTS / 4
 PHASES = ("k_prepass", "k_phase_a", "k_phase_b", "k_phase_c")
 # 3-D diagnostics the default configuration computes
 DIAG3 = ("refl10cm", "re_cloud", "re_ice", "re_snow")
@@ -53,19 +61,23 @@ def algorithmic_bytes(nz, ncol, width, fields_in, fields_out):
     }
 
 
-def cpu_baseline(ctx, cols, workers, steps):
-    """The reference -O2 build on the host cores, fed the GPU-built tables."""
+def cpu_baseline(ctx, cols, workers, repeats):
+    """The reference -O2 build on the host cores, fed the GPU-built tables:
+    the seeded step of the first ``cols`` columns, ``repeats`` times."""
     with tempfile.TemporaryDirectory(prefix="tempo_tables_") as d:
         ctx.write_tables(d)
         cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--tables", d, "--workload", "aerosol",
-               "--cols", str(cols), "--steps", str(steps)]
+               "--cols", str(cols), "--steps", "1", "--repeats", str(repeats)]
         if workers:
             cmd += ["--workers", str(workers)]
-        r = subprocess.run(cmd, cwd=HERE, capture_output=True, text=True, timeout=600)
+        r = subprocess.run(cmd, cwd=HERE, capture_output=True, text=True, timeout=900)
     if r.returncode != 0:
         raise RuntimeError(f"cpu baseline failed:\n{r.stdout}\n{r.stderr}")
     out = json.loads(r.stdout.strip().splitlines()[-1])
-    return {k: out[k] for k in ("value", "unit", "cores", "kind", "sample")}, out
+    cb = {k: out[k] for k in ("value", "unit", "cores", "kind", "sample")}
+    cb["cpu_model"] = out["cpu_model"]
+    cb["affinity_cores"] = out["affinity_cores"]
+    return cb, out
 
 
 def adjoint_timing(steps):
@@ -112,48 +124,42 @@ def adjoint_timing(steps):
             "mcl_per_s_forward_adjoint": round(ncol * nz / (fa_ms * 1e-3) / 1e6, 2)}
 
 
-def fp32_parity(ncol=8192):
-    """The FP32 path against the FP64 path on the same FP32-representable
-    inputs (one step of the bench workload's first ncol columns): relative
-    L2 of each tendency over the points whose result did not move by more
-    than rounding explains (1e-3 of the value + 1e-6 of the field maximum),
-    the fraction of points that did (branch flips), and the same for total
-    surface precipitation per column.  The FP64 path matches the reference's
-    FP64 build to 1e-12 (the parity tests)."""
+def fp32_parity(ncol=8192, steps_before=8):
+    """The FP32 path against the FP64 path with branch accounting
+    (tempo_amd/branch_report.py) on the bench workload's first ncol columns:
+    the seeded step, and the step after ``steps_before`` FP64 steps, when
+    rain has reached the ground (the seeded step has no surface
+    precipitation yet).  Per tendency: relative L2 over the points that took
+    the FP64 run's branches against the FP64 result rounded to FP32, and
+    over all points against the unrounded FP64 result; the fraction of points
+    with a flipped branch; surface precipitation over all columns.  The FP64
+    path matches the reference's FP64 build to 1e-12 (the parity tests)."""
     import torch
+    from tempo_amd import branch_report as br
     from tempo_amd import generator as gen
     from tempo_amd import host
     ctx = host.TempoHip(aerosolaware_flag=1, hailaware_flag=0)
-    st, m = gen.generate_torch("aerosol", ncol=ncol, device="cuda", dtype=torch.float64)
-    inp = {k: v.float().double() for k, v in st.items()}
-    o64 = {k: v.clone() for k, v in inp.items()}
-    o32 = {k: v.float() for k, v in inp.items()}
-    d64 = ctx.run(o64, m["dt"], m["itimestep"])
-    d32 = ctx.run(o32, m["dt"], m["itimestep"])
+    st, m = gen.generate("aerosol", ncol=ncol)
+
+    def brief(rep):
+        s = br.summary(rep)
+        return {"tendency_rel_l2": {k: float(f"{e['rel_l2']:.3g}") for k, e in s["fields"].items()},
+                "tendency_rel_l2_all_points_unrounded": {k: float(f"{e['rel_l2_raw']:.3g}")
+                                                         for k, e in s["fields"].items()},
+                "flipped_points": float(f"{s['flipped_points']:.3g}"),
+                "flipped_points_discrete": float(f"{s['flipped_points_discrete']:.3g}"),
+                "flipped_sedimentation_columns": float(f"{s['flipped_sedimentation']:.3g}"),
+                "precip_rel_l2_all_columns": float(f"{s['precip']['rel_l2_all']:.3g}"),
+                "precip_columns": s["precip"]["nonzero_columns"]}
+    seeded = brief(br.compare(ctx, st, m["dt"], m["itimestep"]))
+    dev = {k: torch.tensor(v, dtype=torch.float64, device="cuda") for k, v in st.items()}
+    for i in range(steps_before):
+        ctx.run(dev, m["dt"], m["itimestep"] + i)
     torch.cuda.synchronize()
-
-    def cmp(a, b, a_ten=None, b_ten=None):
-        # flipped: the value moved by more than rounding explains; the error
-        # is measured on the tendency (or the value itself) elsewhere
-        flip = (a - b).abs() > 1e-3 * b.abs() + 1e-6 * b.abs().max()
-        keep = ~flip
-        x, y = (a, b) if a_ten is None else (a_ten, b_ten)
-        nrm = torch.linalg.norm(y[keep])
-        rel = float(torch.linalg.norm((x - y)[keep]) / nrm) if nrm > 0 else 0.0
-        return float(f"{rel:.3g}"), float(f"{float(flip.double().mean()):.3g}")
-
-    ten = {}
-    for k in ("t", "qv", "qc", "qr", "qi", "qs", "qg", "ni", "nr", "nc", "nwfa", "nifa"):
-        if k in inp:
-            a32 = o32[k].double()
-            ten[k] = cmp(a32, o64[k], (a32 - inp[k]) / m["dt"], (o64[k] - inp[k]) / m["dt"])
-    names = ("rain_precip", "ice_liquid_equiv_precip", "snow_liquid_equiv_precip", "graupel_liquid_equiv_precip")
-    p64 = sum(d64[n] for n in names)
-    p32 = sum(d32[n].double() for n in names)
-    # (no surface precipitation yet in the first step of this workload: None)
-    prec = cmp(p32, p64) if bool((p64 > 0).any()) else None
-    return {"sample": f"{ncol} columns of the bench workload, one step", "tendency_rel_l2_and_flipped": ten,
-            "precip_rel_l2_and_flipped": prec, "tolerance": {"tendency_rel_l2": 1e-4, "precip_rel_l2": 1e-5}}
+    later = brief(br.compare(ctx, {k: v.cpu().numpy() for k, v in dev.items()}, m["dt"],
+                             m["itimestep"] + steps_before))
+    return {"sample": f"{ncol} columns of the bench workload", "seeded_step": seeded,
+            f"after_{steps_before}_steps": later, "tolerance": {"tendency_rel_l2": 1e-4, "precip_rel_l2": 1e-5}}
 
 
 def ragged_graph(steps_per_graph=10, replays=3, tune=True):
@@ -203,6 +209,23 @@ def ragged_graph(steps_per_graph=10, replays=3, tune=True):
                         f"25 % convective, {steps_per_graph} steps per HIP graph, fp32",
             "ms_per_graph": round(ms, 3), "mcl_per_s": round(levels * steps_per_graph / (ms * 1e-3) / 1e6, 2),
             "tuned": None if best is None else {k: best[k] for k in ("block_a", "block_c", "order", "sed_wave")}}
+
+
+def valu_rates(kernel, precision, kernel_ms):
+    """VALU and transcendental issue of a kernel against the chip's peaks:
+    instruction counts per launch from the committed counter pass
+    (profiles/counters_<kernel>_aerosol_<precision>.json) over this run's
+    event-timed duration; None when there is no counter pass for it."""
+    path = os.path.join(HERE, "profiles", f"counters_{kernel}_aerosol_{precision}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        c = json.load(f)
+    sec = kernel_ms * 1e-3
+    return {"valu_wave_insts": c["valu_wave_insts"], "trans_wave_insts": c["trans_wave_insts"],
+            "valu_frac": round(c["valu_wave_insts"] / sec / VALU_PEAK_WAVE_INSTS, 4),
+            "trans_frac": round(c["trans_wave_insts"] / sec / TRANS_PEAK_WAVE_INSTS, 4),
+            "active_lanes": c.get("active_lanes"), "source": os.path.relpath(path, HERE)}
 
 
 def multistep_adjoint(nsteps=12, ncol=262_144):
@@ -273,16 +296,16 @@ def main():
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
     ap.add_argument("--ncol", type=int, default=100_000)
     ap.add_argument("--cpu-cols", type=int, default=65536, help="columns in the CPU baseline sample")
-    ap.add_argument("--cpu-steps", type=int, default=3, help="consecutive steps in the CPU baseline sample")
-    ap.add_argument("--cpu-workers", type=int, default=0, help="0: the host's cores (at most 16)")
+    ap.add_argument("--cpu-repeats", type=int, default=3, help="seeded steps of the CPU baseline sample")
+    ap.add_argument("--cpu-workers", type=int, default=0, help="0: every core this process may run on")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=float, default=0.0, help="run only a copy of this many GiB")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint timing")
     ap.add_argument("--adjoint-steps", type=int, default=3)
     ap.add_argument("--no-ragged", action="store_true", help="skip the uneven-column HIP-graph run")
     ap.add_argument("--no-parity", action="store_true", help="skip the FP32-against-FP64 report")
-    ap.add_argument("--multistep-adjoint", type=int, default=0,
-                    help="also time a multi-step adjoint over this many columns (262144 for the full run)")
+    ap.add_argument("--multistep-adjoint", type=int, default=262_144,
+                    help="columns of the 12-step chained adjoint timing (0: skip)")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -303,41 +326,56 @@ def main():
     width = 4 if dtype == torch.float32 else 8
 
     ctx = host.TempoHip(aerosolaware_flag=1, hailaware_flag=0)
-    state, meta = gen.generate_torch("aerosol", ncol=a.ncol, device=dev, dtype=dtype)
+    seeded, meta = gen.generate_torch("aerosol", ncol=a.ncol, device=dev, dtype=dtype)
+    state = {k: v.clone() for k, v in seeded.items()}
     nz, ncol = state["p"].shape
     diags = {n: torch.zeros(ncol, dtype=dtype, device=dev) for n in host.DIAG2_NAMES}
     diags.update({n: torch.zeros((nz, ncol), dtype=dtype, device=dev) for n in DIAG3})
     ctx.reserve(ncol, nz)
     stream = torch.cuda.current_stream(dev)
-    it = meta["itimestep"]
 
-    def step():
-        nonlocal it
-        ctx.run(state, meta["dt"], it, diags=diags)
-        it += 1
+    def restore():
+        for k, v in seeded.items():
+            state[k].copy_(v)
 
+    # the workload's step: every timed step from the seeded inputs, restored
+    # by a device copy between the events
     for _ in range(a.warmup):
-        step()
+        restore()
+        ctx.run(state, meta["dt"], meta["itimestep"], diags=diags)
     torch.cuda.synchronize()
-    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0.record(stream)
-    for _ in range(a.steps):
-        step()
-    t1.record(stream)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    for e0, e1 in ev:
+        restore()
+        e0.record(stream)
+        ctx.run(state, meta["dt"], meta["itimestep"], diags=diags)
+        e1.record(stream)
     torch.cuda.synchronize()
-    ms = t0.elapsed_time(t1) / a.steps
+    ms = sum(e0.elapsed_time(e1) for e0, e1 in ev) / a.steps
 
-    # per-kernel durations: HIP events recorded by the library around each
-    # launch, on the stream the kernels run on, over a second set of steps
+    # per-kernel durations of the same seeded step: HIP events recorded by
+    # the library around each launch, on the stream the kernels run on
     ctx.profile(True)
     for _ in range(a.steps):
-        step()
+        restore()
+        ctx.run(state, meta["dt"], meta["itimestep"], diags=diags)
     phase_ms, runs = ctx.profile_read()
     ctx.profile(False)
     phase_ms = phase_ms / max(runs, 1)
     for v in state.values():
         if not torch.isfinite(v).all():
             raise RuntimeError("non-finite state after the timed steps")
+
+    # the same number of steps in place, the state evolving
+    restore()
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for i in range(a.steps):
+        ctx.run(state, meta["dt"], meta["itimestep"] + i, diags=diags)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms_inplace = t0.elapsed_time(t1) / a.steps
 
     levels = nz * ncol
     value = levels / (ms * 1e-3) / 1e6
@@ -354,6 +392,7 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
 
+    valu = valu_rates(dom_name, a.precision, float(phase_ms[dom]))
     out = {
         "metric": "Mcolumn-levels/s per step (forward)",
         "value": round(value, 3),
@@ -367,15 +406,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if width == 4 else "f64",
         "data": "synthetic (seeded counter-based column generator, device-side)",
-        "config": {"workload": f"aerosol-aware step, {ncol} columns x {nz} levels, dt {meta['dt']:g} s",
+        "config": {"workload": f"aerosol-aware step from the seeded inputs, {ncol} columns x {nz} levels, "
+                               f"dt {meta['dt']:g} s",
                    "columns": ncol, "levels": nz, "dt": meta["dt"], "precision": a.precision,
                    "table_build_ms": round(ctx.table_build_ms, 2)},
+        "in_place": {"ms_per_step": round(ms_inplace, 4),
+                     "value": round(levels / (ms_inplace * 1e-3) / 1e6, 3),
+                     "note": f"{a.steps} consecutive steps of the evolving state, itimestep advancing"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": dom_name, "kernel_ms": round(float(phase_ms[dom]), 4),
                      "algorithmic_bytes": dom_bytes,
                      "step_algorithmic_bytes": ab["step"],
-                     "step_achieved": round(ab["step"] / (ms * 1e-3) / 1e9, 2)},
+                     "step_achieved": round(ab["step"] / (ms * 1e-3) / 1e9, 2),
+                     "valu": valu},
         "phase_ms": {PHASES[i]: round(float(phase_ms[i]), 4) for i in range(4)},
         "cpu_baseline": None,
     }
@@ -388,7 +432,7 @@ def main():
     if a.multistep_adjoint:
         out["multistep_adjoint"] = multistep_adjoint(ncol=a.multistep_adjoint)
     if not a.no_cpu_baseline:
-        cb, raw = cpu_baseline(ctx, a.cpu_cols, a.cpu_workers, a.cpu_steps)
+        cb, raw = cpu_baseline(ctx, a.cpu_cols, a.cpu_workers, a.cpu_repeats)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = round(value / cb["value"], 2)
         out["cpu_baseline_wall_s"] = round(raw["wall_s"], 3)

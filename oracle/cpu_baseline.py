@@ -40,18 +40,35 @@ _TABLES = None
 
 
 def _worker(args):
-    workload, col0, ncol, steps, barrier, q = args
+    workload, col0, ncol, steps, repeats, barrier, q = args
     ref.select("o2")
     ref.init(_TABLES, **FLAGS[workload])
-    st, m = gen.generate(workload, ncol=ncol, col0=col0)
+    seeded, m = gen.generate(workload, ncol=ncol, col0=col0)
     if "nlev" in m:
         raise SystemExit("ragged workloads are timed per column by the caller")
-    levels = st["p"].size
+    levels = seeded["p"].size
     barrier.wait()
-    t0 = time.perf_counter()
-    for i in range(steps):
-        ref.run(st, m["dt"], m["itimestep"] + i)
-    q.put((levels * steps, time.perf_counter() - t0))
+    wall = 0.0
+    for r in range(repeats):
+        # every repeat starts again from the seeded state (restored outside the
+        # timed region), so a timed step is the workload's own step
+        st = {k: v.copy() for k, v in seeded.items()}
+        t0 = time.perf_counter()
+        for i in range(steps):
+            ref.run(st, m["dt"], m["itimestep"] + i)
+        wall += time.perf_counter() - t0
+    q.put((levels * steps * repeats, wall))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main(argv=None):
@@ -62,18 +79,21 @@ def main(argv=None):
     ap.add_argument("--cols", type=int, default=65536)
     ap.add_argument("--workers", type=int, default=0)
     ap.add_argument("--col0", type=int, default=0)
-    ap.add_argument("--steps", type=int, default=1, help="consecutive steps of each slice's evolving state")
+    ap.add_argument("--steps", type=int, default=1, help="consecutive steps from the seeded state")
+    ap.add_argument("--repeats", type=int, default=1, help="times the steps are run, each from the seeded state")
     a = ap.parse_args(argv)
     if not ref.available("o2"):
         raise SystemExit("reference -O2 build missing: make -C oracle ref")
-    workers = a.workers or min(16, len(os.sched_getaffinity(0)))
+    affinity = len(os.sched_getaffinity(0))
+    workers = a.workers or affinity
     _TABLES = tio.read_dir(a.tables)
     ctx = mp.get_context("fork")
     barrier = ctx.Barrier(workers)
     q = ctx.Queue()
     per = [a.cols // workers + (1 if i < a.cols % workers else 0) for i in range(workers)]
     starts = [a.col0 + sum(per[:i]) for i in range(workers)]
-    procs = [ctx.Process(target=_worker, args=((a.workload, s, n, a.steps, barrier, q),)) for s, n in zip(starts, per)]
+    procs = [ctx.Process(target=_worker, args=((a.workload, s, n, a.steps, a.repeats, barrier, q),))
+             for s, n in zip(starts, per)]
     for p in procs:
         p.start()
     res = [q.get() for _ in procs]
@@ -84,12 +104,14 @@ def main(argv=None):
     levels = sum(r[0] for r in res)
     wall = max(r[1] for r in res)
     cpu = sum(r[1] for r in res)
+    nzl = levels // a.cols // a.steps // a.repeats
     print(json.dumps(dict(value=levels / wall / 1e6, unit="Mcolumn-levels/s", cores=workers, kind="reference",
-                          sample=f"{a.workload} workload, {a.cols} columns x {levels // a.cols // a.steps} levels "
-                                 f"(columns {a.col0}..{a.col0 + a.cols - 1}), {a.steps} consecutive step(s), "
-                                 f"reference FP64 -O2 "
-                                 f"build, {workers} processes over disjoint column slices",
-                          wall_s=wall, cpu_s=cpu, per_core=levels / cpu / 1e6)))
+                          sample=f"{a.workload} workload, {a.cols} columns x {nzl} levels "
+                                 f"(columns {a.col0}..{a.col0 + a.cols - 1}), {a.steps} step(s) from the seeded "
+                                 f"state, {a.repeats} repeat(s), reference FP64 -O2 build, {workers} processes "
+                                 f"over disjoint column slices",
+                          wall_s=wall, cpu_s=cpu, per_core=levels / cpu / 1e6, affinity_cores=affinity,
+                          online_cpus=os.cpu_count(), cpu_model=cpu_model())))
 
 
 if __name__ == "__main__":
