@@ -263,6 +263,12 @@ int tempo_hip_substeps(tempo_hip_ctx *ctx, int *host_out, size_t n);
  *                               per column
  *   TEMPO_HIP_TUNE_BLOCK_A   threads per block of the level-local physics
  *   TEMPO_HIP_TUNE_BLOCK_C   ... and of the final update: 64, 128 or 256
+ *   TEMPO_HIP_TUNE_FUSED     1: the whole step as one persistent launch that
+ *                               pulls column groups from a work queue and
+ *                               keeps their scratch in the caches (for
+ *                               flux-form sedimentation on at most 128 levels
+ *                               without the melting-layer reflectivity),
+ *                               0: four launches over the batch (default)
  *   TEMPO_HIP_TUNE_ORDER     point grouping of the level-local kernels, a
  *                               bit mask: bit 0 (level-local physics), bit 1
  *                               (final update) set = each block hands its
@@ -275,6 +281,7 @@ int tempo_hip_substeps(tempo_hip_ctx *ctx, int *host_out, size_t n);
 #define TEMPO_HIP_TUNE_BLOCK_A 2
 #define TEMPO_HIP_TUNE_BLOCK_C 3
 #define TEMPO_HIP_TUNE_ORDER 4
+#define TEMPO_HIP_TUNE_FUSED 5
 int tempo_hip_tune(tempo_hip_ctx *ctx, int precision, int key, int value);
 
 /* Per-phase timing of subsequent runs (no counterpart in the reference):

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -19,6 +20,10 @@ namespace tempo {
 // defined in tempo_step_f32.hip / tempo_step_f64.hip / tempo_tables.hip
 hipError_t launch_step(const StepArgs<float>& a, const LaunchCfg& lc, double* vtbuf, hipStream_t s, hipEvent_t* ev);
 hipError_t launch_step(const StepArgs<double>& a, const LaunchCfg& lc, double* vtbuf, hipStream_t s, hipEvent_t* ev);
+hipError_t launch_step_fused(const StepArgs<float>& a, int grid, hipStream_t s, hipEvent_t* ev);
+hipError_t launch_step_fused(const StepArgs<double>& a, int grid, hipStream_t s, hipEvent_t* ev);
+int fused_grid_size(const StepArgs<float>& a);
+int fused_grid_size(const StepArgs<double>& a);
 hipError_t launch_step_sig(const StepArgs<float>& a, const LaunchCfg& lc, double* vtbuf, hipStream_t s);
 hipError_t launch_step_sig(const StepArgs<double>& a, const LaunchCfg& lc, double* vtbuf, hipStream_t s);
 hipError_t launch_step_tlm(const StepArgs<Dual<float>>& a, Dual<double>* vtbuf, hipStream_t s);
@@ -171,6 +176,7 @@ struct tempo_hip_ctx {
   // scratch
   DevBuf sr, sd, pf, cf_, precip, ksub, vtbuf, kmelt, radar;
   DevBuf semiw;  // semi-Lagrangian sedimentation work (allocated only with semi_sedi_flag)
+  DevBuf slots, queue;  // the fused step's block-private scratch slots and its work counter
   DevBuf tl_state, tl_diag;  // dual (value, tangent) copies for the tangent-linear step
   DevBuf adj_in, adj_out, adj_mem;  // adjoint: packed inputs, new state, cotangent/Jacobian planes
   DevBuf adj_snap;                  // adjoint: scratch after phases A and B
@@ -511,6 +517,36 @@ void diag_ptrs(const tempo_hip_diags* d, void* (&d2)[D2_N], void* (&d3)[D3_N]) {
   for (int i = 0; i < D3_N; ++i) d3[i] = v3[i];
 }
 
+// whether a step of a runs as one persistent launch: flux-form
+// sedimentation on at most kSedLevels levels, without the melting-layer
+// reflectivity (whose melting level is a column property of the final state)
+template <class R>
+bool fused_applies(const tempo_hip_ctx* x, const StepArgs<R>& a) {
+  const LaunchCfg& lc = x->launch[sizeof(S_t<R>) == 8 ? 1 : 0];
+  return lc.fused && lc.sed_wave && a.nz <= kSedLevels && !x->cfg.semi_sedi_flag &&
+         !(x->cfg.refl10cm_from_melting_flag && x->cfg.refl10cm_flag);
+}
+
+// the fused step's buffers: per-block scratch slots of one column tile and
+// the per-column results
+template <class R>
+int ensure_fused(tempo_hip_ctx* x, StepArgs<R>& a, int grid, ScratchPtrs& sc) {
+  const size_t sp = fused_slot_plane(a.nz), g = size_t(grid);
+  const size_t bsr = g * SR_N * sp * sizeof(R), bsd = g * SD_N * sp * sizeof(Sd_t<R>);
+  const size_t bpf = g * sp * sizeof(unsigned);
+  HIP_TRY(x->slots.ensure(ScratchSizes::up(bsr) + ScratchSizes::up(bsd) + bpf));
+  HIP_TRY(x->queue.ensure(sizeof(int)));
+  const size_t ncol = size_t(a.ncol);
+  HIP_TRY(x->cf_.ensure(ncol * sizeof(unsigned)));
+  HIP_TRY(x->kmelt.ensure(ncol * sizeof(int)));
+  HIP_TRY(x->precip.ensure(5 * ncol * sizeof(W_t<R>)));
+  HIP_TRY(x->ksub.ensure(10 * ncol * sizeof(int)));
+  char* base = static_cast<char*>(x->slots.p);
+  sc = ScratchPtrs{base, base + ScratchSizes::up(bsr), base + ScratchSizes::up(bsr) + ScratchSizes::up(bsd), nullptr,
+                   x->cf_.p, x->kmelt.p, x->precip.p, x->ksub.p, nullptr};
+  return TEMPO_HIP_OK;
+}
+
 // sig (device, may be NULL): record the level-local kernels' branch
 // signatures there (a signature run; no per-phase timing).  sc: scratch to
 // use instead of the context's own (no allocation, no per-phase timing).
@@ -543,6 +579,26 @@ int run_impl(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags*
   a.dldk = dm->dldk;
   a.dldj = dm->dldj;
   a.d2ldj = dm->d2ldj;
+  if (!sig && !sc && fused_applies(x, a)) {
+    const int grid = fused_grid_size(a);
+    ScratchPtrs fs;
+    int rc = ensure_fused(x, a, grid, fs);
+    if (rc) return rc;
+    if ((rc = bind_common(x, a, dt, itimestep, &fs))) return rc;
+    a.plane = fused_slot_plane(a.nz);
+    a.queue = x->queue.as<int>();
+    hipEvent_t* ev = nullptr;
+    if (x->profile) {
+      for (int i = 0; i < 5; ++i) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        x->events.push_back(e);
+      }
+      ev = &x->events[x->events.size() - 5];
+    }
+    HIP_TRY(launch_step_fused(a, grid, s, ev));
+    return TEMPO_HIP_OK;
+  }
   int rc = bind_common(x, a, dt, itimestep, sc);
   if (rc) return rc;
   double* vt = static_cast<double*>(x->last_vt);
@@ -762,7 +818,21 @@ int tempo_hip_init(const tempo_hip_cfg* cfg, const char* table_dir, const char* 
 
 int tempo_hip_reserve(tempo_hip_ctx* x, int ncol, int nz) {
   if (!x || ncol <= 0 || nz <= 0) return fail(TEMPO_HIP_E_ARG, "tempo_hip_reserve: bad arguments");
-  return ensure_scratch(x, size_t(ncol), size_t(nz));
+  int rc = ensure_scratch(x, size_t(ncol), size_t(nz));
+  if (rc) return rc;
+  StepArgs<double> a;
+  std::memset(&a, 0, sizeof(a));
+  a.ncol = ncol;
+  a.nz = nz;
+  if (fused_applies(x, a)) {  // the larger (FP64) slots serve both precisions
+    ScratchPtrs fs;
+    StepArgs<float> af;
+    std::memset(&af, 0, sizeof(af));
+    af.ncol = ncol;
+    af.nz = nz;
+    if ((rc = ensure_fused(x, a, std::max(fused_grid_size(a), fused_grid_size(af)), fs))) return rc;
+  }
+  return TEMPO_HIP_OK;
 }
 
 static int check_run_args(const tempo_hip_fields* f, const tempo_hip_dims* dm, double dt) {
@@ -1114,6 +1184,10 @@ int tempo_hip_tune(tempo_hip_ctx* x, int precision, int key, int value) {
     case TEMPO_HIP_TUNE_ORDER:
       if (value < 0 || value > 3) return fail(TEMPO_HIP_E_ARG, "order must be 0..3");
       lc.order = value;
+      return TEMPO_HIP_OK;
+    case TEMPO_HIP_TUNE_FUSED:
+      if (value != 0 && value != 1) return fail(TEMPO_HIP_E_ARG, "fused must be 0 or 1");
+      lc.fused = value;
       return TEMPO_HIP_OK;
     default:
       return fail(TEMPO_HIP_E_ARG, "unknown tuning key");

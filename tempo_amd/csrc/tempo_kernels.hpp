@@ -172,12 +172,12 @@ dim3 level_grid(const StepArgs<R>& a, int threads) {
 
 // ----------------------------------------------------------- phase A ----
 template <class R>
-__device__ __forceinline__ R default_aerosol(const StepArgs<R>& a, int c, int k, R n0, R n1) {
+__device__ __forceinline__ R default_aerosol(const Scr<R>& scr, int c, int k, R n0, R n1) {
   // init_water/ice_friendly_aerosols (src/module_mp_tempo_aerosols.F90:14-73):
   // hgt(1) = 0, so h_01 = 0.8; level 1 uses hgt(2)
-  const R* hgt = a.sr + size_t(SR_HGT) * a.plane;
-  const R h1 = hgt[TEMPO_IDX_SCR(a, c, 0)];
-  const R hk = hgt[TEMPO_IDX_SCR(a, c, (k == 0 ? 1 : k))];
+  const R* hgt = scr.sr + size_t(SR_HGT) * scr.plane;
+  const R h1 = hgt[scr.at(c, 0)];
+  const R hk = hgt[scr.at(c, (k == 0 ? 1 : k))];
   const R ni3 = R(-1.0) * mlog(n1 / n0) / R(0.8);
   return n1 + n0 * mexp(-((hk - h1) / R(1000.0)) * ni3);
 }
@@ -225,17 +225,14 @@ struct SigScope<Sig<T>> {
   }
 };
 
+// the level-local physics of point (c, k), from the initial checks to the
+// fall speeds (src/module_mp_tempo_main.F90:320-588); scratch in scr
 template <class R>
-__global__ void __launch_bounds__(256) TEMPO_ATTR_A k_phase_a(StepArgs<R> a) {
-  const int2 pt = block_point<R, false>(a);
-  const int c = pt.x, k = pt.y;
-  if (c >= a.ncol || k >= a.nz) return;
-  if (a.nlev && k >= a.nlev[c]) return;
-  SigScope<R> scope(a, c, k, 0);
+__device__ __forceinline__ void phase_a_point(const StepArgs<R>& a, const Scr<R>& scr, int c, int k) {
   const Const<S_t<R>>& C = *a.cst;
   const size_t o = TEMPO_IDX_STATE(a, c, k);
-  const size_t s = TEMPO_IDX_SCR(a, c, k);
-  const size_t plane = a.plane;
+  const size_t s = scr.at(c, k);
+  const size_t plane = scr.plane;
   const R dt = R(a.dt), odt = R(a.odt);
   const unsigned cf = a.cf[c];
   const bool has_nc = a.f[F_NC], has_nwfa = a.f[F_NWFA], has_nifa = a.f[F_NIFA];
@@ -256,8 +253,8 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_A k_phase_a(StepArgs<R> a) {
   L.pres = a.f[F_P][o];
   L.rho = rho_of(C, L.pres, L.temp, L.qv);
   // aerosols: a present field overrides the default profile (:328-345)
-  L.nwfa = has_nwfa ? nwfa1d * L.rho : default_aerosol(a, c, k, R(kNaccn0), R(kNaccn1));
-  L.nifa = has_nifa ? nifa1d * L.rho : default_aerosol(a, c, k, R(kNain0), R(kNain1));
+  L.nwfa = has_nwfa ? nwfa1d * L.rho : default_aerosol(scr, c, k, R(kNaccn0), R(kNaccn1));
+  L.nifa = has_nifa ? nifa1d * L.rho : default_aerosol(scr, c, k, R(kNain0), R(kNain1));
   L.nwfa = mx(R(kNwfaDefault) * L.rho, mn(R(kAeroMax) * L.rho, L.nwfa));
   L.nifa = mx(R(kNifaDefault) * L.rho, mn(R(kAeroMax) * L.rho, L.nifa));
 
@@ -273,7 +270,7 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_A k_phase_a(StepArgs<R> a) {
   if (a.fl.ml_for_nc && !has_nc) {
     const R xrx = qc1d <= R(S_t<R>(1.e-12)) ? R(0.0) : qc1d;
     ncv = ml_cloud_number(xrx, qr1d, qi1d, qs1d, L.pres, L.temp, w) * L.rho;
-    a.sr[SR_NCSAVE * plane + s] = ncv;
+    scr.sr[SR_NCSAVE * plane + s] = ncv;
   } else {
     ncv = column_nc(a, c);
   }
@@ -431,7 +428,7 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_A k_phase_a(StepArgs<R> a) {
     vtnc = L.rhof * R(kAvC) * C.ccg[3][nu] * C.ocg1[nu] * mpow(L.ilamc, R(kBvC));
   }
 
-  R* sr = a.sr;
+  R* sr = scr.sr;
   sr[SR_TEMP * plane + s] = L.temp;
   sr[SR_QV * plane + s] = L.qv;
   sr[SR_RHO * plane + s] = L.rho;
@@ -453,7 +450,7 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_A k_phase_a(StepArgs<R> a) {
     sr[SR_VISCO * plane + s] = L.visco;
   }
   using D = Sd_t<R>;
-  D* sd = a.sd;
+  D* sd = scr.sd;
   sd[SD_RR * plane + s] = D(L.rr);
   sd[SD_NR * plane + s] = D(L.nr);
   sd[SD_RG * plane + s] = D(L.rg);
@@ -474,7 +471,17 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_A k_phase_a(StepArgs<R> a) {
   sd[SD_NITEN * plane + s] = D(L.niten);
   sd[SD_QCTEN * plane + s] = D(L.qcten);
   sd[SD_NCTEN * plane + s] = D(L.ncten);
-  a.pf[s] = pf;
+  scr.pf[s] = pf;
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) TEMPO_ATTR_A k_phase_a(StepArgs<R> a) {
+  const int2 pt = block_point<R, false>(a);
+  const int c = pt.x, k = pt.y;
+  if (c >= a.ncol || k >= a.nz) return;
+  if (a.nlev && k >= a.nlev[c]) return;
+  SigScope<R> scope(a, c, k, 0);
+  phase_a_point(a, batch_scr(a), c, k);
 }
 
 // ----------------------------------------------------------- phase B ----
@@ -1078,17 +1085,14 @@ __device__ __forceinline__ void atomic_max_pos(Sig<T>* p, Sig<T> v) {
   atomic_max_pos(&p->v, v.v);
 }
 
+// freeze/melt, final update, final checks and diagnostics of point (c, k)
+// (src/module_mp_tempo_main.F90:729-863); scratch in scr
 template <class R>
-__global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
-  const int2 pt = block_point<R, true>(a);
-  const int c = pt.x, k = pt.y;
-  if (c >= a.ncol || k >= a.nz) return;
-  if (a.nlev && k >= a.nlev[c]) return;
-  SigScope<R> scope(a, c, k, 1);
+__device__ __forceinline__ void phase_c_point(const StepArgs<R>& a, const Scr<R>& scr, int c, int k) {
   const Const<S_t<R>>& C = *a.cst;
   const size_t o = TEMPO_IDX_STATE(a, c, k);
-  const size_t s = TEMPO_IDX_SCR(a, c, k);
-  const size_t plane = a.plane;
+  const size_t s = scr.at(c, k);
+  const size_t plane = scr.plane;
   const size_t d3 = size_t(c % a.ni) + size_t(k) * a.dldk + size_t(c / a.ni) * a.dldj;
   const R dt = R(a.dt), odt = R(a.odt);
   const bool live = !(a.cf[c] & CF_EXIT);
@@ -1101,8 +1105,8 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
     return;
   }
   Lvl<R> L;
-  const R* sr = a.sr;
-  const Sd_t<R>* sd = a.sd;
+  const R* sr = scr.sr;
+  const Sd_t<R>* sd = scr.sd;
   L.temp = sr[SR_TEMP * plane + s];
   L.qv = sr[SR_QV * plane + s];
   L.rho = sr[SR_RHO * plane + s];
@@ -1243,6 +1247,16 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
   if (a.d3[D3_RE_C]) a.d3[D3_RE_C][d3] = re_c;
   if (a.d3[D3_RE_I]) a.d3[D3_RE_I][d3] = re_i;
   if (a.d3[D3_RE_S]) a.d3[D3_RE_S][d3] = re_s;
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
+  const int2 pt = block_point<R, true>(a);
+  const int c = pt.x, k = pt.y;
+  if (c >= a.ncol || k >= a.nz) return;
+  if (a.nlev && k >= a.nlev[c]) return;
+  SigScope<R> scope(a, c, k, 1);
+  phase_c_point(a, batch_scr(a), c, k);
 }
 
 // ------------------------------------ public check-and-update routines ----

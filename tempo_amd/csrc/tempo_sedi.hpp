@@ -76,6 +76,7 @@ struct SedTile {
   int c0, ncol, nz;
   int nzs;      // level stride of a scratch column group
   int row;      // LDS row of one column (sed_row(nz))
+  size_t sbase; // scratch offset of the group's first element
 
   __device__ double* slot(int s) const { return lds + s * (kSedCols * row); }
   // Several scratch planes of one stage (the block's column group of each is
@@ -92,7 +93,7 @@ struct SedTile {
   __device__ void load_many(const int (&s)[N], const T* const (&plane)[N]) const {
     T v[kSedChunks][N];
     const int n = kSedCols * nz;
-    const size_t base = size_t(c0) * nzs;
+    const size_t base = sbase;
 #pragma unroll
     for (int it = 0; it < kSedChunks; ++it) {
       if (it * 64 * kSedCols >= n) break;
@@ -117,7 +118,7 @@ struct SedTile {
     unsigned vf[kSedChunks];
     R vz[kSedChunks], vh[kSedChunks], vr[kSedChunks];
     const int n = kSedCols * nz;
-    const size_t base = size_t(c0) * nzs;
+    const size_t base = sbase;
 #pragma unroll
     for (int it = 0; it < kSedChunks; ++it) {
       if (it * 64 * kSedCols >= n) break;
@@ -143,7 +144,7 @@ struct SedTile {
   template <class T>
   __device__ void store(int s, T* plane) const {
     const double* d = slot(s);
-    T* dst = plane + size_t(c0) * nzs;
+    T* dst = plane + sbase;
     for (int e = threadIdx.x; e < kSedCols * nz; e += blockDim.x) {
       const int cc = e % kSedCols, k = e / kSedCols;
       if (c0 + cc < ncol) dst[e] = T(d[cc * row + k]);
@@ -250,19 +251,20 @@ __device__ __forceinline__ double wave_sediment(const SedCol& g, double (&xr)[kS
   return x0 > kLowLimitPrecip ? s0 * dt * ost : 0.0;
 }
 
+// Sedimentation of the column group c0 .. c0 + kSedCols - 1, one wavefront
+// per column (64 * kSedCols threads); scr holds the group's scratch,
+// tile_mem sed_lds_bytes(nz) of LDS.
 template <class R>
-__global__ void __launch_bounds__(64 * kSedCols) __attribute__((amdgpu_waves_per_eu(6))) k_phase_b_wave(StepArgs<R> a) {
-  extern __shared__ double tile_mem[];  // sed_lds_bytes(nz)
+__device__ __forceinline__ void phase_b_group(const StepArgs<R>& a, const Scr<R>& scr, int c0, double* tile_mem) {
   __shared__ unsigned blk_any;
   const int w = int(threadIdx.x >> 6);
-  const int c0 = int(blockIdx.x) * kSedCols;
   const int c = c0 + w;
   const bool col_ok = c < a.ncol;
-  const size_t plane = a.plane;
-  const SedTile T{tile_mem, c0, a.ncol, a.nz, a.nzs, sed_row(a.nz)};
+  const size_t plane = scr.plane;
+  const SedTile T{tile_mem, c0, a.ncol, a.nz, scr.nzs, sed_row(a.nz), scr.at(c0, 0)};
   const double dt = a.dt;
-  Sd_t<R>* sd = a.sd;
-  const R* sr = a.sr;
+  Sd_t<R>* sd = scr.sd;
+  const R* sr = scr.sr;
 
   SedCol g;
   g.nzc = col_ok ? (a.nlev ? min(max(a.nlev[c], 1), a.nz) : a.nz) : 0;
@@ -271,7 +273,7 @@ __global__ void __launch_bounds__(64 * kSedCols) __attribute__((amdgpu_waves_per
 
   // column flags (which species hold mass) and the level geometry
   if (threadIdx.x == 0) blk_any = 0;
-  T.load_head(a, a.pf, sr + SR_RHO * plane, sd + SD_RR * plane);
+  T.load_head(a, scr.pf, sr + SR_RHO * plane, sd + SD_RR * plane);
   __syncthreads();
   unsigned pfl[kSedChunks];
   R dz[kSedChunks], rho[kSedChunks];
@@ -482,11 +484,17 @@ __global__ void __launch_bounds__(64 * kSedCols) __attribute__((amdgpu_waves_per
     a.d2[D2_FROZEN][d2] = live ? (ice + snow + graupel) / (ice + snow + graupel + rain + R(kR1)) : R(0);
   if (a.d2[D2_FRZRAIN]) {
     R frz = R(0);
-    if (live && (a.sr[SR_TEMP * plane + TEMPO_IDX_SCR(a, c, 0)] - R(kT0)) < R(-0.5)) frz = rain + cloud;
+    if (live && (sr[SR_TEMP * plane + scr.at(c, 0)] - R(kT0)) < R(-0.5)) frz = rain + cloud;
     a.d2[D2_FRZRAIN][d2] = frz;
   }
   if (a.d2[D2_HAIL_SFC]) a.d2[D2_HAIL_SFC][d2] = R(0);
   if (a.d2[D2_HAIL_COL]) a.d2[D2_HAIL_COL][d2] = R(0);
+}
+
+template <class R>
+__global__ void __launch_bounds__(64 * kSedCols) __attribute__((amdgpu_waves_per_eu(6))) k_phase_b_wave(StepArgs<R> a) {
+  extern __shared__ double tile_mem[];  // sed_lds_bytes(nz)
+  phase_b_group(a, batch_scr(a), int(blockIdx.x) * kSedCols, tile_mem);
 }
 
 // ------------------------------------------------------------- launch ----

@@ -259,6 +259,8 @@ struct StepArgs {
   // x + (-x/dt)*dt of a fully removed mass from cancelling
   double odt;
   int first_step;
+  // work queue of the fused step (a counter of column groups handed out)
+  int* queue;
   // branch signatures of the level-local kernels (signature runs only):
   // 4 x nz x ncol words, phase p, class q (0 comparisons, 1 discrete
   // outcomes), point (c, k) at (2p + q)*nz*ncol + k*ncol + c
@@ -291,6 +293,7 @@ struct DualGeom {
 
 // launch shape of one step (host-side choice, see tempo_hip_tune)
 struct LaunchCfg {
+  int fused = 0;      // 1: the whole step in one persistent launch where it applies (k_step_fused)
   int sed_wave = 1;   // sedimentation: 1 = one wavefront per column (nz <= 128), 0 = one lane per column
   int order = 0;      // point grouping within blocks: bit 0 phase A, bit 1 final update
   int block_a = 256;  // threads per block of the level-local kernels (64, 128 or 256)
@@ -308,8 +311,31 @@ struct LaunchCfg {
 constexpr int kScrW = 8;
 constexpr int kScrLevels = 16;
 __host__ __device__ constexpr int scratch_levels(int nz) { return (nz + kScrLevels - 1) / kScrLevels * kScrLevels; }
+// elements of one plane of a fused-step block's scratch slot: one column tile
+__host__ __device__ constexpr size_t fused_slot_plane(int nz) { return size_t(kScrW) * size_t(scratch_levels(nz)); }
 #define TEMPO_IDX_SCR(a, c, k) \
   ((size_t((c) / kScrW) * size_t((a).nzs) + size_t(k)) * kScrW + size_t((c) % kScrW))
+
+// Where the per-point scratch of a kernel lives: the batch's column-tiled
+// planes (c0 = 0; element of (c, k) at TEMPO_IDX_SCR), or the tile of one
+// column group starting at column c0 (the fused step's block-private slot),
+// indexed the same way relative to c0.
+template <class R>
+struct Scr {
+  R* sr;
+  Sd_t<R>* sd;
+  unsigned* pf;
+  size_t plane;  // elements per plane
+  int c0, nzs;
+  __device__ __forceinline__ size_t at(int c, int k) const {
+    const int cc = c - c0;
+    return (size_t(cc / kScrW) * size_t(nzs) + size_t(k)) * kScrW + size_t(cc % kScrW);
+  }
+};
+template <class R>
+__device__ __forceinline__ Scr<R> batch_scr(const StepArgs<R>& a) {
+  return Scr<R>{a.sr, a.sd, a.pf, a.plane, 0, a.nzs};
+}
 
 // ------------------------------------------------ level-local physics ----
 // Per-level working state: the scalars of one (column, level) point.

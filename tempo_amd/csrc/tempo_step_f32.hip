@@ -2,6 +2,7 @@
 // fluxes carried in FP64 inside phase B).
 #include "tempo_kernels.hpp"
 #include "tempo_sedi.hpp"
+#include "tempo_fused.hpp"
 #include "tempo_adjoint_wave.hpp"
 
 namespace tempo {
@@ -10,6 +11,12 @@ namespace tempo {
 hipError_t launch_step(const StepArgs<float>& a, const LaunchCfg& lc, double* vtbuf, hipStream_t s, hipEvent_t* ev) {
   return launch_step_impl(a, a, lc, vtbuf, s, ev);
 }
+
+// the step in one persistent launch of `grid` blocks (fused_grid)
+hipError_t launch_step_fused(const StepArgs<float>& a, int grid, hipStream_t s, hipEvent_t* ev) {
+  return launch_step_fused_impl(a, grid, s, ev);
+}
+int fused_grid_size(const StepArgs<float>& a) { return fused_grid<float>(a.ncol, a.nz); }
 
 // the same step with the level-local kernels recording branch signatures
 // into a.sig

@@ -2,6 +2,7 @@
 // every operation rounds as in the reference FP64 build.
 #include "tempo_kernels.hpp"
 #include "tempo_sedi.hpp"
+#include "tempo_fused.hpp"
 #include "tempo_adjoint_wave.hpp"
 
 namespace tempo {
@@ -10,6 +11,12 @@ namespace tempo {
 hipError_t launch_step(const StepArgs<double>& a, const LaunchCfg& lc, double* vtbuf, hipStream_t s, hipEvent_t* ev) {
   return launch_step_impl(a, a, lc, vtbuf, s, ev);
 }
+
+// the step in one persistent launch of `grid` blocks (fused_grid)
+hipError_t launch_step_fused(const StepArgs<double>& a, int grid, hipStream_t s, hipEvent_t* ev) {
+  return launch_step_fused_impl(a, grid, s, ev);
+}
+int fused_grid_size(const StepArgs<double>& a) { return fused_grid<double>(a.ncol, a.nz); }
 
 // the same step with the level-local kernels recording branch signatures
 // into a.sig

@@ -873,6 +873,35 @@ def test_gpu_launch_shapes_give_identical_results(hip, dtype):
         print(wl, dtype, "max substeps", base[2][0::2].max(), "identical over", len(results), "settings")
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("workload", ["aerosol", "ragged", "multistep"])
+def test_gpu_fused_step_equals_four_launches(hip, workload, dtype):
+    """The persistent work-queued launch (k_step_fused) gives the four-launch
+    step's state, diagnostics and substep counts bit for bit, on
+    uniform, ragged and hail- and aerosol-aware columns"""
+    from tempo_amd import generator as gen
+    st, m = gen.generate(workload, ncol=1500, col0=4000)
+    flags = dict(aerosolaware_flag=int(m["aerosol"]), hailaware_flag=int(m["hail"]))
+    prec = hip.FP32 if dtype == torch.float32 else hip.FP64
+    nl = None if "nlev" not in m else torch.tensor(m["nlev"], dtype=torch.int32, device="cuda")
+    out = []
+    for fused in (0, 1):
+        ctx = hip.TempoHip(**flags)
+        ctx.tune(prec, hip.TUNE_FUSED, fused)
+        dev = to_dev(st, dtype)
+        for i in range(2):
+            d = ctx.run(dev, m["dt"], m["itimestep"] + i, nlev=nl)
+        torch.cuda.synchronize()
+        out.append((dev, d, ctx.substeps(st["p"].shape[1])))
+        ctx.close()
+    (a, da, ka), (b, db, kb) = out
+    for k in a:
+        assert torch.equal(a[k], b[k]), k
+    for k in da:
+        assert torch.equal(da[k], db[k]), k
+    np.testing.assert_array_equal(ka, kb)
+
+
 def test_gpu_tune_rejects_bad_values(hip):
     ctx = context(hip, dict(aerosolaware=False, hailaware=False))
     for key, val in ((hip.TUNE_SED_WAVE, 2), (hip.TUNE_BLOCK_A, 96), (hip.TUNE_BLOCK_C, 512), (hip.TUNE_ORDER, 4),
