@@ -628,13 +628,14 @@ int run_tlm(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fields*
   using D = Dual<T>;
   if (x->cfg.semi_sedi_flag)
     return fail(TEMPO_HIP_E_UNSUPPORTED, "the tangent-linear step covers flux-form sedimentation only");
-  if (x->cfg.ml_for_nc_flag || f->qc_bl || f->qcfrac_bl)
-    return fail(TEMPO_HIP_E_UNSUPPORTED, "the tangent-linear step does not cover the cloud-number network");
+  if ((f->qc_bl || f->qcfrac_bl) && !(x->cfg.ml_for_nc_flag || x->cfg.ml_for_bl_nc_flag))
+    return fail(TEMPO_HIP_E_ARG, "qc_bl/qcfrac_bl need the cloud-number network (ml_for_bl_nc_flag or ml_for_nc_flag)");
   if (x->cfg.refl10cm_from_melting_flag)
     return fail(TEMPO_HIP_E_UNSUPPORTED, "the tangent-linear step does not cover the melting-layer reflectivity");
   const int ncol = dm->ni * dm->nj, nz = dm->nz;
   const size_t n3 = size_t(ncol) * nz;
-  HIP_TRY(x->tl_state.ensure(F_NFIELDS * n3 * sizeof(D)));
+  const bool bl = f->qc_bl && f->qcfrac_bl;
+  HIP_TRY(x->tl_state.ensure((F_NFIELDS + (bl ? 2 : 0)) * n3 * sizeof(D)));
   HIP_TRY(x->tl_diag.ensure((D2_N * size_t(ncol) + D3_N * n3) * sizeof(D)));
   StepArgs<D> a;
   std::memset(&a, 0, sizeof(a));
@@ -647,6 +648,13 @@ int run_tlm(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fields*
     if (!fp[i]) continue;
     a.f[i] = st + size_t(i) * n3;
     HIP_TRY(launch_dual_pack(static_cast<const T*>(fp[i]), static_cast<const T*>(tp[i]), g3, a.f[i], s));
+  }
+  if (bl) {  // boundary-layer cloud: constants of the step (zero tangent)
+    D* qbl = st + F_NFIELDS * n3;
+    HIP_TRY(launch_dual_pack(static_cast<const T*>(f->qc_bl), static_cast<const T*>(nullptr), g3, qbl, s));
+    HIP_TRY(launch_dual_pack(static_cast<const T*>(f->qcfrac_bl), static_cast<const T*>(nullptr), g3, qbl + n3, s));
+    a.qc_bl = qbl;
+    a.qcfrac_bl = qbl + n3;
   }
   void *d2[D2_N] = {}, *d3[D3_N] = {}, *t2[D2_N] = {}, *t3[D3_N] = {};
   if (d) diag_ptrs(d, d2, d3);
@@ -688,8 +696,10 @@ int run_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fie
   using D = Dual<T>;
   if (x->cfg.semi_sedi_flag)
     return fail(TEMPO_HIP_E_UNSUPPORTED, "the adjoint covers flux-form sedimentation only");
-  if (x->cfg.ml_for_nc_flag || f->qc_bl || f->qcfrac_bl)
-    return fail(TEMPO_HIP_E_UNSUPPORTED, "the adjoint does not cover the cloud-number network");
+  if ((f->qc_bl || f->qcfrac_bl) && !(x->cfg.ml_for_nc_flag || x->cfg.ml_for_bl_nc_flag))
+    return fail(TEMPO_HIP_E_ARG, "qc_bl/qcfrac_bl need the cloud-number network (ml_for_bl_nc_flag or ml_for_nc_flag)");
+  // qc_bl and qcfrac_bl enter only the effective radii, which are not among
+  // the adjoint's outputs: they are not bound
   if (x->cfg.refl10cm_from_melting_flag)
     return fail(TEMPO_HIP_E_UNSUPPORTED, "the adjoint does not cover the melting-layer reflectivity");
   if (!f->t) return fail(TEMPO_HIP_E_UNSUPPORTED, "the adjoint takes temperature input t (not th/pii)");

@@ -291,6 +291,43 @@ def test_twelve_step_adjoint_at_4096_columns(hip, dtype, tol):
     assert abs(lin - adj) <= tol * abs(lin)
 
 
+def fd_match(tl_all, F, s0, eps, fields):
+    """Per column: does the tangent tl_all match the central difference of
+    F (a reference step at input + e * direction) to 1e-5 of the column's
+    tangent scale (plus 2 % of the spread of the one-sided differences where
+    the step is strongly curved), and does it match that or the one-sided
+    difference on one side (relative 1e-3)?  A column that matches neither
+    at eps (thresholds on both sides of the perturbation, or curvature) is
+    tried again with steps of eps/10 and eps/100, where the thresholds it met
+    fall outside the perturbation.  Returns (central_ok, any_ok, retried)."""
+    def check(e):
+        sp, sm = F(e), F(-e)
+        ncol = s0["p"].shape[1]
+        central_ok = np.ones(ncol, bool)
+        any_ok = np.ones(ncol, bool)
+        for k in fields:
+            if k not in tl_all:
+                continue
+            tl = tl_all[k]
+            cd = (sp[k] - sm[k]) / (2 * e)
+            fp = (sp[k] - s0[k]) / e
+            fm = (s0[k] - sm[k]) / e
+            scale = np.maximum(np.abs(tl).max(axis=0), 1e-300)
+            ec = np.abs(cd - tl).max(axis=0) / scale
+            es = np.minimum(np.abs(fp - tl).max(axis=0), np.abs(fm - tl).max(axis=0)) / scale
+            spread = np.abs(fp - fm).max(axis=0) / scale
+            central_ok &= ec <= 1e-5 + 0.02 * spread
+            any_ok &= (ec <= 1e-5 + 0.02 * spread) | (es <= 1e-3)
+        return central_ok, any_ok
+    central_ok, any_ok = check(eps)
+    retried = ~any_ok
+    for e2 in (eps / 10, eps / 100):
+        if any_ok.all():
+            break
+        any_ok |= check(e2)[1]
+    return central_ok, any_ok, retried
+
+
 def test_twelve_step_tlm_against_reference_fd(hip, ref_tables):
     """The chained 12-step tangent-linear model, link by link, against central
     finite differences of the reference FP64 build (relative step 1e-6) on 64
@@ -345,35 +382,8 @@ def test_twelve_step_tlm_against_reference_fd(hip, ref_tables):
             y["refl10cm"] = d3["refl10cm"]
             return y
 
-        def check(e, s0):
-            sp, sm = F(e), F(-e)
-            central_ok = np.ones(ncol, bool)
-            any_ok = np.ones(ncol, bool)
-            for k in fields:
-                if k not in tl_all:
-                    continue
-                tl = tl_all[k]
-                cd = (sp[k] - sm[k]) / (2 * e)
-                fp = (sp[k] - s0[k]) / e
-                fm = (s0[k] - sm[k]) / e
-                scale = np.maximum(np.abs(tl).max(axis=0), 1e-300)
-                ec = np.abs(cd - tl).max(axis=0) / scale
-                es = np.minimum(np.abs(fp - tl).max(axis=0), np.abs(fm - tl).max(axis=0)) / scale
-                spread = np.abs(fp - fm).max(axis=0) / scale
-                central_ok &= ec <= 1e-5 + 0.02 * spread
-                any_ok &= (ec <= 1e-5 + 0.02 * spread) | (es <= 1e-3)
-            return central_ok, any_ok
         s0 = F(0.0)
-        central_ok, any_ok = check(eps, s0)
-        # a column that matches neither difference at 1e-6 (thresholds on both
-        # sides of the perturbation, or curvature) is tried again with smaller
-        # steps, where the thresholds it met fall outside the perturbation
-        retried = ~any_ok
-        for e2 in (1e-7, 1e-8):
-            if any_ok.all():
-                break
-            c2, a2 = check(e2, s0)
-            any_ok |= a2
+        central_ok, any_ok, retried = fd_match(tl_all, F, s0, eps, fields)
         print(f"step {n + 1}: central {int(central_ok.sum())}/{ncol}, one-sided for the rest "
               f"{int((any_ok & ~central_ok & ~retried).sum())}, with a smaller step {int((any_ok & retried).sum())}, "
               f"unmatched {int((~any_ok).sum())}")
@@ -386,3 +396,84 @@ def test_twelve_step_tlm_against_reference_fd(hip, ref_tables):
     print("12-step TLM link by link:", n_central, "central and", n_ok, "matching of", n_all)
     assert n_ok == n_all
     assert n_central >= 0.9 * n_all
+
+
+# the cloud-number network (src/module_mp_tempo_ml.F90): workload, library
+# and reference flags
+ML = {"ml_for_nc": ("plain", dict(aerosolaware_flag=0, hailaware_flag=0, ml_for_nc_flag=1),
+                    dict(aerosolaware=False, hailaware=False, ml_for_nc=True)),
+      "ml_for_bl_nc": ("aerosol", dict(aerosolaware_flag=1, hailaware_flag=0, ml_for_bl_nc_flag=1),
+                       dict(aerosolaware=True, hailaware=False, ml_for_bl_nc=True))}
+
+
+@pytest.mark.parametrize("which", sorted(ML))
+def test_cloud_number_network_derivatives(hip, ref_tables, which):
+    """Derivatives with the cloud-number network on 128 generated columns:
+    ml_for_nc (one-moment cloud, its number predicted at every level; the
+    final update reads the prediction back) and ml_for_bl_nc (boundary-layer
+    cloud water and fraction, constants of the step, in the cloud radius).
+    The FP64 tangent-linear values are the step's bit for bit, effective
+    radii included; the adjoint passes the dot-product test against the
+    tangent-linear model (relative 1e-10); and the tangent-linear model
+    matches central differences of the reference FP64 build (relative step
+    1e-6) on every output field, reflectivity and cloud radius included --
+    per column as in the 12-step link test (every column matching the
+    central or the forward branch's one-sided difference, at least 90 % the
+    central)."""
+    from oracle import ref
+    from tempo_amd import generator as gen
+    workload, cfg, rflags = ML[which]
+    ctx = hip.TempoHip(**cfg)
+    st, m = gen.generate(workload, ncol=128, col0=7000)
+    nz, ncol = st["p"].shape
+    rng = np.random.default_rng(21)
+    st["t"] = st["t"] + rng.uniform(-0.01, 0.01, st["t"].shape)  # off the tables' half-integer jumps
+    if which == "ml_for_bl_nc":
+        qbl = np.where(rng.random((nz, ncol)) < 0.4, rng.uniform(1e-8, 1e-3, (nz, ncol)), 0.0)
+        st["qc_bl"] = np.ascontiguousarray(qbl)
+        st["qcfrac_bl"] = np.ascontiguousarray(np.where(qbl > 0.0, rng.uniform(0.05, 1.0, (nz, ncol)), 0.0))
+    names = [k for k in X_NAMES if k in st]
+    v = {k: st[k] * rng.uniform(-1, 1, st[k].shape) for k in names}
+    # values
+    a = dev(st)
+    da = ctx.run(a, m["dt"], m["itimestep"])
+    b = dev(st)
+    db, dtl, tb = ctx.tlm(b, {k: torch.tensor(x, device="cuda") for k, x in v.items()}, m["dt"], m["itimestep"])
+    torch.cuda.synchronize()
+    for k in a:
+        assert torch.equal(a[k], b[k]), k
+    for k in ("rain_precip", "refl10cm", "re_cloud"):
+        assert torch.equal(da[k], db[k]), k
+    assert float((da["re_cloud"] > 0).double().mean()) > 0.05
+    # adjoint vs tangent-linear
+    ybar = {k: torch.tensor(rng.standard_normal((nz, ncol)) / (np.abs(st[k]).max() + 1e-30), device="cuda")
+            for k in names}
+    dbar = {"refl10cm": torch.tensor(rng.standard_normal((nz, ncol)) * 0.01, device="cuda"),
+            "rain_precip": torch.tensor(rng.standard_normal(ncol) * 1e3, device="cuda")}
+    lhs = sum(float((ybar[k] * tb[k]).sum()) for k in ybar) + sum(float((dbar[k] * dtl[k]).sum()) for k in dbar)
+    xbar = ctx.adjoint(dev(st), ybar, dbar, m["dt"], m["itimestep"], wrt=tuple(names))
+    torch.cuda.synchronize()
+    rhs = sum(float((xbar[k] * torch.tensor(v[k], device="cuda")).sum()) for k in names)
+    print(which, "dot test", lhs, rhs, abs(lhs - rhs) / abs(lhs))
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    # tangent-linear vs the reference
+    if not ref.available("o0"):
+        pytest.skip("reference build not present")
+    ref.init(ref_tables, variant="o0", **rflags)
+    tl_all = {k: tb[k].cpu().numpy() for k in tb}
+    for k in ("refl10cm", "re_cloud"):
+        tl_all[k] = dtl[k].cpu().numpy()
+
+    def F(e):
+        y = {k: x.copy() for k, x in st.items()}
+        for k in v:
+            y[k] = y[k] + e * v[k]
+        _, d3 = ref.run(y, m["dt"], m["itimestep"], variant="o0")
+        y["refl10cm"], y["re_cloud"] = d3["refl10cm"], d3["re_cloud"]
+        return y
+    fields = ("t", "qv", "qc", "qr", "qi", "qs", "qg", "nr", "nc", "ni", "refl10cm", "re_cloud")
+    central_ok, any_ok, retried = fd_match(tl_all, F, F(0.0), 1e-6, fields)
+    print(which, "TLM vs reference: central", int(central_ok.sum()), "one-sided or smaller step",
+          int((any_ok & ~central_ok).sum()), "unmatched", int((~any_ok).sum()), "of", ncol)
+    assert any_ok.all(), np.where(~any_ok)[0]
+    assert central_ok.mean() >= 0.9

@@ -793,17 +793,23 @@ def test_gpu_error_codes(hip):
         semi.tlm(dict(dev), {k: torch.zeros_like(v) for k, v in dev.items()}, m["dt"], 2)
     assert e.value.code == hip.E_UNSUPPORTED
     semi.close()
-    # boundary-layer cloud without the network; derivatives with the network
+    # boundary-layer cloud without the network, in the step and its derivatives
     bl = dict(dev, qc_bl=torch.zeros_like(dev["qc"]), qcfrac_bl=torch.zeros_like(dev["qc"]))
     with pytest.raises(hip.TempoError) as e:
         ctx.run(bl, m["dt"], 2)
     assert e.value.code == hip.E_ARG
-    for opt in ("ml_for_nc_flag", "refl10cm_from_melting_flag"):
-        other = hip.TempoHip(aerosolaware_flag=0, hailaware_flag=0, **{opt: 1})
-        with pytest.raises(hip.TempoError) as e:
-            other.tlm(dict(dev), {k: torch.zeros_like(v) for k, v in dev.items()}, m["dt"], 2)
-        assert e.value.code == hip.E_UNSUPPORTED, opt
-        other.close()
+    with pytest.raises(hip.TempoError) as e:
+        ctx.tlm(dict(bl), {k: torch.zeros_like(v) for k, v in dev.items()}, m["dt"], 2)
+    assert e.value.code == hip.E_ARG
+    with pytest.raises(hip.TempoError) as e:
+        ctx.adjoint(dict(bl), {}, {}, m["dt"], 2)
+    assert e.value.code == hip.E_ARG
+    # the melting-layer reflectivity (binned integrals) has no derivative here
+    other = hip.TempoHip(aerosolaware_flag=0, hailaware_flag=0, refl10cm_from_melting_flag=1)
+    with pytest.raises(hip.TempoError) as e:
+        other.tlm(dict(dev), {k: torch.zeros_like(v) for k, v in dev.items()}, m["dt"], 2)
+    assert e.value.code == hip.E_UNSUPPORTED
+    other.close()
     with tempfile.TemporaryDirectory() as d:
         with pytest.raises(hip.TempoError) as e:
             hip.TempoHip(table_dir=d)
