@@ -31,10 +31,12 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 HBM_PEAK_GBS = 8000.0
-# VALU issue of the chip: 256 CUs x 4 SIMDs, one wave-instruction per cycle
-# at 2.4 GHz; transcendental instructions issue at a quarter of that rate
-VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9
-TRANS_PEAK_WAVE_INSTS = VALU_PEAK_WAVE_INSTS / 4
+# VALU issue of the chip: 256 CUs x 4 SIMDs of 16 lanes at 2.4 GHz, so a
+# wave64 VALU instruction (FP32 unpacked, as this library is built) holds
+# its SIMD for 4 cycles
+VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
+SIMDS = 256 * 4
+XCDS = 8
 PHASES = ("k_prepass", "k_phase_a", "k_phase_b", "k_phase_c")
 # 3-D diagnostics the default configuration computes
 DIAG3 = ("refl10cm", "re_cloud", "re_ice", "re_snow")
@@ -212,20 +214,30 @@ def ragged_graph(steps_per_graph=10, replays=3, tune=True):
 
 
 def valu_rates(kernel, precision, kernel_ms):
-    """VALU and transcendental issue of a kernel against the chip's peaks:
-    instruction counts per launch from the committed counter pass
-    (profiles/counters_<kernel>_aerosol_<precision>.json) over this run's
-    event-timed duration; None when there is no counter pass for it."""
+    """VALU use of a kernel from the committed counter pass
+    (profiles/counters_<kernel>_aerosol_<precision>.json, per-launch means):
+      issue_frac  its VALU wave-instructions over this run's event-timed
+                  duration, against the chip's issue rate (VALU_PEAK_WAVE_INSTS);
+      busy_frac   SQ_ACTIVE_INST_VALU (quad-cycles a VALU instruction holds
+                  its SIMD, transcendental ones longer) over the SIMD-cycles
+                  of the launch (GRBM_GUI_ACTIVE per XCD x SIMDs), both from
+                  the counter pass -- the share of the SIMDs' time spent
+                  issuing VALU work;
+      active_lanes  SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU): the
+                  lanes that do useful work in an issued VALU instruction.
+    None when there is no counter pass for the kernel."""
     path = os.path.join(HERE, "profiles", f"counters_{kernel}_aerosol_{precision}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         c = json.load(f)
     sec = kernel_ms * 1e-3
+    busy = c["active_inst_valu"] * 4 / (SIMDS * c["gui_active"] / XCDS)
     return {"valu_wave_insts": c["valu_wave_insts"], "trans_wave_insts": c["trans_wave_insts"],
-            "valu_frac": round(c["valu_wave_insts"] / sec / VALU_PEAK_WAVE_INSTS, 4),
-            "trans_frac": round(c["trans_wave_insts"] / sec / TRANS_PEAK_WAVE_INSTS, 4),
-            "active_lanes": c.get("active_lanes"), "source": os.path.relpath(path, HERE)}
+            "issue_frac": round(c["valu_wave_insts"] / sec / VALU_PEAK_WAVE_INSTS, 4),
+            "busy_frac": round(busy, 4),
+            "active_lanes": round(c["thread_cycles_valu"] / (64 * c["active_inst_valu"]), 4),
+            "source": os.path.relpath(path, HERE)}
 
 
 def multistep_adjoint(nsteps=12, ncol=262_144):
