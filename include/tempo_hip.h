@@ -155,12 +155,17 @@ int tempo_hip_run_noalloc(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const t
  * q = 0 every comparison of an `if` condition on a value; q = 1 the discrete
  * outcomes -- every table index, nint and bin count, and the checks'
  * presence flags (l_q*), number clamps and graupel density index.  sig
- * (device, 4*nz*ni*nj words): sig[(2p + q)*nz*ncol + k*ncol + c] for point
+ * (device, 6*nz*ni*nj words): sig[(2p + q)*nz*ncol + k*ncol + c] for point
  * (column c = i + j*ni, level k), p = 0 the physics up to sedimentation
  * (src/module_mp_tempo_main.F90:320-588), p = 1 the final update
- * (:729-863); levels k >= nlev[c] are not written.  Two runs took the same
- * decisions of a class at a point where its words of that class agree; the
- * sedimentation's substep counts and tops follow from tempo_hip_substeps. */
+ * (:729-863), p = 2 the semi-Lagrangian sedimentation (:1562-1749; its
+ * interface-speed replacements and limiters, monotone limiter and clamps
+ * as comparisons, each level's remap source cells, the surface-flux cells
+ * and the checks between substeps as outcomes; 0 in flux form, whose
+ * decisions are per column); levels k >= nlev[c] are not written.  Two
+ * runs took the same decisions of a class at a point where its words of
+ * that class agree; the sedimentation's substep counts and tops follow
+ * from tempo_hip_substeps. */
 int tempo_hip_branch_signature(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_diags *d,
                                const tempo_hip_dims *dims, double dt, int itimestep, int precision, unsigned *sig,
                                void *stream);
@@ -255,14 +260,20 @@ double tempo_hip_table_build_ms(tempo_hip_ctx *ctx);
 
 /* Per-column substep counts and sedimentation tops of the last run:
  * out[q*ncol + c], q = rain steps, rain ktop, graupel steps, graupel ktop,
- * snow steps, snow ktop, ice steps, ice ktop, cloud steps, cloud ktop. */
+ * snow steps, snow ktop, ice steps, ice ktop, cloud steps, cloud ktop;
+ * n = 10*ncol, or 12*ncol for two more rows: 32-bit hashes of the decisions
+ * of the semi-Lagrangian rain and graupel sedimentation (semi_sedi_flag;
+ * 0 in flux form), equal in two runs that took the same decisions. */
 int tempo_hip_substeps(tempo_hip_ctx *ctx, int *host_out, size_t n);
 
 /* Launch-shape tuning of subsequent runs in one precision (no counterpart in
  * the reference; set by tempo_amd/autotune.py).  Keys and values:
  *   TEMPO_HIP_TUNE_SED_WAVE  1: sedimentation with one wavefront per column
  *                               (default; used when nz <= 128), 0: one lane
- *                               per column
+ *                               per column, 2: the wavefront kernel issuing
+ *                               every load pass of its LDS staging, also
+ *                               those past the column group's end (a check
+ *                               form of the staging)
  *   TEMPO_HIP_TUNE_BLOCK_A   threads per block of the level-local physics
  *   TEMPO_HIP_TUNE_BLOCK_C   ... and of the final update: 64, 128 or 256
  *   TEMPO_HIP_TUNE_FUSED     1: the whole step as one persistent launch that

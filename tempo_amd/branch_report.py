@@ -2,12 +2,16 @@
 
 Both precisions run the same step on the same FP32-representable inputs
 through tempo_hip_branch_signature, which records per point two hashes of
-the decisions the level-local physics took -- its comparisons, and its
-discrete outcomes (table indices, nint, presence flags, clamps) -- and
+the decisions the level-local physics and the semi-Lagrangian sedimentation
+took -- their comparisons, and their discrete outcomes (table indices, nint,
+presence flags, clamps, remap source cells) -- and
 tempo_hip_substeps, which gives each column's sedimentation substep counts
-and tops.  A point *flipped* a branch when a hash of either class differs
-between the precisions, or its column's sedimentation substeps or tops
-differ.
+and tops and, for the semi-Lagrangian rain and graupel sedimentation, hashes
+of its decisions (the limited interface speeds, the monotone limiter, the
+source cells of each level's remap, the clamps, the surface-flux cells, the
+checks between substeps).  A point *flipped* a branch when a hash of either
+class differs between the precisions, or its column's sedimentation record
+differs.
 
 Errors are measured on the step's tendency (output - input)/dt of each
 prognostic field, as relative L2 norms against the FP64 twin's:
@@ -22,8 +26,7 @@ prognostic field, as relative L2 norms against the FP64 twin's:
 Surface precipitation is compared per column over all columns.  Columns
 whose precipitation moved by more than 1e-4 of its value (plus 1e-7 of the
 largest) are counted with whether a branch flipped at their two lowest
-levels or in their sedimentation's substeps and tops; the decisions inside
-the semi-Lagrangian remap are not recorded.
+levels or in their sedimentation record.
 """
 import numpy as np
 
@@ -41,17 +44,17 @@ def _rel(a, b):
 def signature_run(ctx, state, dt, itimestep, dtype, nlev=None):
     """one step of ``state`` (numpy float64 (nz, ncol), values taken as
     given) in ``dtype`` on the current device: (outputs, diagnostics,
-    signatures (2, 2, nz, ncol) int32, substeps (10, ncol)) as numpy"""
+    signatures (3, 2, nz, ncol) int32, sedimentation record (12, ncol)) as numpy"""
     import torch
     nz, ncol = state["p"].shape
     dev = {k: torch.tensor(v, dtype=dtype, device="cuda") for k, v in state.items()}
-    sig = torch.zeros((2, 2, nz, ncol), dtype=torch.int32, device="cuda")
+    sig = torch.zeros((3, 2, nz, ncol), dtype=torch.int32, device="cuda")
     nl = None if nlev is None else torch.as_tensor(nlev, dtype=torch.int32, device="cuda")
     d = ctx.run(dev, dt, itimestep, nlev=nl, signature=sig)
     torch.cuda.synchronize()
     out = {k: v.double().cpu().numpy() for k, v in dev.items()}
     diags = {k: v.double().cpu().numpy() for k, v in d.items()}
-    return out, diags, sig.cpu().numpy(), ctx.substeps(ncol)
+    return out, diags, sig.cpu().numpy(), ctx.substeps(ncol, decisions=True)
 
 
 def _f32(a):
@@ -70,8 +73,8 @@ def compare(ctx, state, dt, itimestep, nlev=None):
     valid = np.ones((nz, ncol), bool)
     if nlev is not None:
         valid = np.arange(nz)[:, None] < np.asarray(nlev)[None, :]
-    flip_cmp = ((s32[0, 0] != s64[0, 0]) | (s32[1, 0] != s64[1, 0])) & valid
-    flip_idx = ((s32[0, 1] != s64[0, 1]) | (s32[1, 1] != s64[1, 1])) & valid
+    flip_cmp = (s32[:, 0] != s64[:, 0]).any(axis=0) & valid
+    flip_idx = (s32[:, 1] != s64[:, 1]).any(axis=0) & valid
     sed_flip = (k32 != k64).any(axis=0)
     flip = flip_cmp | flip_idx | (sed_flip[None, :] & valid)
     keep = valid & ~flip
@@ -102,10 +105,10 @@ def compare(ctx, state, dt, itimestep, nlev=None):
                 flipped_points_discrete=float(flip_idx.sum() / nv),     # table indices, presence, clamps
                 flipped_points_comparisons=float(flip_cmp.sum() / nv),  # `if` comparisons
                 flipped_sedimentation=float(sed_flip.mean()),           # columns
-                points=int(nv), columns=int(ncol),
+                points=int(nv), columns=int(ncol), keep=keep,
                 outputs=(o32, d32, o64, d64), signatures=(s32, s64), substeps=(k32, k64))
 
 
 def summary(rep):
     """the report without its arrays (JSON-ready)"""
-    return {k: v for k, v in rep.items() if k not in ("outputs", "signatures", "substeps")}
+    return {k: v for k, v in rep.items() if k not in ("outputs", "signatures", "substeps", "keep")}

@@ -151,6 +151,9 @@ template <class D, class FXR, class FVT, class FODZ, class FRHO>
 __device__ void sed_adjoint(const StepArgs<D>& a, const SedAdjCol& q, const AdjArgs& g, FXR xr0, FVT vt, FODZ odz,
                             FRHO orho, const double* tenbar, double limit, int nsub, int steps, int ktop,
                             double pbar, double* vbar, double* rhobar) {
+  // every operation rounded on its own, as in the forward sedimentation it
+  // recomputes (so that the recomputed masses and clamps are the forward's)
+#pragma clang fp contract(off)
   double* xr = g.work + AW_XR * q.plane;
   double* xb = g.work + AW_XBAR * q.plane;
   double* fb = g.work + AW_FBAR * q.plane;
@@ -210,6 +213,9 @@ __device__ void sed_adjoint(const StepArgs<D>& a, const SedAdjCol& q, const AdjA
 
 template <class D>
 __global__ void __launch_bounds__(64) k_adj_sedimentation(StepArgs<D> a, AdjArgs g) {
+  // every operation rounded on its own, as in the forward sedimentation it
+  // recomputes (so that the recomputed masses and clamps are the forward's)
+#pragma clang fp contract(off)
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.ncol) return;
   const int nzc = a.nlev ? min(max(a.nlev[c], 1), a.nz) : a.nz;

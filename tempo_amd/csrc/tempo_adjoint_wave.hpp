@@ -99,6 +99,9 @@ __device__ __forceinline__ void wave_sed_adjoint(const WaveCol& q, const double 
                                                  double limit, int nsub, int steps, int ktop, double pbar,
                                                  double (&xb)[kSedChunks], double (&vb)[kSedChunks],
                                                  double (&rhob)[kSedChunks]) {
+  // every operation rounded on its own, as in the forward sedimentation it
+  // recomputes (so that the recomputed masses and clamps are the forward's)
+#pragma clang fp contract(off)
   const double ost = 1.0 / double(steps), dt = q.dt;
   auto updated = [&](int k) { return k < q.nzc && (k == q.top || k <= ktop - 1); };
   for (int n = nsub - 1; n >= 0; --n) {
@@ -156,6 +159,9 @@ __device__ __forceinline__ void wave_sed_adjoint(const WaveCol& q, const double 
 
 template <class D>
 __global__ void __launch_bounds__(64 * kScrW) k_adj_sed_wave(StepArgs<D> a, AdjArgs g) {
+  // every operation rounded on its own, as in the forward sedimentation it
+  // recomputes (so that the recomputed masses and clamps are the forward's)
+#pragma clang fp contract(off)
   __shared__ double rows[kScrW][kSedLevels];
   const int w = int(threadIdx.x >> 6);
   const int c = int(blockIdx.x) * kScrW + w;

@@ -378,7 +378,7 @@ struct ScratchSizes {
     cf = ncol * sizeof(unsigned);
     kmelt = ncol * sizeof(int);
     precip = 5 * ncol * bw;
-    ksub = 10 * ncol * sizeof(int);
+    ksub = kSubRows * ncol * sizeof(int);
     semiw = semi ? kSemiArrays * (nz + 2) * ncol * bw : 0;
   }
   // the buffers carved in this order from one block, each 256-byte aligned
@@ -523,7 +523,7 @@ void diag_ptrs(const tempo_hip_diags* d, void* (&d2)[D2_N], void* (&d3)[D3_N]) {
 template <class R>
 bool fused_applies(const tempo_hip_ctx* x, const StepArgs<R>& a) {
   const LaunchCfg& lc = x->launch[sizeof(S_t<R>) == 8 ? 1 : 0];
-  return lc.fused && lc.sed_wave && a.nz <= kSedLevels && !x->cfg.semi_sedi_flag &&
+  return lc.fused && lc.sed_wave == 1 && a.nz <= kSedLevels && !x->cfg.semi_sedi_flag &&
          !(x->cfg.refl10cm_from_melting_flag && x->cfg.refl10cm_flag);
 }
 
@@ -540,7 +540,7 @@ int ensure_fused(tempo_hip_ctx* x, StepArgs<R>& a, int grid, ScratchPtrs& sc) {
   HIP_TRY(x->cf_.ensure(ncol * sizeof(unsigned)));
   HIP_TRY(x->kmelt.ensure(ncol * sizeof(int)));
   HIP_TRY(x->precip.ensure(5 * ncol * sizeof(W_t<R>)));
-  HIP_TRY(x->ksub.ensure(10 * ncol * sizeof(int)));
+  HIP_TRY(x->ksub.ensure(kSubRows * ncol * sizeof(int)));
   char* base = static_cast<char*>(x->slots.p);
   sc = ScratchPtrs{base, base + ScratchSizes::up(bsr), base + ScratchSizes::up(bsr) + ScratchSizes::up(bsd), nullptr,
                    x->cf_.p, x->kmelt.p, x->precip.p, x->ksub.p, nullptr};
@@ -1183,7 +1183,7 @@ int tempo_hip_tune(tempo_hip_ctx* x, int precision, int key, int value) {
   LaunchCfg& lc = x->launch[precision == TEMPO_HIP_FP64 ? 1 : 0];
   switch (key) {
     case TEMPO_HIP_TUNE_SED_WAVE:
-      if (value != 0 && value != 1) return fail(TEMPO_HIP_E_ARG, "sed_wave must be 0 or 1");
+      if (value < 0 || value > 2) return fail(TEMPO_HIP_E_ARG, "sed_wave must be 0, 1 or 2");
       lc.sed_wave = value;
       return TEMPO_HIP_OK;
     case TEMPO_HIP_TUNE_BLOCK_A:
@@ -1230,7 +1230,8 @@ int tempo_hip_profile_read(tempo_hip_ctx* x, double* ms, int n, int* runs) {
 
 int tempo_hip_substeps(tempo_hip_ctx* x, int* out, size_t n) {
   if (!x || !out) return fail(TEMPO_HIP_E_ARG, "NULL argument");
-  if (n != size_t(10) * x->last_ncol) return fail(TEMPO_HIP_E_ARG, "need 10*ncol entries");
+  if (n != size_t(10) * x->last_ncol && n != size_t(kSubRows) * x->last_ncol)
+    return fail(TEMPO_HIP_E_ARG, "need 10*ncol or 12*ncol entries");
   HIP_TRY(hipMemcpy(out, x->last_ksub, n * sizeof(int), hipMemcpyDeviceToHost));
   return TEMPO_HIP_OK;
 }

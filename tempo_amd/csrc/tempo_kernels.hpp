@@ -493,6 +493,10 @@ template <class R>
 __device__ void sediment(const StepArgs<R>& a, int c, int nzc, W_t<R>* __restrict__ xr,
                          const W_t<R>* __restrict__ vt, W_t<R>* __restrict__ xten, double limit, int steps, int ktop,
                          W_t<R>* precip) {
+  // every operation rounded on its own, in every sedimentation kernel, so
+  // that the kernels agree bit for bit also where the working precision's
+  // build lets the compiler fuse multiply-adds
+#pragma clang fp contract(off)
   using W = W_t<R>;
   const size_t plane = a.plane;
   const R* __restrict__ rho = a.sr + SR_RHO * plane;
@@ -524,6 +528,7 @@ template <class R>
 __device__ void fill_fallspeed(const StepArgs<R>& a, int c, int nzc, const W_t<R>* __restrict__ mass,
                                const R* __restrict__ raw1, const R* __restrict__ raw2, W_t<R>* __restrict__ out1,
                                W_t<R>* __restrict__ out2, bool use_max, int& substeps, int& ktop) {
+#pragma clang fp contract(off)
   using W = W_t<R>;
   const double dt = a.dt;
   W v1 = W(0.0), v2 = W(0.0);
@@ -550,6 +555,24 @@ __device__ void fill_fallspeed(const StepArgs<R>& a, int c, int nzc, const W_t<R
   if (ktop == nzc) ktop = nzc - 1;
 }
 
+// Decisions of the semi-Lagrangian sedimentation of one column: a 32-bit
+// hash per column of its structural decisions (h), and in a signature run
+// per level, in the sedimentation's planes of the signature array (w0:
+// comparisons, w1: discrete outcomes, point (c, k) at k*ncol + c).
+struct SemiRec {
+  unsigned h = 0;
+  unsigned *w0 = nullptr, *w1 = nullptr;
+  size_t ncol = 0;
+  int c = 0;
+  __device__ void cmp(int k, unsigned v) {
+    if (w0) w0[size_t(k) * ncol + size_t(c)] = decision(w0[size_t(k) * ncol + size_t(c)], v);
+  }
+  __device__ void outcome(int k, unsigned v, bool column = true) {
+    if (column) h = decision(h, v);
+    if (w1) w1[size_t(k) * ncol + size_t(c)] = decision(w1[size_t(k) * ncol + size_t(c)], v);
+  }
+};
+
 // semi-Lagrangian sedimentation of one field over one substep
 // (semilagrangian_sedimentation, :1562-1749, after Juang and Hong 2010) for
 // the column c of nzc levels: interface speeds by third-order interpolation
@@ -561,7 +584,11 @@ __device__ void fill_fallspeed(const StepArgs<R>& a, int c, int nzc, const W_t<R
 constexpr int kSemiArrays = 8;
 template <class R>
 __device__ void semi_sediment(const StepArgs<R>& a, int c, int nzc, W_t<R>* xr, W_t<R>* xten, const W_t<R>* vt,
-                              int steps, double limit, W_t<R>* precip) {
+                              int steps, double limit, W_t<R>* precip, SemiRec& rec) {
+  // rec: the decisions, per column (structural: each level's source cells
+  // kb and kt, where the remap ends early, which cells feed the surface
+  // flux) and, in a signature run, per level (also the interface-speed
+  // replacements and limiters, the monotone limiter and the clamps)
   // The remap and the surface flux branch on differences of nearly equal
   // positions (departure point against interface, za against 0), where the
   // reference's algorithm is discontinuous: its rounding is kept exactly,
@@ -587,12 +614,18 @@ __device__ void semi_sediment(const StepArgs<R>& a, int c, int nzc, W_t<R>* xr, 
   arr(WI, nz - 1) = W(0.5) * (V(nz - 1) + V(nz - 2));
   arr(WI, nz) = V(nz);
   for (int k = 1; k < nz; ++k)
-    if (V(k) == W(0.0)) arr(WI, k) = V(k - 1);
+    if (V(k) == W(0.0)) {
+      arr(WI, k) = V(k - 1);
+      rec.cmp(k, 1u);
+    }
   // diffusivity limit, top-down (:1614-1621)
   const W con1 = W(0.05);
   for (int k = nz - 1; k >= 0; --k) {
     const W decfl = (arr(WI, k + 1) - arr(WI, k)) * dt * ost / DZ(k);
-    if (decfl > con1) arr(WI, k) = arr(WI, k + 1) - con1 * DZ(k) * odt * W(double(steps));
+    if (decfl > con1) {
+      arr(WI, k) = arr(WI, k + 1) - con1 * DZ(k) * odt * W(double(steps));
+      rec.cmp(k, 2u);
+    }
   }
   // departure points and cell sizes (:1623-1635)
   for (int k = 0; k < nz + 1; ++k) arr(ZA, k) = arr(ZI, k) - arr(WI, k) * dt * ost;
@@ -607,12 +640,14 @@ __device__ void semi_sediment(const StepArgs<R>& a, int c, int nzc, W_t<R>* xr, 
     if (dip * dim <= W(0.0)) {
       arr(QMI, k) = arr(QA, k);
       arr(QPI, k) = arr(QA, k);
+      rec.cmp(k, 3u);
     } else {
       arr(QPI, k) = arr(QA, k) + W(0.5) * (dip + dim) * arr(DZA, k);
       arr(QMI, k) = W(2.0) * arr(QA, k) - arr(QPI, k);
       if (arr(QPI, k) > W(0.0) || arr(QMI, k) < W(0.0)) {
         arr(QPI, k) = arr(QA, k);
         arr(QMI, k) = arr(QA, k);
+        rec.cmp(k, 4u);
       }
     }
   }
@@ -626,7 +661,10 @@ __device__ void semi_sediment(const StepArgs<R>& a, int c, int nzc, W_t<R>* xr, 
   for (int k = 1; k <= nz; ++k) {
     kb = max(kb - 1, 1);
     kt = max(kt - 1, 1);
-    if (arr(ZI, k - 1) >= arr(ZA, nz)) break;
+    if (arr(ZI, k - 1) >= arr(ZA, nz)) {
+      rec.outcome(k - 1, 0x100u + unsigned(k));
+      break;
+    }
     for (int kk = kb; kk <= nz; ++kk)
       if (arr(ZI, k - 1) <= arr(ZA, kk)) {
         kb = kk;
@@ -638,6 +676,7 @@ __device__ void semi_sediment(const StepArgs<R>& a, int c, int nzc, W_t<R>* xr, 
         break;
       }
     kt = kt - 1;
+    rec.outcome(k - 1, (unsigned(kb) << 16) ^ unsigned(kt));
     W& x = X(k - 1);
     if (kt == kb) {
       const W tl = (arr(ZI, k - 1) - arr(ZA, kb - 1)) / arr(DZA, kb - 1);
@@ -669,6 +708,7 @@ __device__ void semi_sediment(const StepArgs<R>& a, int c, int nzc, W_t<R>* xr, 
       x = qsum / zsum;
     }
     const W orho = W(1.0) / W(rho[TEMPO_IDX_SCR(a, c, k - 1)]);
+    if (!(x > W(limit))) rec.cmp(k - 1, 5u);
     x = x > W(limit) ? x : W(limit);
     W& ten = xten[TEMPO_IDX_SCR(a, c, k - 1)];
     ten = ten + (x - arr(SAVE, k - 1)) * orho * odt;
@@ -678,7 +718,9 @@ __device__ void semi_sediment(const StepArgs<R>& a, int c, int nzc, W_t<R>* xr, 
   for (int k = 0; k < nz; ++k) {
     if (arr(ZA, k) < W(0.0) && arr(ZA, k + 1) <= W(0.0)) {
       *precip = *precip + arr(QA, k) * arr(DZA, k);
+      rec.outcome(k, 0x200u + unsigned(k));
     } else if (arr(ZA, k) < W(0.0) && arr(ZA, k + 1) > W(0.0)) {
+      rec.outcome(k, 0x300u + unsigned(k));
       const W th = (W(0.0) - arr(ZA, k)) / arr(DZA, k);
       const W th2 = th * th;
       const W qqd = W(0.5) * (arr(QPI, k) - arr(QMI, k));
@@ -695,7 +737,8 @@ __device__ void semi_sediment(const StepArgs<R>& a, int c, int nzc, W_t<R>* xr, 
 // (top-down fill from 0 above the column).  A tendency keeps its FP64
 // accumulator unless the check rewrites it.
 template <class R>
-__device__ void semi_rain_refresh(const StepArgs<R>& a, int c, int nzc, W_t<R>* sd, W_t<R>* vt1, W_t<R>* vt2) {
+__device__ void semi_rain_refresh(const StepArgs<R>& a, int c, int nzc, W_t<R>* sd, W_t<R>* vt1, W_t<R>* vt2,
+                                  SemiRec& rec) {
   using W = W_t<R>;
   const Const<S_t<R>>& C = *a.cst;
   const size_t plane = a.plane;
@@ -713,6 +756,7 @@ __device__ void semi_rain_refresh(const StepArgs<R>& a, int c, int nzc, W_t<R>* 
     const R q0 = L.qrten, n0 = L.nrten;
     R x = a.fa[F_QR][o], xn = a.fa[F_NR][o];
     rain_check(C, L, x, xn, dt, odt);
+    rec.outcome(k, L.l_qr ? 1u : 0u, false);
     if (L.qrten != q0) qrten = W(L.qrten);
     if (L.nrten != n0) nrten = W(L.nrten);
     sd[SD_RR * plane + s] = W(L.rr);
@@ -728,7 +772,8 @@ __device__ void semi_rain_refresh(const StepArgs<R>& a, int c, int nzc, W_t<R>* 
 }
 
 template <class R>
-__device__ void semi_graupel_refresh(const StepArgs<R>& a, int c, int nzc, W_t<R>* sd, W_t<R>* vt1, W_t<R>* vt2) {
+__device__ void semi_graupel_refresh(const StepArgs<R>& a, int c, int nzc, W_t<R>* sd, W_t<R>* vt1, W_t<R>* vt2,
+                                     SemiRec& rec) {
   using W = W_t<R>;
   const Const<S_t<R>>& C = *a.cst;
   const size_t plane = a.plane;
@@ -750,6 +795,7 @@ __device__ void semi_graupel_refresh(const StepArgs<R>& a, int c, int nzc, W_t<R
     R x = a.fa[F_QG][o];
     R xg = has_ngqb ? a.fa[F_NG][o] : R(0.0), xb = has_ngqb ? a.fa[F_QB][o] : R(0.0);
     graupel_check(C, L, x, xg, xb, has_ngqb, dt, odt);
+    rec.outcome(k, (L.l_qg ? 1u : 0u) | (unsigned(L.idx_bg) << 1), false);
     if (L.qgten != g0) qgten = W(L.qgten);
     if (L.ngten != n0) ngten = W(L.ngten);
     if (L.qbten != b0) qbten = W(L.qbten);
@@ -786,7 +832,19 @@ __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, W_t<R>* vtbuf) {
   const size_t plane = a.plane;
   W pr[5] = {W(0.0), W(0.0), W(0.0), W(0.0), W(0.0)};  // rain graupel snow ice cloud
   const unsigned cf = a.cf[c];
-  int kst[10] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+  int kst[kSubRows] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 0, 0};
+  // decisions of the semi-Lagrangian rain and graupel sedimentation (per
+  // level in the signature array's sedimentation planes, cleared here, in a
+  // signature run)
+  SemiRec hr, hg;
+  if (a.sig) {
+    const size_t n3 = size_t(a.ncol) * size_t(a.nz);
+    hr.w0 = hg.w0 = a.sig + 4 * n3;
+    hr.w1 = hg.w1 = a.sig + 5 * n3;
+    hr.ncol = hg.ncol = size_t(a.ncol);
+    hr.c = hg.c = c;
+    for (int k = 0; k < nzc; ++k) hr.w0[size_t(k) * a.ncol + c] = hr.w1[size_t(k) * a.ncol + c] = 0u;
+  }
   if (!(cf & CF_EXIT)) {
     unsigned any = 0;
     for (int k = 0; k < nzc; ++k) any |= a.pf[TEMPO_IDX_SCR(a, c, k)];
@@ -811,9 +869,9 @@ __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, W_t<R>* vtbuf) {
       if (a.fl.semi_sedi) {  // (:599-615): far fewer substeps, fall speeds refreshed after each
         steps = max(int(R(steps) / R(10.0)) + 1, 1);
         for (int n = 0; n < steps; ++n) {
-          semi_sediment(a, c, nzc, sd + SD_RR * plane, sd + SD_QRTEN * plane, vr, steps, kR1, &pr[0]);
-          semi_sediment(a, c, nzc, sd + SD_NR * plane, sd + SD_NRTEN * plane, v2, steps, kR2, (W*)nullptr);
-          semi_rain_refresh(a, c, nzc, sd, vr, v2);
+          semi_sediment(a, c, nzc, sd + SD_RR * plane, sd + SD_QRTEN * plane, vr, steps, kR1, &pr[0], hr);
+          semi_sediment(a, c, nzc, sd + SD_NR * plane, sd + SD_NRTEN * plane, v2, steps, kR2, (W*)nullptr, hr);
+          semi_rain_refresh(a, c, nzc, sd, vr, v2, hr);
         }
       } else {
         for (int n = 0; n < steps; ++n) {
@@ -832,10 +890,10 @@ __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, W_t<R>* vtbuf) {
       if (a.fl.semi_sedi) {  // (:641-667)
         steps = max(int(R(steps) / R(10.0)) + 1, 1);
         for (int n = 0; n < steps; ++n) {
-          semi_sediment(a, c, nzc, sd + SD_RG * plane, sd + SD_QGTEN * plane, v1, steps, kR1, &pr[1]);
-          semi_sediment(a, c, nzc, sd + SD_NG * plane, sd + SD_NGTEN * plane, v2, steps, kR2, (W*)nullptr);
-          semi_sediment(a, c, nzc, sd + SD_RB * plane, sd + SD_QBTEN * plane, v1, steps, rb_lim, (W*)nullptr);
-          semi_graupel_refresh(a, c, nzc, sd, v1, v2);
+          semi_sediment(a, c, nzc, sd + SD_RG * plane, sd + SD_QGTEN * plane, v1, steps, kR1, &pr[1], hg);
+          semi_sediment(a, c, nzc, sd + SD_NG * plane, sd + SD_NGTEN * plane, v2, steps, kR2, (W*)nullptr, hg);
+          semi_sediment(a, c, nzc, sd + SD_RB * plane, sd + SD_QBTEN * plane, v1, steps, rb_lim, (W*)nullptr, hg);
+          semi_graupel_refresh(a, c, nzc, sd, v1, v2, hg);
         }
       } else {
         for (int n = 0; n < steps; ++n) {
@@ -850,6 +908,7 @@ __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, W_t<R>* vtbuf) {
     if (any & PF_QS) {
       // snow_fallspeed mixes in rain where snow melts (:1886-1891), using the
       // rain mass after rain sedimentation and the filled rain fall speed
+#pragma clang fp contract(off)
       const double dt = a.dt;
       W vprev = W(0.0);
       steps = 1;
@@ -915,8 +974,10 @@ __global__ void __launch_bounds__(128) k_phase_b(StepArgs<R> a, W_t<R>* vtbuf) {
         }
   }
   for (int q = 0; q < 5; ++q) a.precip[size_t(q) * a.ncol + c] = pr[q];
+  kst[10] = int(hr.h);
+  kst[11] = int(hg.h);
   if (a.ksub)
-    for (int q = 0; q < 10; ++q) a.ksub[size_t(q) * a.ncol + c] = kst[q];
+    for (int q = 0; q < kSubRows; ++q) a.ksub[size_t(q) * a.ncol + c] = kst[q];
   // column diagnostics (src/module_mp_tempo_main.F90:784-794, driver :452-457)
   const size_t d2 = size_t(c % a.ni) + size_t(c / a.ni) * a.d2ldj;
   const bool live = !(cf & CF_EXIT);

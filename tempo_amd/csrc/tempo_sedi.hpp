@@ -71,6 +71,10 @@ __device__ __forceinline__ T above(const T (&x)[kSedChunks], int j) {
 // Block-wide staging of (level, column) planes through LDS slots.  Loads
 // and stores issue without barriers so that all planes of a stage are in
 // flight together; the caller separates the phases with __syncthreads().
+// kEveryPass: issue every load pass, also those past the group's end (all
+// lanes then load the group's last element and write nothing) -- a check
+// form of the staging, selected with sed_wave = 2; results are identical.
+template <bool kEveryPass = false>
 struct SedTile {
   double* lds;  // kSedSlots x kSedCols x row
   int c0, ncol, nz;
@@ -96,7 +100,7 @@ struct SedTile {
     const size_t base = sbase;
 #pragma unroll
     for (int it = 0; it < kSedChunks; ++it) {
-      if (it * 64 * kSedCols >= n) break;
+      if (!kEveryPass && it * 64 * kSedCols >= n) break;
       const int e = min(int(threadIdx.x) + it * 64 * kSedCols, n - 1);
 #pragma unroll
       for (int p = 0; p < N; ++p) v[it][p] = plane[p][base + e];
@@ -121,7 +125,7 @@ struct SedTile {
     const size_t base = sbase;
 #pragma unroll
     for (int it = 0; it < kSedChunks; ++it) {
-      if (it * 64 * kSedCols >= n) break;
+      if (!kEveryPass && it * 64 * kSedCols >= n) break;
       const int e = min(int(threadIdx.x) + it * 64 * kSedCols, n - 1);
       const int cc = e % kSedCols, k = e / kSedCols, c = min(c0 + cc, ncol - 1);
       vf[it] = pf[base + e];
@@ -185,6 +189,7 @@ __device__ __forceinline__ void wave_fill(const double (&mass)[kSedChunks], cons
                                           const R (&raw2)[kSedChunks], bool two, bool use_max,
                                           const R (&dz)[kSedChunks], int nzc, double dt, double (&out1)[kSedChunks],
                                           double (&out2)[kSedChunks], int& substeps, int& ktop) {
+#pragma clang fp contract(off)  // as the lane-per-column kernel: every operation rounded on its own
   unsigned long long m[kSedChunks];
 #pragma unroll
   for (int j = 0; j < kSedChunks; ++j) {
@@ -230,6 +235,7 @@ __device__ __forceinline__ void wave_fill(const double (&mass)[kSedChunks], cons
 __device__ __forceinline__ double wave_sediment(const SedCol& g, double (&xr)[kSedChunks],
                                                 const double (&vt)[kSedChunks], double (&xten)[kSedChunks],
                                                 double limit, int steps, int ktop, double dt, bool precip) {
+#pragma clang fp contract(off)  // as the lane-per-column kernel: every operation rounded on its own
   const double ost = 1.0 / double(steps);
   double sed[kSedChunks];
 #pragma unroll
@@ -254,20 +260,28 @@ __device__ __forceinline__ double wave_sediment(const SedCol& g, double (&xr)[kS
 // Sedimentation of the column group c0 .. c0 + kSedCols - 1, one wavefront
 // per column (64 * kSedCols threads); scr holds the group's scratch,
 // tile_mem sed_lds_bytes(nz) of LDS.
-template <class R>
+template <class R, bool kEveryPass = false>
 __device__ __forceinline__ void phase_b_group(const StepArgs<R>& a, const Scr<R>& scr, int c0, double* tile_mem) {
   __shared__ unsigned blk_any;
   const int w = int(threadIdx.x >> 6);
   const int c = c0 + w;
   const bool col_ok = c < a.ncol;
   const size_t plane = scr.plane;
-  const SedTile T{tile_mem, c0, a.ncol, a.nz, scr.nzs, sed_row(a.nz), scr.at(c0, 0)};
+  const SedTile<kEveryPass> T{tile_mem, c0, a.ncol, a.nz, scr.nzs, sed_row(a.nz), scr.at(c0, 0)};
   const double dt = a.dt;
   Sd_t<R>* sd = scr.sd;
   const R* sr = scr.sr;
 
   SedCol g;
   g.nzc = col_ok ? (a.nlev ? min(max(a.nlev[c], 1), a.nz) : a.nz) : 0;
+  if (a.sig && col_ok) {  // signature run: this kernel's decisions are per column (tempo_hip_substeps)
+    const size_t n3 = size_t(a.ncol) * size_t(a.nz);
+#pragma unroll
+    for (int j = 0; j < kSedChunks; ++j) {
+      const int k = j * 64 + wv::lane();
+      if (k < g.nzc) a.sig[4 * n3 + size_t(k) * a.ncol + c] = a.sig[5 * n3 + size_t(k) * a.ncol + c] = 0u;
+    }
+  }
   const unsigned cf = col_ok ? a.cf[c] : CF_EXIT;
   const bool live = col_ok && !(cf & CF_EXIT);
 
@@ -302,7 +316,7 @@ __device__ __forceinline__ void phase_b_group(const StepArgs<R>& a, const Scr<R>
   if (bany & PF_QR) {
     double xn[kSedChunks], tr[kSedChunks], tn[kSedChunks], v2[kSedChunks];
     R raw1[kSedChunks], raw2[kSedChunks];
-    T.load_many<R, 5>({0, 1, 2, 4, 5}, {sd + SD_NR * plane, sd + SD_QRTEN * plane, sd + SD_NRTEN * plane,
+    T.template load_many<R, 5>({0, 1, 2, 4, 5}, {sd + SD_NR * plane, sd + SD_QRTEN * plane, sd + SD_NRTEN * plane,
                                         sr + SR_VTRR * plane, sr + SR_VTNR * plane});
     __syncthreads();
     T.get(0, w, xn);
@@ -333,7 +347,7 @@ __device__ __forceinline__ void phase_b_group(const StepArgs<R>& a, const Scr<R>
     double xg[kSedChunks], xn[kSedChunks], xb[kSedChunks], tg[kSedChunks], tn[kSedChunks], tb[kSedChunks];
     double v1[kSedChunks], v2[kSedChunks];
     R raw1[kSedChunks], raw2[kSedChunks];
-    T.load_many<R, 8>({0, 1, 2, 4, 5, 6, 7, 3},
+    T.template load_many<R, 8>({0, 1, 2, 4, 5, 6, 7, 3},
                       {sd + SD_RG * plane, sd + SD_NG * plane, sd + SD_RB * plane, sd + SD_QGTEN * plane,
                        sd + SD_NGTEN * plane, sd + SD_QBTEN * plane, sr + SR_VTRG * plane, sr + SR_VTNG * plane});
     __syncthreads();
@@ -372,12 +386,13 @@ __device__ __forceinline__ void phase_b_group(const StepArgs<R>& a, const Scr<R>
   if (bany & PF_QS) {
     double xs[kSedChunks], ts[kSedChunks], v1[kSedChunks];
     R raw[kSedChunks];
-    T.load_many<R, 3>({0, 1, 2}, {sd + SD_RS * plane, sd + SD_QSTEN * plane, sr + SR_VTS * plane});
+    T.template load_many<R, 3>({0, 1, 2}, {sd + SD_RS * plane, sd + SD_QSTEN * plane, sr + SR_VTS * plane});
     __syncthreads();
     T.get(0, w, xs);
     T.get(1, w, ts);
     T.get(2, w, raw);
     if (any & PF_QS) {
+#pragma clang fp contract(off)  // the snow-rain speed mix, as the lane-per-column kernel
       R vraw[kSedChunks];
 #pragma unroll
       for (int j = 0; j < kSedChunks; ++j) {
@@ -405,7 +420,7 @@ __device__ __forceinline__ void phase_b_group(const StepArgs<R>& a, const Scr<R>
   if (bany & PF_QI) {
     double xi[kSedChunks], xn[kSedChunks], ti[kSedChunks], tn[kSedChunks], v1[kSedChunks], v2[kSedChunks];
     R raw1[kSedChunks], raw2[kSedChunks];
-    T.load_many<R, 6>({0, 1, 2, 4, 5, 6}, {sd + SD_RI * plane, sd + SD_NI * plane, sd + SD_QITEN * plane,
+    T.template load_many<R, 6>({0, 1, 2, 4, 5, 6}, {sd + SD_RI * plane, sd + SD_NI * plane, sd + SD_QITEN * plane,
                                            sd + SD_NITEN * plane, sr + SR_VTRI * plane, sr + SR_VTNI * plane});
     __syncthreads();
     T.get(0, w, xi);
@@ -434,7 +449,7 @@ __device__ __forceinline__ void phase_b_group(const StepArgs<R>& a, const Scr<R>
   if (bany & PF_QC) {
     double xc[kSedChunks], xn[kSedChunks], tc[kSedChunks], tn[kSedChunks], v1[kSedChunks], v2[kSedChunks];
     R raw1[kSedChunks], raw2[kSedChunks];
-    T.load_many<R, 6>({0, 1, 2, 4, 5, 6}, {sd + SD_RC * plane, sd + SD_NC * plane, sd + SD_QCTEN * plane,
+    T.template load_many<R, 6>({0, 1, 2, 4, 5, 6}, {sd + SD_RC * plane, sd + SD_NC * plane, sd + SD_QCTEN * plane,
                                            sd + SD_NCTEN * plane, sr + SR_VTRC * plane, sr + SR_VTNC * plane});
     __syncthreads();
     T.get(0, w, xc);
@@ -472,7 +487,7 @@ __device__ __forceinline__ void phase_b_group(const StepArgs<R>& a, const Scr<R>
   if (!col_ok || wv::lane() != 0) return;
   for (int q = 0; q < 5; ++q) a.precip[size_t(q) * a.ncol + c] = pr[q];
   if (a.ksub)
-    for (int q = 0; q < 10; ++q) a.ksub[size_t(q) * a.ncol + c] = kst[q];
+    for (int q = 0; q < kSubRows; ++q) a.ksub[size_t(q) * a.ncol + c] = q < 10 ? kst[q] : 0;
   // column diagnostics (src/module_mp_tempo_main.F90:784-794, driver :452-457)
   const size_t d2 = size_t(c % a.ni) + size_t(c / a.ni) * a.d2ldj;
   const R rain = R(pr[0]), graupel = R(pr[1]), snow = R(pr[2]), ice = R(pr[3]), cloud = R(pr[4]);
@@ -491,10 +506,10 @@ __device__ __forceinline__ void phase_b_group(const StepArgs<R>& a, const Scr<R>
   if (a.d2[D2_HAIL_COL]) a.d2[D2_HAIL_COL][d2] = R(0);
 }
 
-template <class R>
+template <class R, bool kEveryPass = false>
 __global__ void __launch_bounds__(64 * kSedCols) __attribute__((amdgpu_waves_per_eu(6))) k_phase_b_wave(StepArgs<R> a) {
   extern __shared__ double tile_mem[];  // sed_lds_bytes(nz)
-  phase_b_group(a, batch_scr(a), int(blockIdx.x) * kSedCols, tile_mem);
+  phase_b_group<R, kEveryPass>(a, batch_scr(a), int(blockIdx.x) * kSedCols, tile_mem);
 }
 
 // ------------------------------------------------------------- launch ----
@@ -511,7 +526,10 @@ hipError_t launch_step_impl(const StepArgs<T>& a, const StepArgs<RA>& ga, const 
   if (ev) (void)hipEventRecord(ev[1], s);
   hipLaunchKernelGGL(k_phase_a<RA>, level_grid(a, lc.block_a), dim3(unsigned(lc.block_a)), 0, s, ga);
   if (ev) (void)hipEventRecord(ev[2], s);
-  if (lc.sed_wave && a.nz <= kSedLevels && !a.fl.semi_sedi)
+  if (lc.sed_wave == 2 && a.nz <= kSedLevels && !a.fl.semi_sedi)
+    hipLaunchKernelGGL((k_phase_b_wave<T, true>), dim3(unsigned((a.ncol + kSedCols - 1) / kSedCols)),
+                       dim3(64 * kSedCols), sed_lds_bytes(a.nz), s, a);
+  else if (lc.sed_wave && a.nz <= kSedLevels && !a.fl.semi_sedi)
     hipLaunchKernelGGL(k_phase_b_wave<T>, dim3(unsigned((a.ncol + kSedCols - 1) / kSedCols)), dim3(64 * kSedCols),
                        sed_lds_bytes(a.nz), s, a);
   else

@@ -222,6 +222,13 @@ struct Tables {
   const R* tpi_ide;
 };
 
+// rows of the per-column sedimentation record (tempo_hip_substeps): substep
+// count and top of rain, graupel, snow, ice and cloud, then hashes of the
+// semi-Lagrangian rain and graupel sedimentation's decisions (0 in flux form)
+constexpr int kSubRows = 12;
+// one more decision folded into a 32-bit hash (FNV-1a step)
+__host__ __device__ constexpr unsigned decision(unsigned h, unsigned v) { return (h ^ v) * 16777619u; }
+
 template <class R>
 struct StepArgs {
   R* f[F_NFIELDS];   // nullptr = absent (optional fields); the step's inputs
@@ -244,7 +251,7 @@ struct StepArgs {
   unsigned* pf;      // nz x ncol point flags
   unsigned* cf;      // ncol column flags
   W_t<R>* precip;    // 5 x ncol: rain, graupel, snow, ice, cloud
-  int* ksub;         // 10 x ncol: substeps and ktop per species (diagnostic)
+  int* ksub;         // kSubRows x ncol: substeps and ktop per species, semi-Lagrangian decision hashes
   W_t<R>* semiw;     // semi-Lagrangian sedimentation work: kSemiArrays x (nz + 2) x ncol, or nullptr
   const Const<S_t<R>>* cst;  // derived constants of this context (wave-uniform reads)
   // level-local kernels: bit 0 (phase A) / bit 1 (final update) set = each
@@ -261,9 +268,10 @@ struct StepArgs {
   int first_step;
   // work queue of the fused step (a counter of column groups handed out)
   int* queue;
-  // branch signatures of the level-local kernels (signature runs only):
-  // 4 x nz x ncol words, phase p, class q (0 comparisons, 1 discrete
-  // outcomes), point (c, k) at (2p + q)*nz*ncol + k*ncol + c
+  // branch signatures (signature runs only): 6 x nz x ncol words, phase p
+  // (0 the level-local physics, 1 the final update, 2 the sedimentation),
+  // class q (0 comparisons, 1 discrete outcomes), point (c, k) at
+  // (2p + q)*nz*ncol + k*ncol + c
   unsigned* sig;
 };
 
@@ -294,7 +302,8 @@ struct DualGeom {
 // launch shape of one step (host-side choice, see tempo_hip_tune)
 struct LaunchCfg {
   int fused = 0;      // 1: the whole step in one persistent launch where it applies (k_step_fused)
-  int sed_wave = 1;   // sedimentation: 1 = one wavefront per column (nz <= 128), 0 = one lane per column
+  int sed_wave = 1;   // sedimentation: 1 = one wavefront per column (nz <= 128), 0 = one lane per column,
+                      // 2 = the wavefront kernel issuing every staging pass (check form)
   int order = 0;      // point grouping within blocks: bit 0 phase A, bit 1 final update
   int block_a = 256;  // threads per block of the level-local kernels (64, 128 or 256)
   int block_c = 256;

@@ -197,7 +197,7 @@ class TempoHip:
 
         ``diags``: True allocates every diagnostic, False none, or a dict of
         preallocated tensors (name -> (ncol,) or (nz, ncol)) to write into.
-        ``signature``: an int32 cuda tensor of shape (2, 2, nz, ncol) runs
+        ``signature``: an int32 cuda tensor of shape (3, 2, nz, ncol) runs
         the same step through tempo_hip_branch_signature and receives the
         branch signatures of its points ([phase A, final update][comparisons,
         discrete outcomes]).
@@ -248,9 +248,9 @@ class TempoHip:
         s = stream if stream is not None else torch.cuda.current_stream(ref.device).cuda_stream
         prec = FP32 if dtype == torch.float32 else FP64
         if signature is not None:
-            if signature.dtype != torch.int32 or tuple(signature.shape) != (2, 2, nz, ncol) or \
+            if signature.dtype != torch.int32 or tuple(signature.shape) != (3, 2, nz, ncol) or \
                     not signature.is_contiguous() or not signature.is_cuda:
-                raise ValueError(f"signature: need a contiguous int32 cuda tensor of shape {(2, 2, nz, ncol)}")
+                raise ValueError(f"signature: need a contiguous int32 cuda tensor of shape {(3, 2, nz, ncol)}")
             _check(lib().tempo_hip_branch_signature(self._ctx, ctypes.byref(f), ctypes.byref(d), ctypes.byref(dims),
                                                     float(dt), int(itimestep), prec, signature.data_ptr(),
                                                     ctypes.c_void_p(s)))
@@ -444,10 +444,14 @@ class TempoHip:
         _check(lib().tempo_hip_profile_read(self._ctx, ms.ctypes.data, 4, ctypes.byref(runs)))
         return ms, runs.value
 
-    def substeps(self, ncol):
-        out = np.empty(10 * ncol, dtype=np.int32)
+    def substeps(self, ncol, decisions=False):
+        """(10, ncol) substep counts and tops of the last run (rain, graupel,
+        snow, ice, cloud); decisions: also the rows of the semi-Lagrangian
+        rain and graupel decision hashes, (12, ncol)"""
+        rows = 12 if decisions else 10
+        out = np.empty(rows * ncol, dtype=np.int32)
         _check(lib().tempo_hip_substeps(self._ctx, out.ctypes.data, out.size))
-        return out.reshape(10, ncol)
+        return out.reshape(rows, ncol)
 
 
 # ---- reference-named wrappers ------------------------------------------------

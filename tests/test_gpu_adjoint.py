@@ -243,12 +243,18 @@ def test_multistep_adjoint_matches_chained_tlm(hip):
 
 def chained(ctx, st, m, nsteps, v, w_state, w_rain, w_dbz, dtype=torch.float64):
     """<w, J v> of the chained tangent-linear model over nsteps steps, and
-    the final tangents"""
+    the final tangents.  Each link is linearised at the state the step
+    itself (tempo_hip_run) reaches, as the autograd adjoint's checkpoints
+    are: in FP32 the tangent-linear step's own values differ from the
+    step's in rounding (the dual type's operations are functions of their
+    own, which the step's scopes without fused multiply-adds do not reach),
+    so chaining its states would linearise a different trajectory."""
     s = {k: torch.tensor(x, dtype=dtype, device="cuda") for k, x in st.items()}
     tan = {k: x.clone().to(dtype) for k, x in v.items()}
     lin = 0.0
     for n in range(nsteps):
-        _, dtl, tan = ctx.tlm(s, tan, m["dt"], m["itimestep"] + n)
+        _, dtl, tan = ctx.tlm({k: x.clone() for k, x in s.items()}, tan, m["dt"], m["itimestep"] + n)
+        ctx.run(s, m["dt"], m["itimestep"] + n)
         lin += float((w_rain[n] * dtl["rain_precip"].double()).sum() + (w_dbz[n] * dtl["refl10cm"].double()).sum())
     lin += sum(float((w_state[k] * tan[k].double()).sum()) for k in w_state)
     return lin, tan

@@ -218,41 +218,122 @@ def test_gpu_fp64_stepwise_on_ill_conditioned_case(hip, gpu_tables, name):
     print(name, meta["steps"], "steps, max rel per step", worst)
 
 
+def reference_steps(gpu_tables, flags, st, dt, itimestep, steps=1, nlev=None):
+    """The reference FP64 build (oracle/_ref/libtempo_ref.so, -O0, compiled
+    from the reference sources) over ``steps`` steps of a copy of ``st``;
+    ragged columns (nlev) run as columns of their own level counts, levels
+    above them untouched.  Returns (state, d2, d3) of the last step."""
+    from oracle import ref
+    if not ref.available("o0"):
+        pytest.skip("reference build not present")
+    ref.init(gpu_tables, variant="o0", **flags)
+    out = {k: v.copy() for k, v in st.items()}
+    if nlev is None:
+        for s in range(steps):
+            d2, d3 = ref.run(out, dt, itimestep + s, variant="o0")
+        return out, d2, d3
+    nz, ncol = st["p"].shape
+    d2 = {k: np.zeros(ncol) for k in ref.DIAG2_FIELDS}
+    d3 = {k: np.zeros((nz, ncol)) for k in ref.DIAG3_FIELDS}
+    nlev = np.asarray(nlev)
+    for n in np.unique(nlev):
+        cols = np.where(nlev == n)[0]
+        sub = {k: np.ascontiguousarray(v[:n][:, cols]) for k, v in out.items()}
+        for s in range(steps):
+            e2, e3 = ref.run(sub, dt, itimestep + s, variant="o0")
+        for k, v in sub.items():
+            out[k][:n, cols] = v
+        for k, v in e2.items():
+            d2[k][cols] = v
+        for k, v in e3.items():
+            d3[k][:n, cols] = v
+    return out, d2, d3
+
+
+def check_against_reference(out, d, ref_out, d2, d3, inp, what, d3_names=None):
+    """FP64 twin vs the reference build: state, 2-D and 3-D diagnostics to
+    1e-12 relative (frozen_fraction over the columns holding hydrometeors:
+    the reference leaves it undefined after an early exit)"""
+    has_q = np.zeros(inp["p"].shape[1], bool)
+    for q in ("qc", "qr", "qi", "qs", "qg"):
+        has_q |= (inp[q] > 1e-12).any(axis=0)
+    worst = {}
+    for k in ref_out:
+        worst[k] = rel_inf(out[k], ref_out[k])
+    for k, v in d2.items():
+        g = d[D2_MAP[k]]
+        if k == "frozen_fraction":
+            g, v = g[has_q], v[has_q]
+        if g.size:
+            worst[k] = rel_inf(g, v)
+    for k in (d3_names or d3):
+        worst[k] = rel_inf(d[D3_MAP[k]], d3[k])
+    print(what, "FP64 vs reference build, max rel", max(worst.values()))
+    for k, e in worst.items():
+        assert e <= 1e-12, (what, k, e)
+
+
 @pytest.fixture(scope="module")
 def big_case(hip, gpu_tables):
-    """aerosol- and hail-aware columns at a size the oracle finishes in seconds"""
-    from oracle import oracle as orc
+    """2,048 aerosol- and hail-aware columns of 90 levels, one step of the
+    reference build"""
     from tempo_amd import generator as gen
     st, m = gen.generate("multistep", ncol=2048, col0=50_000)
-    orc.init(gpu_tables, aerosolaware=True, hailaware=True)
-    ref = {k: v.copy() for k, v in st.items()}
-    d2, d3 = orc.run(ref, m["dt"], m["itimestep"])
+    ref, d2, d3 = reference_steps(gpu_tables, dict(aerosolaware=True, hailaware=True), st, m["dt"], m["itimestep"])
     return st, m, ref, d2, d3
 
 
-def test_gpu_fp64_matches_oracle(hip, big_case):
+def test_gpu_fp64_matches_reference_at_2048_columns(hip, big_case):
     st, m, ref, d2, d3 = big_case
     ctx = context(hip, dict(aerosolaware=True, hailaware=True))
     out, d = gpu_run(ctx, st, m["dt"], m["itimestep"])
+    check_against_reference(out, d, ref, d2, d3, st, "2048 columns", ("refl10cm", "re_cloud", "re_ice", "re_snow"))
+
+
+def test_gpu_oracle_is_the_reference_at_2048_columns(hip, gpu_tables, big_case):
+    """the C restatement on the same columns, bit for bit"""
+    from oracle import oracle as orc
+    st, m, ref, d2, d3 = big_case
+    orc.init(gpu_tables, aerosolaware=True, hailaware=True)
+    o = {k: v.copy() for k, v in st.items()}
+    e2, e3 = orc.run(o, m["dt"], m["itimestep"])
     for k in ref:
-        assert rel_inf(out[k], ref[k]) <= 1e-12, (k, rel_inf(out[k], ref[k]))
-    for k, v in d2.items():
-        assert rel_inf(d[D2_MAP[k]], v) <= 1e-12, k
-    for k, v in d3.items():
-        assert rel_inf(d[D3_MAP[k]], v) <= 1e-12, k
+        np.testing.assert_array_equal(o[k], ref[k], err_msg=k)
+
+
+def test_gpu_single_column_of_two_levels(hip, gpu_tables):
+    """nz = 2, ncol = 1 (the smallest step the library takes): FP64 twin vs
+    the reference build, and vs the oracle"""
+    from oracle import oracle as orc
+    from tempo_amd import generator as gen
+    flags = dict(aerosolaware=True, hailaware=True)
+    full, m = gen.generate("multistep", ncol=64, col0=3)
+    nz = full["p"].shape[0]
+    # the two levels of a column around its cloud
+    has = (full["qc"] > 1e-12) | (full["qr"] > 1e-12) | (full["qs"] > 1e-12)
+    c = int(np.argmax(has.sum(axis=0)))
+    k0 = int(np.where(has[:, c])[0][0]) if has[:, c].any() else 0
+    k0 = min(k0, nz - 2)
+    st = {k: np.ascontiguousarray(v[k0:k0 + 2, c:c + 1]) for k, v in full.items()}
+    ref, d2, d3 = reference_steps(gpu_tables, flags, st, m["dt"], m["itimestep"], steps=3)
+    out, d = gpu_run(context(hip, flags), st, m["dt"], m["itimestep"], steps=3)
+    check_against_reference(out, d, ref, d2, d3, st, "2 levels x 1 column")
+    orc.init(gpu_tables, **flags)
+    o = {k: v.copy() for k, v in st.items()}
+    for s in range(3):
+        orc.run(o, m["dt"], m["itimestep"] + s)
+    for k in ref:
+        np.testing.assert_array_equal(o[k], ref[k], err_msg=k)
 
 
 def test_gpu_semi_sedi_fp64_matches_oracle(hip, gpu_tables):
     """Semi-Lagrangian rain and graupel sedimentation on 512 aerosol- and
     hail-aware columns at dt = 120 s (several semi-Lagrangian substeps in the
     convective columns): FP64 path vs the oracle."""
-    from oracle import oracle as orc
     from tempo_amd import generator as gen
     st, m = gen.generate("multistep", ncol=512, col0=70_000)
     flags = dict(aerosolaware=True, hailaware=True, semi_sedi=True)
-    orc.init(gpu_tables, **flags)
-    ref = {k: v.copy() for k, v in st.items()}
-    d2, d3 = orc.run(ref, 120.0, m["itimestep"])
+    ref, d2, d3 = reference_steps(gpu_tables, flags, st, 120.0, m["itimestep"])
     out, d = gpu_run(context(hip, flags), st, 120.0, m["itimestep"])
     for k in ref:
         assert rel_inf(out[k], ref[k]) <= 1e-12, (k, rel_inf(out[k], ref[k]))
@@ -284,15 +365,15 @@ def check_fp32_report(rep, precip_all=True, tendencies=True, theirs=None, semi=F
     """the FP32 tolerances (tempo_amd/branch_report.py defines the measures):
     every tendency within 1e-4 relative L2 over the points that took the FP64
     run's branches (tendencies) -- or, for a field whose error in the
-    reference's own single-precision build on the same inputs (theirs)
-    exceeds 1e-4, within twice that; surface precipitation within 1e-5
+    reference's own single-precision build on the same inputs and the same
+    points (theirs) exceeds 1e-4, within twice that; surface precipitation within 1e-5
     relative L2 over all columns (precip_all), or else over the columns that
     did not move by more than 1e-4, and (flux form) every column that did
     moved with a branch flip at its lowest two levels or in its
     sedimentation"""
     if tendencies:
         for k, e in rep["fields"].items():
-            tol = 1e-4 if theirs is None else max(1e-4, 2.0 * theirs[k])
+            tol = 1e-4 if theirs is None else max(1e-4, 2.0 * theirs[k]["rel_l2"])
             assert e["rel_l2"] <= tol, (k, e, None if theirs is None else theirs[k])
     p = rep["precip"]
     if precip_all:
@@ -321,37 +402,36 @@ def report_line(tag, rep):
 
 
 def reference_sp_raw(gpu_tables, flags, rep, st, dt, itimestep):
-    """relative L2 of each tendency of the reference's own single-precision
-    build against the same (unrounded) FP64 run, or None without that build"""
+    """the reference's own single-precision build (oracle/_ref, -DSINGLE_PREC)
+    on the same FP32-representable inputs, per tendency: rel_l2 over the
+    points where our FP32 step took the FP64 run's branches, against the
+    FP64 output rounded to FP32 -- our rel_l2's measure on the same points
+    -- and rel_l2_raw over all points against the unrounded FP64 output;
+    None without that build"""
     from oracle import ref
     from tempo_amd import branch_report as br
     if not ref.available("sp"):
         return None
     inp = {k: v.astype(np.float32).astype(np.float64) for k, v in st.items()}
     _, _, o64, _ = rep["outputs"]
+    keep = rep["keep"]
     ref.init(gpu_tables, variant="sp", **flags)
     osp = {k: v.copy() for k, v in inp.items()}
     ref.run(osp, dt, itimestep, variant="sp")
-    return {k: br._rel((osp[k] - inp[k]) / dt, (o64[k] - inp[k]) / dt) for k in rep["fields"]}
-
-
-def check_raw_against_reference_sp(rep, theirs):
-    """a field whose error over the points without branch flips exceeds 1e-4
-    (its FP32 inputs alone limit it: a small tendency of a large state, or
-    qv - qvs cancelling near saturation) is, against the unrounded FP64 run,
-    within twice the reference's own FP32 build's error on the same inputs.
-    (Over all points the errors of the other fields are set by the few
-    points whose branches flipped; those are counted, not bounded here.)"""
-    if theirs is None:
-        return
-    for k, e in rep["fields"].items():
-        if e["rel_l2"] > 1e-4:
-            assert e["rel_l2_raw"] <= 2.0 * theirs[k], (k, e["rel_l2_raw"], theirs[k])
+    out = {}
+    for k in rep["fields"]:
+        t_sp = (osp[k] - inp[k]) / dt
+        t64r = (o64[k].astype(np.float32).astype(np.float64) - inp[k]) / dt
+        out[k] = dict(rel_l2=br._rel(t_sp[keep], t64r[keep]), rel_l2_raw=br._rel(t_sp, (o64[k] - inp[k]) / dt))
+    return out
 
 
 def print_theirs(theirs):
     if theirs is not None:
-        print("  reference FP32 build against unrounded FP64:", {k: f"{v:.2e}" for k, v in theirs.items()})
+        print("  reference FP32 build, same points, against FP64 rounded:",
+              {k: f"{v['rel_l2']:.2e}" for k, v in theirs.items()})
+        print("  reference FP32 build, all points, against unrounded FP64:",
+              {k: f"{v['rel_l2_raw']:.2e}" for k, v in theirs.items()})
 
 
 @pytest.mark.parametrize("dtype", ["fp32", "fp64"])
@@ -373,7 +453,8 @@ def test_gpu_signature_run_is_the_step(hip, big_case, dtype):
         np.testing.assert_array_equal(dg[k], dw[k], err_msg=k)
     _, _, sig2, _ = br.signature_run(ctx, sub, m["dt"], m["itimestep"], tdt)
     np.testing.assert_array_equal(sig, sig2)
-    assert (sig != 0).all()
+    assert (sig[:2] != 0).all()  # the level-local phases record every point; flux-form sedimentation none
+    assert (sig[2] == 0).all()
     # a point with only cloud water, at the presence threshold
     k, c = 30, 17
     a = {kk: v.copy() for kk, v in sub.items()}
@@ -408,7 +489,6 @@ def test_gpu_fp32_within_tolerance(hip, big_case, gpu_tables, semi):
     print_theirs(theirs)
     check_fp32_report(rep, precip_all=not semi, theirs=theirs, semi=semi)
     assert rep["flipped_points"] <= 0.25, rep["flipped_points"]
-    check_raw_against_reference_sp(rep, theirs)
     from oracle import ref
     if ref.available("sp"):
         # the reference's own single-precision build on the same inputs,
@@ -445,7 +525,6 @@ def test_gpu_fp32_bench_workload(hip, gpu_tables, workload, aero):
     print_theirs(theirs)
     check_fp32_report(rep, theirs=theirs)
     assert rep["flipped_points"] <= 0.25, rep["flipped_points"]
-    check_raw_against_reference_sp(rep, theirs)
     st, _ = gpu_run(ctx, st, m["dt"], m["itimestep"], steps=8)
     rep = br.compare(ctx, st, m["dt"], m["itimestep"] + 8)
     theirs = reference_sp_raw(gpu_tables, flags, rep, st, m["dt"], m["itimestep"] + 8)
@@ -453,7 +532,6 @@ def test_gpu_fp32_bench_workload(hip, gpu_tables, workload, aero):
     print_theirs(theirs)
     assert rep["precip"]["nonzero_columns"] > 0
     check_fp32_report(rep, theirs=theirs)
-    check_raw_against_reference_sp(rep, theirs)
 
 
 @pytest.mark.parametrize("semi", [False, True])
@@ -476,7 +554,6 @@ def test_gpu_fp32_cold_surface(hip, gpu_tables, semi):
         # each kind on its own, over all columns
         assert p[n]["rel_l2_all"] <= 1e-4, (n, p[n])
     check_fp32_report(rep, precip_all=not semi, theirs=theirs, semi=semi)
-    check_raw_against_reference_sp(rep, theirs)
 
 
 def ml_columns(ncol, col0, with_bl):
@@ -506,13 +583,9 @@ def test_gpu_ml_cloud_number_matches_oracle(hip, gpu_tables, which):
     boundary-layer cloud in the cloud radius (ml_for_bl_nc with qc_bl,
     qcfrac_bl), FP64 path vs the oracle on 512 columns, two steps; and the
     FP32 path vs the FP64 twin on the cloud diagnostics the network feeds."""
-    from oracle import oracle as orc
     with_bl, flags = ML_CASES[which]
     st, m = ml_columns(512, 4000, with_bl)
-    orc.init(gpu_tables, **flags)
-    ref = {k: v.copy() for k, v in st.items()}
-    for s in range(2):
-        d2, d3 = orc.run(ref, m["dt"], m["itimestep"] + s)
+    ref, d2, d3 = reference_steps(gpu_tables, flags, st, m["dt"], m["itimestep"], steps=2)
     ctx = context(hip, flags)
     out, d = gpu_run(ctx, st, m["dt"], m["itimestep"], steps=2)
     for k in ref:
@@ -546,22 +619,17 @@ def test_gpu_melting_reflectivity_matches_oracle(hip, gpu_tables):
     (refl10cm_from_melting_flag, src/module_mp_tempo_diags.F90:59-395) on 512
     hail- and aerosol-aware columns: FP64 path vs the oracle; FP32 path vs the
     FP64 twin (dBZ)."""
-    from oracle import oracle as orc
     from tempo_amd import generator as gen
     st, m = gen.generate("multistep", ncol=512, col0=91_000)
     flags = dict(aerosolaware=True, hailaware=True, refl_from_melting=True)
-    orc.init(gpu_tables, **flags)
-    ref = {k: v.copy() for k, v in st.items()}
-    d2, d3 = orc.run(ref, m["dt"], m["itimestep"])
+    ref, d2, d3 = reference_steps(gpu_tables, flags, st, m["dt"], m["itimestep"])
     ctx = context(hip, flags)
     out, d = gpu_run(ctx, st, m["dt"], m["itimestep"])
     for k in ref:
         assert rel_inf(out[k], ref[k]) <= 1e-12, k
     assert rel_inf(d["refl10cm"], d3["refl10cm"]) <= 1e-12, rel_inf(d["refl10cm"], d3["refl10cm"])
     # the melting branch is reached
-    orc.init(gpu_tables, **dict(flags, refl_from_melting=False))
-    dry = {k: v.copy() for k, v in st.items()}
-    _, e3 = orc.run(dry, m["dt"], m["itimestep"])
+    _, _, e3 = reference_steps(gpu_tables, dict(flags, refl_from_melting=False), st, m["dt"], m["itimestep"])
     assert (np.abs(e3["refl10cm"] - d3["refl10cm"]) > 0.01).sum() > 100
     inp = {k: v.astype(np.float32).astype(np.float64) for k, v in st.items()}
     o64, e64 = gpu_run(ctx, inp, m["dt"], m["itimestep"], dtype=torch.float64)
@@ -633,14 +701,20 @@ def test_gpu_run_cfg_changes_between_steps(hip):
     ctx.close()
 
 
-def test_gpu_ragged_matches_oracle(hip, gpu_tables):
+def test_gpu_ragged_matches_reference(hip, gpu_tables):
+    """512 columns of 30-128 active levels: the FP64 twin vs the reference
+    build run on each column's own levels, and vs the oracle's ragged step"""
     from oracle import oracle as orc
     from tempo_amd import generator as gen
     st, m = gen.generate("ragged", ncol=512, col0=1000)
-    orc.init(gpu_tables, aerosolaware=False, hailaware=True)
-    ref = {k: v.copy() for k, v in st.items()}
-    orc.run(ref, m["dt"], m["itimestep"], nlev=m["nlev"])
-    ctx = context(hip, dict(aerosolaware=False, hailaware=True))
+    flags = dict(aerosolaware=False, hailaware=True)
+    ref, _, _ = reference_steps(gpu_tables, flags, st, m["dt"], m["itimestep"], nlev=m["nlev"])
+    orc.init(gpu_tables, **flags)
+    o = {k: v.copy() for k, v in st.items()}
+    orc.run(o, m["dt"], m["itimestep"], nlev=m["nlev"])
+    for k in ref:
+        np.testing.assert_array_equal(o[k], ref[k], err_msg=k)
+    ctx = context(hip, flags)
     out, _ = gpu_run(ctx, st, m["dt"], m["itimestep"], nlev=m["nlev"])
     for k in ref:
         assert rel_inf(out[k], ref[k]) <= 1e-12, k
@@ -763,15 +837,17 @@ def test_gpu_host_entry_points_share_one_context(hip):
         c.close()
 
 
-def test_gpu_single_column_and_tiny_grid(hip):
+def test_gpu_single_column_and_tiny_grid(hip, gpu_tables):
+    """one column of the plain workload, and its lowest two levels: the FP64
+    twin vs the reference build"""
     from tempo_amd import generator as gen
-    from oracle import oracle as orc
+    flags = dict(aerosolaware=False, hailaware=False)
     st, m = gen.generate("plain", ncol=1, col0=3)
-    ctx = context(hip, dict(aerosolaware=False, hailaware=False))
-    out, _ = gpu_run(ctx, st, m["dt"], m["itimestep"])
-    small = {k: np.ascontiguousarray(v[:2]) for k, v in st.items()}
-    out2, _ = gpu_run(ctx, small, m["dt"], m["itimestep"])
-    assert np.isfinite(out["t"]).all() and np.isfinite(out2["t"]).all()
+    ctx = context(hip, flags)
+    for inp in (st, {k: np.ascontiguousarray(v[:2]) for k, v in st.items()}):
+        out, d = gpu_run(ctx, inp, m["dt"], m["itimestep"])
+        ref, d2, d3 = reference_steps(gpu_tables, flags, inp, m["dt"], m["itimestep"])
+        check_against_reference(out, d, ref, d2, d3, inp, f"{inp['p'].shape[0]} levels x 1 column")
 
 
 def test_gpu_error_codes(hip):
@@ -842,7 +918,9 @@ def test_gpu_launch_shapes_give_identical_results(hip, dtype):
     produces the same bits: the wavefront-per-column sedimentation performs
     the serial kernel's arithmetic level by level, and the point order only
     regroups independent points into wavefronts.  The 64-level case fills
-    exactly one staging pass of the sedimentation kernel's blocks."""
+    exactly one staging pass of the sedimentation kernel's blocks; the check
+    form of its staging (sed_wave 2) also issues the second pass there, every
+    lane loading the group's last element and writing nothing."""
     from tempo_amd import generator as gen
     tdt = getattr(torch, dtype)
     prec = hip.FP32 if dtype == "float32" else hip.FP64
@@ -854,7 +932,7 @@ def test_gpu_launch_shapes_give_identical_results(hip, dtype):
         nl = None if "nlev" not in m else torch.tensor(m["nlev"], dtype=torch.int32, device="cuda")
         results = []
         for sed, ba, bc, order in ((0, 256, 256, 0), (1, 256, 256, 0), (1, 64, 128, 0), (1, 128, 64, 1),
-                                   (1, 256, 256, 3), (0, 64, 64, 2), (1, 128, 256, 2)):
+                                   (1, 256, 256, 3), (0, 64, 64, 2), (1, 128, 256, 2), (2, 256, 256, 0)):
             ctx.tune(prec, hip.TUNE_SED_WAVE, sed)
             ctx.tune(prec, hip.TUNE_BLOCK_A, ba)
             ctx.tune(prec, hip.TUNE_BLOCK_C, bc)
@@ -910,7 +988,7 @@ def test_gpu_fused_step_equals_four_launches(hip, workload, dtype):
 
 def test_gpu_tune_rejects_bad_values(hip):
     ctx = context(hip, dict(aerosolaware=False, hailaware=False))
-    for key, val in ((hip.TUNE_SED_WAVE, 2), (hip.TUNE_BLOCK_A, 96), (hip.TUNE_BLOCK_C, 512), (hip.TUNE_ORDER, 4),
+    for key, val in ((hip.TUNE_SED_WAVE, 3), (hip.TUNE_BLOCK_A, 96), (hip.TUNE_BLOCK_C, 512), (hip.TUNE_ORDER, 4),
                      (99, 1)):
         with pytest.raises(hip.TempoError) as e:
             ctx.tune(hip.FP32, key, val)
