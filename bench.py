@@ -17,9 +17,10 @@ Prints one JSON line with the metric, the roofline of the dominant kernel
 (algorithmic HBM bytes over its event-timed average duration, with its VALU
 and transcendental issue rates from the committed counter pass), the FP32
 against FP64 report with branch accounting, the forward + adjoint timings
-(16,384 x 64 single step; 12 steps x 262,144 x 90) and the reference CPU
-step timed on this host's cores over a bounded sample of the same seeded
-step (oracle/cpu_baseline.py, run in a child process).
+(16,384 x 64 single step; 12 steps x 262,144 x 90), the step on host
+arrays with its PCIe transfers (tempo_hip_run_host_prec, FP64 and FP32) and
+the reference CPU step timed on this host's cores over a bounded sample of
+the same seeded step (oracle/cpu_baseline.py, run in a child process).
 """
 import argparse
 import json
@@ -162,6 +163,35 @@ def fp32_parity(ncol=8192, steps_before=8):
                              m["itimestep"] + steps_before))
     return {"sample": f"{ncol} columns of the bench workload", "seeded_step": seeded,
             f"after_{steps_before}_steps": later, "tolerance": {"tendency_rel_l2": 1e-4, "precip_rel_l2": 1e-5}}
+
+
+def host_path(ncol=100_000, reps=3):
+    """The step on host arrays (tempo_hip_run_host_prec, the Fortran
+    binding's path): the aerosol workload's FP64 host arrays copied to the
+    device, stepped in FP64 and in FP32, and the updated fields and
+    diagnostics copied back, PCIe transfers included."""
+    import time
+    import numpy as np
+    from tempo_amd import generator as gen
+    from tempo_amd import host
+    ctx = host.TempoHip(aerosolaware_flag=1, hailaware_flag=0)
+    st, m = gen.generate("aerosol", ncol=ncol)
+    nz = st["p"].shape[0]
+    out = {"workload": f"aerosol-aware step on host float64 arrays, {ncol} columns x {nz} levels, transfers included"}
+    for name, prec in (("fp64", host.FP64), ("fp32", host.FP32)):
+        work = {k: v.copy() for k, v in st.items()}
+        ctx.run_host(work, m["dt"], m["itimestep"], precision=prec)
+        ts = []
+        for _ in range(reps):
+            for k in work:
+                np.copyto(work[k], st[k])
+            t0 = time.perf_counter()
+            ctx.run_host(work, m["dt"], m["itimestep"], precision=prec)
+            ts.append(time.perf_counter() - t0)
+        ms = 1e3 * min(ts)
+        out[name] = {"ms": round(ms, 2), "mcl_per_s": round(ncol * nz / (ms * 1e-3) / 1e6, 1)}
+    ctx.close()
+    return out
 
 
 def ragged_graph(steps_per_graph=10, replays=3, tune=True):
@@ -315,6 +345,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint timing")
     ap.add_argument("--adjoint-steps", type=int, default=3)
     ap.add_argument("--no-ragged", action="store_true", help="skip the uneven-column HIP-graph run")
+    ap.add_argument("--no-host-path", action="store_true", help="skip the host-array (PCIe-inclusive) timing")
     ap.add_argument("--no-parity", action="store_true", help="skip the FP32-against-FP64 report")
     ap.add_argument("--multistep-adjoint", type=int, default=262_144,
                     help="columns of the 12-step chained adjoint timing (0: skip)")
@@ -441,6 +472,8 @@ def main():
         out["adjoint"] = adjoint_timing(a.adjoint_steps)
     if not a.no_ragged:
         out["ragged_graph"] = ragged_graph()
+    if not a.no_host_path:
+        out["host_path"] = host_path()
     if a.multistep_adjoint:
         out["multistep_adjoint"] = multistep_adjoint(ncol=a.multistep_adjoint)
     if not a.no_cpu_baseline:

@@ -171,10 +171,21 @@ int tempo_hip_branch_signature(tempo_hip_ctx *ctx, const tempo_hip_fields *f, co
                                void *stream);
 
 /* As tempo_hip_run with host arrays (FP64, Fortran layout of `dims`): copies
- * to the device, runs and copies back; for hosts that keep state on the CPU.
- * land and nlev are host pointers here. */
+ * to the device, runs and copies back; for hosts that keep state on the CPU
+ * (tempo_run, src/module_mp_tempo_driver.F90:187-279, on host memory).
+ * land and nlev are host pointers here.  The arrays move through two pinned
+ * chunks of the context in turn (host threads copy one while the copy
+ * engine moves the other); only the fields the step updates come back, and
+ * the diagnostics go in only where the step leaves points of their spans
+ * unwritten (padded rows, levels k >= nlev[c]).  Returns when the caller's
+ * arrays hold the results. */
 int tempo_hip_run_host(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_diags *d,
                        const tempo_hip_dims *dims, double dt, int itimestep);
+/* The same, stepped in `precision` on the device (TEMPO_HIP_FP32: on FP32
+ * copies of the caller's FP64 arrays, converted back at the points the step
+ * writes; halos and padding keep their FP64 values). */
+int tempo_hip_run_host_prec(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_diags *d,
+                            const tempo_hip_dims *dims, double dt, int itimestep, int precision);
 
 /* Replaces tempo_aerosol_surface_emissions
  * (src/module_mp_tempo_driver.F90:516-529): nwfa(i, k, j) += nwfa2d(i, j) * dt

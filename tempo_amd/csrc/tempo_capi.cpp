@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -183,9 +184,14 @@ struct tempo_hip_ctx {
   int last_ncol = 0;
   void* last_ksub = nullptr;  // substep counts of the last run (its scratch)
   void* last_vt = nullptr;    // its wide sedimentation planes
-  // host-array staging
-  std::vector<DevBuf> stage;
+  // host-array staging: device copies of the caller's arrays (FP64, and
+  // their FP32 working copies), two pinned chunks, the host runs' stream
+  std::vector<DevBuf> stage, stage32;
   DevBuf stage_land, stage_nlev;
+  void* pin[2] = {nullptr, nullptr};
+  size_t pin_bytes = 0;
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
+  hipStream_t host_stream = nullptr;
   double build_ms = 0.0;
   LaunchCfg launch[2];  // [FP32, FP64]
   // per-phase timing: 5 events per profiled run
@@ -193,6 +199,11 @@ struct tempo_hip_ctx {
   std::vector<hipEvent_t> events;
   ~tempo_hip_ctx() {
     for (auto e : events) (void)hipEventDestroy(e);
+    for (int i = 0; i < 2; ++i) {
+      if (pin[i]) (void)hipHostFree(pin[i]);
+      if (pin_ev[i]) (void)hipEventDestroy(pin_ev[i]);
+    }
+    if (host_stream) (void)hipStreamDestroy(host_stream);
   }
 };
 
@@ -785,6 +796,224 @@ int run_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fie
   return TEMPO_HIP_OK;
 }
 
+int check_run_args(const tempo_hip_fields* f, const tempo_hip_dims* dm, double dt) {
+  if (!f || !dm) return fail(TEMPO_HIP_E_ARG, "NULL fields/dims");
+  if (dm->ni <= 0 || dm->nj <= 0 || dm->nz < 2) return fail(TEMPO_HIP_E_ARG, "bad dims (need ni,nj >= 1, nz >= 2)");
+  if (dm->ldk < dm->ni) return fail(TEMPO_HIP_E_ARG, "ldk smaller than ni");
+  if (dm->nj > 1 && dm->ldj < dm->ldk * dm->nz) return fail(TEMPO_HIP_E_ARG, "ldj too small");
+  if (!(dt > 0.0)) return fail(TEMPO_HIP_E_ARG, "dt must be positive");
+  if (!f->p || !f->w || !f->dz || !f->qv || !f->qc || !f->qr || !f->qi || !f->qs || !f->qg || !f->ni || !f->nr)
+    return fail(TEMPO_HIP_E_ARG, "missing required field (p, w, dz, qv, qc, qr, qi, qs, qg, ni, nr)");
+  if (!f->t && !(f->th && f->pii)) return fail(TEMPO_HIP_E_ARG, "need t, or th and pii");
+  return TEMPO_HIP_OK;
+}
+
+// ---- host arrays ---------------------------------------------------------
+// A host-array step moves each array through two pinned chunks in turn:
+// host threads copy one chunk between the caller's (pageable) array and
+// pinned memory while the copy engine moves the other across the link, so
+// the transfers run near the pinned rate instead of the pageable one.
+
+constexpr size_t kPinChunk = size_t(64) << 20;
+
+// dst <- src, n bytes, split over host threads
+void par_copy(void* dst, const void* src, size_t n) {
+  const size_t kMin = size_t(4) << 20;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const unsigned nt = unsigned(std::min<size_t>(std::min(8u, hw), (n + kMin - 1) / kMin));
+  if (nt <= 1) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t per = (n + nt - 1) / nt;
+  for (unsigned i = 0; i < nt; ++i) {
+    const size_t a = std::min(n, size_t(i) * per), b = std::min(n, a + per);
+    if (b > a)
+      th.emplace_back([=] { std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a); });
+  }
+  for (auto& t : th) t.join();
+}
+
+int ensure_pinned(tempo_hip_ctx* x) {
+  if (!x->host_stream) HIP_TRY(hipStreamCreateWithFlags(&x->host_stream, hipStreamNonBlocking));
+  for (int i = 0; i < 2; ++i) {
+    if (!x->pin_ev[i]) HIP_TRY(hipEventCreateWithFlags(&x->pin_ev[i], hipEventDisableTiming));
+    if (!x->pin[i]) HIP_TRY(hipHostMalloc(&x->pin[i], kPinChunk, hipHostMallocDefault));
+  }
+  x->pin_bytes = kPinChunk;
+  return TEMPO_HIP_OK;
+}
+
+struct Xfer {
+  void* host;
+  void* dev;
+  size_t n;
+};
+
+// host -> device for every transfer, chunk by chunk through the pinned pair
+int h2d_all(tempo_hip_ctx* x, const std::vector<Xfer>& xs) {
+  int b = 0;
+  for (const Xfer& t : xs)
+    for (size_t off = 0; off < t.n; off += x->pin_bytes) {
+      const size_t len = std::min(x->pin_bytes, t.n - off);
+      HIP_TRY(hipEventSynchronize(x->pin_ev[b]));  // this chunk's previous copy has left it
+      par_copy(x->pin[b], static_cast<const char*>(t.host) + off, len);
+      HIP_TRY(hipMemcpyAsync(static_cast<char*>(t.dev) + off, x->pin[b], len, hipMemcpyHostToDevice, x->host_stream));
+      HIP_TRY(hipEventRecord(x->pin_ev[b], x->host_stream));
+      b ^= 1;
+    }
+  return TEMPO_HIP_OK;
+}
+
+// device -> host: the copy of chunk i+1 runs while the host moves chunk i
+int d2h_all(tempo_hip_ctx* x, const std::vector<Xfer>& xs) {
+  std::vector<Xfer> ch;
+  for (const Xfer& t : xs)
+    for (size_t off = 0; off < t.n; off += x->pin_bytes)
+      ch.push_back({static_cast<char*>(t.host) + off, static_cast<char*>(t.dev) + off, std::min(x->pin_bytes, t.n - off)});
+  auto issue = [&](size_t i) -> int {
+    const int b = int(i & 1);
+    HIP_TRY(hipMemcpyAsync(x->pin[b], ch[i].dev, ch[i].n, hipMemcpyDeviceToHost, x->host_stream));
+    HIP_TRY(hipEventRecord(x->pin_ev[b], x->host_stream));
+    return TEMPO_HIP_OK;
+  };
+  int rc;
+  if (!ch.empty() && (rc = issue(0))) return rc;
+  for (size_t i = 0; i < ch.size(); ++i) {
+    if (i + 1 < ch.size() && (rc = issue(i + 1))) return rc;
+    HIP_TRY(hipEventSynchronize(x->pin_ev[i & 1]));
+    par_copy(ch[i].host, x->pin[i & 1], ch[i].n);
+  }
+  return TEMPO_HIP_OK;
+}
+
+// element conversion over a whole span, or over the points of a tile (the
+// points a step writes: i < ni, k < nk and, with nlev, k < nlev[i + j*ni])
+template <class S, class D>
+__global__ void __launch_bounds__(256) k_cast(const S* __restrict__ in, D* __restrict__ out, size_t n) {
+  for (size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += size_t(gridDim.x) * blockDim.x)
+    out[e] = D(in[e]);
+}
+template <class S, class D>
+__global__ void __launch_bounds__(256) k_cast_tile(const S* __restrict__ in, D* __restrict__ out, DualGeom g,
+                                                   const int* __restrict__ nlev) {
+  const int i = int(blockIdx.x) * 256 + int(threadIdx.x), k = int(blockIdx.y), j = int(blockIdx.z);
+  if (i >= g.ni) return;
+  if (nlev && k >= nlev[i + j * g.ni]) return;
+  const size_t e = size_t(i) + size_t(k) * size_t(g.ldk) + size_t(j) * size_t(g.ldj);
+  out[e] = D(in[e]);
+}
+template <class S, class D>
+hipError_t cast_span(const S* in, D* out, size_t n, hipStream_t s) {
+  const unsigned grid = unsigned(std::min<size_t>((n + 255) / 256, 65536));
+  if (n) hipLaunchKernelGGL((k_cast<S, D>), dim3(grid), dim3(256), 0, s, in, out, n);
+  return hipGetLastError();
+}
+template <class S, class D>
+hipError_t cast_tile(const S* in, D* out, const DualGeom& g, const int* nlev, hipStream_t s) {
+  hipLaunchKernelGGL((k_cast_tile<S, D>), dim3(unsigned((g.ni + 255) / 256), unsigned(g.nk), unsigned(g.nj)), dim3(256),
+                     0, s, in, out, g, nlev);
+  return hipGetLastError();
+}
+
+// tempo_hip_run_host(_prec): the caller's FP64 host arrays are copied to the
+// device (the diagnostics only where the step does not write every point of
+// their spans: padded rows, or ragged columns), stepped in `precision` (FP32:
+// on FP32 copies, converted back at the points the step writes) and the
+// updated fields and diagnostics copied back.
+int run_host_impl(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags* d, const tempo_hip_dims* dm,
+                  double dt, int itimestep, int precision) {
+  if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
+  int rc = check_run_args(f, dm, dt);
+  if (rc) return rc;
+  if ((rc = ensure_pinned(x))) return rc;
+  hipStream_t s = x->host_stream;
+  const bool fp32 = precision == TEMPO_HIP_FP32;
+  const size_t n3 = size_t(dm->ldj) * (dm->nj - 1) + size_t(dm->ldk) * (dm->nz - 1) + dm->ni;
+  const size_t nd3 = size_t(dm->dldj) * (dm->nj - 1) + size_t(dm->dldk) * (dm->nz - 1) + dm->ni;
+  const size_t nd2 = size_t(dm->d2ldj) * (dm->nj - 1) + dm->ni;
+  constexpr int kNf = 21;  // the 19 fields, qc_bl, qcfrac_bl
+  void* const hf[kNf] = {f->t,  f->th, f->pii, f->p,  f->w,  f->dz,   f->qv,   f->qc, f->qr,   f->qi,    f->qs,
+                         f->qg, f->ni, f->nr,  f->nc, f->nwfa, f->nifa, f->ng, f->qb, const_cast<void*>(f->qc_bl),
+                         const_cast<void*>(f->qcfrac_bl)};
+  // fields the step writes: temperature (t, else th) and the water species,
+  // numbers and aerosols
+  auto written = [&](int i) { return i == 0 || (i == 1 && !f->t) || (i >= 6 && i <= 18); };
+  const int nd = sizeof(tempo_hip_diags) / sizeof(void*);
+  if (x->stage.size() < size_t(kNf + nd)) x->stage.resize(kNf + nd);
+  if (fp32 && x->stage32.size() < size_t(kNf + nd)) x->stage32.resize(kNf + nd);
+  const size_t w = fp32 ? sizeof(float) : sizeof(double);
+  tempo_hip_fields df = *f;
+  void** dfp[kNf] = {&df.t,  &df.th, &df.pii, &df.p,  &df.w,  &df.dz,   &df.qv,   &df.qc, &df.qr,    &df.qi,    &df.qs,
+                     &df.qg, &df.ni, &df.nr,  &df.nc, &df.nwfa, &df.nifa, &df.ng, &df.qb,
+                     const_cast<void**>(&df.qc_bl), const_cast<void**>(&df.qcfrac_bl)};
+  std::vector<Xfer> up, down;
+  for (int i = 0; i < kNf; ++i) {
+    if (!hf[i]) continue;
+    HIP_TRY(x->stage[i].ensure(n3 * sizeof(double)));
+    up.push_back({hf[i], x->stage[i].p, n3 * sizeof(double)});
+    if (written(i)) down.push_back({hf[i], x->stage[i].p, n3 * sizeof(double)});
+    if (fp32) HIP_TRY(x->stage32[i].ensure(n3 * w));
+    *dfp[i] = fp32 ? x->stage32[i].p : x->stage[i].p;
+  }
+  // diagnostics: uploaded where the step leaves points of their spans
+  // unwritten (padding between rows, levels k >= nlev[c]), so that those
+  // come back unchanged
+  const bool dense3 = !dm->nlev && dm->dldk == dm->ni && (dm->nj == 1 || dm->dldj == dm->dldk * dm->nz);
+  const bool dense2 = dm->nj == 1 || dm->d2ldj == dm->ni;
+  tempo_hip_diags dd{};
+  void** hd = reinterpret_cast<void**>(const_cast<tempo_hip_diags*>(d));
+  void** ddp = reinterpret_cast<void**>(&dd);
+  for (int i = 0; d && i < nd; ++i) {
+    if (!hd[i]) continue;
+    const bool two = i < D2_N;
+    const size_t n = two ? nd2 : nd3;
+    DevBuf& b = x->stage[kNf + i];
+    HIP_TRY(b.ensure(n * sizeof(double)));
+    if (!(two ? dense2 : dense3)) up.push_back({hd[i], b.p, n * sizeof(double)});
+    down.push_back({hd[i], b.p, n * sizeof(double)});
+    if (fp32) HIP_TRY(x->stage32[kNf + i].ensure(n * w));
+    ddp[i] = fp32 ? x->stage32[kNf + i].p : b.p;
+  }
+  tempo_hip_dims dmd = *dm;
+  const int ncol = dm->ni * dm->nj;
+  if (f->land) {
+    HIP_TRY(x->stage_land.ensure(ncol * sizeof(int)));
+    HIP_TRY(hipMemcpyAsync(x->stage_land.p, f->land, ncol * sizeof(int), hipMemcpyHostToDevice, s));
+    df.land = x->stage_land.as<int>();
+  }
+  if (dm->nlev) {
+    HIP_TRY(x->stage_nlev.ensure(ncol * sizeof(int)));
+    HIP_TRY(hipMemcpyAsync(x->stage_nlev.p, dm->nlev, ncol * sizeof(int), hipMemcpyHostToDevice, s));
+    dmd.nlev = x->stage_nlev.as<int>();
+  }
+  HIP_TRY(hipStreamSynchronize(s));  // land and nlev are pageable: their copies finish before the caller's return
+  if ((rc = h2d_all(x, up))) return rc;
+  const DualGeom g3{dm->ni, dm->nj, dm->nz, dm->ldk, dm->ldj};
+  const DualGeom gd3{dm->ni, dm->nj, dm->nz, dm->dldk, dm->dldj};
+  const DualGeom gd2{dm->ni, dm->nj, 1, 0, dm->d2ldj};
+  if (fp32) {
+    for (int i = 0; i < kNf; ++i)
+      if (hf[i]) HIP_TRY(cast_span(x->stage[i].as<double>(), x->stage32[i].as<float>(), n3, s));
+    for (int i = 0; d && i < nd; ++i)
+      if (hd[i] && !(i < D2_N ? dense2 : dense3))
+        HIP_TRY(cast_span(x->stage[kNf + i].as<double>(), x->stage32[kNf + i].as<float>(), i < D2_N ? nd2 : nd3, s));
+    rc = run_impl<float>(x, &df, d ? &dd : nullptr, &dmd, dt, itimestep, s);
+    if (rc) return rc;
+    for (int i = 0; i < kNf; ++i)
+      if (hf[i] && written(i)) HIP_TRY(cast_tile(x->stage32[i].as<float>(), x->stage[i].as<double>(), g3, dmd.nlev, s));
+    for (int i = 0; d && i < nd; ++i)
+      if (hd[i])
+        HIP_TRY(i < D2_N ? cast_tile(x->stage32[kNf + i].as<float>(), x->stage[kNf + i].as<double>(), gd2, nullptr, s)
+                         : cast_tile(x->stage32[kNf + i].as<float>(), x->stage[kNf + i].as<double>(), gd3, dmd.nlev, s));
+  } else {
+    rc = run_impl<double>(x, &df, d ? &dd : nullptr, &dmd, dt, itimestep, s);
+    if (rc) return rc;
+  }
+  return d2h_all(x, down);
+}
+
 }  // namespace
 
 extern "C" {
@@ -845,17 +1074,6 @@ int tempo_hip_reserve(tempo_hip_ctx* x, int ncol, int nz) {
   return TEMPO_HIP_OK;
 }
 
-static int check_run_args(const tempo_hip_fields* f, const tempo_hip_dims* dm, double dt) {
-  if (!f || !dm) return fail(TEMPO_HIP_E_ARG, "NULL fields/dims");
-  if (dm->ni <= 0 || dm->nj <= 0 || dm->nz < 2) return fail(TEMPO_HIP_E_ARG, "bad dims (need ni,nj >= 1, nz >= 2)");
-  if (dm->ldk < dm->ni) return fail(TEMPO_HIP_E_ARG, "ldk smaller than ni");
-  if (dm->nj > 1 && dm->ldj < dm->ldk * dm->nz) return fail(TEMPO_HIP_E_ARG, "ldj too small");
-  if (!(dt > 0.0)) return fail(TEMPO_HIP_E_ARG, "dt must be positive");
-  if (!f->p || !f->w || !f->dz || !f->qv || !f->qc || !f->qr || !f->qi || !f->qs || !f->qg || !f->ni || !f->nr)
-    return fail(TEMPO_HIP_E_ARG, "missing required field (p, w, dz, qv, qc, qr, qi, qs, qg, ni, nr)");
-  if (!f->t && !(f->th && f->pii)) return fail(TEMPO_HIP_E_ARG, "need t, or th and pii");
-  return TEMPO_HIP_OK;
-}
 
 int tempo_hip_run(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags* d, const tempo_hip_dims* dm,
                   double dt, int itimestep, int precision, void* stream) {
@@ -1044,72 +1262,14 @@ int tempo_hip_aerosol_surface_emissions_host(tempo_hip_ctx* x, double dt, double
 
 int tempo_hip_run_host(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags* d,
                        const tempo_hip_dims* dm, double dt, int itimestep) {
-  if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
-  int rc = check_run_args(f, dm, dt);
-  if (rc) return rc;
-  const size_t n3 = size_t(dm->ldj) * (dm->nj - 1) + size_t(dm->ldk) * (dm->nz - 1) + dm->ni;
-  const size_t nd3 = size_t(dm->dldj) * (dm->nj - 1) + size_t(dm->dldk) * (dm->nz - 1) + dm->ni;
-  const size_t nd2 = size_t(dm->d2ldj) * (dm->nj - 1) + dm->ni;
-  void* const hf[19] = {f->t,  f->th, f->pii, f->p,  f->w,  f->dz, f->qv, f->qc,   f->qr,   f->qi,
-                        f->qs, f->qg, f->ni,  f->nr, f->nc, f->nwfa, f->nifa, f->ng, f->qb};
-  tempo_hip_diags dd{};
-  void** hd = reinterpret_cast<void**>(const_cast<tempo_hip_diags*>(d));
-  void** ddp = reinterpret_cast<void**>(&dd);
-  const int nd = sizeof(tempo_hip_diags) / sizeof(void*);
-  if (x->stage.size() < size_t(19 + nd + 2)) x->stage.resize(19 + nd + 2);
-  tempo_hip_fields df = *f;
-  const void* const hbl[2] = {f->qc_bl, f->qcfrac_bl};
-  const void** dbl[2] = {&df.qc_bl, &df.qcfrac_bl};
-  for (int i = 0; i < 2; ++i) {
-    if (!hbl[i]) continue;
-    DevBuf& b = x->stage[19 + nd + i];
-    HIP_TRY(b.ensure(n3 * sizeof(double)));
-    HIP_TRY(hipMemcpy(b.p, hbl[i], n3 * sizeof(double), hipMemcpyHostToDevice));
-    *dbl[i] = b.p;
-  }
-  void** dfp[19] = {&df.t,  &df.th, &df.pii, &df.p,  &df.w,  &df.dz, &df.qv, &df.qc,   &df.qr,   &df.qi,
-                    &df.qs, &df.qg, &df.ni,  &df.nr, &df.nc, &df.nwfa, &df.nifa, &df.ng, &df.qb};
-  for (int i = 0; i < 19; ++i) {
-    if (!hf[i]) continue;
-    HIP_TRY(x->stage[i].ensure(n3 * sizeof(double)));
-    HIP_TRY(hipMemcpy(x->stage[i].p, hf[i], n3 * sizeof(double), hipMemcpyHostToDevice));
-    *dfp[i] = x->stage[i].p;
-  }
-  // diagnostics: the caller's values go in first, so that points the step
-  // does not write (halos and padding between rows, levels k >= nlev[c])
-  // come back unchanged
-  for (int i = 0; d && i < nd; ++i) {
-    if (!hd[i]) continue;
-    const size_t n = (i < 8 ? nd2 : nd3) * sizeof(double);
-    HIP_TRY(x->stage[19 + i].ensure(n));
-    HIP_TRY(hipMemcpy(x->stage[19 + i].p, hd[i], n, hipMemcpyHostToDevice));
-    ddp[i] = x->stage[19 + i].p;
-  }
-  tempo_hip_dims dmd = *dm;
-  const int ncol = dm->ni * dm->nj;
-  if (f->land) {
-    HIP_TRY(x->stage_land.ensure(ncol * sizeof(int)));
-    HIP_TRY(hipMemcpy(x->stage_land.p, f->land, ncol * sizeof(int), hipMemcpyHostToDevice));
-    df.land = x->stage_land.as<int>();
-  }
-  if (dm->nlev) {
-    HIP_TRY(x->stage_nlev.ensure(ncol * sizeof(int)));
-    HIP_TRY(hipMemcpy(x->stage_nlev.p, dm->nlev, ncol * sizeof(int), hipMemcpyHostToDevice));
-    dmd.nlev = x->stage_nlev.as<int>();
-  }
-  rc = run_impl<double>(x, &df, d ? &dd : nullptr, &dmd, dt, itimestep, nullptr);
-  if (rc) return rc;
-  HIP_TRY(hipDeviceSynchronize());
-  for (int i = 0; i < 19; ++i) {
-    if (!hf[i]) continue;
-    HIP_TRY(hipMemcpy(hf[i], x->stage[i].p, n3 * sizeof(double), hipMemcpyDeviceToHost));
-  }
-  for (int i = 0; d && i < nd; ++i) {
-    if (!hd[i]) continue;
-    const size_t n = (i < 8 ? nd2 : nd3) * sizeof(double);
-    HIP_TRY(hipMemcpy(hd[i], ddp[i], n, hipMemcpyDeviceToHost));
-  }
-  return TEMPO_HIP_OK;
+  return run_host_impl(x, f, d, dm, dt, itimestep, TEMPO_HIP_FP64);
+}
+
+int tempo_hip_run_host_prec(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags* d,
+                            const tempo_hip_dims* dm, double dt, int itimestep, int precision) {
+  if (precision != TEMPO_HIP_FP32 && precision != TEMPO_HIP_FP64)
+    return fail(TEMPO_HIP_E_ARG, "precision must be TEMPO_HIP_FP32 or TEMPO_HIP_FP64");
+  return run_host_impl(x, f, d, dm, dt, itimestep, precision);
 }
 
 int tempo_hip_table_export(tempo_hip_ctx* x, int table, double* out, size_t n) {

@@ -26,7 +26,8 @@ module tempo_hip_mod
   public :: tempo_aerosol_surface_emissions
   public :: cloud_check_and_update, ice_check_and_update, snow_check_and_update
   public :: tempo_hip_cfg, tempo_hip_fields, tempo_hip_diags, tempo_hip_dims
-  public :: tempo_hip_cfg_default, tempo_hip_init, tempo_hip_run, tempo_hip_run_host, tempo_hip_last_error
+  public :: tempo_hip_cfg_default, tempo_hip_init, tempo_hip_run, tempo_hip_run_host, tempo_hip_run_host_prec
+  public :: tempo_hip_last_error
 
   integer, parameter :: wp = c_double
 
@@ -95,6 +96,16 @@ module tempo_hip_mod
       type(tempo_hip_dims), intent(in) :: dims
       real(c_double), value :: dt
       integer(c_int), value :: itimestep
+    end function
+    integer(c_int) function tempo_hip_run_host_prec(ctx, f, d, dims, dt, itimestep, precision) &
+        bind(c, name='tempo_hip_run_host_prec')
+      import :: tempo_hip_fields, tempo_hip_diags, tempo_hip_dims, c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      type(tempo_hip_fields), intent(in) :: f
+      type(tempo_hip_diags), intent(in) :: d
+      type(tempo_hip_dims), intent(in) :: dims
+      real(c_double), value :: dt
+      integer(c_int), value :: itimestep, precision
     end function
     type(c_ptr) function tempo_hip_last_error_c() bind(c, name='tempo_hip_last_error')
       import :: c_ptr
@@ -166,6 +177,9 @@ module tempo_hip_mod
     logical :: rain_med_vol_diam_flag = .false.
     logical :: graupel_med_vol_diam_flag = .false.
     logical :: cloud_number_mixing_ratio_flag = .false.
+    ! the step on host arrays in single precision on the device
+    ! (tempo_hip_run_host_prec; the arrays stay real(8))
+    logical :: single_precision_step = .false.
     character(len=256) :: table_dir = ''   ! '' = build the tables on the GPU
     character(len=256) :: ccn_path = ''    ! '' = <table_dir>/ccn_activate.bin
     type(c_ptr) :: ctx = c_null_ptr
@@ -524,7 +538,11 @@ contains
     if (allocated(tempo_diags%cloud_number_mixing_ratio)) &
       d%cloud_number_mixing_ratio = c_loc(tempo_diags%cloud_number_mixing_ratio(its, kts, jts))
 
-    rc = tempo_hip_run_host(tempo_cfgs%ctx, f, d, dm, real(dt, c_double), int(itimestep, c_int))
+    if (tempo_cfgs%single_precision_step) then
+      rc = tempo_hip_run_host_prec(tempo_cfgs%ctx, f, d, dm, real(dt, c_double), int(itimestep, c_int), 4_c_int)
+    else
+      rc = tempo_hip_run_host(tempo_cfgs%ctx, f, d, dm, real(dt, c_double), int(itimestep, c_int))
+    endif
     if (rc /= 0) call fail('tempo_run', rc)
   end subroutine
 

@@ -94,6 +94,8 @@ def lib():
                                     ctypes.c_double, ctypes.c_int, ctypes.c_int, vp]
         L.tempo_hip_run_host.argtypes = [vp, ctypes.POINTER(Fields), ctypes.POINTER(Diags), ctypes.POINTER(Dims),
                                          ctypes.c_double, ctypes.c_int]
+        L.tempo_hip_run_host_prec.argtypes = [vp, ctypes.POINTER(Fields), ctypes.POINTER(Diags), ctypes.POINTER(Dims),
+                                              ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.tempo_hip_branch_signature.argtypes = [vp, ctypes.POINTER(Fields), ctypes.POINTER(Diags),
                                                  ctypes.POINTER(Dims), ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                                  vp, vp]
@@ -350,8 +352,9 @@ class TempoHip:
                                        ctypes.c_void_p(s)))
         return xbar
 
-    def run_host(self, state, dt, itimestep, nlev=None, land=None):
-        """tempo_run on host float64 numpy arrays (staged through device memory)."""
+    def run_host(self, state, dt, itimestep, nlev=None, land=None, precision=FP64):
+        """tempo_run on host float64 numpy arrays (staged through device
+        memory), stepped in ``precision`` (FP64 or FP32)."""
         nz, ncol = state["p"].shape
         f = Fields()
         for name in FIELD_NAMES + BL_NAMES:
@@ -369,8 +372,8 @@ class TempoHip:
         if nlev is not None:
             nlev = np.ascontiguousarray(nlev, dtype=np.int32)
             dims.nlev = nlev.ctypes.data
-        _check(lib().tempo_hip_run_host(self._ctx, ctypes.byref(f), ctypes.byref(d), ctypes.byref(dims),
-                                        float(dt), int(itimestep)))
+        _check(lib().tempo_hip_run_host_prec(self._ctx, ctypes.byref(f), ctypes.byref(d), ctypes.byref(dims),
+                                             float(dt), int(itimestep), int(precision)))
         return out
 
     # argument order of the public check routines (src/module_mp_tempo_main.F90:891-1134)

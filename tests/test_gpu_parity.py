@@ -725,19 +725,28 @@ def test_gpu_ragged_matches_reference(hip, gpu_tables):
             np.testing.assert_array_equal(out[k][n:, c], st[k][n:, c])
 
 
-def test_gpu_host_path_equals_device_path(hip):
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_gpu_host_path_equals_device_path(hip, prec):
+    """tempo_hip_run_host(_prec) on host arrays gives the device path's step
+    bit for bit, in FP64 and in FP32 (FP32-representable inputs), over
+    transfers larger than one pinned chunk (100,000 x 90 columns: 72 MB per
+    field)"""
     from tempo_amd import generator as gen
-    st, m = gen.generate("aerosol", ncol=300, col0=77)
+    st, m = gen.generate("aerosol", ncol=100_000, col0=77)
+    tdt = torch.float64 if prec == "fp64" else torch.float32
+    if prec == "fp32":
+        st = {k: v.astype(np.float32).astype(np.float64) for k, v in st.items()}
     ctx = context(hip, dict(aerosolaware=True, hailaware=False))
-    dev, dd = gpu_run(ctx, st, m["dt"], m["itimestep"])
+    dev, dd = gpu_run(ctx, st, m["dt"], m["itimestep"], dtype=tdt)
     hst = {k: v.copy() for k, v in st.items()}
-    hd = ctx.run_host(hst, m["dt"], m["itimestep"])
+    hd = ctx.run_host(hst, m["dt"], m["itimestep"], precision=hip.FP64 if prec == "fp64" else hip.FP32)
     for k in st:
         np.testing.assert_array_equal(hst[k], dev[k], err_msg=k)
-    np.testing.assert_array_equal(hd["refl10cm"], dd["refl10cm"])
+    for k in dd:
+        np.testing.assert_array_equal(hd[k], dd[k], err_msg=k)
 
 
-def _host_call(ctx, hip, st, m, diag, dims, nlev=None):
+def _host_call(ctx, hip, st, m, diag, dims, nlev=None, precision=None):
     """tempo_hip_run_host through ctypes on numpy arrays with the given diag
     arrays (name -> float64 array) and geometry"""
     import ctypes
@@ -748,22 +757,29 @@ def _host_call(ctx, hip, st, m, diag, dims, nlev=None):
     d = hip.Diags()
     for n, a in diag.items():
         setattr(d, n, a.ctypes.data)
-    hip._check(hip.lib().tempo_hip_run_host(ctx._ctx, ctypes.byref(f), ctypes.byref(d), ctypes.byref(dims),
-                                            float(m["dt"]), int(m["itimestep"])))
+    if precision is None:
+        hip._check(hip.lib().tempo_hip_run_host(ctx._ctx, ctypes.byref(f), ctypes.byref(d), ctypes.byref(dims),
+                                                float(m["dt"]), int(m["itimestep"])))
+    else:
+        hip._check(hip.lib().tempo_hip_run_host_prec(ctx._ctx, ctypes.byref(f), ctypes.byref(d), ctypes.byref(dims),
+                                                     float(m["dt"]), int(m["itimestep"]), int(precision)))
 
 
-def test_gpu_host_path_padded_diagnostics_and_ragged_columns(hip):
-    """tempo_hip_run_host with diagnostics whose rows are padded (d2ldj,
-    dldk > ni, as a Fortran host's halo'd tile) and ragged columns (nlev):
-    tile points equal the device path's, and every point the step does not
-    write -- padding between rows, levels k >= nlev[c] -- keeps the caller's
-    value"""
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_gpu_host_path_padded_diagnostics_and_ragged_columns(hip, prec):
+    """tempo_hip_run_host(_prec) with diagnostics whose rows are padded
+    (d2ldj, dldk > ni, as a Fortran host's halo'd tile) and ragged columns
+    (nlev): tile points equal the device path's, and every point the step
+    does not write -- padding between rows, levels k >= nlev[c] -- keeps the
+    caller's value (in FP32 too: those points are not converted)"""
     from tempo_amd import generator as gen
     st, m = gen.generate("ragged", ncol=96, col0=123)
+    if prec == "fp32":
+        st = {k: v.astype(np.float32).astype(np.float64) for k, v in st.items()}
     nz, ncol = st["p"].shape
     nlev = m["nlev"]
     ctx = context(hip, dict(aerosolaware=False, hailaware=True))
-    dev, dd = gpu_run(ctx, st, m["dt"], m["itimestep"], nlev=nlev)
+    dev, dd = gpu_run(ctx, st, m["dt"], m["itimestep"], nlev=nlev, dtype=torch.float64 if prec == "fp64" else torch.float32)
     ni, nj, pad = 32, 3, 5
     assert ni * nj == ncol
     sentinel = -7.25
@@ -778,7 +794,7 @@ def test_gpu_host_path_padded_diagnostics_and_ragged_columns(hip):
     d3 = {n: np.full(ldk3 * nz * nj, sentinel) for n in hip.DIAG3_NAMES}
     nl = np.ascontiguousarray(nlev, dtype=np.int32)
     dims = hip.Dims(ni, nj, nz, ni, ni * nz, ldk3, ldk3 * nz, ld2, nl.ctypes.data)
-    _host_call(ctx, hip, hst, m, dict(d2, **d3), dims)
+    _host_call(ctx, hip, hst, m, dict(d2, **d3), dims, precision=None if prec == "fp64" else hip.FP32)
     for k in st:
         np.testing.assert_array_equal(hst[k].transpose(1, 0, 2).reshape(nz, ncol), dev[k], err_msg=k)
     i = np.arange(ncol) % ni
