@@ -506,8 +506,9 @@ def test_gpu_fp32_within_tolerance(hip, big_case, gpu_tables, semi):
 
 @pytest.mark.parametrize("workload,aero", [("aerosol", True), ("plain", False)])
 def test_gpu_fp32_bench_workload(hip, gpu_tables, workload, aero):
-    """FP32 against the FP64 twin on 2,048 columns of the benchmark's
-    aerosol-aware workload and of the one-moment-cloud workload: the seeded
+    """FP32 against the FP64 twin on the benchmark's FP32 report sample (the
+    first 8,192 columns of its aerosol-aware workload) and on 8,192 columns
+    of the one-moment-cloud workload: the seeded
     step (each tendency within 1e-4 relative L2 over the points that took the
     same branches), and the step after eight FP64 steps, when rain has
     reached the ground (surface precipitation within 1e-5 over all columns).
@@ -516,7 +517,7 @@ def test_gpu_fp32_bench_workload(hip, gpu_tables, workload, aero):
     single-precision build's error on the same state (or 1e-4)."""
     from tempo_amd import branch_report as br
     from tempo_amd import generator as gen
-    st, m = gen.generate(workload, ncol=2048)
+    st, m = gen.generate(workload, ncol=8192)
     flags = dict(aerosolaware=aero, hailaware=False)
     ctx = context(hip, flags)
     rep = br.compare(ctx, st, m["dt"], m["itimestep"])
