@@ -10,8 +10,10 @@ namespace tempo {
 // Waves per SIMD the level-local kernels are compiled for, per working type.
 // The FP32 level physics needs 130 VGPRs unconstrained (3 waves per SIMD);
 // held to 128 it runs 4 waves with a few bytes more of private memory and
-// is measured faster.  Tighter budgets spill heavily and are slower; the
-// FP64 and dual instantiations keep the compiler's choice.
+// is measured faster.  Tighter budgets spill heavily and are slower.  The
+// FP64 level physics sits at the edge of three waves (168 VGPRs): held
+// there, so that a few more registers elsewhere in the kernel do not drop
+// it to two.  The dual instantiations keep the compiler's choice.
 template <class R>
 struct WavesPerEu {
   static constexpr int a = 1, c = 1;
@@ -19,6 +21,10 @@ struct WavesPerEu {
 template <>
 struct WavesPerEu<float> {
   static constexpr int a = 4, c = 1;
+};
+template <>
+struct WavesPerEu<double> {
+  static constexpr int a = 3, c = 1;
 };
 }  // namespace tempo
 
