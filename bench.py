@@ -169,7 +169,10 @@ def host_path(ncol=100_000, reps=3):
     """The step on host arrays (tempo_hip_run_host_prec, the Fortran
     binding's path): the aerosol workload's FP64 host arrays copied to the
     device, stepped in FP64 and in FP32, and the updated fields and
-    diagnostics copied back, PCIe transfers included."""
+    diagnostics copied back, PCIe transfers included; with the arrays in
+    pageable memory (staged through pinned buffers by host threads) and in
+    page-locked memory (tempo_hip_host_alloc: moved directly).  bytes_in /
+    bytes_out are the bytes each step moves across the link."""
     import time
     import numpy as np
     from tempo_amd import generator as gen
@@ -178,18 +181,32 @@ def host_path(ncol=100_000, reps=3):
     st, m = gen.generate("aerosol", ncol=ncol)
     nz = st["p"].shape[0]
     out = {"workload": f"aerosol-aware step on host float64 arrays, {ncol} columns x {nz} levels, transfers included"}
-    for name, prec in (("fp64", host.FP64), ("fp32", host.FP32)):
-        work = {k: v.copy() for k, v in st.items()}
-        ctx.run_host(work, m["dt"], m["itimestep"], precision=prec)
-        ts = []
-        for _ in range(reps):
+    written = [k for k in st if k not in ("p", "w", "dz")]
+    nd = len(host.DIAG2_NAMES) / nz + len(host.DIAG3_NAMES)
+    up = 8 * ncol * nz * len(st)
+    down = 8 * ncol * nz * (len(written) + nd)
+    out["bytes_in"], out["bytes_out"] = up, int(down)
+    for memory in ("pageable", "page_locked"):
+        alloc = np.empty if memory == "pageable" else host.pinned_empty
+        work = {k: alloc(v.shape) for k, v in st.items()}
+        diags = {n: alloc(ncol) for n in host.DIAG2_NAMES}
+        diags.update({n: alloc((nz, ncol)) for n in host.DIAG3_NAMES})
+        for name, prec in (("fp64", host.FP64), ("fp32", host.FP32)):
             for k in work:
                 np.copyto(work[k], st[k])
-            t0 = time.perf_counter()
-            ctx.run_host(work, m["dt"], m["itimestep"], precision=prec)
-            ts.append(time.perf_counter() - t0)
-        ms = 1e3 * min(ts)
-        out[name] = {"ms": round(ms, 2), "mcl_per_s": round(ncol * nz / (ms * 1e-3) / 1e6, 1)}
+            ctx.run_host(work, m["dt"], m["itimestep"], precision=prec, diags=diags)
+            ts = []
+            for _ in range(reps):
+                for k in work:
+                    np.copyto(work[k], st[k])
+                t0 = time.perf_counter()
+                ctx.run_host(work, m["dt"], m["itimestep"], precision=prec, diags=diags)
+                ts.append(time.perf_counter() - t0)
+            ms = 1e3 * min(ts)
+            key = name if memory == "pageable" else f"{name}_page_locked"
+            out[key] = {"ms": round(ms, 2), "mcl_per_s": round(ncol * nz / (ms * 1e-3) / 1e6, 1),
+                        "link_gb_per_s": round((up + down) / (ms * 1e-3) / 1e9, 1)}
+        del work, diags
     ctx.close()
     return out
 

@@ -173,12 +173,18 @@ int tempo_hip_branch_signature(tempo_hip_ctx *ctx, const tempo_hip_fields *f, co
 /* As tempo_hip_run with host arrays (FP64, Fortran layout of `dims`): copies
  * to the device, runs and copies back; for hosts that keep state on the CPU
  * (tempo_run, src/module_mp_tempo_driver.F90:187-279, on host memory).
- * land and nlev are host pointers here.  The arrays move through two pinned
- * chunks of the context in turn (host threads copy one while the copy
- * engine moves the other); only the fields the step updates come back, and
- * the diagnostics go in only where the step leaves points of their spans
- * unwritten (padded rows, levels k >= nlev[c]).  Returns when the caller's
- * arrays hold the results. */
+ * land and nlev are host pointers here.  The step runs as a pipeline over
+ * slabs of columns (ranges of j, or of i when nj = 1; TEMPO_HIP_TUNE_HOST_*):
+ * host threads copy a slab's inputs into pinned memory of the context (not
+ * for arrays already in page-locked memory, tempo_hip_host_alloc), a copy
+ * stream moves them up, the slab is stepped, a second copy stream moves its
+ * results down while later slabs go up, and host threads copy them into the
+ * caller's arrays.  Only the fields the step updates come back, and the
+ * diagnostics go in only where the step leaves points of their spans
+ * unwritten (padded rows, levels k >= nlev[c]).  Diagnostic strides must not
+ * overlap (dldk >= ni, dldj >= dldk*nz, d2ldj >= ni where nj > 1; E_ARG).
+ * After a step in more than one slab, tempo_hip_substeps has no record.
+ * Returns when the caller's arrays hold the results. */
 int tempo_hip_run_host(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_diags *d,
                        const tempo_hip_dims *dims, double dt, int itimestep);
 /* The same, stepped in `precision` on the device (TEMPO_HIP_FP32: on FP32
@@ -186,6 +192,15 @@ int tempo_hip_run_host(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const temp
  * writes; halos and padding keep their FP64 values). */
 int tempo_hip_run_host_prec(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_diags *d,
                             const tempo_hip_dims *dims, double dt, int itimestep, int precision);
+
+/* Page-locked host memory (hipHostMalloc), and its release; no counterpart
+ * in the reference.  A host that keeps its state and diagnostic arrays in
+ * such memory lets tempo_hip_run_host(_prec) move them between host and
+ * device directly, without the staging copies through the context's pinned
+ * buffers (each array is checked, so pageable and page-locked arrays can be
+ * mixed).  *ptr is NULL after an error. */
+int tempo_hip_host_alloc(size_t bytes, void **ptr);
+int tempo_hip_host_free(void *ptr);
 
 /* Replaces tempo_aerosol_surface_emissions
  * (src/module_mp_tempo_driver.F90:516-529): nwfa(i, k, j) += nwfa2d(i, j) * dt
@@ -274,7 +289,8 @@ double tempo_hip_table_build_ms(tempo_hip_ctx *ctx);
  * snow steps, snow ktop, ice steps, ice ktop, cloud steps, cloud ktop;
  * n = 10*ncol, or 12*ncol for two more rows: 32-bit hashes of the decisions
  * of the semi-Lagrangian rain and graupel sedimentation (semi_sedi_flag;
- * 0 in flux form), equal in two runs that took the same decisions. */
+ * 0 in flux form), equal in two runs that took the same decisions.  E_ARG
+ * when there is no record: no step yet, or a host-array step in slabs. */
 int tempo_hip_substeps(tempo_hip_ctx *ctx, int *host_out, size_t n);
 
 /* Launch-shape tuning of subsequent runs in one precision (no counterpart in
@@ -300,12 +316,24 @@ int tempo_hip_substeps(tempo_hip_ctx *ctx, int *host_out, size_t n);
  *                               wavefronts, so that they do not share
  *                               wavefronts with idle points; clear = lane i
  *                               takes column i (default 0)
+ *   TEMPO_HIP_TUNE_HOST_SLABS   host-array steps (tempo_hip_run_host,
+ *                               _run_host_prec): slabs of columns of the
+ *                               transfer pipeline, 1..4096 (fewer where the
+ *                               tile has fewer j rows, or fewer than 256
+ *                               columns per slab when nj = 1); 0 = about
+ *                               64 MB of uploads per slab, at most 32
+ *                               (default)
+ *   TEMPO_HIP_TUNE_HOST_THREADS ... and host threads copying between the
+ *                               caller's arrays and pinned memory, 1..64
+ *                               (default 8)
  * Every setting gives identical results. */
 #define TEMPO_HIP_TUNE_SED_WAVE 1
 #define TEMPO_HIP_TUNE_BLOCK_A 2
 #define TEMPO_HIP_TUNE_BLOCK_C 3
 #define TEMPO_HIP_TUNE_ORDER 4
 #define TEMPO_HIP_TUNE_FUSED 5
+#define TEMPO_HIP_TUNE_HOST_SLABS 6
+#define TEMPO_HIP_TUNE_HOST_THREADS 7
 int tempo_hip_tune(tempo_hip_ctx *ctx, int precision, int key, int value);
 
 /* Per-phase timing of subsequent runs (no counterpart in the reference):

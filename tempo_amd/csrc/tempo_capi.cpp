@@ -4,8 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <type_traits>
@@ -160,6 +163,92 @@ T bswap(T v) {
   return r;
 }
 
+// one host copy of the host-array staging
+struct Piece {
+  char* dst;
+  const char* src;
+  size_t n;
+};
+
+// Host threads for the host-array staging: run() splits the bytes of a list
+// of copies evenly over the pool's threads and the calling thread and
+// returns when every copy is done.
+class CopyPool {
+ public:
+  explicit CopyPool(unsigned threads) : nt_(std::max(1u, threads)) {
+    for (unsigned i = 1; i < nt_; ++i) th_.emplace_back([this, i] { work(i); });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> g(m_);
+      stop_ = true;
+    }
+    go_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  CopyPool(const CopyPool&) = delete;
+  CopyPool& operator=(const CopyPool&) = delete;
+  unsigned threads() const { return nt_; }
+  void run(const std::vector<Piece>& p) {
+    size_t total = 0;
+    for (const Piece& q : p) total += q.n;
+    if (nt_ == 1 || total < (size_t(1) << 20)) {
+      share(p, total, 0, 1);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> g(m_);
+      job_ = &p;
+      total_ = total;
+      pending_ = nt_ - 1;
+      ++gen_;
+    }
+    go_.notify_all();
+    share(p, total, 0, nt_);
+    std::unique_lock<std::mutex> l(m_);
+    done_.wait(l, [this] { return pending_ == 0; });
+  }
+
+ private:
+  // thread id of n copies bytes [total*id/n, total*(id+1)/n) of the list
+  static void share(const std::vector<Piece>& p, size_t total, unsigned id, unsigned n) {
+    const size_t a = total * id / n, b = total * (id + 1) / n;
+    size_t at = 0;
+    for (const Piece& q : p) {
+      if (at >= b) break;
+      const size_t lo = std::max(a, at), hi = std::min(b, at + q.n);
+      if (lo < hi) std::memcpy(q.dst + (lo - at), q.src + (lo - at), hi - lo);
+      at += q.n;
+    }
+  }
+  void work(unsigned id) {
+    unsigned seen = 0;
+    for (;;) {
+      const std::vector<Piece>* job;
+      size_t total;
+      {
+        std::unique_lock<std::mutex> l(m_);
+        go_.wait(l, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        job = job_;
+        total = total_;
+      }
+      share(*job, total, id, nt_);
+      std::lock_guard<std::mutex> g(m_);
+      if (--pending_ == 0) done_.notify_one();
+    }
+  }
+  unsigned nt_;
+  std::vector<std::thread> th_;
+  std::mutex m_;
+  std::condition_variable go_, done_;
+  const std::vector<Piece>* job_ = nullptr;
+  size_t total_ = 0;
+  unsigned pending_ = 0, gen_ = 0;
+  bool stop_ = false;
+};
+
 }  // namespace
 
 struct tempo_hip_ctx {
@@ -185,13 +274,19 @@ struct tempo_hip_ctx {
   void* last_ksub = nullptr;  // substep counts of the last run (its scratch)
   void* last_vt = nullptr;    // its wide sedimentation planes
   // host-array staging: device copies of the caller's arrays (FP64, and
-  // their FP32 working copies), two pinned chunks, the host runs' stream
+  // their FP32 working copies); pinned slab buffers, two for uploads and
+  // three for downloads, with their events; the compute, upload and
+  // download streams of host runs; the copy threads
   std::vector<DevBuf> stage, stage32;
   DevBuf stage_land, stage_nlev;
-  void* pin[2] = {nullptr, nullptr};
-  size_t pin_bytes = 0;
-  hipEvent_t pin_ev[2] = {nullptr, nullptr};
-  hipStream_t host_stream = nullptr;
+  void* pin_up[2] = {nullptr, nullptr};
+  void* pin_dn[3] = {nullptr, nullptr, nullptr};
+  size_t pin_up_bytes = 0, pin_dn_bytes = 0;
+  hipEvent_t ev_up[2] = {nullptr, nullptr};
+  hipEvent_t ev_dn[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_step = nullptr;
+  hipStream_t host_stream = nullptr, up_stream = nullptr, down_stream = nullptr;
+  std::unique_ptr<CopyPool> pool;
   double build_ms = 0.0;
   LaunchCfg launch[2];  // [FP32, FP64]
   // per-phase timing: 5 events per profiled run
@@ -199,11 +294,19 @@ struct tempo_hip_ctx {
   std::vector<hipEvent_t> events;
   ~tempo_hip_ctx() {
     for (auto e : events) (void)hipEventDestroy(e);
-    for (int i = 0; i < 2; ++i) {
-      if (pin[i]) (void)hipHostFree(pin[i]);
-      if (pin_ev[i]) (void)hipEventDestroy(pin_ev[i]);
-    }
-    if (host_stream) (void)hipStreamDestroy(host_stream);
+    for (hipStream_t st : {host_stream, up_stream, down_stream})
+      if (st) (void)hipStreamSynchronize(st);
+    for (void* b : pin_up)
+      if (b) (void)hipHostFree(b);
+    for (void* b : pin_dn)
+      if (b) (void)hipHostFree(b);
+    for (hipEvent_t e : ev_up)
+      if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev_dn)
+      if (e) (void)hipEventDestroy(e);
+    if (ev_step) (void)hipEventDestroy(ev_step);
+    for (hipStream_t st : {host_stream, up_stream, down_stream})
+      if (st) (void)hipStreamDestroy(st);
   }
 };
 
@@ -809,92 +912,181 @@ int check_run_args(const tempo_hip_fields* f, const tempo_hip_dims* dm, double d
 }
 
 // ---- host arrays ---------------------------------------------------------
-// A host-array step moves each array through two pinned chunks in turn:
-// host threads copy one chunk between the caller's (pageable) array and
-// pinned memory while the copy engine moves the other across the link, so
-// the transfers run near the pinned rate instead of the pageable one.
+// A host-array step runs as a pipeline over slabs of columns (ranges of j,
+// or of i when nj == 1).  Host threads copy slab s's inputs from the
+// caller's (pageable) arrays into pinned memory and the upload stream moves
+// them to the device copies; the compute stream steps the slab's columns
+// there; the download stream moves the slab's updated fields and
+// diagnostics into pinned memory, from which host threads copy them into
+// the caller's arrays two slabs later.  Uploads and downloads of different
+// slabs overlap (the link is full duplex), the host copies overlap both,
+// and each column's step is the device path's.
 
-constexpr size_t kPinChunk = size_t(64) << 20;
+// default upload bytes per slab, and the most slabs
+constexpr size_t kSlabBytes = size_t(128) << 20;
+constexpr int kMaxSlabs = 32;
 
-// dst <- src, n bytes, split over host threads
-void par_copy(void* dst, const void* src, size_t n) {
-  const size_t kMin = size_t(4) << 20;
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const unsigned nt = unsigned(std::min<size_t>(std::min(8u, hw), (n + kMin - 1) / kMin));
-  if (nt <= 1) {
-    std::memcpy(dst, src, n);
-    return;
-  }
-  std::vector<std::thread> th;
-  const size_t per = (n + nt - 1) / nt;
-  for (unsigned i = 0; i < nt; ++i) {
-    const size_t a = std::min(n, size_t(i) * per), b = std::min(n, a + per);
-    if (b > a)
-      th.emplace_back([=] { std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a); });
-  }
-  for (auto& t : th) t.join();
-}
+// elements of one array in one slab: n rows of len elements at element
+// offset base + r*pitch of the array (packed row after row in pinned memory)
+struct Span {
+  size_t base, len, pitch;
+  int n;
+};
 
-int ensure_pinned(tempo_hip_ctx* x) {
-  if (!x->host_stream) HIP_TRY(hipStreamCreateWithFlags(&x->host_stream, hipStreamNonBlocking));
-  for (int i = 0; i < 2; ++i) {
-    if (!x->pin_ev[i]) HIP_TRY(hipEventCreateWithFlags(&x->pin_ev[i], hipEventDisableTiming));
-    if (!x->pin[i]) HIP_TRY(hipHostMalloc(&x->pin[i], kPinChunk, hipHostMallocDefault));
-  }
-  x->pin_bytes = kPinChunk;
-  return TEMPO_HIP_OK;
-}
+// array kinds: state (ldk, ldj), 3-D diagnostic (dldk, dldj), 2-D
+// diagnostic (d2ldj)
+enum { K_STATE = 0, K_DIAG3 = 1, K_DIAG2 = 2 };
 
 struct Xfer {
   void* host;
   void* dev;
-  size_t n;
+  int kind;
+  bool pinned;  // the caller's array is page-locked: the copy engines move it directly
 };
 
-// host -> device for every transfer, chunk by chunk through the pinned pair
-int h2d_all(tempo_hip_ctx* x, const std::vector<Xfer>& xs) {
-  int b = 0;
-  for (const Xfer& t : xs)
-    for (size_t off = 0; off < t.n; off += x->pin_bytes) {
-      const size_t len = std::min(x->pin_bytes, t.n - off);
-      HIP_TRY(hipEventSynchronize(x->pin_ev[b]));  // this chunk's previous copy has left it
-      par_copy(x->pin[b], static_cast<const char*>(t.host) + off, len);
-      HIP_TRY(hipMemcpyAsync(static_cast<char*>(t.dev) + off, x->pin[b], len, hipMemcpyHostToDevice, x->host_stream));
-      HIP_TRY(hipEventRecord(x->pin_ev[b], x->host_stream));
-      b ^= 1;
+// whether [p, p + bytes) starts and ends in page-locked host memory
+bool host_pinned(const void* p, size_t bytes) {
+  for (const char* q : {static_cast<const char*>(p), static_cast<const char*>(p) + bytes - 1}) {
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, q) != hipSuccess) {
+      (void)hipGetLastError();  // pageable memory: not an error of the step
+      return false;
     }
+    if (at.type != hipMemoryTypeHost) return false;
+  }
+  return true;
+}
+
+struct SlabPlan {
+  bool along_j;  // slabs of j (whole i rows), else of i (nj == 1)
+  int count, width, total;
+  int lo(int s) const { return s * width; }
+  int hi(int s) const { return std::min(total, (s + 1) * width); }
+};
+
+SlabPlan plan_slabs(const tempo_hip_dims* dm, size_t up_bytes, int tuned) {
+  SlabPlan p{dm->nj > 1, 1, 0, 0};
+  p.total = p.along_j ? dm->nj : dm->ni;
+  const int most = p.along_j ? dm->nj : std::max(1, dm->ni / 256);  // i slabs of at least 256 columns
+  int n = tuned > 0 ? tuned : int(std::min<size_t>(kMaxSlabs, (up_bytes + kSlabBytes - 1) / kSlabBytes));
+  n = std::max(1, std::min(n, most));
+  if (n == 1) p.along_j = true;  // one slab: the arrays' whole spans
+  if (p.along_j) p.total = dm->nj;
+  p.width = (p.total + n - 1) / n;
+  p.count = (p.total + p.width - 1) / p.width;
+  return p;
+}
+
+Span slab_span(const tempo_hip_dims* dm, const SlabPlan& p, int s, int kind) {
+  const size_t lo = size_t(p.lo(s)), m = size_t(p.hi(s)) - lo;
+  const size_t ldk = kind == K_STATE ? size_t(dm->ldk) : kind == K_DIAG3 ? size_t(dm->dldk) : 0;
+  const size_t ldj = kind == K_STATE ? size_t(dm->ldj) : kind == K_DIAG3 ? size_t(dm->dldj) : size_t(dm->d2ldj);
+  const size_t nk = kind == K_DIAG2 ? 1 : size_t(dm->nz);
+  if (p.along_j) {
+    const size_t len = (m - 1) * ldj + (nk - 1) * ldk + size_t(dm->ni);
+    return Span{lo * ldj, len, len, 1};
+  }
+  if (nk == 1 || ldk == m) return Span{lo, m * nk, m * nk, 1};
+  return Span{lo, m, ldk, int(nk)};
+}
+
+// the slab's sub-box of an array of the given kind: element offset of its
+// first point, and its geometry for the conversion kernels
+size_t slab_base(const tempo_hip_dims* dm, const SlabPlan& p, int s, int kind) {
+  const size_t lo = size_t(p.lo(s));
+  if (!p.along_j) return lo;
+  return lo * size_t(kind == K_STATE ? dm->ldj : kind == K_DIAG3 ? dm->dldj : dm->d2ldj);
+}
+DualGeom slab_geom(const tempo_hip_dims* dm, const SlabPlan& p, int s, int kind) {
+  const int m = p.hi(s) - p.lo(s);
+  const int ni = p.along_j ? dm->ni : m, nj = p.along_j ? m : 1;
+  if (kind == K_STATE) return DualGeom{ni, nj, dm->nz, dm->ldk, dm->ldj};
+  if (kind == K_DIAG3) return DualGeom{ni, nj, dm->nz, dm->dldk, dm->dldj};
+  return DualGeom{ni, nj, 1, 0, dm->d2ldj};
+}
+
+int ensure_host_staging(tempo_hip_ctx* x, unsigned threads) {
+  for (hipStream_t* st : {&x->host_stream, &x->up_stream, &x->down_stream}) {
+    if (!*st) HIP_TRY(hipStreamCreateWithFlags(st, hipStreamNonBlocking));
+    HIP_TRY(hipStreamSynchronize(*st));  // nothing of an earlier call still uses the pinned buffers
+  }
+  for (hipEvent_t& e : x->ev_up)
+    if (!e) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  for (hipEvent_t& e : x->ev_dn)
+    if (!e) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  if (!x->ev_step) HIP_TRY(hipEventCreateWithFlags(&x->ev_step, hipEventDisableTiming));
+  if (!x->pool || x->pool->threads() != threads) x->pool.reset(new CopyPool(threads));
   return TEMPO_HIP_OK;
 }
 
-// device -> host: the copy of chunk i+1 runs while the host moves chunk i
-int d2h_all(tempo_hip_ctx* x, const std::vector<Xfer>& xs) {
-  std::vector<Xfer> ch;
-  for (const Xfer& t : xs)
-    for (size_t off = 0; off < t.n; off += x->pin_bytes)
-      ch.push_back({static_cast<char*>(t.host) + off, static_cast<char*>(t.dev) + off, std::min(x->pin_bytes, t.n - off)});
-  auto issue = [&](size_t i) -> int {
-    const int b = int(i & 1);
-    HIP_TRY(hipMemcpyAsync(x->pin[b], ch[i].dev, ch[i].n, hipMemcpyDeviceToHost, x->host_stream));
-    HIP_TRY(hipEventRecord(x->pin_ev[b], x->host_stream));
-    return TEMPO_HIP_OK;
-  };
-  int rc;
-  if (!ch.empty() && (rc = issue(0))) return rc;
-  for (size_t i = 0; i < ch.size(); ++i) {
-    if (i + 1 < ch.size() && (rc = issue(i + 1))) return rc;
-    HIP_TRY(hipEventSynchronize(x->pin_ev[i & 1]));
-    par_copy(ch[i].host, x->pin[i & 1], ch[i].n);
+// n pinned buffers of at least `need` bytes each (grown in 16 MB steps)
+int ensure_pinned(void** b, int n, size_t& have, size_t need) {
+  if (need <= have) return TEMPO_HIP_OK;
+  need = (need + (size_t(16) << 20) - 1) / (size_t(16) << 20) * (size_t(16) << 20);
+  for (int i = 0; i < n; ++i) {
+    if (b[i]) HIP_TRY(hipHostFree(b[i]));
+    b[i] = nullptr;
+  }
+  have = 0;
+  for (int i = 0; i < n; ++i) HIP_TRY(hipHostMalloc(&b[i], need, hipHostMallocDefault));
+  have = need;
+  return TEMPO_HIP_OK;
+}
+
+// bytes of the transfers' spans in slab s
+size_t slab_bytes(const tempo_hip_dims* dm, const SlabPlan& p, int s, const std::vector<Xfer>& xs) {
+  size_t n = 0;
+  for (const Xfer& t : xs) {
+    if (t.pinned) continue;
+    const Span sp = slab_span(dm, p, s, t.kind);
+    n += sp.len * size_t(sp.n) * sizeof(double);
+  }
+  return n;
+}
+
+// host copies between the caller's arrays and a slab's pinned buffer
+void slab_host_copy(tempo_hip_ctx* x, const tempo_hip_dims* dm, const SlabPlan& p, int s,
+                    const std::vector<Xfer>& xs, char* pin, bool to_pinned, std::vector<Piece>& pieces) {
+  pieces.clear();
+  size_t off = 0;
+  for (const Xfer& t : xs) {
+    if (t.pinned) continue;
+    const Span sp = slab_span(dm, p, s, t.kind);
+    char* h = static_cast<char*>(t.host);
+    for (int r = 0; r < sp.n; ++r) {
+      char* a = h + (sp.base + size_t(r) * sp.pitch) * sizeof(double);
+      const size_t n = sp.len * sizeof(double);
+      pieces.push_back(to_pinned ? Piece{pin + off, a, n} : Piece{a, pin + off, n});
+      off += n;
+    }
+  }
+  x->pool->run(pieces);
+}
+
+// copy-engine moves between the device copies and a slab's pinned buffer,
+// or the caller's array itself where that is page-locked
+int slab_dma(const tempo_hip_dims* dm, const SlabPlan& p, int s, const std::vector<Xfer>& xs, char* pin, bool up,
+             hipStream_t st) {
+  size_t off = 0;
+  for (const Xfer& t : xs) {
+    const Span sp = slab_span(dm, p, s, t.kind);
+    char* dv = static_cast<char*>(t.dev) + sp.base * sizeof(double);
+    char* hv = t.pinned ? static_cast<char*>(t.host) + sp.base * sizeof(double) : pin + off;
+    const size_t w = sp.len * sizeof(double), dpitch = sp.pitch * sizeof(double), hpitch = t.pinned ? dpitch : w;
+    if (sp.n == 1) {
+      HIP_TRY(up ? hipMemcpyAsync(dv, hv, w, hipMemcpyHostToDevice, st)
+                 : hipMemcpyAsync(hv, dv, w, hipMemcpyDeviceToHost, st));
+    } else {
+      HIP_TRY(up ? hipMemcpy2DAsync(dv, dpitch, hv, hpitch, w, size_t(sp.n), hipMemcpyHostToDevice, st)
+                 : hipMemcpy2DAsync(hv, hpitch, dv, dpitch, w, size_t(sp.n), hipMemcpyDeviceToHost, st));
+    }
+    if (!t.pinned) off += w * size_t(sp.n);
   }
   return TEMPO_HIP_OK;
 }
 
-// element conversion over a whole span, or over the points of a tile (the
-// points a step writes: i < ni, k < nk and, with nlev, k < nlev[i + j*ni])
-template <class S, class D>
-__global__ void __launch_bounds__(256) k_cast(const S* __restrict__ in, D* __restrict__ out, size_t n) {
-  for (size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += size_t(gridDim.x) * blockDim.x)
-    out[e] = D(in[e]);
-}
+// element conversion over the points of a tile (the points a step writes:
+// i < ni, k < nk and, with nlev, k < nlev[i + j*ni])
 template <class S, class D>
 __global__ void __launch_bounds__(256) k_cast_tile(const S* __restrict__ in, D* __restrict__ out, DualGeom g,
                                                    const int* __restrict__ nlev) {
@@ -903,12 +1095,6 @@ __global__ void __launch_bounds__(256) k_cast_tile(const S* __restrict__ in, D* 
   if (nlev && k >= nlev[i + j * g.ni]) return;
   const size_t e = size_t(i) + size_t(k) * size_t(g.ldk) + size_t(j) * size_t(g.ldj);
   out[e] = D(in[e]);
-}
-template <class S, class D>
-hipError_t cast_span(const S* in, D* out, size_t n, hipStream_t s) {
-  const unsigned grid = unsigned(std::min<size_t>((n + 255) / 256, 65536));
-  if (n) hipLaunchKernelGGL((k_cast<S, D>), dim3(grid), dim3(256), 0, s, in, out, n);
-  return hipGetLastError();
 }
 template <class S, class D>
 hipError_t cast_tile(const S* in, D* out, const DualGeom& g, const int* nlev, hipStream_t s) {
@@ -921,15 +1107,16 @@ hipError_t cast_tile(const S* in, D* out, const DualGeom& g, const int* nlev, hi
 // device (the diagnostics only where the step does not write every point of
 // their spans: padded rows, or ragged columns), stepped in `precision` (FP32:
 // on FP32 copies, converted back at the points the step writes) and the
-// updated fields and diagnostics copied back.
+// updated fields and diagnostics copied back, slab by slab.
 int run_host_impl(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_diags* d, const tempo_hip_dims* dm,
                   double dt, int itimestep, int precision) {
   if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
   int rc = check_run_args(f, dm, dt);
   if (rc) return rc;
-  if ((rc = ensure_pinned(x))) return rc;
-  hipStream_t s = x->host_stream;
   const bool fp32 = precision == TEMPO_HIP_FP32;
+  const LaunchCfg& lc = x->launch[fp32 ? 0 : 1];
+  if ((rc = ensure_host_staging(x, unsigned(lc.host_threads)))) return rc;
+  hipStream_t cs = x->host_stream;
   const size_t n3 = size_t(dm->ldj) * (dm->nj - 1) + size_t(dm->ldk) * (dm->nz - 1) + dm->ni;
   const size_t nd3 = size_t(dm->dldj) * (dm->nj - 1) + size_t(dm->dldk) * (dm->nz - 1) + dm->ni;
   const size_t nd2 = size_t(dm->d2ldj) * (dm->nj - 1) + dm->ni;
@@ -941,77 +1128,151 @@ int run_host_impl(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_d
   // numbers and aerosols
   auto written = [&](int i) { return i == 0 || (i == 1 && !f->t) || (i >= 6 && i <= 18); };
   const int nd = sizeof(tempo_hip_diags) / sizeof(void*);
+  void** hd = reinterpret_cast<void**>(const_cast<tempo_hip_diags*>(d));
+  if (d) {
+    bool has3 = false, has2 = false;
+    for (int i = 0; i < nd; ++i)
+      if (hd[i]) (i < D2_N ? has2 : has3) = true;
+    if (has3 && (dm->dldk < dm->ni || (dm->nj > 1 && dm->dldj < dm->dldk * dm->nz)))
+      return fail(TEMPO_HIP_E_ARG, "3-D diagnostic strides overlap (need dldk >= ni, dldj >= dldk*nz)");
+    if (has2 && dm->nj > 1 && dm->d2ldj < dm->ni)
+      return fail(TEMPO_HIP_E_ARG, "2-D diagnostic stride overlaps (need d2ldj >= ni)");
+  }
   if (x->stage.size() < size_t(kNf + nd)) x->stage.resize(kNf + nd);
   if (fp32 && x->stage32.size() < size_t(kNf + nd)) x->stage32.resize(kNf + nd);
-  const size_t w = fp32 ? sizeof(float) : sizeof(double);
-  tempo_hip_fields df = *f;
-  void** dfp[kNf] = {&df.t,  &df.th, &df.pii, &df.p,  &df.w,  &df.dz,   &df.qv,   &df.qc, &df.qr,    &df.qi,    &df.qs,
-                     &df.qg, &df.ni, &df.nr,  &df.nc, &df.nwfa, &df.nifa, &df.ng, &df.qb,
-                     const_cast<void**>(&df.qc_bl), const_cast<void**>(&df.qcfrac_bl)};
   std::vector<Xfer> up, down;
   for (int i = 0; i < kNf; ++i) {
     if (!hf[i]) continue;
     HIP_TRY(x->stage[i].ensure(n3 * sizeof(double)));
-    up.push_back({hf[i], x->stage[i].p, n3 * sizeof(double)});
-    if (written(i)) down.push_back({hf[i], x->stage[i].p, n3 * sizeof(double)});
-    if (fp32) HIP_TRY(x->stage32[i].ensure(n3 * w));
-    *dfp[i] = fp32 ? x->stage32[i].p : x->stage[i].p;
+    const bool pin = host_pinned(hf[i], n3 * sizeof(double));
+    up.push_back({hf[i], x->stage[i].p, K_STATE, pin});
+    if (written(i)) down.push_back({hf[i], x->stage[i].p, K_STATE, pin});
+    if (fp32) HIP_TRY(x->stage32[i].ensure(n3 * sizeof(float)));
   }
   // diagnostics: uploaded where the step leaves points of their spans
   // unwritten (padding between rows, levels k >= nlev[c]), so that those
   // come back unchanged
   const bool dense3 = !dm->nlev && dm->dldk == dm->ni && (dm->nj == 1 || dm->dldj == dm->dldk * dm->nz);
   const bool dense2 = dm->nj == 1 || dm->d2ldj == dm->ni;
-  tempo_hip_diags dd{};
-  void** hd = reinterpret_cast<void**>(const_cast<tempo_hip_diags*>(d));
-  void** ddp = reinterpret_cast<void**>(&dd);
   for (int i = 0; d && i < nd; ++i) {
     if (!hd[i]) continue;
     const bool two = i < D2_N;
     const size_t n = two ? nd2 : nd3;
-    DevBuf& b = x->stage[kNf + i];
-    HIP_TRY(b.ensure(n * sizeof(double)));
-    if (!(two ? dense2 : dense3)) up.push_back({hd[i], b.p, n * sizeof(double)});
-    down.push_back({hd[i], b.p, n * sizeof(double)});
-    if (fp32) HIP_TRY(x->stage32[kNf + i].ensure(n * w));
-    ddp[i] = fp32 ? x->stage32[kNf + i].p : b.p;
+    HIP_TRY(x->stage[kNf + i].ensure(n * sizeof(double)));
+    const bool pin = host_pinned(hd[i], n * sizeof(double));
+    if (!(two ? dense2 : dense3)) up.push_back({hd[i], x->stage[kNf + i].p, two ? K_DIAG2 : K_DIAG3, pin});
+    down.push_back({hd[i], x->stage[kNf + i].p, two ? K_DIAG2 : K_DIAG3, pin});
+    if (fp32) HIP_TRY(x->stage32[kNf + i].ensure(n * sizeof(float)));
   }
-  tempo_hip_dims dmd = *dm;
   const int ncol = dm->ni * dm->nj;
+  const int* land = nullptr;
+  const int* nlev = nullptr;
   if (f->land) {
     HIP_TRY(x->stage_land.ensure(ncol * sizeof(int)));
-    HIP_TRY(hipMemcpyAsync(x->stage_land.p, f->land, ncol * sizeof(int), hipMemcpyHostToDevice, s));
-    df.land = x->stage_land.as<int>();
+    HIP_TRY(hipMemcpyAsync(x->stage_land.p, f->land, ncol * sizeof(int), hipMemcpyHostToDevice, cs));
+    land = x->stage_land.as<int>();
   }
   if (dm->nlev) {
     HIP_TRY(x->stage_nlev.ensure(ncol * sizeof(int)));
-    HIP_TRY(hipMemcpyAsync(x->stage_nlev.p, dm->nlev, ncol * sizeof(int), hipMemcpyHostToDevice, s));
-    dmd.nlev = x->stage_nlev.as<int>();
+    HIP_TRY(hipMemcpyAsync(x->stage_nlev.p, dm->nlev, ncol * sizeof(int), hipMemcpyHostToDevice, cs));
+    nlev = x->stage_nlev.as<int>();
   }
-  HIP_TRY(hipStreamSynchronize(s));  // land and nlev are pageable: their copies finish before the caller's return
-  if ((rc = h2d_all(x, up))) return rc;
-  const DualGeom g3{dm->ni, dm->nj, dm->nz, dm->ldk, dm->ldj};
-  const DualGeom gd3{dm->ni, dm->nj, dm->nz, dm->dldk, dm->dldj};
-  const DualGeom gd2{dm->ni, dm->nj, 1, 0, dm->d2ldj};
-  if (fp32) {
+  HIP_TRY(hipStreamSynchronize(cs));  // land and nlev are pageable: their copies finish before the caller's return
+
+  size_t up_total = 0;  // all uploads, pinned or not: the slabs also pipeline the copy engines' moves
+  for (const Xfer& t : up) up_total += (t.kind == K_STATE ? n3 : t.kind == K_DIAG3 ? nd3 : nd2) * sizeof(double);
+  const SlabPlan plan = plan_slabs(dm, up_total, lc.host_slabs);
+  // slab 0 is the widest
+  if ((rc = ensure_pinned(x->pin_up, 2, x->pin_up_bytes, slab_bytes(dm, plan, 0, up)))) return rc;
+  if ((rc = ensure_pinned(x->pin_dn, 3, x->pin_dn_bytes, slab_bytes(dm, plan, 0, down)))) return rc;
+
+  // the step of slab s on the compute stream, on the device copies
+  auto step = [&](int s) -> int {
+    const int lo = plan.lo(s), m = plan.hi(s) - lo;
+    const size_t col0 = plan.along_j ? size_t(lo) * dm->ni : size_t(lo);
+    const size_t b3 = slab_base(dm, plan, s, K_STATE), bd3 = slab_base(dm, plan, s, K_DIAG3),
+                 bd2 = slab_base(dm, plan, s, K_DIAG2);
+    const DualGeom g3 = slab_geom(dm, plan, s, K_STATE), gd3 = slab_geom(dm, plan, s, K_DIAG3),
+                   gd2 = slab_geom(dm, plan, s, K_DIAG2);
+    const int* nl = nlev ? nlev + col0 : nullptr;
+    tempo_hip_fields df = *f;
+    void** dfp[kNf] = {&df.t,  &df.th, &df.pii, &df.p,  &df.w,  &df.dz,   &df.qv,   &df.qc, &df.qr,    &df.qi,    &df.qs,
+                       &df.qg, &df.ni, &df.nr,  &df.nc, &df.nwfa, &df.nifa, &df.ng, &df.qb,
+                       const_cast<void**>(&df.qc_bl), const_cast<void**>(&df.qcfrac_bl)};
+    for (int i = 0; i < kNf; ++i) {
+      if (!hf[i]) continue;
+      double* s64 = x->stage[i].as<double>() + b3;
+      if (fp32) {
+        float* s32 = x->stage32[i].as<float>() + b3;
+        HIP_TRY(cast_tile(s64, s32, g3, nullptr, cs));
+        *dfp[i] = s32;
+      } else {
+        *dfp[i] = s64;
+      }
+    }
+    df.land = land ? land + col0 : nullptr;
+    tempo_hip_diags dd{};
+    void** ddp = reinterpret_cast<void**>(&dd);
+    for (int i = 0; d && i < nd; ++i) {
+      if (!hd[i]) continue;
+      const bool two = i < D2_N;
+      double* s64 = x->stage[kNf + i].as<double>() + (two ? bd2 : bd3);
+      if (fp32) {
+        float* s32 = x->stage32[kNf + i].as<float>() + (two ? bd2 : bd3);
+        if (!(two ? dense2 : dense3)) HIP_TRY(cast_tile(s64, s32, two ? gd2 : gd3, nullptr, cs));
+        ddp[i] = s32;
+      } else {
+        ddp[i] = s64;
+      }
+    }
+    tempo_hip_dims dms = *dm;
+    (plan.along_j ? dms.nj : dms.ni) = m;
+    dms.nlev = nl;
+    int r = fp32 ? run_impl<float>(x, &df, d ? &dd : nullptr, &dms, dt, itimestep, cs)
+                 : run_impl<double>(x, &df, d ? &dd : nullptr, &dms, dt, itimestep, cs);
+    if (r || !fp32) return r;
     for (int i = 0; i < kNf; ++i)
-      if (hf[i]) HIP_TRY(cast_span(x->stage[i].as<double>(), x->stage32[i].as<float>(), n3, s));
+      if (hf[i] && written(i))
+        HIP_TRY(cast_tile(x->stage32[i].as<float>() + b3, x->stage[i].as<double>() + b3, g3, nl, cs));
     for (int i = 0; d && i < nd; ++i)
-      if (hd[i] && !(i < D2_N ? dense2 : dense3))
-        HIP_TRY(cast_span(x->stage[kNf + i].as<double>(), x->stage32[kNf + i].as<float>(), i < D2_N ? nd2 : nd3, s));
-    rc = run_impl<float>(x, &df, d ? &dd : nullptr, &dmd, dt, itimestep, s);
-    if (rc) return rc;
-    for (int i = 0; i < kNf; ++i)
-      if (hf[i] && written(i)) HIP_TRY(cast_tile(x->stage32[i].as<float>(), x->stage[i].as<double>(), g3, dmd.nlev, s));
-    for (int i = 0; d && i < nd; ++i)
-      if (hd[i])
-        HIP_TRY(i < D2_N ? cast_tile(x->stage32[kNf + i].as<float>(), x->stage[kNf + i].as<double>(), gd2, nullptr, s)
-                         : cast_tile(x->stage32[kNf + i].as<float>(), x->stage[kNf + i].as<double>(), gd3, dmd.nlev, s));
-  } else {
-    rc = run_impl<double>(x, &df, d ? &dd : nullptr, &dmd, dt, itimestep, s);
-    if (rc) return rc;
+      if (hd[i]) {
+        const bool two = i < D2_N;
+        const size_t b = two ? bd2 : bd3;
+        HIP_TRY(cast_tile(x->stage32[kNf + i].as<float>() + b, x->stage[kNf + i].as<double>() + b, two ? gd2 : gd3,
+                          two ? nullptr : nl, cs));
+      }
+    return TEMPO_HIP_OK;
+  };
+
+  std::vector<Piece> pieces;
+  for (int s = 0; s < plan.count + 2; ++s) {
+    if (s < plan.count) {
+      const int b = s & 1;
+      char* pu = static_cast<char*>(x->pin_up[b]);
+      char* pd = static_cast<char*>(x->pin_dn[s % 3]);
+      if (s >= 2) HIP_TRY(hipEventSynchronize(x->ev_up[b]));  // slab s-2's upload has left this buffer
+      slab_host_copy(x, dm, plan, s, up, pu, true, pieces);
+      if ((rc = slab_dma(dm, plan, s, up, pu, true, x->up_stream))) return rc;
+      HIP_TRY(hipEventRecord(x->ev_up[b], x->up_stream));
+      HIP_TRY(hipStreamWaitEvent(cs, x->ev_up[b], 0));
+      if ((rc = step(s))) return rc;
+      HIP_TRY(hipEventRecord(x->ev_step, cs));
+      // the host copied slab s-3 out of this buffer in the previous round
+      HIP_TRY(hipStreamWaitEvent(x->down_stream, x->ev_step, 0));
+      if ((rc = slab_dma(dm, plan, s, down, pd, false, x->down_stream))) return rc;
+      HIP_TRY(hipEventRecord(x->ev_dn[s % 3], x->down_stream));
+    }
+    if (s >= 2) {
+      const int o = s - 2;
+      HIP_TRY(hipEventSynchronize(x->ev_dn[o % 3]));
+      slab_host_copy(x, dm, plan, o, down, static_cast<char*>(x->pin_dn[o % 3]), false, pieces);
+    }
   }
-  return d2h_all(x, down);
+  if (plan.count > 1) {  // the scratch holds the last slab's record only
+    x->last_ncol = 0;
+    x->last_ksub = nullptr;
+  }
+  return TEMPO_HIP_OK;
 }
 
 }  // namespace
@@ -1272,6 +1533,19 @@ int tempo_hip_run_host_prec(tempo_hip_ctx* x, const tempo_hip_fields* f, const t
   return run_host_impl(x, f, d, dm, dt, itimestep, precision);
 }
 
+int tempo_hip_host_alloc(size_t bytes, void** ptr) {
+  if (!ptr) return fail(TEMPO_HIP_E_ARG, "NULL pointer argument");
+  *ptr = nullptr;
+  if (bytes == 0) return fail(TEMPO_HIP_E_ARG, "zero bytes");
+  HIP_TRY(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+  return TEMPO_HIP_OK;
+}
+
+int tempo_hip_host_free(void* ptr) {
+  if (ptr) HIP_TRY(hipHostFree(ptr));
+  return TEMPO_HIP_OK;
+}
+
 int tempo_hip_table_export(tempo_hip_ctx* x, int table, double* out, size_t n) {
   if (!x || !out) return fail(TEMPO_HIP_E_ARG, "NULL argument");
   const DevBuf* b = nullptr;
@@ -1359,6 +1633,14 @@ int tempo_hip_tune(tempo_hip_ctx* x, int precision, int key, int value) {
       if (value != 0 && value != 1) return fail(TEMPO_HIP_E_ARG, "fused must be 0 or 1");
       lc.fused = value;
       return TEMPO_HIP_OK;
+    case TEMPO_HIP_TUNE_HOST_SLABS:
+      if (value < 0 || value > 4096) return fail(TEMPO_HIP_E_ARG, "host slabs must be 0..4096");
+      lc.host_slabs = value;
+      return TEMPO_HIP_OK;
+    case TEMPO_HIP_TUNE_HOST_THREADS:
+      if (value < 1 || value > 64) return fail(TEMPO_HIP_E_ARG, "host threads must be 1..64");
+      lc.host_threads = value;
+      return TEMPO_HIP_OK;
     default:
       return fail(TEMPO_HIP_E_ARG, "unknown tuning key");
   }
@@ -1390,6 +1672,8 @@ int tempo_hip_profile_read(tempo_hip_ctx* x, double* ms, int n, int* runs) {
 
 int tempo_hip_substeps(tempo_hip_ctx* x, int* out, size_t n) {
   if (!x || !out) return fail(TEMPO_HIP_E_ARG, "NULL argument");
+  if (!x->last_ksub)
+    return fail(TEMPO_HIP_E_ARG, "no substep record (no step yet, or the last step ran on host arrays in slabs)");
   if (n != size_t(10) * x->last_ncol && n != size_t(kSubRows) * x->last_ncol)
     return fail(TEMPO_HIP_E_ARG, "need 10*ncol or 12*ncol entries");
   HIP_TRY(hipMemcpy(out, x->last_ksub, n * sizeof(int), hipMemcpyDeviceToHost));

@@ -307,6 +307,8 @@ struct LaunchCfg {
   int order = 0;      // point grouping within blocks: bit 0 phase A, bit 1 final update
   int block_a = 256;  // threads per block of the level-local kernels (64, 128 or 256)
   int block_c = 256;
+  int host_slabs = 0;    // host-array steps: slabs of columns (0: about 64 MB of uploads each)
+  int host_threads = 8;  // ... and host threads copying between the caller's arrays and pinned memory
 };
 
 #define TEMPO_IDX_STATE(a, c, k) (size_t((c) % (a).ni) + size_t(k) * (a).ldk + size_t((c) / (a).ni) * (a).ldj)

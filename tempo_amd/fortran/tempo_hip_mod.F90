@@ -28,6 +28,7 @@ module tempo_hip_mod
   public :: tempo_hip_cfg, tempo_hip_fields, tempo_hip_diags, tempo_hip_dims
   public :: tempo_hip_cfg_default, tempo_hip_init, tempo_hip_run, tempo_hip_run_host, tempo_hip_run_host_prec
   public :: tempo_hip_last_error
+  public :: tempo_hip_host_alloc, tempo_hip_host_free, tempo_alloc_page_locked, tempo_free_page_locked
 
   integer, parameter :: wp = c_double
 
@@ -106,6 +107,15 @@ module tempo_hip_mod
       type(tempo_hip_dims), intent(in) :: dims
       real(c_double), value :: dt
       integer(c_int), value :: itimestep, precision
+    end function
+    integer(c_int) function tempo_hip_host_alloc(bytes, ptr) bind(c, name='tempo_hip_host_alloc')
+      import :: c_int, c_size_t, c_ptr
+      integer(c_size_t), value :: bytes
+      type(c_ptr), intent(out) :: ptr
+    end function
+    integer(c_int) function tempo_hip_host_free(ptr) bind(c, name='tempo_hip_host_free')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ptr
     end function
     type(c_ptr) function tempo_hip_last_error_c() bind(c, name='tempo_hip_last_error')
       import :: c_ptr
@@ -326,6 +336,30 @@ contains
     if (rc /= 0) call fail('tempo_init', rc)
     last_ctx = tempo_cfgs%ctx
   end subroutine
+
+  ! A real(8) array with bounds lb:ub in page-locked host memory
+  ! (tempo_hip_host_alloc; no counterpart in the reference).  tempo_run moves
+  ! state and diagnostic arrays held in such memory between host and device
+  ! without staging copies.  Release with tempo_free_page_locked.
+  subroutine tempo_alloc_page_locked(a, lb, ub)
+    real(wp), pointer, intent(out) :: a(:,:,:)
+    integer, intent(in) :: lb(3), ub(3)
+    real(wp), pointer :: flat(:,:,:)
+    type(c_ptr) :: p
+    integer(c_size_t) :: n
+    n = int(ub(1) - lb(1) + 1, c_size_t) * int(ub(2) - lb(2) + 1, c_size_t) * int(ub(3) - lb(3) + 1, c_size_t)
+    if (tempo_hip_host_alloc(max(n, 1_c_size_t) * 8_c_size_t, p) /= 0) &
+      error stop 'tempo_alloc_page_locked: tempo_hip_host_alloc failed'
+    call c_f_pointer(p, flat, ub - lb + 1)
+    a(lb(1):, lb(2):, lb(3):) => flat
+  end subroutine tempo_alloc_page_locked
+
+  subroutine tempo_free_page_locked(a)
+    real(wp), pointer, intent(inout) :: a(:,:,:)
+    if (.not. associated(a)) return
+    if (tempo_hip_host_free(c_loc(a)) /= 0) error stop 'tempo_free_page_locked: tempo_hip_host_free failed'
+    nullify(a)
+  end subroutine tempo_free_page_locked
 
   subroutine tempo_finalize(tempo_cfgs)
     type(ty_tempo_cfgs), intent(inout) :: tempo_cfgs

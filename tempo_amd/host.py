@@ -27,6 +27,7 @@ OK, E_ARG, E_UNSUPPORTED, E_TABLE, E_HIP, E_NOMEM = 0, -1, -2, -3, -4, -5
 FP32, FP64 = 4, 8
 TABLE_QG, TABLE_QS, TABLE_FZ = 1, 2, 3
 TUNE_SED_WAVE, TUNE_BLOCK_A, TUNE_BLOCK_C, TUNE_ORDER, TUNE_FUSED = 1, 2, 3, 4, 5
+TUNE_HOST_SLABS, TUNE_HOST_THREADS = 6, 7
 TABLE_SIZES = {TABLE_QG: 5 * 16_867_449, TABLE_QS: 12 * 455_877, TABLE_FZ: 4 * 3_388_275 + 2 * 9_157_500}
 
 
@@ -96,6 +97,8 @@ def lib():
                                          ctypes.c_double, ctypes.c_int]
         L.tempo_hip_run_host_prec.argtypes = [vp, ctypes.POINTER(Fields), ctypes.POINTER(Diags), ctypes.POINTER(Dims),
                                               ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.tempo_hip_host_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(vp)]
+        L.tempo_hip_host_free.argtypes = [vp]
         L.tempo_hip_branch_signature.argtypes = [vp, ctypes.POINTER(Fields), ctypes.POINTER(Diags),
                                                  ctypes.POINTER(Dims), ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                                  vp, vp]
@@ -352,9 +355,11 @@ class TempoHip:
                                        ctypes.c_void_p(s)))
         return xbar
 
-    def run_host(self, state, dt, itimestep, nlev=None, land=None, precision=FP64):
+    def run_host(self, state, dt, itimestep, nlev=None, land=None, precision=FP64, diags=None):
         """tempo_run on host float64 numpy arrays (staged through device
-        memory), stepped in ``precision`` (FP64 or FP32)."""
+        memory), stepped in ``precision`` (FP64 or FP32); ``diags``: the
+        diagnostic arrays to fill (name -> float64 array of ncol or
+        (nz, ncol)), else new ones."""
         nz, ncol = state["p"].shape
         f = Fields()
         for name in FIELD_NAMES + BL_NAMES:
@@ -365,9 +370,14 @@ class TempoHip:
         if land is not None:
             land = np.ascontiguousarray(land, dtype=np.int32)
             f.land = land.ctypes.data
-        out = {n: np.zeros(ncol) for n in DIAG2_NAMES}
-        out.update({n: np.zeros((nz, ncol)) for n in DIAG3_NAMES})
-        d = Diags(*[out[n].ctypes.data for n in DIAG2_NAMES + DIAG3_NAMES])
+        if diags is None:
+            out = {n: np.zeros(ncol) for n in DIAG2_NAMES}
+            out.update({n: np.zeros((nz, ncol)) for n in DIAG3_NAMES})
+        else:
+            out = diags
+            for n, a in out.items():
+                assert a.dtype == np.float64 and a.flags.c_contiguous and a.shape == ((ncol,) if n in DIAG2_NAMES else (nz, ncol)), n
+        d = Diags(*[out[n].ctypes.data if n in out else None for n in DIAG2_NAMES + DIAG3_NAMES])
         dims = Dims(ncol, 1, nz, ncol, nz * ncol, ncol, nz * ncol, ncol, None)
         if nlev is not None:
             nlev = np.ascontiguousarray(nlev, dtype=np.int32)
@@ -455,6 +465,31 @@ class TempoHip:
         out = np.empty(rows * ncol, dtype=np.int32)
         _check(lib().tempo_hip_substeps(self._ctx, out.ctypes.data, out.size))
         return out.reshape(rows, ncol)
+
+
+class _PageLocked:
+    """one tempo_hip_host_alloc block, released with its last array"""
+
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        _check(lib().tempo_hip_host_alloc(int(nbytes), ctypes.byref(p)))
+        self.ptr = p.value
+
+    def __del__(self):
+        if self.ptr:
+            lib().tempo_hip_host_free(self.ptr)
+            self.ptr = None
+
+
+def pinned_empty(shape, dtype=np.float64):
+    """an uninitialised numpy array in page-locked host memory
+    (tempo_hip_host_alloc): run_host moves such arrays without staging copies"""
+    dt = np.dtype(dtype)
+    n = int(np.prod(shape)) * dt.itemsize
+    owner = _PageLocked(max(n, 1))
+    buf = (ctypes.c_char * max(n, 1)).from_address(owner.ptr)
+    buf._owner = owner  # the array keeps buf, buf keeps the block
+    return np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
 
 
 # ---- reference-named wrappers ------------------------------------------------

@@ -2,7 +2,10 @@
 ! host model drives the reference: tempo_init, then tempo_run on a tile of
 ! arrays with a one-point halo in i and k and two j rows.
 !
-!   run_case <input.bin> <output.bin> <ccn_activate.bin>
+!   run_case <input.bin> <output.bin> <ccn_activate.bin> [locked]
+!
+! With "locked" the state arrays live in page-locked host memory
+! (tempo_hip_host_alloc), which tempo_run moves without staging copies.
 !
 ! input.bin (native endian): int32 ni, nj, nz, itimestep; float64 dt; then
 ! 17 float64 fields t p w dz qv qc qr qi qs qg ni nr nc nwfa nifa ng qb, each
@@ -19,7 +22,10 @@ program run_case
   integer(c_int) :: hdr(4)
   real(c_double) :: dt
   integer :: ni, nj, nz, its, f, u, ios
-  real(c_double), allocatable, target :: st(:,:,:,:)
+  real(c_double), pointer :: st(:,:,:,:), st_flat(:,:,:,:), probe(:,:,:)
+  type(c_ptr) :: blk
+  character(len=16) :: mode
+  logical :: locked
   integer, allocatable :: land(:,:)
   real(c_double), allocatable :: buf(:,:,:)
   type(ty_tempo_cfgs) :: cfg
@@ -32,11 +38,28 @@ program run_case
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
   call get_command_argument(3, fccn)
+  mode = ''
+  if (command_argument_count() >= 4) call get_command_argument(4, mode)
+  locked = trim(mode) == 'locked'
   open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old', iostat=ios)
   if (ios /= 0) error stop 2
   read(u) hdr, dt
   ni = hdr(1); nj = hdr(2); nz = hdr(3)
-  allocate(st(0:ni + 1, 0:nz + 1, 1:nj, nf), buf(ni, nz, nj), land(0:ni + 1, 1:nj))
+  if (locked) then
+    if (tempo_hip_host_alloc(int(ni + 2, c_size_t) * (nz + 2) * nj * nf * 8, blk) /= 0) error stop 3
+    call c_f_pointer(blk, st_flat, [ni + 2, nz + 2, nj, nf])
+    st(0:, 0:, 1:, 1:) => st_flat
+    ! the 3-D helper: bounds as asked, usable, released
+    call tempo_alloc_page_locked(probe, [0, 0, 1], [ni + 1, nz + 1, nj])
+    if (any(lbound(probe) /= [0, 0, 1]) .or. any(ubound(probe) /= [ni + 1, nz + 1, nj])) error stop 4
+    probe = 1.0_c_double
+    if (sum(probe) /= real((ni + 2) * (nz + 2) * nj, c_double)) error stop 5
+    call tempo_free_page_locked(probe)
+    if (associated(probe)) error stop 6
+  else
+    allocate(st(0:ni + 1, 0:nz + 1, 1:nj, nf))
+  end if
+  allocate(buf(ni, nz, nj), land(0:ni + 1, 1:nj))
   st = sentinel
   do f = 1, nf
     read(u) buf
@@ -80,4 +103,9 @@ program run_case
   write(u) merge(1_c_int, 0_c_int, halo_ok)
   close(u)
   call tempo_finalize(cfg)
+  if (locked) then
+    if (tempo_hip_host_free(blk) /= 0) error stop 7
+  else
+    deallocate(st)
+  end if
 end program run_case

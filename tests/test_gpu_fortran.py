@@ -20,7 +20,11 @@ D2 = ("rain_precip", "ice_liquid_equiv_precip", "snow_liquid_equiv_precip", "gra
 D3 = ("refl10cm", "re_cloud", "re_ice", "re_snow")
 
 
-def test_fortran_binding_matches_c_abi(tmp_path):
+@pytest.mark.parametrize("memory", ["pageable", "locked"])
+def test_fortran_binding_matches_c_abi(tmp_path, memory):
+    """pageable: the state in allocatable arrays; locked: in page-locked
+    memory from tempo_hip_host_alloc, moved without staging copies (and the
+    3-D helper tempo_alloc_page_locked checked for its bounds)"""
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     assert os.path.exists(RUN_CASE), "Fortran driver not built (make -C tempo_amd/fortran)"
@@ -39,8 +43,8 @@ def test_fortran_binding_matches_c_abi(tmp_path):
         for k in FIELDS:
             fort(st[k]).tofile(f)
         land.tofile(f)
-    print("run_case", ni, nj, nz, flush=True)
-    r = subprocess.run([RUN_CASE, str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), host.CCN_PATH],
+    print("run_case", ni, nj, nz, memory, flush=True)
+    r = subprocess.run([RUN_CASE, str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), host.CCN_PATH, memory],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     raw = np.fromfile(tmp_path / "out.bin", dtype=np.uint8)

@@ -767,37 +767,60 @@ def _host_call(ctx, hip, st, m, diag, dims, nlev=None, precision=None):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_gpu_host_path_padded_diagnostics_and_ragged_columns(hip, prec):
-    """tempo_hip_run_host(_prec) with diagnostics whose rows are padded
-    (d2ldj, dldk > ni, as a Fortran host's halo'd tile) and ragged columns
-    (nlev): tile points equal the device path's, and every point the step
-    does not write -- padding between rows, levels k >= nlev[c] -- keeps the
-    caller's value (in FP32 too: those points are not converted)"""
+@pytest.mark.parametrize("layout", ["j_one_slab", "j_slabs", "i_slabs", "j_slabs_locked", "i_slabs_locked"])
+def test_gpu_host_path_padded_diagnostics_and_ragged_columns(hip, prec, layout):
+    """tempo_hip_run_host(_prec) with padded rows (state ldk > ni; diagnostics
+    dldk, d2ldj > ni, as a Fortran host's halo'd tile) and ragged columns
+    (nlev), stepped in one slab, in slabs of j rows (32 x 3 tile) and in
+    slabs of i ranges (1024 x 1 tile), with pageable arrays and with the
+    state in page-locked memory (moved by the copy engines directly; the
+    diagnostics pageable, or also page-locked in the i-slab case): tile
+    points equal the device path's, and every point the step does not write
+    -- padding between rows, levels k >= nlev[c] -- keeps the caller's value
+    (in FP32 too: those points are not converted)"""
     from tempo_amd import generator as gen
-    st, m = gen.generate("ragged", ncol=96, col0=123)
+    ni, nj, spad, slabs = dict(j_one_slab=(32, 3, 0, 1), j_slabs=(32, 3, 2, 3), i_slabs=(1024, 1, 3, 3),
+                               j_slabs_locked=(32, 3, 2, 3), i_slabs_locked=(1024, 1, 3, 3))[layout]
+    locked = layout.endswith("locked")
+
+    def alloc(shape):
+        return hip.pinned_empty(shape) if locked else np.empty(shape)
+    st, m = gen.generate("ragged", ncol=ni * nj, col0=123)
     if prec == "fp32":
         st = {k: v.astype(np.float32).astype(np.float64) for k, v in st.items()}
     nz, ncol = st["p"].shape
     nlev = m["nlev"]
     ctx = context(hip, dict(aerosolaware=False, hailaware=True))
     dev, dd = gpu_run(ctx, st, m["dt"], m["itimestep"], nlev=nlev, dtype=torch.float64 if prec == "fp64" else torch.float32)
-    ni, nj, pad = 32, 3, 5
-    assert ni * nj == ncol
+    pad = 5
     sentinel = -7.25
-    # state in the Fortran (i, k, j) layout of an (ni, nz, nj) tile, column
-    # c = i + j*ni; diagnostics with rows padded by `pad`
+    # state in the Fortran (i, k, j) layout of an (ni + spad, nz, nj) array,
+    # column c = i + j*ni; diagnostics with rows padded by `pad`
+    ldk = ni + spad
     def fort(a):
-        return np.ascontiguousarray(a.reshape(nz, nj, ni).transpose(1, 0, 2))
+        out = alloc((nj, nz, ldk))
+        out[...] = sentinel
+        out[:, :, :ni] = a.reshape(nz, nj, ni).transpose(1, 0, 2)
+        return out
     hst = {k: fort(v) for k, v in st.items()}
+    dalloc = alloc if layout == "i_slabs_locked" else np.empty
     ld2 = ni + pad
-    d2 = {n: np.full(ld2 * nj, sentinel) for n in hip.DIAG2_NAMES}
+    d2 = {n: dalloc(ld2 * nj) for n in hip.DIAG2_NAMES}
     ldk3 = ni + pad
-    d3 = {n: np.full(ldk3 * nz * nj, sentinel) for n in hip.DIAG3_NAMES}
+    d3 = {n: dalloc(ldk3 * nz * nj) for n in hip.DIAG3_NAMES}
+    for a in list(d2.values()) + list(d3.values()):
+        a[...] = sentinel
     nl = np.ascontiguousarray(nlev, dtype=np.int32)
-    dims = hip.Dims(ni, nj, nz, ni, ni * nz, ldk3, ldk3 * nz, ld2, nl.ctypes.data)
-    _host_call(ctx, hip, hst, m, dict(d2, **d3), dims, precision=None if prec == "fp64" else hip.FP32)
+    dims = hip.Dims(ni, nj, nz, ldk, ldk * nz, ldk3, ldk3 * nz, ld2, nl.ctypes.data)
+    p = hip.FP64 if prec == "fp64" else hip.FP32
+    ctx.tune(p, hip.TUNE_HOST_SLABS, slabs)
+    try:
+        _host_call(ctx, hip, hst, m, dict(d2, **d3), dims, precision=None if prec == "fp64" else hip.FP32)
+    finally:
+        ctx.tune(p, hip.TUNE_HOST_SLABS, 0)
     for k in st:
-        np.testing.assert_array_equal(hst[k].transpose(1, 0, 2).reshape(nz, ncol), dev[k], err_msg=k)
+        np.testing.assert_array_equal(hst[k][:, :, :ni].transpose(1, 0, 2).reshape(nz, ncol), dev[k], err_msg=k)
+        assert (hst[k][:, :, ni:] == sentinel).all(), k
     i = np.arange(ncol) % ni
     j = np.arange(ncol) // ni
     for n in hip.DIAG2_NAMES:
@@ -807,10 +830,13 @@ def test_gpu_host_path_padded_diagnostics_and_ragged_columns(hip, prec):
     for n in hip.DIAG3_NAMES:
         got = d3[n].reshape(nj, nz, ldk3)
         assert (got[:, :, ni:] == sentinel).all(), n
-        for c in range(ncol):
-            col = got[j[c], :, i[c]]
-            np.testing.assert_array_equal(col[:nlev[c]], dd[n][:nlev[c], c], err_msg=n)
-            assert (col[nlev[c]:] == sentinel).all(), (n, c)
+        cols = got.transpose(1, 0, 2)[:, :, :ni].reshape(nz, ncol)
+        above = np.arange(nz)[:, None] >= np.asarray(nlev)[None, :]
+        np.testing.assert_array_equal(np.where(above, 0.0, cols), np.where(above, 0.0, dd[n]), err_msg=n)
+        assert (cols[above] == sentinel).all(), n
+    if slabs > 1:  # no substep record after a step in slabs
+        with pytest.raises(hip.TempoError, match="no substep record"):
+            ctx.substeps(ncol)
 
 
 def test_gpu_host_entry_points_share_one_context(hip):
@@ -1006,7 +1032,7 @@ def test_gpu_fused_step_equals_four_launches(hip, workload, dtype):
 def test_gpu_tune_rejects_bad_values(hip):
     ctx = context(hip, dict(aerosolaware=False, hailaware=False))
     for key, val in ((hip.TUNE_SED_WAVE, 3), (hip.TUNE_BLOCK_A, 96), (hip.TUNE_BLOCK_C, 512), (hip.TUNE_ORDER, 4),
-                     (99, 1)):
+                     (hip.TUNE_HOST_SLABS, -1), (hip.TUNE_HOST_THREADS, 0), (hip.TUNE_HOST_THREADS, 65), (99, 1)):
         with pytest.raises(hip.TempoError) as e:
             ctx.tune(hip.FP32, key, val)
         assert e.value.code == hip.E_ARG
