@@ -8,15 +8,19 @@ Arrays are float64, shape (nz, ncol), C order: element (k, i) at k*ncol + i,
 i.e. the Fortran (i, k) layout of the reference driver
 (src/module_mp_tempo_driver.F90:207-241) with k = 0 at the surface.
 
-Workloads (k = 1 is the surface, dz uniform):
+Workloads (k = 1 is the surface; dz uniform unless stated):
 
   plain      1,024 x 60,  dz 250 m, dt 30 s, aerosol- and hail-aware off
   aerosol    100,000 x 90, dz 200 m, dt 20 s, aerosol-aware (nc, nwfa, nifa)
-  ragged     131,072 columns of U{30..128} active levels, dz 150 m, dt 20 s,
-             hail-aware (ng, qb), 25 % convective columns
+  ragged     131,072 columns of U{30..128} active levels, dt 20 s,
+             hail-aware (ng, qb), 25 % convective columns; layers stretched
+             from 25 m at the surface by 6 % per level up to 200 m (a
+             convection-permitting grid: rain and hail falling through the
+             thin low layers need 1-20 sedimentation substeps per step)
   adjoint    16,384 x 64, dz 250 m, dt 20 s, aerosol-aware
-  multistep  262,144 x 90, dz 200 m, dt 20 s, aerosol- and hail-aware, ragged
-             generator at 90 levels
+  multistep  262,144 x 90, dz 200 m, dt 20 s, aerosol- and hail-aware, 25 %
+             convective columns as in the ragged workload, every level
+             active
   cold       4,096 x 60, dz 250 m, dt 20 s, aerosol- and hail-aware; surface
              temperature U[258, 272.5] K per column and the first cloud layer
              resting on the surface, so that rain, snow, ice and graupel reach
@@ -38,8 +42,8 @@ WORKLOADS = {
                   ragged=False, itimestep=2),
     "aerosol": dict(ncol=100_000, nz=90, dz=200.0, dt=20.0, aerosol=True, hail=False, seed=1002,
                     ragged=False, itimestep=2),
-    "ragged": dict(ncol=131_072, nz=128, dz=150.0, dt=20.0, aerosol=False, hail=True, seed=1003,
-                   ragged=True, itimestep=2, convective=0.25),
+    "ragged": dict(ncol=131_072, nz=128, dz=200.0, dt=20.0, aerosol=False, hail=True, seed=1003,
+                   ragged=True, itimestep=2, convective=0.25, dz_sfc=25.0, dz_growth=1.06),
     "adjoint": dict(ncol=16_384, nz=64, dz=250.0, dt=20.0, aerosol=True, hail=False, seed=1004,
                     ragged=False, itimestep=2),
     "multistep": dict(ncol=262_144, nz=90, dz=200.0, dt=20.0, aerosol=True, hail=True, seed=1005,
@@ -47,6 +51,18 @@ WORKLOADS = {
     "cold": dict(ncol=4096, nz=60, dz=250.0, dt=20.0, aerosol=True, hail=True, seed=1006,
                  ragged=False, itimestep=2, t_sfc=(258.0, 272.5), surface_cloud=True),
 }
+
+
+def levels(cfg):
+    """(layer depths dz, mid-level heights z) of a workload [m], float64 (nz,):
+    uniform dz, or stretched from dz_sfc by dz_growth per level up to dz"""
+    nz, dz = cfg["nz"], cfg["dz"]
+    if "dz_sfc" in cfg:
+        d = np.minimum(dz, cfg["dz_sfc"] * cfg["dz_growth"] ** np.arange(nz, dtype=np.float64))
+    else:
+        d = np.full(nz, dz)
+    z = np.cumsum(d) - 0.5 * d
+    return d, z
 
 
 def _splitmix(x):
@@ -97,13 +113,13 @@ def generate(workload, ncol=None, col0=0, seed=None):
     cfg = dict(WORKLOADS[workload])
     seed = cfg["seed"] if seed is None else seed
     n = cfg["ncol"] if ncol is None else ncol
-    nz, dz = cfg["nz"], cfg["dz"]
+    nz = cfg["nz"]
     cols = np.arange(col0, col0 + n, dtype=np.uint64)[None, :]
     lev = np.arange(nz, dtype=np.uint64)[:, None]
     col1 = cols[0]
     zero = np.zeros(1, dtype=np.uint64)
 
-    z = (np.arange(nz, dtype=np.float64) + 0.5) * dz                     # level mid-points [m]
+    dzl, z = levels(cfg)                                                  # layer depths, mid-points [m]
     zc = np.repeat(z[:, None], n, axis=1)
     g, rd = 9.8, 287.04
     zi = np.concatenate([[0.0], z])
@@ -144,7 +160,7 @@ def generate(workload, ncol=None, col0=0, seed=None):
     def number(q, rho_x, d):
         return q / (np.pi / 6.0 * rho_x * d ** 3)
     w = 0.5 * normal(seed, cols, lev, "w1", "w2")
-    dzf = np.full((nz, n), dz)
+    dzf = np.repeat(dzl[:, None], n, axis=1)
     state = dict(t=t, p=p, w=w, dz=dzf, qv=qv, qc=hyd["qc"], qr=hyd["qr"], qi=hyd["qi"],
                  qs=hyd["qs"], qg=hyd["qg"],
                  ni=number(hyd["qi"], 890.0, 50e-6), nr=number(hyd["qr"], 1000.0, 1e-3))
@@ -226,17 +242,18 @@ def generate_torch(workload, ncol=None, col0=0, seed=None, device="cuda", dtype=
     cfg = dict(WORKLOADS[workload])
     seed = cfg["seed"] if seed is None else seed
     n = cfg["ncol"] if ncol is None else ncol
-    nz, dz = cfg["nz"], cfg["dz"]
+    nz = cfg["nz"]
     f64 = dict(dtype=torch.float64, device=device)
     cols = torch.arange(col0, col0 + n, dtype=torch.int64, device=device)[None, :]
     lev = torch.arange(nz, dtype=torch.int64, device=device)[:, None]
     col1 = cols[0]
     zero = torch.zeros(1, dtype=torch.int64, device=device)
 
-    z = (torch.arange(nz, **f64) + 0.5) * dz
+    dzl, zl = levels(cfg)
+    z = torch.tensor(zl, **f64)
     zc = z[:, None].expand(nz, n)
     g, rd = 9.8, 287.04
-    zi = np.concatenate([[0.0], (np.arange(nz, dtype=np.float64) + 0.5) * dz])
+    zi = np.concatenate([[0.0], zl])
     if "t_sfc" in cfg:
         lo, hi = cfg["t_sfc"]
         tsfc = lo + (hi - lo) * uniform_t(seed, col1, zero, "tsfc")
@@ -282,7 +299,7 @@ def generate_torch(workload, ncol=None, col0=0, seed=None, device="cuda", dtype=
     def number(q, rho_x, d):
         return q / (np.pi / 6.0 * rho_x * d ** 3)
     w = 0.5 * normal_t(seed, cols, lev, "w1", "w2")
-    state = dict(t=t, p=p, w=w, dz=torch.full((nz, n), dz, **f64), qv=qv, qc=hyd["qc"], qr=hyd["qr"],
+    state = dict(t=t, p=p, w=w, dz=torch.tensor(dzl, **f64)[:, None].expand(nz, n), qv=qv, qc=hyd["qc"], qr=hyd["qr"],
                  qi=hyd["qi"], qs=hyd["qs"], qg=hyd["qg"],
                  ni=number(hyd["qi"], 890.0, 50e-6), nr=number(hyd["qr"], 1000.0, 1e-3))
 
