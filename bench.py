@@ -257,7 +257,8 @@ def ragged_graph(steps_per_graph=10, replays=3, tune=True):
     return {"workload": f"{ncol} columns of 30-128 active levels ({levels} column-levels), hail-aware, "
                         f"25 % convective, {steps_per_graph} steps per HIP graph, fp32",
             "ms_per_graph": round(ms, 3), "mcl_per_s": round(levels * steps_per_graph / (ms * 1e-3) / 1e6, 2),
-            "tuned": None if best is None else {k: best[k] for k in ("block_a", "block_c", "order", "sed_wave")}}
+            "tuned": None if best is None else {k: best[k] for k in ("block_a", "block_c", "order", "sed_wave",
+                                                                     "waves_a", "ccn_lds", "phase_a_ms")}}
 
 
 def valu_rates(kernel, precision, kernel_ms):

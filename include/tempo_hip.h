@@ -316,6 +316,14 @@ int tempo_hip_substeps(tempo_hip_ctx *ctx, int *host_out, size_t n);
  *                               wavefronts, so that they do not share
  *                               wavefronts with idle points; clear = lane i
  *                               takes column i (default 0)
+ *   TEMPO_HIP_TUNE_WAVES_A   1: the level-local physics compiled for one
+ *                               wave per SIMD fewer (more registers, fewer
+ *                               spills: 3 instead of 4 in FP32, 2 instead
+ *                               of 3 in FP64), 0: the default
+ *   TEMPO_HIP_TUNE_CCN_LDS   1: each block of the level-local physics
+ *                               copies the CCN activation table's slice
+ *                               (441 floats) into LDS and reads it there,
+ *                               0: read from global memory (default)
  *   TEMPO_HIP_TUNE_HOST_SLABS   host-array steps (tempo_hip_run_host,
  *                               _run_host_prec): slabs of columns of the
  *                               transfer pipeline, 1..4096 (fewer where the
@@ -334,6 +342,8 @@ int tempo_hip_substeps(tempo_hip_ctx *ctx, int *host_out, size_t n);
 #define TEMPO_HIP_TUNE_FUSED 5
 #define TEMPO_HIP_TUNE_HOST_SLABS 6
 #define TEMPO_HIP_TUNE_HOST_THREADS 7
+#define TEMPO_HIP_TUNE_WAVES_A 8
+#define TEMPO_HIP_TUNE_CCN_LDS 9
 int tempo_hip_tune(tempo_hip_ctx *ctx, int precision, int key, int value);
 
 /* Per-phase timing of subsequent runs (no counterpart in the reference):

@@ -9,7 +9,10 @@ choice per phase:
     each block groups its points with microphysics to do into its first
     wavefronts;
   * sedimentation: one wavefront per column with LDS-staged column groups,
-    or one lane per column.
+    or one lane per column;
+  * then, at the chosen blocks and point order, phase A's register budget
+    (compiled for its default waves per SIMD or one fewer) and whether its
+    blocks stage the CCN activation table's slice in LDS.
 
 Every candidate computes identical results (the kernels do the same
 arithmetic per point), so the choice affects time only.  Results are cached
@@ -23,6 +26,8 @@ from . import host
 BLOCKS = (64, 128, 256)
 SED = (1, 0)
 ORDERS = (0, 3)  # point grouping off / on in both level-local kernels
+WAVES_A = (0, 1)  # phase A at its default waves per SIMD / one fewer
+CCN_LDS = (0, 1)  # CCN table slice read from global memory / staged in LDS
 
 _cache = {}
 
@@ -50,7 +55,8 @@ def _phase_ms(ctx, state, dt, itimestep, nlev, reps):
 
 def autotune(ctx, state, dt, itimestep, nlev=None, reps=3, verbose=False):
     """Pick and apply the fastest launch shapes for ``state``'s shape and
-    precision; returns {"block_a": .., "block_c": .., "order": .., "sed_wave": .., "ms": {...}}."""
+    precision; returns {"block_a": .., "block_c": .., "order": .., "sed_wave": .., "waves_a": ..,
+    "ccn_lds": .., "ms": {...}, "phase_a_ms": {...}}."""
     import torch
     nz, ncol = state["p"].shape
     precision = host.FP32 if state["p"].dtype == torch.float32 else host.FP64
@@ -75,11 +81,28 @@ def autotune(ctx, state, dt, itimestep, nlev=None, reps=3, verbose=False):
         block_c, oc = min(((k[0], k[1] & 2) for k in timings),
                           key=lambda p: min(t[3] for k, t in timings.items() if (k[0], k[1] & 2) == p))
         best_s = min(SED, key=lambda s: min(t[2] for k, t in timings.items() if k[2] == s))
-        _cache[key] = dict(block_a=block_a, block_c=block_c, order=oa | oc, sed_wave=best_s,
-                           ms={f"{k[0]}/{k[1]}/{k[2]}": [round(float(x), 4) for x in v] for k, v in timings.items()})
+        # phase A's register budget and CCN staging at the chosen shape
+        ctx.tune(precision, host.TUNE_BLOCK_A, block_a)
+        ctx.tune(precision, host.TUNE_BLOCK_C, block_c)
+        ctx.tune(precision, host.TUNE_ORDER, oa | oc)
+        ctx.tune(precision, host.TUNE_SED_WAVE, best_s)
+        phase_a = {}
+        for w, cl in itertools.product(WAVES_A, CCN_LDS):
+            ctx.tune(precision, host.TUNE_WAVES_A, w)
+            ctx.tune(precision, host.TUNE_CCN_LDS, cl)
+            phase_a[(w, cl)] = float(_phase_ms(ctx, state, dt, itimestep, nlev, reps)[1])
+            if verbose:
+                print("autotune phase A", (w, cl), round(phase_a[(w, cl)], 4))
+        waves_a, ccn_lds = min(phase_a, key=phase_a.get)
+        _cache[key] = dict(block_a=block_a, block_c=block_c, order=oa | oc, sed_wave=best_s, waves_a=waves_a,
+                           ccn_lds=ccn_lds,
+                           ms={f"{k[0]}/{k[1]}/{k[2]}": [round(float(x), 4) for x in v] for k, v in timings.items()},
+                           phase_a_ms={f"waves{k[0]}/ccn{k[1]}": round(v, 4) for k, v in phase_a.items()})
     best = _cache[key]
     ctx.tune(precision, host.TUNE_BLOCK_A, best["block_a"])
     ctx.tune(precision, host.TUNE_BLOCK_C, best["block_c"])
     ctx.tune(precision, host.TUNE_ORDER, best["order"])
     ctx.tune(precision, host.TUNE_SED_WAVE, best["sed_wave"])
+    ctx.tune(precision, host.TUNE_WAVES_A, best["waves_a"])
+    ctx.tune(precision, host.TUNE_CCN_LDS, best["ccn_lds"])
     return best

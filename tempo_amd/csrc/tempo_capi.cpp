@@ -574,6 +574,7 @@ int bind_common(tempo_hip_ctx* x, StepArgs<R>& a, double dt, int itimestep, cons
   x->last_vt = sc->vt;
   const bool r64 = sizeof(S) == 8;
   a.compact = x->launch[r64 ? 1 : 0].order;
+  a.ccn_lds = x->launch[r64 ? 1 : 0].ccn_lds;
   a.tab.qg = static_cast<const S*>(r64 ? x->qg64.p : x->qg32.p);
   a.tab.qs = static_cast<const S*>(r64 ? x->qs64.p : x->qs32.p);
   a.tab.fzr = static_cast<const S*>(r64 ? x->fzr64.p : x->fzr32.p);
@@ -1632,6 +1633,14 @@ int tempo_hip_tune(tempo_hip_ctx* x, int precision, int key, int value) {
     case TEMPO_HIP_TUNE_FUSED:
       if (value != 0 && value != 1) return fail(TEMPO_HIP_E_ARG, "fused must be 0 or 1");
       lc.fused = value;
+      return TEMPO_HIP_OK;
+    case TEMPO_HIP_TUNE_WAVES_A:
+      if (value != 0 && value != 1) return fail(TEMPO_HIP_E_ARG, "waves_a must be 0 or 1");
+      lc.waves_a = value;
+      return TEMPO_HIP_OK;
+    case TEMPO_HIP_TUNE_CCN_LDS:
+      if (value != 0 && value != 1) return fail(TEMPO_HIP_E_ARG, "ccn_lds must be 0 or 1");
+      lc.ccn_lds = value;
       return TEMPO_HIP_OK;
     case TEMPO_HIP_TUNE_HOST_SLABS:
       if (value < 0 || value > 4096) return fail(TEMPO_HIP_E_ARG, "host slabs must be 0..4096");

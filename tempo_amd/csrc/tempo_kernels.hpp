@@ -13,26 +13,25 @@ namespace tempo {
 // is measured faster.  Tighter budgets spill heavily and are slower.  The
 // FP64 level physics sits at the edge of three waves (168 VGPRs): held
 // there, so that a few more registers elsewhere in the kernel do not drop
-// it to two.  The dual instantiations keep the compiler's choice.
+// it to two.  The dual instantiations keep the compiler's choice.  One wave
+// fewer (a_alt: 168 VGPRs in FP32, 256 in FP64) is a tuning option of
+// phase A (TEMPO_HIP_TUNE_WAVES_A).
 template <class R>
 struct WavesPerEu {
-  static constexpr int a = 1, c = 1;
+  static constexpr int a = 1, a_alt = 1, c = 1;
 };
 template <>
 struct WavesPerEu<float> {
-  static constexpr int a = 4, c = 1;
+  static constexpr int a = 4, a_alt = 3, c = 1;
 };
 template <>
 struct WavesPerEu<double> {
-  static constexpr int a = 3, c = 1;
+  static constexpr int a = 3, a_alt = 2, c = 1;
 };
 }  // namespace tempo
 
-// register-budget attributes of the level-local kernels (occupancy against
-// spills); a build may override them
-#ifndef TEMPO_ATTR_A
-#define TEMPO_ATTR_A __attribute__((amdgpu_waves_per_eu(WavesPerEu<R>::a)))
-#endif
+// register-budget attributes of the final update (occupancy against
+// spills); a build may override it
 #ifndef TEMPO_ATTR_C
 #define TEMPO_ATTR_C __attribute__((amdgpu_waves_per_eu(WavesPerEu<R>::c)))
 #endif
@@ -480,8 +479,17 @@ __device__ __forceinline__ void phase_a_point(const StepArgs<R>& a, const Scr<R>
   scr.pf[s] = pf;
 }
 
-template <class R>
-__global__ void __launch_bounds__(256) TEMPO_ATTR_A k_phase_a(StepArgs<R> a) {
+// kWaves: waves per SIMD the kernel is compiled for.  With a.ccn_lds the
+// block first copies the CCN activation table's slice (kCcnSlice floats)
+// into LDS and the activation reads it there.
+template <class R, int kWaves = WavesPerEu<R>::a>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves))) k_phase_a(StepArgs<R> a) {
+  __shared__ float ccn_s[kCcnSlice];
+  if (a.ccn_lds) {
+    for (int i = int(threadIdx.x); i < kCcnSlice; i += int(blockDim.x)) ccn_s[i] = a.tab.ccn[i];
+    __syncthreads();
+    a.tab.ccn = ccn_s;
+  }
   const int2 pt = block_point<R, false>(a);
   const int c = pt.x, k = pt.y;
   if (c >= a.ncol || k >= a.nz) return;

@@ -23,6 +23,7 @@
 #pragma once
 
 #include <cstring>
+#include <type_traits>
 
 #include "tempo_step.hpp"
 
@@ -524,7 +525,11 @@ hipError_t launch_step_impl(const StepArgs<T>& a, const StepArgs<RA>& ga, const 
   if (ev) (void)hipEventRecord(ev[0], s);
   hipLaunchKernelGGL(k_prepass<T>, dim3(gc), dim3(256), 0, s, a);
   if (ev) (void)hipEventRecord(ev[1], s);
-  hipLaunchKernelGGL(k_phase_a<RA>, level_grid(a, lc.block_a), dim3(unsigned(lc.block_a)), 0, s, ga);
+  if (lc.waves_a && std::is_same<RA, T>::value)
+    hipLaunchKernelGGL((k_phase_a<RA, WavesPerEu<RA>::a_alt>), level_grid(a, lc.block_a), dim3(unsigned(lc.block_a)),
+                       0, s, ga);
+  else
+    hipLaunchKernelGGL(k_phase_a<RA>, level_grid(a, lc.block_a), dim3(unsigned(lc.block_a)), 0, s, ga);
   if (ev) (void)hipEventRecord(ev[2], s);
   if (lc.sed_wave == 2 && a.nz <= kSedLevels && !a.fl.semi_sedi)
     hipLaunchKernelGGL((k_phase_b_wave<T, true>), dim3(unsigned((a.ncol + kSedCols - 1) / kSedCols)),

@@ -207,13 +207,15 @@ enum : unsigned { PF_QR = 1u, PF_QG = 2u, PF_QS = 4u, PF_QI = 8u, PF_QC = 16u, P
 // per column flag bits
 enum : unsigned { CF_EXIT = 1u, CF_GINIT = 2u };
 
+constexpr int kCcnSlice = 7 * 9 * 7;  // floats of the CCN activation table the step reads
+
 template <class R>
 struct Tables {
   const R* qg;       // [(i,j,n3,k,m) Fortran index][5]
   const R* qs;       // [(i,j,k,m)][12]
   const R* fzr;      // [(i,j,k,m)][4]
   const R* fzc;      // [(i,j,k,m)][2]
-  const float* ccn;  // tnccn_act(:,:,:,3,2): [k][j][i] 7x9x7
+  const float* ccn;  // tnccn_act(:,:,:,3,2): [k][j][i] 7x9x7 (kCcnSlice)
   const R* efrw;     // (100,100)
   const R* efsw;     // (100,100)
   const R* tnc_wev;  // (100,37,100)
@@ -258,6 +260,7 @@ struct StepArgs {
   // block hands its level's points with microphysics to do to its first
   // lanes, clear = lane i takes column i
   int compact;
+  int ccn_lds;  // phase A reads the CCN table's slice from an LDS copy (1) or from global memory (0)
   Tables<S_t<R>> tab;
   RunFlags fl;
   double dt;
@@ -307,7 +310,9 @@ struct LaunchCfg {
   int order = 0;      // point grouping within blocks: bit 0 phase A, bit 1 final update
   int block_a = 256;  // threads per block of the level-local kernels (64, 128 or 256)
   int block_c = 256;
-  int host_slabs = 0;    // host-array steps: slabs of columns (0: about 64 MB of uploads each)
+  int waves_a = 0;       // phase A compiled for the default waves per SIMD (0) or one fewer (1)
+  int ccn_lds = 0;       // phase A stages the CCN table's slice in LDS (1)
+  int host_slabs = 0;    // host-array steps: slabs of columns (0: about 128 MB of uploads each)
   int host_threads = 8;  // ... and host threads copying between the caller's arrays and pinned memory
 };
 

@@ -27,7 +27,7 @@ OK, E_ARG, E_UNSUPPORTED, E_TABLE, E_HIP, E_NOMEM = 0, -1, -2, -3, -4, -5
 FP32, FP64 = 4, 8
 TABLE_QG, TABLE_QS, TABLE_FZ = 1, 2, 3
 TUNE_SED_WAVE, TUNE_BLOCK_A, TUNE_BLOCK_C, TUNE_ORDER, TUNE_FUSED = 1, 2, 3, 4, 5
-TUNE_HOST_SLABS, TUNE_HOST_THREADS = 6, 7
+TUNE_HOST_SLABS, TUNE_HOST_THREADS, TUNE_WAVES_A, TUNE_CCN_LDS = 6, 7, 8, 9
 TABLE_SIZES = {TABLE_QG: 5 * 16_867_449, TABLE_QS: 12 * 455_877, TABLE_FZ: 4 * 3_388_275 + 2 * 9_157_500}
 
 

@@ -73,10 +73,12 @@ def test_autotune_choice_keeps_results(hip):
     a = {k: v.clone() for k, v in st0.items()}
     ctx.run(a, m["dt"], m["itimestep"])
     best = autotune.autotune(ctx, st0, m["dt"], m["itimestep"])
-    print("autotuned", {k: best[k] for k in ("block_a", "block_c", "order", "sed_wave")})
+    print("autotuned", {k: best[k] for k in ("block_a", "block_c", "order", "sed_wave", "waves_a", "ccn_lds")},
+          best["phase_a_ms"])
     b = {k: v.clone() for k, v in st0.items()}
     ctx.run(b, m["dt"], m["itimestep"])
     torch.cuda.synchronize()
     for k in a:
         assert torch.equal(a[k], b[k]), k
     assert best["block_a"] in (64, 128, 256) and best["block_c"] in (64, 128, 256) and best["sed_wave"] in (0, 1)
+    assert best["waves_a"] in (0, 1) and best["ccn_lds"] in (0, 1)

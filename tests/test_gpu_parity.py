@@ -957,8 +957,9 @@ def test_gpu_table_file_roundtrip(hip):
 
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
 def test_gpu_launch_shapes_give_identical_results(hip, dtype):
-    """Every tuning setting (sedimentation kernel, block sizes, point order)
-    produces the same bits: the wavefront-per-column sedimentation performs
+    """Every tuning setting (sedimentation kernel, block sizes, point order,
+    phase A's register budget, the CCN table in LDS) produces the same bits:
+    the wavefront-per-column sedimentation performs
     the serial kernel's arithmetic level by level, and the point order only
     regroups independent points into wavefronts.  The 64-level case fills
     exactly one staging pass of the sedimentation kernel's blocks; the check
@@ -974,12 +975,15 @@ def test_gpu_launch_shapes_give_identical_results(hip, dtype):
         ctx = context(hip, flags)
         nl = None if "nlev" not in m else torch.tensor(m["nlev"], dtype=torch.int32, device="cuda")
         results = []
-        for sed, ba, bc, order in ((0, 256, 256, 0), (1, 256, 256, 0), (1, 64, 128, 0), (1, 128, 64, 1),
-                                   (1, 256, 256, 3), (0, 64, 64, 2), (1, 128, 256, 2), (2, 256, 256, 0)):
+        for sed, ba, bc, order, wa, cl in ((0, 256, 256, 0, 0, 0), (1, 256, 256, 0, 0, 0), (1, 64, 128, 0, 1, 0),
+                                           (1, 128, 64, 1, 0, 1), (1, 256, 256, 3, 1, 1), (0, 64, 64, 2, 0, 0),
+                                           (1, 128, 256, 2, 1, 0), (2, 256, 256, 0, 0, 1)):
             ctx.tune(prec, hip.TUNE_SED_WAVE, sed)
             ctx.tune(prec, hip.TUNE_BLOCK_A, ba)
             ctx.tune(prec, hip.TUNE_BLOCK_C, bc)
             ctx.tune(prec, hip.TUNE_ORDER, order)
+            ctx.tune(prec, hip.TUNE_WAVES_A, wa)
+            ctx.tune(prec, hip.TUNE_CCN_LDS, cl)
             dev = {k: torch.tensor(v, dtype=tdt, device="cuda") for k, v in st.items()}
             d = ctx.run(dev, m["dt"], m["itimestep"], nlev=nl)
             torch.cuda.synchronize()
@@ -990,6 +994,8 @@ def test_gpu_launch_shapes_give_identical_results(hip, dtype):
         ctx.tune(prec, hip.TUNE_BLOCK_A, 256)
         ctx.tune(prec, hip.TUNE_BLOCK_C, 256)
         ctx.tune(prec, hip.TUNE_ORDER, 0)
+        ctx.tune(prec, hip.TUNE_WAVES_A, 0)
+        ctx.tune(prec, hip.TUNE_CCN_LDS, 0)
         base = results[0]
         for r in results[1:]:
             for k in base[0]:
@@ -1032,7 +1038,8 @@ def test_gpu_fused_step_equals_four_launches(hip, workload, dtype):
 def test_gpu_tune_rejects_bad_values(hip):
     ctx = context(hip, dict(aerosolaware=False, hailaware=False))
     for key, val in ((hip.TUNE_SED_WAVE, 3), (hip.TUNE_BLOCK_A, 96), (hip.TUNE_BLOCK_C, 512), (hip.TUNE_ORDER, 4),
-                     (hip.TUNE_HOST_SLABS, -1), (hip.TUNE_HOST_THREADS, 0), (hip.TUNE_HOST_THREADS, 65), (99, 1)):
+                     (hip.TUNE_HOST_SLABS, -1), (hip.TUNE_HOST_THREADS, 0), (hip.TUNE_HOST_THREADS, 65),
+                     (hip.TUNE_WAVES_A, 2), (hip.TUNE_CCN_LDS, -1), (99, 1)):
         with pytest.raises(hip.TempoError) as e:
             ctx.tune(hip.FP32, key, val)
         assert e.value.code == hip.E_ARG
