@@ -11,11 +11,11 @@ namespace tempo {
 // The FP32 level physics needs 130 VGPRs unconstrained (3 waves per SIMD);
 // held to 128 it runs 4 waves with a few bytes more of private memory and
 // is measured faster.  Tighter budgets spill heavily and are slower.  The
-// FP64 level physics sits at the edge of three waves (168 VGPRs): held
-// there, so that a few more registers elsewhere in the kernel do not drop
-// it to two.  The dual instantiations keep the compiler's choice.  One wave
-// fewer (a_alt: 168 VGPRs in FP32, 256 in FP64) is a tuning option of
-// phase A (TEMPO_HIP_TUNE_WAVES_A).
+// FP64 level physics is held to three waves (168 VGPRs, a few hundred
+// bytes of spills per lane), measured faster than two waves without
+// spills.  The dual instantiations keep the compiler's choice.  One wave
+// fewer (a_alt: 3 in FP32, 2 in FP64) is a tuning option of phase A
+// (TEMPO_HIP_TUNE_WAVES_A).
 template <class R>
 struct WavesPerEu {
   static constexpr int a = 1, a_alt = 1, c = 1;
@@ -143,7 +143,7 @@ __device__ bool point_active_c(const StepArgs<R>& a, int c, int k) {
 // this thread; every thread of the block calls it, and the point may lie
 // outside the batch (c >= ncol or k >= nz) in the last blocks.
 template <class R, bool kFinal>
-__device__ int2 block_point(const StepArgs<R>& a) {
+__device__ __forceinline__ int2 block_point(const StepArgs<R>& a) {
   const int lane = int(threadIdx.x & 63u), w = int(threadIdx.x >> 6);
   const int c0 = int(blockIdx.x) * 64, k0 = int(blockIdx.y) * int(blockDim.x >> 6);
   if (!(a.compact & (kFinal ? 2 : 1))) return make_int2(c0 + lane, k0 + w);
