@@ -475,7 +475,7 @@ hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* 
   hipError_t e;
   auto cp = [&](void* dst, const void* src, size_t n) { return hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, s); };
   auto phase_pre_a = [&]() {
-    hipLaunchKernelGGL(k_prepass<D>, dim3(gc), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_prepass<D>, prepass_grid(a), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_phase_a<D>, level_grid(a, 256), dim3(256), 0, s, a);
   };
   auto phase_c = [&]() { hipLaunchKernelGGL(k_phase_c<D>, level_grid(a, 256), dim3(256), 0, s, a); };

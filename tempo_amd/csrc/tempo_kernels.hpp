@@ -52,33 +52,47 @@ __device__ __forceinline__ R column_nc(const StepArgs<R>& a, int c) {
 }
 
 // ------------------------------------------------------------ pre-pass ----
+// Blocks of 64 columns x 4 waves: wave w tests levels w, w + 4, ... of its
+// lane's column, so that every load is one coalesced row of 64 columns and
+// the chip holds four times as many waves as columns/64; the waves' answers
+// are combined through LDS.  The tests are ORs over levels, so their order
+// does not matter; the first-step sums and the height prefix keep the
+// reference's sequential order in wave 0.
+constexpr int kPrepassCols = 64;
 template <class R>
 __global__ void __launch_bounds__(256) k_prepass(StepArgs<R> a) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= a.ncol) return;
-  const int nzc = a.nlev ? min(max(a.nlev[c], 1), a.nz) : a.nz;
-  unsigned cf = 0;
+  __shared__ unsigned char flag[4][kPrepassCols];
+  const int lane = int(threadIdx.x & 63u), w = int(threadIdx.x >> 6);
+  const int c = int(blockIdx.x) * kPrepassCols + lane;
+  const bool in = c < a.ncol;
+  const int nzc = !in ? 0 : a.nlev ? min(max(a.nlev[c], 1), a.nz) : a.nz;
   // early exit (:416-418): any hydrometeor above r1 after the initial checks
   // (tendencies are still zero, so l_q == q1d > r1), or ice supersaturation
-  bool any_q = false;
-  for (int k = 0; k < nzc && !any_q; ++k) {
+  bool q = false;
+  for (int k = w; k < nzc && !q; k += 4) {
     const size_t o = TEMPO_IDX_STATE(a, c, k);
-    any_q = a.f[F_QC][o] > R(kR1) || a.f[F_QR][o] > R(kR1) || a.f[F_QI][o] > R(kR1) || a.f[F_QS][o] > R(kR1) ||
-            a.f[F_QG][o] > R(kR1);
+    q = a.f[F_QC][o] > R(kR1) || a.f[F_QR][o] > R(kR1) || a.f[F_QI][o] > R(kR1) || a.f[F_QS][o] > R(kR1) ||
+        a.f[F_QG][o] > R(kR1);
   }
-  if (!any_q) {
-    bool sup = false;
-    for (int k = 0; k < nzc && !sup; ++k) {
-      const size_t o = TEMPO_IDX_STATE(a, c, k);
-      const R temp = load_t(a, o), pres = a.f[F_P][o], qv = mx(R(kMinQv), a.f[F_QV][o]);
-      const R qvs = rslf(pres, temp);
-      const R qvsi = (temp - R(kT0) <= R(0.0)) ? rsif(pres, temp) : qvs;
-      R ssati = qv / qvsi - R(1.0);
-      if (mabs(ssati) < R(kEps)) ssati = R(0.0);
-      sup = ssati > R(0.0);
-    }
-    if (!sup) cf |= CF_EXIT;
+  flag[w][lane] = q;
+  __syncthreads();
+  const bool any_q = flag[0][lane] | flag[1][lane] | flag[2][lane] | flag[3][lane];
+  bool sup = false;
+  for (int k = w; !any_q && k < nzc && !sup; k += 4) {
+    const size_t o = TEMPO_IDX_STATE(a, c, k);
+    const R temp = load_t(a, o), pres = a.f[F_P][o], qv = mx(R(kMinQv), a.f[F_QV][o]);
+    const R qvs = rslf(pres, temp);
+    const R qvsi = (temp - R(kT0) <= R(0.0)) ? rsif(pres, temp) : qvs;
+    R ssati = qv / qvsi - R(1.0);
+    if (mabs(ssati) < R(kEps)) ssati = R(0.0);
+    sup = ssati > R(0.0);
   }
+  __syncthreads();  // every wave has read the first answers
+  flag[w][lane] = sup;
+  __syncthreads();
+  if (w != 0 || !in) return;
+  unsigned cf = 0;
+  if (!any_q && !(flag[0][lane] | flag[1][lane] | flag[2][lane] | flag[3][lane])) cf |= CF_EXIT;
   // first-step graupel initialisation test (:387-393)
   if (a.first_step && a.f[F_NG] && a.f[F_QB]) {
     R sqg = R(0), sng = R(0), sqb = R(0);
@@ -99,6 +113,10 @@ __global__ void __launch_bounds__(256) k_prepass(StepArgs<R> a) {
     }
   }
   a.cf[c] = cf;
+}
+template <class R>
+dim3 prepass_grid(const StepArgs<R>& a) {
+  return dim3(unsigned((a.ncol + kPrepassCols - 1) / kPrepassCols));
 }
 
 // ------------------------------------------------ point grouping ----
@@ -1460,8 +1478,7 @@ hipError_t launch_dual_unpack(const Dual<T>* in, const DualGeom& g, T* v, T* t, 
 // the four launches of one tangent-linear step (sedimentation one lane per column)
 template <class D>
 hipError_t launch_step_tlm_impl(const StepArgs<D>& a, Dual<double>* vtbuf, hipStream_t s) {
-  const unsigned gc = unsigned((a.ncol + 255) / 256);
-  hipLaunchKernelGGL(k_prepass<D>, dim3(gc), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_prepass<D>, prepass_grid(a), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_phase_a<D>, level_grid(a, 256), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_phase_b<D>, dim3(unsigned((a.ncol + 127) / 128)), dim3(128), 0, s, a, vtbuf);
   hipLaunchKernelGGL(k_phase_c<D>, level_grid(a, 256), dim3(256), 0, s, a);

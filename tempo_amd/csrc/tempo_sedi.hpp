@@ -521,9 +521,8 @@ __global__ void __launch_bounds__(64 * kSedCols) __attribute__((amdgpu_waves_per
 template <class T, class RA>
 hipError_t launch_step_impl(const StepArgs<T>& a, const StepArgs<RA>& ga, const LaunchCfg& lc, double* vtbuf,
                             hipStream_t s, hipEvent_t* ev) {
-  const unsigned gc = unsigned((a.ncol + 255) / 256);
   if (ev) (void)hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL(k_prepass<T>, dim3(gc), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_prepass<T>, prepass_grid(a), dim3(256), 0, s, a);
   if (ev) (void)hipEventRecord(ev[1], s);
   if (lc.waves_a && std::is_same<RA, T>::value)
     hipLaunchKernelGGL((k_phase_a<RA, WavesPerEu<RA>::a_alt>), level_grid(a, lc.block_a), dim3(unsigned(lc.block_a)),
