@@ -251,10 +251,11 @@ int tempo_hip_snow_check_and_update_host(tempo_hip_ctx *ctx, int n, const double
  * the inputs (NULL entries: zero tangent) and overwritten with the tangents
  * of the outputs; d_tl (may be NULL) receives the tangents of the requested
  * diagnostics.  Thresholds, clamps, table indices and substep counts take
- * their forward branches (branch-frozen derivatives); in FP64 the values
- * are those of tempo_hip_run bit for bit.  qc_bl and qcfrac_bl of f, when
- * given, are constants of the step.  Returns TEMPO_HIP_E_UNSUPPORTED for a
- * context with semi_sedi_flag or refl10cm_from_melting_flag. */
+ * their forward branches (branch-frozen derivatives; with semi_sedi_flag
+ * also the semi-Lagrangian remap's source cells, limiters and clamps); in
+ * FP64 the values are those of tempo_hip_run bit for bit.  qc_bl and
+ * qcfrac_bl of f, when given, are constants of the step.  Returns
+ * TEMPO_HIP_E_UNSUPPORTED for a context with refl10cm_from_melting_flag. */
 int tempo_hip_tlm(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_fields *f_tl,
                   const tempo_hip_diags *d, const tempo_hip_diags *d_tl, const tempo_hip_dims *dims, double dt,
                   int itimestep, int precision, void *stream);
@@ -268,7 +269,9 @@ int tempo_hip_tlm(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip
  * nr, ni, nwfa, nifa, ng and qb into the matching entries of xbar (NULL
  * entries are skipped and cost nothing), with the branches frozen as in
  * tempo_hip_tlm; p, w, dz enter as constants.  Working memory (about 2.5 kB
- * per column-level) is allocated on first use. */
+ * per column-level) is allocated on first use.  Returns
+ * TEMPO_HIP_E_UNSUPPORTED for a context with semi_sedi_flag or
+ * refl10cm_from_melting_flag. */
 int tempo_hip_adjoint(tempo_hip_ctx *ctx, const tempo_hip_fields *f, const tempo_hip_fields *ybar,
                       const tempo_hip_diags *dbar, const tempo_hip_fields *xbar, const tempo_hip_dims *dims, double dt,
                       int itimestep, int precision, void *stream);

@@ -741,8 +741,6 @@ template <class T>
 int run_tlm(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fields* ft, const tempo_hip_diags* d,
             const tempo_hip_diags* dt_, const tempo_hip_dims* dm, double dt, int itimestep, hipStream_t s) {
   using D = Dual<T>;
-  if (x->cfg.semi_sedi_flag)
-    return fail(TEMPO_HIP_E_UNSUPPORTED, "the tangent-linear step covers flux-form sedimentation only");
   if ((f->qc_bl || f->qcfrac_bl) && !(x->cfg.ml_for_nc_flag || x->cfg.ml_for_bl_nc_flag))
     return fail(TEMPO_HIP_E_ARG, "qc_bl/qcfrac_bl need the cloud-number network (ml_for_bl_nc_flag or ml_for_nc_flag)");
   if (x->cfg.refl10cm_from_melting_flag)
