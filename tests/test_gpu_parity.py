@@ -906,10 +906,11 @@ def test_gpu_error_codes(hip):
     with pytest.raises(hip.TempoError) as e:
         ctx.run(dev, -1.0, 2)
     assert e.value.code == hip.E_ARG
-    # the derivative entry points cover the flux-form sedimentation only
+    # the adjoint covers the flux-form sedimentation only (the tangent-linear
+    # step both forms)
     semi = hip.TempoHip(aerosolaware_flag=0, hailaware_flag=0, semi_sedi_flag=1)
     with pytest.raises(hip.TempoError) as e:
-        semi.tlm(dict(dev), {k: torch.zeros_like(v) for k, v in dev.items()}, m["dt"], 2)
+        semi.adjoint(dict(dev), {}, {}, m["dt"], 2)
     assert e.value.code == hip.E_UNSUPPORTED
     semi.close()
     # boundary-layer cloud without the network, in the step and its derivatives
