@@ -290,6 +290,19 @@ def test_gpu_fp64_matches_reference_at_2048_columns(hip, big_case):
     check_against_reference(out, d, ref, d2, d3, st, "2048 columns", ("refl10cm", "re_cloud", "re_ice", "re_snow"))
 
 
+def test_gpu_fp64_matches_reference_on_the_plain_configuration(hip, gpu_tables):
+    """The reference's CPU-runnable configuration at its full size: 1,024
+    columns x 60 levels, dt 30 s, aerosol- and hail-aware off, two steps --
+    the FP64 twin against the reference FP64 build on the same inputs"""
+    from tempo_amd import generator as gen
+    st, m = gen.generate("plain")
+    assert st["p"].shape == (60, 1024) and m["dt"] == 30.0
+    flags = dict(aerosolaware=False, hailaware=False)
+    ref, d2, d3 = reference_steps(gpu_tables, flags, st, m["dt"], m["itimestep"], steps=2)
+    out, d = gpu_run(context(hip, flags), st, m["dt"], m["itimestep"], steps=2)
+    check_against_reference(out, d, ref, d2, d3, st, "1024 x 60 plain", ("refl10cm", "re_cloud", "re_ice", "re_snow"))
+
+
 def test_gpu_oracle_is_the_reference_at_2048_columns(hip, gpu_tables, big_case):
     """the C restatement on the same columns, bit for bit"""
     from oracle import oracle as orc
