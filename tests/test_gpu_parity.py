@@ -414,13 +414,14 @@ def report_line(tag, rep):
               f"{p[n]['moved_columns']} moved")
 
 
-def reference_sp_raw(gpu_tables, flags, rep, st, dt, itimestep):
+def reference_sp_raw(gpu_tables, flags, rep, st, dt, itimestep, nlev=None):
     """the reference's own single-precision build (oracle/_ref, -DSINGLE_PREC)
     on the same FP32-representable inputs, per tendency: rel_l2 over the
     points where our FP32 step took the FP64 run's branches, against the
     FP64 output rounded to FP32 -- our rel_l2's measure on the same points
-    -- and rel_l2_raw over all points against the unrounded FP64 output;
-    None without that build"""
+    -- and rel_l2_raw over all points against the unrounded FP64 output
+    (ragged columns, nlev: each column run on its own levels, the points
+    above its top left out); None without that build"""
     from oracle import ref
     from tempo_amd import branch_report as br
     if not ref.available("sp"):
@@ -430,12 +431,23 @@ def reference_sp_raw(gpu_tables, flags, rep, st, dt, itimestep):
     keep = rep["keep"]
     ref.init(gpu_tables, variant="sp", **flags)
     osp = {k: v.copy() for k, v in inp.items()}
-    ref.run(osp, dt, itimestep, variant="sp")
+    valid = np.ones(inp["p"].shape, bool)
+    if nlev is None:
+        ref.run(osp, dt, itimestep, variant="sp")
+    else:
+        valid = np.arange(inp["p"].shape[0])[:, None] < np.asarray(nlev)[None, :]
+        for c in range(inp["p"].shape[1]):
+            n = int(nlev[c])
+            sub = {k: np.ascontiguousarray(v[:n, c:c + 1]) for k, v in osp.items()}
+            ref.run(sub, dt, itimestep, variant="sp")
+            for k, v in sub.items():
+                osp[k][:n, c] = v[:, 0]
     out = {}
     for k in rep["fields"]:
         t_sp = (osp[k] - inp[k]) / dt
         t64r = (o64[k].astype(np.float32).astype(np.float64) - inp[k]) / dt
-        out[k] = dict(rel_l2=br._rel(t_sp[keep], t64r[keep]), rel_l2_raw=br._rel(t_sp, (o64[k] - inp[k]) / dt))
+        out[k] = dict(rel_l2=br._rel(t_sp[keep], t64r[keep]),
+                      rel_l2_raw=br._rel(t_sp[valid], ((o64[k] - inp[k]) / dt)[valid]))
     return out
 
 
@@ -515,6 +527,26 @@ def test_gpu_fp32_within_tolerance(hip, big_case, gpu_tables, semi):
         print("points moved > 1e-3, ours vs the reference FP32 build:", ours, theirs)
         for k in ours:
             assert ours[k] <= 2.0 * theirs[k] + 1e-4, (k, ours[k], theirs[k])
+
+
+def test_gpu_fp32_uneven_columns(hip, gpu_tables):
+    """FP32 against the FP64 twin on 2,048 columns of the uneven-column
+    workload (30-128 active levels, hail-aware, convective columns,
+    stretched layers), with branch accounting over each column's active
+    levels: every tendency within 1e-4 relative L2 over the branch-agreeing
+    points, or within twice the reference FP32 build's error on the same
+    points (each column run on its own levels there)"""
+    from tempo_amd import branch_report as br
+    from tempo_amd import generator as gen
+    st, m = gen.generate("ragged", ncol=2048, col0=7777)
+    flags = dict(aerosolaware=False, hailaware=True)
+    ctx = context(hip, flags)
+    rep = br.compare(ctx, st, m["dt"], m["itimestep"], nlev=m["nlev"])
+    theirs = reference_sp_raw(gpu_tables, flags, rep, st, m["dt"], m["itimestep"], nlev=m["nlev"])
+    report_line("FP32 vs FP64, uneven columns:", rep)
+    print_theirs(theirs)
+    check_fp32_report(rep, theirs=theirs)
+    assert rep["flipped_points"] <= 0.15, rep["flipped_points"]
 
 
 @pytest.mark.parametrize("workload,aero", [("aerosol", True), ("plain", False)])
