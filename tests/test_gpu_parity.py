@@ -812,14 +812,16 @@ def test_gpu_host_path_equals_device_path(hip, prec):
         np.testing.assert_array_equal(hd[k], dd[k], err_msg=k)
 
 
-def _host_call(ctx, hip, st, m, diag, dims, nlev=None, precision=None):
+def _host_call(ctx, hip, st, m, diag, dims, nlev=None, precision=None, land=None):
     """tempo_hip_run_host through ctypes on numpy arrays with the given diag
-    arrays (name -> float64 array) and geometry"""
+    arrays (name -> float64 array), geometry and land mask (int32 per column)"""
     import ctypes
     f = hip.Fields()
     for name in hip.FIELD_NAMES:
         if name in st:
             setattr(f, name, st[name].ctypes.data)
+    if land is not None:
+        f.land = land.ctypes.data
     d = hip.Diags()
     for n, a in diag.items():
         setattr(d, n, a.ctypes.data)
@@ -835,8 +837,8 @@ def _host_call(ctx, hip, st, m, diag, dims, nlev=None, precision=None):
 @pytest.mark.parametrize("layout", ["j_one_slab", "j_slabs", "i_slabs", "j_slabs_locked", "i_slabs_locked"])
 def test_gpu_host_path_padded_diagnostics_and_ragged_columns(hip, prec, layout):
     """tempo_hip_run_host(_prec) with padded rows (state ldk > ni; diagnostics
-    dldk, d2ldj > ni, as a Fortran host's halo'd tile) and ragged columns
-    (nlev), stepped in one slab, in slabs of j rows (32 x 3 tile) and in
+    dldk, d2ldj > ni, as a Fortran host's halo'd tile), ragged columns
+    (nlev) and a land mask, stepped in one slab, in slabs of j rows (32 x 3 tile) and in
     slabs of i ranges (1024 x 1 tile), with pageable arrays and with the
     state in page-locked memory (moved by the copy engines directly; the
     diagnostics pageable, or also page-locked in the i-slab case): tile
@@ -856,7 +858,10 @@ def test_gpu_host_path_padded_diagnostics_and_ragged_columns(hip, prec, layout):
     nz, ncol = st["p"].shape
     nlev = m["nlev"]
     ctx = context(hip, dict(aerosolaware=False, hailaware=True))
-    dev, dd = gpu_run(ctx, st, m["dt"], m["itimestep"], nlev=nlev, dtype=torch.float64 if prec == "fp64" else torch.float32)
+    # a land mask (the one-moment cloud number of these columns depends on it)
+    land = np.ascontiguousarray((np.arange(ncol) % 3 == 0).astype(np.int32))
+    dev, dd = gpu_run(ctx, st, m["dt"], m["itimestep"], nlev=nlev, land=land,
+                      dtype=torch.float64 if prec == "fp64" else torch.float32)
     pad = 5
     sentinel = -7.25
     # state in the Fortran (i, k, j) layout of an (ni + spad, nz, nj) array,
@@ -880,7 +885,8 @@ def test_gpu_host_path_padded_diagnostics_and_ragged_columns(hip, prec, layout):
     p = hip.FP64 if prec == "fp64" else hip.FP32
     ctx.tune(p, hip.TUNE_HOST_SLABS, slabs)
     try:
-        _host_call(ctx, hip, hst, m, dict(d2, **d3), dims, precision=None if prec == "fp64" else hip.FP32)
+        _host_call(ctx, hip, hst, m, dict(d2, **d3), dims, precision=None if prec == "fp64" else hip.FP32,
+                   land=land)
     finally:
         ctx.tune(p, hip.TUNE_HOST_SLABS, 0)
     for k in st:
