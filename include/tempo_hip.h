@@ -332,7 +332,7 @@ int tempo_hip_substeps(tempo_hip_ctx *ctx, int *host_out, size_t n);
  *                               transfer pipeline, 1..4096 (fewer where the
  *                               tile has fewer j rows, or fewer than 256
  *                               columns per slab when nj = 1); 0 = about
- *                               64 MB of uploads per slab, at most 32
+ *                               128 MB of uploads per slab, at most 32
  *                               (default)
  *   TEMPO_HIP_TUNE_HOST_THREADS ... and host threads copying between the
  *                               caller's arrays and pinned memory, 1..64

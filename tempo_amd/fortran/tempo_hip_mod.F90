@@ -10,7 +10,8 @@
 ! diagnostics come back in a ty_tempo_driver_diags with the reference's
 ! component names (:18-34), allocated on the first call as the reference
 ! does (:304-386).  Arrays are the host's real(8) arrays; the library stages
-! them through device memory (tempo_hip_run_host).  Hosts whose state already
+! them through device memory (tempo_hip_run_host), without host copies for
+! arrays allocated with tempo_alloc_page_locked.  Hosts whose state already
 ! lives in device memory call the C entry point tempo_hip_run directly (the
 ! interface block below).
 !
