@@ -49,6 +49,7 @@ __host__ __device__ constexpr int adj_y_field(int i) {
 
 struct AdjArgs {
   const double* ybar;   // AY_N planes (compact c + k*ncol), absent outputs all zero
+  int* lvl_on;          // nz: 1 where some output cotangent of the level is non-zero
   const double* pbar;   // 4 x ncol: rain, graupel, snow, ice precipitation
   double* xbar;         // AX_N planes (compact)
   double* ubar;         // 10 planes (compact): tendency cotangents, SD_QRTEN order
@@ -118,12 +119,36 @@ __global__ void __launch_bounds__(256) k_adj_zero(T* __restrict__ out, size_t n)
   if (i < n) out[i] = T(0);
 }
 
-// out[e] (+)= sum over outputs of ybar * tangent
+// the levels whose output cotangents are not all zero (g.lvl_on, cleared
+// by k_adj_level_zero): the final update's passes contribute nothing at
+// the others
+template <class D>
+__global__ void __launch_bounds__(256) k_adj_level_zero(StepArgs<D> a, AdjArgs g) {
+  const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k < a.nz) g.lvl_on[k] = 0;
+}
+template <class D>
+__global__ void __launch_bounds__(256) k_adj_level_flags(StepArgs<D> a, AdjArgs g) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+  bool nz = false;
+  if (c < a.ncol) {
+    const size_t e = size_t(c) + size_t(k) * a.ncol, n3 = size_t(a.ncol) * a.nz;
+    for (int q = 0; q < AY_N; ++q) nz = nz || g.ybar[size_t(q) * n3 + e] != 0.0;
+  }
+  if (__ballot(nz) != 0ull && (threadIdx.x & 63u) == 0) atomicOr(&g.lvl_on[k], 1);
+}
+
+// out[e] (+)= sum over outputs of ybar * tangent (0 at a level without
+// output cotangents, whose final update a.lvl_on skipped)
 template <class D>
 __global__ void __launch_bounds__(256) k_adj_contract(StepArgs<D> a, AdjArgs g, double* out, int accumulate) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
   if (c >= a.ncol) return;
   const size_t o = TEMPO_IDX_STATE(a, c, k), e = size_t(c) + size_t(k) * a.ncol, n3 = size_t(a.ncol) * a.nz;
+  if (a.lvl_on && !a.lvl_on[k]) {
+    if (!accumulate) out[e] = 0.0;
+    return;
+  }
   double acc = 0.0;
   for (int q = 0; q < AY_REFL; ++q) {
     const D* f = a.fo[adj_y_field(q)];

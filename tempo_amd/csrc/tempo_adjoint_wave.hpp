@@ -478,7 +478,13 @@ hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* 
     hipLaunchKernelGGL(k_prepass<D>, prepass_grid(a), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_phase_a<D>, level_grid(a, 256), dim3(256), 0, s, a);
   };
-  auto phase_c = [&]() { hipLaunchKernelGGL(k_phase_c<D>, level_grid(a, 256), dim3(256), 0, s, a); };
+  // the final update's passes and contractions skip the levels without
+  // output cotangents
+  StepArgs<D> ac = a;
+  ac.lvl_on = g.lvl_on;
+  hipLaunchKernelGGL(k_adj_level_zero<D>, dim3(unsigned((a.nz + 255) / 256)), dim3(256), 0, s, a, g);
+  hipLaunchKernelGGL(k_adj_level_flags<D>, g2, dim3(256), 0, s, a, g);
+  auto phase_c = [&]() { hipLaunchKernelGGL(k_phase_c<D>, level_grid(a, 256), dim3(256), 0, s, ac); };
 
   // tangent-free forward: scratch snapshots after A and after B
   phase_pre_a();
@@ -500,7 +506,7 @@ hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* 
     }
     hipLaunchKernelGGL(k_adj_seed_ten<D>, g2, dim3(256), 0, s, a, m, m_prev);
     phase_c();
-    hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, a, g, ub, 0);
+    hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, ac, g, ub, 0);
     m_prev = m;
   }
   // S^T on the values after phase A
@@ -523,7 +529,7 @@ hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* 
     hipLaunchKernelGGL(k_phase_a<D>, level_grid(a, 256), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_adj_capture_merge<D>, g2, dim3(256), 0, s, a, g, xb, post_sd);
     phase_c();
-    hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, a, g, xb, 1);
+    hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, ac, g, xb, 1);
     prev = i;
   }
   return hipGetLastError();

@@ -1348,6 +1348,7 @@ __global__ void __launch_bounds__(256) TEMPO_ATTR_C k_phase_c(StepArgs<R> a) {
   const int c = pt.x, k = pt.y;
   if (c >= a.ncol || k >= a.nz) return;
   if (a.nlev && k >= a.nlev[c]) return;
+  if (a.lvl_on && !a.lvl_on[k]) return;
   SigScope<R> scope(a, c, k, 1);
   phase_c_point(a, batch_scr(a), c, k);
 }

@@ -205,16 +205,21 @@ def test_semi_lagrangian_tlm(hip, sample, ref_tables):
     assert central_ok.mean() >= 0.8
 
 
-@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-10), (torch.float32, 1e-4)])
-def test_adjoint_dot_product(ctx, sample, dtype, tol):
+@pytest.mark.parametrize("dtype,tol,sparse", [(torch.float64, 1e-10, False), (torch.float32, 1e-4, False),
+                                              (torch.float64, 1e-10, True)])
+def test_adjoint_dot_product(ctx, sample, dtype, tol, sparse):
+    """<ybar, J v> = <J^T ybar, v>; sparse: the output cotangents non-zero
+    at three levels only (as the bench loss's reflectivity), so that the
+    final update's passes skip the other levels"""
     st, m = sample
     rng = np.random.default_rng(9)
     v = direction(st, 11)
-    ybar = {k: torch.tensor(rng.standard_normal(st[k].shape) / (np.abs(st[k]).max() + 1e-30), dtype=dtype,
+    nz, ncol = st["p"].shape
+    rows = np.isin(np.arange(nz), LEVELS)[:, None] if sparse else np.ones((nz, 1), bool)
+    ybar = {k: torch.tensor(rng.standard_normal(st[k].shape) / (np.abs(st[k]).max() + 1e-30) * rows, dtype=dtype,
                             device="cuda") for k in ("t", "qv", "qc", "qr", "qi", "qs", "qg", "ni", "nr", "nc", "nwfa",
                                                      "nifa") if k in st}
-    nz, ncol = st["p"].shape
-    dbar = {"refl10cm": torch.tensor(rng.standard_normal((nz, ncol)) * 0.01, dtype=dtype, device="cuda")}
+    dbar = {"refl10cm": torch.tensor(rng.standard_normal((nz, ncol)) * 0.01 * rows, dtype=dtype, device="cuda")}
     for k in ("rain_precip", "ice_liquid_equiv_precip", "snow_liquid_equiv_precip", "graupel_liquid_equiv_precip"):
         dbar[k] = torch.tensor(rng.standard_normal(ncol) * 1e3, dtype=dtype, device="cuda")
     b = dev(st, dtype)
