@@ -69,16 +69,29 @@ __global__ void __launch_bounds__(256) k_adj_seed_input(StepArgs<D> a, int field
   if (a.f[field]) a.f[field][o].d = 1;
 }
 
+// the values of the ten post-sedimentation tendency planes after the
+// tangent-free forward's sedimentation, plane m at post[m * plane + s]
+template <class D>
+__global__ void __launch_bounds__(256) k_adj_keep_post(StepArgs<D> a, double* __restrict__ post) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+  if (c >= a.ncol) return;
+  const size_t s = TEMPO_IDX_SCR(a, c, k);
+#pragma unroll
+  for (int m = 0; m < 10; ++m) post[size_t(m) * a.plane + s] = a.sd[size_t(SD_QRTEN + m) * a.plane + s].v;
+}
+
 // after phase A of a local pass: the S^T part of xbar_i, (dz/dx_i) . zbar,
 // from the tangents of the sedimentation inputs (masses, numbers, raw fall
 // speeds, density) and the cotangents zbar that S^T has already produced;
 // then the values of the pre-sedimentation tendencies are replaced by the
-// forward's post-sedimentation ones, keeping their local tangents (S frozen
-// out).  The final update reads neither the sedimenting masses nor the fall
-// speeds, so their tangents need not be cleared.
+// forward's post-sedimentation ones (k_adj_keep_post), keeping their local
+// tangents (S frozen out), at the levels whose final update runs (a.lvl_on;
+// the next pass's phase A rewrites the planes).  The final update reads
+// neither the sedimenting masses nor the fall speeds, so their tangents need
+// not be cleared.
 template <class D>
 __global__ void __launch_bounds__(256) k_adj_capture_merge(StepArgs<D> a, AdjArgs g, double* __restrict__ xb,
-                                                           const Dual<double>* __restrict__ post_sd) {
+                                                           const double* __restrict__ post) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
   if (c >= a.ncol) return;
   const size_t s = TEMPO_IDX_SCR(a, c, k), e = size_t(c) + size_t(k) * a.ncol, n3 = size_t(a.ncol) * a.nz;
@@ -93,11 +106,9 @@ __global__ void __launch_bounds__(256) k_adj_capture_merge(StepArgs<D> a, AdjArg
     acc += double(a.sr[size_t(srf[q]) * a.plane + s].d) * g.zbar[size_t(AZ_VTRR + q) * n3 + e];
   acc += double(a.sr[size_t(SR_RHO) * a.plane + s].d) * g.zbar[size_t(AZ_RHO) * n3 + e];
   xb[e] = acc;
+  if (a.lvl_on && !a.lvl_on[k]) return;
 #pragma unroll
-  for (int m = 0; m < 10; ++m) {
-    const size_t x = size_t(SD_QRTEN + m) * a.plane + s;
-    a.sd[x].v = post_sd[x].v;
-  }
+  for (int m = 0; m < 10; ++m) a.sd[size_t(SD_QRTEN + m) * a.plane + s].v = post[size_t(m) * a.plane + s];
 }
 
 // seed the tangent of one post-sedimentation tendency plane (u pass),

@@ -459,7 +459,7 @@ __global__ void __launch_bounds__(64 * kScrW) k_adj_sed_wave(StepArgs<D> a, AdjA
 struct AdjSnap {
   void* sr_a;  // sr_bytes
   void* sd_a;  // sd_bytes
-  void* sd_b;  // sd_bytes
+  void* sd_b;  // sd_bytes (holds 10 FP64 planes of the post-sedimentation tendencies)
 };
 
 // The passes of one adjoint.  The state is out of place: a.f holds the
@@ -491,7 +491,8 @@ hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* 
   if ((e = cp(snap.sr_a, a.sr, sr_bytes)) != hipSuccess) return e;
   if ((e = cp(snap.sd_a, a.sd, sd_bytes)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_phase_b<D>, dim3(unsigned((a.ncol + 127) / 128)), dim3(128), 0, s, a, vtbuf);
-  if ((e = cp(snap.sd_b, a.sd, sd_bytes)) != hipSuccess) return e;
+  double* post = static_cast<double*>(snap.sd_b);
+  hipLaunchKernelGGL(k_adj_keep_post<D>, g2, dim3(256), 0, s, a, post);
   // passes seeded on the post-sedimentation tendencies (the final update
   // reads the inputs, a.fa and the scratch, and writes only a.fo).  Without
   // prognostic graupel number and volume the final update does not read
@@ -520,14 +521,13 @@ hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* 
   // + the local part through A and C, phase B replaced by the forward's
   // post-sedimentation tendency values.  The pre-pass results (column flags,
   // heights) depend on values only and stay from the first pass.
-  const Dual<double>* post_sd = static_cast<const Dual<double>*>(snap.sd_b);
   int prev = -1;
   for (int i = 0; i < AX_N; ++i) {
     if (!have_x[i]) continue;
     double* xb = g.xbar + size_t(i) * n3;
     hipLaunchKernelGGL(k_adj_seed_input<D>, g2, dim3(256), 0, s, a, adj_x_field(i), prev < 0 ? -1 : adj_x_field(prev));
     hipLaunchKernelGGL(k_phase_a<D>, level_grid(a, 256), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_adj_capture_merge<D>, g2, dim3(256), 0, s, a, g, xb, post_sd);
+    hipLaunchKernelGGL(k_adj_capture_merge<D>, g2, dim3(256), 0, s, ac, g, xb, post);
     phase_c();
     hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, ac, g, xb, 1);
     prev = i;
