@@ -490,7 +490,13 @@ hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* 
   phase_pre_a();
   if ((e = cp(snap.sr_a, a.sr, sr_bytes)) != hipSuccess) return e;
   if ((e = cp(snap.sd_a, a.sd, sd_bytes)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_phase_b<D>, dim3(unsigned((a.ncol + 127) / 128)), dim3(128), 0, s, a, vtbuf);
+  // its sedimentation on the values, one wavefront per column (the tangents
+  // are zero in this pass)
+  if (a.nz <= kSedLevels)
+    hipLaunchKernelGGL(k_phase_b_wave<D>, dim3(unsigned((a.ncol + kSedCols - 1) / kSedCols)), dim3(64 * kSedCols),
+                       sed_lds_bytes(a.nz), s, a);
+  else
+    hipLaunchKernelGGL(k_phase_b<D>, dim3(unsigned((a.ncol + 127) / 128)), dim3(128), 0, s, a, vtbuf);
   double* post = static_cast<double*>(snap.sd_b);
   hipLaunchKernelGGL(k_adj_keep_post<D>, g2, dim3(256), 0, s, a, post);
   // passes seeded on the post-sedimentation tendencies (the final update

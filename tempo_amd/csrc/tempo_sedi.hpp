@@ -69,6 +69,15 @@ __device__ __forceinline__ T above(const T (&x)[kSedChunks], int j) {
 }
 }  // namespace wv
 
+// the value of a stored number as a double (the staging carries values
+// only: a dual plane's tangent is not staged)
+__device__ __forceinline__ double sval(double x) { return x; }
+__device__ __forceinline__ double sval(float x) { return double(x); }
+template <class T>
+__device__ __forceinline__ double sval(const Dual<T>& x) {
+  return double(x.v);
+}
+
 // Block-wide staging of (level, column) planes through LDS slots.  Loads
 // and stores issue without barriers so that all planes of a stage are in
 // flight together; the caller separates the phases with __syncthreads().
@@ -94,9 +103,11 @@ struct SedTile {
   // block-uniform test), and its elements past the group's end load the last
   // one (the index is clamped, not branched on per lane, so that the loads
   // issue back to back) and are not written.
-  template <class T, int N>
-  __device__ void load_many(const int (&s)[N], const T* const (&plane)[N]) const {
-    T v[kSedChunks][N];
+  template <class TA, int NA, class TB, int NB>
+  __device__ void load_many(const int (&sa)[NA], const TA* const (&pa)[NA], const int (&sb)[NB],
+                            const TB* const (&pb)[NB]) const {
+    TA va[kSedChunks][NA];
+    TB vb[kSedChunks][NB];
     const int n = kSedCols * nz;
     const size_t base = sbase;
 #pragma unroll
@@ -104,7 +115,9 @@ struct SedTile {
       if (!kEveryPass && it * 64 * kSedCols >= n) break;
       const int e = min(int(threadIdx.x) + it * 64 * kSedCols, n - 1);
 #pragma unroll
-      for (int p = 0; p < N; ++p) v[it][p] = plane[p][base + e];
+      for (int p = 0; p < NA; ++p) va[it][p] = pa[p][base + e];
+#pragma unroll
+      for (int p = 0; p < NB; ++p) vb[it][p] = pb[p][base + e];
     }
 #pragma unroll
     for (int it = 0; it < kSedChunks; ++it) {
@@ -112,16 +125,20 @@ struct SedTile {
       if (e < n) {
         const int cc = e % kSedCols, k = e / kSedCols;
 #pragma unroll
-        for (int p = 0; p < N; ++p) slot(s[p])[cc * row + k] = double(v[it][p]);
+        for (int p = 0; p < NA; ++p) slot(sa[p])[cc * row + k] = sval(va[it][p]);
+#pragma unroll
+        for (int p = 0; p < NB; ++p) slot(sb[p])[cc * row + k] = sval(vb[it][p]);
       }
     }
   }
   // the first stage in the same way: per-point flags (slot 0), layer depths
   // from the state (slot 1), density (slot 2) and rain mass (slot 3)
-  template <class R>
-  __device__ void load_head(const StepArgs<R>& a, const unsigned* pf, const R* rho, const R* rr) const {
+  template <class R, class H, class M>
+  __device__ void load_head(const StepArgs<R>& a, const unsigned* pf, const H* rho, const M* rr) const {
     unsigned vf[kSedChunks];
-    R vz[kSedChunks], vh[kSedChunks], vr[kSedChunks];
+    R vz[kSedChunks];
+    H vh[kSedChunks];
+    M vr[kSedChunks];
     const int n = kSedCols * nz;
     const size_t base = sbase;
 #pragma unroll
@@ -140,9 +157,9 @@ struct SedTile {
       if (e < n) {
         const int cc = e % kSedCols, k = e / kSedCols, x = cc * row + k;
         slot(0)[x] = double(vf[it]);
-        slot(1)[x] = c0 + cc < ncol ? double(vz[it]) : 0.0;
-        slot(2)[x] = double(vh[it]);
-        slot(3)[x] = double(vr[it]);
+        slot(1)[x] = c0 + cc < ncol ? sval(vz[it]) : 0.0;
+        slot(2)[x] = sval(vh[it]);
+        slot(3)[x] = sval(vr[it]);
       }
     }
   }
@@ -260,7 +277,9 @@ __device__ __forceinline__ double wave_sediment(const SedCol& g, double (&xr)[kS
 
 // Sedimentation of the column group c0 .. c0 + kSedCols - 1, one wavefront
 // per column (64 * kSedCols threads); scr holds the group's scratch,
-// tile_mem sed_lds_bytes(nz) of LDS.
+// tile_mem sed_lds_bytes(nz) of LDS.  For a dual working type it runs on the
+// values (V, the scalar type) and writes tendencies with zero tangents: the
+// tangent-free forward of an adjoint.
 template <class R, bool kEveryPass = false>
 __device__ __forceinline__ void phase_b_group(const StepArgs<R>& a, const Scr<R>& scr, int c0, double* tile_mem) {
   __shared__ unsigned blk_any;
@@ -272,6 +291,7 @@ __device__ __forceinline__ void phase_b_group(const StepArgs<R>& a, const Scr<R>
   const double dt = a.dt;
   Sd_t<R>* sd = scr.sd;
   const R* sr = scr.sr;
+  using V = typename Num<R>::S;
 
   SedCol g;
   g.nzc = col_ok ? (a.nlev ? min(max(a.nlev[c], 1), a.nz) : a.nz) : 0;
@@ -291,7 +311,7 @@ __device__ __forceinline__ void phase_b_group(const StepArgs<R>& a, const Scr<R>
   T.load_head(a, scr.pf, sr + SR_RHO * plane, sd + SD_RR * plane);
   __syncthreads();
   unsigned pfl[kSedChunks];
-  R dz[kSedChunks], rho[kSedChunks];
+  V dz[kSedChunks], rho[kSedChunks];
   double rr[kSedChunks], vr[kSedChunks];
   T.get(0, w, pfl);
   T.get(1, w, dz);
@@ -316,9 +336,9 @@ __device__ __forceinline__ void phase_b_group(const StepArgs<R>& a, const Scr<R>
   // ---- rain (:603-630) ----
   if (bany & PF_QR) {
     double xn[kSedChunks], tr[kSedChunks], tn[kSedChunks], v2[kSedChunks];
-    R raw1[kSedChunks], raw2[kSedChunks];
-    T.template load_many<R, 5>({0, 1, 2, 4, 5}, {sd + SD_NR * plane, sd + SD_QRTEN * plane, sd + SD_NRTEN * plane,
-                                        sr + SR_VTRR * plane, sr + SR_VTNR * plane});
+    V raw1[kSedChunks], raw2[kSedChunks];
+    T.load_many({0, 1, 2}, {sd + SD_NR * plane, sd + SD_QRTEN * plane, sd + SD_NRTEN * plane}, {4, 5},
+                {sr + SR_VTRR * plane, sr + SR_VTNR * plane});
     __syncthreads();
     T.get(0, w, xn);
     T.get(1, w, tr);
@@ -347,10 +367,11 @@ __device__ __forceinline__ void phase_b_group(const StepArgs<R>& a, const Scr<R>
   if (bany & PF_QG) {
     double xg[kSedChunks], xn[kSedChunks], xb[kSedChunks], tg[kSedChunks], tn[kSedChunks], tb[kSedChunks];
     double v1[kSedChunks], v2[kSedChunks];
-    R raw1[kSedChunks], raw2[kSedChunks];
-    T.template load_many<R, 8>({0, 1, 2, 4, 5, 6, 7, 3},
-                      {sd + SD_RG * plane, sd + SD_NG * plane, sd + SD_RB * plane, sd + SD_QGTEN * plane,
-                       sd + SD_NGTEN * plane, sd + SD_QBTEN * plane, sr + SR_VTRG * plane, sr + SR_VTNG * plane});
+    V raw1[kSedChunks], raw2[kSedChunks];
+    T.load_many({0, 1, 2, 4, 5, 6},
+                {sd + SD_RG * plane, sd + SD_NG * plane, sd + SD_RB * plane, sd + SD_QGTEN * plane,
+                 sd + SD_NGTEN * plane, sd + SD_QBTEN * plane},
+                {7, 3}, {sr + SR_VTRG * plane, sr + SR_VTNG * plane});
     __syncthreads();
     T.get(0, w, xg);
     T.get(1, w, xn);
@@ -386,21 +407,21 @@ __device__ __forceinline__ void phase_b_group(const StepArgs<R>& a, const Scr<R>
   // with the rain mass after rain sedimentation ----
   if (bany & PF_QS) {
     double xs[kSedChunks], ts[kSedChunks], v1[kSedChunks];
-    R raw[kSedChunks];
-    T.template load_many<R, 3>({0, 1, 2}, {sd + SD_RS * plane, sd + SD_QSTEN * plane, sr + SR_VTS * plane});
+    V raw[kSedChunks];
+    T.load_many({0, 1}, {sd + SD_RS * plane, sd + SD_QSTEN * plane}, {2}, {sr + SR_VTS * plane});
     __syncthreads();
     T.get(0, w, xs);
     T.get(1, w, ts);
     T.get(2, w, raw);
     if (any & PF_QS) {
 #pragma clang fp contract(off)  // the snow-rain speed mix, as the lane-per-column kernel
-      R vraw[kSedChunks];
+      V vraw[kSedChunks];
 #pragma unroll
       for (int j = 0; j < kSedChunks; ++j) {
-        R v = raw[j];
+        V v = raw[j];
         if (pfl[j] & PF_SNOWMIX) {
-          const R sr_ = R(xs[j] / (xs[j] + rr[j]));
-          v = v * sr_ + (R(1.0) - sr_) * R(vr[j]);
+          const V sr_ = V(xs[j] / (xs[j] + rr[j]));
+          v = v * sr_ + (V(1.0) - sr_) * V(vr[j]);
         }
         vraw[j] = v;
       }
@@ -420,9 +441,9 @@ __device__ __forceinline__ void phase_b_group(const StepArgs<R>& a, const Scr<R>
   // ---- ice (:699-715): one call with steps = substeps ----
   if (bany & PF_QI) {
     double xi[kSedChunks], xn[kSedChunks], ti[kSedChunks], tn[kSedChunks], v1[kSedChunks], v2[kSedChunks];
-    R raw1[kSedChunks], raw2[kSedChunks];
-    T.template load_many<R, 6>({0, 1, 2, 4, 5, 6}, {sd + SD_RI * plane, sd + SD_NI * plane, sd + SD_QITEN * plane,
-                                           sd + SD_NITEN * plane, sr + SR_VTRI * plane, sr + SR_VTNI * plane});
+    V raw1[kSedChunks], raw2[kSedChunks];
+    T.load_many({0, 1, 2, 4}, {sd + SD_RI * plane, sd + SD_NI * plane, sd + SD_QITEN * plane, sd + SD_NITEN * plane},
+                {5, 6}, {sr + SR_VTRI * plane, sr + SR_VTNI * plane});
     __syncthreads();
     T.get(0, w, xi);
     T.get(1, w, xn);
@@ -449,9 +470,9 @@ __device__ __forceinline__ void phase_b_group(const StepArgs<R>& a, const Scr<R>
   // ---- cloud (:717-727, cloud_fallspeed :1945-1977): below 500 m only ----
   if (bany & PF_QC) {
     double xc[kSedChunks], xn[kSedChunks], tc[kSedChunks], tn[kSedChunks], v1[kSedChunks], v2[kSedChunks];
-    R raw1[kSedChunks], raw2[kSedChunks];
-    T.template load_many<R, 6>({0, 1, 2, 4, 5, 6}, {sd + SD_RC * plane, sd + SD_NC * plane, sd + SD_QCTEN * plane,
-                                           sd + SD_NCTEN * plane, sr + SR_VTRC * plane, sr + SR_VTNC * plane});
+    V raw1[kSedChunks], raw2[kSedChunks];
+    T.load_many({0, 1, 2, 4}, {sd + SD_RC * plane, sd + SD_NC * plane, sd + SD_QCTEN * plane, sd + SD_NCTEN * plane},
+                {5, 6}, {sr + SR_VTRC * plane, sr + SR_VTNC * plane});
     __syncthreads();
     T.get(0, w, xc);
     T.get(1, w, xn);
@@ -461,11 +482,11 @@ __device__ __forceinline__ void phase_b_group(const StepArgs<R>& a, const Scr<R>
     T.get(6, w, raw2);
     if (any & PF_QC) {
       int kt = 1;
-      R hgt = R(0.0);
+      V hgt = V(0.0);
       for (int k = 0; k < g.nzc - 1; ++k) {  // uniform loop: every lane follows level k
         if (wv::at_level(xc, k) > kR2) kt = k + 1;
         hgt = hgt + wv::at_level(dz, k);
-        if (hgt > R(500.0)) break;
+        if (hgt > V(500.0)) break;
       }
 #pragma unroll
       for (int j = 0; j < kSedChunks; ++j) {
