@@ -51,6 +51,14 @@ template <class R>
 using W_t = typename Num<R>::W;
 template <class R>
 using Sd_t = typename Num<R>::D;
+template <class R>
+struct IsDual {
+  static constexpr bool value = false;
+};
+template <class T>
+struct IsDual<Dual<T>> {
+  static constexpr bool value = true;
+};
 
 template <class R>
 __host__ __device__ __forceinline__ S_t<R> value(const R& x) { return x; }

@@ -271,9 +271,10 @@ struct StepArgs {
   int first_step;
   // work queue of the fused step (a counter of column groups handed out)
   int* queue;
-  // adjoint passes of the final update: levels whose output cotangents are
-  // all zero (lvl_on[k] == 0) are skipped, their contributions being zero;
-  // nullptr = every level
+  // adjoint passes of the final update (levels whose output cotangents are
+  // all zero) and of a local pass's phase A (levels where, besides, the
+  // sedimentation cotangents are all zero): levels with lvl_on[k] == 0 are
+  // skipped, their contributions being zero; nullptr = every level
   const int* lvl_on;
   // branch signatures (signature runs only): 6 x nz x ncol words, phase p
   // (0 the level-local physics, 1 the final update, 2 the sedimentation),
