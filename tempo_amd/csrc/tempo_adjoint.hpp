@@ -50,7 +50,7 @@ __host__ __device__ constexpr int adj_y_field(int i) {
 struct AdjArgs {
   const double* ybar;   // AY_N planes (compact c + k*ncol), absent outputs all zero
   int* lvl_on;          // nz: 1 where some output cotangent of the level is non-zero
-  int* lvl_loc;         // nz: lvl_on, or 1 where some zbar of the level is non-zero
+  int* pt_loc;          // compact plane: 1 where a local pass's phase A runs (see k_adj_point_flags)
   const double* pbar;   // 4 x ncol: rain, graupel, snow, ice precipitation
   double* xbar;         // AX_N planes (compact)
   double* ubar;         // 10 planes (compact): tendency cotangents, SD_QRTEN order
@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(256) k_adj_zero(T* __restrict__ out, size_t n)
 template <class D>
 __global__ void __launch_bounds__(256) k_adj_level_zero(StepArgs<D> a, AdjArgs g) {
   const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
-  if (k < a.nz) g.lvl_on[k] = g.lvl_loc[k] = 0;
+  if (k < a.nz) g.lvl_on[k] = 0;
 }
 template <class D>
 __global__ void __launch_bounds__(256) k_adj_level_flags(StepArgs<D> a, AdjArgs g) {
@@ -149,20 +149,19 @@ __global__ void __launch_bounds__(256) k_adj_level_flags(StepArgs<D> a, AdjArgs 
   }
   if (__ballot(nz) != 0ull && (threadIdx.x & 63u) == 0) atomicOr(&g.lvl_on[k], 1);
 }
-// after S^T: the levels whose local passes contribute (g.lvl_loc).  At a
-// level where every zbar is zero and the final update is skipped, the
-// capture adds tangent * 0 and nothing else reads phase A's results there,
-// so a local pass's phase A may skip it (the tangents left from the
-// tangent-free forward are zero).
+// after S^T: the points where the local passes' phase A runs (g.pt_loc).
+// At a point whose zbar are all zero, on a level whose final update is
+// skipped, the capture adds tangent * 0 and nothing else reads phase A's
+// results, so every local pass skips it; its tangents stay those of the
+// tangent-free forward (zero).
 template <class D>
-__global__ void __launch_bounds__(256) k_adj_level_zbar(StepArgs<D> a, AdjArgs g) {
+__global__ void __launch_bounds__(256) k_adj_point_flags(StepArgs<D> a, AdjArgs g) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
-  bool nz = g.lvl_on[k] != 0;
-  if (!nz && c < a.ncol) {
-    const size_t e = size_t(c) + size_t(k) * a.ncol, n3 = size_t(a.ncol) * a.nz;
-    for (int q = 0; q < AZ_N; ++q) nz = nz || g.zbar[size_t(q) * n3 + e] != 0.0;
-  }
-  if (__ballot(nz) != 0ull && (threadIdx.x & 63u) == 0) atomicOr(&g.lvl_loc[k], 1);
+  if (c >= a.ncol) return;
+  const size_t e = size_t(c) + size_t(k) * a.ncol, n3 = size_t(a.ncol) * a.nz;
+  bool on = g.lvl_on[k] != 0;
+  for (int q = 0; q < AZ_N && !on; ++q) on = g.zbar[size_t(q) * n3 + e] != 0.0;
+  g.pt_loc[e] = on ? 1 : 0;
 }
 
 // out[e] (+)= sum over outputs of ybar * tangent (0 at a level without

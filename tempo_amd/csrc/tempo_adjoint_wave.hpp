@@ -523,10 +523,10 @@ hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* 
     hipLaunchKernelGGL(k_adj_sed_wave<D>, dim3(unsigned((a.ncol + kScrW - 1) / kScrW)), dim3(64 * kScrW), 0, s, a, g);
   else
     hipLaunchKernelGGL(k_adj_sedimentation<D>, dim3(unsigned((a.ncol + 63) / 64)), dim3(64), 0, s, a, g);
-  // the local passes' phase A runs at the levels where they contribute
+  // the local passes' phase A runs at the points where they contribute
   StepArgs<D> al = a;
-  al.lvl_on = g.lvl_loc;
-  hipLaunchKernelGGL(k_adj_level_zbar<D>, g2, dim3(256), 0, s, a, g);
+  al.pt_on = g.pt_loc;
+  hipLaunchKernelGGL(k_adj_point_flags<D>, g2, dim3(256), 0, s, a, g);
   // local passes, one per requested input field: xbar_i = (dz/dx_i) . zbar
   // + the local part through A and C, phase B replaced by the forward's
   // post-sedimentation tendency values.  The pre-pass results (column flags,

@@ -819,7 +819,7 @@ int run_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fie
   const int ncol = dm->ni * dm->nj, nz = dm->nz;
   const size_t n3 = size_t(ncol) * nz, sb = F_NFIELDS * n3 * sizeof(D);
   const size_t plane = scratch_plane(size_t(ncol), size_t(nz));
-  const size_t nd = (AY_N + AX_N + 10 + AZ_N) * n3 + 4 * size_t(ncol) + AW_N * plane + size_t(nz);
+  const size_t nd = (AY_N + AX_N + 10 + AZ_N) * n3 + 4 * size_t(ncol) + AW_N * plane + (size_t(nz) + 1) / 2 + (n3 + 1) / 2;
   HIP_TRY(x->tl_state.ensure(sb));
   HIP_TRY(x->adj_in.ensure(sb));
   HIP_TRY(x->adj_out.ensure(sb));
@@ -836,7 +836,7 @@ int run_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fie
   g.pbar = pbar;
   g.work = pbar + 4 * size_t(ncol);
   g.lvl_on = reinterpret_cast<int*>(g.work + AW_N * plane);
-  g.lvl_loc = g.lvl_on + nz;
+  g.pt_loc = reinterpret_cast<int*>(g.work + AW_N * plane + (size_t(nz) + 1) / 2);
 
   StepArgs<D> a;
   std::memset(&a, 0, sizeof(a));

@@ -512,7 +512,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves
   const int c = pt.x, k = pt.y;
   if (c >= a.ncol || k >= a.nz) return;
   if (a.nlev && k >= a.nlev[c]) return;
-  if (IsDual<R>::value && a.lvl_on && !a.lvl_on[k]) return;  // an adjoint's local pass
+  if (IsDual<R>::value && a.pt_on && !a.pt_on[size_t(c) + size_t(k) * a.ncol]) return;  // an adjoint's local pass
   SigScope<R> scope(a, c, k, 0);
   phase_a_point(a, batch_scr(a), c, k);
 }

@@ -271,11 +271,14 @@ struct StepArgs {
   int first_step;
   // work queue of the fused step (a counter of column groups handed out)
   int* queue;
-  // adjoint passes of the final update (levels whose output cotangents are
-  // all zero) and of a local pass's phase A (levels where, besides, the
-  // sedimentation cotangents are all zero): levels with lvl_on[k] == 0 are
-  // skipped, their contributions being zero; nullptr = every level
+  // adjoint passes of the final update: levels whose output cotangents are
+  // all zero (lvl_on[k] == 0) are skipped, their contributions being zero;
+  // nullptr = every level
   const int* lvl_on;
+  // an adjoint's local passes of phase A: points (c, k) with
+  // pt_on[c + k * ncol] == 0 (no sedimentation cotangent, final update
+  // skipped) are skipped; nullptr = every point
+  const int* pt_on;
   // branch signatures (signature runs only): 6 x nz x ncol words, phase p
   // (0 the level-local physics, 1 the final update, 2 the sedimentation),
   // class q (0 comparisons, 1 discrete outcomes), point (c, k) at
