@@ -96,6 +96,10 @@ __global__ void __launch_bounds__(256) k_adj_capture_merge(StepArgs<D> a, AdjArg
   const int c = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
   if (c >= a.ncol) return;
   const size_t s = TEMPO_IDX_SCR(a, c, k), e = size_t(c) + size_t(k) * a.ncol, n3 = size_t(a.ncol) * a.nz;
+  if (!g.pt_loc[e]) {  // skipped by phase A: zero tangents against zero zbar, no final update
+    xb[e] = 0.0;
+    return;
+  }
   const int sdf[10] = {SD_RR, SD_NR, SD_RG, SD_NG, SD_RB, SD_RS, SD_RI, SD_NI, SD_RC, SD_NC};
   const int srf[9] = {SR_VTRR, SR_VTNR, SR_VTRG, SR_VTNG, SR_VTS, SR_VTRI, SR_VTNI, SR_VTRC, SR_VTNC};
   double acc = 0.0;
