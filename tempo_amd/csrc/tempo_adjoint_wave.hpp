@@ -175,14 +175,21 @@ __global__ void __launch_bounds__(64 * kScrW) k_adj_sed_wave(StepArgs<D> a, AdjA
   q.dt = a.dt;
   auto S = [&](int k) { return TEMPO_IDX_SCR(a, c, k); };
   auto E = [&](int k) { return size_t(c) + size_t(k) * a.ncol; };
-  // zero this column's cotangents
-  for (int z = 0; z < AZ_N; ++z)
+  // this column's cotangents: a plane's first contribution is stored (as
+  // 0 + x, the sum into a zeroed plane), later ones added; the planes and
+  // levels no contribution reaches are zeroed at the end
+  auto zero_z = [&](int z, int k0) {
 #pragma unroll
     for (int j = 0; j < kSedChunks; ++j) {
       const int k = j * 64 + l;
-      if (k < a.nz) g.zbar[size_t(z) * n3 + E(k)] = 0.0;
+      if (k >= k0 && k < a.nz) g.zbar[size_t(z) * n3 + E(k)] = 0.0;
     }
-  if (a.cf[c] & CF_EXIT) return;
+  };
+  if (a.cf[c] & CF_EXIT) {
+    for (int z = 0; z < AZ_N; ++z) zero_z(z, 0);
+    return;
+  }
+  unsigned wrote = 0;  // planes with a stored contribution (wavefront-uniform)
   auto load_sd = [&](int f, double (&x)[kSedChunks]) {
 #pragma unroll
     for (int j = 0; j < kSedChunks; ++j) {
@@ -205,11 +212,16 @@ __global__ void __launch_bounds__(64 * kScrW) k_adj_sed_wave(StepArgs<D> a, AdjA
     }
   };
   auto add_z = [&](int z, const double (&x)[kSedChunks]) {
+    const bool first = !(wrote & (1u << z));
 #pragma unroll
     for (int j = 0; j < kSedChunks; ++j) {
       const int k = j * 64 + l;
-      if (k < q.nzc) g.zbar[size_t(z) * n3 + E(k)] += x[j];
+      if (k < q.nzc) {
+        double* p = g.zbar + size_t(z) * n3 + E(k);
+        *p = first ? 0.0 + x[j] : *p + x[j];
+      }
     }
+    wrote |= 1u << z;
   };
   auto zero = [&](double (&x)[kSedChunks]) {
 #pragma unroll
@@ -447,6 +459,7 @@ __global__ void __launch_bounds__(64 * kScrW) k_adj_sed_wave(StepArgs<D> a, AdjA
     add_z(AZ_RR, rrbar);
   }
   add_z(AZ_RHO, rhob);
+  for (int z = 0; z < AZ_N; ++z) zero_z(z, (wrote & (1u << z)) ? q.nzc : 0);
 }
 
 // All tangent passes and S^T of one adjoint step.  a: dual step arguments
