@@ -102,14 +102,21 @@ __global__ void __launch_bounds__(256) k_adj_capture_merge(StepArgs<D> a, AdjArg
   }
   const int sdf[10] = {SD_RR, SD_NR, SD_RG, SD_NG, SD_RB, SD_RS, SD_RI, SD_NI, SD_RC, SD_NC};
   const int srf[9] = {SR_VTRR, SR_VTNR, SR_VTRG, SR_VTNG, SR_VTS, SR_VTRI, SR_VTNI, SR_VTRC, SR_VTNC};
+  // a tangent is read only where its cotangent is non-zero (a finite
+  // tangent times zero adds nothing)
   double acc = 0.0;
 #pragma unroll
-  for (int q = 0; q < 10; ++q)
-    acc += double(a.sd[size_t(sdf[q]) * a.plane + s].d) * g.zbar[size_t(AZ_RR + q) * n3 + e];
+  for (int q = 0; q < 10; ++q) {
+    const double z = g.zbar[size_t(AZ_RR + q) * n3 + e];
+    if (z != 0.0) acc += double(a.sd[size_t(sdf[q]) * a.plane + s].d) * z;
+  }
 #pragma unroll
-  for (int q = 0; q < 9; ++q)
-    acc += double(a.sr[size_t(srf[q]) * a.plane + s].d) * g.zbar[size_t(AZ_VTRR + q) * n3 + e];
-  acc += double(a.sr[size_t(SR_RHO) * a.plane + s].d) * g.zbar[size_t(AZ_RHO) * n3 + e];
+  for (int q = 0; q < 9; ++q) {
+    const double z = g.zbar[size_t(AZ_VTRR + q) * n3 + e];
+    if (z != 0.0) acc += double(a.sr[size_t(srf[q]) * a.plane + s].d) * z;
+  }
+  const double zr = g.zbar[size_t(AZ_RHO) * n3 + e];
+  if (zr != 0.0) acc += double(a.sr[size_t(SR_RHO) * a.plane + s].d) * zr;
   xb[e] = acc;
   if (a.lvl_on && !a.lvl_on[k]) return;
 #pragma unroll
@@ -179,12 +186,17 @@ __global__ void __launch_bounds__(256) k_adj_contract(StepArgs<D> a, AdjArgs g, 
     if (!accumulate) out[e] = 0.0;
     return;
   }
-  double acc = 0.0;
+  double acc = 0.0;  // tangents read only where their cotangent is non-zero
   for (int q = 0; q < AY_REFL; ++q) {
     const D* f = a.fo[adj_y_field(q)];
-    if (f) acc += g.ybar[size_t(q) * n3 + e] * double(f[o].d);
+    if (!f) continue;
+    const double y = g.ybar[size_t(q) * n3 + e];
+    if (y != 0.0) acc += y * double(f[o].d);
   }
-  if (a.d3[D3_REFL]) acc += g.ybar[size_t(AY_REFL) * n3 + e] * double(a.d3[D3_REFL][o].d);
+  if (a.d3[D3_REFL]) {
+    const double y = g.ybar[size_t(AY_REFL) * n3 + e];
+    if (y != 0.0) acc += y * double(a.d3[D3_REFL][o].d);
+  }
   out[e] = accumulate ? out[e] + acc : acc;
 }
 
