@@ -468,7 +468,8 @@ __global__ void __launch_bounds__(64 * kScrW) k_adj_sed_wave(StepArgs<D> a, AdjA
 // state and scratch after phase A and after phase B of a tangent-free run,
 // so that the later passes restart from them instead of recomputing.
 // scratch snapshots of the adjoint: working-precision and FP64 planes after
-// phase A, FP64 planes after sedimentation
+// phase A (used for nz > 128 only), and the post-sedimentation tendency
+// values
 struct AdjSnap {
   void* sr_a;  // sr_bytes
   void* sd_a;  // sd_bytes
@@ -499,13 +500,20 @@ hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* 
   hipLaunchKernelGGL(k_adj_level_flags<D>, g2, dim3(256), 0, s, a, g);
   auto phase_c = [&]() { hipLaunchKernelGGL(k_phase_c<D>, level_grid(a, 256), dim3(256), 0, s, ac); };
 
-  // tangent-free forward: scratch snapshots after A and after B
+  // tangent-free forward.  Its sedimentation on the values, one wavefront
+  // per column (the tangents are zero in this pass), writes only the ten
+  // tendency planes, which S^T does not read and every later reader finds
+  // rewritten by a local pass's phase A (it writes all planes of each live
+  // point it runs), so the scratch after A needs no snapshot.  The
+  // lane-per-column form (nz > 128) uses state planes as work space: the
+  // scratch after A is snapshotted and restored before S^T.
+  const bool wave_b = a.nz <= kSedLevels;
   phase_pre_a();
-  if ((e = cp(snap.sr_a, a.sr, sr_bytes)) != hipSuccess) return e;
-  if ((e = cp(snap.sd_a, a.sd, sd_bytes)) != hipSuccess) return e;
-  // its sedimentation on the values, one wavefront per column (the tangents
-  // are zero in this pass)
-  if (a.nz <= kSedLevels)
+  if (!wave_b) {
+    if ((e = cp(snap.sr_a, a.sr, sr_bytes)) != hipSuccess) return e;
+    if ((e = cp(snap.sd_a, a.sd, sd_bytes)) != hipSuccess) return e;
+  }
+  if (wave_b)
     hipLaunchKernelGGL(k_phase_b_wave<D>, dim3(unsigned((a.ncol + kSedCols - 1) / kSedCols)), dim3(64 * kSedCols),
                        sed_lds_bytes(a.nz), s, a);
   else
@@ -530,9 +538,11 @@ hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* 
     m_prev = m;
   }
   // S^T on the values after phase A
-  if ((e = cp(a.sr, snap.sr_a, sr_bytes)) != hipSuccess) return e;
-  if ((e = cp(a.sd, snap.sd_a, sd_bytes)) != hipSuccess) return e;
-  if (a.nz <= kSedLevels)
+  if (!wave_b) {
+    if ((e = cp(a.sr, snap.sr_a, sr_bytes)) != hipSuccess) return e;
+    if ((e = cp(a.sd, snap.sd_a, sd_bytes)) != hipSuccess) return e;
+  }
+  if (wave_b)
     hipLaunchKernelGGL(k_adj_sed_wave<D>, dim3(unsigned((a.ncol + kScrW - 1) / kScrW)), dim3(64 * kScrW), 0, s, a, g);
   else
     hipLaunchKernelGGL(k_adj_sedimentation<D>, dim3(unsigned((a.ncol + 63) / 64)), dim3(64), 0, s, a, g);
