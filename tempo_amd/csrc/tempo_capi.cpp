@@ -269,7 +269,7 @@ struct tempo_hip_ctx {
   DevBuf slots, queue;  // the fused step's block-private scratch slots and its work counter
   DevBuf tl_state, tl_diag;  // dual (value, tangent) copies for the tangent-linear step
   DevBuf adj_in, adj_out, adj_mem;  // adjoint: packed inputs, new state, cotangent/Jacobian planes
-  DevBuf adj_snap;                  // adjoint: scratch after phases A and B
+  DevBuf adj_snap;                  // adjoint: post-sedimentation tendencies (and scratch after A)
   int last_ncol = 0;
   void* last_ksub = nullptr;  // substep counts of the last run (its scratch)
   void* last_vt = nullptr;    // its wide sedimentation planes
@@ -882,10 +882,13 @@ int run_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fie
   if (xb) field_ptrs(xb, xp);
   bool have_x[AX_N];
   for (int i = 0; i < AX_N; ++i) have_x[i] = fp[adj_x_field(i)] && xp[adj_x_field(i)];
+  // snapshots: the post-sedimentation tendency values (10 FP64 planes), and
+  // for the lane-per-column sedimentation (nz > 128) the scratch after A
   const size_t srb = SR_N * a.plane * sizeof(D), sdb = SD_N * a.plane * sizeof(Dual<double>);
-  HIP_TRY(x->adj_snap.ensure(srb + 2 * sdb));
+  const bool wave_b = nz <= kSedLevels;
+  HIP_TRY(x->adj_snap.ensure(wave_b ? 10 * a.plane * sizeof(double) : srb + 2 * sdb));
   char* sp = static_cast<char*>(x->adj_snap.p);
-  const AdjSnap snap{sp, sp + srb, sp + srb + sdb};
+  const AdjSnap snap = wave_b ? AdjSnap{nullptr, nullptr, sp} : AdjSnap{sp, sp + srb, sp + srb + sdb};
   // out of place: inputs in adj_in, post-check state in tl_state, new state in adj_out
   D* out = x->adj_out.as<D>();
   for (int i = 0; i < F_NFIELDS; ++i) {
