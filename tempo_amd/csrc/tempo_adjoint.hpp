@@ -83,16 +83,13 @@ __global__ void __launch_bounds__(256) k_adj_keep_post(StepArgs<D> a, double* __
 
 // after phase A of a local pass: the S^T part of xbar_i, (dz/dx_i) . zbar,
 // from the tangents of the sedimentation inputs (masses, numbers, raw fall
-// speeds, density) and the cotangents zbar that S^T has already produced;
-// then the values of the pre-sedimentation tendencies are replaced by the
-// forward's post-sedimentation ones (k_adj_keep_post), keeping their local
-// tangents (S frozen out), at the levels whose final update runs (a.lvl_on;
-// the next pass's phase A rewrites the planes).  The final update reads
-// neither the sedimenting masses nor the fall speeds, so their tangents need
-// not be cleared.
+// speeds, density) and the cotangents zbar that S^T has already produced.
+// The final update then reads the forward's post-sedimentation tendency
+// values (k_adj_keep_post, a.post) with the local tangents phase A left in
+// the scratch (S frozen out); it reads neither the sedimenting masses nor
+// the fall speeds, so their tangents need not be cleared.
 template <class D>
-__global__ void __launch_bounds__(256) k_adj_capture_merge(StepArgs<D> a, AdjArgs g, double* __restrict__ xb,
-                                                           const double* __restrict__ post) {
+__global__ void __launch_bounds__(256) k_adj_capture(StepArgs<D> a, AdjArgs g, double* __restrict__ xb) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
   if (c >= a.ncol) return;
   const size_t s = TEMPO_IDX_SCR(a, c, k), e = size_t(c) + size_t(k) * a.ncol, n3 = size_t(a.ncol) * a.nz;
@@ -118,9 +115,6 @@ __global__ void __launch_bounds__(256) k_adj_capture_merge(StepArgs<D> a, AdjArg
   const double zr = g.zbar[size_t(AZ_RHO) * n3 + e];
   if (zr != 0.0) acc += double(a.sr[size_t(SR_RHO) * a.plane + s].d) * zr;
   xb[e] = acc;
-  if (a.lvl_on && !a.lvl_on[k]) return;
-#pragma unroll
-  for (int m = 0; m < 10; ++m) a.sd[size_t(SD_QRTEN + m) * a.plane + s].v = post[size_t(m) * a.plane + s];
 }
 
 // seed the tangent of one post-sedimentation tendency plane (u pass),

@@ -546,9 +546,11 @@ hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* 
     hipLaunchKernelGGL(k_adj_sed_wave<D>, dim3(unsigned((a.ncol + kScrW - 1) / kScrW)), dim3(64 * kScrW), 0, s, a, g);
   else
     hipLaunchKernelGGL(k_adj_sedimentation<D>, dim3(unsigned((a.ncol + 63) / 64)), dim3(64), 0, s, a, g);
-  // the local passes' phase A runs at the points where they contribute
-  StepArgs<D> al = a;
+  // the local passes' phase A runs at the points where they contribute,
+  // their final update on the post-sedimentation tendency values
+  StepArgs<D> al = a, acl = ac;
   al.pt_on = g.pt_loc;
+  acl.post = post;
   hipLaunchKernelGGL(k_adj_point_flags<D>, g2, dim3(256), 0, s, a, g);
   // local passes, one per requested input field: xbar_i = (dz/dx_i) . zbar
   // + the local part through A and C, phase B replaced by the forward's
@@ -560,8 +562,8 @@ hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* 
     double* xb = g.xbar + size_t(i) * n3;
     hipLaunchKernelGGL(k_adj_seed_input<D>, g2, dim3(256), 0, s, a, adj_x_field(i), prev < 0 ? -1 : adj_x_field(prev));
     hipLaunchKernelGGL(k_phase_a<D>, level_grid(a, 256), dim3(256), 0, s, al);
-    hipLaunchKernelGGL(k_adj_capture_merge<D>, g2, dim3(256), 0, s, ac, g, xb, post);
-    phase_c();
+    hipLaunchKernelGGL(k_adj_capture<D>, g2, dim3(256), 0, s, a, g, xb);
+    hipLaunchKernelGGL(k_phase_c<D>, level_grid(a, 256), dim3(256), 0, s, acl);
     hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, ac, g, xb, 1);
     prev = i;
   }

@@ -1208,16 +1208,25 @@ __device__ __forceinline__ void phase_c_point(const StepArgs<R>& a, const Scr<R>
   L.qvten = sr[SR_QVTEN * plane + s];
   L.nwfaten = sr[SR_NWFATEN * plane + s];
   L.nifaten = sr[SR_NIFATEN * plane + s];
-  L.qrten = R(sd[SD_QRTEN * plane + s]);
-  L.nrten = R(sd[SD_NRTEN * plane + s]);
-  L.qgten = R(sd[SD_QGTEN * plane + s]);
-  L.ngten = R(sd[SD_NGTEN * plane + s]);
-  L.qbten = R(sd[SD_QBTEN * plane + s]);
-  L.qsten = R(sd[SD_QSTEN * plane + s]);
-  L.qiten = R(sd[SD_QITEN * plane + s]);
-  L.niten = R(sd[SD_NITEN * plane + s]);
-  L.qcten = R(sd[SD_QCTEN * plane + s]);
-  L.ncten = R(sd[SD_NCTEN * plane + s]);
+  // the post-sedimentation tendencies; an adjoint's local pass takes their
+  // values from a.post and their (local) tangents from the scratch
+  auto ten = [&](int m) {
+    const Sd_t<R>& x = sd[size_t(SD_QRTEN + m) * plane + s];
+    if constexpr (IsDual<R>::value) {
+      if (a.post) return R(Sd_t<R>(a.post[size_t(m) * plane + s], x.d));
+    }
+    return R(x);
+  };
+  L.qrten = ten(0);
+  L.nrten = ten(1);
+  L.qgten = ten(2);
+  L.ngten = ten(3);
+  L.qbten = ten(4);
+  L.qsten = ten(5);
+  L.qiten = ten(6);
+  L.niten = ten(7);
+  L.qcten = ten(8);
+  L.ncten = ten(9);
   L.pres = a.f[F_P][o];
   const R lvap = R(kLvap0) + R(2106.0 - 4218.0) * (L.temp - R(kT0));
   const R ocp = R(1.0) / (R(kCp) * (R(1.0) + R(0.887) * L.qv));

@@ -279,6 +279,10 @@ struct StepArgs {
   // pt_on[c + k * ncol] == 0 (no sedimentation cotangent, final update
   // skipped) are skipped; nullptr = every point
   const int* pt_on;
+  // an adjoint's local passes of the final update: the post-sedimentation
+  // tendency values, 10 planes (SD_QRTEN order) at post[m * plane + s],
+  // replacing the scratch's values; nullptr = the scratch's values
+  const double* post;
   // branch signatures (signature runs only): 6 x nz x ncol words, phase p
   // (0 the level-local physics, 1 the final update, 2 the sedimentation),
   // class q (0 comparisons, 1 discrete outcomes), point (c, k) at
