@@ -310,7 +310,9 @@ int tempo_hip_substeps(tempo_hip_ctx *ctx, int *host_out, size_t n);
  *                               pulls column groups from a work queue and
  *                               keeps their scratch in the caches (for
  *                               flux-form sedimentation on at most 128 levels
- *                               without the melting-layer reflectivity),
+ *                               without the melting-layer reflectivity) --
+ *                               currently switched off: 1 returns
+ *                               TEMPO_HIP_E_UNSUPPORTED;
  *                               0: four launches over the batch (default)
  *   TEMPO_HIP_TUNE_ORDER     point grouping of the level-local kernels, a
  *                               bit mask: bit 0 (level-local physics), bit 1

@@ -1635,6 +1635,11 @@ int tempo_hip_tune(tempo_hip_ctx* x, int precision, int key, int value) {
       return TEMPO_HIP_OK;
     case TEMPO_HIP_TUNE_FUSED:
       if (value != 0 && value != 1) return fail(TEMPO_HIP_E_ARG, "fused must be 0 or 1");
+      // the persistent launch is switched off: its equality test stopped
+      // finishing after a change to the final update's code generation, and
+      // the cause is not known
+      if (value == 1)
+        return fail(TEMPO_HIP_E_UNSUPPORTED, "the fused step is disabled (a test of it stopped finishing; under study)");
       lc.fused = value;
       return TEMPO_HIP_OK;
     case TEMPO_HIP_TUNE_WAVES_A:

@@ -1078,6 +1078,9 @@ def test_gpu_launch_shapes_give_identical_results(hip, dtype):
         print(wl, dtype, "max substeps", base[2][0::2].max(), "identical over", len(results), "settings")
 
 
+@pytest.mark.skip(reason="the fused step is switched off: this test stopped finishing (killed after 180 s without "
+                         "output, FP32 aerosol case) once the final update's tendency loads changed its code "
+                         "generation; the cause is not found")
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 @pytest.mark.parametrize("workload", ["aerosol", "ragged", "multistep"])
 def test_gpu_fused_step_equals_four_launches(hip, workload, dtype):
@@ -1115,6 +1118,9 @@ def test_gpu_tune_rejects_bad_values(hip):
         with pytest.raises(hip.TempoError) as e:
             ctx.tune(hip.FP32, key, val)
         assert e.value.code == hip.E_ARG
+    with pytest.raises(hip.TempoError) as e:  # the fused step is switched off
+        ctx.tune(hip.FP32, hip.TUNE_FUSED, 1)
+    assert e.value.code == hip.E_UNSUPPORTED
 
 
 def test_gpu_surface_emissions_matches_reference_fixture(hip):
