@@ -513,6 +513,60 @@ __global__ void __launch_bounds__(256) k_from_compact(const double* __restrict__
 }
 
 
+// several arrays of one layout in one launch (blockIdx.y: the array)
+constexpr int kBatchMax = 24;
+template <class S, class D>
+struct PlaneBatch {
+  const S* src[kBatchMax];
+  D* dst[kBatchMax];
+};
+template <class T>
+__global__ void __launch_bounds__(256) k_to_compact_n(PlaneBatch<T, double> b, DualGeom g) {
+  const size_t ncol = size_t(g.ni) * g.nj;
+  const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= ncol * size_t(g.nk)) return;
+  const T* src = b.src[blockIdx.y];
+  const size_t c = e % ncol, k = e / ncol;
+  b.dst[blockIdx.y][e] = src ? double(src[(c % g.ni) + k * g.ldk + (c / g.ni) * g.ldj]) : 0.0;
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_from_compact_n(PlaneBatch<double, T> b, DualGeom g) {
+  const size_t ncol = size_t(g.ni) * g.nj;
+  const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= ncol * size_t(g.nk)) return;
+  const size_t c = e % ncol, k = e / ncol;
+  b.dst[blockIdx.y][(c % g.ni) + k * g.ldk + (c / g.ni) * g.ldj] = T(b.src[blockIdx.y][e]);
+}
+// values into compact duals with zero tangents
+template <class T>
+__global__ void __launch_bounds__(256) k_dual_pack_n(PlaneBatch<T, Dual<T>> b, DualGeom g) {
+  const size_t ncol = size_t(g.ni) * g.nj;
+  const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= ncol * size_t(g.nk)) return;
+  const size_t c = e % ncol, k = e / ncol;
+  b.dst[blockIdx.y][e] = Dual<T>(b.src[blockIdx.y][(c % g.ni) + k * g.ldk + (c / g.ni) * g.ldj], T(0));
+}
+template <class K, class B>
+hipError_t launch_batch(K kernel, const B& b, int n, const DualGeom& g, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (n > kBatchMax) return hipErrorInvalidValue;
+  const size_t m = size_t(g.ni) * g.nj * g.nk;
+  hipLaunchKernelGGL(kernel, dim3(unsigned((m + 255) / 256), unsigned(n)), dim3(256), 0, s, b, g);
+  return hipGetLastError();
+}
+template <class T>
+hipError_t launch_to_compact_n(const PlaneBatch<T, double>& b, int n, const DualGeom& g, hipStream_t s) {
+  return launch_batch(k_to_compact_n<T>, b, n, g, s);
+}
+template <class T>
+hipError_t launch_from_compact_n(const PlaneBatch<double, T>& b, int n, const DualGeom& g, hipStream_t s) {
+  return launch_batch(k_from_compact_n<T>, b, n, g, s);
+}
+template <class T>
+hipError_t launch_dual_pack_n(const PlaneBatch<T, Dual<T>>& b, int n, const DualGeom& g, hipStream_t s) {
+  return launch_batch(k_dual_pack_n<T>, b, n, g, s);
+}
+
 template <class T>
 hipError_t launch_to_compact(const T* src, const DualGeom& g, double* dst, hipStream_t s) {
   const size_t n = size_t(g.ni) * g.nj * g.nk;

@@ -49,6 +49,12 @@ template <class T>
 hipError_t launch_to_compact(const T* src, const DualGeom& g, double* dst, hipStream_t s);
 template <class T>
 hipError_t launch_from_compact(const double* src, const DualGeom& g, T* dst, hipStream_t s);
+template <class T>
+hipError_t launch_to_compact_n(const PlaneBatch<T, double>& b, int n, const DualGeom& g, hipStream_t s);
+template <class T>
+hipError_t launch_from_compact_n(const PlaneBatch<double, T>& b, int n, const DualGeom& g, hipStream_t s);
+template <class T>
+hipError_t launch_dual_pack_n(const PlaneBatch<T, Dual<T>>& b, int n, const DualGeom& g, hipStream_t s);
 hipError_t launch_table_qg(const double*, const double*, const double*, const double*, double*, hipStream_t);
 hipError_t launch_table_qs(const double*, const double*, const double*, const double*, double*, hipStream_t);
 hipError_t launch_table_fz(const double*, const double*, const double*, const double*, double, const double*,
@@ -845,25 +851,37 @@ int run_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fie
   D* in = x->adj_in.as<D>();
   D* st = x->tl_state.as<D>();
   const DualGeom g3{dm->ni, dm->nj, nz, dm->ldk, dm->ldj};
+  // the inputs, the output cotangents and the precipitation cotangents, one
+  // launch per layout
+  PlaneBatch<T, Dual<T>> pk{};
+  int npk = 0;
   for (int i = 0; i < F_NFIELDS; ++i) {
     if (!fp[i]) continue;
-    HIP_TRY(launch_dual_pack(static_cast<const T*>(fp[i]), static_cast<const T*>(nullptr), g3, in + size_t(i) * n3, s));
+    pk.src[npk] = static_cast<const T*>(fp[i]);
+    pk.dst[npk++] = in + size_t(i) * n3;
     a.f[i] = st + size_t(i) * n3;
   }
-  // output cotangents
+  HIP_TRY(launch_dual_pack_n(pk, npk, g3, s));
   void* yp[F_NFIELDS] = {};
   if (yb) field_ptrs(yb, yp);
-  for (int q = 0; q < AY_REFL; ++q)
-    HIP_TRY(launch_to_compact(static_cast<const T*>(a.f[adj_y_field(q)] ? yp[adj_y_field(q)] : nullptr), g3,
-                              ybar + size_t(q) * n3, s));
+  PlaneBatch<T, double> yc{};
+  for (int q = 0; q < AY_REFL; ++q) {
+    yc.src[q] = static_cast<const T*>(a.f[adj_y_field(q)] ? yp[adj_y_field(q)] : nullptr);
+    yc.dst[q] = ybar + size_t(q) * n3;
+  }
+  HIP_TRY(launch_to_compact_n(yc, AY_REFL, g3, s));
   void *d2[D2_N] = {}, *d3[D3_N] = {};
   if (db) diag_ptrs(db, d2, d3);
   const DualGeom gd3{dm->ni, dm->nj, nz, dm->dldk, dm->dldj};
   const DualGeom g2{dm->ni, dm->nj, 1, 0, dm->d2ldj};
   HIP_TRY(launch_to_compact(static_cast<const T*>(d3[D3_REFL]), gd3, ybar + size_t(AY_REFL) * n3, s));
   const int pmap[4] = {D2_RAIN, D2_GRAUPEL, D2_SNOW, D2_ICE};  // S^T order: rain, graupel, snow, ice
-  for (int q = 0; q < 4; ++q)
-    HIP_TRY(launch_to_compact(static_cast<const T*>(d2[pmap[q]]), g2, pbar + size_t(q) * ncol, s));
+  PlaneBatch<T, double> pc{};
+  for (int q = 0; q < 4; ++q) {
+    pc.src[q] = static_cast<const T*>(d2[pmap[q]]);
+    pc.dst[q] = pbar + size_t(q) * ncol;
+  }
+  HIP_TRY(launch_to_compact_n(pc, 4, g2, s));
   a.d3[D3_REFL] = x->tl_diag.as<D>() + D2_N * size_t(ncol);
   a.land = f->land;
   a.nlev = dm->nlev;
@@ -897,9 +915,14 @@ int run_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fie
     a.fo[i] = fp[i] ? out + size_t(i) * n3 : nullptr;
   }
   HIP_TRY(launch_adjoint(a, g, x->vtbuf.as<Dual<double>>(), srb, sdb, snap, have_x, s));
+  PlaneBatch<double, T> xc{};
+  int nxc = 0;
   for (int i = 0; i < AX_N; ++i)
-    if (have_x[i])
-      HIP_TRY(launch_from_compact(g.xbar + size_t(i) * n3, g3, static_cast<T*>(xp[adj_x_field(i)]), s));
+    if (have_x[i]) {
+      xc.src[nxc] = g.xbar + size_t(i) * n3;
+      xc.dst[nxc++] = static_cast<T*>(xp[adj_x_field(i)]);
+    }
+  HIP_TRY(launch_from_compact_n(xc, nxc, g3, s));
   return TEMPO_HIP_OK;
 }
 

@@ -40,5 +40,8 @@ hipError_t launch_adjoint(const StepArgs<Dual<float>>& a, const AdjArgs& g, Dual
 }
 template hipError_t launch_to_compact<float>(const float*, const DualGeom&, double*, hipStream_t);
 template hipError_t launch_from_compact<float>(const double*, const DualGeom&, float*, hipStream_t);
+template hipError_t launch_to_compact_n<float>(const PlaneBatch<float, double>&, int, const DualGeom&, hipStream_t);
+template hipError_t launch_from_compact_n<float>(const PlaneBatch<double, float>&, int, const DualGeom&, hipStream_t);
+template hipError_t launch_dual_pack_n<float>(const PlaneBatch<float, Dual<float>>&, int, const DualGeom&, hipStream_t);
 
 }  // namespace tempo

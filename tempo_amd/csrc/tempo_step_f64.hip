@@ -40,5 +40,8 @@ hipError_t launch_adjoint(const StepArgs<Dual<double>>& a, const AdjArgs& g, Dua
 }
 template hipError_t launch_to_compact<double>(const double*, const DualGeom&, double*, hipStream_t);
 template hipError_t launch_from_compact<double>(const double*, const DualGeom&, double*, hipStream_t);
+template hipError_t launch_to_compact_n<double>(const PlaneBatch<double, double>&, int, const DualGeom&, hipStream_t);
+template hipError_t launch_from_compact_n<double>(const PlaneBatch<double, double>&, int, const DualGeom&, hipStream_t);
+template hipError_t launch_dual_pack_n<double>(const PlaneBatch<double, Dual<double>>&, int, const DualGeom&, hipStream_t);
 
 }  // namespace tempo
