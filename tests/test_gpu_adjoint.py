@@ -234,6 +234,47 @@ def test_adjoint_dot_product(ctx, sample, dtype, tol, sparse):
     assert abs(lhs - rhs) <= tol * abs(lhs)
 
 
+def refine(st, nz_new):
+    """the columns interpolated in level index onto nz_new levels, layer
+    depths scaled so that each column keeps its height"""
+    nz, ncol = st["p"].shape
+    x_old = np.arange(nz, dtype=np.float64)
+    x_new = np.linspace(0.0, nz - 1.0, nz_new)
+    out = {}
+    for k, v in st.items():
+        r = np.empty((nz_new, ncol))
+        for c in range(ncol):
+            r[:, c] = np.interp(x_new, x_old, v[:, c])
+        out[k] = r
+    out["dz"] = out["dz"] * (nz / nz_new)
+    return {k: np.ascontiguousarray(v) for k, v in out.items()}
+
+
+def test_adjoint_dot_product_above_128_levels(ctx, sample):
+    """<ybar, J v> = <J^T ybar, v> on 150 levels: the sedimentation runs one
+    lane per column there (the wavefront kernels hold at most 128 levels),
+    in the TLM and in the adjoint's forward, S^T and scratch snapshots"""
+    st0, m = sample
+    st = refine({k: v[:, :64] for k, v in st0.items()}, 150)
+    rng = np.random.default_rng(19)
+    v = direction(st, 23)
+    nz, ncol = st["p"].shape
+    ybar = {k: torch.tensor(rng.standard_normal(st[k].shape) / (np.abs(st[k]).max() + 1e-30), dtype=torch.float64,
+                            device="cuda") for k in ("t", "qv", "qc", "qr", "qi", "qs", "qg", "ni", "nr", "nc", "nwfa",
+                                                     "nifa") if k in st}
+    dbar = {"refl10cm": torch.tensor(rng.standard_normal((nz, ncol)) * 0.01, device="cuda")}
+    for k in ("rain_precip", "ice_liquid_equiv_precip", "snow_liquid_equiv_precip", "graupel_liquid_equiv_precip"):
+        dbar[k] = torch.tensor(rng.standard_normal(ncol) * 1e3, device="cuda")
+    _, dtl, tb = ctx.tlm(dev(st), {k: torch.tensor(x, device="cuda") for k, x in v.items()}, m["dt"], m["itimestep"])
+    lhs = sum(float((ybar[k] * tb[k].double()).sum()) for k in ybar)
+    lhs += sum(float((dbar[k] * dtl[k].double()).sum()) for k in dbar)
+    xbar = ctx.adjoint(dev(st), ybar, dbar, m["dt"], m["itimestep"])
+    torch.cuda.synchronize()
+    rhs = sum(float((xbar[k].double() * torch.tensor(v[k], device="cuda")).sum()) for k in v)
+    print("150 levels: dot test", lhs, rhs, abs(lhs - rhs) / abs(lhs))
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+
+
 def loss_weights(st, seed=3):
     rng = np.random.default_rng(seed)
     nz, ncol = st["p"].shape
