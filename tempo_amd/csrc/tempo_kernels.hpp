@@ -13,9 +13,10 @@ namespace tempo {
 // is measured faster.  Tighter budgets spill heavily and are slower.  The
 // FP64 level physics is held to three waves (168 VGPRs, a few hundred
 // bytes of spills per lane), measured faster than two waves without
-// spills.  The dual instantiations keep the compiler's choice.  One wave
-// fewer (a_alt: 3 in FP32, 2 in FP64) is a tuning option of phase A
-// (TEMPO_HIP_TUNE_WAVES_A).
+// spills.  The dual instantiations keep the compiler's choice (the FP32
+// dual phase A takes 244 VGPRs, two waves; held to three waves it spilled
+// and made the adjoint 7-13 % slower).  One wave fewer (a_alt: 3 in FP32,
+// 2 in FP64) is a tuning option of phase A (TEMPO_HIP_TUNE_WAVES_A).
 template <class R>
 struct WavesPerEu {
   static constexpr int a = 1, a_alt = 1, c = 1;
