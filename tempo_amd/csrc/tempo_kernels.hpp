@@ -15,7 +15,8 @@ namespace tempo {
 // bytes of spills per lane), measured faster than two waves without
 // spills.  The dual instantiations keep the compiler's choice (the FP32
 // dual phase A takes 244 VGPRs, two waves; held to three waves it spilled
-// and made the adjoint 7-13 % slower).  One wave fewer (a_alt: 3 in FP32,
+// and made the adjoint 7-13 % slower; its final update at four waves
+// instead of three gained nothing).  One wave fewer (a_alt: 3 in FP32,
 // 2 in FP64) is a tuning option of phase A (TEMPO_HIP_TUNE_WAVES_A).
 template <class R>
 struct WavesPerEu {
