@@ -169,29 +169,47 @@ __global__ void __launch_bounds__(256) k_adj_point_flags(StepArgs<D> a, AdjArgs 
   g.pt_loc[e] = on ? 1 : 0;
 }
 
+// the seed of the next pass, moved by the contraction that ends a pass
+// (the seeding runs on the same grid): kind 1 an input field's tangent
+// (a.f), kind 2 a post-sedimentation tendency plane's (scratch); the
+// field or plane `clear` loses its seed, `set` gets one (< 0: none)
+struct AdjSeed {
+  int kind, set, clear;
+};
+
 // out[e] (+)= sum over outputs of ybar * tangent (0 at a level without
-// output cotangents, whose final update a.lvl_on skipped)
+// output cotangents, whose final update a.lvl_on skipped); then the next
+// pass's seed at this point
 template <class D>
-__global__ void __launch_bounds__(256) k_adj_contract(StepArgs<D> a, AdjArgs g, double* out, int accumulate) {
+__global__ void __launch_bounds__(256) k_adj_contract(StepArgs<D> a, AdjArgs g, double* out, int accumulate,
+                                                      AdjSeed nx) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
   if (c >= a.ncol) return;
   const size_t o = TEMPO_IDX_STATE(a, c, k), e = size_t(c) + size_t(k) * a.ncol, n3 = size_t(a.ncol) * a.nz;
   if (a.lvl_on && !a.lvl_on[k]) {
     if (!accumulate) out[e] = 0.0;
-    return;
+  } else {
+    double acc = 0.0;  // tangents read only where their cotangent is non-zero
+    for (int q = 0; q < AY_REFL; ++q) {
+      const D* f = a.fo[adj_y_field(q)];
+      if (!f) continue;
+      const double y = g.ybar[size_t(q) * n3 + e];
+      if (y != 0.0) acc += y * double(f[o].d);
+    }
+    if (a.d3[D3_REFL]) {
+      const double y = g.ybar[size_t(AY_REFL) * n3 + e];
+      if (y != 0.0) acc += y * double(a.d3[D3_REFL][o].d);
+    }
+    out[e] = accumulate ? out[e] + acc : acc;
   }
-  double acc = 0.0;  // tangents read only where their cotangent is non-zero
-  for (int q = 0; q < AY_REFL; ++q) {
-    const D* f = a.fo[adj_y_field(q)];
-    if (!f) continue;
-    const double y = g.ybar[size_t(q) * n3 + e];
-    if (y != 0.0) acc += y * double(f[o].d);
+  if (nx.kind == 1) {
+    if (nx.clear >= 0 && a.f[nx.clear]) a.f[nx.clear][o].d = 0;
+    if (nx.set >= 0 && a.f[nx.set]) a.f[nx.set][o].d = 1;
+  } else if (nx.kind == 2) {
+    const size_t s = TEMPO_IDX_SCR(a, c, k);
+    if (nx.clear >= 0) a.sd[size_t(SD_QRTEN + nx.clear) * a.plane + s].d = 0;
+    if (nx.set >= 0) a.sd[size_t(SD_QRTEN + nx.set) * a.plane + s].d = 1;
   }
-  if (a.d3[D3_REFL]) {
-    const double y = g.ybar[size_t(AY_REFL) * n3 + e];
-    if (y != 0.0) acc += y * double(a.d3[D3_REFL][o].d);
-  }
-  out[e] = accumulate ? out[e] + acc : acc;
 }
 
 

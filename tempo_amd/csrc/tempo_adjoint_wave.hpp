@@ -524,18 +524,22 @@ hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* 
   // reads the inputs, a.fa and the scratch, and writes only a.fo).  Without
   // prognostic graupel number and volume the final update does not read
   // their tendencies, so those cotangents are zero and need no pass.
+  // Each pass's contraction moves the seed on to the next pass.
   const bool ngqb = a.f[F_NG] && a.f[F_QB];
-  int m_prev = -1;
+  int ms[10], nm = 0;
   for (int m = 0; m < 10; ++m) {
-    double* ub = g.ubar + size_t(m) * n3;
     if (!ngqb && (SD_QRTEN + m == SD_NGTEN || SD_QRTEN + m == SD_QBTEN)) {
+      double* ub = g.ubar + size_t(m) * n3;
       hipLaunchKernelGGL(k_adj_zero<double>, dim3(unsigned((n3 + 255) / 256)), dim3(256), 0, s, ub, n3);
       continue;
     }
-    hipLaunchKernelGGL(k_adj_seed_ten<D>, g2, dim3(256), 0, s, a, m, m_prev);
+    ms[nm++] = m;
+  }
+  if (nm > 0) hipLaunchKernelGGL(k_adj_seed_ten<D>, g2, dim3(256), 0, s, a, ms[0], -1);
+  for (int j = 0; j < nm; ++j) {
     phase_c();
-    hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, ac, g, ub, 0);
-    m_prev = m;
+    const AdjSeed nx{2, j + 1 < nm ? ms[j + 1] : -1, j + 1 < nm ? ms[j] : -1};
+    hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, ac, g, g.ubar + size_t(ms[j]) * n3, 0, nx);
   }
   // S^T on the values after phase A
   if (!wave_b) {
@@ -556,16 +560,17 @@ hipError_t adjoint_passes(const StepArgs<D>& a, const AdjArgs& g, Dual<double>* 
   // + the local part through A and C, phase B replaced by the forward's
   // post-sedimentation tendency values.  The pre-pass results (column flags,
   // heights) depend on values only and stay from the first pass.
-  int prev = -1;
-  for (int i = 0; i < AX_N; ++i) {
-    if (!have_x[i]) continue;
-    double* xb = g.xbar + size_t(i) * n3;
-    hipLaunchKernelGGL(k_adj_seed_input<D>, g2, dim3(256), 0, s, a, adj_x_field(i), prev < 0 ? -1 : adj_x_field(prev));
+  int xs[AX_N], nxs = 0;
+  for (int i = 0; i < AX_N; ++i)
+    if (have_x[i]) xs[nxs++] = i;
+  if (nxs > 0) hipLaunchKernelGGL(k_adj_seed_input<D>, g2, dim3(256), 0, s, a, adj_x_field(xs[0]), -1);
+  for (int j = 0; j < nxs; ++j) {
+    double* xb = g.xbar + size_t(xs[j]) * n3;
     hipLaunchKernelGGL(k_phase_a<D>, level_grid(a, 256), dim3(256), 0, s, al);
     hipLaunchKernelGGL(k_adj_capture<D>, g2, dim3(256), 0, s, a, g, xb);
     hipLaunchKernelGGL(k_phase_c<D>, level_grid(a, 256), dim3(256), 0, s, acl);
-    hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, ac, g, xb, 1);
-    prev = i;
+    const AdjSeed nx{1, j + 1 < nxs ? adj_x_field(xs[j + 1]) : -1, j + 1 < nxs ? adj_x_field(xs[j]) : -1};
+    hipLaunchKernelGGL(k_adj_contract<D>, g2, dim3(256), 0, s, ac, g, xb, 1, nx);
   }
   return hipGetLastError();
 }
