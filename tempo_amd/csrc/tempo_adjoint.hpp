@@ -179,24 +179,29 @@ struct AdjSeed {
 
 // out[e] (+)= sum over outputs of ybar * tangent (0 at a level without
 // output cotangents, whose final update a.lvl_on skipped); then the next
-// pass's seed at this point
+// pass's seed at this point.  A point the step leaves as it is (an
+// early-exit column, a level at or above the column's nlev) has no final
+// update writing its new state: its output tangents are its input tangents,
+// and its reflectivity is no function of the inputs (a constant in an
+// early-exit column, left as it is above nlev).
 template <class D>
 __global__ void __launch_bounds__(256) k_adj_contract(StepArgs<D> a, AdjArgs g, double* out, int accumulate,
                                                       AdjSeed nx) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
   if (c >= a.ncol) return;
   const size_t o = TEMPO_IDX_STATE(a, c, k), e = size_t(c) + size_t(k) * a.ncol, n3 = size_t(a.ncol) * a.nz;
+  const bool kept = (a.cf[c] & CF_EXIT) || (a.nlev && k >= a.nlev[c]);
   if (a.lvl_on && !a.lvl_on[k]) {
     if (!accumulate) out[e] = 0.0;
   } else {
     double acc = 0.0;  // tangents read only where their cotangent is non-zero
     for (int q = 0; q < AY_REFL; ++q) {
-      const D* f = a.fo[adj_y_field(q)];
+      const D* f = kept ? a.f[adj_y_field(q)] : a.fo[adj_y_field(q)];
       if (!f) continue;
       const double y = g.ybar[size_t(q) * n3 + e];
       if (y != 0.0) acc += y * double(f[o].d);
     }
-    if (a.d3[D3_REFL]) {
+    if (a.d3[D3_REFL] && !kept) {
       const double y = g.ybar[size_t(AY_REFL) * n3 + e];
       if (y != 0.0) acc += y * double(a.d3[D3_REFL][o].d);
     }

@@ -10,7 +10,9 @@
     threshold within the perturbation on the other side, where no
     derivative exists.
   * Adjoint vs tangent-linear (dot-product test <ybar, J v> = <J^T ybar, v>):
-    relative 1e-10 in FP64, 1e-4 in FP32.
+    relative 1e-10 in FP64, 1e-4 in FP32; also with cotangents at three
+    levels only, on 150 levels, and on tiny grids with an early-exit column
+    and with levels above a column's nlev (where the step is the identity).
   * torch.autograd gradient of the loss (weighted surface precipitation +
     reflectivity at levels 5, 20, 40) vs central finite differences of the
     reference FP64 build along a random direction: relative 1e-5.
@@ -273,6 +275,40 @@ def test_adjoint_dot_product_above_128_levels(ctx, sample):
     rhs = sum(float((xbar[k].double() * torch.tensor(v[k], device="cuda")).sum()) for k in v)
     print("150 levels: dot test", lhs, rhs, abs(lhs - rhs) / abs(lhs))
     assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+
+
+@pytest.mark.parametrize("ncol,nz,nlev", [(3, 5, None), (1, 2, None), (3, 5, (5, 3, 2))])
+def test_adjoint_dot_product_tiny_grids(ctx, sample, ncol, nz, nlev):
+    """the dot-product test on a partial wavefront of columns and a few
+    levels (the per-level and per-point flags, S^T and the seeding on grids
+    smaller than one block).  The single column on two levels holds no
+    hydrometeor and takes the early exit; with nlev the levels at and above
+    each column's count are left as they are.  At both the step is the
+    identity on the state, which the adjoint must return."""
+    st0, m = sample
+    cols = [5, 77, 140][:ncol]
+    st = refine({k: v[:, cols] for k, v in st0.items()}, nz)
+    rng = np.random.default_rng(29)
+    v = direction(st, 31)
+    ybar = {k: torch.tensor(rng.standard_normal(st[k].shape) / (np.abs(st[k]).max() + 1e-30), dtype=torch.float64,
+                            device="cuda") for k in ("t", "qv", "qc", "qr", "qi", "qs", "qg", "ni", "nr", "nc", "nwfa",
+                                                     "nifa") if k in st}
+    active = np.ones((nz, ncol), bool) if nlev is None else np.arange(nz)[:, None] < np.asarray(nlev)[None, :]
+    # the step writes no reflectivity above a column's levels
+    dbar = {"refl10cm": torch.tensor(rng.standard_normal((nz, ncol)) * 0.01 * active, device="cuda")}
+    for k in ("rain_precip", "ice_liquid_equiv_precip", "snow_liquid_equiv_precip", "graupel_liquid_equiv_precip"):
+        dbar[k] = torch.tensor(rng.standard_normal(ncol) * 1e3, device="cuda")
+    nl = None if nlev is None else torch.tensor(nlev, dtype=torch.int32, device="cuda")
+    _, dtl, tb = ctx.tlm(dev(st), {k: torch.tensor(x, device="cuda") for k, x in v.items()}, m["dt"], m["itimestep"],
+                         nlev=nl)
+    lhs = sum(float((ybar[k] * tb[k].double()).sum()) for k in ybar)
+    lhs += sum(float((dbar[k] * dtl[k].double()).sum()) for k in dbar)
+    xbar = ctx.adjoint(dev(st), ybar, dbar, m["dt"], m["itimestep"], nlev=nl)
+    torch.cuda.synchronize()
+    rhs = sum(float((xbar[k].double() * torch.tensor(v[k], device="cuda")).sum()) for k in v)
+    print(ncol, "x", nz, "dot test", lhs, rhs)
+    assert np.isfinite(lhs) and np.isfinite(rhs)
+    assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), 1e-30)
 
 
 def loss_weights(st, seed=3):
