@@ -11,8 +11,9 @@
     derivative exists.
   * Adjoint vs tangent-linear (dot-product test <ybar, J v> = <J^T ybar, v>):
     relative 1e-10 in FP64, 1e-4 in FP32; also with cotangents at three
-    levels only, on 150 levels, and on tiny grids with an early-exit column
-    and with levels above a column's nlev (where the step is the identity).
+    levels only, on 150 levels, on tiny grids with an early-exit column and
+    with levels above a column's nlev (where the step is the identity), and
+    on a first step that initialises graupel number and volume.
   * torch.autograd gradient of the loss (weighted surface precipitation +
     reflectivity at levels 5, 20, 40) vs central finite differences of the
     reference FP64 build along a random direction: relative 1e-5.
@@ -309,6 +310,40 @@ def test_adjoint_dot_product_tiny_grids(ctx, sample, ncol, nz, nlev):
     print(ncol, "x", nz, "dot test", lhs, rhs)
     assert np.isfinite(lhs) and np.isfinite(rhs)
     assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), 1e-30)
+
+
+def test_adjoint_dot_product_first_step_graupel_init(hip):
+    """the first step of hail-aware columns whose graupel has no number or
+    volume yet: the step initialises both from the graupel mass
+    (src/module_mp_tempo_main.F90:1234-1260); FP64 dot-product test over
+    every prognostic field"""
+    from tempo_amd import generator as gen
+    from tempo_amd.autograd import Y_NAMES
+    ctx = hip.TempoHip(aerosolaware_flag=1, hailaware_flag=1)
+    st, m = gen.generate("multistep", ncol=96, col0=7_000)
+    st["ng"] = np.zeros_like(st["ng"])
+    st["qb"] = np.zeros_like(st["qb"])
+    nz, ncol = st["p"].shape
+    assert (st["qg"] > 1e-12).any(axis=0).sum() > 10  # columns the initialisation reaches
+    rng = np.random.default_rng(37)
+    prog = [k for k in Y_NAMES if k in st]
+    v = {k: st[k] * rng.uniform(-1, 1, st[k].shape) + (rng.uniform(-1, 1, st[k].shape) * 1e-6 if k in ("ng", "qb")
+                                                         else 0.0) for k in prog}
+    ybar = {k: torch.tensor(rng.standard_normal(st[k].shape) / (np.abs(st[k]).max() or 1.0), device="cuda")
+            for k in prog}
+    dbar = {"refl10cm": torch.tensor(rng.standard_normal((nz, ncol)) * 0.01, device="cuda"),
+            "rain_precip": torch.tensor(rng.standard_normal(ncol) * 1e3, device="cuda"),
+            "graupel_liquid_equiv_precip": torch.tensor(rng.standard_normal(ncol) * 1e3, device="cuda")}
+    b = dev(st)
+    _, dtl, tb = ctx.tlm(b, {k: torch.tensor(x, device="cuda") for k, x in v.items()}, m["dt"], 1)
+    assert float(b["ng"].max()) > 0.0 and float(b["qb"].max()) > 0.0  # the initialisation ran
+    lhs = sum(float((ybar[k] * tb[k].double()).sum()) for k in ybar)
+    lhs += sum(float((dbar[k] * dtl[k].double()).sum()) for k in dbar)
+    xbar = ctx.adjoint(dev(st), ybar, dbar, m["dt"], 1, wrt=tuple(prog))
+    torch.cuda.synchronize()
+    rhs = sum(float((xbar[k].double() * torch.tensor(v[k], device="cuda")).sum()) for k in v)
+    print("first step, graupel initialised: dot test", lhs, rhs, abs(lhs - rhs) / abs(lhs))
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
 
 
 def loss_weights(st, seed=3):
