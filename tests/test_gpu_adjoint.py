@@ -12,8 +12,9 @@
   * Adjoint vs tangent-linear (dot-product test <ybar, J v> = <J^T ybar, v>):
     relative 1e-10 in FP64, 1e-4 in FP32; also with cotangents at three
     levels only, on 150 levels, on tiny grids with an early-exit column and
-    with levels above a column's nlev (where the step is the identity), and
-    on a first step that initialises graupel number and volume.
+    with levels above a column's nlev (where the step is the identity), on a
+    first step that initialises graupel number and volume, and without the
+    aerosol fields (default profiles, cloud number from a land mask).
   * torch.autograd gradient of the loss (weighted surface precipitation +
     reflectivity at levels 5, 20, 40) vs central finite differences of the
     reference FP64 build along a random direction: relative 1e-5.
@@ -343,6 +344,34 @@ def test_adjoint_dot_product_first_step_graupel_init(hip):
     torch.cuda.synchronize()
     rhs = sum(float((xbar[k].double() * torch.tensor(v[k], device="cuda")).sum()) for k in v)
     print("first step, graupel initialised: dot test", lhs, rhs, abs(lhs - rhs) / abs(lhs))
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+
+
+def test_adjoint_dot_product_without_aerosols(hip):
+    """a context without the aerosol fields (nwfa/nifa absent: the default
+    profiles from the column heights) and without prognostic cloud number
+    (its one-moment value from a land mask): FP64 dot-product test"""
+    from tempo_amd import generator as gen
+    ctx = hip.TempoHip(aerosolaware_flag=0, hailaware_flag=0)
+    st, m = gen.generate("plain", ncol=128, col0=300)
+    nz, ncol = st["p"].shape
+    rng = np.random.default_rng(41)
+    land = torch.tensor(rng.integers(1, 3, ncol), dtype=torch.int32, device="cuda")  # 1 land, 2 water
+    names = [k for k in ("t", "qv", "qc", "qr", "qi", "qs", "qg", "ni", "nr") if k in st]
+    v = {k: st[k] * rng.uniform(-1, 1, st[k].shape) for k in names}
+    ybar = {k: torch.tensor(rng.standard_normal(st[k].shape) / (np.abs(st[k]).max() or 1.0), device="cuda")
+            for k in names}
+    dbar = {"refl10cm": torch.tensor(rng.standard_normal((nz, ncol)) * 0.01, device="cuda"),
+            "rain_precip": torch.tensor(rng.standard_normal(ncol) * 1e3, device="cuda"),
+            "snow_liquid_equiv_precip": torch.tensor(rng.standard_normal(ncol) * 1e3, device="cuda")}
+    _, dtl, tb = ctx.tlm(dev(st), {k: torch.tensor(x, device="cuda") for k, x in v.items()}, m["dt"],
+                         m["itimestep"], land=land)
+    lhs = sum(float((ybar[k] * tb[k].double()).sum()) for k in ybar)
+    lhs += sum(float((dbar[k] * dtl[k].double()).sum()) for k in dbar)
+    xbar = ctx.adjoint(dev(st), ybar, dbar, m["dt"], m["itimestep"], land=land, wrt=tuple(names))
+    torch.cuda.synchronize()
+    rhs = sum(float((xbar[k].double() * torch.tensor(v[k], device="cuda")).sum()) for k in v)
+    print("without aerosols: dot test", lhs, rhs, abs(lhs - rhs) / abs(lhs))
     assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
 
 
