@@ -94,7 +94,10 @@ typedef struct {
  * pointer at the tile origin); of a 3-D diagnostic at i + k*dldk + j*dldj;
  * of a 2-D diagnostic at i + j*d2ldj.  Column index c = i + j*ni.
  * nlev (device, ni*nj ints, may be NULL) gives each column's active level
- * count (levels k >= nlev[c] are not touched). */
+ * count (levels k >= nlev[c] are not touched).  An empty batch (ni or nj
+ * zero) is a no-op: the step entry points and the surface emissions return
+ * TEMPO_HIP_OK without reading the arrays, as the reference's loops over an
+ * empty tile do nothing. */
 typedef struct {
   int ni, nj, nz;
   long long ldk, ldj;

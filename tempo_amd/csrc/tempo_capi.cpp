@@ -926,9 +926,17 @@ int run_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_fie
   return TEMPO_HIP_OK;
 }
 
+// check_run_args and check_emission_args: an empty batch (ni or nj zero) is
+// a no-op, as the reference's loops over an empty tile; the entry points
+// return TEMPO_HIP_OK for it without looking at the arrays
+constexpr int kEmptyBatch = 1;
+int ok_if_empty(int rc) { return rc == kEmptyBatch ? TEMPO_HIP_OK : rc; }
+
 int check_run_args(const tempo_hip_fields* f, const tempo_hip_dims* dm, double dt) {
   if (!f || !dm) return fail(TEMPO_HIP_E_ARG, "NULL fields/dims");
-  if (dm->ni <= 0 || dm->nj <= 0 || dm->nz < 2) return fail(TEMPO_HIP_E_ARG, "bad dims (need ni,nj >= 1, nz >= 2)");
+  if (dm->ni < 0 || dm->nj < 0) return fail(TEMPO_HIP_E_ARG, "bad dims (negative ni or nj)");
+  if (dm->ni == 0 || dm->nj == 0) return kEmptyBatch;
+  if (dm->nz < 2) return fail(TEMPO_HIP_E_ARG, "bad dims (need nz >= 2)");
   if (dm->ldk < dm->ni) return fail(TEMPO_HIP_E_ARG, "ldk smaller than ni");
   if (dm->nj > 1 && dm->ldj < dm->ldk * dm->nz) return fail(TEMPO_HIP_E_ARG, "ldj too small");
   if (!(dt > 0.0)) return fail(TEMPO_HIP_E_ARG, "dt must be positive");
@@ -1139,7 +1147,7 @@ int run_host_impl(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_d
                   double dt, int itimestep, int precision) {
   if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
   int rc = check_run_args(f, dm, dt);
-  if (rc) return rc;
+  if (rc) return ok_if_empty(rc);
   const bool fp32 = precision == TEMPO_HIP_FP32;
   const LaunchCfg& lc = x->launch[fp32 ? 0 : 1];
   if ((rc = ensure_host_staging(x, unsigned(lc.host_threads)))) return rc;
@@ -1367,7 +1375,7 @@ int tempo_hip_run(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_d
                   double dt, int itimestep, int precision, void* stream) {
   if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
   int rc = check_run_args(f, dm, dt);
-  if (rc) return rc;
+  if (rc) return ok_if_empty(rc);
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (precision == TEMPO_HIP_FP32) return run_impl<float>(x, f, d, dm, dt, itimestep, s);
   if (precision == TEMPO_HIP_FP64) return run_impl<double>(x, f, d, dm, dt, itimestep, s);
@@ -1379,7 +1387,7 @@ int tempo_hip_branch_signature(tempo_hip_ctx* x, const tempo_hip_fields* f, cons
                                void* stream) {
   if (!x || !sig) return fail(TEMPO_HIP_E_ARG, "NULL context or signature array");
   int rc = check_run_args(f, dm, dt);
-  if (rc) return rc;
+  if (rc) return ok_if_empty(rc);
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (precision == TEMPO_HIP_FP32) return run_impl<float>(x, f, d, dm, dt, itimestep, s, sig);
   if (precision == TEMPO_HIP_FP64) return run_impl<double>(x, f, d, dm, dt, itimestep, s, sig);
@@ -1387,8 +1395,11 @@ int tempo_hip_branch_signature(tempo_hip_ctx* x, const tempo_hip_fields* f, cons
 }
 
 static int check_emission_args(const void* nwfa, const void* nwfa2d, const tempo_hip_dims* dm, int k) {
-  if (!nwfa || !nwfa2d || !dm) return fail(TEMPO_HIP_E_ARG, "NULL nwfa, nwfa2d or dims");
-  if (dm->ni <= 0 || dm->nj <= 0 || dm->nz <= 0 || k < 0 || k >= dm->nz)
+  if (!dm) return fail(TEMPO_HIP_E_ARG, "NULL dims");
+  if (dm->ni < 0 || dm->nj < 0) return fail(TEMPO_HIP_E_ARG, "bad dims (negative ni or nj)");
+  if (dm->ni == 0 || dm->nj == 0) return kEmptyBatch;
+  if (!nwfa || !nwfa2d) return fail(TEMPO_HIP_E_ARG, "NULL nwfa or nwfa2d");
+  if (dm->nz <= 0 || k < 0 || k >= dm->nz)
     return fail(TEMPO_HIP_E_ARG, "bad dims or level (need 0 <= k < nz)");
   if (dm->ldk < dm->ni || (dm->nj > 1 && (dm->ldj < dm->ldk * dm->nz || dm->d2ldj < dm->ni)))
     return fail(TEMPO_HIP_E_ARG, "leading dimensions too small");
@@ -1399,7 +1410,7 @@ int tempo_hip_aerosol_surface_emissions(tempo_hip_ctx* x, double dt, void* nwfa,
                                         const tempo_hip_dims* dm, int k, int precision, void* stream) {
   if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
   int rc = check_emission_args(nwfa, nwfa2d, dm, k);
-  if (rc) return rc;
+  if (rc) return ok_if_empty(rc);
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (precision == TEMPO_HIP_FP32)
     HIP_TRY(launch_surface_emissions(static_cast<float*>(nwfa), static_cast<const float*>(nwfa2d), dm->ni, dm->nj,
@@ -1533,7 +1544,7 @@ int tempo_hip_aerosol_surface_emissions_host(tempo_hip_ctx* x, double dt, double
                                              const tempo_hip_dims* dm, int k) {
   if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
   int rc = check_emission_args(nwfa, nwfa2d, dm, k);
-  if (rc) return rc;
+  if (rc) return ok_if_empty(rc);
   const size_t n3 = size_t(dm->ldj) * (dm->nj - 1) + size_t(dm->ldk) * (dm->nz - 1) + dm->ni;
   const size_t n2 = size_t(dm->d2ldj) * (dm->nj - 1) + dm->ni;
   if (x->stage.size() < 2) x->stage.resize(2);
@@ -1617,7 +1628,7 @@ int tempo_hip_tlm(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hip_f
                   void* stream) {
   if (!x || !f_tl) return fail(TEMPO_HIP_E_ARG, "NULL context or tangent fields");
   int rc = check_run_args(f, dm, dt);
-  if (rc) return rc;
+  if (rc) return ok_if_empty(rc);
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (precision == TEMPO_HIP_FP32) return run_tlm<float>(x, f, f_tl, d, d_tl, dm, dt, itimestep, s);
   if (precision == TEMPO_HIP_FP64) return run_tlm<double>(x, f, f_tl, d, d_tl, dm, dt, itimestep, s);
@@ -1629,7 +1640,7 @@ int tempo_hip_adjoint(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_h
                       int itimestep, int precision, void* stream) {
   if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
   int rc = check_run_args(f, dm, dt);
-  if (rc) return rc;
+  if (rc) return ok_if_empty(rc);
   if (!x->cfg.refl10cm_flag && dbar && dbar->refl10cm)
     return fail(TEMPO_HIP_E_ARG, "reflectivity cotangent given but refl10cm_flag is off");
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -1733,7 +1744,7 @@ int run_with_scratch(tempo_hip_ctx* x, const tempo_hip_fields* f, const tempo_hi
                      double dt, int itimestep, int precision, void* stream, void* base, size_t bytes) {
   if (!x) return fail(TEMPO_HIP_E_ARG, "NULL context");
   int rc = check_run_args(f, dm, dt);
-  if (rc) return rc;
+  if (rc) return ok_if_empty(rc);
   if (precision != TEMPO_HIP_FP32 && precision != TEMPO_HIP_FP64)
     return fail(TEMPO_HIP_E_ARG, "precision must be TEMPO_HIP_FP32 or TEMPO_HIP_FP64");
   int dev = -1;

@@ -964,6 +964,32 @@ def test_gpu_single_column_and_tiny_grid(hip, gpu_tables):
         check_against_reference(out, d, ref, d2, d3, inp, f"{inp['p'].shape[0]} levels x 1 column")
 
 
+def test_gpu_empty_batch_is_a_no_op(hip):
+    """zero columns: the step entry points (device arrays in both precisions,
+    host arrays, tangent-linear, adjoint) and the surface emissions return
+    without error and touch nothing, as the reference's loops over an empty
+    tile (src/module_mp_tempo_driver.F90:400-449)"""
+    ctx = context(hip, dict(aerosolaware=False, hailaware=False))
+    nz = 30
+    names = ("t", "p", "w", "dz", "qv", "qc", "qr", "qi", "qs", "qg", "ni", "nr")
+    for dtype in (torch.float32, torch.float64):
+        print("empty batch, device arrays,", dtype, flush=True)
+        dev = {k: torch.empty((nz, 0), dtype=dtype, device="cuda") for k in names}
+        d = ctx.run(dev, 20.0, 2)
+        assert all(v.numel() == 0 for v in d.values())
+    dev = {k: torch.empty((nz, 0), dtype=torch.float64, device="cuda") for k in names}
+    print("empty batch, tangent-linear and adjoint", flush=True)
+    ctx.tlm(dev, {k: torch.empty_like(v) for k, v in dev.items()}, 20.0, 2)
+    xbar = ctx.adjoint(dev, {}, {}, 20.0, 2)
+    assert all(v.numel() == 0 for v in xbar.values())
+    print("empty batch, host arrays and emissions", flush=True)
+    ctx.run_host({k: np.empty((nz, 0)) for k in names}, 20.0, 2)
+    ctx.surface_emissions(torch.empty((nz, 0), dtype=torch.float64, device="cuda"),
+                          torch.empty((0,), dtype=torch.float64, device="cuda"), 20.0)
+    ctx.surface_emissions_host(np.empty((nz, 0)), np.empty((0,)), 20.0)
+    torch.cuda.synchronize()
+
+
 def test_gpu_error_codes(hip):
     from tempo_amd import generator as gen
     ctx = context(hip, dict(aerosolaware=False, hailaware=False))
