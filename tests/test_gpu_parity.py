@@ -964,6 +964,25 @@ def test_gpu_single_column_and_tiny_grid(hip, gpu_tables):
         check_against_reference(out, d, ref, d2, d3, inp, f"{inp['p'].shape[0]} levels x 1 column")
 
 
+def test_gpu_fp64_matches_reference_above_128_levels(hip, gpu_tables):
+    """64 columns of the aerosol workload interpolated onto 160 levels (layer
+    depths scaled so that each column keeps its height): the sedimentation
+    runs one lane per column there (the wavefront kernel holds at most 128
+    levels); the FP64 twin vs the reference build"""
+    from tempo_amd import generator as gen
+    flags = dict(aerosolaware=True, hailaware=False)
+    st, m = gen.generate("aerosol", ncol=64, col0=1234)
+    nz0, nz = st["p"].shape[0], 160
+    x_old, x_new = np.arange(nz0, dtype=np.float64), np.linspace(0.0, nz0 - 1.0, nz)
+    deep = {k: np.ascontiguousarray(np.stack([np.interp(x_new, x_old, v[:, c]) for c in range(v.shape[1])], axis=1))
+            for k, v in st.items()}
+    deep["dz"] = deep["dz"] * (nz0 / nz)
+    ctx = context(hip, flags)
+    out, d = gpu_run(ctx, deep, m["dt"], m["itimestep"])
+    ref, d2, d3 = reference_steps(gpu_tables, flags, deep, m["dt"], m["itimestep"])
+    check_against_reference(out, d, ref, d2, d3, deep, f"{nz} levels x 64 columns")
+
+
 def test_gpu_empty_batch_is_a_no_op(hip):
     """zero columns: the step entry points (device arrays in both precisions,
     host arrays, tangent-linear, adjoint) and the surface emissions return
