@@ -157,6 +157,7 @@ __device__ __forceinline__ void wave_sed_adjoint(const WaveCol& q, const double 
   }
 }
 
+// (126 VGPRs, four waves per SIMD; held to five it was measured slower)
 template <class D>
 __global__ void __launch_bounds__(64 * kScrW) k_adj_sed_wave(StepArgs<D> a, AdjArgs g) {
   // every operation rounded on its own, as in the forward sedimentation it
